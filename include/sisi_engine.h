@@ -1,0 +1,209 @@
+/*
+ * sisi_engine.h -- C ABI of the MI355X CCSD / (T) contraction engine.
+ *
+ * Drop-in lower boundary for the CCSD / (T) hot path of sisi4s.  The
+ * reference evaluates this path with Cyclops (CTF) expressions inside its
+ * algorithm plugins; each entry point below replaces a named piece of that
+ * path (cited as reference file:line).  The host-side plugin mirror
+ * (CoulombIntegralsFromVertex, CcsdEnergyFromCoulombIntegrals,
+ * CcsdPerturbativeTriples) calls only these functions.
+ *
+ * Conventions
+ *  - Every function returns an int status: SISI_OK (0) or a SISI_ERR_* code.
+ *    Nothing throws across the ABI; sisi_last_error() returns a thread-local
+ *    message for the last failure on the calling thread.
+ *  - All tensors are real FP64, dense, first index fastest (CTF global
+ *    order): Gamma[G,p,q] at G + NG*(p + Np*q); T2[a,b,i,j] at
+ *    a + Nv*(b + Nv*(i + No*j)); T1[a,i] at a + Nv*i.  p,q run over
+ *    Np = No + Nv orbitals, holes first.
+ *  - The caller owns host arrays; the engine owns device memory and one HIP
+ *    stream.  A handle is not thread-safe (the reference drives the path
+ *    from a single host thread per rank).
+ *  - Precision: SISI_FP64 is the plain path.  SISI_FP32 and SISI_BF16X3 run
+ *    the dense contractions on MFMA with FP32 accumulation; energies,
+ *    denominators, the DIIS solve and all reductions stay FP64.
+ */
+#ifndef SISI_ENGINE_H
+#define SISI_ENGINE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SISI_OK 0
+#define SISI_ERR_INVALID 1 /* bad argument / dimension */
+#define SISI_ERR_HIP 2     /* HIP runtime error (no device, launch failure) */
+#define SISI_ERR_STATE 3   /* call out of order (e.g. no vertex set) */
+#define SISI_ERR_NUMERIC 4 /* NaN/Inf input or singular DIIS system */
+#define SISI_ERR_ALLOC 5   /* device allocation failed */
+
+#define SISI_FP64 0
+#define SISI_FP32 1
+#define SISI_BF16X3 2
+
+#define SISI_MIXER_LINEAR 0
+#define SISI_MIXER_DIIS 1
+
+/* Integral classes (CoulombIntegralsFromVertex output names). */
+#define SISI_PPHH 0 /* V_abij  [Nv,Nv,No,No] */
+#define SISI_PHPH 1 /* V_aibj  [Nv,No,Nv,No] */
+#define SISI_HHHH 2 /* V_ijkl  [No,No,No,No] */
+#define SISI_HHHP 3 /* V_ijka  [No,No,No,Nv] */
+#define SISI_HHPP 4 /* V_ijab  [No,No,Nv,Nv] */
+
+typedef struct sisi_engine sisi_engine;
+
+typedef struct {
+  int No; /* occupied orbitals   (HoleEigenEnergies length)     */
+  int Nv; /* virtual orbitals    (ParticleEigenEnergies length) */
+  int NG; /* auxiliary index     (CoulombVertex first length)   */
+} sisi_dims;
+
+/* Arguments of the CCSD driver; names and defaults follow
+ * ClusterSinglesDoublesAlgorithm.hpp:42-46, .cxx:48-83, DiisMixer.cxx:55,
+ * LinearMixer.cxx:17, CcsdEnergyFromCoulombIntegrals.cxx:374-384. */
+typedef struct {
+  int mixer;                     /* SISI_MIXER_LINEAR (default) or _DIIS  */
+  int max_residua;               /* maxResidua, default 4                 */
+  double mixing_ratio;           /* mixingRatio, default 1.0              */
+  int max_iterations;            /* maxIterations, default 16             */
+  double amplitudes_convergence; /* amplitudesConvergence, default 1e-5   */
+  double energy_convergence;     /* energyConvergence, default 1e-6       */
+  double level_shift;            /* levelShift, default 0                 */
+  int slice_size;                /* integralsSliceSize, <= 0 means No     */
+  int precision;                 /* bulk precision: SISI_FP64/_FP32/_BF16X3 */
+  double switch_threshold;       /* |dT|/|T| at which a mixed run switches
+                                    to FP64 iterations (default 1e-5)      */
+} sisi_ccsd_opts;
+
+typedef struct {
+  double energy;      /* correlation energy after the last iteration     */
+  double direct;      /* direct term of getEnergy                        */
+  double exchange;    /* exchange term of getEnergy                      */
+  double delta;       /* |dT|/|T| of the last iteration                  */
+  int iterations;     /* iterations run so far                           */
+  int converged;      /* reference convergence test met                  */
+  int fp64_iterations;/* iterations run with FP64 contractions           */
+  int precision;      /* precision of the last iteration                 */
+} sisi_ccsd_report;
+
+typedef struct {
+  double gemm_seconds[3];   /* device time of MFMA GEMM launches per precision */
+  double gemm_flops[3];     /* algorithmic flops of those launches             */
+  int64_t gemm_launches[3]; /* launch count per precision                      */
+  double other_seconds;     /* device time of the bandwidth kernels           */
+  double other_bytes;       /* algorithmic bytes of the bandwidth kernels     */
+  int64_t other_launches;
+} sisi_stats;
+
+/* Last error message of the calling thread ("" if none). */
+const char *sisi_last_error(void);
+
+/* Fill opts with the reference defaults (see sisi_ccsd_opts). */
+void sisi_ccsd_opts_default(sisi_ccsd_opts *opts);
+
+/* Create an engine on the current HIP device.  Replaces the implicit CTF
+ * world of the reference run loop (src/Sisi4s.cxx:280-282). */
+int sisi_engine_create(const sisi_dims *dims, sisi_engine **out);
+int sisi_engine_destroy(sisi_engine *e);
+
+/* Upload the real Coulomb vertex Gamma[NG,Np,Np] and the eigenenergies.
+ * Replaces the reads of "CoulombVertex", "HoleEigenEnergies" and
+ * "ParticleEigenEnergies" (CoulombIntegralsFromVertex.cxx:23-38,
+ * CcsdEnergyFromCoulombIntegrals.cxx:58-115). */
+int sisi_engine_set_vertex(sisi_engine *e, const double *gamma);
+int sisi_engine_set_eigenenergies(sisi_engine *e, const double *eps_i,
+                                  const double *eps_a);
+
+/* CoulombIntegralsFromVertex::calculateRealIntegrals
+ * (CoulombIntegralsFromVertex.cxx:203-571): builds PPHH, PHPH, HHHH, HHHP
+ * (and HHPP as a transposed view) on the device. */
+int sisi_integrals_from_vertex(sisi_engine *e);
+
+/* Copy one integral class (SISI_PPHH ...) to host memory. */
+int sisi_get_integrals(sisi_engine *e, int which, double *out);
+
+/* Precision used by the dense contractions of the next residual call. */
+int sisi_engine_set_precision(sisi_engine *e, int precision);
+
+/* CcsdEnergyFromCoulombIntegrals::getResiduum, real path
+ * (CcsdEnergyFromCoulombIntegrals.cxx:27-494) with the vertex-built ladder
+ * (cxx:336-453, ClusterSinglesDoublesAlgorithm.cxx:887-915).
+ * iteration == 0 and !initial_doubles gives the MP2 branch (cxx:47-51). */
+int sisi_ccsd_residual(sisi_engine *e, int iteration, int initial_doubles,
+                       int slice_size, const double *T1, const double *T2,
+                       double *R1, double *R2);
+
+/* ClusterSinglesDoublesAlgorithm::getEnergy (cxx:131-213), restricted,
+ * non-antisymmetrized: out[0] = E, out[1] = direct, out[2] = exchange. */
+int sisi_ccsd_energy(sisi_engine *e, const double *T1, const double *T2,
+                     double out[3]);
+
+/* MP2 energy as the reference obtains it: iteration 0 of the driver
+ * (R2 = V_abij, T2 = -R2/D) followed by getEnergy. */
+int sisi_mp2_energy(sisi_engine *e, double *energy);
+
+/* ClusterSinglesDoublesAlgorithm::run<double> (cxx:38-128), split so a
+ * caller can time iterations: begin() sets up amplitudes and the mixer
+ * (createAmplitudes cxx:240-273; T1/T2 NULL means zero amplitudes, i.e. the
+ * MP2 start), iterate() runs up to n iterations and stops early when the
+ * reference convergence test (cxx:103-106) is met. */
+int sisi_ccsd_begin(sisi_engine *e, const sisi_ccsd_opts *opts,
+                    const double *T1_init, const double *T2_init);
+int sisi_ccsd_iterate(sisi_engine *e, int n, sisi_ccsd_report *report);
+
+/* begin + iterate(max_iterations): the whole CcsdEnergyFromCoulombIntegrals
+ * run (ClusterSinglesDoublesAlgorithm.cxx:25-35). */
+int sisi_ccsd_solve(sisi_engine *e, const sisi_ccsd_opts *opts,
+                    const double *T1_init, const double *T2_init,
+                    sisi_ccsd_report *report);
+
+/* storeAmplitudes (cxx:288-299): copy the current T1/T2 to the host. */
+int sisi_ccsd_get_amplitudes(sisi_engine *e, double *T1, double *T2);
+
+/* Per-iteration energies of the current run (energy.value per iteration,
+ * cxx:185-213); *n receives the count (at most max). */
+int sisi_ccsd_get_energies(sisi_engine *e, double *energies, int max, int *n);
+
+/* CcsdPerturbativeTriples::run (CcsdPerturbativeTriples.cxx:119-248):
+ * E_(T) (without the CCSD energy) for amplitudes T1/T2 (NULL: the engine's
+ * current amplitudes).  precision selects the triples contractions;
+ * energies are always accumulated in FP64. */
+int sisi_triples_energy(sisi_engine *e, const double *T1, const double *T2,
+                        int precision, double *energy);
+
+/* Same, restricted to the nblocks sorted blocks (i0<=i1<=i2) listed in
+ * blocks[3*b..3*b+2]; used to time a sample of the triple loop. */
+int sisi_triples_energy_blocks(sisi_engine *e, const double *T1,
+                               const double *T2, int precision,
+                               const int *blocks, int nblocks, double *energy);
+
+/* Device timing of kernel families since the last reset (collected only
+ * while enabled; enabling adds one event pair per launch). */
+int sisi_engine_enable_stats(sisi_engine *e, int enable);
+int sisi_engine_get_stats(sisi_engine *e, sisi_stats *stats);
+int sisi_engine_reset_stats(sisi_engine *e);
+
+/* Synthetic seeded system (host, no device needed): Philox-4x64-10 vertex
+ * Gamma[G,p,q] = Gamma[G,q,p] ~ N(0,1) exp(-G/100)/sqrt(NG), eps_i in
+ * U(-2,-0.5), eps_a in U(0.2,3).  Replaces the reference's random-tensor
+ * generators (GenerateRandomTensor.cxx:19-29, math/RandomTensor.hpp:38-58),
+ * whose mt19937-per-rank streams are not reproducible. */
+int sisi_generate_system(uint64_t seed, int No, int Nv, int NG, double *gamma,
+                         double *eps_i, double *eps_a);
+
+/* One MFMA GEMM C = alpha*op(A)*op(B) + beta*C on host arrays (column-major,
+ * op = transpose when trans* != 0), in the given precision.  No counterpart
+ * in the reference (CTF calls BLAS internally); exposed so the kernel family
+ * every contraction uses can be tested on its own. */
+int sisi_gemm_host(int precision, int transA, int transB, int M, int N, int K,
+                   double alpha, const double *A, int lda, const double *B,
+                   int ldb, double beta, double *C, int ldc);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SISI_ENGINE_H */

@@ -1,0 +1,380 @@
+// extern "C" boundary: argument checks, exception -> status conversion,
+// host <-> device staging.  See include/sisi_engine.h for the contract.
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "ccsd.hpp"
+#include "contract.hpp"
+#include "engine.hpp"
+#include "tensor_ops.hpp"
+#include "triples.hpp"
+
+extern "C" int sisi_generate_system_impl(uint64_t seed, int No, int Nv, int NG, double *gamma,
+                                         double *eps_i, double *eps_a);
+
+namespace sisi {
+
+static thread_local std::string g_last_error;
+
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+
+template <typename F>
+static int guarded(F &&f) {
+  try {
+    f();
+    g_last_error.clear();
+    return SISI_OK;
+  } catch (const Error &err) {
+    g_last_error = err.what();
+    return err.code;
+  } catch (const std::bad_alloc &) {
+    g_last_error = "host allocation failed";
+    return SISI_ERR_ALLOC;
+  } catch (const std::exception &ex) {
+    g_last_error = ex.what();
+    return SISI_ERR_INVALID;
+  } catch (...) {
+    g_last_error = "unknown error";
+    return SISI_ERR_INVALID;
+  }
+}
+
+static void check_finite(const double *p, size_t n, const char *what) {
+  for (size_t i = 0; i < n; ++i)
+    SISI_REQUIRE(std::isfinite(p[i]), SISI_ERR_NUMERIC, std::string(what) + " has NaN/Inf");
+}
+
+}  // namespace sisi
+
+using namespace sisi;
+
+struct sisi_engine {
+  Engine e;
+  explicit sisi_engine(const sisi_dims &d) : e(d) {}
+};
+
+#define ENGINE_REQUIRED(h) SISI_REQUIRE((h) != nullptr, SISI_ERR_INVALID, "null engine handle")
+
+extern "C" {
+
+const char *sisi_last_error(void) { return g_last_error.c_str(); }
+
+void sisi_ccsd_opts_default(sisi_ccsd_opts *o) {
+  if (!o) return;
+  o->mixer = SISI_MIXER_LINEAR;
+  o->max_residua = 4;
+  o->mixing_ratio = 1.0;
+  o->max_iterations = 16;
+  o->amplitudes_convergence = 1e-5;
+  o->energy_convergence = 1e-6;
+  o->level_shift = 0.0;
+  o->slice_size = -1;
+  o->precision = SISI_FP64;
+  o->switch_threshold = 1e-5;
+}
+
+int sisi_engine_create(const sisi_dims *dims, sisi_engine **out) {
+  return guarded([&] {
+    SISI_REQUIRE(dims != nullptr && out != nullptr, SISI_ERR_INVALID, "null argument");
+    *out = nullptr;
+    *out = new sisi_engine(*dims);
+  });
+}
+
+int sisi_engine_destroy(sisi_engine *h) {
+  return guarded([&] { delete h; });
+}
+
+int sisi_engine_set_vertex(sisi_engine *h, const double *gamma) {
+  return guarded([&] {
+    ENGINE_REQUIRED(h);
+    Engine &e = h->e;
+    const size_t n = size_t(e.NG) * e.Np * e.Np;
+    SISI_REQUIRE(gamma != nullptr, SISI_ERR_INVALID, "null CoulombVertex");
+    check_finite(gamma, n, "CoulombVertex");
+    e.upload(e.gamma, gamma, n);
+    e.have_vertex = true;
+    e.have_integrals = false;
+  });
+}
+
+int sisi_engine_set_eigenenergies(sisi_engine *h, const double *eps_i, const double *eps_a) {
+  return guarded([&] {
+    ENGINE_REQUIRED(h);
+    Engine &e = h->e;
+    SISI_REQUIRE(eps_i && eps_a, SISI_ERR_INVALID, "null eigenenergies");
+    check_finite(eps_i, size_t(e.No), "HoleEigenEnergies");
+    check_finite(eps_a, size_t(e.Nv), "ParticleEigenEnergies");
+    e.upload(e.eps_i, eps_i, size_t(e.No));
+    e.upload(e.eps_a, eps_a, size_t(e.Nv));
+    e.have_eps = true;
+  });
+}
+
+int sisi_integrals_from_vertex(sisi_engine *h) {
+  return guarded([&] {
+    ENGINE_REQUIRED(h);
+    integrals_from_vertex(h->e);
+  });
+}
+
+int sisi_get_integrals(sisi_engine *h, int which, double *out) {
+  return guarded([&] {
+    ENGINE_REQUIRED(h);
+    Engine &e = h->e;
+    SISI_REQUIRE(e.have_integrals, SISI_ERR_STATE, "integrals not built");
+    const size_t o = size_t(e.No), v = size_t(e.Nv);
+    switch (which) {
+      case SISI_PPHH: e.download(out, e.Vabij, v * v * o * o); break;
+      case SISI_PHPH: e.download(out, e.Vaibj, v * o * v * o); break;
+      case SISI_HHHH: e.download(out, e.Vijkl, o * o * o * o); break;
+      case SISI_HHHP: e.download(out, e.Vijka, o * o * o * v); break;
+      case SISI_HHPP: {
+        DBuf t(&e.pool, o * o * v * v);
+        sum(e, 1.0, dense(e.Vabij.get(), {int64_t(v), int64_t(v), int64_t(o), int64_t(o)}), "abij",
+            0.0, dense(t.get(), {int64_t(o), int64_t(o), int64_t(v), int64_t(v)}), "ijab");
+        e.download(out, t, o * o * v * v);
+        break;
+      }
+      default: throw Error(SISI_ERR_INVALID, "unknown integral class");
+    }
+  });
+}
+
+int sisi_engine_set_precision(sisi_engine *h, int precision) {
+  return guarded([&] {
+    ENGINE_REQUIRED(h);
+    set_precision(h->e, precision);
+  });
+}
+
+int sisi_ccsd_residual(sisi_engine *h, int iteration, int initial_doubles, int slice_size,
+                       const double *T1, const double *T2, double *R1, double *R2) {
+  return guarded([&] {
+    ENGINE_REQUIRED(h);
+    Engine &e = h->e;
+    SISI_REQUIRE(T1 && T2 && R1 && R2, SISI_ERR_INVALID, "null amplitude/residual pointer");
+    SISI_REQUIRE(iteration >= 0, SISI_ERR_INVALID, "negative iteration");
+    DBuf t1, t2, r1(&e.pool, e.n1()), r2(&e.pool, e.n2());
+    e.upload(t1, T1, e.n1());
+    e.upload(t2, T2, e.n2());
+    ccsd_residual(e, iteration, initial_doubles != 0, slice_size, t1.get(), t2.get(), r1.get(),
+                  r2.get());
+    e.download(R1, r1, e.n1());
+    e.download(R2, r2, e.n2());
+  });
+}
+
+int sisi_ccsd_energy(sisi_engine *h, const double *T1, const double *T2, double out[3]) {
+  return guarded([&] {
+    ENGINE_REQUIRED(h);
+    Engine &e = h->e;
+    SISI_REQUIRE(T1 && T2 && out, SISI_ERR_INVALID, "null pointer");
+    DBuf t1, t2;
+    e.upload(t1, T1, e.n1());
+    e.upload(t2, T2, e.n2());
+    ccsd_energy(e, t1.get(), t2.get(), out[0], out[1], out[2]);
+  });
+}
+
+int sisi_mp2_energy(sisi_engine *h, double *energy) {
+  return guarded([&] {
+    ENGINE_REQUIRED(h);
+    Engine &e = h->e;
+    SISI_REQUIRE(energy, SISI_ERR_INVALID, "null pointer");
+    DBuf t1(&e.pool, e.n1()), t2(&e.pool, e.n2()), r1(&e.pool, e.n1()), r2(&e.pool, e.n2()),
+        d1(&e.pool, e.n1()), d2(&e.pool, e.n2());
+    zero(e, t1.get(), int64_t(e.n1()));
+    zero(e, t2.get(), int64_t(e.n2()));
+    ccsd_residual(e, 0, false, -1, t1.get(), t2.get(), r1.get(), r2.get());
+    estimate_amplitudes(e, 2, r1.get(), t1.get(), d1.get(), 0.0);
+    estimate_amplitudes(e, 4, r2.get(), t2.get(), d2.get(), 0.0);
+    double dir, exc;
+    ccsd_energy(e, r1.get(), r2.get(), *energy, dir, exc);
+  });
+}
+
+int sisi_ccsd_begin(sisi_engine *h, const sisi_ccsd_opts *opts, const double *T1, const double *T2) {
+  return guarded([&] {
+    ENGINE_REQUIRED(h);
+    sisi_ccsd_opts o;
+    if (opts) o = *opts;
+    else sisi_ccsd_opts_default(&o);
+    ccsd_begin(h->e, o, T1, T2);
+  });
+}
+
+int sisi_ccsd_iterate(sisi_engine *h, int n, sisi_ccsd_report *report) {
+  return guarded([&] {
+    ENGINE_REQUIRED(h);
+    SISI_REQUIRE(n >= 0, SISI_ERR_INVALID, "negative iteration count");
+    ccsd_iterate(h->e, n);
+    if (report) fill_report(h->e, report);
+  });
+}
+
+int sisi_ccsd_solve(sisi_engine *h, const sisi_ccsd_opts *opts, const double *T1,
+                    const double *T2, sisi_ccsd_report *report) {
+  return guarded([&] {
+    ENGINE_REQUIRED(h);
+    sisi_ccsd_opts o;
+    if (opts) o = *opts;
+    else sisi_ccsd_opts_default(&o);
+    ccsd_begin(h->e, o, T1, T2);
+    ccsd_iterate(h->e, o.max_iterations);
+    Engine &e = h->e;
+    if (o.max_iterations == 0) {  // energy of the given amplitudes (cxx:116-119)
+      double E, d, x;
+      ccsd_energy(e, e.T1.get(), e.T2.get(), E, d, x);
+      e.last_energy = E;
+      e.last_direct = d;
+      e.last_exchange = x;
+    }
+    if (report) fill_report(e, report);
+  });
+}
+
+int sisi_ccsd_get_amplitudes(sisi_engine *h, double *T1, double *T2) {
+  return guarded([&] {
+    ENGINE_REQUIRED(h);
+    Engine &e = h->e;
+    SISI_REQUIRE(e.run_active, SISI_ERR_STATE, "no CCSD run");
+    if (T1) e.download(T1, e.T1, e.n1());
+    if (T2) e.download(T2, e.T2, e.n2());
+  });
+}
+
+int sisi_ccsd_get_energies(sisi_engine *h, double *energies, int max, int *n) {
+  return guarded([&] {
+    ENGINE_REQUIRED(h);
+    Engine &e = h->e;
+    const int cnt = int(e.energies.size());
+    if (n) *n = cnt;
+    if (energies)
+      for (int k = 0; k < cnt && k < max; ++k) energies[k] = e.energies[k];
+  });
+}
+
+static int triples_common(sisi_engine *h, const double *T1, const double *T2, int precision,
+                          const int *blocks, int nblocks, double *energy) {
+  return guarded([&] {
+    ENGINE_REQUIRED(h);
+    Engine &e = h->e;
+    SISI_REQUIRE(energy, SISI_ERR_INVALID, "null pointer");
+    SISI_REQUIRE(precision == SISI_FP64 || precision == SISI_FP32 || precision == SISI_BF16X3,
+                 SISI_ERR_INVALID, "unknown precision");
+    SISI_REQUIRE((T1 == nullptr) == (T2 == nullptr), SISI_ERR_INVALID,
+                 "give both amplitudes or neither");
+    DBuf t1, t2;
+    const double *d1, *d2;
+    if (T1) {
+      e.upload(t1, T1, e.n1());
+      e.upload(t2, T2, e.n2());
+      d1 = t1.get();
+      d2 = t2.get();
+    } else {
+      SISI_REQUIRE(e.run_active, SISI_ERR_STATE, "no amplitudes: run CCSD or pass T1/T2");
+      d1 = e.T1.get();
+      d2 = e.T2.get();
+    }
+    std::vector<std::array<int, 3>> bl;
+    if (blocks) {
+      SISI_REQUIRE(nblocks > 0, SISI_ERR_INVALID, "empty block list");
+      for (int b = 0; b < nblocks; ++b) bl.push_back({blocks[3 * b], blocks[3 * b + 1], blocks[3 * b + 2]});
+    }
+    *energy = triples_energy(e, d1, d2, precision, bl);
+  });
+}
+
+int sisi_triples_energy(sisi_engine *h, const double *T1, const double *T2, int precision,
+                        double *energy) {
+  return triples_common(h, T1, T2, precision, nullptr, 0, energy);
+}
+
+int sisi_triples_energy_blocks(sisi_engine *h, const double *T1, const double *T2, int precision,
+                               const int *blocks, int nblocks, double *energy) {
+  if (!blocks) {
+    g_last_error = "null block list";
+    return SISI_ERR_INVALID;
+  }
+  return triples_common(h, T1, T2, precision, blocks, nblocks, energy);
+}
+
+int sisi_engine_enable_stats(sisi_engine *h, int enable) {
+  return guarded([&] {
+    ENGINE_REQUIRED(h);
+    h->e.timers->enabled = enable != 0;
+  });
+}
+
+int sisi_engine_get_stats(sisi_engine *h, sisi_stats *stats) {
+  return guarded([&] {
+    ENGINE_REQUIRED(h);
+    SISI_REQUIRE(stats, SISI_ERR_INVALID, "null pointer");
+    h->e.timers->collect(*stats);
+  });
+}
+
+int sisi_engine_reset_stats(sisi_engine *h) {
+  return guarded([&] {
+    ENGINE_REQUIRED(h);
+    h->e.timers->reset();
+  });
+}
+
+int sisi_generate_system(uint64_t seed, int No, int Nv, int NG, double *gamma, double *eps_i,
+                         double *eps_a) {
+  const int rc = sisi_generate_system_impl(seed, No, Nv, NG, gamma, eps_i, eps_a);
+  g_last_error = rc == SISI_OK ? "" : "generate_system: No, Nv, NG must be positive";
+  return rc;
+}
+
+int sisi_gemm_host(int precision, int transA, int transB, int M, int N, int K, double alpha,
+                   const double *A, int lda, const double *B, int ldb, double beta, double *C,
+                   int ldc) {
+  return guarded([&] {
+    SISI_REQUIRE(M >= 0 && N >= 0 && K >= 0, SISI_ERR_INVALID, "negative dimension");
+    SISI_REQUIRE(precision == SISI_FP64 || precision == SISI_FP32 || precision == SISI_BF16X3,
+                 SISI_ERR_INVALID, "unknown precision");
+    SISI_REQUIRE(A && B && C, SISI_ERR_INVALID, "null pointer");
+    const int64_t ra = transA ? K : M, ca = transA ? M : K;
+    const int64_t rb = transB ? N : K, cb = transB ? K : N;
+    SISI_REQUIRE(lda >= std::max<int64_t>(ra, 1) && ldb >= std::max<int64_t>(rb, 1) &&
+                     ldc >= std::max(M, 1),
+                 SISI_ERR_INVALID, "leading dimension too small");
+    sisi_dims d{1, 1, 1};
+    Engine e(d);
+    // host elements actually present in a column-major (rows x cols, ld) matrix
+    auto span = [](int64_t rows, int64_t cols, int64_t ld) -> int64_t {
+      return (rows == 0 || cols == 0) ? 0 : ld * (cols - 1) + rows;
+    };
+    const int64_t na = span(ra, ca, lda), nb = span(rb, cb, ldb), nc = span(M, N, ldc);
+    DBuf a(&e.pool, size_t(std::max<int64_t>(na, 1))), b(&e.pool, size_t(std::max<int64_t>(nb, 1))),
+        c(&e.pool, size_t(std::max<int64_t>(nc, 1)));
+    if (na > 0) e.upload(a, A, size_t(na));
+    if (nb > 0) e.upload(b, B, size_t(nb));
+    if (nc > 0) e.upload(c, C, size_t(nc));
+    GemmArgs g;
+    g.A = a.get();
+    g.B = b.get();
+    g.C = c.get();
+    g.M = M;
+    g.N = N;
+    g.K = K;
+    g.sam = transA ? lda : 1;
+    g.sak = transA ? 1 : lda;
+    g.sbk = transB ? ldb : 1;
+    g.sbn = transB ? 1 : ldb;
+    g.cm0 = 1;
+    g.cn0 = ldc;
+    g.dm0 = std::max(M, 1);
+    g.dn0 = std::max(N, 1);
+    g.alpha = alpha;
+    g.beta = beta;
+    gemm(e, precision, g);
+    if (nc > 0) e.download(C, c, size_t(nc));
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,30 @@
+// CCSD on the device: integrals from the vertex, the real residual, the
+// driver (update, mixers, energy, convergence).
+#pragma once
+
+#include "engine.hpp"
+
+namespace sisi {
+
+// CoulombIntegralsFromVertex::calculateRealIntegrals (cxx:203-571).
+void integrals_from_vertex(Engine &e);
+
+// CcsdEnergyFromCoulombIntegrals::getResiduum, real (cxx:27-494).  All
+// pointers are device pointers; R1/R2 are overwritten.
+void ccsd_residual(Engine &e, int iteration, bool initial_doubles, int slice_size,
+                   const double *T1, const double *T2, double *R1, double *R2);
+
+// getEnergy (ClusterSinglesDoublesAlgorithm.cxx:131-213).
+void ccsd_energy(Engine &e, const double *T1, const double *T2, double &E, double &direct,
+                 double &exchange);
+
+// Driver (ClusterSinglesDoublesAlgorithm.cxx:38-128).
+void ccsd_begin(Engine &e, const sisi_ccsd_opts &opts, const double *T1h, const double *T2h);
+void ccsd_iterate(Engine &e, int n);
+void fill_report(const Engine &e, sisi_ccsd_report *r);
+
+// Precision of the dense contractions: FP64 everywhere, FP32 everywhere, or
+// FP32 with the particle-particle ladder in split BF16x3.
+void set_precision(Engine &e, int precision);
+
+}  // namespace sisi

@@ -1,0 +1,181 @@
+// Common infrastructure of the engine: error propagation (no exceptions
+// cross the C ABI), device memory pool, strided tensor views.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/sisi_engine.h"
+
+namespace sisi {
+
+// Internal failures are C++ exceptions carrying a SISI_ERR_* code; the C ABI
+// layer converts them into a status and the thread-local message.
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string &msg);
+
+#define SISI_HIP(call)                                                      \
+  do {                                                                      \
+    hipError_t e_ = (call);                                                 \
+    if (e_ != hipSuccess)                                                   \
+      throw ::sisi::Error(SISI_ERR_HIP, std::string(#call) + ": " +         \
+                                            hipGetErrorString(e_) + " (" + \
+                                            __FILE__ + ":" +                \
+                                            std::to_string(__LINE__) + ")"); \
+  } while (0)
+
+#define SISI_REQUIRE(cond, code, msg)                                       \
+  do {                                                                      \
+    if (!(cond)) throw ::sisi::Error((code), (msg));                        \
+  } while (0)
+
+// Device memory pool: buffers are recycled by exact byte size, so the
+// allocation pattern of one CCSD iteration is served without hipMalloc after
+// the first iteration.  Not thread-safe (one engine, one host thread).
+class Pool {
+ public:
+  ~Pool() { release_all(); }
+  void *get(size_t bytes) {
+    if (bytes == 0) bytes = 8;
+    bytes = (bytes + 255) & ~size_t(255);
+    auto it = free_.find(bytes);
+    if (it != free_.end() && !it->second.empty()) {
+      void *p = it->second.back();
+      it->second.pop_back();
+      return p;
+    }
+    void *p = nullptr;
+    hipError_t err = hipMalloc(&p, bytes);
+    if (err != hipSuccess) {
+      (void)hipGetLastError();
+      // Drop cached buffers and retry once.
+      trim();
+      err = hipMalloc(&p, bytes);
+      if (err != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error(SISI_ERR_ALLOC, "device allocation of " +
+                                        std::to_string(bytes) + " bytes failed");
+      }
+    }
+    size_[p] = bytes;
+    return p;
+  }
+  void put(void *p) {
+    if (!p) return;
+    auto it = size_.find(p);
+    if (it == size_.end()) return;
+    free_[it->second].push_back(p);
+  }
+  void trim() {
+    (void)hipDeviceSynchronize();
+    for (auto &kv : free_)
+      for (void *p : kv.second) {
+        (void)hipFree(p);
+        size_.erase(p);
+      }
+    free_.clear();
+  }
+  void release_all() {
+    (void)hipDeviceSynchronize();
+    for (auto &kv : size_) (void)hipFree(kv.first);
+    size_.clear();
+    free_.clear();
+  }
+
+ private:
+  std::map<size_t, std::vector<void *>> free_;
+  std::map<void *, size_t> size_;
+};
+
+// Owning device buffer of doubles, returned to the pool on destruction.
+class DBuf {
+ public:
+  DBuf() = default;
+  DBuf(Pool *pool, size_t n) : pool_(pool), n_(n) {
+    p_ = static_cast<double *>(pool->get(n * sizeof(double)));
+  }
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  DBuf(DBuf &&o) noexcept { swap(o); }
+  DBuf &operator=(DBuf &&o) noexcept {
+    reset();
+    swap(o);
+    return *this;
+  }
+  ~DBuf() { reset(); }
+  void reset() {
+    if (pool_ && p_) pool_->put(p_);
+    p_ = nullptr;
+    n_ = 0;
+  }
+  double *get() const { return p_; }
+  size_t size() const { return n_; }
+
+ private:
+  void swap(DBuf &o) {
+    std::swap(pool_, o.pool_);
+    std::swap(p_, o.p_);
+    std::swap(n_, o.n_);
+  }
+  Pool *pool_ = nullptr;
+  double *p_ = nullptr;
+  size_t n_ = 0;
+};
+
+constexpr int kMaxOrder = 6;
+
+// Strided view of a real tensor: element (x0..x_{order-1}) lives at
+// p[sum_k x_k * stride[k]].  Dense tensors have stride[0] = 1 and
+// stride[k] = stride[k-1] * len[k-1] (first index fastest).
+struct View {
+  double *p = nullptr;
+  int order = 0;
+  int64_t len[kMaxOrder] = {0};
+  int64_t stride[kMaxOrder] = {0};
+
+  int64_t size() const {
+    int64_t n = 1;
+    for (int k = 0; k < order; ++k) n *= len[k];
+    return n;
+  }
+};
+
+inline View dense(double *p, std::initializer_list<int64_t> lens) {
+  View v;
+  v.p = p;
+  v.order = 0;
+  int64_t s = 1;
+  for (int64_t l : lens) {
+    v.len[v.order] = l;
+    v.stride[v.order] = s;
+    s *= l;
+    v.order++;
+  }
+  return v;
+}
+
+// Sub-block [lo_k, hi_k) of a view (same order).
+inline View subview(const View &v, std::initializer_list<int64_t> lo,
+                    std::initializer_list<int64_t> hi) {
+  View r = v;
+  int k = 0;
+  auto l = lo.begin();
+  auto h = hi.begin();
+  for (; k < v.order; ++k, ++l, ++h) {
+    r.p += (*l) * v.stride[k];
+    r.len[k] = (*h) - (*l);
+  }
+  return r;
+}
+
+}  // namespace sisi

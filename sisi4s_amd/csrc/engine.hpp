@@ -1,0 +1,131 @@
+// Engine state: device copies of the inputs, integrals, CCSD iteration state
+// and the kernel-family timers.  One engine = one HIP stream on the device
+// that was current at creation.
+#pragma once
+
+#include <array>
+#include <map>
+#include <string>
+#include <memory>
+#include <vector>
+
+#include "common.hpp"
+#include "gemm.hpp"
+
+namespace sisi {
+
+enum class Kind { Gemm64 = 0, Gemm32 = 1, GemmB3 = 2, Bandwidth = 3 };
+
+// Records (start, stop) event pairs around launches while stats are on.
+class Timers {
+ public:
+  explicit Timers(hipStream_t s) : stream_(s) {}
+  ~Timers();
+  void begin(Kind k, double work);
+  void end();
+  void collect(sisi_stats &out);
+  void reset();
+  bool enabled = false;
+
+ private:
+  struct Rec {
+    Kind kind;
+    double work;
+    hipEvent_t a, b;
+  };
+  hipEvent_t take();
+  hipStream_t stream_;
+  std::vector<Rec> recs_;
+  std::vector<hipEvent_t> spare_;
+  sisi_stats acc_{};
+  bool open_ = false;
+};
+
+// Scoped timing of one launch (no-op when stats are off).
+class Timed {
+ public:
+  Timed(Timers &t, Kind k, double work) : t_(t) {
+    if (t_.enabled) t_.begin(k, work);
+  }
+  ~Timed() {
+    if (t_.enabled) t_.end();
+  }
+
+ private:
+  Timers &t_;
+};
+
+struct DiisSlot {
+  DBuf t1, t2;   // estimated amplitudes
+  DBuf d1, d2;   // amplitude change
+};
+
+class Engine {
+ public:
+  explicit Engine(const sisi_dims &d);
+  ~Engine();
+
+  int No, Nv, NG, Np;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  Pool pool;
+  std::unique_ptr<Timers> timers;
+
+  // precision of general contractions and of the particle-particle ladder
+  int precision = SISI_FP64;
+  int prec_ladder = SISI_FP64;
+
+  // Operands marked constant for the duration of one residual evaluation:
+  // the contraction planner caches their permuted dense copies.
+  std::vector<const double *> frozen;
+  std::map<std::string, DBuf> perm_cache;
+  bool is_frozen(const double *p) const {
+    for (auto q : frozen)
+      if (q == p) return true;
+    return false;
+  }
+  void thaw() {
+    frozen.clear();
+    perm_cache.clear();
+  }
+
+  // inputs
+  DBuf gamma, eps_i, eps_a;
+  DBuf Gij, Gai, Gab;  // dense slices [NG,No,No], [NG,Nv,No], [NG,Nv,Nv]
+  bool have_vertex = false, have_eps = false, have_integrals = false;
+
+  // integrals
+  DBuf Vabij, Vaibj, Vijkl, Vijka;
+  DBuf Vbaij;  // V_abij with a,b exchanged (energy exchange term)
+  DBuf Uabij;  // 2 V_abij - V_baij
+
+  // CCSD run state
+  sisi_ccsd_opts opts{};
+  bool run_active = false;
+  bool run_initial = false;
+  int iteration = 0;
+  int fp64_iterations = 0;
+  int cur_precision = SISI_FP64;
+  double prev_energy = 0.0, last_energy = 0.0, last_direct = 0.0,
+         last_exchange = 0.0, last_delta = 0.0;
+  bool converged = false;
+  DBuf T1, T2;
+  std::vector<double> energies;
+  // DIIS ring (DiisMixer.cxx:48-180)
+  std::vector<DiisSlot> diis;
+  std::vector<std::vector<double>> B;  // (N+1)x(N+1)
+  int diis_next = 0, diis_count = 0;
+  // linear mixer (LinearMixer.cxx:30-55) keeps only the last vector
+  DBuf lin1, lin2;
+  bool lin_have = false;
+
+  // helpers
+  void require_inputs() const;
+  void upload(DBuf &dst, const double *src, size_t n);
+  void download(double *dst, const DBuf &src, size_t n);
+  void sync();
+  size_t n1() const { return size_t(Nv) * No; }
+  size_t n2() const { return size_t(Nv) * Nv * No * No; }
+};
+
+}  // namespace sisi

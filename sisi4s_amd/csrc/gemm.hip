@@ -1,0 +1,530 @@
+// MFMA GEMM kernels for gfx950 (CDNA4).
+//
+// Block tile 128 x 128, 256 threads = 4 wave64s in a 2 x 2 arrangement, each
+// wave owning a 64 x 64 sub-tile as 4 x 4 MFMA 16x16 tiles.  Operands are
+// staged through LDS in k-chunks; the next chunk is fetched into registers
+// while the current one is multiplied.
+//
+// MFMA operand roles: the instruction's "A" operand is fed with B(k,n) and
+// its "B" operand with A(m,k), so the 16x16 result tile is C^T in MFMA
+// terms: lane l holds C(m = l%16, n = 4*(l/16) + r), r = 0..3.  Lanes then
+// run along m, which is the unit-stride index of the column-major output,
+// and each register's 16-lane group stores 128 contiguous bytes.
+//
+//   FP64   : v_mfma_f64_16x16x4_f64, FP64 accumulation.
+//   FP32   : v_mfma_f32_16x16x4_f32 on operands rounded to FP32, FP32
+//            accumulation; the epilogue and split-K reduction are FP64.
+//   BF16X3 : v_mfma_f32_16x16x32_bf16 three times per tile,
+//            a_hi*b_hi + a_hi*b_lo + a_lo*b_hi with x = hi + lo split of the
+//            FP32-rounded operand, FP32 accumulation.
+#include "engine.hpp"
+#include "gemm.hpp"
+
+namespace sisi {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+constexpr int BM = 128, BN = 128;
+
+__device__ __forceinline__ int64_t out_off_m(const GemmArgs &g, int m) {
+  if (!g.split_out) return int64_t(m) * g.cm0;
+  int m0 = m % g.dm0;
+  int t = m / g.dm0;
+  int m1 = t % g.dm1;
+  int m2 = t / g.dm1;
+  return int64_t(m0) * g.cm0 + int64_t(m1) * g.cm1 + int64_t(m2) * g.cm2;
+}
+
+__device__ __forceinline__ int64_t out_off_n(const GemmArgs &g, int n) {
+  if (!g.split_out) return int64_t(n) * g.cn0;
+  int n0 = n % g.dn0;
+  int t = n / g.dn0;
+  int n1 = t % g.dn1;
+  int n2 = t / g.dn1;
+  return int64_t(n0) * g.cn0 + int64_t(n1) * g.cn1 + int64_t(n2) * g.cn2;
+}
+
+// Tile coordinates: groups of 8 m-tiles walk the n direction together so that
+// concurrently running blocks share A row panels and B column panels in L2.
+__device__ __forceinline__ void tile_coords(const GemmArgs &g, int pid, int &tm,
+                                            int &tn) {
+  const int group = 8;
+  int per_group = group * g.tiles_n;
+  int gid = pid / per_group;
+  int first_m = gid * group;
+  int gsize = min(g.tiles_m - first_m, group);
+  int r = pid % per_group;
+  tm = first_m + r % gsize;
+  tn = r / gsize;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 / FP32 kernel (LDS element type LT; k-chunk 16; MFMA k-step 4)
+// ---------------------------------------------------------------------------
+
+template <typename LT>
+__device__ __forceinline__ void mfma_k4(d4 &acc, double a, double b);
+
+template <int PREC, bool AK, bool BNF>
+__global__ __launch_bounds__(256, 2) void gemm_k4_kernel(GemmArgs g) {
+  using LT = typename std::conditional<PREC == SISI_FP64, double, float>::type;
+  using AT = typename std::conditional<PREC == SISI_FP64, d4, f4>::type;
+  constexpr int BK = 16;
+  constexpr int PAD = (PREC == SISI_FP64) ? 2 : 4;
+  __shared__ LT As[BK][BM + PAD];
+  __shared__ LT Bs[BK][BN + PAD];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  int tm, tn;
+  tile_coords(g, blockIdx.x, tm, tn);
+  const int ks = blockIdx.y;
+  const int bz = blockIdx.z;
+  const int m0 = tm * BM, n0 = tn * BN;
+  const int kbeg = ks * g.kchunk;
+  const int kend = min(g.K, kbeg + g.kchunk);
+
+  const double *A = g.A + int64_t(bz) * g.bsa;
+  const double *B = g.B + int64_t(bz) * g.bsb;
+
+  double ra[8], rb[8];
+
+  auto load = [&](int k0) {
+    if (!AK) {  // A m-fastest: 32 threads x 4 consecutive m, k rows t/32 and +8
+      const int mm = (tid & 31) * 4, kk = tid >> 5;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int k = k0 + kk + 8 * h;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int m = m0 + mm + j;
+          ra[h * 4 + j] = (m < g.M && k < kend) ? A[int64_t(m) * g.sam + int64_t(k) * g.sak] : 0.0;
+        }
+      }
+    } else {  // A k-fastest: 16 threads along k, m = t/16 + 16 j
+      const int kk = tid & 15, mm = tid >> 4;
+      const int k = k0 + kk;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int m = m0 + mm + 16 * j;
+        ra[j] = (m < g.M && k < kend) ? A[int64_t(m) * g.sam + int64_t(k) * g.sak] : 0.0;
+      }
+    }
+    if (BNF) {  // B n-fastest
+      const int nn = (tid & 31) * 4, kk = tid >> 5;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int k = k0 + kk + 8 * h;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int n = n0 + nn + j;
+          rb[h * 4 + j] = (n < g.N && k < kend) ? B[int64_t(k) * g.sbk + int64_t(n) * g.sbn] : 0.0;
+        }
+      }
+    } else {  // B k-fastest
+      const int kk = tid & 15, nn = tid >> 4;
+      const int k = k0 + kk;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int n = n0 + nn + 16 * j;
+        rb[j] = (n < g.N && k < kend) ? B[int64_t(k) * g.sbk + int64_t(n) * g.sbn] : 0.0;
+      }
+    }
+  };
+
+  auto store = [&]() {
+    if (!AK) {
+      const int mm = (tid & 31) * 4, kk = tid >> 5;
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) As[kk + 8 * h][mm + j] = LT(ra[h * 4 + j]);
+    } else {
+      const int kk = tid & 15, mm = tid >> 4;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) As[kk][mm + 16 * j] = LT(ra[j]);
+    }
+    if (BNF) {
+      const int nn = (tid & 31) * 4, kk = tid >> 5;
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) Bs[kk + 8 * h][nn + j] = LT(rb[h * 4 + j]);
+    } else {
+      const int kk = tid & 15, nn = tid >> 4;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) Bs[kk][nn + 16 * j] = LT(rb[j]);
+    }
+  };
+
+  AT acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = AT{0, 0, 0, 0};
+
+  const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+  const int l16 = lane & 15, lk = lane >> 4;
+
+  int k0 = kbeg;
+  if (k0 < kend) load(k0);
+  while (k0 < kend) {
+    store();
+    __syncthreads();
+    const int k1 = k0 + BK;
+    if (k1 < kend) load(k1);
+#pragma unroll
+    for (int kq = 0; kq < BK / 4; ++kq) {
+      const int k = kq * 4 + lk;
+      LT av[4], bv[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) av[i] = As[k][wm + i * 16 + l16];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) bv[j] = Bs[k][wn + j * 16 + l16];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          if constexpr (PREC == SISI_FP64)
+            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j], av[i], acc[i][j], 0, 0, 0);
+          else
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(bv[j], av[i], acc[i][j], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+    k0 = k1;
+  }
+
+  // epilogue
+  if (g.ksplit == 1) {
+    double *C = g.C + int64_t(bz) * g.bsc;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int m = m0 + wm + i * 16 + l16;
+      if (m >= g.M) continue;
+      const int64_t om = out_off_m(g, m);
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int n = n0 + wn + j * 16 + 4 * lk + r;
+          if (n < g.N) {
+            const int64_t o = om + out_off_n(g, n);
+            double v = g.alpha * double(acc[i][j][r]);
+            if (g.beta != 0.0) v += g.beta * C[o];
+            C[o] = v;
+          }
+        }
+    }
+  } else {
+    double *W = g.work + (int64_t(bz) * g.ksplit + ks) * int64_t(g.M) * g.N;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int m = m0 + wm + i * 16 + l16;
+      if (m >= g.M) continue;
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int n = n0 + wn + j * 16 + 4 * lk + r;
+          if (n < g.N) W[m + int64_t(n) * g.M] = double(acc[i][j][r]);
+        }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// BF16x3 kernel (k-chunk 32 = one MFMA k-step; LDS rows along k)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ void split_bf16(double x, __bf16 &hi, __bf16 &lo) {
+  float xf = float(x);
+  hi = __bf16(xf);
+  lo = __bf16(xf - float(hi));
+}
+
+template <bool AK, bool BNF>
+__global__ __launch_bounds__(256, 2) void gemm_bf16x3_kernel(GemmArgs g) {
+  constexpr int BK = 32;
+  constexpr int RS = BK + 8;  // row stride in bf16 (keeps 16-byte alignment)
+  __shared__ __attribute__((aligned(16))) __bf16 Ah[BM * RS];
+  __shared__ __attribute__((aligned(16))) __bf16 Al[BM * RS];
+  __shared__ __attribute__((aligned(16))) __bf16 Bh[BN * RS];
+  __shared__ __attribute__((aligned(16))) __bf16 Bl[BN * RS];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  int tm, tn;
+  tile_coords(g, blockIdx.x, tm, tn);
+  const int ks = blockIdx.y;
+  const int bz = blockIdx.z;
+  const int m0 = tm * BM, n0 = tn * BN;
+  const int kbeg = ks * g.kchunk;
+  const int kend = min(g.K, kbeg + g.kchunk);
+  const double *A = g.A + int64_t(bz) * g.bsa;
+  const double *B = g.B + int64_t(bz) * g.bsb;
+
+  // 128 x 32 elements per operand = 16 per thread
+  double ra[16], rb[16];
+  auto load = [&](int k0) {
+    if (!AK) {  // m fastest: 32 threads x 4 m, 8 k rows, 4 passes
+      const int mm = (tid & 31) * 4, kk = tid >> 5;
+#pragma unroll
+      for (int h = 0; h < 4; ++h) {
+        const int k = k0 + kk + 8 * h;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int m = m0 + mm + j;
+          ra[h * 4 + j] = (m < g.M && k < kend) ? A[int64_t(m) * g.sam + int64_t(k) * g.sak] : 0.0;
+        }
+      }
+    } else {  // k fastest: 32 threads along k, m = t/32 + 8 j
+      const int kk = tid & 31, mm = tid >> 5;
+      const int k = k0 + kk;
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        const int m = m0 + mm + 8 * j;
+        ra[j] = (m < g.M && k < kend) ? A[int64_t(m) * g.sam + int64_t(k) * g.sak] : 0.0;
+      }
+    }
+    if (BNF) {
+      const int nn = (tid & 31) * 4, kk = tid >> 5;
+#pragma unroll
+      for (int h = 0; h < 4; ++h) {
+        const int k = k0 + kk + 8 * h;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int n = n0 + nn + j;
+          rb[h * 4 + j] = (n < g.N && k < kend) ? B[int64_t(k) * g.sbk + int64_t(n) * g.sbn] : 0.0;
+        }
+      }
+    } else {
+      const int kk = tid & 31, nn = tid >> 5;
+      const int k = k0 + kk;
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        const int n = n0 + nn + 8 * j;
+        rb[j] = (n < g.N && k < kend) ? B[int64_t(k) * g.sbk + int64_t(n) * g.sbn] : 0.0;
+      }
+    }
+  };
+  auto store = [&]() {
+    __bf16 h, l;
+    if (!AK) {
+      const int mm = (tid & 31) * 4, kk = tid >> 5;
+#pragma unroll
+      for (int hh = 0; hh < 4; ++hh)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          split_bf16(ra[hh * 4 + j], h, l);
+          Ah[(mm + j) * RS + kk + 8 * hh] = h;
+          Al[(mm + j) * RS + kk + 8 * hh] = l;
+        }
+    } else {
+      const int kk = tid & 31, mm = tid >> 5;
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        split_bf16(ra[j], h, l);
+        Ah[(mm + 8 * j) * RS + kk] = h;
+        Al[(mm + 8 * j) * RS + kk] = l;
+      }
+    }
+    if (BNF) {
+      const int nn = (tid & 31) * 4, kk = tid >> 5;
+#pragma unroll
+      for (int hh = 0; hh < 4; ++hh)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          split_bf16(rb[hh * 4 + j], h, l);
+          Bh[(nn + j) * RS + kk + 8 * hh] = h;
+          Bl[(nn + j) * RS + kk + 8 * hh] = l;
+        }
+    } else {
+      const int kk = tid & 31, nn = tid >> 5;
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        split_bf16(rb[j], h, l);
+        Bh[(nn + 8 * j) * RS + kk] = h;
+        Bl[(nn + 8 * j) * RS + kk] = l;
+      }
+    }
+  };
+
+  f4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = f4{0, 0, 0, 0};
+  const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+  const int l16 = lane & 15, lk = lane >> 4;
+
+  int k0 = kbeg;
+  if (k0 < kend) load(k0);
+  while (k0 < kend) {
+    store();
+    __syncthreads();
+    const int k1 = k0 + BK;
+    if (k1 < kend) load(k1);
+    bf8 ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int row = wm + i * 16 + l16;
+      ah[i] = *reinterpret_cast<const bf8 *>(&Ah[row * RS + 8 * lk]);
+      al[i] = *reinterpret_cast<const bf8 *>(&Al[row * RS + 8 * lk]);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int col = wn + j * 16 + l16;
+      bh[j] = *reinterpret_cast<const bf8 *>(&Bh[col * RS + 8 * lk]);
+      bl[j] = *reinterpret_cast<const bf8 *>(&Bl[col * RS + 8 * lk]);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[j], ah[i], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], al[i], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], ah[i], acc[i][j], 0, 0, 0);
+      }
+    __syncthreads();
+    k0 = k1;
+  }
+
+  if (g.ksplit == 1) {
+    double *C = g.C + int64_t(bz) * g.bsc;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int m = m0 + wm + i * 16 + l16;
+      if (m >= g.M) continue;
+      const int64_t om = out_off_m(g, m);
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int n = n0 + wn + j * 16 + 4 * lk + r;
+          if (n < g.N) {
+            const int64_t o = om + out_off_n(g, n);
+            double v = g.alpha * double(acc[i][j][r]);
+            if (g.beta != 0.0) v += g.beta * C[o];
+            C[o] = v;
+          }
+        }
+    }
+  } else {
+    double *W = g.work + (int64_t(bz) * g.ksplit + ks) * int64_t(g.M) * g.N;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int m = m0 + wm + i * 16 + l16;
+      if (m >= g.M) continue;
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int n = n0 + wn + j * 16 + 4 * lk + r;
+          if (n < g.N) W[m + int64_t(n) * g.M] = double(acc[i][j][r]);
+        }
+    }
+  }
+}
+
+// Split-K reduction: C(m,n) = alpha * sum_s W[s](m,n) + beta * C(m,n), the
+// partial sums added in split order.
+__global__ void gemm_reduce_kernel(GemmArgs g) {
+  const int64_t MN = int64_t(g.M) * g.N;
+  const int bz = blockIdx.y;
+  double *C = g.C + int64_t(bz) * g.bsc;
+  const double *W = g.work + int64_t(bz) * g.ksplit * MN;
+  for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < MN;
+       e += int64_t(gridDim.x) * blockDim.x) {
+    const int m = int(e % g.M);
+    const int n = int(e / g.M);
+    double s = 0.0;
+    for (int k = 0; k < g.ksplit; ++k) s += W[k * MN + e];
+    const int64_t o = out_off_m(g, m) + out_off_n(g, n);
+    double v = g.alpha * s;
+    if (g.beta != 0.0) v += g.beta * C[o];
+    C[o] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Launcher
+// ---------------------------------------------------------------------------
+
+template <int PREC>
+static void launch_k4(GemmArgs &g, dim3 grid, hipStream_t s, bool AK, bool BNF) {
+  if (!AK && !BNF) hipLaunchKernelGGL((gemm_k4_kernel<PREC, false, false>), grid, dim3(256), 0, s, g);
+  else if (!AK && BNF) hipLaunchKernelGGL((gemm_k4_kernel<PREC, false, true>), grid, dim3(256), 0, s, g);
+  else if (AK && !BNF) hipLaunchKernelGGL((gemm_k4_kernel<PREC, true, false>), grid, dim3(256), 0, s, g);
+  else hipLaunchKernelGGL((gemm_k4_kernel<PREC, true, true>), grid, dim3(256), 0, s, g);
+}
+
+static void launch_b3(GemmArgs &g, dim3 grid, hipStream_t s, bool AK, bool BNF) {
+  if (!AK && !BNF) hipLaunchKernelGGL((gemm_bf16x3_kernel<false, false>), grid, dim3(256), 0, s, g);
+  else if (!AK && BNF) hipLaunchKernelGGL((gemm_bf16x3_kernel<false, true>), grid, dim3(256), 0, s, g);
+  else if (AK && !BNF) hipLaunchKernelGGL((gemm_bf16x3_kernel<true, false>), grid, dim3(256), 0, s, g);
+  else hipLaunchKernelGGL((gemm_bf16x3_kernel<true, true>), grid, dim3(256), 0, s, g);
+}
+
+void gemm(Engine &eng, int precision, GemmArgs g) {
+  if (g.M <= 0 || g.N <= 0 || g.batch <= 0) return;
+  if (g.K < 0) g.K = 0;  // K == 0 gives C = beta * C through the epilogue
+  SISI_REQUIRE(g.sam == 1 || g.sak == 1 || g.K <= 1 || g.M <= 1, SISI_ERR_INVALID,
+               "gemm: A needs a unit stride");
+  SISI_REQUIRE(g.sbk == 1 || g.sbn == 1 || g.K <= 1 || g.N <= 1, SISI_ERR_INVALID,
+               "gemm: B needs a unit stride");
+  // The address formula is the same for every load mapping; the mapping only
+  // decides which index consecutive threads walk (coalescing).
+  const bool AKf = (g.sak == 1) && (g.sam != 1);
+  const bool BNFf = (g.sbn == 1) && (g.sbk != 1);
+  g.tiles_m = (g.M + BM - 1) / BM;
+  g.tiles_n = (g.N + BN - 1) / BN;
+  const int64_t tiles = int64_t(g.tiles_m) * g.tiles_n;
+  SISI_REQUIRE(tiles < (int64_t(1) << 31), SISI_ERR_INVALID, "gemm: too many tiles");
+  g.split_out = !(g.dm0 >= g.M && g.dn0 >= g.N);
+  const int kq = (precision == SISI_BF16X3) ? 32 : 16;
+  // split-K when the tile grid alone cannot fill the 256 CUs
+  int ksplit = 1;
+  const int64_t want = 1024;
+  if (g.K > 4 * kq && tiles * g.batch < want) {
+    int64_t s = (want + tiles * g.batch - 1) / (tiles * g.batch);
+    int64_t smax = std::max<int64_t>(1, g.K / (8 * kq));
+    ksplit = int(std::min<int64_t>(std::min<int64_t>(s, smax), 64));
+  }
+  int kchunk = (g.K + ksplit - 1) / ksplit;
+  kchunk = ((kchunk + kq - 1) / kq) * kq;
+  if (kchunk == 0) kchunk = kq;
+  ksplit = (g.K + kchunk - 1) / kchunk;
+  if (ksplit < 1) ksplit = 1;
+  g.ksplit = ksplit;
+  g.kchunk = kchunk;
+  DBuf work;
+  if (ksplit > 1) {
+    work = DBuf(&eng.pool, size_t(ksplit) * g.batch * g.M * g.N);
+    g.work = work.get();
+  }
+  SISI_REQUIRE(g.batch <= 65535 && ksplit <= 65535, SISI_ERR_INVALID, "gemm: grid too large");
+  dim3 grid(unsigned(tiles), unsigned(ksplit), unsigned(g.batch));
+  const Kind kind = precision == SISI_FP64 ? Kind::Gemm64
+                    : precision == SISI_FP32 ? Kind::Gemm32 : Kind::GemmB3;
+  {
+    Timed t(*eng.timers, kind, gemm_flops(g));
+    if (precision == SISI_FP64) launch_k4<SISI_FP64>(g, grid, eng.stream, AKf, BNFf);
+    else if (precision == SISI_FP32) launch_k4<SISI_FP32>(g, grid, eng.stream, AKf, BNFf);
+    else launch_b3(g, grid, eng.stream, AKf, BNFf);
+    SISI_HIP(hipGetLastError());
+  }
+  if (ksplit > 1) {
+    Timed t(*eng.timers, Kind::Bandwidth, 8.0 * (ksplit + 2) * double(g.M) * g.N * g.batch);
+    const int64_t MN = int64_t(g.M) * g.N;
+    int blocks = int(std::min<int64_t>((MN + 255) / 256, 4096));
+    hipLaunchKernelGGL(gemm_reduce_kernel, dim3(blocks, g.batch), dim3(256), 0, eng.stream, g);
+    SISI_HIP(hipGetLastError());
+  }
+}
+
+}  // namespace sisi

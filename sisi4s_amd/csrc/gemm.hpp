@@ -1,0 +1,46 @@
+// MFMA GEMM family used by every dense contraction of the engine.
+//
+//   C(m,n) = alpha * sum_k A(m,k) B(k,n) + beta * C(m,n)      (per batch)
+//
+//   A(m,k) = A[m*sam + k*sak]   with sam == 1 or sak == 1
+//   B(k,n) = B[k*sbk + n*sbn]   with sbk == 1 or sbn == 1
+//   C(m,n) = C[off_m(m) + off_n(n)], where the row index m is split as
+//   m = m0 + dm0*(m1 + dm1*m2) and off_m = m0*cm0 + m1*cm1 + m2*cm2 (same
+//   for n).  The split output lets a GEMM write straight into a permuted
+//   tensor layout (e.g. the ladder's V_xycd from a (x,c) x (y,d) product).
+//
+// Split-K partial sums go to a dense FP64 workspace and are reduced in a
+// fixed order (deterministic results).
+#pragma once
+
+#include "common.hpp"
+
+namespace sisi {
+
+struct GemmArgs {
+  const double *A = nullptr;
+  const double *B = nullptr;
+  double *C = nullptr;
+  double *work = nullptr;
+  int64_t sam = 1, sak = 0, sbk = 1, sbn = 0;
+  int64_t bsa = 0, bsb = 0, bsc = 0;
+  int64_t cm0 = 1, cm1 = 0, cm2 = 0, cn0 = 0, cn1 = 0, cn2 = 0;
+  int M = 0, N = 0, K = 0;
+  int dm0 = 1, dm1 = 1, dn0 = 1, dn1 = 1;
+  int batch = 1, ksplit = 1, kchunk = 0;
+  int tiles_m = 0, tiles_n = 0;
+  int split_out = 0;  // 1: output index splits need div/mod
+  double alpha = 1.0, beta = 0.0;
+};
+
+class Engine;
+
+// Launch on the engine's stream.  precision: SISI_FP64/_FP32/_BF16X3.
+void gemm(Engine &eng, int precision, GemmArgs g);
+
+// Algorithmic flops of one call (2*M*N*K*batch).
+inline double gemm_flops(const GemmArgs &g) {
+  return 2.0 * double(g.M) * double(g.N) * double(g.K) * double(g.batch);
+}
+
+}  // namespace sisi

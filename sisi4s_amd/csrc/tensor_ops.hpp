@@ -1,0 +1,40 @@
+// Bandwidth (HBM-bound) kernels: strided copies / permutations, elementwise
+// updates and deterministic FP64 reductions.
+#pragma once
+
+#include "engine.hpp"
+
+namespace sisi {
+
+// dst = alpha * src + beta * dst over dst's logical shape; src must have the
+// same order and lengths (any strides).  beta == 0 never reads dst.
+void copy(Engine &e, const View &dst, const View &src, double alpha = 1.0,
+          double beta = 0.0);
+
+// dst = 0 (any strides).
+void zero(Engine &e, const View &dst);
+void zero(Engine &e, double *p, int64_t n);
+
+// y = a*x + b*y on contiguous arrays.
+void axpby(Engine &e, int64_t n, double a, const double *x, double b, double *y);
+
+// X[a,b,i,j] = T2[a,b,i,j] + f * T1[a,i] * T1[b,j]  (X may alias T2).
+void add_t1t1(Engine &e, double *X, const double *T2, const double *T1,
+              double f, int Nv, int No);
+
+// Deterministic FP64 sum_k x[k]*y[k] (returned to the host).
+double dot(Engine &e, int64_t n, const double *x, const double *y);
+
+// Amplitude estimate of ClusterSinglesDoublesAlgorithm.cxx:302-331 on one
+// component (order 2 for T1, 4 for T2), fused with the amplitude change:
+//   Test = -(R - shift*T) / (D + shift),  dT = Test - T,
+// D from the eigenenergies.  R is overwritten by Test.
+void estimate_amplitudes(Engine &e, int order, double *R, const double *T,
+                         double *dT, double shift);
+
+// Energy terms of getEnergy (cxx:131-189): direct = 2 sum tau V,
+// exchange = -sum tau Vbaij, tau = T2 + T1 T1.
+void energy_terms(Engine &e, const double *T1, const double *T2,
+                  double &direct, double &exchange);
+
+}  // namespace sisi

@@ -1,0 +1,15 @@
+// (T) triples on the device (CcsdPerturbativeTriples.cxx:119-248).
+#pragma once
+
+#include <array>
+#include <vector>
+
+#include "engine.hpp"
+
+namespace sisi {
+
+// E_(T) over the given sorted blocks (i0 <= i1 <= i2); all blocks when empty.
+double triples_energy(Engine &e, const double *T1dev, const double *T2dev, int precision,
+                      const std::vector<std::array<int, 3>> &blocks);
+
+}  // namespace sisi

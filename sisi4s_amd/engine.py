@@ -1,0 +1,341 @@
+"""ctypes binding of the C ABI in include/sisi_engine.h.
+
+Arrays cross the boundary as float64 buffers in the reference's CTF layout
+(first index fastest).  Python callers pass and receive logical tensors
+(T2[a,b,i,j] etc.); the conversion to the buffer layout is done here.
+
+The binding refuses to run without the compiled library: there is no CPU
+fallback for any entry point.
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsisi_engine.so")
+
+SISI_OK = 0
+SISI_ERR_INVALID = 1
+SISI_ERR_HIP = 2
+SISI_ERR_STATE = 3
+SISI_ERR_NUMERIC = 4
+SISI_ERR_ALLOC = 5
+
+FP64, FP32, BF16X3 = 0, 1, 2
+PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16x3": BF16X3}
+MIXER_LINEAR, MIXER_DIIS = 0, 1
+PPHH, PHPH, HHHH, HHHP, HHPP = 0, 1, 2, 3, 4
+
+_c_double_p = ctypes.POINTER(ctypes.c_double)
+_c_int_p = ctypes.POINTER(ctypes.c_int)
+
+
+class Dims(ctypes.Structure):
+    _fields_ = [("No", ctypes.c_int), ("Nv", ctypes.c_int), ("NG", ctypes.c_int)]
+
+
+class CcsdOpts(ctypes.Structure):
+    _fields_ = [("mixer", ctypes.c_int), ("max_residua", ctypes.c_int),
+                ("mixing_ratio", ctypes.c_double), ("max_iterations", ctypes.c_int),
+                ("amplitudes_convergence", ctypes.c_double),
+                ("energy_convergence", ctypes.c_double), ("level_shift", ctypes.c_double),
+                ("slice_size", ctypes.c_int), ("precision", ctypes.c_int),
+                ("switch_threshold", ctypes.c_double)]
+
+
+class CcsdReport(ctypes.Structure):
+    _fields_ = [("energy", ctypes.c_double), ("direct", ctypes.c_double),
+                ("exchange", ctypes.c_double), ("delta", ctypes.c_double),
+                ("iterations", ctypes.c_int), ("converged", ctypes.c_int),
+                ("fp64_iterations", ctypes.c_int), ("precision", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("gemm_seconds", ctypes.c_double * 3), ("gemm_flops", ctypes.c_double * 3),
+                ("gemm_launches", ctypes.c_int64 * 3), ("other_seconds", ctypes.c_double),
+                ("other_bytes", ctypes.c_double), ("other_launches", ctypes.c_int64)]
+
+    def as_dict(self):
+        return dict(gemm_seconds=list(self.gemm_seconds), gemm_flops=list(self.gemm_flops),
+                    gemm_launches=list(self.gemm_launches), other_seconds=self.other_seconds,
+                    other_bytes=self.other_bytes, other_launches=self.other_launches)
+
+
+class EngineError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("sisi engine error %d: %s" % (code, msg))
+        self.code = code
+
+
+# (name, restype, argtypes) of every exported function
+SIGNATURES = [
+    ("sisi_last_error", ctypes.c_char_p, []),
+    ("sisi_ccsd_opts_default", None, [ctypes.POINTER(CcsdOpts)]),
+    ("sisi_engine_create", ctypes.c_int, [ctypes.POINTER(Dims), ctypes.POINTER(ctypes.c_void_p)]),
+    ("sisi_engine_destroy", ctypes.c_int, [ctypes.c_void_p]),
+    ("sisi_engine_set_vertex", ctypes.c_int, [ctypes.c_void_p, _c_double_p]),
+    ("sisi_engine_set_eigenenergies", ctypes.c_int, [ctypes.c_void_p, _c_double_p, _c_double_p]),
+    ("sisi_integrals_from_vertex", ctypes.c_int, [ctypes.c_void_p]),
+    ("sisi_get_integrals", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _c_double_p]),
+    ("sisi_engine_set_precision", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("sisi_ccsd_residual", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_double_p, _c_double_p,
+      _c_double_p, _c_double_p]),
+    ("sisi_ccsd_energy", ctypes.c_int, [ctypes.c_void_p, _c_double_p, _c_double_p, _c_double_p]),
+    ("sisi_mp2_energy", ctypes.c_int, [ctypes.c_void_p, _c_double_p]),
+    ("sisi_ccsd_begin", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.POINTER(CcsdOpts), _c_double_p, _c_double_p]),
+    ("sisi_ccsd_iterate", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(CcsdReport)]),
+    ("sisi_ccsd_solve", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.POINTER(CcsdOpts), _c_double_p, _c_double_p,
+      ctypes.POINTER(CcsdReport)]),
+    ("sisi_ccsd_get_amplitudes", ctypes.c_int, [ctypes.c_void_p, _c_double_p, _c_double_p]),
+    ("sisi_ccsd_get_energies", ctypes.c_int, [ctypes.c_void_p, _c_double_p, ctypes.c_int, _c_int_p]),
+    ("sisi_triples_energy", ctypes.c_int,
+     [ctypes.c_void_p, _c_double_p, _c_double_p, ctypes.c_int, _c_double_p]),
+    ("sisi_triples_energy_blocks", ctypes.c_int,
+     [ctypes.c_void_p, _c_double_p, _c_double_p, ctypes.c_int, _c_int_p, ctypes.c_int,
+      _c_double_p]),
+    ("sisi_engine_enable_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("sisi_engine_get_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Stats)]),
+    ("sisi_engine_reset_stats", ctypes.c_int, [ctypes.c_void_p]),
+    ("sisi_generate_system", ctypes.c_int,
+     [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_double_p, _c_double_p,
+      _c_double_p]),
+    ("sisi_gemm_host", ctypes.c_int,
+     [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+      ctypes.c_double, _c_double_p, ctypes.c_int, _c_double_p, ctypes.c_int, ctypes.c_double,
+      _c_double_p, ctypes.c_int]),
+]
+
+_lib = None
+
+
+def load(path=None):
+    """Load the engine library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise FileNotFoundError(
+            "HIP engine library %s is missing: build it with `python -m sisi4s_amd.build`" % p)
+    lib = ctypes.CDLL(p)
+    for name, res, args in SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _check(rc):
+    if rc != SISI_OK:
+        msg = load().sisi_last_error()
+        raise EngineError(rc, msg.decode() if msg else "")
+
+
+def _buf(x):
+    """Logical float64 tensor -> contiguous first-index-fastest buffer."""
+    b = np.asarray(x, dtype=np.float64).ravel(order="F")
+    return np.ascontiguousarray(b)
+
+
+def _ptr(b):
+    return b.ctypes.data_as(_c_double_p) if b is not None else None
+
+
+def _unbuf(b, shape):
+    return b.reshape(shape, order="F")
+
+
+def default_opts(**kw):
+    o = CcsdOpts()
+    load().sisi_ccsd_opts_default(ctypes.byref(o))
+    for k, v in kw.items():
+        if k == "mixer" and isinstance(v, str):
+            v = {"LinearMixer": MIXER_LINEAR, "DiisMixer": MIXER_DIIS}[v]
+        if k == "precision" and isinstance(v, str):
+            v = PRECISIONS[v]
+        setattr(o, k, v)
+    return o
+
+
+def generate_system(seed, No, Nv, NG):
+    """Seeded synthetic (Gamma[G,p,q], eps_i, eps_a) from the engine's
+    host generator (no device needed)."""
+    Np = No + Nv
+    g = np.empty(NG * Np * Np)
+    ei = np.empty(No)
+    ea = np.empty(Nv)
+    _check(load().sisi_generate_system(int(seed), No, Nv, NG, _ptr(g), _ptr(ei), _ptr(ea)))
+    return _unbuf(g, (NG, Np, Np)), ei, ea
+
+
+def gemm(A, B, C=None, alpha=1.0, beta=0.0, transA=False, transB=False, precision=FP64):
+    """C = alpha op(A) op(B) + beta C through one MFMA GEMM launch."""
+    A = np.asfortranarray(A, dtype=np.float64)
+    B = np.asfortranarray(B, dtype=np.float64)
+    M = A.shape[1] if transA else A.shape[0]
+    K = A.shape[0] if transA else A.shape[1]
+    N = B.shape[0] if transB else B.shape[1]
+    Cb = np.zeros((M, N), order="F") if C is None else np.array(C, dtype=np.float64, order="F")
+    _check(load().sisi_gemm_host(int(precision), int(transA), int(transB), M, N, K, alpha,
+                                 A.ctypes.data_as(_c_double_p), max(A.shape[0], 1),
+                                 B.ctypes.data_as(_c_double_p), max(B.shape[0], 1), beta,
+                                 Cb.ctypes.data_as(_c_double_p), max(M, 1)))
+    return Cb
+
+
+class Engine:
+    """One engine handle = device copies of one system + one HIP stream."""
+
+    def __init__(self, No, Nv, NG):
+        self.No, self.Nv, self.NG = No, Nv, NG
+        self.Np = No + Nv
+        self._lib = load()
+        h = ctypes.c_void_p()
+        _check(self._lib.sisi_engine_create(ctypes.byref(Dims(No, Nv, NG)), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.sisi_engine_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # --- inputs -----------------------------------------------------------
+    def set_vertex(self, gamma):
+        g = _buf(gamma)
+        assert g.size == self.NG * self.Np * self.Np
+        _check(self._lib.sisi_engine_set_vertex(self._h, _ptr(g)))
+
+    def set_eigenenergies(self, eps_i, eps_a):
+        ei = _buf(eps_i)
+        ea = _buf(eps_a)
+        _check(self._lib.sisi_engine_set_eigenenergies(self._h, _ptr(ei), _ptr(ea)))
+
+    def set_system(self, gamma, eps_i, eps_a, integrals=True):
+        self.set_vertex(gamma)
+        self.set_eigenenergies(eps_i, eps_a)
+        if integrals:
+            self.integrals_from_vertex()
+
+    def integrals_from_vertex(self):
+        _check(self._lib.sisi_integrals_from_vertex(self._h))
+
+    def get_integrals(self, which):
+        o, v = self.No, self.Nv
+        shapes = {PPHH: (v, v, o, o), PHPH: (v, o, v, o), HHHH: (o, o, o, o),
+                  HHHP: (o, o, o, v), HHPP: (o, o, v, v)}
+        shape = shapes[which]
+        b = np.empty(int(np.prod(shape)))
+        _check(self._lib.sisi_get_integrals(self._h, which, _ptr(b)))
+        return _unbuf(b, shape)
+
+    def set_precision(self, precision):
+        if isinstance(precision, str):
+            precision = PRECISIONS[precision]
+        _check(self._lib.sisi_engine_set_precision(self._h, int(precision)))
+
+    # --- residual / energies ---------------------------------------------
+    def residual(self, iteration, T1, T2, slice_size=-1, initial_doubles=False):
+        v, o = self.Nv, self.No
+        t1, t2 = _buf(T1), _buf(T2)
+        r1 = np.empty(v * o)
+        r2 = np.empty(v * v * o * o)
+        _check(self._lib.sisi_ccsd_residual(self._h, iteration, int(initial_doubles),
+                                            int(slice_size), _ptr(t1), _ptr(t2), _ptr(r1),
+                                            _ptr(r2)))
+        return _unbuf(r1, (v, o)), _unbuf(r2, (v, v, o, o))
+
+    def energy(self, T1, T2):
+        out = np.empty(3)
+        _check(self._lib.sisi_ccsd_energy(self._h, _ptr(_buf(T1)), _ptr(_buf(T2)), _ptr(out)))
+        return tuple(out)
+
+    def mp2_energy(self):
+        e = ctypes.c_double()
+        _check(self._lib.sisi_mp2_energy(self._h, ctypes.byref(e)))
+        return e.value
+
+    # --- driver ------------------------------------------------------------
+    def ccsd_begin(self, opts=None, T1=None, T2=None):
+        o = opts if opts is not None else default_opts()
+        t1 = _buf(T1) if T1 is not None else None
+        t2 = _buf(T2) if T2 is not None else None
+        _check(self._lib.sisi_ccsd_begin(self._h, ctypes.byref(o), _ptr(t1), _ptr(t2)))
+
+    def ccsd_iterate(self, n):
+        r = CcsdReport()
+        _check(self._lib.sisi_ccsd_iterate(self._h, int(n), ctypes.byref(r)))
+        return r.as_dict()
+
+    def ccsd_solve(self, opts=None, T1=None, T2=None):
+        o = opts if opts is not None else default_opts()
+        t1 = _buf(T1) if T1 is not None else None
+        t2 = _buf(T2) if T2 is not None else None
+        r = CcsdReport()
+        _check(self._lib.sisi_ccsd_solve(self._h, ctypes.byref(o), _ptr(t1), _ptr(t2),
+                                         ctypes.byref(r)))
+        return r.as_dict()
+
+    def amplitudes(self):
+        v, o = self.Nv, self.No
+        t1 = np.empty(v * o)
+        t2 = np.empty(v * v * o * o)
+        _check(self._lib.sisi_ccsd_get_amplitudes(self._h, _ptr(t1), _ptr(t2)))
+        return _unbuf(t1, (v, o)), _unbuf(t2, (v, v, o, o))
+
+    def energies(self):
+        n = ctypes.c_int()
+        _check(self._lib.sisi_ccsd_get_energies(self._h, None, 0, ctypes.byref(n)))
+        out = np.empty(max(n.value, 1))
+        _check(self._lib.sisi_ccsd_get_energies(self._h, _ptr(out), n.value, ctypes.byref(n)))
+        return out[:n.value].copy()
+
+    # --- (T) ---------------------------------------------------------------
+    def triples(self, T1=None, T2=None, precision=FP64, blocks=None):
+        if isinstance(precision, str):
+            precision = PRECISIONS[precision]
+        e = ctypes.c_double()
+        t1 = _buf(T1) if T1 is not None else None
+        t2 = _buf(T2) if T2 is not None else None
+        if blocks is None:
+            _check(self._lib.sisi_triples_energy(self._h, _ptr(t1), _ptr(t2), int(precision),
+                                                 ctypes.byref(e)))
+        else:
+            bl = np.ascontiguousarray(np.asarray(blocks, dtype=np.int32).reshape(-1))
+            _check(self._lib.sisi_triples_energy_blocks(
+                self._h, _ptr(t1), _ptr(t2), int(precision), bl.ctypes.data_as(_c_int_p),
+                len(bl) // 3, ctypes.byref(e)))
+        return e.value
+
+    # --- timers ------------------------------------------------------------
+    def enable_stats(self, on=True):
+        _check(self._lib.sisi_engine_enable_stats(self._h, int(bool(on))))
+
+    def reset_stats(self):
+        _check(self._lib.sisi_engine_reset_stats(self._h))
+
+    def stats(self):
+        s = Stats()
+        _check(self._lib.sisi_engine_get_stats(self._h, ctypes.byref(s)))
+        return s.as_dict()

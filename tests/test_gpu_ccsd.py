@@ -1,0 +1,151 @@
+"""CCSD parity: the HIP path through the C ABI against the FP64 oracle on the
+same seeded inputs.  Tolerances: FP64 path energies to 1e-8 Eh (config 1
+target), residual elements to 1e-11 relative to the residual's max norm."""
+
+import numpy as np
+import pytest
+
+from conftest import announce, random_amplitudes
+from oracle import ccsd, formulations, philox
+
+pytestmark = pytest.mark.gpu
+
+
+def make_engine(seed, No, Nv, NG):
+    from sisi4s_amd import engine
+    g, ei, ea = philox.system(seed, No, Nv, NG)
+    e = engine.Engine(No, Nv, NG)
+    e.set_system(g, ei, ea)
+    return e, g, ei, ea
+
+
+def test_integrals_match_oracle():
+    from sisi4s_amd import engine
+    announce("integrals No=10 Nv=50 NG=300")
+    e, g, ei, ea = make_engine(20261202, 10, 50, 300)
+    ints = ccsd.integrals_from_vertex(g, 10)
+    for cls, name in [(engine.PPHH, "PPHH"), (engine.PHPH, "PHPH"), (engine.HHHH, "HHHH"),
+                      (engine.HHHP, "HHHP"), (engine.HHPP, "HHPP")]:
+        got = e.get_integrals(cls)
+        ref = ints[name]
+        assert got.shape == ref.shape
+        assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max(), name
+
+
+@pytest.mark.parametrize("No,Nv,NG,slices", [(4, 9, 20, (1, 2, 4, 9)), (3, 13, 17, (5, 13)),
+                                             (10, 50, 300, (10, 16))])
+def test_residual_matches_oracle(No, Nv, NG, slices):
+    e, g, ei, ea = make_engine(31 + No, No, Nv, NG)
+    ints = ccsd.integrals_from_vertex(g, No)
+    T1, T2 = random_amplitudes(Nv, No, seed=No + Nv)
+    announce("residual it=0 No=%d Nv=%d NG=%d" % (No, Nv, NG))
+    R1, R2 = e.residual(0, T1, T2)
+    np.testing.assert_array_equal(R2, ints["PPHH"])
+    assert np.abs(R1).max() == 0.0
+    for s in slices:
+        announce("residual it=1 No=%d Nv=%d NG=%d slice=%d" % (No, Nv, NG, s))
+        R1, R2 = e.residual(1, T1, T2, slice_size=s)
+        O1, O2 = ccsd.residual(1, T1, T2, g, ints, slice_size=s)
+        scale = np.abs(O2).max()
+        assert np.abs(R2 - O2).max() <= 1e-11 * scale
+        assert np.abs(R1 - O1).max() <= 1e-11 * max(np.abs(O1).max(), 1e-300) + 1e-13 * scale
+
+
+def test_residual_initial_doubles_and_energy():
+    No, Nv, NG = 4, 9, 20
+    e, g, ei, ea = make_engine(5, No, Nv, NG)
+    ints = ccsd.integrals_from_vertex(g, No)
+    T1, T2 = random_amplitudes(Nv, No, seed=9)
+    announce("residual initial doubles it=0")
+    R1, R2 = e.residual(0, T1, T2, initial_doubles=True)
+    O1, O2 = ccsd.residual(0, T1, T2, g, ints, initial_doubles=True)
+    assert np.abs(R2 - O2).max() <= 1e-11 * np.abs(O2).max()
+    announce("energy")
+    E, d, x = e.energy(T1, T2)
+    Eo, do, xo = ccsd.energy(T1, T2, ints["PPHH"])
+    assert abs(E - Eo) <= 1e-12 * abs(Eo) and abs(d - do) <= 1e-12 * abs(do)
+    announce("mp2")
+    assert abs(e.mp2_energy() - formulations.mp2_closed_form(g, ei, ea)) <= 1e-12 * abs(
+        formulations.mp2_closed_form(g, ei, ea))
+
+
+def test_config1_diis6_30_iterations():
+    """Config 1: No=10, Nv=50, NG=300, MP2 start, DIIS(6), 30 iterations;
+    every iteration's energy within 1e-8 Eh of the oracle."""
+    from sisi4s_amd import engine
+    No, Nv, NG = 10, 50, 300
+    e, g, ei, ea = make_engine(20261202, No, Nv, NG)
+    opts = engine.default_opts(mixer="DiisMixer", max_residua=6, max_iterations=30,
+                               amplitudes_convergence=1e-12, energy_convergence=1e-14)
+    announce("config1 DIIS(6) 30 iterations")
+    rep = e.ccsd_solve(opts)
+    gpu = e.energies()
+    ref = ccsd.solve(g, ei, ea, mixer="DiisMixer", max_residua=6, max_iterations=30,
+                     amplitudes_convergence=1e-12, energy_convergence=1e-14)
+    assert rep["iterations"] == 30 and len(gpu) == 30
+    diff = np.abs(gpu - np.array(ref["energies"]))
+    assert diff.max() <= 1e-8, diff
+    T1, T2 = e.amplitudes()
+    assert np.abs(T2 - ref["T2"]).max() <= 1e-8 * np.abs(ref["T2"]).max()
+
+
+def test_linear_mixer_and_convergence_flag():
+    from sisi4s_amd import engine
+    No, Nv, NG = 4, 9, 20
+    e, g, ei, ea = make_engine(77, No, Nv, NG)
+    opts = engine.default_opts(mixer="LinearMixer", mixing_ratio=0.7, max_iterations=6,
+                               slice_size=3)
+    announce("linear mixer")
+    rep = e.ccsd_solve(opts)
+    ref = ccsd.solve(g, ei, ea, mixer="LinearMixer", mixing_ratio=0.7, max_iterations=6,
+                     slice_size=3)
+    np.testing.assert_allclose(e.energies(), ref["energies"], rtol=0, atol=1e-10)
+    assert rep["converged"] == int(ref["converged"])
+
+
+def test_initial_amplitudes_and_zero_iterations():
+    from sisi4s_amd import engine
+    No, Nv, NG = 4, 9, 20
+    e, g, ei, ea = make_engine(78, No, Nv, NG)
+    T1, T2 = random_amplitudes(Nv, No, seed=1, scale=0.02)
+    announce("initial amplitudes, 3 iterations")
+    opts = engine.default_opts(mixer="DiisMixer", max_iterations=3)
+    e.ccsd_solve(opts, T1=T1, T2=T2)
+    ref = ccsd.solve(g, ei, ea, mixer="DiisMixer", max_iterations=3, initial=(T1, T2))
+    np.testing.assert_allclose(e.energies(), ref["energies"], rtol=0, atol=1e-10)
+    announce("maxIterations 0")
+    rep = e.ccsd_solve(engine.default_opts(max_iterations=0), T1=T1, T2=T2)
+    Eo = ccsd.energy(T1, T2, ccsd.integrals_from_vertex(g, No)["PPHH"])[0]
+    assert abs(rep["energy"] - Eo) <= 1e-12 * abs(Eo)
+
+
+@pytest.mark.parametrize("prec,tol", [("fp32", 1e-5), ("bf16x3", 1e-4)])
+def test_reduced_precision_residual_error(prec, tol):
+    No, Nv, NG = 10, 50, 300
+    e, g, ei, ea = make_engine(20261202, No, Nv, NG)
+    ints = ccsd.integrals_from_vertex(g, No)
+    T1, T2 = random_amplitudes(Nv, No, seed=2, scale=0.02)
+    e.set_precision(prec)
+    announce("residual precision=%s" % prec)
+    R1, R2 = e.residual(1, T1, T2)
+    O1, O2 = ccsd.residual(1, T1, T2, g, ints)
+    assert np.abs(R2 - O2).max() <= tol * np.abs(O2).max()
+    e.set_precision("fp64")
+
+
+def test_mixed_precision_solve_reaches_fp64_energy():
+    """FP32/BF16x3 bulk iterations, FP64 refinement: the converged energy
+    matches the FP64 run to 1 uEh."""
+    from sisi4s_amd import engine
+    No, Nv, NG = 6, 24, 60
+    e, g, ei, ea = make_engine(12, No, Nv, NG)
+    kw = dict(mixer="DiisMixer", max_residua=6, max_iterations=80,
+              amplitudes_convergence=1e-8, energy_convergence=1e-10)
+    announce("fp64 solve small")
+    r64 = e.ccsd_solve(engine.default_opts(**kw))
+    assert r64["converged"] == 1
+    announce("bf16x3 mixed solve small")
+    rmx = e.ccsd_solve(engine.default_opts(precision="bf16x3", switch_threshold=1e-5, **kw))
+    assert rmx["converged"] == 1
+    assert rmx["fp64_iterations"] < rmx["iterations"]
+    assert abs(rmx["energy"] - r64["energy"]) <= 1e-6

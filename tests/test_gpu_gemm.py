@@ -1,0 +1,46 @@
+"""MFMA GEMM family through the C ABI (sisi_gemm_host) against numpy FP64."""
+
+import numpy as np
+import pytest
+
+from conftest import announce
+
+pytestmark = pytest.mark.gpu
+
+# error bound relative to max_{mn} sum_k |A_mk||B_kn|
+TOL = {0: 1e-14, 1: 4e-7, 2: 4e-5}
+
+
+def _run(prec, ta, tb, M, N, K, alpha=1.0, beta=0.0, seed=0):
+    from sisi4s_amd import engine
+    rng = np.random.default_rng(seed)
+    A = rng.standard_normal((K, M) if ta else (M, K))
+    B = rng.standard_normal((N, K) if tb else (K, N))
+    C0 = rng.standard_normal((M, N))
+    announce("gemm prec=%d ta=%d tb=%d M=%d N=%d K=%d" % (prec, ta, tb, M, N, K))
+    C = engine.gemm(A, B, C0, alpha=alpha, beta=beta, transA=ta, transB=tb, precision=prec)
+    Aop = A.T if ta else A
+    Bop = B.T if tb else B
+    ref = alpha * Aop @ Bop + beta * C0
+    bound = np.abs(alpha) * (np.abs(Aop) @ np.abs(Bop)).max() + abs(beta) * np.abs(C0).max()
+    err = np.abs(C - ref).max()
+    assert err <= TOL[prec] * max(bound, 1.0) * max(1.0, np.sqrt(K) / 8), (err, bound)
+
+
+@pytest.mark.parametrize("prec", [0, 1, 2])
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+@pytest.mark.parametrize("M,N,K", [(1, 1, 1), (17, 33, 5), (130, 129, 70), (256, 384, 96)])
+def test_gemm_shapes(prec, ta, tb, M, N, K):
+    _run(prec, ta, tb, M, N, K, seed=M + N + K)
+
+
+@pytest.mark.parametrize("prec", [0, 1, 2])
+def test_gemm_alpha_beta_and_empty_k(prec):
+    _run(prec, 0, 0, 40, 50, 60, alpha=-0.5, beta=2.0)
+    _run(prec, 1, 0, 40, 50, 0, alpha=1.0, beta=-3.0)
+
+
+@pytest.mark.parametrize("prec", [0, 1, 2])
+def test_gemm_split_k(prec):
+    # few output tiles and a long k: the launcher splits k and reduces in FP64
+    _run(prec, 1, 0, 64, 48, 40000, seed=3)

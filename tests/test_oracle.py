@@ -1,0 +1,156 @@
+"""Pins the CPU oracle (test infrastructure) against independent checks:
+numpy's Philox, the reference's full-integral CCSD formulation
+(CcsdEnergyFromCoulombIntegralsReference.cxx), the unsliced 6-index (T)
+(PerturbativeTriples.cxx:19-97), the MP2 closed form, DIIS identities,
+symmetries and limits."""
+
+import numpy as np
+import pytest
+
+from conftest import random_amplitudes
+from oracle import ccsd, formulations, philox, triples
+
+
+def test_philox_matches_numpy_bit_generator():
+    # numpy.random.Philox is Philox-4x64-10; it increments its counter
+    # before each block, so counter c-1 yields philox(c).
+    for key, ctr in [((5, 7), (10, 0, 0, 0)), ((2**63 + 11, 3), (123456789, 42, 7, 1)),
+                     ((0, 0), (1, 0, 0, 0))]:
+        bg = np.random.Philox(key=np.array(key, dtype=np.uint64),
+                              counter=np.array((ctr[0] - 1,) + ctr[1:], dtype=np.uint64))
+        ref = bg.random_raw(4)
+        out = philox.philox4x64(tuple(np.uint64(c) for c in ctr), tuple(np.uint64(k) for k in key))
+        assert [int(x) for x in out] == [int(x) for x in ref]
+
+
+def test_generator_properties():
+    g, ei, ea = philox.system(3, 5, 12, 40)
+    assert g.shape == (40, 17, 17)
+    np.testing.assert_array_equal(g, g.transpose(0, 2, 1))
+    assert ei.min() >= -2.0 and ei.max() < -0.5
+    assert ea.min() >= 0.2 and ea.max() < 3.0
+    # N(0,1) scaled by exp(-G/100)/sqrt(NG): check the row variance trend
+    z = g * np.sqrt(40.0) / np.exp(-np.arange(40) / 100.0)[:, None, None]
+    assert abs(z.std() - 1.0) < 0.05 and abs(z.mean()) < 0.05
+
+
+@pytest.mark.parametrize("No,Nv,NG", [(3, 7, 11), (4, 9, 20)])
+def test_residual_matches_full_integral_formulation(No, Nv, NG):
+    g, ei, ea = philox.system(11, No, Nv, NG)
+    T1, T2 = random_amplitudes(Nv, No, seed=No)
+    ints = ccsd.integrals_from_vertex(g, No)
+    F1, F2 = formulations.residual_full_integrals(1, T1, T2, g, No)
+    scale = np.abs(F2).max()
+    for s in sorted({1, 2, 3, No, Nv}):
+        R1, R2 = ccsd.residual(1, T1, T2, g, ints, slice_size=s)
+        assert np.abs(R1 - F1).max() <= 1e-12 * scale
+        assert np.abs(R2 - F2).max() <= 1e-12 * scale
+
+
+def test_residual_planted_error_is_detected():
+    # A dropped PPL transpose (the a > b branch of sliceIntoResiduum) must be
+    # caught by the full-integral comparison.
+    No, Nv, NG = 3, 7, 11
+    g, _, _ = philox.system(11, No, Nv, NG)
+    T1, T2 = random_amplitudes(Nv, No, seed=2)
+    ints = ccsd.integrals_from_vertex(g, No)
+    orig = ccsd.slice_into_residuum
+
+    def broken(Rxyij, a, b, R2):
+        R2[a:a + Rxyij.shape[0], b:b + Rxyij.shape[1]] += Rxyij
+
+    ccsd.slice_into_residuum = broken
+    try:
+        _, R2 = ccsd.residual(1, T1, T2, g, ints, slice_size=2)
+    finally:
+        ccsd.slice_into_residuum = orig
+    _, F2 = formulations.residual_full_integrals(1, T1, T2, g, No)
+    assert np.abs(R2 - F2).max() > 1e-3 * np.abs(F2).max()
+
+
+def test_residual_limits_and_symmetry():
+    No, Nv, NG = 3, 7, 11
+    g, _, _ = philox.system(5, No, Nv, NG)
+    ints = ccsd.integrals_from_vertex(g, No)
+    zero1, zero2 = np.zeros((Nv, No)), np.zeros((Nv, Nv, No, No))
+    # MP2 branch and the T = 0 limit both give R2 = V_abij, R1 = 0
+    for it in (0, 3):
+        R1, R2 = ccsd.residual(it, zero1, zero2, g, ints)
+        np.testing.assert_allclose(R2, ints["PPHH"], rtol=0, atol=1e-14)
+        assert np.abs(R1).max() == 0.0
+    T1, T2 = random_amplitudes(Nv, No, seed=3)
+    _, R2 = ccsd.residual(2, T1, T2, g, ints, slice_size=2)
+    np.testing.assert_allclose(R2, R2.transpose(1, 0, 3, 2), rtol=0, atol=1e-13 * np.abs(R2).max())
+
+
+@pytest.mark.parametrize("No,Nv,NG", [(3, 5, 9), (4, 6, 13)])
+def test_triples_sliced_matches_unsliced(No, Nv, NG):
+    g, ei, ea = philox.system(21, No, Nv, NG)
+    T1, T2 = random_amplitudes(Nv, No, seed=4)
+    e_sl = triples.triples_energy(T1, T2, g, ei, ea)
+    e_un = formulations.triples_unsliced(T1, T2, g, ei, ea)
+    assert abs(e_sl - e_un) <= 1e-12 * abs(e_un)
+
+
+def test_triples_diagonal_blocks_vanish_and_relabel_invariance():
+    No, Nv, NG = 3, 5, 9
+    g, ei, ea = philox.system(22, No, Nv, NG)
+    T1, T2 = random_amplitudes(Nv, No, seed=5)
+    inp = triples.TriplesInputs(T1, T2, g, ei, ea)
+    for k in range(No):
+        assert abs(triples.block_energy(inp, (k, k, k))) < 1e-12
+    e0 = triples.triples_energy(T1, T2, g, ei, ea)
+    perm = np.array([2, 0, 1])
+    gp = g.copy()
+    gp[:, :No, :] = g[:, perm, :]
+    gp2 = gp.copy()
+    gp2[:, :, :No] = gp[:, :, perm]
+    e1 = triples.triples_energy(T1[:, perm], T2[:, :, perm][:, :, :, perm], gp2, ei[perm], ea)
+    assert abs(e1 - e0) <= 1e-12 * abs(e0)
+
+
+def test_mp2_driver_matches_closed_form():
+    g, ei, ea = philox.system(9, 4, 10, 30)
+    e_drv = ccsd.mp2_energy(g, ei, ea)
+    e_cf = formulations.mp2_closed_form(g, ei, ea)
+    assert abs(e_drv - e_cf) <= 1e-13 * abs(e_cf)
+    r = ccsd.solve(g, ei, ea, mixer="DiisMixer", max_residua=4, max_iterations=1)
+    assert abs(r["energies"][0] - e_cf) <= 1e-13 * abs(e_cf)
+
+
+def test_diis_weights():
+    rng = np.random.default_rng(0)
+    mix = ccsd.DiisMixer(3)
+    vecs = [[rng.standard_normal(5), rng.standard_normal(7)] for _ in range(5)]
+    res = [[rng.standard_normal(5), rng.standard_normal(7)] for _ in range(5)]
+    mix.append(vecs[0], res[0])
+    assert mix.weights == [1.0]
+    np.testing.assert_allclose(mix.get()[0], vecs[0][0])
+    for k in range(1, 5):
+        mix.append(vecs[k], res[k])
+        assert abs(sum(mix.weights) - 1.0) < 1e-12
+    # brute force: minimise |sum_j w_j r_j|^2 subject to sum w = 1 over the
+    # stored (ring) vectors, newest first
+    stored = [res[4], res[3], res[2]]
+    Rm = np.array([np.concatenate(r) for r in stored])
+    Bm = Rm @ Rm.T
+    w = np.linalg.solve(Bm, np.ones(3))
+    w /= w.sum()
+    np.testing.assert_allclose(mix.weights, w, rtol=1e-10, atol=1e-12)
+
+
+def test_golden_fixture_regression():
+    d = np.load(__import__("os").path.join(__import__("os").path.dirname(__file__), "golden",
+                                           "tiny_system.npz"))
+    No, Nv, NG = (int(x) for x in d["dims"])
+    g, ei, ea = philox.system(int(d["seed"]), No, Nv, NG)
+    np.testing.assert_allclose(g, d["gamma"], rtol=0, atol=1e-15)
+    ints = ccsd.integrals_from_vertex(g, No)
+    R1, R2 = ccsd.residual(1, d["T1"], d["T2"], g, ints, slice_size=int(d["slice"]))
+    np.testing.assert_allclose(R1, d["R1"], rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(R2, d["R2"], rtol=1e-12, atol=1e-14)
+    et = triples.triples_energy(d["T1"], d["T2"], g, ei, ea)
+    assert abs(et - float(d["E_T"])) <= 1e-12 * abs(float(d["E_T"]))
+    r = ccsd.solve(g, ei, ea, mixer="DiisMixer", max_residua=6, max_iterations=8,
+                   amplitudes_convergence=0.0, energy_convergence=0.0)
+    np.testing.assert_allclose(r["energies"], d["energies"], rtol=1e-11)
