@@ -1,0 +1,223 @@
+#!/usr/bin/env python3
+"""CCSD / (T) benchmark on one MI355X.
+
+Workload (a "step" is one CCSD iteration): plain FP64 CCSD with No=40
+occupied, Nv=400 virtual, NG=1600 auxiliary orbitals, the vertex-built ladder
+in virtual slices of 64, DIIS(6), MP2 start, on a seeded synthetic vertex.
+All tensors are resident in HBM when the timed region starts.
+
+Printed as one JSON line:
+  value        seconds per FP64 CCSD iteration (lower is better)
+  roofline     the FP64 MFMA GEMM kernel family, timed with HIP events on
+               the engine's stream (algorithmic 2*M*N*K per launch)
+  cpu_baseline the FP64 CPU oracle timed on a sample of the same workload
+  extra        iteration TFLOP/s from the reference's algorithmic flop
+               count, split-BF16x3 iteration time, (T) block sample
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("CCSD iter time (s) + (T) time, FP64 vs mixed at |ΔE|≤1µEh; "
+          "TFLOP/s % peak")
+PEAK_FP64_TFLOPS = 78.6
+
+
+def slice_pairs(Nv, s):
+    n = [min(s, Nv - k) for k in range(0, Nv, s)]
+    return (sum(n) ** 2 + sum(x * x for x in n)) / 2.0
+
+
+def ccsd_iteration_flops(No, Nv, NG, s):
+    """Algorithmic flops of one CCSD iteration (real arithmetic, 2*M*N*K per
+    term): ladder 2 P(s) Nv^2 (NG + No^2), seven O(No^3 Nv^3) ring products,
+    five K=NG products and the O(No^4 Nv^2) hole ladder."""
+    return (2.0 * slice_pairs(Nv, s) * Nv ** 2 * (NG + No ** 2) + 14.0 * Nv ** 3 * No ** 3
+            + 10.0 * NG * Nv ** 2 * No ** 2 + 6.0 * Nv ** 2 * No ** 4)
+
+
+def triples_flops(No, Nv, NG):
+    return No ** 3 * (2.0 * Nv ** 4 + 2.0 * Nv ** 3 * No) + 2.0 * NG * Nv ** 3 * No
+
+
+def cpu_sample(g, No, Nv, NG, s, seconds_budget=30.0):
+    """FP64 CPU oracle on one full-size ladder slice pair (s x s virtual
+    block at the workload's Nv, NG, No): the dominant term of the iteration.
+    Returns (seconds, flops) of the sample."""
+    from oracle import ccsd as oc
+    rng = np.random.default_rng(0)
+    Gab = g[:, No:, No:]
+    X = 0.01 * rng.standard_normal((Nv, Nv, No, No))
+    R2 = np.zeros((Nv, Nv, No, No))
+    t = time.perf_counter()
+    Vxycd = oc.ein("Gxc,Gyd->xycd", Gab[:, s:2 * s, :], Gab[:, 0:s, :])
+    Rxyij = oc.ein("xycd,cdij->xyij", Vxycd, X)
+    oc.slice_into_residuum(Rxyij, s, 0, R2)
+    dt = time.perf_counter() - t
+    return dt, 2.0 * s * s * Nv * Nv * (NG + No * No)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--No", type=int, default=40)
+    ap.add_argument("--Nv", type=int, default=400)
+    ap.add_argument("--NG", type=int, default=1600)
+    ap.add_argument("--slice", type=int, default=64)
+    ap.add_argument("--seed", type=int, default=20261202)
+    ap.add_argument("--mixed-steps", type=int, default=2)
+    ap.add_argument("--triples-blocks", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one MI355X: this build has no multi-GPU path "
+              "(requested --gpus %d)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    from sisi4s_amd import engine as eng
+
+    No, Nv, NG, s = args.No, args.Nv, args.NG, args.slice
+    log = lambda m: print("[bench] " + m, file=sys.stderr, flush=True)  # noqa: E731
+    t0 = time.perf_counter()
+    g, ei, ea = eng.generate_system(args.seed, No, Nv, NG)
+    log("generated system No=%d Nv=%d NG=%d in %.1fs" % (No, Nv, NG, time.perf_counter() - t0))
+
+    e = eng.Engine(No, Nv, NG)
+    e.set_vertex(g)
+    e.set_eigenenergies(ei, ea)
+    t0 = time.perf_counter()
+    e.integrals_from_vertex()
+    t_int = time.perf_counter() - t0
+    log("integrals %.2fs" % t_int)
+
+    opts = eng.default_opts(mixer="DiisMixer", max_residua=6, max_iterations=10000,
+                            slice_size=s, amplitudes_convergence=1e-8, energy_convergence=1e-10)
+    e.ccsd_begin(opts)
+    for k in range(args.warmup):
+        t0 = time.perf_counter()
+        rep = e.ccsd_iterate(1)
+        log("warmup iteration %d: %.2fs E=%.10f" % (k + 1, time.perf_counter() - t0, rep["energy"]))
+    e.enable_stats(True)
+    e.reset_stats()
+    times = []
+    for k in range(args.steps):
+        t0 = time.perf_counter()
+        rep = e.ccsd_iterate(1)
+        times.append(time.perf_counter() - t0)
+        log("timed iteration %d: %.3fs E=%.10f dT/T=%.2e" % (k + 1, times[-1], rep["energy"],
+                                                             rep["delta"]))
+    st = e.stats()
+    e.enable_stats(False)
+    t_iter = float(np.sum(times)) / max(len(times), 1)
+    flops_it = ccsd_iteration_flops(No, Nv, NG, s)
+    g_sec, g_flops, g_n = st["gemm_seconds"][0], st["gemm_flops"][0], st["gemm_launches"][0]
+    achieved = g_flops / g_sec / 1e12 if g_sec > 0 else 0.0
+    other_bw = st["other_bytes"] / st["other_seconds"] / 1e9 if st["other_seconds"] > 0 else 0.0
+
+    # split-BF16x3 ladder + FP32 iterations of the mixed schedule (timed the same way)
+    mixed = {}
+    if args.mixed_steps > 0:
+        e.set_precision("bf16x3")
+        om = eng.default_opts(mixer="DiisMixer", max_residua=6, max_iterations=10000,
+                              slice_size=s, precision="bf16x3", switch_threshold=0.0)
+        e.ccsd_begin(om)
+        e.ccsd_iterate(2)
+        e.enable_stats(True)
+        e.reset_stats()
+        mt = []
+        for k in range(args.mixed_steps):
+            t0 = time.perf_counter()
+            e.ccsd_iterate(1)
+            mt.append(time.perf_counter() - t0)
+        sm = e.stats()
+        e.enable_stats(False)
+        mixed = dict(seconds_per_iteration=float(np.mean(mt)),
+                     gemm_tflops_fp32=(sm["gemm_flops"][1] / sm["gemm_seconds"][1] / 1e12
+                                       if sm["gemm_seconds"][1] > 0 else None),
+                     gemm_tflops_bf16x3=(sm["gemm_flops"][2] / sm["gemm_seconds"][2] / 1e12
+                                         if sm["gemm_seconds"][2] > 0 else None),
+                     speedup_per_iteration=t_iter / float(np.mean(mt)))
+        log("mixed iteration %.3fs" % mixed["seconds_per_iteration"])
+
+    # (T): sample of distinct-index blocks at the workload size, extrapolated
+    triples = {}
+    if args.triples_blocks > 0:
+        rng = np.random.default_rng(1)
+        blocks = []
+        while len(blocks) < args.triples_blocks:
+            b = sorted(rng.choice(No, 3, replace=False).tolist())
+            if b not in blocks:
+                blocks.append(b)
+        e.triples(precision="fp64", blocks=[blocks[0]])  # builds the (T) integrals
+        t0 = time.perf_counter()
+        e.triples(precision="fp64", blocks=blocks)
+        tb = (time.perf_counter() - t0) / len(blocks)
+        nblk = No * (No + 1) * (No + 2) // 6
+        triples = dict(seconds_per_block=tb, blocks=nblk,
+                       seconds_extrapolated=tb * nblk,
+                       tflops=triples_flops(No, Nv, NG) / (tb * nblk) / 1e12,
+                       sample="%d random i<j<k blocks, FP64" % len(blocks))
+        log("(T) %.3fs per block, %.0fs extrapolated" % (tb, tb * nblk))
+
+    cpu = None
+    if not args.no_cpu:
+        dt, fl = cpu_sample(g, No, Nv, NG, s)
+        rate = fl / dt
+        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        cpu = dict(value=flops_it / rate, unit="s/iteration (extrapolated)", cores=cores,
+                   kind="port",
+                   sample=("oracle ladder slice pair %dx%d at Nv=%d NG=%d No=%d: %.1f s, "
+                           "%.1f GFLOP/s; iteration = algorithmic flops / rate"
+                           % (s, s, Nv, NG, No, dt, rate / 1e9)))
+        log("cpu sample %.1fs" % dt)
+
+    out = {
+        "metric": METRIC,
+        "value": t_iter,
+        "unit": "s/iteration (CCSD, FP64)",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * t_iter,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Philox vertex, No=%d Nv=%d NG=%d)" % (No, Nv, NG),
+        "config": {"workload": "CCSD FP64 iteration, No=%d Nv=%d NG=%d, ladder slices of %d, "
+                               "DIIS(6), MP2 start" % (No, Nv, NG, s),
+                   "No": No, "Nv": Nv, "NG": NG, "slice": s, "mixer": "DiisMixer(6)"},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "kernel": "gemm_k4_kernel<FP64>", "launches": g_n,
+                     "avg_launch_ms": 1e3 * g_sec / max(g_n, 1)},
+        "cpu_baseline": cpu,
+        "extra": {
+            "iteration_tflops": flops_it / t_iter / 1e12,
+            "iteration_pct_fp64_peak": 100.0 * flops_it / t_iter / 1e12 / PEAK_FP64_TFLOPS,
+            "iteration_algorithmic_flops": flops_it,
+            "gemm_seconds_per_iteration": g_sec / args.steps,
+            "bandwidth_kernels_seconds_per_iteration": st["other_seconds"] / args.steps,
+            "bandwidth_kernels_GBps": other_bw,
+            "integrals_seconds": t_int,
+            "energy_after_timed_iterations": rep["energy"],
+            "mixed_bf16x3": mixed,
+            "triples": triples,
+        },
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

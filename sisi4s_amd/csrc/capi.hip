@@ -96,6 +96,8 @@ int sisi_engine_set_vertex(sisi_engine *h, const double *gamma) {
     e.upload(e.gamma, gamma, n);
     e.have_vertex = true;
     e.have_integrals = false;
+    e.Vt.reset();
+    e.VP.reset();
   });
 }
 

@@ -116,6 +116,8 @@ Engine::~Engine() {
   Vijka.reset();
   Vbaij.reset();
   Uabij.reset();
+  Vt.reset();
+  VP.reset();
   pool.release_all();
   if (stream) (void)hipStreamDestroy(stream);
 }

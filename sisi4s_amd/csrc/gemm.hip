@@ -7,7 +7,8 @@
 //
 // MFMA operand roles: the instruction's "A" operand is fed with B(k,n) and
 // its "B" operand with A(m,k), so the 16x16 result tile is C^T in MFMA
-// terms: lane l holds C(m = l%16, n = 4*(l/16) + r), r = 0..3.  Lanes then
+// terms: lane l holds C(m = l%16, n = 4*(l/16) + r), r = 0..3, for the f32
+// and bf16 forms and C(m = l%16, n = l/16 + 4*r) for the f64 form.  Lanes then
 // run along m, which is the unit-stride index of the column-major output,
 // and each register's 16-lane group stores 128 contiguous bytes.
 //
@@ -73,6 +74,10 @@ __global__ __launch_bounds__(256, 2) void gemm_k4_kernel(GemmArgs g) {
   using AT = typename std::conditional<PREC == SISI_FP64, d4, f4>::type;
   constexpr int BK = 16;
   constexpr int PAD = (PREC == SISI_FP64) ? 2 : 4;
+  // result layout of the 16x16 tile: row (here n) = LSTR*(lane/16) + RSTR*reg
+  // (v_mfma_f64_16x16x4f64: lane/16 + 4*reg; the f32 form: 4*(lane/16) + reg)
+  constexpr int LSTR = (PREC == SISI_FP64) ? 1 : 4;
+  constexpr int RSTR = (PREC == SISI_FP64) ? 4 : 1;
   __shared__ LT As[BK][BM + PAD];
   __shared__ LT Bs[BK][BN + PAD];
 
@@ -210,7 +215,7 @@ __global__ __launch_bounds__(256, 2) void gemm_k4_kernel(GemmArgs g) {
       for (int j = 0; j < 4; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int n = n0 + wn + j * 16 + 4 * lk + r;
+          const int n = n0 + wn + j * 16 + LSTR * lk + RSTR * r;
           if (n < g.N) {
             const int64_t o = om + out_off_n(g, n);
             double v = g.alpha * double(acc[i][j][r]);
@@ -229,7 +234,7 @@ __global__ __launch_bounds__(256, 2) void gemm_k4_kernel(GemmArgs g) {
       for (int j = 0; j < 4; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int n = n0 + wn + j * 16 + 4 * lk + r;
+          const int n = n0 + wn + j * 16 + LSTR * lk + RSTR * r;
           if (n < g.N) W[m + int64_t(n) * g.M] = double(acc[i][j][r]);
         }
     }

@@ -169,14 +169,17 @@ double triples_energy(Engine &e, const double *T1, const double *T2, int prec,
   for (auto &b : blocks)
     SISI_REQUIRE(b[0] >= 0 && b[0] <= b[1] && b[1] <= b[2] && b[2] < o, SISI_ERR_INVALID,
                  "triples: blocks must be sorted i0 <= i1 <= i2 < No");
-  // V_bdck = sum_F Gamma_Fbd Gamma_Fck stored as Vt[d,b,c,k] (cxx:88-91)
-  DBuf Vt(&e.pool, v * v * v * o);
-  ctr(e, 1.0, "Fbd,Fck->dbck", dense(e.Gab.get(), {G, v, v}), dense(e.Gai.get(), {G, v, o}), 0.0,
-      dense(Vt.get(), {v, v, v, o}), prec);
-  // V_jklc (HHHP) as [l, c, j, k] so each (j,k) slice is a [l, c] matrix
-  DBuf VP(&e.pool, o * v * o * o);
-  sum(e, 1.0, dense(e.Vijka.get(), {o, o, o, v}), "jklc", 0.0, dense(VP.get(), {o, v, o, o}),
-      "lcjk");
+  if (!e.Vt.get()) {
+    // V_bdck = sum_F Gamma_Fbd Gamma_Fck stored as Vt[d,b,c,k] (cxx:88-91), FP64
+    e.Vt = DBuf(&e.pool, v * v * v * o);
+    ctr(e, 1.0, "Fbd,Fck->dbck", dense(e.Gab.get(), {G, v, v}), dense(e.Gai.get(), {G, v, o}),
+        0.0, dense(e.Vt.get(), {v, v, v, o}), SISI_FP64);
+    // V_jklc (HHHP) as [l, c, j, k] so each (j,k) slice is a [l, c] matrix
+    e.VP = DBuf(&e.pool, o * v * o * o);
+    sum(e, 1.0, dense(e.Vijka.get(), {o, o, o, v}), "jklc", 0.0, dense(e.VP.get(), {o, v, o, o}),
+        "lcjk");
+  }
+  DBuf &Vt = e.Vt, &VP = e.VP;
   std::vector<double> eps_i(o);
   e.download(eps_i.data(), e.eps_i, size_t(o));
 

@@ -41,7 +41,7 @@ def test_triples_blocks_partition_the_sum():
     inp = triples.TriplesInputs(T1, T2, g, ei, ea)
     for b, p in zip(blocks[:6], parts):
         ref = triples.block_energy(inp, b)
-        assert abs(p - ref) <= 1e-11 * max(abs(ref), 1e-12)
+        assert abs(p - ref) <= 1e-12 * abs(total) + 1e-11 * abs(ref)
 
 
 def test_triples_reduced_precision():
