@@ -182,6 +182,13 @@ def main():
                            % (s, s, Nv, NG, No, dt, rate / 1e9)))
         log("cpu sample %.1fs" % dt)
 
+    traffic = None
+    tj = os.path.join(ROOT, "profiles", "ccsd_fp64_No40_Nv400_NG1600_traffic.json")
+    if (No, Nv, NG, s) == (40, 400, 1600, 64) and os.path.exists(tj):
+        # HBM bytes per launch of the FP64 GEMM family from the committed
+        # rocprofv3 FETCH_SIZE / WRITE_SIZE passes (calibrated; see profiles/)
+        traffic = json.load(open(tj))["hbm_bytes_per_launch"]
+
     out = {
         "metric": METRIC,
         "value": t_iter,
@@ -199,7 +206,7 @@ def main():
                                "DIIS(6), MP2 start" % (No, Nv, NG, s),
                    "No": No, "Nv": Nv, "NG": NG, "slice": s, "mixer": "DiisMixer(6)"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
                      "kernel": "gemm_k4_kernel<FP64>", "launches": g_n,
                      "avg_launch_ms": 1e3 * g_sec / max(g_n, 1)},
         "cpu_baseline": cpu,

@@ -104,6 +104,7 @@ Engine::~Engine() {
   lin1.reset();
   lin2.reset();
   perm_cache.clear();
+  split_cache.clear();
   gamma.reset();
   eps_i.reset();
   eps_a.reset();

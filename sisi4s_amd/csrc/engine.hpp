@@ -79,6 +79,7 @@ class Engine {
   // the contraction planner caches their permuted dense copies.
   std::vector<const double *> frozen;
   std::map<std::string, DBuf> perm_cache;
+  std::map<std::string, DBuf> split_cache;  // bf16 hi/lo planes of frozen operands
   bool is_frozen(const double *p) const {
     for (auto q : frozen)
       if (q == p) return true;
@@ -87,6 +88,7 @@ class Engine {
   void thaw() {
     frozen.clear();
     perm_cache.clear();
+    split_cache.clear();
   }
 
   // inputs

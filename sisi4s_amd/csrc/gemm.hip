@@ -18,6 +18,8 @@
 //   BF16X3 : v_mfma_f32_16x16x32_bf16 three times per tile,
 //            a_hi*b_hi + a_hi*b_lo + a_lo*b_hi with x = hi + lo split of the
 //            FP32-rounded operand, FP32 accumulation.
+#include <cstdlib>
+
 #include "engine.hpp"
 #include "gemm.hpp"
 
@@ -47,11 +49,19 @@ __device__ __forceinline__ int64_t out_off_n(const GemmArgs &g, int n) {
   return int64_t(n0) * g.cn0 + int64_t(n1) * g.cn1 + int64_t(n2) * g.cn2;
 }
 
-// Tile coordinates: groups of 8 m-tiles walk the n direction together so that
-// concurrently running blocks share A row panels and B column panels in L2.
-__device__ __forceinline__ void tile_coords(const GemmArgs &g, int pid, int &tm,
-                                            int &tn) {
-  const int group = 8;
+// Tile coordinates.  Workgroups are dealt to the 8 XCDs round-robin
+// (workgroup id mod 8), and each XCD has its own L2.  With xcd_map the id is
+// remapped so XCD x walks the contiguous tile range [x*T/8, (x+1)*T/8); inside
+// a range, groups of `group` m-tiles walk the n direction together, so the
+// workgroups resident on one XCD form a compact block of tiles that share A
+// row panels and B column panels in that XCD's L2.
+__device__ __forceinline__ void tile_coords(const GemmArgs &g, int pid, int &tm, int &tn) {
+  if (g.xcd_map) {
+    const int tiles = g.tiles_m * g.tiles_n;
+    const int per = tiles >> 3;
+    if (pid < (per << 3)) pid = (pid & 7) * per + (pid >> 3);
+  }
+  const int group = g.group;
   int per_group = group * g.tiles_n;
   int gid = pid / per_group;
   int first_m = gid * group;
@@ -475,8 +485,23 @@ static void launch_b3(GemmArgs &g, dim3 grid, hipStream_t s, bool AK, bool BNF) 
   else hipLaunchKernelGGL((gemm_bf16x3_kernel<true, true>), grid, dim3(256), 0, s, g);
 }
 
+void gemm_reduce(Engine &eng, const GemmArgs &g) {
+  Timed t(*eng.timers, Kind::Bandwidth, 8.0 * (g.ksplit + 2) * double(g.M) * g.N * g.batch);
+  const int64_t MN = int64_t(g.M) * g.N;
+  int blocks = int(std::min<int64_t>((MN + 255) / 256, 4096));
+  hipLaunchKernelGGL(gemm_reduce_kernel, dim3(blocks, g.batch), dim3(256), 0, eng.stream, g);
+  SISI_HIP(hipGetLastError());
+}
+
 void gemm(Engine &eng, int precision, GemmArgs g) {
   if (g.M <= 0 || g.N <= 0 || g.batch <= 0) return;
+  if (precision == SISI_BF16X3 && g.batch == 1 && g.K > 0) {
+    static const int env_b3 = getenv("SISI_B3_PRESPLIT") ? atoi(getenv("SISI_B3_PRESPLIT")) : 1;
+    if (env_b3) {
+      gemm_b3(eng, g);
+      return;
+    }
+  }
   if (g.K < 0) g.K = 0;  // K == 0 gives C = beta * C through the epilogue
   SISI_REQUIRE(g.sam == 1 || g.sak == 1 || g.K <= 1 || g.M <= 1, SISI_ERR_INVALID,
                "gemm: A needs a unit stride");
@@ -491,6 +516,12 @@ void gemm(Engine &eng, int precision, GemmArgs g) {
   const int64_t tiles = int64_t(g.tiles_m) * g.tiles_n;
   SISI_REQUIRE(tiles < (int64_t(1) << 31), SISI_ERR_INVALID, "gemm: too many tiles");
   g.split_out = !(g.dm0 >= g.M && g.dn0 >= g.N);
+  {
+    static const int env_map = getenv("SISI_GEMM_XCD") ? atoi(getenv("SISI_GEMM_XCD")) : 1;
+    static const int env_group = getenv("SISI_GEMM_GROUP") ? atoi(getenv("SISI_GEMM_GROUP")) : 8;
+    g.xcd_map = env_map;
+    g.group = env_group > 0 ? env_group : 8;
+  }
   const int kq = (precision == SISI_BF16X3) ? 32 : 16;
   // split-K when the tile grid alone cannot fill the 256 CUs
   int ksplit = 1;
@@ -523,13 +554,7 @@ void gemm(Engine &eng, int precision, GemmArgs g) {
     else launch_b3(g, grid, eng.stream, AKf, BNFf);
     SISI_HIP(hipGetLastError());
   }
-  if (ksplit > 1) {
-    Timed t(*eng.timers, Kind::Bandwidth, 8.0 * (ksplit + 2) * double(g.M) * g.N * g.batch);
-    const int64_t MN = int64_t(g.M) * g.N;
-    int blocks = int(std::min<int64_t>((MN + 255) / 256, 4096));
-    hipLaunchKernelGGL(gemm_reduce_kernel, dim3(blocks, g.batch), dim3(256), 0, eng.stream, g);
-    SISI_HIP(hipGetLastError());
-  }
+  if (ksplit > 1) gemm_reduce(eng, g);
 }
 
 }  // namespace sisi

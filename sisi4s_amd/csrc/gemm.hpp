@@ -30,6 +30,8 @@ struct GemmArgs {
   int batch = 1, ksplit = 1, kchunk = 0;
   int tiles_m = 0, tiles_n = 0;
   int split_out = 0;  // 1: output index splits need div/mod
+  int xcd_map = 1;    // 1: give each of the 8 XCDs a contiguous tile range
+  int group = 8;      // m-tiles per group in the tile walk
   double alpha = 1.0, beta = 0.0;
 };
 
@@ -37,6 +39,12 @@ class Engine;
 
 // Launch on the engine's stream.  precision: SISI_FP64/_FP32/_BF16X3.
 void gemm(Engine &eng, int precision, GemmArgs g);
+
+// Split-BF16x3 GEMM on operands split into bf16 planes (batch == 1).
+void gemm_b3(Engine &eng, GemmArgs g);
+
+// Split-K reduction of g.work into C (deterministic order).
+void gemm_reduce(Engine &eng, const GemmArgs &g);
 
 // Algorithmic flops of one call (2*M*N*K*batch).
 inline double gemm_flops(const GemmArgs &g) {

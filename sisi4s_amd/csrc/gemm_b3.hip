@@ -1,0 +1,320 @@
+// Split-BF16x3 GEMM on pre-split operands.
+//
+// Each FP64 operand is rounded to FP32 and split once into two bf16 planes,
+// x ~ hi + lo, stored k-contiguous ([rows][Kp], rows padded to the block
+// tile, Kp to 32, zero-filled).  The GEMM then streams the planes with
+// 16-byte loads and evaluates a_hi*b_lo + a_lo*b_hi + a_hi*b_hi per tile with
+// v_mfma_f32_16x16x32_bf16 (FP32 accumulation); the FP64 epilogue applies
+// alpha/beta and the split output addressing of the FP64 kernels.
+#include <cstdlib>
+
+#include "engine.hpp"
+#include "gemm.hpp"
+
+namespace sisi {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef unsigned short u16;
+struct alignas(16) U128 {
+  unsigned int x, y, z, w;
+};
+
+// ---------------------------------------------------------------------------
+// operand split: (R x K) strided FP64 view -> hi/lo [Rp][Kp] bf16
+// ---------------------------------------------------------------------------
+
+struct SplitArgs {
+  const double *src;
+  int64_t sr, sk;  // element (r,k) at src[r*sr + k*sk]
+  int R, K, Rp, Kp;
+  u16 *hi, *lo;
+};
+
+__device__ __forceinline__ u16 bf16_bits(float f) {
+  __bf16 b = __bf16(f);
+  return *reinterpret_cast<u16 *>(&b);
+}
+
+// 64 rows x 64 k per block: read along the unit-stride index into LDS, then
+// each thread converts 16 consecutive k of one row.
+__global__ __launch_bounds__(256) void split_bf16_kernel(SplitArgs a) {
+  __shared__ float t[64][65];
+  const int r0 = blockIdx.y * 64, k0 = blockIdx.x * 64;
+  const int tid = threadIdx.x;
+  if (a.sk == 1) {
+    for (int e = tid; e < 4096; e += 256) {
+      const int kk = e & 63, rr = e >> 6;
+      const int r = r0 + rr, k = k0 + kk;
+      t[rr][kk] = (r < a.R && k < a.K) ? float(a.src[int64_t(r) * a.sr + k]) : 0.f;
+    }
+  } else {
+    for (int e = tid; e < 4096; e += 256) {
+      const int rr = e & 63, kk = e >> 6;
+      const int r = r0 + rr, k = k0 + kk;
+      t[rr][kk] = (r < a.R && k < a.K) ? float(a.src[int64_t(r) * a.sr + int64_t(k) * a.sk]) : 0.f;
+    }
+  }
+  __syncthreads();
+  const int rr = tid >> 2, kq = (tid & 3) * 16;
+  const int r = r0 + rr, k = k0 + kq;
+  if (r >= a.Rp || k >= a.Kp) return;
+  u16 h[16], l[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const float x = t[rr][kq + j];
+    h[j] = bf16_bits(x);
+    __bf16 hb = *reinterpret_cast<__bf16 *>(&h[j]);
+    l[j] = bf16_bits(x - float(hb));
+  }
+  U128 *ph = reinterpret_cast<U128 *>(a.hi + int64_t(r) * a.Kp + k);
+  U128 *pl = reinterpret_cast<U128 *>(a.lo + int64_t(r) * a.Kp + k);
+  ph[0] = *reinterpret_cast<U128 *>(&h[0]);
+  ph[1] = *reinterpret_cast<U128 *>(&h[8]);
+  pl[0] = *reinterpret_cast<U128 *>(&l[0]);
+  pl[1] = *reinterpret_cast<U128 *>(&l[8]);
+}
+
+// ---------------------------------------------------------------------------
+// GEMM on split planes
+// ---------------------------------------------------------------------------
+
+struct B3Args {
+  const u16 *Ah, *Al, *Bh, *Bl;  // [Mp][Kp], [Np][Kp]
+  int64_t Kp;
+  GemmArgs g;  // sizes, output addressing, alpha/beta, split-K
+};
+
+__device__ __forceinline__ int64_t b3_off_m(const GemmArgs &g, int m) {
+  if (!g.split_out) return int64_t(m) * g.cm0;
+  int m0 = m % g.dm0, t = m / g.dm0, m1 = t % g.dm1, m2 = t / g.dm1;
+  return int64_t(m0) * g.cm0 + int64_t(m1) * g.cm1 + int64_t(m2) * g.cm2;
+}
+__device__ __forceinline__ int64_t b3_off_n(const GemmArgs &g, int n) {
+  if (!g.split_out) return int64_t(n) * g.cn0;
+  int n0 = n % g.dn0, t = n / g.dn0, n1 = t % g.dn1, n2 = t / g.dn1;
+  return int64_t(n0) * g.cn0 + int64_t(n1) * g.cn1 + int64_t(n2) * g.cn2;
+}
+
+// BM x BN block, 4 waves of WM x WN, k-chunk 32, planes staged through LDS
+// with 16-byte stores; the next chunk is prefetched into registers.
+template <int BM, int BN, int WM, int WN>
+__global__ __launch_bounds__(256, 1) void gemm_b3p_kernel(B3Args p) {
+  constexpr int BK = 32, RS = BK + 8;  // LDS row stride in bf16 (80 bytes)
+  constexpr int NA = BM * 4 / 256, NB = BN * 4 / 256;  // 16-byte chunks per thread per plane
+  static_assert((BM / WM) * (BN / WN) == 4, "four waves");
+  __shared__ __attribute__((aligned(16))) u16 Ah[BM * RS];
+  __shared__ __attribute__((aligned(16))) u16 Al[BM * RS];
+  __shared__ __attribute__((aligned(16))) u16 Bh[BN * RS];
+  __shared__ __attribute__((aligned(16))) u16 Bl[BN * RS];
+  const GemmArgs &g = p.g;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // tile walk: XCD-contiguous ranges, groups of 8 m-tiles
+  int pid = blockIdx.x;
+  {
+    const int tiles = g.tiles_m * g.tiles_n, per = tiles >> 3;
+    if (g.xcd_map && pid < (per << 3)) pid = (pid & 7) * per + (pid >> 3);
+  }
+  const int per_group = 8 * g.tiles_n, gid = pid / per_group, first_m = gid * 8;
+  const int gsize = min(g.tiles_m - first_m, 8), rr = pid % per_group;
+  const int tm = first_m + rr % gsize, tn = rr / gsize;
+  const int m0 = tm * BM, n0 = tn * BN;
+  const int ks = blockIdx.y, bz = blockIdx.z;
+  const int kbeg = ks * g.kchunk;
+  const int kend = min(int(p.Kp), kbeg + g.kchunk);
+  const u16 *Agh = p.Ah + int64_t(m0) * p.Kp, *Agl = p.Al + int64_t(m0) * p.Kp;
+  const u16 *Bgh = p.Bh + int64_t(n0) * p.Kp, *Bgl = p.Bl + int64_t(n0) * p.Kp;
+
+  U128 ra[2 * NA], rb[2 * NB];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int i = 0; i < NA; ++i) {
+      const int c = tid + 256 * i, row = c >> 2, kq = (c & 3) * 8;
+      ra[i] = *reinterpret_cast<const U128 *>(Agh + int64_t(row) * p.Kp + k0 + kq);
+      ra[NA + i] = *reinterpret_cast<const U128 *>(Agl + int64_t(row) * p.Kp + k0 + kq);
+    }
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      const int c = tid + 256 * i, row = c >> 2, kq = (c & 3) * 8;
+      rb[i] = *reinterpret_cast<const U128 *>(Bgh + int64_t(row) * p.Kp + k0 + kq);
+      rb[NB + i] = *reinterpret_cast<const U128 *>(Bgl + int64_t(row) * p.Kp + k0 + kq);
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int i = 0; i < NA; ++i) {
+      const int c = tid + 256 * i, row = c >> 2, kq = (c & 3) * 8;
+      *reinterpret_cast<U128 *>(&Ah[row * RS + kq]) = ra[i];
+      *reinterpret_cast<U128 *>(&Al[row * RS + kq]) = ra[NA + i];
+    }
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      const int c = tid + 256 * i, row = c >> 2, kq = (c & 3) * 8;
+      *reinterpret_cast<U128 *>(&Bh[row * RS + kq]) = rb[i];
+      *reinterpret_cast<U128 *>(&Bl[row * RS + kq]) = rb[NB + i];
+    }
+  };
+
+  constexpr int TI = WM / 16, TJ = WN / 16;
+  f4 acc[TI][TJ];
+#pragma unroll
+  for (int i = 0; i < TI; ++i)
+#pragma unroll
+    for (int j = 0; j < TJ; ++j) acc[i][j] = f4{0, 0, 0, 0};
+  constexpr int WX = BN / WN;
+  const int wm = (wave / WX) * WM, wn = (wave % WX) * WN;
+  const int l16 = lane & 15, lk = lane >> 4;
+
+  int k0 = kbeg;
+  if (k0 < kend) load(k0);
+  while (k0 < kend) {
+    store();
+    __syncthreads();
+    const int k1 = k0 + BK;
+    if (k1 < kend) load(k1);
+    bf8 bh[TJ], bl[TJ];
+#pragma unroll
+    for (int j = 0; j < TJ; ++j) {
+      const int col = wn + j * 16 + l16;
+      bh[j] = *reinterpret_cast<const bf8 *>(&Bh[col * RS + 8 * lk]);
+      bl[j] = *reinterpret_cast<const bf8 *>(&Bl[col * RS + 8 * lk]);
+    }
+#pragma unroll
+    for (int i = 0; i < TI; ++i) {
+      const int row = wm + i * 16 + l16;
+      const bf8 ah = *reinterpret_cast<const bf8 *>(&Ah[row * RS + 8 * lk]);
+      const bf8 al = *reinterpret_cast<const bf8 *>(&Al[row * RS + 8 * lk]);
+#pragma unroll
+      for (int j = 0; j < TJ; ++j) {
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[j], ah, acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], al, acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], ah, acc[i][j], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+    k0 = k1;
+  }
+
+  if (g.ksplit == 1) {
+    double *C = g.C + int64_t(bz) * g.bsc;
+#pragma unroll
+    for (int i = 0; i < TI; ++i) {
+      const int m = m0 + wm + i * 16 + l16;
+      if (m >= g.M) continue;
+      const int64_t om = b3_off_m(g, m);
+#pragma unroll
+      for (int j = 0; j < TJ; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int n = n0 + wn + j * 16 + 4 * lk + r;
+          if (n < g.N) {
+            const int64_t o = om + b3_off_n(g, n);
+            double v = g.alpha * double(acc[i][j][r]);
+            if (g.beta != 0.0) v += g.beta * C[o];
+            C[o] = v;
+          }
+        }
+    }
+  } else {
+    double *W = g.work + (int64_t(bz) * g.ksplit + ks) * int64_t(g.M) * g.N;
+#pragma unroll
+    for (int i = 0; i < TI; ++i) {
+      const int m = m0 + wm + i * 16 + l16;
+      if (m >= g.M) continue;
+#pragma unroll
+      for (int j = 0; j < TJ; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int n = n0 + wn + j * 16 + 4 * lk + r;
+          if (n < g.N) W[m + int64_t(n) * g.M] = double(acc[i][j][r]);
+        }
+    }
+  }
+}
+
+
+namespace {
+
+struct Planes {
+  DBuf store;  // hi and lo planes in one allocation (doubles as raw storage)
+  const u16 *hi = nullptr, *lo = nullptr;
+};
+
+// Split one (rows x K) operand view; frozen sources are split once per
+// residual evaluation and reused.
+const Planes &split_operand(Engine &e, const double *src, int64_t sr, int64_t sk, int R, int K,
+                            int Rp, int Kp, Planes &local) {
+  std::string key;
+  const bool cached = e.is_frozen(src);
+  if (cached) {
+    key = "b3:" + std::to_string(reinterpret_cast<uintptr_t>(src)) + ":" + std::to_string(sr) +
+          ":" + std::to_string(sk) + ":" + std::to_string(R) + ":" + std::to_string(K) + ":" +
+          std::to_string(Rp) + ":" + std::to_string(Kp);
+    auto it = e.split_cache.find(key);
+    if (it != e.split_cache.end()) {
+      local.hi = reinterpret_cast<const u16 *>(it->second.get());
+      local.lo = local.hi + int64_t(Rp) * Kp;
+      return local;
+    }
+  }
+  const int64_t elems = int64_t(Rp) * Kp;        // bf16 per plane
+  const size_t doubles = size_t((2 * elems * 2 + 7) / 8);
+  local.store = DBuf(&e.pool, doubles);
+  u16 *hi = reinterpret_cast<u16 *>(local.store.get());
+  u16 *lo = hi + elems;
+  SplitArgs a{src, sr, sk, R, K, Rp, Kp, hi, lo};
+  {
+    Timed t(*e.timers, Kind::Bandwidth, 8.0 * double(R) * K + 4.0 * double(elems));
+    dim3 grid(unsigned((Kp + 63) / 64), unsigned((Rp + 63) / 64));
+    hipLaunchKernelGGL(split_bf16_kernel, grid, dim3(256), 0, e.stream, a);
+    SISI_HIP(hipGetLastError());
+  }
+  local.hi = hi;
+  local.lo = lo;
+  if (cached) {
+    e.split_cache[key] = std::move(local.store);
+    local.hi = reinterpret_cast<const u16 *>(e.split_cache[key].get());
+    local.lo = local.hi + elems;
+  }
+  return local;
+}
+
+}  // namespace
+
+void gemm_b3(Engine &e, GemmArgs g) {
+  constexpr int BM = 256, BN = 128, WM = 128, WN = 64;
+  const int Kp = ((g.K + 31) / 32) * 32;
+  const int Mp = ((g.M + BM - 1) / BM) * BM, Np = ((g.N + BN - 1) / BN) * BN;
+  Planes pa, pb;
+  const Planes &A = split_operand(e, g.A + 0, g.sam, g.sak, g.M, g.K, Mp, Kp, pa);
+  const Planes &B = split_operand(e, g.B + 0, g.sbn, g.sbk, g.N, g.K, Np, Kp, pb);
+  g.tiles_m = Mp / BM;
+  g.tiles_n = Np / BN;
+  const int64_t tiles = int64_t(g.tiles_m) * g.tiles_n;
+  g.split_out = !(g.dm0 >= g.M && g.dn0 >= g.N);
+  int ksplit = 1;
+  if (Kp > 4 * 32 && tiles < 512) {
+    const int64_t want = (512 + tiles - 1) / tiles;
+    ksplit = int(std::min<int64_t>(std::min<int64_t>(want, Kp / 256), 64));
+    if (ksplit < 1) ksplit = 1;
+  }
+  int kchunk = ((Kp + ksplit - 1) / ksplit + 31) / 32 * 32;
+  ksplit = (Kp + kchunk - 1) / kchunk;
+  g.ksplit = ksplit;
+  g.kchunk = kchunk;
+  DBuf work;
+  if (ksplit > 1) {
+    work = DBuf(&e.pool, size_t(ksplit) * g.M * g.N);
+    g.work = work.get();
+  }
+  B3Args p{A.hi, A.lo, B.hi, B.lo, Kp, g};
+  dim3 grid(unsigned(tiles), unsigned(ksplit), 1);
+  {
+    Timed t(*e.timers, Kind::GemmB3, gemm_flops(g));
+    hipLaunchKernelGGL((gemm_b3p_kernel<BM, BN, WM, WN>), grid, dim3(256), 0, e.stream, p);
+    SISI_HIP(hipGetLastError());
+  }
+  if (ksplit > 1) gemm_reduce(e, g);
+}
+
+}  // namespace sisi
