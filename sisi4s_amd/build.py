@@ -15,6 +15,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libsisi_engine.so")
+RUNNER = os.path.join(HERE, "sisi4s-mi355x")
+HOST = os.path.join(HERE, "host")
+CXX = os.environ.get("CXX", "g++")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
@@ -59,7 +62,22 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
     os.replace(tmp, LIB)
+    build_runner()
     return LIB
+
+
+def build_runner():
+    """Host step runner (C++ mirror of the plugin surface) linked to the engine."""
+    srcs = sorted(os.path.join(HOST, f) for f in os.listdir(HOST) if f.endswith(".cpp"))
+    tmp = RUNNER + ".tmp"
+    cmd = [CXX, "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(HERE, "..", "include")] + srcs + \
+        ["-L" + HERE, "-lsisi_engine", "-Wl,-rpath,$ORIGIN", "-L/opt/rocm/lib",
+         "-Wl,-rpath-link,/opt/rocm/lib", "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("runner build failed:\n" + r.stdout + r.stderr)
+    os.replace(tmp, RUNNER)
+    return RUNNER
 
 
 if __name__ == "__main__":

@@ -83,6 +83,7 @@ void ccsd_residual(Engine &e, int it, bool initial_doubles, int slice_size, cons
   }
   zero(e, R2p, n2);
   const int prec = e.precision;
+  e.thaw();
   e.frozen = {T1p, T2p, e.Vabij.get(), e.Uabij.get(), e.Vaibj.get(), e.Vijka.get(),
               e.Gij.get(), e.Gai.get(), e.Gab.get()};
 
@@ -164,6 +165,7 @@ void ccsd_residual(Engine &e, int it, bool initial_doubles, int slice_size, cons
     e.frozen.push_back(Xb3.get());
     ctr(e, -1.0, "akci,cbkj->abij", Xakci, T2, 1.0, R2, prec);   // cxx:288
     ctr(e, -1.0, "bkci,ackj->abij", Xakci, T2, 1.0, R2, prec);   // cxx:289
+    e.unfreeze(Xb3.get());  // Xb3 returns to the pool at the end of this block
   }
   {
     // R_abij += R_baji (cxx:292)

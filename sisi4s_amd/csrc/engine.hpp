@@ -90,6 +90,22 @@ class Engine {
     perm_cache.clear();
     split_cache.clear();
   }
+  // Forget one frozen operand and every cached copy of it (before its buffer
+  // goes back to the pool, whose next user must not inherit the caches).
+  void unfreeze(const double *p) {
+    for (size_t k = 0; k < frozen.size();)
+      if (frozen[k] == p) frozen.erase(frozen.begin() + k);
+      else ++k;
+    const std::string tag = std::to_string(reinterpret_cast<uintptr_t>(p));
+    for (auto *cache : {&perm_cache, &split_cache})
+      for (auto it = cache->begin(); it != cache->end();) {
+        const std::string &k = it->first;  // "<ptr>:..." or "b3:<ptr>:..."
+        const bool hit = k.compare(0, tag.size() + 1, tag + ":") == 0 ||
+                         k.compare(0, tag.size() + 4, "b3:" + tag + ":") == 0;
+        if (hit) it = cache->erase(it);
+        else ++it;
+      }
+  }
 
   // inputs
   DBuf gamma, eps_i, eps_a;

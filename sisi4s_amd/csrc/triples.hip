@@ -160,6 +160,7 @@ void add_w(Engine &e, int prec, const double *T2, const double *Vt, const double
 double triples_energy(Engine &e, const double *T1, const double *T2, int prec,
                       const std::vector<std::array<int, 3>> &blocks_in) {
   SISI_REQUIRE(e.have_integrals, SISI_ERR_STATE, "triples: integrals not built");
+  e.thaw();
   const int64_t o = e.No, v = e.Nv, G = e.NG;
   std::vector<std::array<int, 3>> blocks = blocks_in;
   if (blocks.empty())

@@ -1,0 +1,79 @@
+"""The C++ step runner (host mirror of the plugin surface): input parsing on
+the CPU; end-to-end config-1 CCSD + (T) on the GPU against the oracle."""
+
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, announce
+
+RUNNER = os.path.join(ROOT, "sisi4s_amd", "sisi4s-mi355x")
+INPUT = os.path.join(ROOT, "tests", "inputs", "config1_ccsd_t.yaml")
+
+
+def run(args, **kw):
+    return subprocess.run([RUNNER] + args, capture_output=True, text=True, timeout=600, **kw)
+
+
+def test_runner_lists_plugins():
+    r = run(["--list"])
+    assert r.returncode == 0
+    names = r.stdout.split()
+    for n in ("CoulombIntegralsFromVertex", "CcsdEnergyFromCoulombIntegrals",
+              "CcsdPerturbativeTriples", "GenerateSyntheticSystem"):
+        assert n in names
+
+
+def test_runner_parses_reference_style_input():
+    r = run(["-i", INPUT, "--dry"])
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.strip().splitlines()
+    assert [l.split()[0] for l in lines] == ["Nop", "GenerateSyntheticSystem",
+                                             "CoulombIntegralsFromVertex",
+                                             "CcsdEnergyFromCoulombIntegrals",
+                                             "CcsdPerturbativeTriples",
+                                             "CcsdEnergyFromCoulombIntegrals"]
+    ccsd = lines[3]
+    # aliases resolved, symbols kept
+    assert 'mixer: "DiisMixer"' in ccsd and 'maxResidua: "6"' in ccsd
+    assert "CoulombVertex: $CoulombVertex" in ccsd and "CcsdEnergy: $CcsdEnergy" in ccsd
+    assert "(fallible)" in lines[5]
+
+
+def test_runner_rejects_unknown_algorithm(tmp_path):
+    p = tmp_path / "bad.yaml"
+    p.write_text("- name: NoSuchAlgorithm\n  in:\n    x: 1\n")
+    r = run(["-i", str(p), "--dry"])
+    assert r.returncode == 1 and "Algorithm not implemented" in r.stderr
+
+
+def test_runner_rejects_malformed_yaml(tmp_path):
+    p = tmp_path / "bad.yaml"
+    p.write_text("- name: Nop\n  in:\n    a: *undefined\n")
+    r = run(["-i", str(p), "--dry"])
+    assert r.returncode == 1 and "unknown alias" in r.stderr
+
+
+@pytest.mark.gpu
+def test_runner_config1_ccsd_and_triples(tmp_path):
+    from oracle import ccsd, philox, triples
+    out = tmp_path / "out.yaml"
+    announce("runner config1 ccsd + (T)")
+    r = run(["-i", INPUT, "-o", str(out)])
+    assert r.returncode == 0, r.stdout + r.stderr
+    vals = {}
+    for line in r.stdout.splitlines():
+        if "=" in line:
+            k, v = line.split("=")
+            vals[k.strip()] = float(v)
+    assert "failed" in r.stderr and "NoSuchMixer" in r.stderr  # fallible step
+    g, ei, ea = philox.system(20261202, 10, 50, 300)
+    ref = ccsd.solve(g, ei, ea, mixer="DiisMixer", max_residua=6, max_iterations=30,
+                     amplitudes_convergence=1e-12, energy_convergence=1e-14)
+    assert abs(vals["CcsdEnergy"] - ref["energy"]) <= 1e-8
+    et = triples.triples_energy(ref["T1"], ref["T2"], g, ei, ea)
+    assert abs(vals["CcsdPerturbativeTriplesEnergy"] - (ref["energy"] + et)) <= 1e-8
+    text = out.read_text()
+    assert text.count("iteration:") == 30 and "realtime:" in text
