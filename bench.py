@@ -78,6 +78,14 @@ def main():
     ap.add_argument("--mixed-steps", type=int, default=2)
     ap.add_argument("--triples-blocks", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mode", choices=["iteration", "converge", "batched"], default="iteration",
+                    help="iteration: timed FP64 iterations (default line); converge: FP64 and "
+                         "mixed runs to |dT|/|T| <= 1e-8 with (T); batched: 64 small systems")
+    ap.add_argument("--systems", type=int, default=64)
+    ap.add_argument("--workers", type=int, default=8)
+    ap.add_argument("--level-shift", type=float, default=0.0)
+    ap.add_argument("--max-iterations", type=int, default=400)
+    ap.add_argument("--precisions", default="fp64,mixed")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -86,6 +94,11 @@ def main():
         sys.exit(2)
 
     from sisi4s_amd import engine as eng
+
+    if args.mode == "converge":
+        return converge(args)
+    if args.mode == "batched":
+        return batched(args)
 
     No, Nv, NG, s = args.No, args.Nv, args.NG, args.slice
     log = lambda m: print("[bench] " + m, file=sys.stderr, flush=True)  # noqa: E731
@@ -223,6 +236,115 @@ def main():
             "triples": triples,
         },
     }
+    print(json.dumps(out), flush=True)
+
+
+def converge(args):
+    """FP64 and mixed CCSD to |dT|/|T| <= 1e-8 on the same system, then (T)
+    on each run's amplitudes: time to the converged energy and |dE|."""
+    from sisi4s_amd import engine as eng
+    No, Nv, NG, s = args.No, args.Nv, args.NG, args.slice
+    log = lambda m: print("[converge] " + m, file=sys.stderr, flush=True)  # noqa: E731
+    g, ei, ea = eng.generate_system(args.seed, No, Nv, NG)
+    e = eng.Engine(No, Nv, NG)
+    e.set_system(g, ei, ea)
+    runs = {}
+    todo = [r for r in (("fp64", "fp64", "fp64"), ("mixed", "bf16x3", "fp32"))
+            if r[0] in args.precisions.split(",")]
+    for name, prec, tprec in todo:
+        o = eng.default_opts(mixer="DiisMixer", max_residua=6, max_iterations=args.max_iterations,
+                             slice_size=s, amplitudes_convergence=1e-8, energy_convergence=1e-10,
+                             precision=prec, switch_threshold=1e-5, level_shift=args.level_shift)
+        t0 = time.perf_counter()
+        e.ccsd_begin(o)
+        rep = None
+        while True:
+            try:
+                rep = e.ccsd_iterate(5)
+            except eng.EngineError as ex:
+                log("%s failed: %s" % (name, ex))
+                rep = dict(rep or {}, converged=0, failed=str(ex))
+                break
+            log("%s it=%d E=%.10f dT/T=%.2e prec=%d (%.0fs)" % (
+                name, rep["iterations"], rep["energy"], rep["delta"], rep["precision"],
+                time.perf_counter() - t0))
+            if rep["converged"] or rep["iterations"] >= o.max_iterations:
+                break
+        t_ccsd = time.perf_counter() - t0
+        if rep.get("failed"):
+            runs[name] = dict(failed=rep["failed"], ccsd_seconds=t_ccsd)
+            continue
+        blocks = None
+        if args.triples_blocks > 0:
+            rng = np.random.default_rng(1)
+            blocks = []
+            while len(blocks) < args.triples_blocks:
+                b = sorted(rng.choice(No, 3, replace=False).tolist())
+                if b not in blocks:
+                    blocks.append(b)
+        t0 = time.perf_counter()
+        et = e.triples(precision=tprec, blocks=blocks)
+        t_t = time.perf_counter() - t0
+        nblk = No * (No + 1) * (No + 2) // 6
+        runs[name] = dict(ccsd_seconds=t_ccsd, iterations=rep["iterations"],
+                          fp64_iterations=rep["fp64_iterations"], energy=rep["energy"],
+                          converged=bool(rep["converged"]), triples_precision=tprec,
+                          triples_seconds=(t_t if blocks is None else t_t * nblk / len(blocks)),
+                          triples_sampled=blocks is not None, triples_energy_sample=et)
+        log("%s done: %s" % (name, runs[name]))
+    if len(runs) < 2:
+        print(json.dumps({"mode": "converge", "level_shift": args.level_shift, "runs": runs}))
+        return
+    f, m = runs["fp64"], runs["mixed"]
+    out = {"metric": METRIC, "mode": "converge", "level_shift": args.level_shift,
+           "config": {"workload": "CCSD to |dT|/|T|<=1e-8 + (T), No=%d Nv=%d NG=%d slice=%d DIIS(6)"
+                                  % (No, Nv, NG, s)},
+           "fp64": f, "mixed": m,
+           "ccsd_dE": abs(m["energy"] - f["energy"]),
+           "triples_dE_sample": abs(m["triples_energy_sample"] - f["triples_energy_sample"]),
+           "speedup_ccsd": f["ccsd_seconds"] / m["ccsd_seconds"],
+           "speedup_total": (f["ccsd_seconds"] + f["triples_seconds"])
+           / (m["ccsd_seconds"] + m["triples_seconds"])}
+    print(json.dumps(out), flush=True)
+
+
+def batched(args):
+    """Config 5: independent seeded systems, No in [4,16], Nv in [30,120],
+    NG = 4 Nv, FP64 and mixed runs to 1e-8; per-system |dE|."""
+    from sisi4s_amd import engine as eng
+    rng = np.random.default_rng(args.seed)
+    specs = []
+    for k in range(args.systems):
+        No = int(rng.integers(4, 17))
+        Nv = int(rng.integers(30, 121))
+        specs.append((args.seed + k, No, Nv, 4 * Nv))
+    systems = [eng.generate_system(*sp) for sp in specs]
+    kw = dict(mixer="DiisMixer", max_residua=6, max_iterations=300,
+              amplitudes_convergence=1e-8, energy_convergence=1e-10)
+    res = {}
+    for name, prec in (("fp64", "fp64"), ("mixed", "bf16x3")):
+        t0 = time.perf_counter()
+        E, reps = eng.ccsd_solve_batched(systems, eng.default_opts(precision=prec, **kw),
+                                         workers=args.workers, strict=False)
+        res[name] = dict(seconds=time.perf_counter() - t0, energies=E.tolist(),
+                         failed=[k for k, r in enumerate(reps) if r["status"] != 0],
+                         converged=int(sum(r["converged"] for r in reps)),
+                         iterations=int(sum(r["iterations"] for r in reps)),
+                         fp64_iterations=int(sum(r["fp64_iterations"] for r in reps)))
+        print("[batched] %s %.1fs" % (name, res[name]["seconds"]), file=sys.stderr, flush=True)
+    ok = [k for k in range(len(specs)) if k not in res["fp64"]["failed"]]
+    dE = np.abs(np.array(res["fp64"]["energies"])[ok] - np.array(res["mixed"]["energies"])[ok])
+    out = {"metric": METRIC, "mode": "batched",
+           "config": {"workload": "%d seeded systems, No U[4,16], Nv U[30,120], NG=4Nv, DIIS(6), "
+                                  "to 1e-8" % args.systems, "workers": args.workers},
+           "fp64_seconds": res["fp64"]["seconds"], "mixed_seconds": res["mixed"]["seconds"],
+           "speedup": res["fp64"]["seconds"] / res["mixed"]["seconds"],
+           "max_dE": float(np.nanmax(dE)) if len(dE) else None,
+           "systems_compared": len(ok),
+           "systems_within_1uEh": int((dE <= 1e-6).sum()),
+           "fp64_diverged": res["fp64"]["failed"],
+           "fp64": {k: v for k, v in res["fp64"].items() if k != "energies"},
+           "mixed": {k: v for k, v in res["mixed"].items() if k != "energies"}}
     print(json.dumps(out), flush=True)
 
 

@@ -87,6 +87,7 @@ typedef struct {
   int converged;      /* reference convergence test met                  */
   int fp64_iterations;/* iterations run with FP64 contractions           */
   int precision;      /* precision of the last iteration                 */
+  int status;         /* SISI_OK, or the error of this system in a batch */
 } sisi_ccsd_report;
 
 typedef struct {
@@ -179,6 +180,19 @@ int sisi_triples_energy(sisi_engine *e, const double *T1, const double *T2,
 int sisi_triples_energy_blocks(sisi_engine *e, const double *T1,
                                const double *T2, int precision,
                                const int *blocks, int nblocks, double *energy);
+
+/* Many independent CCSD runs (config-5 style batches of small systems of
+ * uneven No/Nv/NG) on one GPU: systems are taken from a shared queue by
+ * `workers` host threads, each driving its own engine and HIP stream, so the
+ * small kernels of different systems overlap on the device.  Each system is
+ * ClusterSinglesDoublesAlgorithm::run (cxx:38-128) with the same opts.
+ * dims/gammas/eps_i/eps_a hold nsys entries; energies[s] and reports[s]
+ * (may be NULL) receive the results (a failed system gets NaN and its status
+ * in reports[s].status).  Returns the first failing status. */
+int sisi_ccsd_solve_batched(int nsys, const sisi_dims *dims, const double *const *gammas,
+                            const double *const *eps_i, const double *const *eps_a,
+                            const sisi_ccsd_opts *opts, int workers, double *energies,
+                            sisi_ccsd_report *reports);
 
 /* Device timing of kernel families since the last reset (collected only
  * while enabled; enabling adds one event pair per launch). */

@@ -1,7 +1,9 @@
 // extern "C" boundary: argument checks, exception -> status conversion,
 // host <-> device staging.  See include/sisi_engine.h for the contract.
+#include <atomic>
 #include <cmath>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "ccsd.hpp"
@@ -301,6 +303,65 @@ int sisi_triples_energy_blocks(sisi_engine *h, const double *T1, const double *T
     return SISI_ERR_INVALID;
   }
   return triples_common(h, T1, T2, precision, blocks, nblocks, energy);
+}
+
+int sisi_ccsd_solve_batched(int nsys, const sisi_dims *dims, const double *const *gammas,
+                            const double *const *eps_i, const double *const *eps_a,
+                            const sisi_ccsd_opts *opts, int workers, double *energies,
+                            sisi_ccsd_report *reports) {
+  return guarded([&] {
+    SISI_REQUIRE(nsys >= 0 && dims && gammas && eps_i && eps_a && energies, SISI_ERR_INVALID,
+                 "null argument");
+    sisi_ccsd_opts o;
+    if (opts) o = *opts;
+    else sisi_ccsd_opts_default(&o);
+    if (workers < 1) workers = 1;
+    workers = std::min(workers, std::max(nsys, 1));
+    int device = 0;
+    SISI_HIP(hipGetDevice(&device));
+    std::atomic<int> next(0);
+    std::vector<int> status(size_t(nsys), SISI_OK);
+    std::vector<std::string> msg(static_cast<size_t>(nsys));
+    auto worker = [&]() {
+      (void)hipSetDevice(device);
+      for (int s = next++; s < nsys; s = next++) {
+        const int rc = guarded([&] {
+          Engine e(dims[s]);
+          const size_t np = size_t(dims[s].No) + dims[s].Nv;
+          SISI_REQUIRE(gammas[s] && eps_i[s] && eps_a[s], SISI_ERR_INVALID, "null system input");
+          check_finite(gammas[s], size_t(dims[s].NG) * np * np, "CoulombVertex");
+          check_finite(eps_i[s], size_t(dims[s].No), "HoleEigenEnergies");
+          check_finite(eps_a[s], size_t(dims[s].Nv), "ParticleEigenEnergies");
+          e.upload(e.gamma, gammas[s], size_t(dims[s].NG) * np * np);
+          e.have_vertex = true;
+          e.upload(e.eps_i, eps_i[s], size_t(dims[s].No));
+          e.upload(e.eps_a, eps_a[s], size_t(dims[s].Nv));
+          e.have_eps = true;
+          integrals_from_vertex(e);
+          ccsd_begin(e, o, nullptr, nullptr);
+          ccsd_iterate(e, o.max_iterations);
+          energies[s] = e.last_energy;
+          if (reports) fill_report(e, &reports[s]);
+        });
+        status[s] = rc;
+        if (rc != SISI_OK) {
+          msg[s] = g_last_error;
+          energies[s] = std::nan("");
+          if (reports) {
+            reports[s] = sisi_ccsd_report{};
+            reports[s].energy = std::nan("");
+          }
+        }
+        if (reports) reports[s].status = rc;
+      }
+    };
+    std::vector<std::thread> pool;
+    for (int w = 0; w < workers; ++w) pool.emplace_back(worker);
+    for (auto &t : pool) t.join();
+    for (int s = 0; s < nsys; ++s)
+      if (status[s] != SISI_OK)
+        throw Error(status[s], "system " + std::to_string(s) + ": " + msg[s]);
+  });
 }
 
 int sisi_engine_enable_stats(sisi_engine *h, int enable) {

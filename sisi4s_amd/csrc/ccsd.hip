@@ -263,8 +263,9 @@ std::vector<double> solve_small(std::vector<std::vector<double>> A, std::vector<
     int p = c;
     for (int r = c + 1; r < n; ++r)
       if (std::fabs(A[r][c]) > std::fabs(A[p][c])) p = r;
-    SISI_REQUIRE(A[p][c] != 0.0 && std::isfinite(A[p][c]), SISI_ERR_NUMERIC,
-                 "DIIS: singular overlap matrix");
+    SISI_REQUIRE(std::isfinite(A[p][c]), SISI_ERR_NUMERIC,
+                 "DIIS: non-finite overlap (the iteration diverged)");
+    SISI_REQUIRE(A[p][c] != 0.0, SISI_ERR_NUMERIC, "DIIS: singular overlap matrix");
     std::swap(A[p], A[c]);
     std::swap(b[p], b[c]);
     for (int r = c + 1; r < n; ++r) {
@@ -400,6 +401,9 @@ void ccsd_iterate(Engine &e, int n) {
     }
     double E, dir, exc;
     ccsd_energy(e, e.T1.get(), e.T2.get(), E, dir, exc);
+    SISI_REQUIRE(std::isfinite(E) && std::isfinite(e.last_delta), SISI_ERR_NUMERIC,
+                 "CCSD iteration " + std::to_string(e.iteration + 1) +
+                     " diverged (non-finite energy or amplitudes)");
     e.last_energy = E;
     e.last_direct = dir;
     e.last_exchange = exc;
@@ -427,6 +431,7 @@ void fill_report(const Engine &e, sisi_ccsd_report *r) {
   r->converged = e.converged ? 1 : 0;
   r->fp64_iterations = e.fp64_iterations;
   r->precision = e.cur_precision;
+  r->status = SISI_OK;
 }
 
 }  // namespace sisi

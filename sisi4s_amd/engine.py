@@ -49,7 +49,8 @@ class CcsdReport(ctypes.Structure):
     _fields_ = [("energy", ctypes.c_double), ("direct", ctypes.c_double),
                 ("exchange", ctypes.c_double), ("delta", ctypes.c_double),
                 ("iterations", ctypes.c_int), ("converged", ctypes.c_int),
-                ("fp64_iterations", ctypes.c_int), ("precision", ctypes.c_int)]
+                ("fp64_iterations", ctypes.c_int), ("precision", ctypes.c_int),
+                ("status", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -101,6 +102,10 @@ SIGNATURES = [
     ("sisi_triples_energy_blocks", ctypes.c_int,
      [ctypes.c_void_p, _c_double_p, _c_double_p, ctypes.c_int, _c_int_p, ctypes.c_int,
       _c_double_p]),
+    ("sisi_ccsd_solve_batched", ctypes.c_int,
+     [ctypes.c_int, ctypes.POINTER(Dims), ctypes.POINTER(_c_double_p), ctypes.POINTER(_c_double_p),
+      ctypes.POINTER(_c_double_p), ctypes.POINTER(CcsdOpts), ctypes.c_int, _c_double_p,
+      ctypes.POINTER(CcsdReport)]),
     ("sisi_engine_enable_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("sisi_engine_get_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Stats)]),
     ("sisi_engine_reset_stats", ctypes.c_int, [ctypes.c_void_p]),
@@ -191,6 +196,32 @@ def gemm(A, B, C=None, alpha=1.0, beta=0.0, transA=False, transB=False, precisio
                                  B.ctypes.data_as(_c_double_p), max(B.shape[0], 1), beta,
                                  Cb.ctypes.data_as(_c_double_p), max(M, 1)))
     return Cb
+
+
+def ccsd_solve_batched(systems, opts=None, workers=8, strict=True):
+    """CCSD on many independent systems [(gamma, eps_i, eps_a), ...] in one
+    call; returns (energies, reports).  With strict=False a failed system
+    gets NaN and its status in reports[s]["status"] instead of an error."""
+    n = len(systems)
+    dims = (Dims * max(n, 1))()
+    bufs = []
+    gp = (_c_double_p * max(n, 1))()
+    ip = (_c_double_p * max(n, 1))()
+    ap = (_c_double_p * max(n, 1))()
+    for s, (g, ei, ea) in enumerate(systems):
+        No, Nv = len(ei), len(ea)
+        dims[s] = Dims(No, Nv, int(np.shape(g)[0]))
+        b = (_buf(g), _buf(ei), _buf(ea))
+        bufs.append(b)
+        gp[s], ip[s], ap[s] = _ptr(b[0]), _ptr(b[1]), _ptr(b[2])
+    o = opts if opts is not None else default_opts()
+    energies = np.zeros(max(n, 1))
+    reps = (CcsdReport * max(n, 1))()
+    rc = load().sisi_ccsd_solve_batched(n, dims, gp, ip, ap, ctypes.byref(o), int(workers),
+                                        _ptr(energies), reps)
+    if strict:
+        _check(rc)
+    return energies[:n].copy(), [reps[s].as_dict() for s in range(n)]
 
 
 class Engine:

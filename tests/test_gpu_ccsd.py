@@ -163,3 +163,58 @@ def test_mixed_precision_solve_reaches_fp64_energy():
           % (rmx["iterations"], rmx["fp64_iterations"], r64["iterations"],
              abs(rmx["energy"] - r64["energy"])), flush=True)
     assert abs(rmx["energy"] - r64["energy"]) <= 1e-6
+
+
+def config5_like_systems(n, base=20261202, no=(3, 6), nv=(10, 24)):
+    rng = np.random.default_rng(base)
+    out = []
+    for s in range(n):
+        No = int(rng.integers(no[0], no[1] + 1))
+        Nv = int(rng.integers(nv[0], nv[1] + 1))
+        out.append((base + s, No, Nv, 4 * Nv))
+    return out
+
+
+def test_batched_ccsd_matches_per_system_oracle():
+    """Batched CCSD over uneven systems (config-5 style): each system's
+    10-iteration DIIS(6) energy equals the oracle's to 1e-8 Eh."""
+    from sisi4s_amd import engine
+    specs = config5_like_systems(8)
+    systems = [philox.system(seed, No, Nv, NG) for seed, No, Nv, NG in specs]
+    opts = engine.default_opts(mixer="DiisMixer", max_residua=6, max_iterations=10,
+                               amplitudes_convergence=0.0, energy_convergence=0.0)
+    announce("batched ccsd 8 systems, 4 workers")
+    energies, reports = engine.ccsd_solve_batched(systems, opts, workers=4)
+    for (seed, No, Nv, NG), (g, ei, ea), E, rep in zip(specs, systems, energies, reports):
+        ref = ccsd.solve(g, ei, ea, mixer="DiisMixer", max_residua=6, max_iterations=10,
+                         amplitudes_convergence=0.0, energy_convergence=0.0)
+        assert rep["iterations"] == 10
+        assert abs(E - ref["energy"]) <= 1e-8, (seed, No, Nv, E, ref["energy"])
+
+
+def test_batched_mixed_precision_within_1uEh():
+    from sisi4s_amd import engine
+    specs = config5_like_systems(6, base=99)
+    systems = [philox.system(seed, No, Nv, NG) for seed, No, Nv, NG in specs]
+    kw = dict(mixer="DiisMixer", max_residua=6, max_iterations=200,
+              amplitudes_convergence=1e-8, energy_convergence=1e-10)
+    announce("batched fp64 vs bf16x3 mixed, 6 systems")
+    e64, r64 = engine.ccsd_solve_batched(systems, engine.default_opts(**kw), workers=3)
+    emx, rmx = engine.ccsd_solve_batched(
+        systems, engine.default_opts(precision="bf16x3", switch_threshold=1e-5, **kw), workers=3)
+    assert all(r["converged"] for r in r64) and all(r["converged"] for r in rmx)
+    dmax = float(np.max(np.abs(e64 - emx)))
+    print("batched mixed vs fp64 max |dE| = %.2e" % dmax, flush=True)
+    assert dmax <= 1e-6
+
+
+def test_batched_reports_failure_of_one_system():
+    from sisi4s_amd import engine
+    g, ei, ea = philox.system(1, 3, 8, 32)
+    bad = ea.copy()
+    bad[0] = np.nan
+    announce("batched with one invalid system")
+    with pytest.raises(engine.EngineError) as ex:
+        engine.ccsd_solve_batched([(g, ei, ea), (g, ei, bad)],
+                                  engine.default_opts(max_iterations=2), workers=2)
+    assert "system 1" in str(ex.value)
