@@ -57,6 +57,7 @@ void integrals_from_vertex(Engine &e) {
   axpby(e, v * v * o * o, 2.0, e.Vabij.get(), 0.0, e.Uabij.get());
   axpby(e, v * v * o * o, -1.0, e.Vbaij.get(), 1.0, e.Uabij.get());
   e.Vt.reset();
+  e.Vt2.reset();
   e.VP.reset();
   e.have_integrals = true;
   e.sync();

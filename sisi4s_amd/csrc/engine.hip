@@ -118,6 +118,7 @@ Engine::~Engine() {
   Vbaij.reset();
   Uabij.reset();
   Vt.reset();
+  Vt2.reset();
   VP.reset();
   pool.release_all();
   if (stream) (void)hipStreamDestroy(stream);

@@ -116,8 +116,9 @@ class Engine {
   DBuf Vabij, Vaibj, Vijkl, Vijka;
   DBuf Vbaij;  // V_abij with a,b exchanged (energy exchange term)
   DBuf Uabij;  // 2 V_abij - V_baij
-  // (T) integrals, built on first use: V_bdck as [d,b,c,k] and V_jklc as [l,c,j,k]
-  DBuf Vt, VP;
+  // (T) integrals, built on first use: V_bdck as [d,b,c,k] and as [d,c,b,k],
+  // V_jklc as [l,c,j,k]
+  DBuf Vt, Vt2, VP;
 
   // CCSD run state
   sisi_ccsd_opts opts{};

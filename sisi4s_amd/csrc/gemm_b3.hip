@@ -16,9 +16,7 @@ namespace sisi {
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef unsigned short u16;
-struct alignas(16) U128 {
-  unsigned int x, y, z, w;
-};
+typedef unsigned int U128 __attribute__((ext_vector_type(4)));
 
 // ---------------------------------------------------------------------------
 // operand split: (R x K) strided FP64 view -> hi/lo [Rp][Kp] bf16
@@ -97,9 +95,10 @@ __device__ __forceinline__ int64_t b3_off_n(const GemmArgs &g, int n) {
 }
 
 // BM x BN block, 4 waves of WM x WN, k-chunk 32, planes staged through LDS
-// with 16-byte stores; the next chunk is prefetched into registers.
-template <int BM, int BN, int WM, int WN>
-__global__ __launch_bounds__(256, 1) void gemm_b3p_kernel(B3Args p) {
+// with 16-byte stores.  NBUF register sets keep NBUF chunks of global loads in
+// flight, so a chunk's load latency is covered by NBUF chunks of MFMA work.
+template <int BM, int BN, int WM, int WN, int NBUF>
+__global__ __launch_bounds__(256) void gemm_b3p_kernel(B3Args p) {
   constexpr int BK = 32, RS = BK + 8;  // LDS row stride in bf16 (80 bytes)
   constexpr int NA = BM * 4 / 256, NB = BN * 4 / 256;  // 16-byte chunks per thread per plane
   static_assert((BM / WM) * (BN / WN) == 4, "four waves");
@@ -122,38 +121,37 @@ __global__ __launch_bounds__(256, 1) void gemm_b3p_kernel(B3Args p) {
   const int ks = blockIdx.y, bz = blockIdx.z;
   const int kbeg = ks * g.kchunk;
   const int kend = min(int(p.Kp), kbeg + g.kchunk);
+  const int nk = kend > kbeg ? (kend - kbeg) / BK : 0;
   const u16 *Agh = p.Ah + int64_t(m0) * p.Kp, *Agl = p.Al + int64_t(m0) * p.Kp;
   const u16 *Bgh = p.Bh + int64_t(n0) * p.Kp, *Bgl = p.Bl + int64_t(n0) * p.Kp;
 
-  U128 ra[2 * NA], rb[2 * NB];
-  auto load = [&](int k0) {
-#pragma unroll
-    for (int i = 0; i < NA; ++i) {
-      const int c = tid + 256 * i, row = c >> 2, kq = (c & 3) * 8;
-      ra[i] = *reinterpret_cast<const U128 *>(Agh + int64_t(row) * p.Kp + k0 + kq);
-      ra[NA + i] = *reinterpret_cast<const U128 *>(Agl + int64_t(row) * p.Kp + k0 + kq);
-    }
-#pragma unroll
-    for (int i = 0; i < NB; ++i) {
-      const int c = tid + 256 * i, row = c >> 2, kq = (c & 3) * 8;
-      rb[i] = *reinterpret_cast<const U128 *>(Bgh + int64_t(row) * p.Kp + k0 + kq);
-      rb[NB + i] = *reinterpret_cast<const U128 *>(Bgl + int64_t(row) * p.Kp + k0 + kq);
-    }
-  };
-  auto store = [&]() {
-#pragma unroll
-    for (int i = 0; i < NA; ++i) {
-      const int c = tid + 256 * i, row = c >> 2, kq = (c & 3) * 8;
-      *reinterpret_cast<U128 *>(&Ah[row * RS + kq]) = ra[i];
-      *reinterpret_cast<U128 *>(&Al[row * RS + kq]) = ra[NA + i];
-    }
-#pragma unroll
-    for (int i = 0; i < NB; ++i) {
-      const int c = tid + 256 * i, row = c >> 2, kq = (c & 3) * 8;
-      *reinterpret_cast<U128 *>(&Bh[row * RS + kq]) = rb[i];
-      *reinterpret_cast<U128 *>(&Bl[row * RS + kq]) = rb[NB + i];
-    }
-  };
+  U128 ra[NBUF][2 * NA], rb[NBUF][2 * NB];
+#define B3_LOAD(b, k0)                                                                   \
+  do {                                                                                  \
+    _Pragma("unroll") for (int i = 0; i < NA; ++i) {                                    \
+      const int c = tid + 256 * i, row = c >> 2, kq = (c & 3) * 8;                       \
+      ra[b][i] = *reinterpret_cast<const U128 *>(Agh + int64_t(row) * p.Kp + (k0) + kq);      \
+      ra[b][NA + i] = *reinterpret_cast<const U128 *>(Agl + int64_t(row) * p.Kp + (k0) + kq); \
+    }                                                                                   \
+    _Pragma("unroll") for (int i = 0; i < NB; ++i) {                                    \
+      const int c = tid + 256 * i, row = c >> 2, kq = (c & 3) * 8;                       \
+      rb[b][i] = *reinterpret_cast<const U128 *>(Bgh + int64_t(row) * p.Kp + (k0) + kq);      \
+      rb[b][NB + i] = *reinterpret_cast<const U128 *>(Bgl + int64_t(row) * p.Kp + (k0) + kq); \
+    }                                                                                   \
+  } while (0)
+#define B3_STORE(b)                                                                      \
+  do {                                                                                  \
+    _Pragma("unroll") for (int i = 0; i < NA; ++i) {                                    \
+      const int c = tid + 256 * i, row = c >> 2, kq = (c & 3) * 8;                       \
+      *reinterpret_cast<U128 *>(&Ah[row * RS + kq]) = ra[b][i];                          \
+      *reinterpret_cast<U128 *>(&Al[row * RS + kq]) = ra[b][NA + i];                     \
+    }                                                                                   \
+    _Pragma("unroll") for (int i = 0; i < NB; ++i) {                                    \
+      const int c = tid + 256 * i, row = c >> 2, kq = (c & 3) * 8;                       \
+      *reinterpret_cast<U128 *>(&Bh[row * RS + kq]) = rb[b][i];                          \
+      *reinterpret_cast<U128 *>(&Bl[row * RS + kq]) = rb[b][NB + i];                     \
+    }                                                                                   \
+  } while (0)
 
   constexpr int TI = WM / 16, TJ = WN / 16;
   f4 acc[TI][TJ];
@@ -165,14 +163,10 @@ __global__ __launch_bounds__(256, 1) void gemm_b3p_kernel(B3Args p) {
   const int wm = (wave / WX) * WM, wn = (wave % WX) * WN;
   const int l16 = lane & 15, lk = lane >> 4;
 
-  int k0 = kbeg;
-  if (k0 < kend) load(k0);
-  while (k0 < kend) {
-    store();
-    __syncthreads();
-    const int k1 = k0 + BK;
-    if (k1 < kend) load(k1);
-    bf8 bh[TJ], bl[TJ];
+  // Three passes (hi*lo, lo*hi, hi*hi) over all tiles: consecutive MFMAs
+  // update different accumulators, so none waits on the previous result.
+  auto compute = [&]() {
+    bf8 bh[TJ], bl[TJ], ah[TI], al[TI];
 #pragma unroll
     for (int j = 0; j < TJ; ++j) {
       const int col = wn + j * 16 + l16;
@@ -182,18 +176,44 @@ __global__ __launch_bounds__(256, 1) void gemm_b3p_kernel(B3Args p) {
 #pragma unroll
     for (int i = 0; i < TI; ++i) {
       const int row = wm + i * 16 + l16;
-      const bf8 ah = *reinterpret_cast<const bf8 *>(&Ah[row * RS + 8 * lk]);
-      const bf8 al = *reinterpret_cast<const bf8 *>(&Al[row * RS + 8 * lk]);
+      ah[i] = *reinterpret_cast<const bf8 *>(&Ah[row * RS + 8 * lk]);
+      al[i] = *reinterpret_cast<const bf8 *>(&Al[row * RS + 8 * lk]);
+    }
 #pragma unroll
-      for (int j = 0; j < TJ; ++j) {
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[j], ah, acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], al, acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], ah, acc[i][j], 0, 0, 0);
+    for (int i = 0; i < TI; ++i)
+#pragma unroll
+      for (int j = 0; j < TJ; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[j], ah[i], acc[i][j], 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < TI; ++i)
+#pragma unroll
+      for (int j = 0; j < TJ; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], al[i], acc[i][j], 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < TI; ++i)
+#pragma unroll
+      for (int j = 0; j < TJ; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], ah[i], acc[i][j], 0, 0, 0);
+  };
+
+#pragma unroll
+  for (int b = 0; b < NBUF; ++b)
+    if (b < nk) B3_LOAD(b, kbeg + b * BK);
+  for (int kc = 0; kc < nk; kc += NBUF) {
+#pragma unroll
+    for (int b = 0; b < NBUF; ++b) {
+      const int c = kc + b;
+      if (c < nk) {
+        B3_STORE(b);
+        __syncthreads();
+        if (c + NBUF < nk) B3_LOAD(b, kbeg + (c + NBUF) * BK);
+        compute();
+        __syncthreads();
       }
     }
-    __syncthreads();
-    k0 = k1;
   }
+#undef B3_LOAD
+#undef B3_STORE
 
   if (g.ksplit == 1) {
     double *C = g.C + int64_t(bz) * g.bsc;
@@ -231,7 +251,6 @@ __global__ __launch_bounds__(256, 1) void gemm_b3p_kernel(B3Args p) {
     }
   }
 }
-
 
 namespace {
 
@@ -281,8 +300,8 @@ const Planes &split_operand(Engine &e, const double *src, int64_t sr, int64_t sk
 
 }  // namespace
 
-void gemm_b3(Engine &e, GemmArgs g) {
-  constexpr int BM = 256, BN = 128, WM = 128, WN = 64;
+template <int BM, int BN, int WM, int WN, int NBUF>
+static void run_b3(Engine &e, GemmArgs g) {
   const int Kp = ((g.K + 31) / 32) * 32;
   const int Mp = ((g.M + BM - 1) / BM) * BM, Np = ((g.N + BN - 1) / BN) * BN;
   Planes pa, pb;
@@ -311,10 +330,21 @@ void gemm_b3(Engine &e, GemmArgs g) {
   dim3 grid(unsigned(tiles), unsigned(ksplit), 1);
   {
     Timed t(*e.timers, Kind::GemmB3, gemm_flops(g));
-    hipLaunchKernelGGL((gemm_b3p_kernel<BM, BN, WM, WN>), grid, dim3(256), 0, e.stream, p);
+    hipLaunchKernelGGL((gemm_b3p_kernel<BM, BN, WM, WN, NBUF>), grid, dim3(256), 0, e.stream, p);
     SISI_HIP(hipGetLastError());
   }
   if (ksplit > 1) gemm_reduce(e, g);
+}
+
+void gemm_b3(Engine &e, GemmArgs g) {
+  static const int cfg = getenv("SISI_B3_CFG") ? atoi(getenv("SISI_B3_CFG")) : 0;
+  switch (cfg) {
+    case 1: run_b3<256, 128, 128, 64, 1>(e, g); break;
+    case 2: run_b3<128, 128, 64, 64, 1>(e, g); break;
+    case 3: run_b3<128, 128, 64, 64, 2>(e, g); break;
+    case 4: run_b3<256, 128, 128, 64, 3>(e, g); break;
+    default: run_b3<256, 128, 128, 64, 2>(e, g); break;
+  }
 }
 
 }  // namespace sisi

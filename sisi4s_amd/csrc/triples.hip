@@ -122,9 +122,14 @@ __global__ void sum_into_kernel(const double *part, int n, double *out) {
   if (threadIdx.x == 0) *out = s;
 }
 
-// W(x,y,z)[u0,u1,u2] accumulated (beta = 1) at Wsym with output strides st.
-void add_w(Engine &e, int prec, const double *T2, const double *Vt, const double *VijkaP,
-           int x, int y, int z, double *Wp, const int64_t st[3]) {
+// W(x,y,z)[u0,u1,u2] accumulated (beta = 1) at Wp with output strides st.
+// The epilogue's lanes walk the output's unit-stride index when it is the
+// leading index of the GEMM's row or column group, so the operand orders are
+// chosen per permutation: V_bdck as [d,c,b] when c lands on the unit stride,
+// T2 as [b,a,x,l] when b does (two copies of the same tensors, exact).
+void add_w(Engine &e, int prec, const double *T2, const double *T2t, const double *Vt,
+           const double *Vt2, const double *VijkaP, int x, int y, int z, double *Wp,
+           const int64_t st[3]) {
   const int64_t o = e.No, v = e.Nv;
   View C;
   C.p = Wp;
@@ -134,10 +139,15 @@ void add_w(Engine &e, int prec, const double *T2, const double *Vt, const double
     C.stride[k] = st[k];
   }
   View Txy = dense(const_cast<double *>(T2) + (x + o * y) * v * v, {v, v});
-  View Vz = dense(const_cast<double *>(Vt) + z * v * v * v, {v, v, v});
-  ctr(e, 1.0, "ad,dbc->abc", Txy, Vz, 1.0, C, prec);  // sum_d T_ad,xy V_bdcz
+  if (st[2] == 1) {
+    View Vz = dense(const_cast<double *>(Vt2) + z * v * v * v, {v, v, v});
+    ctr(e, 1.0, "ad,dcb->abc", Txy, Vz, 1.0, C, prec);  // sum_d T_ad,xy V_bdcz
+  } else {
+    View Vz = dense(const_cast<double *>(Vt) + z * v * v * v, {v, v, v});
+    ctr(e, 1.0, "ad,dbc->abc", Txy, Vz, 1.0, C, prec);
+  }
   View Tx;
-  Tx.p = const_cast<double *>(T2) + x * v * v;
+  Tx.p = const_cast<double *>(st[1] == 1 ? T2t : T2) + x * v * v;
   Tx.order = 3;
   Tx.len[0] = v;
   Tx.len[1] = v;
@@ -152,7 +162,8 @@ void add_w(Engine &e, int prec, const double *T2, const double *Vt, const double
   Vyz.len[1] = v;
   Vyz.stride[0] = 1;
   Vyz.stride[1] = o;
-  ctr(e, -1.0, "abl,lc->abc", Tx, Vyz, 1.0, C, prec);  // - sum_l T_ab,xl V_yzlc
+  // - sum_l T_ab,xl V_yzlc (T2t[b,a,x,l] = T2[a,b,x,l])
+  ctr(e, -1.0, st[1] == 1 ? "bal,lc->abc" : "abl,lc->abc", Tx, Vyz, 1.0, C, prec);
 }
 
 }  // namespace
@@ -175,12 +186,19 @@ double triples_energy(Engine &e, const double *T1, const double *T2, int prec,
     e.Vt = DBuf(&e.pool, v * v * v * o);
     ctr(e, 1.0, "Fbd,Fck->dbck", dense(e.Gab.get(), {G, v, v}), dense(e.Gai.get(), {G, v, o}),
         0.0, dense(e.Vt.get(), {v, v, v, o}), SISI_FP64);
+    e.Vt2 = DBuf(&e.pool, v * v * v * o);
+    sum(e, 1.0, dense(e.Vt.get(), {v, v, v, o}), "dbck", 0.0, dense(e.Vt2.get(), {v, v, v, o}),
+        "dcbk");
     // V_jklc (HHHP) as [l, c, j, k] so each (j,k) slice is a [l, c] matrix
     e.VP = DBuf(&e.pool, o * v * o * o);
     sum(e, 1.0, dense(e.Vijka.get(), {o, o, o, v}), "jklc", 0.0, dense(e.VP.get(), {o, v, o, o}),
         "lcjk");
   }
-  DBuf &Vt = e.Vt, &VP = e.VP;
+  DBuf &Vt = e.Vt, &Vt2 = e.Vt2, &VP = e.VP;
+  // T2 with a and b exchanged, for the permutations that put b on the unit stride
+  DBuf T2t(&e.pool, v * v * o * o);
+  sum(e, 1.0, dense(const_cast<double *>(T2), {v, v, o, o}), "abij", 0.0,
+      dense(T2t.get(), {v, v, o, o}), "baij");
   std::vector<double> eps_i(o);
   e.download(eps_i.data(), e.eps_i, size_t(o));
 
@@ -210,7 +228,8 @@ double triples_energy(Engine &e, const double *T1, const double *T2, int prec,
     if (!degenerate) {
       for (int p = 0; p < 6; ++p) {
         const int64_t st[3] = {s[perms[p][0]], s[perms[p][1]], s[perms[p][2]]};
-        add_w(e, prec, T2, Vt.get(), VP.get(), tup[p][0], tup[p][1], tup[p][2], Wsym.get(), st);
+        add_w(e, prec, T2, T2t.get(), Vt.get(), Vt2.get(), VP.get(), tup[p][0], tup[p][1], tup[p][2],
+              Wsym.get(), st);
       }
     } else {
       zero(e, Wdist.get(), v * v * v);
@@ -220,7 +239,8 @@ double triples_energy(Engine &e, const double *T1, const double *T2, int prec,
         if (distinct[p]) {
           zero(e, Wp, v * v * v);
           const int64_t st[3] = {1, v, v * v};
-          add_w(e, prec, T2, Vt.get(), VP.get(), tup[p][0], tup[p][1], tup[p][2], Wp, st);
+          add_w(e, prec, T2, T2t.get(), Vt.get(), Vt2.get(), VP.get(), tup[p][0], tup[p][1],
+                tup[p][2], Wp, st);
         }
         View Wv = dense(Wp, {v, v, v});
         View Ws, Wd;
