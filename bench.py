@@ -86,6 +86,9 @@ def main():
     ap.add_argument("--level-shift", type=float, default=0.0)
     ap.add_argument("--max-iterations", type=int, default=400)
     ap.add_argument("--precisions", default="fp64,mixed")
+    ap.add_argument("--refine-interval", type=int, default=8,
+                    help="mixed runs: FP64 anchor every k iterations after the switch (0: plain "
+                         "FP64 iterations)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -254,7 +257,8 @@ def converge(args):
     for name, prec, tprec in todo:
         o = eng.default_opts(mixer="DiisMixer", max_residua=6, max_iterations=args.max_iterations,
                              slice_size=s, amplitudes_convergence=1e-8, energy_convergence=1e-10,
-                             precision=prec, switch_threshold=1e-5, level_shift=args.level_shift)
+                             precision=prec, switch_threshold=1e-5, level_shift=args.level_shift,
+                             refine_interval=args.refine_interval if prec != "fp64" else 0)
         t0 = time.perf_counter()
         e.ccsd_begin(o)
         rep = None
@@ -324,7 +328,9 @@ def batched(args):
     res = {}
     for name, prec in (("fp64", "fp64"), ("mixed", "bf16x3")):
         t0 = time.perf_counter()
-        E, reps = eng.ccsd_solve_batched(systems, eng.default_opts(precision=prec, **kw),
+        ri = args.refine_interval if prec != "fp64" else 0
+        E, reps = eng.ccsd_solve_batched(systems, eng.default_opts(precision=prec,
+                                                                   refine_interval=ri, **kw),
                                          workers=args.workers, strict=False)
         res[name] = dict(seconds=time.perf_counter() - t0, energies=E.tolist(),
                          failed=[k for k, r in enumerate(reps) if r["status"] != 0],

@@ -76,6 +76,11 @@ typedef struct {
   int precision;                 /* bulk precision: SISI_FP64/_FP32/_BF16X3 */
   double switch_threshold;       /* |dT|/|T| at which a mixed run switches
                                     to FP64 iterations (default 1e-5)      */
+  int refine_interval;           /* after the switch: 0 = plain FP64
+                                    iterations; k > 0 = an FP64 anchor
+                                    residual every k iterations and
+                                    refinement residuals (anchor + reduced-
+                                    precision change) in between          */
 } sisi_ccsd_opts;
 
 typedef struct {
@@ -136,6 +141,18 @@ int sisi_engine_set_precision(sisi_engine *e, int precision);
 int sisi_ccsd_residual(sisi_engine *e, int iteration, int initial_doubles,
                        int slice_size, const double *T1, const double *T2,
                        double *R1, double *R2);
+
+/* The residual of sisi_ccsd_residual (iteration >= 1) in two parts used by
+ * the mixed-precision refinement: _anchor evaluates it in FP64 at T and
+ * records T, R and the T-dependent intermediates as the anchor; _refine
+ * evaluates it at a new T as R(T_anchor) + [R(T) - R(T_anchor)], with the
+ * bracket expanded so every product carries a factor T - T_anchor and
+ * computed in the precision of sisi_engine_set_precision.  No counterpart
+ * in the reference (its residual is always one FP64 evaluation). */
+int sisi_ccsd_residual_anchor(sisi_engine *e, int slice_size, const double *T1,
+                              const double *T2, double *R1, double *R2);
+int sisi_ccsd_residual_refine(sisi_engine *e, int slice_size, const double *T1,
+                              const double *T2, double *R1, double *R2);
 
 /* ClusterSinglesDoublesAlgorithm::getEnergy (cxx:131-213), restricted,
  * non-antisymmetrized: out[0] = E, out[1] = direct, out[2] = exchange. */

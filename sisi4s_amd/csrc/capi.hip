@@ -74,6 +74,7 @@ void sisi_ccsd_opts_default(sisi_ccsd_opts *o) {
   o->slice_size = -1;
   o->precision = SISI_FP64;
   o->switch_threshold = 1e-5;
+  o->refine_interval = 0;
 }
 
 int sisi_engine_create(const sisi_dims *dims, sisi_engine **out) {
@@ -169,6 +170,33 @@ int sisi_ccsd_residual(sisi_engine *h, int iteration, int initial_doubles, int s
     e.download(R1, r1, e.n1());
     e.download(R2, r2, e.n2());
   });
+}
+
+static int residual_variant(sisi_engine *h, bool anchor, int slice_size, const double *T1,
+                            const double *T2, double *R1, double *R2) {
+  return guarded([&] {
+    ENGINE_REQUIRED(h);
+    Engine &e = h->e;
+    SISI_REQUIRE(T1 && T2 && R1 && R2, SISI_ERR_INVALID, "null amplitude/residual pointer");
+    SISI_REQUIRE(e.have_integrals, SISI_ERR_STATE, "integrals not built");
+    DBuf t1, t2, r1(&e.pool, e.n1()), r2(&e.pool, e.n2());
+    e.upload(t1, T1, e.n1());
+    e.upload(t2, T2, e.n2());
+    if (anchor) ccsd_residual_anchor(e, slice_size, t1.get(), t2.get(), r1.get(), r2.get());
+    else ccsd_residual_delta(e, slice_size, t1.get(), t2.get(), r1.get(), r2.get());
+    e.download(R1, r1, e.n1());
+    e.download(R2, r2, e.n2());
+  });
+}
+
+int sisi_ccsd_residual_anchor(sisi_engine *h, int slice_size, const double *T1,
+                              const double *T2, double *R1, double *R2) {
+  return residual_variant(h, true, slice_size, T1, T2, R1, R2);
+}
+
+int sisi_ccsd_residual_refine(sisi_engine *h, int slice_size, const double *T1,
+                              const double *T2, double *R1, double *R2) {
+  return residual_variant(h, false, slice_size, T1, T2, R1, R2);
 }
 
 int sisi_ccsd_energy(sisi_engine *h, const double *T1, const double *T2, double out[3]) {

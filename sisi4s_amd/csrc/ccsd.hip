@@ -113,6 +113,10 @@ void ccsd_residual(Engine &e, int it, bool initial_doubles, int slice_size, cons
     copy(e, Lki, Kki);  // cxx:177
     ctr(e, 2.0, "klic,cl->ki", Vijka, T1, 1.0, Lki, prec);   // cxx:188
     ctr(e, -1.0, "lkic,cl->ki", Vijka, T1, 1.0, Lki, prec);  // cxx:189
+    e.record("Kac", Kacb.get(), v * v);
+    e.record("Lac", Lacb.get(), v * v);
+    e.record("Kki", Kkib.get(), o * o);
+    e.record("Lki", Lkib.get(), o * o);
     ctr(e, 1.0, "ac,cbij->abij", Lac, T2, 1.0, R2, prec);    // cxx:192
     ctr(e, -1.0, "ki,abkj->abij", Lki, T2, 1.0, R2, prec);   // cxx:195
   }
@@ -123,11 +127,14 @@ void ccsd_residual(Engine &e, int it, bool initial_doubles, int slice_size, cons
     copy(e, dGai, Gai);
     ctr(e, -1.0, "Gki,ak->Gai", Gij, T1, 1.0, dGai, prec);   // cxx:205
     ctr(e, 1.0, "Gbc,cj->Gbj", Gab, T1, 0.0, Q, prec);       // cxx:208 (Gamma_Gbc T_cj)
+    e.record("dGai2b", dGb.get(), G * v * o);
+    e.record("Q", Qb.get(), G * v * o);
     ctr(e, 1.0, "Gai,Gbj->abij", dGai, Q, 1.0, R2, prec);    // cxx:208-209
     ctr(e, -1.0, "jika,bk->abij", Vijka, T1, 1.0, R2, prec); // cxx:213
     DBuf Pb(&e.pool, v * o * o * o);
     View P = V4(Pb.get(), v, o, o, o);
     ctr(e, 1.0, "acik,cj->aikj", V, T1, 0.0, P, prec);       // cxx:214-215
+    e.record("P", Pb.get(), v * o * o * o);
     ctr(e, -1.0, "bk,aikj->abij", T1, P, 1.0, R2, prec);
   }
   {
@@ -146,6 +153,7 @@ void ccsd_residual(Engine &e, int it, bool initial_doubles, int slice_size, cons
     ctr(e, -0.5, "dclk,dail->akic", V, Y, 1.0, Xakic, prec);   // cxx:242
     ctr(e, 1.0, "dclk,adil->akic", V, T2, 1.0, Xakic, prec);   // cxx:243
     ctr(e, -0.5, "cdlk,adil->akic", V, T2, 1.0, Xakic, prec);  // cxx:245
+    e.record("Xakic", Xb2.get(), v * o * o * v);
     // Y = 2 T[cbkj] - T[bckj] (cxx:248-249)
     axpby(e, n2, 2.0, T2p, 0.0, Yb.get());
     sum(e, -1.0, T2, "bckj", 1.0, Y, "cbkj");
@@ -159,11 +167,14 @@ void ccsd_residual(Engine &e, int it, bool initial_doubles, int slice_size, cons
     copy(e, dGij, Gij);
     ctr(e, -1.0, "Gcl,al->Gac", Gai, T1, 1.0, dGab, prec);  // cxx:270
     ctr(e, 1.0, "Gdk,di->Gki", Gai, T1, 1.0, dGij, prec);   // cxx:273
+    e.record("dGab4", dGabb.get(), G * v * v);
+    e.record("dGij4", dGijb.get(), G * o * o);
     DBuf Xb3(&e.pool, v * o * v * o);
     View Xakci = V4(Xb3.get(), v, o, v, o);
     ctr(e, 1.0, "Gac,Gki->akci", dGab, dGij, 0.0, Xakci, prec);  // cxx:277-278
     ctr(e, -0.5, "cdlk,dail->akci", V, T2, 1.0, Xakci, prec);    // cxx:284
     e.frozen.push_back(Xb3.get());
+    e.record("Xakci", Xb3.get(), v * o * v * o);
     ctr(e, -1.0, "akci,cbkj->abij", Xakci, T2, 1.0, R2, prec);   // cxx:288
     ctr(e, -1.0, "bkci,ackj->abij", Xakci, T2, 1.0, R2, prec);   // cxx:289
     e.unfreeze(Xb3.get());  // Xb3 returns to the pool at the end of this block
@@ -182,8 +193,10 @@ void ccsd_residual(Engine &e, int it, bool initial_doubles, int slice_size, cons
     copy(e, Xklij, Vijkl);
     ctr(e, 1.0, "klic,cj->klij", Vijka, T1, 1.0, Xklij, prec);  // cxx:308
     ctr(e, 1.0, "lkjc,ci->klij", Vijka, T1, 1.0, Xklij, prec);  // cxx:309
+    e.record("Xklij1", Xb4.get(), o * o * o * o);
     ctr(e, 1.0, "klij,abkl->abij", Xklij, X, 1.0, R2, prec);    // cxx:313
     ctr(e, 1.0, "cdkl,cdij->klij", V, X, 0.0, Xklij, prec);     // cxx:317
+    e.record("Xklij2", Xb4.get(), o * o * o * o);
     ctr(e, 1.0, "klij,abkl->abij", Xklij, T2, 1.0, R2, prec);   // cxx:330 (ppl = 1)
   }
   {
@@ -193,6 +206,7 @@ void ccsd_residual(Engine &e, int it, bool initial_doubles, int slice_size, cons
     View dGab = V3(dGabb.get(), G, v, v);
     copy(e, dGab, Gab);
     ctr(e, -1.0, "Gbk,ak->Gab", Gai, T1, 1.0, dGab, prec);  // cxx:397-400
+    e.record("dGabP", dGabb.get(), G * v * v);
     const int nslices = int((v + slice_size - 1) / slice_size);
     std::vector<DBuf> sl(nslices);
     std::vector<int64_t> lo(nslices), hi(nslices);
@@ -225,9 +239,11 @@ void ccsd_residual(Engine &e, int it, bool initial_doubles, int slice_size, cons
     DBuf Kckb(&e.pool, v * o), Skib(&e.pool, o * o);
     View Kck = V2(Kckb.get(), v, o);
     ctr(e, 1.0, "cdkl,dl->ck", U, T1, 0.0, Kck, prec);   // cxx:470-471
+    e.record("Kck", Kckb.get(), v * o);
     ctr(e, 2.0, "ck,caki->ai", Kck, T2, 1.0, R1, prec);  // cxx:474
     ctr(e, -1.0, "ck,caik->ai", Kck, T2, 1.0, R1, prec); // cxx:475
     ctr(e, 1.0, "ck,ci->ki", Kck, T1, 0.0, V2(Skib.get(), o, o), prec);  // cxx:476
+    e.record("Ski", Skib.get(), o * o);
     ctr(e, 1.0, "ak,ki->ai", T1, V2(Skib.get(), o, o), 1.0, R1, prec);
     ctr(e, 2.0, "acik,ck->ai", V, T1, 1.0, R1, prec);      // cxx:477
     ctr(e, -1.0, "ciak,ck->ai", Vaibj, T1, 1.0, R1, prec); // cxx:478
@@ -241,6 +257,293 @@ void ccsd_residual(Engine &e, int it, bool initial_doubles, int slice_size, cons
     ctr(e, 1.0, "lkic,ackl->ai", Vijka, X, 1.0, R1, prec);   // cxx:490
   }
   e.thaw();
+}
+
+// Residual at T = T_A + d by refinement around the recorded FP64 anchor
+// T_A: R(T) = R(T_A) + dR, where dR is evaluated from the reference's term
+// list with every product of T-dependent factors expanded as
+//   P(T) Q(T) - P(T_A) Q(T_A) = dP Q(T) + P(T_A) dQ
+// and every product with a constant as c dP.  Each contraction of dR carries
+// a d factor, so its rounding error is relative to |d|, not to |R|: dR can
+// run on the reduced-precision MFMA paths while R keeps FP64 accuracy.
+// Elementwise intermediates (X, Y, dressed vertices) are formed in FP64 at T
+// and differenced.  The anchor must have been recorded (e.recording) by an
+// FP64 residual at T_A with the same slice size.
+void ccsd_residual_delta(Engine &e, int slice_size, const double *T1c, const double *T2c,
+                         double *R1p, double *R2p) {
+  Anchor &an = e.anchor;
+  SISI_REQUIRE(an.valid, SISI_ERR_STATE, "refinement residual without an anchor");
+  const int64_t o = e.No, v = e.Nv, G = e.NG;
+  const int64_t n2 = v * v * o * o, n1 = v * o;
+  const int prec = e.precision;
+  e.thaw();
+  double *T1p = const_cast<double *>(T1c), *T2p = const_cast<double *>(T2c);
+  View T1 = V2(T1p, v, o), T2 = V4(T2p, v, v, o, o);
+  View T1A = V2(an.T1.get(), v, o);
+  View V = V4(e.Vabij.get(), v, v, o, o), U = V4(e.Uabij.get(), v, v, o, o);
+  View Vaibj = V4(e.Vaibj.get(), v, o, v, o), Vijka = V4(e.Vijka.get(), o, o, o, v);
+  View Gij = V3(e.Gij.get(), G, o, o), Gai = V3(e.Gai.get(), G, v, o),
+       Gab = V3(e.Gab.get(), G, v, v);
+  auto A = [&](const char *name) -> double * {
+    auto it = an.v.find(name);
+    SISI_REQUIRE(it != an.v.end(), SISI_ERR_STATE, std::string("anchor lacks ") + name);
+    return it->second.get();
+  };
+  // amplitude changes d1 = T1 - T1_A, d2 = T2 - T2_A (FP64)
+  DBuf d1b(&e.pool, n1), d2b(&e.pool, n2);
+  axpby(e, n1, 1.0, T1p, 0.0, d1b.get());
+  axpby(e, n1, -1.0, an.T1.get(), 1.0, d1b.get());
+  axpby(e, n2, 1.0, T2p, 0.0, d2b.get());
+  axpby(e, n2, -1.0, an.T2.get(), 1.0, d2b.get());
+  View d1 = V2(d1b.get(), v, o), d2 = V4(d2b.get(), v, v, o, o);
+  e.frozen = {T1p, T2p, d1b.get(), d2b.get(), an.T1.get(), an.T2.get(), e.Vabij.get(),
+              e.Uabij.get(), e.Vaibj.get(), e.Vijka.get(), e.Gij.get(), e.Gai.get(), e.Gab.get()};
+  // X = T2 + T1 T1 at T and its change (FP64 elementwise)
+  DBuf Xnb(&e.pool, n2), dXb(&e.pool, n2);
+  add_t1t1(e, Xnb.get(), T2p, T1p, 1.0, int(v), int(o));
+  add_t1t1(e, dXb.get(), an.T2.get(), an.T1.get(), 1.0, int(v), int(o));
+  axpby(e, n2, 1.0, Xnb.get(), -1.0, dXb.get());
+  View Xn = V4(Xnb.get(), v, v, o, o), dX = V4(dXb.get(), v, v, o, o);
+  e.frozen.push_back(Xnb.get());
+  e.frozen.push_back(dXb.get());
+  DBuf dR2b(&e.pool, n2), dR1b(&e.pool, n1);
+  zero(e, dR2b.get(), n2);
+  zero(e, dR1b.get(), n1);
+  View dR2 = V4(dR2b.get(), v, v, o, o), dR1 = V2(dR1b.get(), v, o);
+
+  DBuf dKacb(&e.pool, v * v), dLacb(&e.pool, v * v), dKkib(&e.pool, o * o), dLkib(&e.pool, o * o);
+  View dKac = V2(dKacb.get(), v, v), dLac = V2(dLacb.get(), v, v), dKki = V2(dKkib.get(), o, o),
+       dLki = V2(dLkib.get(), o, o);
+  {
+    ctr(e, -1.0, "cdkl,adkl->ac", U, dX, 0.0, dKac, prec);
+    copy(e, dLac, dKac);
+    DBuf Zb(&e.pool, G), Yb(&e.pool, G * v * o);
+    ctr(e, 1.0, "Gdk,dk->G", Gai, d1, 0.0, V1(Zb.get(), G), prec);
+    ctr(e, 2.0, "Gca,G->ac", Gab, V1(Zb.get(), G), 1.0, dLac, prec);
+    ctr(e, 1.0, "Gda,dk->Gak", Gab, d1, 0.0, V3(Yb.get(), G, v, o), prec);
+    ctr(e, -1.0, "Gck,Gak->ac", Gai, V3(Yb.get(), G, v, o), 1.0, dLac, prec);
+    ctr(e, 1.0, "cdkl,cdil->ki", U, dX, 0.0, dKki, prec);
+    copy(e, dLki, dKki);
+    ctr(e, 2.0, "klic,cl->ki", Vijka, d1, 1.0, dLki, prec);
+    ctr(e, -1.0, "lkic,cl->ki", Vijka, d1, 1.0, dLki, prec);
+    // Lac T2, Lki T2
+    ctr(e, 1.0, "ac,cbij->abij", dLac, T2, 1.0, dR2, prec);
+    ctr(e, 1.0, "ac,cbij->abij", V2(A("Lac"), v, v), d2, 1.0, dR2, prec);
+    ctr(e, -1.0, "ki,abkj->abij", dLki, T2, 1.0, dR2, prec);
+    ctr(e, -1.0, "ki,abkj->abij", V2(A("Lki"), o, o), d2, 1.0, dR2, prec);
+  }
+  {
+    // dressed Gamma_ai (Gamma_ai - Gamma_ki T_ak) times Q = Gamma_bc T_cj
+    DBuf ddGb(&e.pool, G * v * o), dQb(&e.pool, G * v * o), Qnb(&e.pool, G * v * o);
+    View ddG = V3(ddGb.get(), G, v, o), dQ = V3(dQb.get(), G, v, o), Qn = V3(Qnb.get(), G, v, o);
+    ctr(e, -1.0, "Gki,ak->Gai", Gij, d1, 0.0, ddG, prec);
+    ctr(e, 1.0, "Gbc,cj->Gbj", Gab, d1, 0.0, dQ, prec);
+    axpby(e, G * v * o, 1.0, A("Q"), 0.0, Qnb.get());
+    axpby(e, G * v * o, 1.0, dQb.get(), 1.0, Qnb.get());
+    ctr(e, 1.0, "Gai,Gbj->abij", ddG, Qn, 1.0, dR2, prec);
+    ctr(e, 1.0, "Gai,Gbj->abij", V3(A("dGai2b"), G, v, o), dQ, 1.0, dR2, prec);
+    ctr(e, -1.0, "jika,bk->abij", Vijka, d1, 1.0, dR2, prec);
+    // T1 (V T1): d1 P(T) + T1_A dP
+    DBuf dPb(&e.pool, v * o * o * o), Pnb(&e.pool, v * o * o * o);
+    View dP = V4(dPb.get(), v, o, o, o), Pn = V4(Pnb.get(), v, o, o, o);
+    ctr(e, 1.0, "acik,cj->aikj", V, d1, 0.0, dP, prec);
+    axpby(e, v * o * o * o, 1.0, A("P"), 0.0, Pnb.get());
+    axpby(e, v * o * o * o, 1.0, dPb.get(), 1.0, Pnb.get());
+    ctr(e, -1.0, "bk,aikj->abij", d1, Pn, 1.0, dR2, prec);
+    ctr(e, -1.0, "bk,aikj->abij", T1A, dP, 1.0, dR2, prec);
+  }
+  {
+    // Xakic: (dressed Gamma_ai) Gamma_ck + V (Y, T2) terms; then Xakic Y2
+    DBuf ddGb(&e.pool, G * v * o);
+    View ddG = V3(ddGb.get(), G, v, o);
+    ctr(e, -1.0, "Gil,al->Gai", Gij, d1, 0.0, ddG, prec);
+    ctr(e, 1.0, "Gad,di->Gai", Gab, d1, 1.0, ddG, prec);
+    DBuf dXkb(&e.pool, v * o * o * v);
+    View dXakic = V4(dXkb.get(), v, o, o, v);
+    ctr(e, 1.0, "Gai,Gck->akic", ddG, Gai, 0.0, dXakic, prec);
+    DBuf dYb(&e.pool, n2), YAb(&e.pool, n2);
+    add_t1t1(e, dYb.get(), T2p, T1p, 2.0, int(v), int(o));
+    add_t1t1(e, YAb.get(), an.T2.get(), an.T1.get(), 2.0, int(v), int(o));
+    axpby(e, n2, -1.0, YAb.get(), 1.0, dYb.get());
+    YAb.reset();
+    ctr(e, -0.5, "dclk,dail->akic", V, V4(dYb.get(), v, v, o, o), 1.0, dXakic, prec);
+    ctr(e, 1.0, "dclk,adil->akic", V, d2, 1.0, dXakic, prec);
+    ctr(e, -0.5, "cdlk,adil->akic", V, d2, 1.0, dXakic, prec);
+    // Y2 = 2 T2 - T2^(bc) at T and its change
+    DBuf Y2b(&e.pool, n2);
+    View Y2 = V4(Y2b.get(), v, v, o, o), dY2 = V4(dYb.get(), v, v, o, o);
+    axpby(e, n2, 2.0, T2p, 0.0, Y2b.get());
+    sum(e, -1.0, T2, "bckj", 1.0, Y2, "cbkj");
+    axpby(e, n2, 2.0, d2b.get(), 0.0, dYb.get());
+    sum(e, -1.0, d2, "bckj", 1.0, dY2, "cbkj");
+    ctr(e, 1.0, "akic,cbkj->abij", dXakic, Y2, 1.0, dR2, prec);
+    ctr(e, 1.0, "akic,cbkj->abij", V4(A("Xakic"), v, o, o, v), dY2, 1.0, dR2, prec);
+  }
+  {
+    // Xakci = (dressed Gamma_ab)(dressed Gamma_ij) - V T2 / 2; then - Xakci T2 (two terms)
+    DBuf ddGab(&e.pool, G * v * v), ddGij(&e.pool, G * o * o), Gijn(&e.pool, G * o * o);
+    ctr(e, -1.0, "Gcl,al->Gac", Gai, d1, 0.0, V3(ddGab.get(), G, v, v), prec);
+    ctr(e, 1.0, "Gdk,di->Gki", Gai, d1, 0.0, V3(ddGij.get(), G, o, o), prec);
+    axpby(e, G * o * o, 1.0, A("dGij4"), 0.0, Gijn.get());
+    axpby(e, G * o * o, 1.0, ddGij.get(), 1.0, Gijn.get());
+    DBuf dXcb(&e.pool, v * o * v * o);
+    View dXakci = V4(dXcb.get(), v, o, v, o);
+    ctr(e, 1.0, "Gac,Gki->akci", V3(ddGab.get(), G, v, v), V3(Gijn.get(), G, o, o), 0.0, dXakci,
+        prec);
+    ctr(e, 1.0, "Gac,Gki->akci", V3(A("dGab4"), G, v, v), V3(ddGij.get(), G, o, o), 1.0, dXakci,
+        prec);
+    ctr(e, -0.5, "cdlk,dail->akci", V, d2, 1.0, dXakci, prec);
+    View XakciA = V4(A("Xakci"), v, o, v, o);
+    ctr(e, -1.0, "akci,cbkj->abij", dXakci, T2, 1.0, dR2, prec);
+    ctr(e, -1.0, "akci,cbkj->abij", XakciA, d2, 1.0, dR2, prec);
+    ctr(e, -1.0, "bkci,ackj->abij", dXakci, T2, 1.0, dR2, prec);
+    ctr(e, -1.0, "bkci,ackj->abij", XakciA, d2, 1.0, dR2, prec);
+  }
+  {
+    DBuf Rt(&e.pool, n2);
+    copy(e, V4(Rt.get(), v, v, o, o), dR2);
+    sum(e, 1.0, V4(Rt.get(), v, v, o, o), "baji", 1.0, dR2, "abij");
+  }
+  {
+    // Xklij (hole ladder): (Vijkl + Vijka T1) X and (V X) T2
+    DBuf dXk(&e.pool, o * o * o * o);
+    View dXklij = V4(dXk.get(), o, o, o, o);
+    ctr(e, 1.0, "klic,cj->klij", Vijka, d1, 0.0, dXklij, prec);
+    ctr(e, 1.0, "lkjc,ci->klij", Vijka, d1, 1.0, dXklij, prec);
+    ctr(e, 1.0, "klij,abkl->abij", dXklij, Xn, 1.0, dR2, prec);
+    ctr(e, 1.0, "klij,abkl->abij", V4(A("Xklij1"), o, o, o, o), dX, 1.0, dR2, prec);
+    ctr(e, 1.0, "cdkl,cdij->klij", V, dX, 0.0, dXklij, prec);
+    ctr(e, 1.0, "klij,abkl->abij", dXklij, T2, 1.0, dR2, prec);
+    ctr(e, 1.0, "klij,abkl->abij", V4(A("Xklij2"), o, o, o, o), d2, 1.0, dR2, prec);
+  }
+  {
+    // ladder: V~(T) X(T) - V~(T_A) X(T_A) = dV~ X(T) + V~(T_A) dX per slice pair,
+    // dV~ = dG[n]^T G(T)[m] + G_A[n]^T dG[m] for the dressed Gamma_ab
+    if (slice_size <= 0) slice_size = int(o);
+    DBuf ddGb(&e.pool, G * v * v), Gnb(&e.pool, G * v * v);
+    View ddG = V3(ddGb.get(), G, v, v);
+    ctr(e, -1.0, "Gbk,ak->Gab", Gai, d1, 0.0, ddG, prec);
+    axpby(e, G * v * v, 1.0, A("dGabP"), 0.0, Gnb.get());
+    axpby(e, G * v * v, 1.0, ddGb.get(), 1.0, Gnb.get());
+    View GA = V3(A("dGabP"), G, v, v), Gn = V3(Gnb.get(), G, v, v);
+    const int nslices = int((v + slice_size - 1) / slice_size);
+    std::vector<DBuf> sA(nslices), sN(nslices), sD(nslices);
+    std::vector<int64_t> lo(nslices), hi(nslices);
+    for (int s = 0; s < nslices; ++s) {
+      lo[s] = int64_t(s) * slice_size;
+      hi[s] = std::min<int64_t>(lo[s] + slice_size, v);
+      const int64_t n = G * (hi[s] - lo[s]) * v;
+      sA[s] = DBuf(&e.pool, n);
+      sN[s] = DBuf(&e.pool, n);
+      sD[s] = DBuf(&e.pool, n);
+      copy(e, V3(sA[s].get(), G, hi[s] - lo[s], v), subview(GA, {0, lo[s], 0}, {G, hi[s], v}));
+      copy(e, V3(sN[s].get(), G, hi[s] - lo[s], v), subview(Gn, {0, lo[s], 0}, {G, hi[s], v}));
+      copy(e, V3(sD[s].get(), G, hi[s] - lo[s], v), subview(ddG, {0, lo[s], 0}, {G, hi[s], v}));
+      e.frozen.push_back(sA[s].get());
+      e.frozen.push_back(sN[s].get());
+      e.frozen.push_back(sD[s].get());
+    }
+    const int64_t smax = std::min<int64_t>(slice_size, v);
+    DBuf VAb(&e.pool, smax * smax * v * v), dVb(&e.pool, smax * smax * v * v),
+        Rxy(&e.pool, smax * smax * o * o);
+    const int pl = e.prec_ladder;
+    for (int m = 0; m < nslices; ++m)
+      for (int n = m; n < nslices; ++n) {
+        const int64_t Nx = hi[n] - lo[n], Ny = hi[m] - lo[m];
+        View VA = V4(VAb.get(), Nx, Ny, v, v), dV = V4(dVb.get(), Nx, Ny, v, v);
+        View Rxyij = V4(Rxy.get(), Nx, Ny, o, o);
+        ctr(e, 1.0, "Gxc,Gyd->xycd", V3(sA[n].get(), G, Nx, v), V3(sA[m].get(), G, Ny, v), 0.0,
+            VA, pl);
+        ctr(e, 1.0, "Gxc,Gyd->xycd", V3(sD[n].get(), G, Nx, v), V3(sN[m].get(), G, Ny, v), 0.0,
+            dV, pl);
+        ctr(e, 1.0, "Gxc,Gyd->xycd", V3(sA[n].get(), G, Nx, v), V3(sD[m].get(), G, Ny, v), 1.0,
+            dV, pl);
+        ctr(e, 1.0, "xycd,cdij->xyij", dV, Xn, 0.0, Rxyij, pl);
+        ctr(e, 1.0, "xycd,cdij->xyij", VA, dX, 1.0, Rxyij, pl);
+        const int64_t a = lo[n], b = lo[m];
+        copy(e, subview(dR2, {a, b, 0, 0}, {a + Nx, b + Ny, o, o}), Rxyij, 1.0, 1.0);
+        if (a > b)
+          sum(e, 1.0, Rxyij, "xyij", 1.0, subview(dR2, {b, a, 0, 0}, {b + Ny, a + Nx, o, o}),
+              "yxji");
+      }
+    for (int s2 = 0; s2 < nslices; ++s2) {  // slices return to the pool at block end
+      e.unfreeze(sA[s2].get());
+      e.unfreeze(sN[s2].get());
+      e.unfreeze(sD[s2].get());
+    }
+  }
+  {
+    // T1 equations
+    ctr(e, 1.0, "ac,ci->ai", dKac, T1, 1.0, dR1, prec);
+    ctr(e, 1.0, "ac,ci->ai", V2(A("Kac"), v, v), d1, 1.0, dR1, prec);
+    ctr(e, -1.0, "ki,ak->ai", dKki, T1, 1.0, dR1, prec);
+    ctr(e, -1.0, "ki,ak->ai", V2(A("Kki"), o, o), d1, 1.0, dR1, prec);
+    DBuf dKckb(&e.pool, v * o), Kcknb(&e.pool, v * o), dSb(&e.pool, o * o), Snb(&e.pool, o * o);
+    View dKck = V2(dKckb.get(), v, o), Kckn = V2(Kcknb.get(), v, o), KckA = V2(A("Kck"), v, o);
+    ctr(e, 1.0, "cdkl,dl->ck", U, d1, 0.0, dKck, prec);
+    axpby(e, v * o, 1.0, A("Kck"), 0.0, Kcknb.get());
+    axpby(e, v * o, 1.0, dKckb.get(), 1.0, Kcknb.get());
+    ctr(e, 2.0, "ck,caki->ai", dKck, T2, 1.0, dR1, prec);
+    ctr(e, 2.0, "ck,caki->ai", KckA, d2, 1.0, dR1, prec);
+    ctr(e, -1.0, "ck,caik->ai", dKck, T2, 1.0, dR1, prec);
+    ctr(e, -1.0, "ck,caik->ai", KckA, d2, 1.0, dR1, prec);
+    View dS = V2(dSb.get(), o, o), Sn = V2(Snb.get(), o, o);
+    ctr(e, 1.0, "ck,ci->ki", dKck, T1, 0.0, dS, prec);
+    ctr(e, 1.0, "ck,ci->ki", KckA, d1, 1.0, dS, prec);
+    axpby(e, o * o, 1.0, A("Ski"), 0.0, Snb.get());
+    axpby(e, o * o, 1.0, dSb.get(), 1.0, Snb.get());
+    ctr(e, 1.0, "ak,ki->ai", d1, Sn, 1.0, dR1, prec);
+    ctr(e, 1.0, "ak,ki->ai", T1A, dS, 1.0, dR1, prec);
+    ctr(e, 2.0, "acik,ck->ai", V, d1, 1.0, dR1, prec);
+    ctr(e, -1.0, "ciak,ck->ai", Vaibj, d1, 1.0, dR1, prec);
+    DBuf Ygb(&e.pool, G * v * o);
+    View Yg = V3(Ygb.get(), G, v, o);
+    ctr(e, 1.0, "Gdk,cdik->Gci", Gai, dX, 0.0, Yg, prec);
+    ctr(e, 2.0, "Gca,Gci->ai", Gab, Yg, 1.0, dR1, prec);
+    ctr(e, 1.0, "Gck,cdik->Gdi", Gai, dX, 0.0, Yg, prec);
+    ctr(e, -1.0, "Gda,Gdi->ai", Gab, Yg, 1.0, dR1, prec);
+    ctr(e, -2.0, "klic,ackl->ai", Vijka, dX, 1.0, dR1, prec);
+    ctr(e, 1.0, "lkic,ackl->ai", Vijka, dX, 1.0, dR1, prec);
+  }
+  e.thaw();
+  // R(T) = R(T_A) + dR (FP64)
+  axpby(e, n2, 1.0, an.R2.get(), 0.0, R2p);
+  axpby(e, n2, 1.0, dR2b.get(), 1.0, R2p);
+  axpby(e, n1, 1.0, an.R1.get(), 0.0, R1p);
+  axpby(e, n1, 1.0, dR1b.get(), 1.0, R1p);
+}
+
+// FP64 residual at T that also becomes the new refinement anchor.
+void ccsd_residual_anchor(Engine &e, int slice_size, const double *T1, const double *T2,
+                          double *R1, double *R2) {
+  const int64_t n1 = int64_t(e.n1()), n2 = int64_t(e.n2());
+  const int p = e.precision, pl = e.prec_ladder;
+  e.precision = SISI_FP64;
+  e.prec_ladder = SISI_FP64;
+  e.anchor.valid = false;
+  e.recording = true;
+  try {
+    ccsd_residual(e, 1, true, slice_size, T1, T2, R1, R2);
+  } catch (...) {
+    e.recording = false;
+    e.precision = p;
+    e.prec_ladder = pl;
+    throw;
+  }
+  e.recording = false;
+  e.precision = p;
+  e.prec_ladder = pl;
+  Anchor &an = e.anchor;
+  if (an.T1.size() != size_t(n1)) an.T1 = DBuf(&e.pool, n1);
+  if (an.T2.size() != size_t(n2)) an.T2 = DBuf(&e.pool, n2);
+  if (an.R1.size() != size_t(n1)) an.R1 = DBuf(&e.pool, n1);
+  if (an.R2.size() != size_t(n2)) an.R2 = DBuf(&e.pool, n2);
+  axpby(e, n1, 1.0, T1, 0.0, an.T1.get());
+  axpby(e, n2, 1.0, T2, 0.0, an.T2.get());
+  axpby(e, n1, 1.0, R1, 0.0, an.R1.get());
+  axpby(e, n2, 1.0, R2, 0.0, an.R2.get());
+  an.valid = true;
+  an.age = 0;
 }
 
 void ccsd_energy(Engine &e, const double *T1, const double *T2, double &E, double &direct,
@@ -318,6 +621,8 @@ void ccsd_begin(Engine &e, const sisi_ccsd_opts &opts, const double *T1h, const 
   e.lin_have = false;
   e.lin1.reset();
   e.lin2.reset();
+  e.anchor = Anchor();
+  SISI_REQUIRE(opts.refine_interval >= 0, SISI_ERR_INVALID, "refine_interval must be >= 0");
   if (opts.mixer == SISI_MIXER_DIIS) {
     const int N = opts.max_residua;
     e.diis.resize(N);
@@ -334,12 +639,28 @@ void ccsd_iterate(Engine &e, int n) {
   for (int step = 0; step < n && !e.converged && e.iteration < o.max_iterations; ++step) {
     // precision schedule: bulk precision until |dT|/|T| <= switch_threshold,
     // FP64 afterwards (and FP64 throughout for the plain path)
-    const int prec = (o.precision != SISI_FP64 && e.cur_precision != SISI_FP64) ? o.precision
-                                                                                : SISI_FP64;
-    set_precision(e, prec);
+    const bool bulk = o.precision != SISI_FP64 && e.cur_precision != SISI_FP64;
+    const bool refine = !bulk && o.precision != SISI_FP64 && o.refine_interval > 0;
+    int prec = bulk ? o.precision : SISI_FP64;
     DBuf R1(&e.pool, e.n1()), R2(&e.pool, e.n2());
-    ccsd_residual(e, e.iteration, e.run_initial, o.slice_size, e.T1.get(), e.T2.get(), R1.get(),
-                  R2.get());
+    if (refine) {
+      // refinement phase: FP64 anchor every refine_interval iterations,
+      // reduced-precision refinement residuals in between
+      if (!e.anchor.valid || e.anchor.age >= o.refine_interval) {
+        set_precision(e, SISI_FP64);
+        ccsd_residual_anchor(e, o.slice_size, e.T1.get(), e.T2.get(), R1.get(), R2.get());
+        prec = SISI_FP64;
+      } else {
+        set_precision(e, o.precision);
+        ccsd_residual_delta(e, o.slice_size, e.T1.get(), e.T2.get(), R1.get(), R2.get());
+        e.anchor.age++;
+        prec = o.precision;
+      }
+    } else {
+      set_precision(e, prec);
+      ccsd_residual(e, e.iteration, e.run_initial, o.slice_size, e.T1.get(), e.T2.get(), R1.get(),
+                    R2.get());
+    }
     DBuf d1(&e.pool, e.n1()), d2(&e.pool, e.n2());
     estimate_amplitudes(e, 2, R1.get(), e.T1.get(), d1.get(), o.level_shift);
     estimate_amplitudes(e, 4, R2.get(), e.T2.get(), d2.get(), o.level_shift);
@@ -414,11 +735,15 @@ void ccsd_iterate(Engine &e, int n) {
     // hand over to FP64 once the bulk iterations reached the threshold
     if (prec != SISI_FP64 && e.last_delta <= o.switch_threshold && e.iteration > 1)
       e.cur_precision = SISI_FP64;
-    // convergence test (ClusterSinglesDoublesAlgorithm.cxx:103-106), FP64 only
+    // convergence test (ClusterSinglesDoublesAlgorithm.cxx:103-106), declared on
+    // FP64 residuals only; a refinement iteration that meets it asks for an
+    // FP64 anchor next, which then confirms it
     const double de = std::fabs((E - e.prev_energy) / E);
-    if (prec == SISI_FP64 && de < std::fabs(o.energy_convergence) &&
-        e.last_delta * e.last_delta < std::fabs(o.amplitudes_convergence * o.amplitudes_convergence))
-      e.converged = true;
+    const bool met = de < std::fabs(o.energy_convergence) &&
+                     e.last_delta * e.last_delta <
+                         std::fabs(o.amplitudes_convergence * o.amplitudes_convergence);
+    if (met && prec == SISI_FP64) e.converged = true;
+    else if (met && refine) e.anchor.age = o.refine_interval;
     e.prev_energy = E;
   }
 }

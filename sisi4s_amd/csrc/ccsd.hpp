@@ -14,6 +14,13 @@ void integrals_from_vertex(Engine &e);
 void ccsd_residual(Engine &e, int iteration, bool initial_doubles, int slice_size,
                    const double *T1, const double *T2, double *R1, double *R2);
 
+// FP64 residual that records itself (and its intermediates) as the
+// refinement anchor; and the residual at T by refinement around it.
+void ccsd_residual_anchor(Engine &e, int slice_size, const double *T1, const double *T2,
+                          double *R1, double *R2);
+void ccsd_residual_delta(Engine &e, int slice_size, const double *T1, const double *T2, double *R1,
+                         double *R2);
+
 // getEnergy (ClusterSinglesDoublesAlgorithm.cxx:131-213).
 void ccsd_energy(Engine &e, const double *T1, const double *T2, double &E, double &direct,
                  double &exchange);

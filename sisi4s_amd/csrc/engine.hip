@@ -100,6 +100,7 @@ Engine::~Engine() {
   // device buffers return to the pool before it is released
   T1.reset();
   T2.reset();
+  anchor = Anchor();
   diis.clear();
   lin1.reset();
   lin2.reset();
@@ -144,5 +145,12 @@ void Engine::download(double *dst, const DBuf &src, size_t n) {
 }
 
 void Engine::sync() { SISI_HIP(hipStreamSynchronize(stream)); }
+
+void Engine::record(const std::string &name, const double *p, size_t n) {
+  if (!recording) return;
+  DBuf &b = anchor.v[name];
+  if (b.size() != n) b = DBuf(&pool, n);
+  SISI_HIP(hipMemcpyAsync(b.get(), p, n * sizeof(double), hipMemcpyDeviceToDevice, stream));
+}
 
 }  // namespace sisi

@@ -55,6 +55,15 @@ class Timed {
   Timers &t_;
 };
 
+// FP64 evaluation point of the refinement iterations: amplitudes, residual
+// and the T-dependent intermediates of that residual.
+struct Anchor {
+  bool valid = false;
+  int age = 0;  // refinement iterations since the anchor was set
+  DBuf T1, T2, R1, R2;
+  std::map<std::string, DBuf> v;
+};
+
 struct DiisSlot {
   DBuf t1, t2;   // estimated amplitudes
   DBuf d1, d2;   // amplitude change
@@ -119,6 +128,11 @@ class Engine {
   // (T) integrals, built on first use: V_bdck as [d,b,c,k] and as [d,c,b,k],
   // V_jklc as [l,c,j,k]
   DBuf Vt, Vt2, VP;
+
+  // refinement anchor (recorded by an FP64 residual when `recording` is set)
+  Anchor anchor;
+  bool recording = false;
+  void record(const std::string &name, const double *p, size_t n);
 
   // CCSD run state
   sisi_ccsd_opts opts{};

@@ -42,7 +42,7 @@ class CcsdOpts(ctypes.Structure):
                 ("amplitudes_convergence", ctypes.c_double),
                 ("energy_convergence", ctypes.c_double), ("level_shift", ctypes.c_double),
                 ("slice_size", ctypes.c_int), ("precision", ctypes.c_int),
-                ("switch_threshold", ctypes.c_double)]
+                ("switch_threshold", ctypes.c_double), ("refine_interval", ctypes.c_int)]
 
 
 class CcsdReport(ctypes.Structure):
@@ -87,6 +87,10 @@ SIGNATURES = [
     ("sisi_ccsd_residual", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_double_p, _c_double_p,
       _c_double_p, _c_double_p]),
+    ("sisi_ccsd_residual_anchor", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_int, _c_double_p, _c_double_p, _c_double_p, _c_double_p]),
+    ("sisi_ccsd_residual_refine", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_int, _c_double_p, _c_double_p, _c_double_p, _c_double_p]),
     ("sisi_ccsd_energy", ctypes.c_int, [ctypes.c_void_p, _c_double_p, _c_double_p, _c_double_p]),
     ("sisi_mp2_energy", ctypes.c_int, [ctypes.c_void_p, _c_double_p]),
     ("sisi_ccsd_begin", ctypes.c_int,
@@ -295,6 +299,20 @@ class Engine:
         _check(self._lib.sisi_ccsd_residual(self._h, iteration, int(initial_doubles),
                                             int(slice_size), _ptr(t1), _ptr(t2), _ptr(r1),
                                             _ptr(r2)))
+        return _unbuf(r1, (v, o)), _unbuf(r2, (v, v, o, o))
+
+    def residual_anchor(self, T1, T2, slice_size=-1):
+        return self._residual2(self._lib.sisi_ccsd_residual_anchor, T1, T2, slice_size)
+
+    def residual_refine(self, T1, T2, slice_size=-1):
+        return self._residual2(self._lib.sisi_ccsd_residual_refine, T1, T2, slice_size)
+
+    def _residual2(self, fn, T1, T2, slice_size):
+        v, o = self.Nv, self.No
+        t1, t2 = _buf(T1), _buf(T2)
+        r1 = np.empty(v * o)
+        r2 = np.empty(v * v * o * o)
+        _check(fn(self._h, int(slice_size), _ptr(t1), _ptr(t2), _ptr(r1), _ptr(r2)))
         return _unbuf(r1, (v, o)), _unbuf(r2, (v, v, o, o))
 
     def energy(self, T1, T2):

@@ -218,3 +218,57 @@ def test_batched_reports_failure_of_one_system():
         engine.ccsd_solve_batched([(g, ei, ea), (g, ei, bad)],
                                   engine.default_opts(max_iterations=2), workers=2)
     assert "system 1" in str(ex.value)
+
+
+def test_refinement_residual_is_exact_in_fp64():
+    """Anchor at T_A, refinement at T_A + d (FP64): equals the plain residual
+    at T_A + d (the expansion is exact algebra)."""
+    No, Nv, NG = 5, 17, 40
+    e, g, ei, ea = make_engine(61, No, Nv, NG)
+    TA1, TA2 = random_amplitudes(Nv, No, seed=21)
+    D1, D2 = random_amplitudes(Nv, No, seed=22, scale=1e-3)
+    announce("refinement residual fp64 exactness")
+    e.residual_anchor(TA1, TA2, slice_size=4)
+    e.set_precision("fp64")
+    R1, R2 = e.residual_refine(TA1 + D1, TA2 + D2, slice_size=4)
+    P1, P2 = e.residual(1, TA1 + D1, TA2 + D2, slice_size=4)
+    scale = float(np.abs(P2).max())
+    e2, e1 = float(np.abs(R2 - P2).max()), float(np.abs(R1 - P1).max())
+    assert e2 <= 1e-12 * scale and e1 <= 1e-12 * scale, (e1, e2, scale)
+
+
+def test_refinement_residual_reduced_precision_error_scales_with_change():
+    No, Nv, NG = 10, 50, 300
+    e, g, ei, ea = make_engine(20261202, No, Nv, NG)
+    TA1, TA2 = random_amplitudes(Nv, No, seed=2, scale=0.02)
+    D1, D2 = random_amplitudes(Nv, No, seed=3, scale=2e-6)
+    announce("refinement residual bf16x3 error")
+    P1, P2 = e.residual(1, TA1 + D1, TA2 + D2)          # FP64 reference
+    e.residual_anchor(TA1, TA2)
+    e.set_precision("bf16x3")
+    R1, R2 = e.residual_refine(TA1 + D1, TA2 + D2)
+    Q1, Q2 = e.residual(1, TA1 + D1, TA2 + D2)          # plain bf16x3
+    e.set_precision("fp64")
+    scale = float(np.abs(P2).max())
+    err_ref, err_plain = float(np.abs(R2 - P2).max()), float(np.abs(Q2 - P2).max())
+    print("refinement err %.2e, plain bf16x3 err %.2e (rel. to |R| %.2e)"
+          % (err_ref / scale, err_plain / scale, scale), flush=True)
+    assert err_ref <= 1e-3 * err_plain and err_ref <= 1e-9 * scale
+
+
+def test_refined_mixed_solve_matches_fp64():
+    from sisi4s_amd import engine
+    No, Nv, NG = 5, 20, 80
+    e, g, ei, ea = make_engine(7, No, Nv, NG)
+    kw = dict(mixer="DiisMixer", max_residua=6, max_iterations=120,
+              amplitudes_convergence=1e-8, energy_convergence=1e-10)
+    announce("fp64 vs refined mixed solve")
+    r64 = e.ccsd_solve(engine.default_opts(**kw))
+    rr = e.ccsd_solve(engine.default_opts(precision="bf16x3", switch_threshold=1e-5,
+                                          refine_interval=8, **kw))
+    print("refined mixed: %d iterations (%d fp64) vs fp64 %d, |dE| %.2e"
+          % (rr["iterations"], rr["fp64_iterations"], r64["iterations"],
+             abs(rr["energy"] - r64["energy"])), flush=True)
+    assert r64["converged"] and rr["converged"]
+    assert rr["fp64_iterations"] <= rr["iterations"] // 4 + 2
+    assert abs(rr["energy"] - r64["energy"]) <= 1e-6
