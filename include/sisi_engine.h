@@ -225,6 +225,17 @@ int sisi_engine_reset_stats(sisi_engine *e);
 int sisi_generate_system(uint64_t seed, int No, int Nv, int NG, double *gamma,
                          double *eps_i, double *eps_a);
 
+/* Uniform electron gas (host, no device needed): UegVertexGenerator
+ * (UegVertexGenerator.cxx:49-229; arguments No, Nv, rs, hartreeFock,
+ * madelung < 0 to evaluate it) with the plane-wave vertex rewritten in real
+ * orbitals, cos/sin pairs of (k, -k), and real auxiliary functions, cos/sin
+ * pairs of (G, -G), so that Gamma[G,p,q] == Gamma[G,q,p] as the real path
+ * requires; correlation energies equal the plane-wave ones.  Needs closed
+ * shells at No and No+Nv.  Call with gamma == NULL to get *NG; then pass
+ * gamma of NG*Np*Np doubles with *NG holding that count. */
+int sisi_generate_ueg(int No, int Nv, double rs, int hartree_fock, double madelung, int *NG,
+                      double *gamma, double *eps_i, double *eps_a);
+
 /* One MFMA GEMM C = alpha*op(A)*op(B) + beta*C on host arrays (column-major,
  * op = transpose when trans* != 0), in the given precision.  No counterpart
  * in the reference (CTF calls BLAS internally); exposed so the kernel family

@@ -12,6 +12,9 @@
 #include "tensor_ops.hpp"
 #include "triples.hpp"
 
+extern "C" int sisi_generate_ueg_impl(int No, int Nv, double rs, int hartree_fock, double madelung,
+                                      int *NG, double *gamma, double *eps_i, double *eps_a,
+                                      const char **error);
 extern "C" int sisi_generate_system_impl(uint64_t seed, int No, int Nv, int NG, double *gamma,
                                          double *eps_i, double *eps_a);
 
@@ -419,6 +422,20 @@ int sisi_generate_system(uint64_t seed, int No, int Nv, int NG, double *gamma, d
                          double *eps_a) {
   const int rc = sisi_generate_system_impl(seed, No, Nv, NG, gamma, eps_i, eps_a);
   g_last_error = rc == SISI_OK ? "" : "generate_system: No, Nv, NG must be positive";
+  return rc;
+}
+
+int sisi_generate_ueg(int No, int Nv, double rs, int hartree_fock, double madelung, int *NG,
+                      double *gamma, double *eps_i, double *eps_a) {
+  const char *err = "";
+  int rc;
+  try {
+    rc = sisi_generate_ueg_impl(No, Nv, rs, hartree_fock, madelung, NG, gamma, eps_i, eps_a, &err);
+  } catch (const std::exception &ex) {
+    g_last_error = std::string("generate_ueg: ") + ex.what();
+    return SISI_ERR_ALLOC;
+  }
+  g_last_error = err;
   return rc;
 }
 

@@ -116,6 +116,9 @@ SIGNATURES = [
     ("sisi_generate_system", ctypes.c_int,
      [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_double_p, _c_double_p,
       _c_double_p]),
+    ("sisi_generate_ueg", ctypes.c_int,
+     [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double, _c_int_p,
+      _c_double_p, _c_double_p, _c_double_p]),
     ("sisi_gemm_host", ctypes.c_int,
      [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
       ctypes.c_double, _c_double_p, ctypes.c_int, _c_double_p, ctypes.c_int, ctypes.c_double,
@@ -185,6 +188,22 @@ def generate_system(seed, No, Nv, NG):
     ea = np.empty(Nv)
     _check(load().sisi_generate_system(int(seed), No, Nv, NG, _ptr(g), _ptr(ei), _ptr(ea)))
     return _unbuf(g, (NG, Np, Np)), ei, ea
+
+
+def generate_ueg(No, Nv, rs, hartree_fock=True, madelung=-1.0):
+    """Uniform electron gas (Gamma[G,p,q] in real orbitals, eps_i, eps_a)
+    from the engine's host generator (no device needed)."""
+    lib = load()
+    ng = ctypes.c_int(0)
+    _check(lib.sisi_generate_ueg(No, Nv, float(rs), int(bool(hartree_fock)), float(madelung),
+                                 ctypes.byref(ng), None, None, None))
+    Np = No + Nv
+    g = np.empty(ng.value * Np * Np)
+    ei = np.empty(No)
+    ea = np.empty(Nv)
+    _check(lib.sisi_generate_ueg(No, Nv, float(rs), int(bool(hartree_fock)), float(madelung),
+                                 ctypes.byref(ng), _ptr(g), _ptr(ei), _ptr(ea)))
+    return _unbuf(g, (ng.value, Np, Np)), ei, ea
 
 
 def gemm(A, B, C=None, alpha=1.0, beta=0.0, transA=False, transB=False, precision=FP64):

@@ -62,6 +62,38 @@ class GenerateSyntheticSystem : public Algorithm {
   }
 };
 
+// UegVertexGenerator (UegVertexGenerator.cxx:49-229): same arguments and
+// outputs; the vertex is written in real orbitals so that the real path
+// applies (correlation energies equal the plane-wave ones).  NF must be left
+// at its default (the reference's other values only serve profiling, cxx:162-163).
+class UegVertexGenerator : public Algorithm {
+ public:
+  using Algorithm::Algorithm;
+  std::string getName() override { return "UegVertexGenerator"; }
+  void run() override {
+    const int No = int(getIntegerArgument("No")), Nv = int(getIntegerArgument("Nv"));
+    const double rs = getRealArgument("rs");
+    const int hf = int(getIntegerArgument("hartreeFock", 1));
+    const double madelung = getRealArgument("madelung", -1.0);
+    const int64_t nf = getIntegerArgument("NF", 0);
+    int NG = 0;
+    check(sisi_generate_ueg(No, Nv, rs, hf, madelung, &NG, nullptr, nullptr, nullptr),
+          "sisi_generate_ueg");
+    if (nf != 0 && nf != NG)
+      throw StepError("UegVertexGenerator: NF other than the transfer count (" +
+                      std::to_string(NG) + ") is not supported");
+    auto g = make_tensor({NG, No + Nv, No + Nv});
+    auto ei = make_tensor({No}), ea = make_tensor({Nv});
+    check(sisi_generate_ueg(No, Nv, rs, hf, madelung, &NG, g->values.data(), ei->values.data(),
+                            ea->values.data()),
+          "sisi_generate_ueg");
+    allocatedTensorArgument("CoulombVertex", g);
+    allocatedTensorArgument("HoleEigenEnergies", ei);
+    allocatedTensorArgument("ParticleEigenEnergies", ea);
+    emitted.scalars.push_back({"NF", std::to_string(NG)});
+  }
+};
+
 // CoulombIntegralsFromVertex (CoulombIntegralsFromVertex.cxx:23-148, real
 // integrals 203-571): publishes the requested classes.
 class CoulombIntegralsFromVertex : public Algorithm {
@@ -199,6 +231,7 @@ class CcsdPerturbativeTriples : public Algorithm {
 
 static Registrar<GenerateSyntheticSystem> r0("GenerateSyntheticSystem");
 static Registrar<CoulombIntegralsFromVertex> r1("CoulombIntegralsFromVertex");
+static Registrar<UegVertexGenerator> r4("UegVertexGenerator");
 static Registrar<CcsdEnergyFromCoulombIntegrals> r2("CcsdEnergyFromCoulombIntegrals");
 static Registrar<CcsdPerturbativeTriples> r3("CcsdPerturbativeTriples");
 

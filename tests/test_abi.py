@@ -7,6 +7,7 @@ import os
 import re
 
 import numpy as np
+import pytest
 
 from conftest import ROOT
 from oracle import philox
@@ -39,6 +40,26 @@ def test_host_generator_matches_oracle(lib):
         np.testing.assert_array_equal(ei, ei2)
         np.testing.assert_array_equal(ea, ea2)
         assert np.abs(g - g2).max() <= 4e-16 * np.abs(g2).max()
+
+
+def test_host_ueg_generator_matches_oracle(lib):
+    from oracle import ueg
+    from sisi4s_amd import engine
+    for No, Nv, rs in [(7, 26, 1.0), (7, 12, 2.0), (1, 18, 0.5)]:
+        g, ei, ea = engine.generate_ueg(No, Nv, rs)
+        g2, ei2, ea2, _ = ueg.system(No, Nv, rs)
+        assert g.shape == g2.shape
+        np.testing.assert_allclose(ei, ei2, rtol=0, atol=4e-15)
+        np.testing.assert_allclose(ea, ea2, rtol=0, atol=4e-15)
+        assert np.abs(g - g2).max() <= 4e-15
+    # open shells cannot be written in real orbitals; a wrong buffer size is refused
+    with pytest.raises(engine.EngineError, match="closed shells"):
+        engine.generate_ueg(5, 28, 1.0)
+    ng = ctypes.c_int(3)
+    buf = np.zeros(3 * 33 * 33)
+    rc = lib.sisi_generate_ueg(7, 26, 1.0, 1, -1.0, ctypes.byref(ng),
+                               buf.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), None, None)
+    assert rc == engine.SISI_ERR_INVALID and b"NG = 3" in lib.sisi_last_error()
 
 
 def test_invalid_arguments_return_status(lib):

@@ -272,3 +272,25 @@ def test_refined_mixed_solve_matches_fp64():
     assert r64["converged"] and rr["converged"]
     assert rr["fp64_iterations"] <= rr["iterations"] // 4 + 2
     assert abs(rr["energy"] - r64["energy"]) <= 1e-6
+
+
+@pytest.mark.gpu
+def test_ueg_known_answers():
+    """The reference's own known answers for the uniform electron gas, rs=1,
+    No=7, Nv=26 (integration-tests/tests/cc4s/ueg/rs1.0-7occ-26virt/
+    cc4s.correct.out.yaml:128 MP2, :153 CCSD, :169 (T)), with the vertex from
+    the engine's generator: DIIS(4), ten iterations as the reference ran."""
+    from sisi4s_amd import engine
+    announce("ueg rs1 No7 Nv26 known answers")
+    g, ei, ea = engine.generate_ueg(7, 26, 1.0)
+    e = engine.Engine(7, 26, g.shape[0])
+    try:
+        e.set_system(g, ei, ea)
+        assert abs(e.mp2_energy() - -0.36143028565969504) <= 1e-13
+        rep = e.ccsd_solve(engine.default_opts(mixer="DiisMixer", max_residua=4, max_iterations=10,
+                                               amplitudes_convergence=1e-8, energy_convergence=1e-8))
+        assert rep.iterations == 10
+        assert abs(rep.energy - -0.39269658954585018) <= 1e-12
+        assert abs(e.triples() - -0.0063019625641725016) <= 1e-14
+    finally:
+        e.close()

@@ -2,6 +2,7 @@
 the CPU; end-to-end config-1 CCSD + (T) on the GPU against the oracle."""
 
 import os
+import re
 import subprocess
 
 import numpy as np
@@ -11,6 +12,7 @@ from conftest import ROOT, announce
 
 RUNNER = os.path.join(ROOT, "sisi4s_amd", "sisi4s-mi355x")
 INPUT = os.path.join(ROOT, "tests", "inputs", "config1_ccsd_t.yaml")
+UEG_INPUT = os.path.join(ROOT, "tests", "inputs", "ueg_rs1_no7_nv26.yaml")
 
 
 def run(args, **kw):
@@ -22,7 +24,7 @@ def test_runner_lists_plugins():
     assert r.returncode == 0
     names = r.stdout.split()
     for n in ("CoulombIntegralsFromVertex", "CcsdEnergyFromCoulombIntegrals",
-              "CcsdPerturbativeTriples", "GenerateSyntheticSystem"):
+              "CcsdPerturbativeTriples", "GenerateSyntheticSystem", "UegVertexGenerator"):
         assert n in names
 
 
@@ -40,6 +42,14 @@ def test_runner_parses_reference_style_input():
     assert 'mixer: "DiisMixer"' in ccsd and 'maxResidua: "6"' in ccsd
     assert "CoulombVertex: $CoulombVertex" in ccsd and "CcsdEnergy: $CcsdEnergy" in ccsd
     assert "(fallible)" in lines[5]
+
+
+def test_runner_parses_ueg_input():
+    r = run(["-i", UEG_INPUT, "--dry"])
+    assert r.returncode == 0, r.stderr
+    names = [l.split()[0] for l in r.stdout.strip().splitlines()]
+    assert names == ["Nop", "UegVertexGenerator", "CoulombIntegralsFromVertex",
+                     "CcsdEnergyFromCoulombIntegrals", "CcsdPerturbativeTriples"]
 
 
 def test_runner_rejects_unknown_algorithm(tmp_path):
@@ -77,3 +87,24 @@ def test_runner_config1_ccsd_and_triples(tmp_path):
     assert abs(vals["CcsdPerturbativeTriplesEnergy"] - (ref["energy"] + et)) <= 1e-8
     text = out.read_text()
     assert text.count("iteration:") == 30 and "realtime:" in text
+
+
+@pytest.mark.gpu
+def test_runner_ueg_known_answers(tmp_path):
+    """The reference's UEG test end to end: MP2, CCSD and CCSD(T) energies of
+    integration-tests/tests/cc4s/ueg/rs1.0-7occ-26virt/cc4s.correct.out.yaml
+    (:128, :153, :169)."""
+    out = tmp_path / "out.yaml"
+    announce("runner ueg rs1 No7 Nv26 known answers")
+    r = run(["-i", UEG_INPUT, "-o", str(out)])
+    assert r.returncode == 0, r.stdout + r.stderr
+    vals = {}
+    for line in r.stdout.splitlines():
+        if "=" in line:
+            k, v = line.split("=")
+            vals[k.strip()] = float(v)
+    assert abs(vals["CcsdEnergy"] - -0.39269658954585018) <= 1e-12
+    assert abs(vals["CcsdPerturbativeTriplesEnergy"]
+               - (-0.39269658954585018 - 0.0063019625641725016)) <= 1e-12
+    m = re.search(r"iteration: 1, energy: \{value: ([-+0-9.eE]+)\}", out.read_text())
+    assert m and abs(float(m.group(1)) - -0.36143028565969504) <= 1e-12

@@ -2,13 +2,14 @@
 numpy's Philox, the reference's full-integral CCSD formulation
 (CcsdEnergyFromCoulombIntegralsReference.cxx), the unsliced 6-index (T)
 (PerturbativeTriples.cxx:19-97), the MP2 closed form, DIIS identities,
-symmetries and limits."""
+symmetries and limits, and the reference's own known answers for the
+uniform electron gas (rs=1, No=7, Nv=26)."""
 
 import numpy as np
 import pytest
 
 from conftest import random_amplitudes
-from oracle import ccsd, formulations, philox, triples
+from oracle import ccsd, formulations, philox, triples, ueg
 
 
 def test_philox_matches_numpy_bit_generator():
@@ -154,3 +155,48 @@ def test_golden_fixture_regression():
     r = ccsd.solve(g, ei, ea, mixer="DiisMixer", max_residua=6, max_iterations=8,
                    amplitudes_convergence=0.0, energy_convergence=0.0)
     np.testing.assert_allclose(r["energies"], d["energies"], rtol=1e-11)
+
+
+# Known answers of integration-tests/tests/cc4s/ueg/rs1.0-7occ-26virt/
+# cc4s.correct.out.yaml: MP2 (:128, secondOrder), CCSD (:153, correlation of
+# the Ccsd step: DiisMixer, maxResidua 4, energyConvergence 1e-8) and (T) on
+# those amplitudes (:169).  The reference's CCSD stopped after iteration 9
+# (0-based), so the driver here runs exactly 10 iterations.
+UEG_MP2 = -0.36143028565969504
+UEG_CCSD = -0.39269658954585018
+UEG_TRIPLES = -0.0063019625641725016
+
+
+def test_ueg_vertex_properties():
+    g, ei, ea, pw = ueg.system(7, 26, 1.0)
+    assert g.shape == (257, 33, 33)
+    np.testing.assert_array_equal(g, g.transpose(0, 2, 1))
+    # HOMO/LUMO and Madelung constant as the reference prints them
+    assert ei.max() < ea.min()
+    # the rotation to real orbitals preserves every Coulomb integral:
+    # sum_G Gamma_Gpr Gamma_Gqs (real) == U* U U* U (pr|qs) (plane waves)
+    U = ueg.real_orbitals(pw["grid"], 7)
+    c = pw["coeff"]
+    t = pw["momenta"]
+    inv = np.array([np.flatnonzero((t == -x).all(1))[0] for x in t])
+    Vpw = np.einsum("Gpr,Gqs->prqs", c, c[inv])
+    Vrot = np.einsum("Pp,Rr,Qq,Ss,prqs->PRQS", U.conj(), U, U.conj(), U, Vpw, optimize=True)
+    Vreal = np.einsum("Gpr,Gqs->prqs", g, g)
+    assert np.abs(Vrot.imag).max() < 1e-14
+    np.testing.assert_allclose(Vreal, Vrot.real, rtol=0, atol=1e-14)
+
+
+def test_ueg_known_answers():
+    g, ei, ea, _ = ueg.system(7, 26, 1.0)
+    assert abs(ccsd.mp2_energy(g, ei, ea) - UEG_MP2) < 1e-14
+    r = ccsd.solve(g, ei, ea, mixer="DiisMixer", max_residua=4, max_iterations=10,
+                   amplitudes_convergence=1e-8, energy_convergence=1e-8)
+    assert abs(r["energies"][0] - UEG_MP2) < 1e-14
+    assert abs(r["energy"] - UEG_CCSD) < 1e-13
+    et = triples.triples_energy(r["T1"], r["T2"], g, ei, ea)
+    assert abs(et - UEG_TRIPLES) < 1e-15
+
+
+def test_ueg_open_shell_is_rejected():
+    with pytest.raises(ValueError):
+        ueg.system(5, 28, 1.0)
