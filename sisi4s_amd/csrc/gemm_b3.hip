@@ -126,31 +126,34 @@ __global__ __launch_bounds__(256) void gemm_b3p_kernel(B3Args p) {
   const u16 *Bgh = p.Bh + int64_t(n0) * p.Kp, *Bgl = p.Bl + int64_t(n0) * p.Kp;
 
   U128 ra[NBUF][2 * NA], rb[NBUF][2 * NB];
-#define B3_LOAD(b, k0)                                                                   \
-  do {                                                                                  \
-    _Pragma("unroll") for (int i = 0; i < NA; ++i) {                                    \
-      const int c = tid + 256 * i, row = c >> 2, kq = (c & 3) * 8;                       \
-      ra[b][i] = *reinterpret_cast<const U128 *>(Agh + int64_t(row) * p.Kp + (k0) + kq);      \
-      ra[b][NA + i] = *reinterpret_cast<const U128 *>(Agl + int64_t(row) * p.Kp + (k0) + kq); \
-    }                                                                                   \
-    _Pragma("unroll") for (int i = 0; i < NB; ++i) {                                    \
-      const int c = tid + 256 * i, row = c >> 2, kq = (c & 3) * 8;                       \
-      rb[b][i] = *reinterpret_cast<const U128 *>(Bgh + int64_t(row) * p.Kp + (k0) + kq);      \
-      rb[b][NB + i] = *reinterpret_cast<const U128 *>(Bgl + int64_t(row) * p.Kp + (k0) + kq); \
-    }                                                                                   \
+  // The k argument is evaluated once, before the macro declares any names of
+  // its own, so an argument naming an outer variable cannot be captured.
+#define B3_LOAD(b, kexpr)                                                                  \
+  do {                                                                                    \
+    const int kb_ = (kexpr);                                                              \
+    _Pragma("unroll") for (int i_ = 0; i_ < NA; ++i_) {                                   \
+      const int q_ = tid + 256 * i_, row_ = q_ >> 2, kq_ = (q_ & 3) * 8;                  \
+      ra[b][i_] = *reinterpret_cast<const U128 *>(Agh + int64_t(row_) * p.Kp + kb_ + kq_);    \
+      ra[b][NA + i_] = *reinterpret_cast<const U128 *>(Agl + int64_t(row_) * p.Kp + kb_ + kq_); \
+    }                                                                                     \
+    _Pragma("unroll") for (int i_ = 0; i_ < NB; ++i_) {                                   \
+      const int q_ = tid + 256 * i_, row_ = q_ >> 2, kq_ = (q_ & 3) * 8;                  \
+      rb[b][i_] = *reinterpret_cast<const U128 *>(Bgh + int64_t(row_) * p.Kp + kb_ + kq_);    \
+      rb[b][NB + i_] = *reinterpret_cast<const U128 *>(Bgl + int64_t(row_) * p.Kp + kb_ + kq_); \
+    }                                                                                     \
   } while (0)
-#define B3_STORE(b)                                                                      \
-  do {                                                                                  \
-    _Pragma("unroll") for (int i = 0; i < NA; ++i) {                                    \
-      const int c = tid + 256 * i, row = c >> 2, kq = (c & 3) * 8;                       \
-      *reinterpret_cast<U128 *>(&Ah[row * RS + kq]) = ra[b][i];                          \
-      *reinterpret_cast<U128 *>(&Al[row * RS + kq]) = ra[b][NA + i];                     \
-    }                                                                                   \
-    _Pragma("unroll") for (int i = 0; i < NB; ++i) {                                    \
-      const int c = tid + 256 * i, row = c >> 2, kq = (c & 3) * 8;                       \
-      *reinterpret_cast<U128 *>(&Bh[row * RS + kq]) = rb[b][i];                          \
-      *reinterpret_cast<U128 *>(&Bl[row * RS + kq]) = rb[b][NB + i];                     \
-    }                                                                                   \
+#define B3_STORE(b)                                                                        \
+  do {                                                                                    \
+    _Pragma("unroll") for (int i_ = 0; i_ < NA; ++i_) {                                   \
+      const int q_ = tid + 256 * i_, row_ = q_ >> 2, kq_ = (q_ & 3) * 8;                  \
+      *reinterpret_cast<U128 *>(&Ah[row_ * RS + kq_]) = ra[b][i_];                         \
+      *reinterpret_cast<U128 *>(&Al[row_ * RS + kq_]) = ra[b][NA + i_];                    \
+    }                                                                                     \
+    _Pragma("unroll") for (int i_ = 0; i_ < NB; ++i_) {                                   \
+      const int q_ = tid + 256 * i_, row_ = q_ >> 2, kq_ = (q_ & 3) * 8;                  \
+      *reinterpret_cast<U128 *>(&Bh[row_ * RS + kq_]) = rb[b][i_];                         \
+      *reinterpret_cast<U128 *>(&Bl[row_ * RS + kq_]) = rb[b][NB + i_];                    \
+    }                                                                                     \
   } while (0)
 
   constexpr int TI = WM / 16, TJ = WN / 16;
@@ -202,11 +205,11 @@ __global__ __launch_bounds__(256) void gemm_b3p_kernel(B3Args p) {
   for (int kc = 0; kc < nk; kc += NBUF) {
 #pragma unroll
     for (int b = 0; b < NBUF; ++b) {
-      const int c = kc + b;
-      if (c < nk) {
+      const int chunk = kc + b;
+      if (chunk < nk) {
         B3_STORE(b);
         __syncthreads();
-        if (c + NBUF < nk) B3_LOAD(b, kbeg + (c + NBUF) * BK);
+        if (chunk + NBUF < nk) B3_LOAD(b, kbeg + (chunk + NBUF) * BK);
         compute();
         __syncthreads();
       }
