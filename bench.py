@@ -176,15 +176,24 @@ def main():
             if b not in blocks:
                 blocks.append(b)
         e.triples(precision="fp64", blocks=[blocks[0]])  # builds the (T) integrals
-        t0 = time.perf_counter()
-        e.triples(precision="fp64", blocks=blocks)
-        tb = (time.perf_counter() - t0) / len(blocks)
         nblk = No * (No + 1) * (No + 2) // 6
+        per = {}
+        for tp in ("fp64", "fp32", "bf16x3"):
+            e.triples(precision=tp, blocks=[blocks[0]])
+            t0 = time.perf_counter()
+            et = e.triples(precision=tp, blocks=blocks)
+            per[tp] = ((time.perf_counter() - t0) / len(blocks), et)
+            log("(T) %s %.4fs per block, %.0fs extrapolated" % (tp, per[tp][0], per[tp][0] * nblk))
+        tb = per["fp64"][0]
         triples = dict(seconds_per_block=tb, blocks=nblk,
                        seconds_extrapolated=tb * nblk,
                        tflops=triples_flops(No, Nv, NG) / (tb * nblk) / 1e12,
-                       sample="%d random i<j<k blocks, FP64" % len(blocks))
-        log("(T) %.3fs per block, %.0fs extrapolated" % (tb, tb * nblk))
+                       sample="%d random i<j<k blocks, FP64" % len(blocks),
+                       reduced={tp: dict(seconds_per_block=per[tp][0],
+                                         seconds_extrapolated=per[tp][0] * nblk,
+                                         rel_error_sample=abs(per[tp][1] - per["fp64"][1])
+                                         / abs(per["fp64"][1]))
+                                for tp in ("fp32", "bf16x3")})
 
     cpu = None
     if not args.no_cpu:

@@ -22,54 +22,9 @@
 
 #include "engine.hpp"
 #include "gemm.hpp"
+#include "gemm_device.hpp"
 
 namespace sisi {
-
-typedef double d4 __attribute__((ext_vector_type(4)));
-typedef float f4 __attribute__((ext_vector_type(4)));
-typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
-
-constexpr int BM = 128, BN = 128;
-
-__device__ __forceinline__ int64_t out_off_m(const GemmArgs &g, int m) {
-  if (!g.split_out) return int64_t(m) * g.cm0;
-  int m0 = m % g.dm0;
-  int t = m / g.dm0;
-  int m1 = t % g.dm1;
-  int m2 = t / g.dm1;
-  return int64_t(m0) * g.cm0 + int64_t(m1) * g.cm1 + int64_t(m2) * g.cm2;
-}
-
-__device__ __forceinline__ int64_t out_off_n(const GemmArgs &g, int n) {
-  if (!g.split_out) return int64_t(n) * g.cn0;
-  int n0 = n % g.dn0;
-  int t = n / g.dn0;
-  int n1 = t % g.dn1;
-  int n2 = t / g.dn1;
-  return int64_t(n0) * g.cn0 + int64_t(n1) * g.cn1 + int64_t(n2) * g.cn2;
-}
-
-// Tile coordinates.  Workgroups are dealt to the 8 XCDs round-robin
-// (workgroup id mod 8), and each XCD has its own L2.  With xcd_map the id is
-// remapped so XCD x walks the contiguous tile range [x*T/8, (x+1)*T/8); inside
-// a range, groups of `group` m-tiles walk the n direction together, so the
-// workgroups resident on one XCD form a compact block of tiles that share A
-// row panels and B column panels in that XCD's L2.
-__device__ __forceinline__ void tile_coords(const GemmArgs &g, int pid, int &tm, int &tn) {
-  if (g.xcd_map) {
-    const int tiles = g.tiles_m * g.tiles_n;
-    const int per = tiles >> 3;
-    if (pid < (per << 3)) pid = (pid & 7) * per + (pid >> 3);
-  }
-  const int group = g.group;
-  int per_group = group * g.tiles_n;
-  int gid = pid / per_group;
-  int first_m = gid * group;
-  int gsize = min(g.tiles_m - first_m, group);
-  int r = pid % per_group;
-  tm = first_m + r % gsize;
-  tn = r / gsize;
-}
 
 // ---------------------------------------------------------------------------
 // FP64 / FP32 kernel (LDS element type LT; k-chunk 16; MFMA k-step 4)
@@ -254,12 +209,6 @@ __global__ __launch_bounds__(256, 2) void gemm_k4_kernel(GemmArgs g) {
 // ---------------------------------------------------------------------------
 // BF16x3 kernel (k-chunk 32 = one MFMA k-step; LDS rows along k)
 // ---------------------------------------------------------------------------
-
-__device__ __forceinline__ void split_bf16(double x, __bf16 &hi, __bf16 &lo) {
-  float xf = float(x);
-  hi = __bf16(xf);
-  lo = __bf16(xf - float(hi));
-}
 
 template <bool AK, bool BNF>
 __global__ __launch_bounds__(256, 2) void gemm_bf16x3_kernel(GemmArgs g) {

@@ -35,7 +35,32 @@ struct GemmArgs {
   double alpha = 1.0, beta = 0.0;
 };
 
+// One product of a segmented GEMM: A(m,k) = A[m*sam + k*sak],
+// B(k,n) = B[k*sbk + n*sbn], k < K, scaled by `scale`.  ak: A is k-fastest
+// (sak == 1); bnf: B is n-fastest (sbn == 1).
+struct GemmSeg {
+  const double *A = nullptr;
+  const double *B = nullptr;
+  int64_t sam = 1, sak = 0, sbk = 1, sbn = 0;
+  int K = 0;
+  int ak = 0, bnf = 0;
+  double scale = 1.0;
+};
+
+// C = alpha * sum_s scale_s A_s B_s + beta * C in one launch, the products
+// accumulated in the same registers (no batch, no split-K).  The output
+// fields of `g` (C, M, N, cm*/cn*, dm*/dn*, alpha, beta) apply to the sum.
+constexpr int kMaxSegs = 4;
+struct SegGemmArgs {
+  GemmArgs g;
+  int nseg = 0;
+  GemmSeg seg[kMaxSegs];
+};
+
 class Engine;
+
+// Segmented GEMM on the engine's stream, precision SISI_FP64/_FP32/_BF16X3.
+void gemm_segments(Engine &eng, int precision, SegGemmArgs a);
 
 // Launch on the engine's stream.  precision: SISI_FP64/_FP32/_BF16X3.
 void gemm(Engine &eng, int precision, GemmArgs g);

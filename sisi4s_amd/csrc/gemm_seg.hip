@@ -1,0 +1,423 @@
+// Segmented MFMA GEMM: C = alpha * sum_s scale_s A_s B_s + beta * C with up
+// to four products of different K, operands and layouts accumulated in the
+// same registers, so a sum of contractions into one output costs one launch
+// and one epilogue pass instead of one read-modify-write of C per product.
+//
+// Same block tile, wave layout, LDS staging and MFMA forms as the FP64/FP32
+// and on-the-fly BF16x3 kernels of gemm.hip; the operand load mapping
+// (m- or k-fastest A, k- or n-fastest B) is chosen per segment at run time
+// (uniform across the workgroup).  The k loop walks the segments' k-chunks
+// in order, prefetching the next chunk (possibly of the next segment) while
+// the current one is multiplied.
+#include <cstdlib>
+
+#include "engine.hpp"
+#include "gemm.hpp"
+#include "gemm_device.hpp"
+
+namespace sisi {
+
+namespace {
+
+__device__ __forceinline__ GemmSeg seg_at(const SegGemmArgs &a, int s) {
+  return s == 0 ? a.seg[0] : s == 1 ? a.seg[1] : s == 2 ? a.seg[2] : a.seg[3];
+}
+
+__device__ __forceinline__ int seg_k(const SegGemmArgs &a, int s) {
+  return s == 0 ? a.seg[0].K : s == 1 ? a.seg[1].K : s == 2 ? a.seg[2].K : a.seg[3].K;
+}
+
+template <int PREC>
+__global__ __launch_bounds__(256, 2) void gemm_seg_k4_kernel(SegGemmArgs a) {
+  using LT = typename std::conditional<PREC == SISI_FP64, double, float>::type;
+  using AT = typename std::conditional<PREC == SISI_FP64, d4, f4>::type;
+  constexpr int BK = 16;
+  constexpr int PAD = (PREC == SISI_FP64) ? 2 : 4;
+  constexpr int LSTR = (PREC == SISI_FP64) ? 1 : 4;
+  constexpr int RSTR = (PREC == SISI_FP64) ? 4 : 1;
+  __shared__ LT As[BK][BM + PAD];
+  __shared__ LT Bs[BK][BN + PAD];
+  const GemmArgs &g = a.g;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  int tm, tn;
+  tile_coords(g, blockIdx.x, tm, tn);
+  const int m0 = tm * BM, n0 = tn * BN;
+
+  double ra[8], rb[8];
+  int lak = 0, lbnf = 0;
+
+  auto load = [&](int s, int k0) {
+    const GemmSeg sg = seg_at(a, s);
+    const int K = sg.K;
+    const double sc = sg.scale;
+    lak = sg.ak;
+    lbnf = sg.bnf;
+    if (!sg.ak) {  // A m-fastest: 32 threads x 4 consecutive m, k rows t/32 and +8
+      const int mm = (tid & 31) * 4, kk = tid >> 5;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int k = k0 + kk + 8 * h;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int m = m0 + mm + j;
+          ra[h * 4 + j] =
+              (m < g.M && k < K) ? sc * sg.A[int64_t(m) * sg.sam + int64_t(k) * sg.sak] : 0.0;
+        }
+      }
+    } else {  // A k-fastest: 16 threads along k, m = t/16 + 16 j
+      const int kk = tid & 15, mm = tid >> 4;
+      const int k = k0 + kk;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int m = m0 + mm + 16 * j;
+        ra[j] = (m < g.M && k < K) ? sc * sg.A[int64_t(m) * sg.sam + int64_t(k) * sg.sak] : 0.0;
+      }
+    }
+    if (sg.bnf) {  // B n-fastest
+      const int nn = (tid & 31) * 4, kk = tid >> 5;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int k = k0 + kk + 8 * h;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int n = n0 + nn + j;
+          rb[h * 4 + j] =
+              (n < g.N && k < K) ? sg.B[int64_t(k) * sg.sbk + int64_t(n) * sg.sbn] : 0.0;
+        }
+      }
+    } else {  // B k-fastest
+      const int kk = tid & 15, nn = tid >> 4;
+      const int k = k0 + kk;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int n = n0 + nn + 16 * j;
+        rb[j] = (n < g.N && k < K) ? sg.B[int64_t(k) * sg.sbk + int64_t(n) * sg.sbn] : 0.0;
+      }
+    }
+  };
+
+  auto store = [&]() {
+    if (!lak) {
+      const int mm = (tid & 31) * 4, kk = tid >> 5;
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) As[kk + 8 * h][mm + j] = LT(ra[h * 4 + j]);
+    } else {
+      const int kk = tid & 15, mm = tid >> 4;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) As[kk][mm + 16 * j] = LT(ra[j]);
+    }
+    if (lbnf) {
+      const int nn = (tid & 31) * 4, kk = tid >> 5;
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) Bs[kk + 8 * h][nn + j] = LT(rb[h * 4 + j]);
+    } else {
+      const int kk = tid & 15, nn = tid >> 4;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) Bs[kk][nn + 16 * j] = LT(rb[j]);
+    }
+  };
+
+  AT acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = AT{0, 0, 0, 0};
+
+  const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+  const int l16 = lane & 15, lk = lane >> 4;
+
+  int s = 0, k0 = 0;
+  bool have = a.nseg > 0;
+  if (have) load(0, 0);
+  while (have) {
+    store();
+    __syncthreads();
+    int s1 = s, k1 = k0 + BK;
+    if (k1 >= seg_k(a, s)) {
+      s1 = s + 1;
+      k1 = 0;
+    }
+    const bool more = s1 < a.nseg;
+    if (more) load(s1, k1);
+#pragma unroll
+    for (int kq = 0; kq < BK / 4; ++kq) {
+      const int k = kq * 4 + lk;
+      LT av[4], bv[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) av[i] = As[k][wm + i * 16 + l16];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) bv[j] = Bs[k][wn + j * 16 + l16];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          if constexpr (PREC == SISI_FP64)
+            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j], av[i], acc[i][j], 0, 0, 0);
+          else
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(bv[j], av[i], acc[i][j], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+    s = s1;
+    k0 = k1;
+    have = more;
+  }
+
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int m = m0 + wm + i * 16 + l16;
+    if (m >= g.M) continue;
+    const int64_t om = out_off_m(g, m);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = n0 + wn + j * 16 + LSTR * lk + RSTR * r;
+        if (n < g.N) {
+          const int64_t o = om + out_off_n(g, n);
+          double v = g.alpha * double(acc[i][j][r]);
+          if (g.beta != 0.0) v += g.beta * g.C[o];
+          g.C[o] = v;
+        }
+      }
+  }
+}
+
+__global__ __launch_bounds__(256, 2) void gemm_seg_b3_kernel(SegGemmArgs a) {
+  constexpr int BK = 32;
+  constexpr int RS = BK + 8;  // row stride in bf16 (keeps 16-byte alignment)
+  __shared__ __attribute__((aligned(16))) __bf16 Ah[BM * RS];
+  __shared__ __attribute__((aligned(16))) __bf16 Al[BM * RS];
+  __shared__ __attribute__((aligned(16))) __bf16 Bh[BN * RS];
+  __shared__ __attribute__((aligned(16))) __bf16 Bl[BN * RS];
+  const GemmArgs &g = a.g;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  int tm, tn;
+  tile_coords(g, blockIdx.x, tm, tn);
+  const int m0 = tm * BM, n0 = tn * BN;
+
+  double ra[16], rb[16];
+  int lak = 0, lbnf = 0;
+  auto load = [&](int s, int k0) {
+    const GemmSeg sg = seg_at(a, s);
+    const int K = sg.K;
+    const double sc = sg.scale;
+    lak = sg.ak;
+    lbnf = sg.bnf;
+    if (!sg.ak) {  // m fastest: 32 threads x 4 m, 8 k rows, 4 passes
+      const int mm = (tid & 31) * 4, kk = tid >> 5;
+#pragma unroll
+      for (int h = 0; h < 4; ++h) {
+        const int k = k0 + kk + 8 * h;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int m = m0 + mm + j;
+          ra[h * 4 + j] =
+              (m < g.M && k < K) ? sc * sg.A[int64_t(m) * sg.sam + int64_t(k) * sg.sak] : 0.0;
+        }
+      }
+    } else {  // k fastest: 32 threads along k, m = t/32 + 8 j
+      const int kk = tid & 31, mm = tid >> 5;
+      const int k = k0 + kk;
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        const int m = m0 + mm + 8 * j;
+        ra[j] = (m < g.M && k < K) ? sc * sg.A[int64_t(m) * sg.sam + int64_t(k) * sg.sak] : 0.0;
+      }
+    }
+    if (sg.bnf) {
+      const int nn = (tid & 31) * 4, kk = tid >> 5;
+#pragma unroll
+      for (int h = 0; h < 4; ++h) {
+        const int k = k0 + kk + 8 * h;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int n = n0 + nn + j;
+          rb[h * 4 + j] =
+              (n < g.N && k < K) ? sg.B[int64_t(k) * sg.sbk + int64_t(n) * sg.sbn] : 0.0;
+        }
+      }
+    } else {
+      const int kk = tid & 31, nn = tid >> 5;
+      const int k = k0 + kk;
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        const int n = n0 + nn + 8 * j;
+        rb[j] = (n < g.N && k < K) ? sg.B[int64_t(k) * sg.sbk + int64_t(n) * sg.sbn] : 0.0;
+      }
+    }
+  };
+  auto store = [&]() {
+    __bf16 h, l;
+    if (!lak) {
+      const int mm = (tid & 31) * 4, kk = tid >> 5;
+#pragma unroll
+      for (int hh = 0; hh < 4; ++hh)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          split_bf16(ra[hh * 4 + j], h, l);
+          Ah[(mm + j) * RS + kk + 8 * hh] = h;
+          Al[(mm + j) * RS + kk + 8 * hh] = l;
+        }
+    } else {
+      const int kk = tid & 31, mm = tid >> 5;
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        split_bf16(ra[j], h, l);
+        Ah[(mm + 8 * j) * RS + kk] = h;
+        Al[(mm + 8 * j) * RS + kk] = l;
+      }
+    }
+    if (lbnf) {
+      const int nn = (tid & 31) * 4, kk = tid >> 5;
+#pragma unroll
+      for (int hh = 0; hh < 4; ++hh)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          split_bf16(rb[hh * 4 + j], h, l);
+          Bh[(nn + j) * RS + kk + 8 * hh] = h;
+          Bl[(nn + j) * RS + kk + 8 * hh] = l;
+        }
+    } else {
+      const int kk = tid & 31, nn = tid >> 5;
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        split_bf16(rb[j], h, l);
+        Bh[(nn + 8 * j) * RS + kk] = h;
+        Bl[(nn + 8 * j) * RS + kk] = l;
+      }
+    }
+  };
+
+  f4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = f4{0, 0, 0, 0};
+  const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+  const int l16 = lane & 15, lk = lane >> 4;
+
+  int s = 0, k0 = 0;
+  bool have = a.nseg > 0;
+  if (have) load(0, 0);
+  while (have) {
+    store();
+    __syncthreads();
+    int s1 = s, k1 = k0 + BK;
+    if (k1 >= seg_k(a, s)) {
+      s1 = s + 1;
+      k1 = 0;
+    }
+    const bool more = s1 < a.nseg;
+    if (more) load(s1, k1);
+    bf8 ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int row = wm + i * 16 + l16;
+      ah[i] = *reinterpret_cast<const bf8 *>(&Ah[row * RS + 8 * lk]);
+      al[i] = *reinterpret_cast<const bf8 *>(&Al[row * RS + 8 * lk]);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int col = wn + j * 16 + l16;
+      bh[j] = *reinterpret_cast<const bf8 *>(&Bh[col * RS + 8 * lk]);
+      bl[j] = *reinterpret_cast<const bf8 *>(&Bl[col * RS + 8 * lk]);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[j], ah[i], acc[i][j], 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], al[i], acc[i][j], 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], ah[i], acc[i][j], 0, 0, 0);
+    __syncthreads();
+    s = s1;
+    k0 = k1;
+    have = more;
+  }
+
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int m = m0 + wm + i * 16 + l16;
+    if (m >= g.M) continue;
+    const int64_t om = out_off_m(g, m);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = n0 + wn + j * 16 + 4 * lk + r;
+        if (n < g.N) {
+          const int64_t o = om + out_off_n(g, n);
+          double v = g.alpha * double(acc[i][j][r]);
+          if (g.beta != 0.0) v += g.beta * g.C[o];
+          g.C[o] = v;
+        }
+      }
+  }
+}
+
+}  // namespace
+
+void gemm_segments(Engine &eng, int precision, SegGemmArgs a) {
+  GemmArgs &g = a.g;
+  if (g.M <= 0 || g.N <= 0) return;
+  SISI_REQUIRE(a.nseg >= 1 && a.nseg <= kMaxSegs, SISI_ERR_INVALID,
+               "gemm_segments: 1 to 4 segments");
+  SISI_REQUIRE(precision == SISI_FP64 || precision == SISI_FP32 || precision == SISI_BF16X3,
+               SISI_ERR_INVALID, "gemm_segments: unknown precision");
+  double flops = 0.0;
+  for (int s = 0; s < a.nseg; ++s) {
+    GemmSeg &sg = a.seg[s];
+    SISI_REQUIRE(sg.K > 0 && sg.A && sg.B, SISI_ERR_INVALID,
+                 "gemm_segments: every segment needs K > 0 and operands");
+    // the flags only choose which index consecutive threads walk
+    sg.ak = (sg.sak == 1 && sg.sam != 1) ? 1 : 0;
+    sg.bnf = (sg.sbn == 1 && sg.sbk != 1) ? 1 : 0;
+    flops += 2.0 * double(g.M) * double(g.N) * double(sg.K);
+  }
+  g.batch = 1;
+  g.ksplit = 1;
+  g.kchunk = 0;
+  g.tiles_m = (g.M + BM - 1) / BM;
+  g.tiles_n = (g.N + BN - 1) / BN;
+  const int64_t tiles = int64_t(g.tiles_m) * g.tiles_n;
+  SISI_REQUIRE(tiles < (int64_t(1) << 31), SISI_ERR_INVALID, "gemm_segments: too many tiles");
+  g.split_out = !(g.dm0 >= g.M && g.dn0 >= g.N);
+  {
+    static const int env_map = getenv("SISI_GEMM_XCD") ? atoi(getenv("SISI_GEMM_XCD")) : 1;
+    static const int env_group = getenv("SISI_GEMM_GROUP") ? atoi(getenv("SISI_GEMM_GROUP")) : 8;
+    g.xcd_map = env_map;
+    g.group = env_group > 0 ? env_group : 8;
+  }
+  const Kind kind = precision == SISI_FP64 ? Kind::Gemm64
+                    : precision == SISI_FP32 ? Kind::Gemm32 : Kind::GemmB3;
+  Timed t(*eng.timers, kind, flops);
+  const dim3 grid{unsigned(tiles), 1u, 1u};
+  if (precision == SISI_FP64)
+    hipLaunchKernelGGL(gemm_seg_k4_kernel<SISI_FP64>, grid, dim3(256), 0, eng.stream, a);
+  else if (precision == SISI_FP32)
+    hipLaunchKernelGGL(gemm_seg_k4_kernel<SISI_FP32>, grid, dim3(256), 0, eng.stream, a);
+  else
+    hipLaunchKernelGGL(gemm_seg_b3_kernel, grid, dim3(256), 0, eng.stream, a);
+  SISI_HIP(hipGetLastError());
+}
+
+}  // namespace sisi

@@ -1,23 +1,30 @@
 // (T) triples (CcsdPerturbativeTriples.cxx:119-248) on the device.
 //
 // For each sorted block i = (i0 <= i1 <= i2) the reference forms, for the six
-// permutations pi of (i0,i1,i2), W(i.pi)_abc = sum_d T_{ad,x y} V_{bdcz}
-// - sum_l T_{ab,x l} V_{yzlc} and reads them back with "abc"*pi (cxx:159-181).
-// Here the GEMM epilogue writes W(i.pi) straight into the permuted position,
-// so the six GEMM pairs accumulate
-//     Wsym[x] = sum_pi W(i.pi)[x_pi(0), x_pi(1), x_pi(2)].
-// The reference energy (cxx:183-214),
+// permutations pi of (i0,i1,i2), with (x,y,z) = i.pi,
+//     W(x,y,z)_abc = sum_d T_{ad,xy} V_{bdcz} - sum_l T_{ab,xl} V_{yzlc}
+// and reads them back with "abc"*pi (cxx:159-181).  Here
+//     W[x] = sum_{distinct pi} W(i.pi)[x_pi(0), x_pi(1), x_pi(2)]
+// is built by three segmented GEMMs, one per output index j that plays the
+// GEMM row: the particle terms of the permutations with pi(0) = j (K = Nv
+// each) and the hole terms of those with pi(2) = j (K = No each) are summed in
+// the same registers, the two orders of the remaining pair (p, q) coming from
+// transposed copies of V and T2.  The reference energy (cxx:183-214),
 //     E = sum_{distinct pi} sum_x DV[x] sum_sigma sf(sigma) Z_pi[("abc" sigma pi)(x)],
-// DV = Wsym / D, Z_pi = W(i.pi) + S(i.pi), is rewritten by relabelling the
-// summation index as
-//     E = sum_x F[x] Zsym[x],
-//     F[x] = (1/D[x]) sum_sigma sf(sigma) Wsym[x_sigma(0), x_sigma(1), x_sigma(2)],
-//     Zsym[x] = sum_{distinct pi} Z_pi[x_pi(0), x_pi(1), x_pi(2)],
-// with sf = +8 (identity), -4 (transpositions), +2 (3-cycles).  For blocks
-// with three different indices every pi is distinct and Zsym = Wsym + Ssym;
-// degenerate blocks accumulate the distinct W(i.pi) separately.  The energy
-// kernel evaluates F and Zsym on 8x8x8 cubes whose six permuted partner cubes
-// are staged in LDS, and reduces in FP64.
+// DV = Wsym / D, Wsym the sum over all six permutations, Z_pi = W(i.pi) + S(i.pi),
+// is rewritten by relabelling the summation index as
+//     E = sum_x F[x] Z[x],   F[x] = (1/D[x]) sum_sigma sf'(sigma) W[x.sigma],
+//     Z[x] = W[x] + sum_{distinct pi} S(i.pi)[x.pi],
+// with sf = +8 (identity), -4 (transpositions), +2 (3-cycles).  When all three
+// indices differ every permutation is distinct, W = Wsym and sf' = sf.  When
+// two are equal, Wsym[x] = W[x] + W[x.tau] with tau the swap of the equal
+// positions, so sf'(sigma) = sf(sigma) + sf(sigma.tau) = +4 for sigma in
+// {id, tau}, -2 otherwise; when all three are equal sf' = 0 and the block
+// contributes nothing (as in the reference, whose i=j=k terms cancel).
+// The energy kernel walks orbits of 8x8x8 cubes under index permutations:
+// the (up to) six permuted cubes of an orbit are staged in LDS once and every
+// cube of the orbit is evaluated from them, so W is read once; the sum is
+// reduced in FP64.
 #include "triples.hpp"
 
 #include "contract.hpp"
@@ -47,16 +54,33 @@ struct PermTerm {
 };
 
 struct EnergyArgs {
-  const double *W;   // Wsym
-  const double *Z;   // W part of Zsym
+  const double *W;
   const double *T1;  // [Nv, No]
   const double *V;   // Vabij [Nv,Nv,No,No]
   const double *eps_a;
+  const int *orbits;  // X0 | X1 << 10 | X2 << 20, X0 <= X1 <= X2
   double *part;
   double eps_ijk;
-  int Nv, No, ncube, nterms;
+  double sfp[6];
+  int Nv, No, nterms;
   PermTerm terms[6];
 };
+
+// sigma as reads (x_s0, x_s1, x_s2): identity, three transpositions, two 3-cycles
+__constant__ int kSig[6][3] = {{0, 1, 2}, {1, 0, 2}, {2, 1, 0}, {0, 2, 1}, {1, 2, 0}, {2, 0, 1}};
+constexpr int hSig[6][3] = {{0, 1, 2}, {1, 0, 2}, {2, 1, 0}, {0, 2, 1}, {1, 2, 0}, {2, 0, 1}};
+constexpr double hSf[6] = {8.0, -4.0, -4.0, -4.0, 2.0, 2.0};
+
+// index r of the composition sig[r][k] = sig[a][sig[b][k]]
+constexpr int compose_idx(int a, int b) {
+  int r = 0;
+  for (; r < 6; ++r) {
+    bool ok = true;
+    for (int k = 0; k < 3; ++k) ok = ok && (hSig[r][k] == hSig[a][hSig[b][k]]);
+    if (ok) break;
+  }
+  return r;
+}
 
 __device__ __forceinline__ double block_sum_t(double v) {
   __shared__ double sh[256];
@@ -73,43 +97,59 @@ __device__ __forceinline__ double block_sum_t(double v) {
 
 __global__ __launch_bounds__(256) void triples_energy_kernel(EnergyArgs g) {
   __shared__ double tile[6][512];
-  const int nc = g.ncube;
-  const int cb = blockIdx.x;
-  const int X[3] = {cb % nc, (cb / nc) % nc, cb / (nc * nc)};
+  __shared__ int comp[6][6];
+  const int code = g.orbits[blockIdx.x];
+  const int X[3] = {code & 1023, (code >> 10) & 1023, code >> 20};
   const int Nv = g.Nv;
   const int64_t s1 = Nv, s2 = int64_t(Nv) * Nv;
-  // sigma as reads (x_s0, x_s1, x_s2); sf by fixed points
-  const int sig[6][3] = {{0, 1, 2}, {1, 0, 2}, {2, 1, 0}, {0, 2, 1}, {1, 2, 0}, {2, 0, 1}};
-  const double sf[6] = {8.0, -4.0, -4.0, -4.0, 2.0, 2.0};
+  if (threadIdx.x < 36) {
+    const int q = threadIdx.x / 6, t = threadIdx.x % 6;
+    int r = 0;
+    for (; r < 6; ++r)
+      if (kSig[r][0] == kSig[q][kSig[t][0]] && kSig[r][1] == kSig[q][kSig[t][1]] &&
+          kSig[r][2] == kSig[q][kSig[t][2]])
+        break;
+    comp[q][t] = r;
+  }
+  // tile r: the cube X.sigma_r, local (l0, l1, l2) at l0 + 8 l1 + 64 l2
 #pragma unroll
-  for (int s = 0; s < 6; ++s) {
-    const int b0 = X[sig[s][0]] * 8, b1 = X[sig[s][1]] * 8, b2 = X[sig[s][2]] * 8;
+  for (int r = 0; r < 6; ++r) {
+    const int b0 = X[kSig[r][0]] * 8, b1 = X[kSig[r][1]] * 8, b2 = X[kSig[r][2]] * 8;
     for (int e = threadIdx.x; e < 512; e += 256) {
       const int u0 = b0 + (e & 7), u1 = b1 + ((e >> 3) & 7), u2 = b2 + (e >> 6);
-      tile[s][e] = (u0 < Nv && u1 < Nv && u2 < Nv) ? g.W[u0 + u1 * s1 + u2 * s2] : 0.0;
+      tile[r][e] = (u0 < Nv && u1 < Nv && u2 < Nv) ? g.W[u0 + u1 * s1 + u2 * s2] : 0.0;
     }
   }
   __syncthreads();
   double acc = 0.0;
-  for (int e = threadIdx.x; e < 512; e += 256) {
-    const int l[3] = {e & 7, (e >> 3) & 7, e >> 6};
-    const int x[3] = {X[0] * 8 + l[0], X[1] * 8 + l[1], X[2] * 8 + l[2]};
-    if (x[0] >= Nv || x[1] >= Nv || x[2] >= Nv) continue;
-    double f = 0.0;
+  for (int q = 0; q < 6; ++q) {
+    // the cube Y = X.sigma_q, once per distinct cube of the orbit
+    const int Y[3] = {X[kSig[q][0]], X[kSig[q][1]], X[kSig[q][2]]};
+    bool dup = false;
+    for (int p = 0; p < q; ++p)
+      dup = dup || (X[kSig[p][0]] == Y[0] && X[kSig[p][1]] == Y[1] && X[kSig[p][2]] == Y[2]);
+    if (dup) continue;
+    for (int e = threadIdx.x; e < 512; e += 256) {
+      const int l[3] = {e & 7, (e >> 3) & 7, e >> 6};
+      const int x[3] = {Y[0] * 8 + l[0], Y[1] * 8 + l[1], Y[2] * 8 + l[2]};
+      if (x[0] >= Nv || x[1] >= Nv || x[2] >= Nv) continue;
+      double f = 0.0;
 #pragma unroll
-    for (int s = 0; s < 6; ++s)
-      f += sf[s] * tile[s][l[sig[s][0]] + 8 * l[sig[s][1]] + 64 * l[sig[s][2]]];
-    const double D = g.eps_ijk - g.eps_a[x[0]] - g.eps_a[x[1]] - g.eps_a[x[2]];
-    f /= D;
-    double z = g.Z[x[0] + x[1] * s1 + x[2] * s2];
-    for (int t = 0; t < g.nterms; ++t) {
-      const PermTerm &p = g.terms[t];
-      // S(i.pi)[u] = 0.5 T1[u0, i_pi0] V[u1, u2, i_pi1, i_pi2] read at u_k = x_pi(k)
-      const int u0 = x[p.pi[0]], u1 = x[p.pi[1]], u2 = x[p.pi[2]];
-      z += 0.5 * g.T1[u0 + int64_t(Nv) * p.idx[0]] *
-           g.V[u1 + s1 * u2 + s2 * (p.idx[1] + int64_t(g.No) * p.idx[2])];
+      for (int t = 0; t < 6; ++t)
+        f += g.sfp[t] *
+             tile[comp[q][t]][l[kSig[t][0]] + 8 * l[kSig[t][1]] + 64 * l[kSig[t][2]]];
+      const double D = g.eps_ijk - g.eps_a[x[0]] - g.eps_a[x[1]] - g.eps_a[x[2]];
+      f /= D;
+      double z = tile[q][e];
+      for (int t = 0; t < g.nterms; ++t) {
+        const PermTerm &p = g.terms[t];
+        // S(i.pi)[u] = 0.5 T1[u0, i_pi0] V[u1, u2, i_pi1, i_pi2] read at u_k = x_pi(k)
+        const int u0 = x[p.pi[0]], u1 = x[p.pi[1]], u2 = x[p.pi[2]];
+        z += 0.5 * g.T1[u0 + int64_t(Nv) * p.idx[0]] *
+             g.V[u1 + s1 * u2 + s2 * (p.idx[1] + int64_t(g.No) * p.idx[2])];
+      }
+      acc += f * z;
     }
-    acc += f * z;
   }
   acc = block_sum_t(acc);
   if (threadIdx.x == 0) g.part[blockIdx.x] = acc;
@@ -122,48 +162,87 @@ __global__ void sum_into_kernel(const double *part, int n, double *out) {
   if (threadIdx.x == 0) *out = s;
 }
 
-// W(x,y,z)[u0,u1,u2] accumulated (beta = 1) at Wp with output strides st.
-// The epilogue's lanes walk the output's unit-stride index when it is the
-// leading index of the GEMM's row or column group, so the operand orders are
-// chosen per permutation: V_bdck as [d,c,b] when c lands on the unit stride,
-// T2 as [b,a,x,l] when b does (two copies of the same tensors, exact).
-void add_w(Engine &e, int prec, const double *T2, const double *T2t, const double *Vt,
-           const double *Vt2, const double *VijkaP, int x, int y, int z, double *Wp,
-           const int64_t st[3]) {
+// Operands of one (T) block, all device pointers.
+struct TriplesOps {
+  const double *T2, *T2t;      // T2[a,b,i,j] and T2t[a,b,i,j] = T2[b,a,i,j]
+  const double *Vt, *Vt2;      // Vt[d,b,c,k] = V_bdck, Vt2[d,b,c,k] = Vt[d,c,b,k]
+  const double *VP;            // VP[l,c,j,k] = V_jklc
+};
+
+// W[x] = sum over the given permutations of W(i.pi)[x.pi], three launches.
+void build_w(Engine &e, int prec, const TriplesOps &op, const std::array<int, 3> &ib,
+             const std::array<std::array<int, 3>, 6> &perms, const bool *use, double *W) {
   const int64_t o = e.No, v = e.Nv;
-  View C;
-  C.p = Wp;
-  C.order = 3;
-  for (int k = 0; k < 3; ++k) {
-    C.len[k] = v;
-    C.stride[k] = st[k];
+  const int64_t st[3] = {1, v, v * v};
+  bool first = true;
+  for (int j = 0; j < 3; ++j) {
+    const int p = j == 0 ? 1 : 0, q = j == 2 ? 1 : 2;  // the other two positions, p < q
+    SegGemmArgs a;
+    for (int n = 0; n < 6; ++n) {
+      if (!use[n]) continue;
+      const auto &pi = perms[n];
+      const int x = ib[pi[0]], y = ib[pi[1]], z = ib[pi[2]];
+      if (pi[0] == j) {
+        // + sum_d T2[x_j, d, x, y] V[d, x_pi1, x_pi2, z]
+        GemmSeg &sg = a.seg[a.nseg++];
+        sg.A = op.T2 + (x + o * y) * v * v;
+        sg.sam = 1;
+        sg.sak = v;
+        sg.B = (pi[1] == p ? op.Vt : op.Vt2) + z * v * v * v;
+        sg.sbk = 1;
+        sg.sbn = v;
+        sg.K = int(v);
+        sg.scale = 1.0;
+      }
+      if (pi[2] == j) {
+        // - sum_l V_P[l, x_j, y, z] T2[x_pi0, x_pi1, x, l]
+        GemmSeg &sg = a.seg[a.nseg++];
+        sg.A = op.VP + (y + o * z) * o * v;
+        sg.sam = o;
+        sg.sak = 1;
+        sg.B = (pi[0] == p ? op.T2 : op.T2t) + x * v * v;
+        sg.sbk = v * v * o;
+        sg.sbn = 1;
+        sg.K = int(o);
+        sg.scale = -1.0;
+      }
+    }
+    if (a.nseg == 0) continue;
+    GemmArgs &g = a.g;
+    g.C = W;
+    g.alpha = 1.0;
+    g.beta = first ? 0.0 : 1.0;
+    first = false;
+    if (j == 0) {
+      // rows x0 (unit stride), columns (x1, x2) at x1*v + x2*v^2 = n*v
+      g.M = int(v);
+      g.N = int(v * v);
+      g.cm0 = 1;
+      g.dm0 = int(v);
+      g.cn0 = v;
+      g.dn0 = int(v * v);
+    } else {
+      // rows and columns exchanged so that lanes run along x0: C^T = B^T A^T
+      for (int s = 0; s < a.nseg; ++s) {
+        GemmSeg &sg = a.seg[s];
+        std::swap(sg.A, sg.B);
+        const int64_t sam = sg.sbn, sak = sg.sbk, sbk = sg.sak, sbn = sg.sam;
+        sg.sam = sam;
+        sg.sak = sak;
+        sg.sbk = sbk;
+        sg.sbn = sbn;
+      }
+      g.M = int(v * v);
+      g.N = int(v);
+      g.cm0 = st[p];
+      g.dm0 = int(v);
+      g.cm1 = st[q];
+      g.dm1 = int(v);
+      g.cn0 = st[j];
+      g.dn0 = int(v);
+    }
+    gemm_segments(e, prec, a);
   }
-  View Txy = dense(const_cast<double *>(T2) + (x + o * y) * v * v, {v, v});
-  if (st[2] == 1) {
-    View Vz = dense(const_cast<double *>(Vt2) + z * v * v * v, {v, v, v});
-    ctr(e, 1.0, "ad,dcb->abc", Txy, Vz, 1.0, C, prec);  // sum_d T_ad,xy V_bdcz
-  } else {
-    View Vz = dense(const_cast<double *>(Vt) + z * v * v * v, {v, v, v});
-    ctr(e, 1.0, "ad,dbc->abc", Txy, Vz, 1.0, C, prec);
-  }
-  View Tx;
-  Tx.p = const_cast<double *>(st[1] == 1 ? T2t : T2) + x * v * v;
-  Tx.order = 3;
-  Tx.len[0] = v;
-  Tx.len[1] = v;
-  Tx.len[2] = o;
-  Tx.stride[0] = 1;
-  Tx.stride[1] = v;
-  Tx.stride[2] = v * v * o;
-  View Vyz;
-  Vyz.p = const_cast<double *>(VijkaP) + (y + o * z) * o * v;
-  Vyz.order = 2;
-  Vyz.len[0] = o;
-  Vyz.len[1] = v;
-  Vyz.stride[0] = 1;
-  Vyz.stride[1] = o;
-  // - sum_l T_ab,xl V_yzlc (T2t[b,a,x,l] = T2[a,b,x,l])
-  ctr(e, -1.0, st[1] == 1 ? "bal,lc->abc" : "abl,lc->abc", Tx, Vyz, 1.0, C, prec);
 }
 
 }  // namespace
@@ -171,8 +250,11 @@ void add_w(Engine &e, int prec, const double *T2, const double *T2t, const doubl
 double triples_energy(Engine &e, const double *T1, const double *T2, int prec,
                       const std::vector<std::array<int, 3>> &blocks_in) {
   SISI_REQUIRE(e.have_integrals, SISI_ERR_STATE, "triples: integrals not built");
+  SISI_REQUIRE(prec == SISI_FP64 || prec == SISI_FP32 || prec == SISI_BF16X3, SISI_ERR_INVALID,
+               "triples: unknown precision");
   e.thaw();
   const int64_t o = e.No, v = e.Nv, G = e.NG;
+  SISI_REQUIRE(v <= 8 * 1023, SISI_ERR_INVALID, "triples: Nv above 8184");
   std::vector<std::array<int, 3>> blocks = blocks_in;
   if (blocks.empty())
     for (int a = 0; a < o; ++a)
@@ -194,26 +276,34 @@ double triples_energy(Engine &e, const double *T1, const double *T2, int prec,
     sum(e, 1.0, dense(e.Vijka.get(), {o, o, o, v}), "jklc", 0.0, dense(e.VP.get(), {o, v, o, o}),
         "lcjk");
   }
-  DBuf &Vt = e.Vt, &Vt2 = e.Vt2, &VP = e.VP;
-  // T2 with a and b exchanged, for the permutations that put b on the unit stride
+  // T2 with a and b exchanged, for the permutations that put b first
   DBuf T2t(&e.pool, v * v * o * o);
   sum(e, 1.0, dense(const_cast<double *>(T2), {v, v, o, o}), "abij", 0.0,
       dense(T2t.get(), {v, v, o, o}), "baij");
   std::vector<double> eps_i(o);
   e.download(eps_i.data(), e.eps_i, size_t(o));
+  const TriplesOps op{T2, T2t.get(), e.Vt.get(), e.Vt2.get(), e.VP.get()};
 
-  DBuf Wsym(&e.pool, v * v * v), Wdist(&e.pool, v * v * v), Wtmp(&e.pool, 6 * v * v * v);
+  // cube orbits X0 <= X1 <= X2
   const int ncube = int((v + 7) / 8);
-  const int ncubes = ncube * ncube * ncube;
-  DBuf part(&e.pool, ncubes), Eb(&e.pool, blocks.size());
-  const int64_t s[3] = {1, v, v * v};
+  std::vector<int> orbit_codes;
+  for (int a = 0; a < ncube; ++a)
+    for (int b = a; b < ncube; ++b)
+      for (int c = b; c < ncube; ++c) orbit_codes.push_back(a | (b << 10) | (c << 20));
+  const int norbits = int(orbit_codes.size());
+  DBuf orbits_buf(&e.pool, (norbits + 1) / 2);
+  SISI_HIP(hipMemcpyAsync(orbits_buf.get(), orbit_codes.data(), sizeof(int) * norbits,
+                          hipMemcpyHostToDevice, e.stream));
+  DBuf W(&e.pool, v * v * v);
+  DBuf part(&e.pool, norbits), Eb(&e.pool, blocks.size());
+  zero(e, Eb.get(), blocks.size());
   std::array<std::array<int, 3>, 6> perms;
   for (int p = 0; p < 6; ++p) perms[p] = permutation3(p);
 
   for (size_t bi = 0; bi < blocks.size(); ++bi) {
     const auto &ib = blocks[bi];
+    if (ib[0] == ib[2]) continue;  // i0 = i1 = i2: sf' = 0
     bool distinct[6];
-    int equiv[6];
     std::array<int, 3> tup[6];
     for (int p = 0; p < 6; ++p) {
       for (int k = 0; k < 3; ++k) tup[p][k] = ib[perms[p][k]];  // i * pi (Permutation.hpp:93-98)
@@ -221,51 +311,23 @@ double triples_energy(Engine &e, const double *T1, const double *T2, int prec,
       for (; q < p; ++q)
         if (tup[q] == tup[p]) break;
       distinct[p] = (q == p);
-      equiv[p] = q;
     }
-    const bool degenerate = (ib[0] == ib[1]) || (ib[1] == ib[2]);
-    zero(e, Wsym.get(), v * v * v);
-    if (!degenerate) {
-      for (int p = 0; p < 6; ++p) {
-        const int64_t st[3] = {s[perms[p][0]], s[perms[p][1]], s[perms[p][2]]};
-        add_w(e, prec, T2, T2t.get(), Vt.get(), Vt2.get(), VP.get(), tup[p][0], tup[p][1], tup[p][2],
-              Wsym.get(), st);
-      }
-    } else {
-      zero(e, Wdist.get(), v * v * v);
-      for (int p = 0; p < 6; ++p) {
-        const int src = distinct[p] ? p : equiv[p];
-        double *Wp = Wtmp.get() + src * v * v * v;
-        if (distinct[p]) {
-          zero(e, Wp, v * v * v);
-          const int64_t st[3] = {1, v, v * v};
-          add_w(e, prec, T2, T2t.get(), Vt.get(), Vt2.get(), VP.get(), tup[p][0], tup[p][1],
-                tup[p][2], Wp, st);
-        }
-        View Wv = dense(Wp, {v, v, v});
-        View Ws, Wd;
-        Ws.p = Wsym.get();
-        Wd.p = Wdist.get();
-        Ws.order = Wd.order = 3;
-        for (int k = 0; k < 3; ++k) {
-          Ws.len[k] = Wd.len[k] = v;
-          Ws.stride[k] = Wd.stride[k] = s[perms[p][k]];
-        }
-        copy(e, Ws, Wv, 1.0, 1.0);
-        if (distinct[p]) copy(e, Wd, Wv, 1.0, 1.0);
-      }
-    }
+    build_w(e, prec, op, ib, perms, distinct, W.get());
+
     EnergyArgs g{};
-    g.W = Wsym.get();
-    g.Z = degenerate ? Wdist.get() : Wsym.get();
+    g.W = W.get();
     g.T1 = T1;
     g.V = e.Vabij.get();
     g.eps_a = e.eps_a.get();
+    g.orbits = reinterpret_cast<const int *>(orbits_buf.get());
     g.part = part.get();
     g.eps_ijk = eps_i[ib[0]] + eps_i[ib[1]] + eps_i[ib[2]];
     g.Nv = int(v);
     g.No = int(o);
-    g.ncube = ncube;
+    // tau: the swap of the equal positions (sig index 1 swaps 0,1; 3 swaps 1,2)
+    const int tau = ib[0] == ib[1] ? 1 : ib[1] == ib[2] ? 3 : -1;
+    for (int s = 0; s < 6; ++s)
+      g.sfp[s] = hSf[s] + (tau >= 0 ? hSf[compose_idx(s, tau)] : 0.0);
     g.nterms = 0;
     for (int p = 0; p < 6; ++p) {
       if (!distinct[p]) continue;
@@ -276,9 +338,9 @@ double triples_energy(Engine &e, const double *T1, const double *T2, int prec,
       }
     }
     {
-      Timed tm(*e.timers, Kind::Bandwidth, 8.0 * double(v * v * v) * (degenerate ? 8.0 : 7.0));
-      hipLaunchKernelGGL(triples_energy_kernel, dim3(ncubes), dim3(256), 0, e.stream, g);
-      hipLaunchKernelGGL(sum_into_kernel, dim3(1), dim3(256), 0, e.stream, part.get(), ncubes,
+      Timed tm(*e.timers, Kind::Bandwidth, 8.0 * double(v * v * v));
+      hipLaunchKernelGGL(triples_energy_kernel, dim3(norbits), dim3(256), 0, e.stream, g);
+      hipLaunchKernelGGL(sum_into_kernel, dim3(1), dim3(256), 0, e.stream, part.get(), norbits,
                          Eb.get() + bi);
       SISI_HIP(hipGetLastError());
     }
