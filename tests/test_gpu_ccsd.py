@@ -289,8 +289,8 @@ def test_ueg_known_answers():
         assert abs(e.mp2_energy() - -0.36143028565969504) <= 1e-13
         rep = e.ccsd_solve(engine.default_opts(mixer="DiisMixer", max_residua=4, max_iterations=10,
                                                amplitudes_convergence=1e-8, energy_convergence=1e-8))
-        assert rep.iterations == 10
-        assert abs(rep.energy - -0.39269658954585018) <= 1e-12
+        assert rep["iterations"] == 10
+        assert abs(rep["energy"] - -0.39269658954585018) <= 1e-12
         assert abs(e.triples() - -0.0063019625641725016) <= 1e-14
     finally:
         e.close()
