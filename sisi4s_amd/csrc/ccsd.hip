@@ -22,7 +22,9 @@ inline View V1(double *p, int64_t a) { return dense(p, {a}); }
 void set_precision(Engine &e, int precision) {
   SISI_REQUIRE(precision == SISI_FP64 || precision == SISI_FP32 || precision == SISI_BF16X3,
                SISI_ERR_INVALID, "unknown precision");
-  e.precision = precision == SISI_FP64 ? SISI_FP64 : SISI_FP32;
+  // BF16X3: every contraction in split-BF16x3 (the GEMM family itself keeps
+  // FP32 for products too small to pay for the operand split)
+  e.precision = precision;
   e.prec_ladder = precision;
 }
 

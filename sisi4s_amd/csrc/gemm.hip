@@ -48,7 +48,7 @@ __global__ __launch_bounds__(256, 2) void gemm_k4_kernel(GemmArgs g) {
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
-  const int wave = tid >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: scalar branches
   int tm, tn;
   tile_coords(g, blockIdx.x, tm, tn);
   const int ks = blockIdx.y;
@@ -62,71 +62,78 @@ __global__ __launch_bounds__(256, 2) void gemm_k4_kernel(GemmArgs g) {
 
   double ra[8], rb[8];
 
+  // Out-of-range elements load from a clamped (in-range) address, so the
+  // loads need no branch; they are zeroed when the chunk is written to LDS
+  // (multiplying by 0.0, which the compiler cannot fold away), so the loads
+  // stay in flight across the MFMAs of the previous chunk.
+  const int mlast = g.M - 1, nlast = g.N - 1;
+  int lk0 = 0;  // k0 of the chunk held in ra/rb
   auto load = [&](int k0) {
+    const int klast = kend - 1;
+    lk0 = k0;
     if (!AK) {  // A m-fastest: 32 threads x 4 consecutive m, k rows t/32 and +8
       const int mm = (tid & 31) * 4, kk = tid >> 5;
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
-        const int k = k0 + kk + 8 * h;
+        const int k = min(k0 + kk + 8 * h, klast);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const int m = m0 + mm + j;
-          ra[h * 4 + j] = (m < g.M && k < kend) ? A[int64_t(m) * g.sam + int64_t(k) * g.sak] : 0.0;
-        }
+        for (int j = 0; j < 4; ++j)
+          ra[h * 4 + j] = A[int64_t(min(m0 + mm + j, mlast)) * g.sam + int64_t(k) * g.sak];
       }
     } else {  // A k-fastest: 16 threads along k, m = t/16 + 16 j
       const int kk = tid & 15, mm = tid >> 4;
-      const int k = k0 + kk;
+      const int k = min(k0 + kk, klast);
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const int m = m0 + mm + 16 * j;
-        ra[j] = (m < g.M && k < kend) ? A[int64_t(m) * g.sam + int64_t(k) * g.sak] : 0.0;
-      }
+      for (int j = 0; j < 8; ++j)
+        ra[j] = A[int64_t(min(m0 + mm + 16 * j, mlast)) * g.sam + int64_t(k) * g.sak];
     }
     if (BNF) {  // B n-fastest
       const int nn = (tid & 31) * 4, kk = tid >> 5;
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
-        const int k = k0 + kk + 8 * h;
+        const int k = min(k0 + kk + 8 * h, klast);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const int n = n0 + nn + j;
-          rb[h * 4 + j] = (n < g.N && k < kend) ? B[int64_t(k) * g.sbk + int64_t(n) * g.sbn] : 0.0;
-        }
+        for (int j = 0; j < 4; ++j)
+          rb[h * 4 + j] = B[int64_t(k) * g.sbk + int64_t(min(n0 + nn + j, nlast)) * g.sbn];
       }
     } else {  // B k-fastest
       const int kk = tid & 15, nn = tid >> 4;
-      const int k = k0 + kk;
+      const int k = min(k0 + kk, klast);
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const int n = n0 + nn + 16 * j;
-        rb[j] = (n < g.N && k < kend) ? B[int64_t(k) * g.sbk + int64_t(n) * g.sbn] : 0.0;
-      }
+      for (int j = 0; j < 8; ++j)
+        rb[j] = B[int64_t(k) * g.sbk + int64_t(min(n0 + nn + 16 * j, nlast)) * g.sbn];
     }
   };
 
   auto store = [&]() {
+    auto z = [](bool ok) { return ok ? 1.0 : 0.0; };
     if (!AK) {
       const int mm = (tid & 31) * 4, kk = tid >> 5;
 #pragma unroll
       for (int h = 0; h < 2; ++h)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) As[kk + 8 * h][mm + j] = LT(ra[h * 4 + j]);
+        for (int j = 0; j < 4; ++j)
+          As[kk + 8 * h][mm + j] =
+              LT(ra[h * 4 + j] * z(m0 + mm + j <= mlast && lk0 + kk + 8 * h < kend));
     } else {
       const int kk = tid & 15, mm = tid >> 4;
 #pragma unroll
-      for (int j = 0; j < 8; ++j) As[kk][mm + 16 * j] = LT(ra[j]);
+      for (int j = 0; j < 8; ++j)
+        As[kk][mm + 16 * j] = LT(ra[j] * z(m0 + mm + 16 * j <= mlast && lk0 + kk < kend));
     }
     if (BNF) {
       const int nn = (tid & 31) * 4, kk = tid >> 5;
 #pragma unroll
       for (int h = 0; h < 2; ++h)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) Bs[kk + 8 * h][nn + j] = LT(rb[h * 4 + j]);
+        for (int j = 0; j < 4; ++j)
+          Bs[kk + 8 * h][nn + j] =
+              LT(rb[h * 4 + j] * z(n0 + nn + j <= nlast && lk0 + kk + 8 * h < kend));
     } else {
       const int kk = tid & 15, nn = tid >> 4;
 #pragma unroll
-      for (int j = 0; j < 8; ++j) Bs[kk][nn + 16 * j] = LT(rb[j]);
+      for (int j = 0; j < 8; ++j)
+        Bs[kk][nn + 16 * j] = LT(rb[j] * z(n0 + nn + 16 * j <= nlast && lk0 + kk < kend));
     }
   };
 
@@ -138,14 +145,10 @@ __global__ __launch_bounds__(256, 2) void gemm_k4_kernel(GemmArgs g) {
 
   const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
   const int l16 = lane & 15, lk = lane >> 4;
-
-  int k0 = kbeg;
-  if (k0 < kend) load(k0);
-  while (k0 < kend) {
-    store();
-    __syncthreads();
-    const int k1 = k0 + BK;
-    if (k1 < kend) load(k1);
+  // a wave whose 64x64 sub-tile lies wholly outside the output (ragged edge
+  // tiles) skips its MFMAs; the condition is uniform per wave
+  const bool active = (m0 + wm < g.M) && (n0 + wn < g.N);
+  auto mma = [&]() {
 #pragma unroll
     for (int kq = 0; kq < BK / 4; ++kq) {
       const int k = kq * 4 + lk;
@@ -164,6 +167,16 @@ __global__ __launch_bounds__(256, 2) void gemm_k4_kernel(GemmArgs g) {
             acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(bv[j], av[i], acc[i][j], 0, 0, 0);
         }
     }
+  };
+
+  int k0 = kbeg;
+  if (k0 < kend) load(k0);
+  while (k0 < kend) {
+    store();
+    __syncthreads();
+    const int k1 = k0 + BK;
+    if (k1 < kend) load(k1);
+    if (active) mma();
     __syncthreads();
     k0 = k1;
   }
@@ -221,7 +234,7 @@ __global__ __launch_bounds__(256, 2) void gemm_bf16x3_kernel(GemmArgs g) {
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
-  const int wave = tid >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: scalar branches
   int tm, tn;
   tile_coords(g, blockIdx.x, tm, tn);
   const int ks = blockIdx.y;
@@ -234,7 +247,13 @@ __global__ __launch_bounds__(256, 2) void gemm_bf16x3_kernel(GemmArgs g) {
 
   // 128 x 32 elements per operand = 16 per thread
   double ra[16], rb[16];
+  // Out-of-range elements load from a clamped (in-range) address (no
+  // branch) and are zeroed at the LDS store, one chunk later.
+  const int mlast = g.M - 1, nlast = g.N - 1;
+  int lk0 = 0;
   auto load = [&](int k0) {
+    const int klast = kend - 1;
+    lk0 = k0;
     if (!AK) {  // m fastest: 32 threads x 4 m, 8 k rows, 4 passes
       const int mm = (tid & 31) * 4, kk = tid >> 5;
 #pragma unroll
@@ -243,7 +262,7 @@ __global__ __launch_bounds__(256, 2) void gemm_bf16x3_kernel(GemmArgs g) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           const int m = m0 + mm + j;
-          ra[h * 4 + j] = (m < g.M && k < kend) ? A[int64_t(m) * g.sam + int64_t(k) * g.sak] : 0.0;
+          ra[h * 4 + j] = A[int64_t(min(m, mlast)) * g.sam + int64_t(min(k, klast)) * g.sak];
         }
       }
     } else {  // k fastest: 32 threads along k, m = t/32 + 8 j
@@ -252,7 +271,7 @@ __global__ __launch_bounds__(256, 2) void gemm_bf16x3_kernel(GemmArgs g) {
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
         const int m = m0 + mm + 8 * j;
-        ra[j] = (m < g.M && k < kend) ? A[int64_t(m) * g.sam + int64_t(k) * g.sak] : 0.0;
+        ra[j] = A[int64_t(min(m, mlast)) * g.sam + int64_t(min(k, klast)) * g.sak];
       }
     }
     if (BNF) {
@@ -263,7 +282,7 @@ __global__ __launch_bounds__(256, 2) void gemm_bf16x3_kernel(GemmArgs g) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           const int n = n0 + nn + j;
-          rb[h * 4 + j] = (n < g.N && k < kend) ? B[int64_t(k) * g.sbk + int64_t(n) * g.sbn] : 0.0;
+          rb[h * 4 + j] = B[int64_t(min(k, klast)) * g.sbk + int64_t(min(n, nlast)) * g.sbn];
         }
       }
     } else {
@@ -272,7 +291,7 @@ __global__ __launch_bounds__(256, 2) void gemm_bf16x3_kernel(GemmArgs g) {
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
         const int n = n0 + nn + 8 * j;
-        rb[j] = (n < g.N && k < kend) ? B[int64_t(k) * g.sbk + int64_t(n) * g.sbn] : 0.0;
+        rb[j] = B[int64_t(min(k, klast)) * g.sbk + int64_t(min(n, nlast)) * g.sbn];
       }
     }
   };
@@ -284,7 +303,7 @@ __global__ __launch_bounds__(256, 2) void gemm_bf16x3_kernel(GemmArgs g) {
       for (int hh = 0; hh < 4; ++hh)
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-          split_bf16(ra[hh * 4 + j], h, l);
+          split_bf16(ra[hh * 4 + j] * ((m0 + mm + j <= mlast && lk0 + kk + 8 * hh < kend) ? 1.0 : 0.0), h, l);
           Ah[(mm + j) * RS + kk + 8 * hh] = h;
           Al[(mm + j) * RS + kk + 8 * hh] = l;
         }
@@ -292,7 +311,7 @@ __global__ __launch_bounds__(256, 2) void gemm_bf16x3_kernel(GemmArgs g) {
       const int kk = tid & 31, mm = tid >> 5;
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
-        split_bf16(ra[j], h, l);
+        split_bf16(ra[j] * ((m0 + mm + 8 * j <= mlast && lk0 + kk < kend) ? 1.0 : 0.0), h, l);
         Ah[(mm + 8 * j) * RS + kk] = h;
         Al[(mm + 8 * j) * RS + kk] = l;
       }
@@ -303,7 +322,7 @@ __global__ __launch_bounds__(256, 2) void gemm_bf16x3_kernel(GemmArgs g) {
       for (int hh = 0; hh < 4; ++hh)
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-          split_bf16(rb[hh * 4 + j], h, l);
+          split_bf16(rb[hh * 4 + j] * ((n0 + nn + j <= nlast && lk0 + kk + 8 * hh < kend) ? 1.0 : 0.0), h, l);
           Bh[(nn + j) * RS + kk + 8 * hh] = h;
           Bl[(nn + j) * RS + kk + 8 * hh] = l;
         }
@@ -311,7 +330,7 @@ __global__ __launch_bounds__(256, 2) void gemm_bf16x3_kernel(GemmArgs g) {
       const int kk = tid & 31, nn = tid >> 5;
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
-        split_bf16(rb[j], h, l);
+        split_bf16(rb[j] * ((n0 + nn + 8 * j <= nlast && lk0 + kk < kend) ? 1.0 : 0.0), h, l);
         Bh[(nn + 8 * j) * RS + kk] = h;
         Bl[(nn + 8 * j) * RS + kk] = l;
       }
@@ -451,6 +470,9 @@ void gemm_reduce(Engine &eng, const GemmArgs &g) {
 
 void gemm(Engine &eng, int precision, GemmArgs g) {
   if (g.M <= 0 || g.N <= 0 || g.batch <= 0) return;
+  // split-BF16x3 splits both operands first (a bandwidth pass over each); a
+  // product too small to amortise that runs on FP32 MFMA instead
+  if (precision == SISI_BF16X3 && gemm_flops(g) < kB3MinFlops) precision = SISI_FP32;
   if (precision == SISI_BF16X3 && g.batch == 1 && g.K > 0) {
     static const int env_b3 = getenv("SISI_B3_PRESPLIT") ? atoi(getenv("SISI_B3_PRESPLIT")) : 1;
     if (env_b3) {

@@ -71,6 +71,10 @@ void gemm_b3(Engine &eng, GemmArgs g);
 // Split-K reduction of g.work into C (deterministic order).
 void gemm_reduce(Engine &eng, const GemmArgs &g);
 
+// Products below this many flops run on FP32 MFMA when split-BF16x3 is asked
+// for (the split's extra pass over the operands would dominate).
+constexpr double kB3MinFlops = 1e10;
+
 // Algorithmic flops of one call (2*M*N*K*batch).
 inline double gemm_flops(const GemmArgs &g) {
   return 2.0 * double(g.M) * double(g.N) * double(g.K) * double(g.batch);

@@ -41,86 +41,93 @@ __global__ __launch_bounds__(256, 2) void gemm_seg_k4_kernel(SegGemmArgs a) {
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
-  const int wave = tid >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: scalar branches
   int tm, tn;
   tile_coords(g, blockIdx.x, tm, tn);
   const int m0 = tm * BM, n0 = tn * BN;
 
   double ra[8], rb[8];
   int lak = 0, lbnf = 0;
+  // Out-of-range elements load from a clamped (in-range) address, so the
+  // loads need no branch; at the LDS store they are multiplied by zero (the
+  // segment's scale otherwise), one chunk later, so the loads stay in flight
+  // across the MFMAs of the previous chunk.
+  const int mlast = g.M - 1, nlast = g.N - 1;
+  int lk0 = 0, lK = 0;
+  double lsc = 1.0;
 
   auto load = [&](int s, int k0) {
     const GemmSeg sg = seg_at(a, s);
-    const int K = sg.K;
-    const double sc = sg.scale;
+    const int klast = sg.K - 1;
     lak = sg.ak;
     lbnf = sg.bnf;
+    lk0 = k0;
+    lK = sg.K;
+    lsc = sg.scale;
     if (!sg.ak) {  // A m-fastest: 32 threads x 4 consecutive m, k rows t/32 and +8
       const int mm = (tid & 31) * 4, kk = tid >> 5;
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
-        const int k = k0 + kk + 8 * h;
+        const int k = min(k0 + kk + 8 * h, klast);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const int m = m0 + mm + j;
-          ra[h * 4 + j] =
-              (m < g.M && k < K) ? sc * sg.A[int64_t(m) * sg.sam + int64_t(k) * sg.sak] : 0.0;
-        }
+        for (int j = 0; j < 4; ++j)
+          ra[h * 4 + j] = sg.A[int64_t(min(m0 + mm + j, mlast)) * sg.sam + int64_t(k) * sg.sak];
       }
     } else {  // A k-fastest: 16 threads along k, m = t/16 + 16 j
       const int kk = tid & 15, mm = tid >> 4;
-      const int k = k0 + kk;
+      const int k = min(k0 + kk, klast);
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const int m = m0 + mm + 16 * j;
-        ra[j] = (m < g.M && k < K) ? sc * sg.A[int64_t(m) * sg.sam + int64_t(k) * sg.sak] : 0.0;
-      }
+      for (int j = 0; j < 8; ++j)
+        ra[j] = sg.A[int64_t(min(m0 + mm + 16 * j, mlast)) * sg.sam + int64_t(k) * sg.sak];
     }
     if (sg.bnf) {  // B n-fastest
       const int nn = (tid & 31) * 4, kk = tid >> 5;
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
-        const int k = k0 + kk + 8 * h;
+        const int k = min(k0 + kk + 8 * h, klast);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const int n = n0 + nn + j;
-          rb[h * 4 + j] =
-              (n < g.N && k < K) ? sg.B[int64_t(k) * sg.sbk + int64_t(n) * sg.sbn] : 0.0;
-        }
+        for (int j = 0; j < 4; ++j)
+          rb[h * 4 + j] = sg.B[int64_t(k) * sg.sbk + int64_t(min(n0 + nn + j, nlast)) * sg.sbn];
       }
     } else {  // B k-fastest
       const int kk = tid & 15, nn = tid >> 4;
-      const int k = k0 + kk;
+      const int k = min(k0 + kk, klast);
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const int n = n0 + nn + 16 * j;
-        rb[j] = (n < g.N && k < K) ? sg.B[int64_t(k) * sg.sbk + int64_t(n) * sg.sbn] : 0.0;
-      }
+      for (int j = 0; j < 8; ++j)
+        rb[j] = sg.B[int64_t(k) * sg.sbk + int64_t(min(n0 + nn + 16 * j, nlast)) * sg.sbn];
     }
   };
 
   auto store = [&]() {
+    auto za = [&](bool ok) { return ok ? lsc : 0.0; };
+    auto zb = [](bool ok) { return ok ? 1.0 : 0.0; };
     if (!lak) {
       const int mm = (tid & 31) * 4, kk = tid >> 5;
 #pragma unroll
       for (int h = 0; h < 2; ++h)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) As[kk + 8 * h][mm + j] = LT(ra[h * 4 + j]);
+        for (int j = 0; j < 4; ++j)
+          As[kk + 8 * h][mm + j] =
+              LT(ra[h * 4 + j] * za(m0 + mm + j <= mlast && lk0 + kk + 8 * h < lK));
     } else {
       const int kk = tid & 15, mm = tid >> 4;
 #pragma unroll
-      for (int j = 0; j < 8; ++j) As[kk][mm + 16 * j] = LT(ra[j]);
+      for (int j = 0; j < 8; ++j)
+        As[kk][mm + 16 * j] = LT(ra[j] * za(m0 + mm + 16 * j <= mlast && lk0 + kk < lK));
     }
     if (lbnf) {
       const int nn = (tid & 31) * 4, kk = tid >> 5;
 #pragma unroll
       for (int h = 0; h < 2; ++h)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) Bs[kk + 8 * h][nn + j] = LT(rb[h * 4 + j]);
+        for (int j = 0; j < 4; ++j)
+          Bs[kk + 8 * h][nn + j] =
+              LT(rb[h * 4 + j] * zb(n0 + nn + j <= nlast && lk0 + kk + 8 * h < lK));
     } else {
       const int kk = tid & 15, nn = tid >> 4;
 #pragma unroll
-      for (int j = 0; j < 8; ++j) Bs[kk][nn + 16 * j] = LT(rb[j]);
+      for (int j = 0; j < 8; ++j)
+        Bs[kk][nn + 16 * j] = LT(rb[j] * zb(n0 + nn + 16 * j <= nlast && lk0 + kk < lK));
     }
   };
 
@@ -132,6 +139,8 @@ __global__ __launch_bounds__(256, 2) void gemm_seg_k4_kernel(SegGemmArgs a) {
 
   const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
   const int l16 = lane & 15, lk = lane >> 4;
+  // a wave whose 64x64 sub-tile lies wholly outside the output skips its MFMAs
+  const bool active = (m0 + wm < g.M) && (n0 + wn < g.N);
 
   int s = 0, k0 = 0;
   bool have = a.nseg > 0;
@@ -146,23 +155,25 @@ __global__ __launch_bounds__(256, 2) void gemm_seg_k4_kernel(SegGemmArgs a) {
     }
     const bool more = s1 < a.nseg;
     if (more) load(s1, k1);
+    if (active) {
 #pragma unroll
-    for (int kq = 0; kq < BK / 4; ++kq) {
-      const int k = kq * 4 + lk;
-      LT av[4], bv[4];
+      for (int kq = 0; kq < BK / 4; ++kq) {
+        const int k = kq * 4 + lk;
+        LT av[4], bv[4];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) av[i] = As[k][wm + i * 16 + l16];
+        for (int i = 0; i < 4; ++i) av[i] = As[k][wm + i * 16 + l16];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) bv[j] = Bs[k][wn + j * 16 + l16];
+        for (int j = 0; j < 4; ++j) bv[j] = Bs[k][wn + j * 16 + l16];
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+        for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          if constexpr (PREC == SISI_FP64)
-            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j], av[i], acc[i][j], 0, 0, 0);
-          else
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(bv[j], av[i], acc[i][j], 0, 0, 0);
-        }
+          for (int j = 0; j < 4; ++j) {
+            if constexpr (PREC == SISI_FP64)
+              acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j], av[i], acc[i][j], 0, 0, 0);
+            else
+              acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(bv[j], av[i], acc[i][j], 0, 0, 0);
+          }
+      }
     }
     __syncthreads();
     s = s1;
@@ -201,19 +212,26 @@ __global__ __launch_bounds__(256, 2) void gemm_seg_b3_kernel(SegGemmArgs a) {
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
-  const int wave = tid >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: scalar branches
   int tm, tn;
   tile_coords(g, blockIdx.x, tm, tn);
   const int m0 = tm * BM, n0 = tn * BN;
 
   double ra[16], rb[16];
   int lak = 0, lbnf = 0;
+  // Out-of-range elements load from a clamped (in-range) address (no
+  // branch) and are zeroed at the LDS store, one chunk later.
+  const int mlast = g.M - 1, nlast = g.N - 1;
+  int lk0 = 0, lK = 0;
+  double lsc = 1.0;
   auto load = [&](int s, int k0) {
     const GemmSeg sg = seg_at(a, s);
-    const int K = sg.K;
-    const double sc = sg.scale;
+    const int klast = sg.K - 1;
     lak = sg.ak;
     lbnf = sg.bnf;
+    lk0 = k0;
+    lK = sg.K;
+    lsc = sg.scale;
     if (!sg.ak) {  // m fastest: 32 threads x 4 m, 8 k rows, 4 passes
       const int mm = (tid & 31) * 4, kk = tid >> 5;
 #pragma unroll
@@ -222,8 +240,7 @@ __global__ __launch_bounds__(256, 2) void gemm_seg_b3_kernel(SegGemmArgs a) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           const int m = m0 + mm + j;
-          ra[h * 4 + j] =
-              (m < g.M && k < K) ? sc * sg.A[int64_t(m) * sg.sam + int64_t(k) * sg.sak] : 0.0;
+          ra[h * 4 + j] = sg.A[int64_t(min(m, mlast)) * sg.sam + int64_t(min(k, klast)) * sg.sak];
         }
       }
     } else {  // k fastest: 32 threads along k, m = t/32 + 8 j
@@ -232,7 +249,7 @@ __global__ __launch_bounds__(256, 2) void gemm_seg_b3_kernel(SegGemmArgs a) {
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
         const int m = m0 + mm + 8 * j;
-        ra[j] = (m < g.M && k < K) ? sc * sg.A[int64_t(m) * sg.sam + int64_t(k) * sg.sak] : 0.0;
+        ra[j] = sg.A[int64_t(min(m, mlast)) * sg.sam + int64_t(min(k, klast)) * sg.sak];
       }
     }
     if (sg.bnf) {
@@ -243,8 +260,7 @@ __global__ __launch_bounds__(256, 2) void gemm_seg_b3_kernel(SegGemmArgs a) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           const int n = n0 + nn + j;
-          rb[h * 4 + j] =
-              (n < g.N && k < K) ? sg.B[int64_t(k) * sg.sbk + int64_t(n) * sg.sbn] : 0.0;
+          rb[h * 4 + j] = sg.B[int64_t(min(k, klast)) * sg.sbk + int64_t(min(n, nlast)) * sg.sbn];
         }
       }
     } else {
@@ -253,7 +269,7 @@ __global__ __launch_bounds__(256, 2) void gemm_seg_b3_kernel(SegGemmArgs a) {
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
         const int n = n0 + nn + 8 * j;
-        rb[j] = (n < g.N && k < K) ? sg.B[int64_t(k) * sg.sbk + int64_t(n) * sg.sbn] : 0.0;
+        rb[j] = sg.B[int64_t(min(k, klast)) * sg.sbk + int64_t(min(n, nlast)) * sg.sbn];
       }
     }
   };
@@ -265,7 +281,7 @@ __global__ __launch_bounds__(256, 2) void gemm_seg_b3_kernel(SegGemmArgs a) {
       for (int hh = 0; hh < 4; ++hh)
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-          split_bf16(ra[hh * 4 + j], h, l);
+          split_bf16(ra[hh * 4 + j] * ((m0 + mm + j <= mlast && lk0 + kk + 8 * hh < lK) ? lsc : 0.0), h, l);
           Ah[(mm + j) * RS + kk + 8 * hh] = h;
           Al[(mm + j) * RS + kk + 8 * hh] = l;
         }
@@ -273,7 +289,7 @@ __global__ __launch_bounds__(256, 2) void gemm_seg_b3_kernel(SegGemmArgs a) {
       const int kk = tid & 31, mm = tid >> 5;
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
-        split_bf16(ra[j], h, l);
+        split_bf16(ra[j] * ((m0 + mm + 8 * j <= mlast && lk0 + kk < lK) ? lsc : 0.0), h, l);
         Ah[(mm + 8 * j) * RS + kk] = h;
         Al[(mm + 8 * j) * RS + kk] = l;
       }
@@ -284,7 +300,7 @@ __global__ __launch_bounds__(256, 2) void gemm_seg_b3_kernel(SegGemmArgs a) {
       for (int hh = 0; hh < 4; ++hh)
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-          split_bf16(rb[hh * 4 + j], h, l);
+          split_bf16(rb[hh * 4 + j] * ((n0 + nn + j <= nlast && lk0 + kk + 8 * hh < lK) ? 1.0 : 0.0), h, l);
           Bh[(nn + j) * RS + kk + 8 * hh] = h;
           Bl[(nn + j) * RS + kk + 8 * hh] = l;
         }
@@ -292,7 +308,7 @@ __global__ __launch_bounds__(256, 2) void gemm_seg_b3_kernel(SegGemmArgs a) {
       const int kk = tid & 31, nn = tid >> 5;
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
-        split_bf16(rb[j], h, l);
+        split_bf16(rb[j] * ((n0 + nn + 8 * j <= nlast && lk0 + kk < lK) ? 1.0 : 0.0), h, l);
         Bh[(nn + 8 * j) * RS + kk] = h;
         Bl[(nn + 8 * j) * RS + kk] = l;
       }
