@@ -97,8 +97,8 @@ __device__ __forceinline__ int64_t b3_off_n(const GemmArgs &g, int n) {
 // BM x BN block, 4 waves of WM x WN, k-chunk 32, planes staged through LDS
 // with 16-byte stores.  NBUF register sets keep NBUF chunks of global loads in
 // flight, so a chunk's load latency is covered by NBUF chunks of MFMA work.
-template <int BM, int BN, int WM, int WN, int NBUF>
-__global__ __launch_bounds__(256) void gemm_b3p_kernel(B3Args p) {
+template <int BM, int BN, int WM, int WN, int NBUF, int MINB>
+__global__ __launch_bounds__(256, MINB) void gemm_b3p_kernel(B3Args p) {
   constexpr int BK = 32, RS = BK + 8;  // LDS row stride in bf16 (80 bytes)
   constexpr int NA = BM * 4 / 256, NB = BN * 4 / 256;  // 16-byte chunks per thread per plane
   static_assert((BM / WM) * (BN / WN) == 4, "four waves");
@@ -166,10 +166,12 @@ __global__ __launch_bounds__(256) void gemm_b3p_kernel(B3Args p) {
   const int wm = (wave / WX) * WM, wn = (wave % WX) * WN;
   const int l16 = lane & 15, lk = lane >> 4;
 
-  // Three passes (hi*lo, lo*hi, hi*hi) over all tiles: consecutive MFMAs
-  // update different accumulators, so none waits on the previous result.
+  // B fragments stay in registers for the chunk; A fragments are read one
+  // 16-row tile at a time.  Per row tile, three passes (hi*lo, lo*hi, hi*hi)
+  // over the TJ column tiles, so consecutive MFMAs update different
+  // accumulators and none waits on the previous result.
   auto compute = [&]() {
-    bf8 bh[TJ], bl[TJ], ah[TI], al[TI];
+    bf8 bh[TJ], bl[TJ];
 #pragma unroll
     for (int j = 0; j < TJ; ++j) {
       const int col = wn + j * 16 + l16;
@@ -179,24 +181,18 @@ __global__ __launch_bounds__(256) void gemm_b3p_kernel(B3Args p) {
 #pragma unroll
     for (int i = 0; i < TI; ++i) {
       const int row = wm + i * 16 + l16;
-      ah[i] = *reinterpret_cast<const bf8 *>(&Ah[row * RS + 8 * lk]);
-      al[i] = *reinterpret_cast<const bf8 *>(&Al[row * RS + 8 * lk]);
+      const bf8 ah = *reinterpret_cast<const bf8 *>(&Ah[row * RS + 8 * lk]);
+      const bf8 al = *reinterpret_cast<const bf8 *>(&Al[row * RS + 8 * lk]);
+#pragma unroll
+      for (int j = 0; j < TJ; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[j], ah, acc[i][j], 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < TJ; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], al, acc[i][j], 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < TJ; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], ah, acc[i][j], 0, 0, 0);
     }
-#pragma unroll
-    for (int i = 0; i < TI; ++i)
-#pragma unroll
-      for (int j = 0; j < TJ; ++j)
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[j], ah[i], acc[i][j], 0, 0, 0);
-#pragma unroll
-    for (int i = 0; i < TI; ++i)
-#pragma unroll
-      for (int j = 0; j < TJ; ++j)
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], al[i], acc[i][j], 0, 0, 0);
-#pragma unroll
-    for (int i = 0; i < TI; ++i)
-#pragma unroll
-      for (int j = 0; j < TJ; ++j)
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], ah[i], acc[i][j], 0, 0, 0);
   };
 
 #pragma unroll
@@ -303,7 +299,7 @@ const Planes &split_operand(Engine &e, const double *src, int64_t sr, int64_t sk
 
 }  // namespace
 
-template <int BM, int BN, int WM, int WN, int NBUF>
+template <int BM, int BN, int WM, int WN, int NBUF, int MINB = 1>
 static void run_b3(Engine &e, GemmArgs g) {
   const int Kp = ((g.K + 31) / 32) * 32;
   const int Mp = ((g.M + BM - 1) / BM) * BM, Np = ((g.N + BN - 1) / BN) * BN;
@@ -333,20 +329,22 @@ static void run_b3(Engine &e, GemmArgs g) {
   dim3 grid(unsigned(tiles), unsigned(ksplit), 1);
   {
     Timed t(*e.timers, Kind::GemmB3, gemm_flops(g));
-    hipLaunchKernelGGL((gemm_b3p_kernel<BM, BN, WM, WN, NBUF>), grid, dim3(256), 0, e.stream, p);
+    hipLaunchKernelGGL((gemm_b3p_kernel<BM, BN, WM, WN, NBUF, MINB>), grid, dim3(256), 0, e.stream, p);
     SISI_HIP(hipGetLastError());
   }
   if (ksplit > 1) gemm_reduce(e, g);
 }
 
 void gemm_b3(Engine &e, GemmArgs g) {
-  static const int cfg = getenv("SISI_B3_CFG") ? atoi(getenv("SISI_B3_CFG")) : 0;
+  // 128x128 blocks of four 64x64 waves, two workgroups per CU, measured
+  // fastest on the engine's shapes; the others are kept for measurement
+  static const int cfg = getenv("SISI_B3_CFG") ? atoi(getenv("SISI_B3_CFG")) : 2;
   switch (cfg) {
+    case 0: run_b3<256, 128, 128, 64, 2>(e, g); break;
     case 1: run_b3<256, 128, 128, 64, 1>(e, g); break;
-    case 2: run_b3<128, 128, 64, 64, 1>(e, g); break;
     case 3: run_b3<128, 128, 64, 64, 2>(e, g); break;
-    case 4: run_b3<256, 128, 128, 64, 3>(e, g); break;
-    default: run_b3<256, 128, 128, 64, 2>(e, g); break;
+    case 5: run_b3<128, 128, 64, 64, 1, 3>(e, g); break;
+    default: run_b3<128, 128, 64, 64, 1>(e, g); break;
   }
 }
 
