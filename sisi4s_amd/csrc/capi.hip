@@ -102,9 +102,7 @@ int sisi_engine_set_vertex(sisi_engine *h, const double *gamma) {
     e.upload(e.gamma, gamma, n);
     e.have_vertex = true;
     e.have_integrals = false;
-    e.Vt.reset();
-    e.Vt2.reset();
-    e.VP.reset();
+    e.drop_triples_cache();
   });
 }
 

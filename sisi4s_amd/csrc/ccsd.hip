@@ -58,9 +58,7 @@ void integrals_from_vertex(Engine &e) {
   // CcsdEnergyFromCoulombIntegrals.cxx:146-147, 172-173, 462-463
   axpby(e, v * v * o * o, 2.0, e.Vabij.get(), 0.0, e.Uabij.get());
   axpby(e, v * v * o * o, -1.0, e.Vbaij.get(), 1.0, e.Uabij.get());
-  e.Vt.reset();
-  e.Vt2.reset();
-  e.VP.reset();
+  e.drop_triples_cache();
   e.have_integrals = true;
   e.sync();
 }

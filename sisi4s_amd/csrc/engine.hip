@@ -118,9 +118,7 @@ Engine::~Engine() {
   Vijka.reset();
   Vbaij.reset();
   Uabij.reset();
-  Vt.reset();
-  Vt2.reset();
-  VP.reset();
+  drop_triples_cache();
   pool.release_all();
   if (stream) (void)hipStreamDestroy(stream);
 }

@@ -128,6 +128,17 @@ class Engine {
   // (T) integrals, built on first use: V_bdck as [d,b,c,k] and as [d,c,b,k],
   // V_jklc as [l,c,j,k]
   DBuf Vt, Vt2, VP;
+  // the same operands as split-BF16x3 planes (V_jklc with the sign of its
+  // term folded in), built on the first reduced-precision (T) call
+  B3Planes Vt_b3, Vt2_b3, VP_b3;
+  void drop_triples_cache() {
+    Vt.reset();
+    Vt2.reset();
+    VP.reset();
+    Vt_b3 = B3Planes();
+    Vt2_b3 = B3Planes();
+    VP_b3 = B3Planes();
+  }
 
   // refinement anchor (recorded by an FP64 residual when `recording` is set)
   Anchor anchor;

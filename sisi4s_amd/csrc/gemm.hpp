@@ -62,6 +62,37 @@ class Engine;
 // Segmented GEMM on the engine's stream, precision SISI_FP64/_FP32/_BF16X3.
 void gemm_segments(Engine &eng, int precision, SegGemmArgs a);
 
+// An operand split once into bf16 hi/lo planes for split-BF16x3: `batch`
+// matrices of Rp rows by Kp bf16 (k contiguous, zero padded), matrix b's row
+// r at hi + (b*Rp + r)*Kp (lo likewise).
+struct B3Planes {
+  DBuf store;
+  const unsigned short *hi = nullptr, *lo = nullptr;
+  int64_t Rp = 0, Kp = 0, batch = 0;
+  const unsigned short *hi_of(int64_t b) const { return hi + b * Rp * Kp; }
+  const unsigned short *lo_of(int64_t b) const { return lo + b * Rp * Kp; }
+};
+
+// Split `batch` (R x K) views (element (r,k) of matrix b at
+// src[b*bsrc + r*sr + k*sk]), times `scale`, into planes of Rp >= R rows.
+void split_planes(Engine &e, const double *src, int64_t sr, int64_t sk, int R, int K, int Rp,
+                  int batch, int64_t bsrc, double scale, B3Planes &out);
+
+// Segmented split-BF16x3 GEMM on pre-split planes: C = alpha * sum_s A_s B_s
+// + beta * C, A_s rows = output rows (at least the tile-padded M), B_s rows =
+// output columns; each segment has its own planes and Kp.
+struct B3Seg {
+  const unsigned short *Ah = nullptr, *Al = nullptr, *Bh = nullptr, *Bl = nullptr;
+  int Kp = 0;
+};
+struct B3SegArgs {
+  GemmArgs g;
+  int nseg = 0;
+  B3Seg seg[kMaxSegs];
+};
+constexpr int kB3SegTile = 128;  // rows of A and B planes must cover the tile-padded M, N
+void gemm_b3_segments(Engine &e, B3SegArgs a);
+
 // Launch on the engine's stream.  precision: SISI_FP64/_FP32/_BF16X3.
 void gemm(Engine &eng, int precision, GemmArgs g);
 

@@ -27,6 +27,8 @@ struct SplitArgs {
   int64_t sr, sk;  // element (r,k) at src[r*sr + k*sk]
   int R, K, Rp, Kp;
   u16 *hi, *lo;
+  int64_t bsrc = 0, bdst = 0;  // batch strides (grid z) of source and planes
+  double scale = 1.0;          // applied before the split (exact for +-1)
 };
 
 __device__ __forceinline__ u16 bf16_bits(float f) {
@@ -40,17 +42,21 @@ __global__ __launch_bounds__(256) void split_bf16_kernel(SplitArgs a) {
   __shared__ float t[64][65];
   const int r0 = blockIdx.y * 64, k0 = blockIdx.x * 64;
   const int tid = threadIdx.x;
+  a.src += int64_t(blockIdx.z) * a.bsrc;
+  a.hi += int64_t(blockIdx.z) * a.bdst;
+  a.lo += int64_t(blockIdx.z) * a.bdst;
   if (a.sk == 1) {
     for (int e = tid; e < 4096; e += 256) {
       const int kk = e & 63, rr = e >> 6;
       const int r = r0 + rr, k = k0 + kk;
-      t[rr][kk] = (r < a.R && k < a.K) ? float(a.src[int64_t(r) * a.sr + k]) : 0.f;
+      t[rr][kk] = (r < a.R && k < a.K) ? float(a.scale * a.src[int64_t(r) * a.sr + k]) : 0.f;
     }
   } else {
     for (int e = tid; e < 4096; e += 256) {
       const int rr = e & 63, kk = e >> 6;
       const int r = r0 + rr, k = k0 + kk;
-      t[rr][kk] = (r < a.R && k < a.K) ? float(a.src[int64_t(r) * a.sr + int64_t(k) * a.sk]) : 0.f;
+      t[rr][kk] =
+          (r < a.R && k < a.K) ? float(a.scale * a.src[int64_t(r) * a.sr + int64_t(k) * a.sk]) : 0.f;
     }
   }
   __syncthreads();
@@ -251,6 +257,142 @@ __global__ __launch_bounds__(256, MINB) void gemm_b3p_kernel(B3Args p) {
   }
 }
 
+// Segmented form of gemm_b3p_kernel<128,128,64,64,1>: the chunks of all
+// segments are walked in order with the next chunk (possibly of the next
+// segment) prefetched into registers while the current one is multiplied.
+__device__ __forceinline__ B3Seg b3seg_at(const B3SegArgs &a, int s) {
+  return s == 0 ? a.seg[0] : s == 1 ? a.seg[1] : s == 2 ? a.seg[2] : a.seg[3];
+}
+
+__global__ __launch_bounds__(256) void gemm_b3p_seg_kernel(B3SegArgs p) {
+  constexpr int BM = 128, BN = 128, WM = 64, WN = 64;
+  constexpr int BK = 32, RS = BK + 8;
+  constexpr int NA = BM * 4 / 256, NB = BN * 4 / 256;
+  __shared__ __attribute__((aligned(16))) u16 Ah[BM * RS];
+  __shared__ __attribute__((aligned(16))) u16 Al[BM * RS];
+  __shared__ __attribute__((aligned(16))) u16 Bh[BN * RS];
+  __shared__ __attribute__((aligned(16))) u16 Bl[BN * RS];
+  const GemmArgs &g = p.g;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  int pid = blockIdx.x;
+  {
+    const int tiles = g.tiles_m * g.tiles_n, per = tiles >> 3;
+    if (g.xcd_map && pid < (per << 3)) pid = (pid & 7) * per + (pid >> 3);
+  }
+  const int per_group = 8 * g.tiles_n, gid = pid / per_group, first_m = gid * 8;
+  const int gsize = min(g.tiles_m - first_m, 8), rr = pid % per_group;
+  const int tm = first_m + rr % gsize, tn = rr / gsize;
+  const int m0 = tm * BM, n0 = tn * BN;
+
+  U128 ra[2 * NA], rb[2 * NB];
+  auto load = [&](int s, int c) {
+    const B3Seg sg = b3seg_at(p, s);
+    const int64_t Kp = sg.Kp;
+    const int kb = c * BK;
+#pragma unroll
+    for (int i = 0; i < NA; ++i) {
+      const int q = tid + 256 * i, row = q >> 2, kq = (q & 3) * 8;
+      const int64_t off = int64_t(m0 + row) * Kp + kb + kq;
+      ra[i] = *reinterpret_cast<const U128 *>(sg.Ah + off);
+      ra[NA + i] = *reinterpret_cast<const U128 *>(sg.Al + off);
+    }
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      const int q = tid + 256 * i, row = q >> 2, kq = (q & 3) * 8;
+      const int64_t off = int64_t(n0 + row) * Kp + kb + kq;
+      rb[i] = *reinterpret_cast<const U128 *>(sg.Bh + off);
+      rb[NB + i] = *reinterpret_cast<const U128 *>(sg.Bl + off);
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int i = 0; i < NA; ++i) {
+      const int q = tid + 256 * i, row = q >> 2, kq = (q & 3) * 8;
+      *reinterpret_cast<U128 *>(&Ah[row * RS + kq]) = ra[i];
+      *reinterpret_cast<U128 *>(&Al[row * RS + kq]) = ra[NA + i];
+    }
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      const int q = tid + 256 * i, row = q >> 2, kq = (q & 3) * 8;
+      *reinterpret_cast<U128 *>(&Bh[row * RS + kq]) = rb[i];
+      *reinterpret_cast<U128 *>(&Bl[row * RS + kq]) = rb[NB + i];
+    }
+  };
+
+  constexpr int TI = WM / 16, TJ = WN / 16;
+  f4 acc[TI][TJ];
+#pragma unroll
+  for (int i = 0; i < TI; ++i)
+#pragma unroll
+    for (int j = 0; j < TJ; ++j) acc[i][j] = f4{0, 0, 0, 0};
+  const int wm = (wave >> 1) * WM, wn = (wave & 1) * WN;
+  const int l16 = lane & 15, lk = lane >> 4;
+  auto compute = [&]() {
+    bf8 bh[TJ], bl[TJ];
+#pragma unroll
+    for (int j = 0; j < TJ; ++j) {
+      const int col = wn + j * 16 + l16;
+      bh[j] = *reinterpret_cast<const bf8 *>(&Bh[col * RS + 8 * lk]);
+      bl[j] = *reinterpret_cast<const bf8 *>(&Bl[col * RS + 8 * lk]);
+    }
+#pragma unroll
+    for (int i = 0; i < TI; ++i) {
+      const int row = wm + i * 16 + l16;
+      const bf8 ah = *reinterpret_cast<const bf8 *>(&Ah[row * RS + 8 * lk]);
+      const bf8 al = *reinterpret_cast<const bf8 *>(&Al[row * RS + 8 * lk]);
+#pragma unroll
+      for (int j = 0; j < TJ; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[j], ah, acc[i][j], 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < TJ; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], al, acc[i][j], 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < TJ; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], ah, acc[i][j], 0, 0, 0);
+    }
+  };
+
+  int s = 0, c = 0;
+  bool have = p.nseg > 0;
+  if (have) load(0, 0);
+  while (have) {
+    store();
+    __syncthreads();
+    int s1 = s, c1 = c + 1;
+    if (c1 * BK >= b3seg_at(p, s).Kp) {
+      s1 = s + 1;
+      c1 = 0;
+    }
+    const bool more = s1 < p.nseg;
+    if (more) load(s1, c1);
+    compute();
+    __syncthreads();
+    s = s1;
+    c = c1;
+    have = more;
+  }
+
+#pragma unroll
+  for (int i = 0; i < TI; ++i) {
+    const int m = m0 + wm + i * 16 + l16;
+    if (m >= g.M) continue;
+    const int64_t om = b3_off_m(g, m);
+#pragma unroll
+    for (int j = 0; j < TJ; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = n0 + wn + j * 16 + 4 * lk + r;
+        if (n < g.N) {
+          const int64_t o = om + b3_off_n(g, n);
+          double v = g.alpha * double(acc[i][j][r]);
+          if (g.beta != 0.0) v += g.beta * g.C[o];
+          g.C[o] = v;
+        }
+      }
+  }
+}
+
 namespace {
 
 struct Planes {
@@ -333,6 +475,56 @@ static void run_b3(Engine &e, GemmArgs g) {
     SISI_HIP(hipGetLastError());
   }
   if (ksplit > 1) gemm_reduce(e, g);
+}
+
+void split_planes(Engine &e, const double *src, int64_t sr, int64_t sk, int R, int K, int Rp,
+                  int batch, int64_t bsrc, double scale, B3Planes &out) {
+  SISI_REQUIRE(R > 0 && K > 0 && Rp >= R && batch > 0 && batch <= 65535, SISI_ERR_INVALID,
+               "split_planes: bad shape");
+  const int Kp = ((K + 31) / 32) * 32;
+  const int64_t elems = int64_t(batch) * Rp * Kp;  // bf16 per plane
+  out.store = DBuf(&e.pool, size_t((2 * elems * 2 + 7) / 8));
+  u16 *hi = reinterpret_cast<u16 *>(out.store.get());
+  u16 *lo = hi + elems;
+  SplitArgs a{src, sr, sk, R, K, Rp, Kp, hi, lo};
+  a.bsrc = bsrc;
+  a.bdst = int64_t(Rp) * Kp;
+  a.scale = scale;
+  {
+    Timed t(*e.timers, Kind::Bandwidth, double(batch) * (8.0 * double(R) * K + 4.0 * double(Rp) * Kp));
+    dim3 grid(unsigned((Kp + 63) / 64), unsigned((Rp + 63) / 64), unsigned(batch));
+    hipLaunchKernelGGL(split_bf16_kernel, grid, dim3(256), 0, e.stream, a);
+    SISI_HIP(hipGetLastError());
+  }
+  out.hi = hi;
+  out.lo = lo;
+  out.Rp = Rp;
+  out.Kp = Kp;
+  out.batch = batch;
+}
+
+void gemm_b3_segments(Engine &e, B3SegArgs a) {
+  GemmArgs &g = a.g;
+  if (g.M <= 0 || g.N <= 0) return;
+  SISI_REQUIRE(a.nseg >= 1 && a.nseg <= kMaxSegs, SISI_ERR_INVALID,
+               "gemm_b3_segments: 1 to 4 segments");
+  double flops = 0.0;
+  for (int s = 0; s < a.nseg; ++s) {
+    SISI_REQUIRE(a.seg[s].Kp > 0 && a.seg[s].Kp % 32 == 0 && a.seg[s].Ah && a.seg[s].Bh,
+                 SISI_ERR_INVALID, "gemm_b3_segments: planes with Kp a multiple of 32");
+    flops += 2.0 * double(g.M) * double(g.N) * double(a.seg[s].Kp);
+  }
+  g.batch = 1;
+  g.ksplit = 1;
+  g.tiles_m = (g.M + kB3SegTile - 1) / kB3SegTile;
+  g.tiles_n = (g.N + kB3SegTile - 1) / kB3SegTile;
+  g.split_out = !(g.dm0 >= g.M && g.dn0 >= g.N);
+  g.xcd_map = 1;
+  const int64_t tiles = int64_t(g.tiles_m) * g.tiles_n;
+  SISI_REQUIRE(tiles < (int64_t(1) << 31), SISI_ERR_INVALID, "gemm_b3_segments: too many tiles");
+  Timed t(*e.timers, Kind::GemmB3, flops);
+  hipLaunchKernelGGL(gemm_b3p_seg_kernel, dim3(unsigned(tiles)), dim3(256), 0, e.stream, a);
+  SISI_HIP(hipGetLastError());
 }
 
 void gemm_b3(Engine &e, GemmArgs g) {

@@ -245,6 +245,78 @@ void build_w(Engine &e, int prec, const TriplesOps &op, const std::array<int, 3>
   }
 }
 
+// The operands of the block GEMMs as split-BF16x3 planes.
+struct TriplesPlanes {
+  const B3Planes *V, *V2;  // rows (b,c) of V_bdck, k = d: [d,b,c] and [d,c,b] orders
+  const B3Planes *VP;      // rows c, k = l of -V_jklc, one matrix per (j,k)
+  B3Planes T2p;            // rows a, k = d of T2[a,d,x,y], one matrix per (x,y)
+  B3Planes T2h, T2th;      // rows (a,b), k = l of T2[a,b,x,l] and T2[b,a,x,l], per x
+};
+
+// build_w on pre-split planes (split-BF16x3, FP32 accumulation).
+void build_w_b3(Engine &e, const TriplesPlanes &pl, const std::array<int, 3> &ib,
+                const std::array<std::array<int, 3>, 6> &perms, const bool *use, double *W) {
+  const int64_t o = e.No, v = e.Nv;
+  const int64_t st[3] = {1, v, v * v};
+  bool first = true;
+  for (int j = 0; j < 3; ++j) {
+    const int p = j == 0 ? 1 : 0, q = j == 2 ? 1 : 2;
+    B3SegArgs a;
+    for (int n = 0; n < 6; ++n) {
+      if (!use[n]) continue;
+      const auto &pi = perms[n];
+      const int x = ib[pi[0]], y = ib[pi[1]], z = ib[pi[2]];
+      if (pi[0] == j) {  // + T2[x_j, d, x, y] V[d, x_pi1, x_pi2, z]
+        B3Seg &sg = a.seg[a.nseg++];
+        const B3Planes &Vz = pi[1] == p ? *pl.V : *pl.V2;
+        sg.Ah = pl.T2p.hi_of(x + o * y);
+        sg.Al = pl.T2p.lo_of(x + o * y);
+        sg.Bh = Vz.hi_of(z);
+        sg.Bl = Vz.lo_of(z);
+        sg.Kp = int(pl.T2p.Kp);
+      }
+      if (pi[2] == j) {  // - V_P[l, x_j, y, z] T2[x_pi0, x_pi1, x, l] (sign in the planes)
+        B3Seg &sg = a.seg[a.nseg++];
+        const B3Planes &Tx = pi[0] == p ? pl.T2h : pl.T2th;
+        sg.Ah = pl.VP->hi_of(y + o * z);
+        sg.Al = pl.VP->lo_of(y + o * z);
+        sg.Bh = Tx.hi_of(x);
+        sg.Bl = Tx.lo_of(x);
+        sg.Kp = int(pl.VP->Kp);
+      }
+    }
+    if (a.nseg == 0) continue;
+    GemmArgs &g = a.g;
+    g.C = W;
+    g.alpha = 1.0;
+    g.beta = first ? 0.0 : 1.0;
+    first = false;
+    if (j == 0) {
+      g.M = int(v);
+      g.N = int(v * v);
+      g.cm0 = 1;
+      g.dm0 = int(v);
+      g.cn0 = v;
+      g.dn0 = int(v * v);
+    } else {  // C^T = B^T A^T so lanes run along x0
+      for (int s = 0; s < a.nseg; ++s) {
+        B3Seg &sg = a.seg[s];
+        std::swap(sg.Ah, sg.Bh);
+        std::swap(sg.Al, sg.Bl);
+      }
+      g.M = int(v * v);
+      g.N = int(v);
+      g.cm0 = st[p];
+      g.dm0 = int(v);
+      g.cm1 = st[q];
+      g.dm1 = int(v);
+      g.cn0 = st[j];
+      g.dn0 = int(v);
+    }
+    gemm_b3_segments(e, a);
+  }
+}
+
 }  // namespace
 
 double triples_energy(Engine &e, const double *T1, const double *T2, int prec,
@@ -283,6 +355,24 @@ double triples_energy(Engine &e, const double *T1, const double *T2, int prec,
   std::vector<double> eps_i(o);
   e.download(eps_i.data(), e.eps_i, size_t(o));
   const TriplesOps op{T2, T2t.get(), e.Vt.get(), e.Vt2.get(), e.VP.get()};
+  TriplesPlanes pl{};
+  if (prec == SISI_BF16X3) {
+    auto pad = [](int64_t r) { return int(((r + kB3SegTile - 1) / kB3SegTile) * kB3SegTile); };
+    if (!e.Vt_b3.hi) {
+      split_planes(e, e.Vt.get(), v, 1, int(v * v), int(v), pad(v * v), int(o), v * v * v, 1.0,
+                   e.Vt_b3);
+      split_planes(e, e.Vt2.get(), v, 1, int(v * v), int(v), pad(v * v), int(o), v * v * v, 1.0,
+                   e.Vt2_b3);
+      split_planes(e, e.VP.get(), o, 1, int(v), int(o), pad(v), int(o * o), o * v, -1.0, e.VP_b3);
+    }
+    pl.V = &e.Vt_b3;
+    pl.V2 = &e.Vt2_b3;
+    pl.VP = &e.VP_b3;
+    split_planes(e, T2, 1, v, int(v), int(v), pad(v), int(o * o), v * v, 1.0, pl.T2p);
+    split_planes(e, T2, 1, v * v * o, int(v * v), int(o), pad(v * v), int(o), v * v, 1.0, pl.T2h);
+    split_planes(e, T2t.get(), 1, v * v * o, int(v * v), int(o), pad(v * v), int(o), v * v, 1.0,
+                 pl.T2th);
+  }
 
   // cube orbits X0 <= X1 <= X2
   const int ncube = int((v + 7) / 8);
@@ -312,7 +402,8 @@ double triples_energy(Engine &e, const double *T1, const double *T2, int prec,
         if (tup[q] == tup[p]) break;
       distinct[p] = (q == p);
     }
-    build_w(e, prec, op, ib, perms, distinct, W.get());
+    if (prec == SISI_BF16X3) build_w_b3(e, pl, ib, perms, distinct, W.get());
+    else build_w(e, prec, op, ib, perms, distinct, W.get());
 
     EnergyArgs g{};
     g.W = W.get();
