@@ -86,6 +86,10 @@ def main():
     ap.add_argument("--level-shift", type=float, default=0.0)
     ap.add_argument("--max-iterations", type=int, default=400)
     ap.add_argument("--precisions", default="fp64,mixed")
+    ap.add_argument("--refine-precision", default="bf16", choices=["fp32", "bf16x3", "bf16"],
+                    help="mixed runs: precision of the refinement residuals between anchors")
+    ap.add_argument("--triples-precision", default="bf16x3", choices=["fp32", "bf16x3"],
+                    help="(T) precision of the mixed run")
     ap.add_argument("--refine-interval", type=int, default=8,
                     help="mixed runs: FP64 anchor every k iterations after the switch (0: plain "
                          "FP64 iterations)")
@@ -261,13 +265,14 @@ def converge(args):
     e = eng.Engine(No, Nv, NG)
     e.set_system(g, ei, ea)
     runs = {}
-    todo = [r for r in (("fp64", "fp64", "fp64"), ("mixed", "bf16x3", "fp32"))
+    todo = [r for r in (("fp64", "fp64", "fp64"), ("mixed", "bf16x3", args.triples_precision))
             if r[0] in args.precisions.split(",")]
     for name, prec, tprec in todo:
         o = eng.default_opts(mixer="DiisMixer", max_residua=6, max_iterations=args.max_iterations,
                              slice_size=s, amplitudes_convergence=1e-8, energy_convergence=1e-10,
                              precision=prec, switch_threshold=1e-5, level_shift=args.level_shift,
-                             refine_interval=args.refine_interval if prec != "fp64" else 0)
+                             refine_interval=args.refine_interval if prec != "fp64" else 0,
+                             refine_precision=args.refine_precision)
         t0 = time.perf_counter()
         e.ccsd_begin(o)
         rep = None
@@ -295,6 +300,11 @@ def converge(args):
                 b = sorted(rng.choice(No, 3, replace=False).tolist())
                 if b not in blocks:
                     blocks.append(b)
+        # one-time (T) integrals (and planes) outside the timed sample, once
+        # per run: they are per system, not per block
+        t0 = time.perf_counter()
+        e.triples(precision=tprec, blocks=[[0, 1, 2]])
+        t_setup = time.perf_counter() - t0
         t0 = time.perf_counter()
         et = e.triples(precision=tprec, blocks=blocks)
         t_t = time.perf_counter() - t0
@@ -302,7 +312,8 @@ def converge(args):
         runs[name] = dict(ccsd_seconds=t_ccsd, iterations=rep["iterations"],
                           fp64_iterations=rep["fp64_iterations"], energy=rep["energy"],
                           converged=bool(rep["converged"]), triples_precision=tprec,
-                          triples_seconds=(t_t if blocks is None else t_t * nblk / len(blocks)),
+                          triples_seconds=t_setup + (t_t if blocks is None
+                                                     else t_t * nblk / len(blocks)),
                           triples_sampled=blocks is not None, triples_energy_sample=et)
         log("%s done: %s" % (name, runs[name]))
     if len(runs) < 2:

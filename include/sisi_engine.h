@@ -42,6 +42,7 @@ extern "C" {
 #define SISI_FP64 0
 #define SISI_FP32 1
 #define SISI_BF16X3 2
+#define SISI_BF16 3 /* one bf16 product (FP32 accumulation): refinement residuals */
 
 #define SISI_MIXER_LINEAR 0
 #define SISI_MIXER_DIIS 1
@@ -81,6 +82,10 @@ typedef struct {
                                     residual every k iterations and
                                     refinement residuals (anchor + reduced-
                                     precision change) in between          */
+  int refine_precision;          /* precision of those refinement residuals:
+                                    SISI_BF16 (default), SISI_BF16X3 or
+                                    SISI_FP32; their error scales with the
+                                    change since the anchor               */
 } sisi_ccsd_opts;
 
 typedef struct {

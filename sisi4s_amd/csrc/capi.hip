@@ -78,6 +78,7 @@ void sisi_ccsd_opts_default(sisi_ccsd_opts *o) {
   o->precision = SISI_FP64;
   o->switch_threshold = 1e-5;
   o->refine_interval = 0;
+  o->refine_precision = SISI_BF16;
 }
 
 int sisi_engine_create(const sisi_dims *dims, sisi_engine **out) {
@@ -442,8 +443,7 @@ int sisi_gemm_host(int precision, int transA, int transB, int M, int N, int K, d
                    int ldc) {
   return guarded([&] {
     SISI_REQUIRE(M >= 0 && N >= 0 && K >= 0, SISI_ERR_INVALID, "negative dimension");
-    SISI_REQUIRE(precision == SISI_FP64 || precision == SISI_FP32 || precision == SISI_BF16X3,
-                 SISI_ERR_INVALID, "unknown precision");
+    SISI_REQUIRE(is_precision(precision), SISI_ERR_INVALID, "unknown precision");
     SISI_REQUIRE(A && B && C, SISI_ERR_INVALID, "null pointer");
     const int64_t ra = transA ? K : M, ca = transA ? M : K;
     const int64_t rb = transB ? N : K, cb = transB ? K : N;
@@ -479,7 +479,7 @@ int sisi_gemm_host(int precision, int transA, int transB, int M, int N, int K, d
     g.dn0 = std::max(N, 1);
     g.alpha = alpha;
     g.beta = beta;
-    gemm(e, precision, g);
+    gemm(e, precision, g, true);  // the precision asked for, whatever the size
     if (nc > 0) e.download(C, c, size_t(nc));
   });
 }

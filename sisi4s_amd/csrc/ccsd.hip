@@ -20,8 +20,7 @@ inline View V1(double *p, int64_t a) { return dense(p, {a}); }
 }  // namespace
 
 void set_precision(Engine &e, int precision) {
-  SISI_REQUIRE(precision == SISI_FP64 || precision == SISI_FP32 || precision == SISI_BF16X3,
-               SISI_ERR_INVALID, "unknown precision");
+  SISI_REQUIRE(is_precision(precision), SISI_ERR_INVALID, "unknown precision");
   // BF16X3: every contraction in split-BF16x3 (the GEMM family itself keeps
   // FP32 for products too small to pay for the operand split)
   e.precision = precision;
@@ -623,6 +622,9 @@ void ccsd_begin(Engine &e, const sisi_ccsd_opts &opts, const double *T1h, const 
   e.lin2.reset();
   e.anchor = Anchor();
   SISI_REQUIRE(opts.refine_interval >= 0, SISI_ERR_INVALID, "refine_interval must be >= 0");
+  SISI_REQUIRE(opts.refine_precision == SISI_FP32 || opts.refine_precision == SISI_BF16X3 ||
+                   opts.refine_precision == SISI_BF16,
+               SISI_ERR_INVALID, "refine_precision must be SISI_FP32, SISI_BF16X3 or SISI_BF16");
   if (opts.mixer == SISI_MIXER_DIIS) {
     const int N = opts.max_residua;
     e.diis.resize(N);
@@ -651,10 +653,10 @@ void ccsd_iterate(Engine &e, int n) {
         ccsd_residual_anchor(e, o.slice_size, e.T1.get(), e.T2.get(), R1.get(), R2.get());
         prec = SISI_FP64;
       } else {
-        set_precision(e, o.precision);
+        set_precision(e, o.refine_precision);
         ccsd_residual_delta(e, o.slice_size, e.T1.get(), e.T2.get(), R1.get(), R2.get());
         e.anchor.age++;
-        prec = o.precision;
+        prec = o.refine_precision;
       }
     } else {
       set_precision(e, prec);

@@ -468,18 +468,21 @@ void gemm_reduce(Engine &eng, const GemmArgs &g) {
   SISI_HIP(hipGetLastError());
 }
 
-void gemm(Engine &eng, int precision, GemmArgs g) {
+void gemm(Engine &eng, int precision, GemmArgs g, bool exact) {
   if (g.M <= 0 || g.N <= 0 || g.batch <= 0) return;
   // split-BF16x3 splits both operands first (a bandwidth pass over each); a
   // product too small to amortise that runs on FP32 MFMA instead
-  if (precision == SISI_BF16X3 && gemm_flops(g) < kB3MinFlops) precision = SISI_FP32;
-  if (precision == SISI_BF16X3 && g.batch == 1 && g.K > 0) {
+  if ((precision == SISI_BF16X3 || precision == SISI_BF16) && !exact &&
+      gemm_flops(g) < kB3MinFlops)
+    precision = SISI_FP32;
+  if ((precision == SISI_BF16X3 || precision == SISI_BF16) && g.batch == 1 && g.K > 0) {
     static const int env_b3 = getenv("SISI_B3_PRESPLIT") ? atoi(getenv("SISI_B3_PRESPLIT")) : 1;
     if (env_b3) {
-      gemm_b3(eng, g);
+      gemm_b3(eng, g, precision == SISI_BF16 ? 1 : 3);
       return;
     }
   }
+  if (precision == SISI_BF16) precision = SISI_BF16X3;  // batched: the three-product kernel
   if (g.K < 0) g.K = 0;  // K == 0 gives C = beta * C through the epilogue
   SISI_REQUIRE(g.sam == 1 || g.sak == 1 || g.K <= 1 || g.M <= 1, SISI_ERR_INVALID,
                "gemm: A needs a unit stride");

@@ -17,6 +17,11 @@
 
 namespace sisi {
 
+// The precisions of the dense contractions (SISI_FP64/_FP32/_BF16X3/_BF16).
+inline bool is_precision(int p) {
+  return p == SISI_FP64 || p == SISI_FP32 || p == SISI_BF16X3 || p == SISI_BF16;
+}
+
 struct GemmArgs {
   const double *A = nullptr;
   const double *B = nullptr;
@@ -93,11 +98,13 @@ struct B3SegArgs {
 constexpr int kB3SegTile = 128;  // rows of A and B planes must cover the tile-padded M, N
 void gemm_b3_segments(Engine &e, B3SegArgs a);
 
-// Launch on the engine's stream.  precision: SISI_FP64/_FP32/_BF16X3.
-void gemm(Engine &eng, int precision, GemmArgs g);
+// Launch on the engine's stream.  precision: SISI_FP64/_FP32/_BF16X3/_BF16.
+// Products too small for the bf16 split run in FP32 unless `exact` is set.
+void gemm(Engine &eng, int precision, GemmArgs g, bool exact = false);
 
-// Split-BF16x3 GEMM on operands split into bf16 planes (batch == 1).
-void gemm_b3(Engine &eng, GemmArgs g);
+// Split-BF16x3 GEMM on operands split into bf16 planes (batch == 1);
+// passes = 1 keeps the hi*hi product only (plain bf16).
+void gemm_b3(Engine &eng, GemmArgs g, int passes = 3);
 
 // Split-K reduction of g.work into C (deterministic order).
 void gemm_reduce(Engine &eng, const GemmArgs &g);

@@ -103,7 +103,9 @@ __device__ __forceinline__ int64_t b3_off_n(const GemmArgs &g, int n) {
 // BM x BN block, 4 waves of WM x WN, k-chunk 32, planes staged through LDS
 // with 16-byte stores.  NBUF register sets keep NBUF chunks of global loads in
 // flight, so a chunk's load latency is covered by NBUF chunks of MFMA work.
-template <int BM, int BN, int WM, int WN, int NBUF, int MINB>
+// PASSES = 3: split-BF16x3 (hi*lo + lo*hi + hi*hi); PASSES = 1: the hi*hi
+// product alone (plain bf16 operands, FP32 accumulation), lo planes unread.
+template <int BM, int BN, int WM, int WN, int NBUF, int MINB, int PASSES>
 __global__ __launch_bounds__(256, MINB) void gemm_b3p_kernel(B3Args p) {
   constexpr int BK = 32, RS = BK + 8;  // LDS row stride in bf16 (80 bytes)
   constexpr int NA = BM * 4 / 256, NB = BN * 4 / 256;  // 16-byte chunks per thread per plane
@@ -140,12 +142,14 @@ __global__ __launch_bounds__(256, MINB) void gemm_b3p_kernel(B3Args p) {
     _Pragma("unroll") for (int i_ = 0; i_ < NA; ++i_) {                                   \
       const int q_ = tid + 256 * i_, row_ = q_ >> 2, kq_ = (q_ & 3) * 8;                  \
       ra[b][i_] = *reinterpret_cast<const U128 *>(Agh + int64_t(row_) * p.Kp + kb_ + kq_);    \
-      ra[b][NA + i_] = *reinterpret_cast<const U128 *>(Agl + int64_t(row_) * p.Kp + kb_ + kq_); \
+      if (PASSES == 3)                                                                    \
+        ra[b][NA + i_] = *reinterpret_cast<const U128 *>(Agl + int64_t(row_) * p.Kp + kb_ + kq_); \
     }                                                                                     \
     _Pragma("unroll") for (int i_ = 0; i_ < NB; ++i_) {                                   \
       const int q_ = tid + 256 * i_, row_ = q_ >> 2, kq_ = (q_ & 3) * 8;                  \
       rb[b][i_] = *reinterpret_cast<const U128 *>(Bgh + int64_t(row_) * p.Kp + kb_ + kq_);    \
-      rb[b][NB + i_] = *reinterpret_cast<const U128 *>(Bgl + int64_t(row_) * p.Kp + kb_ + kq_); \
+      if (PASSES == 3)                                                                    \
+        rb[b][NB + i_] = *reinterpret_cast<const U128 *>(Bgl + int64_t(row_) * p.Kp + kb_ + kq_); \
     }                                                                                     \
   } while (0)
 #define B3_STORE(b)                                                                        \
@@ -153,12 +157,12 @@ __global__ __launch_bounds__(256, MINB) void gemm_b3p_kernel(B3Args p) {
     _Pragma("unroll") for (int i_ = 0; i_ < NA; ++i_) {                                   \
       const int q_ = tid + 256 * i_, row_ = q_ >> 2, kq_ = (q_ & 3) * 8;                  \
       *reinterpret_cast<U128 *>(&Ah[row_ * RS + kq_]) = ra[b][i_];                         \
-      *reinterpret_cast<U128 *>(&Al[row_ * RS + kq_]) = ra[b][NA + i_];                    \
+      if (PASSES == 3) *reinterpret_cast<U128 *>(&Al[row_ * RS + kq_]) = ra[b][NA + i_];   \
     }                                                                                     \
     _Pragma("unroll") for (int i_ = 0; i_ < NB; ++i_) {                                   \
       const int q_ = tid + 256 * i_, row_ = q_ >> 2, kq_ = (q_ & 3) * 8;                  \
       *reinterpret_cast<U128 *>(&Bh[row_ * RS + kq_]) = rb[b][i_];                         \
-      *reinterpret_cast<U128 *>(&Bl[row_ * RS + kq_]) = rb[b][NB + i_];                    \
+      if (PASSES == 3) *reinterpret_cast<U128 *>(&Bl[row_ * RS + kq_]) = rb[b][NB + i_];   \
     }                                                                                     \
   } while (0)
 
@@ -182,24 +186,27 @@ __global__ __launch_bounds__(256, MINB) void gemm_b3p_kernel(B3Args p) {
     for (int j = 0; j < TJ; ++j) {
       const int col = wn + j * 16 + l16;
       bh[j] = *reinterpret_cast<const bf8 *>(&Bh[col * RS + 8 * lk]);
-      bl[j] = *reinterpret_cast<const bf8 *>(&Bl[col * RS + 8 * lk]);
+      if (PASSES == 3) bl[j] = *reinterpret_cast<const bf8 *>(&Bl[col * RS + 8 * lk]);
     }
 #pragma unroll
     for (int i = 0; i < TI; ++i) {
       const int row = wm + i * 16 + l16;
       const bf8 ah = *reinterpret_cast<const bf8 *>(&Ah[row * RS + 8 * lk]);
-      const bf8 al = *reinterpret_cast<const bf8 *>(&Al[row * RS + 8 * lk]);
+      if constexpr (PASSES == 3) {
+        const bf8 al = *reinterpret_cast<const bf8 *>(&Al[row * RS + 8 * lk]);
 #pragma unroll
-      for (int j = 0; j < TJ; ++j)
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[j], ah, acc[i][j], 0, 0, 0);
+        for (int j = 0; j < TJ; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[j], ah, acc[i][j], 0, 0, 0);
 #pragma unroll
-      for (int j = 0; j < TJ; ++j)
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], al, acc[i][j], 0, 0, 0);
+        for (int j = 0; j < TJ; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], al, acc[i][j], 0, 0, 0);
+      }
 #pragma unroll
       for (int j = 0; j < TJ; ++j)
         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], ah, acc[i][j], 0, 0, 0);
     }
   };
+
 
 #pragma unroll
   for (int b = 0; b < NBUF; ++b)
@@ -441,7 +448,7 @@ const Planes &split_operand(Engine &e, const double *src, int64_t sr, int64_t sk
 
 }  // namespace
 
-template <int BM, int BN, int WM, int WN, int NBUF, int MINB = 1>
+template <int BM, int BN, int WM, int WN, int NBUF, int MINB = 1, int PASSES = 3>
 static void run_b3(Engine &e, GemmArgs g) {
   const int Kp = ((g.K + 31) / 32) * 32;
   const int Mp = ((g.M + BM - 1) / BM) * BM, Np = ((g.N + BN - 1) / BN) * BN;
@@ -471,7 +478,8 @@ static void run_b3(Engine &e, GemmArgs g) {
   dim3 grid(unsigned(tiles), unsigned(ksplit), 1);
   {
     Timed t(*e.timers, Kind::GemmB3, gemm_flops(g));
-    hipLaunchKernelGGL((gemm_b3p_kernel<BM, BN, WM, WN, NBUF, MINB>), grid, dim3(256), 0, e.stream, p);
+    hipLaunchKernelGGL((gemm_b3p_kernel<BM, BN, WM, WN, NBUF, MINB, PASSES>), grid, dim3(256), 0,
+                       e.stream, p);
     SISI_HIP(hipGetLastError());
   }
   if (ksplit > 1) gemm_reduce(e, g);
@@ -527,10 +535,14 @@ void gemm_b3_segments(Engine &e, B3SegArgs a) {
   SISI_HIP(hipGetLastError());
 }
 
-void gemm_b3(Engine &e, GemmArgs g) {
+void gemm_b3(Engine &e, GemmArgs g, int passes) {
   // 128x128 blocks of four 64x64 waves, two workgroups per CU, measured
   // fastest on the engine's shapes; the others are kept for measurement
   static const int cfg = getenv("SISI_B3_CFG") ? atoi(getenv("SISI_B3_CFG")) : 2;
+  if (passes == 1) {
+    run_b3<128, 128, 64, 64, 1, 1, 1>(e, g);
+    return;
+  }
   switch (cfg) {
     case 0: run_b3<256, 128, 128, 64, 2>(e, g); break;
     case 1: run_b3<256, 128, 128, 64, 1>(e, g); break;

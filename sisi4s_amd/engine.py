@@ -23,8 +23,8 @@ SISI_ERR_STATE = 3
 SISI_ERR_NUMERIC = 4
 SISI_ERR_ALLOC = 5
 
-FP64, FP32, BF16X3 = 0, 1, 2
-PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16x3": BF16X3}
+FP64, FP32, BF16X3, BF16 = 0, 1, 2, 3
+PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16x3": BF16X3, "bf16": BF16}
 MIXER_LINEAR, MIXER_DIIS = 0, 1
 PPHH, PHPH, HHHH, HHHP, HHPP = 0, 1, 2, 3, 4
 
@@ -42,7 +42,8 @@ class CcsdOpts(ctypes.Structure):
                 ("amplitudes_convergence", ctypes.c_double),
                 ("energy_convergence", ctypes.c_double), ("level_shift", ctypes.c_double),
                 ("slice_size", ctypes.c_int), ("precision", ctypes.c_int),
-                ("switch_threshold", ctypes.c_double), ("refine_interval", ctypes.c_int)]
+                ("switch_threshold", ctypes.c_double), ("refine_interval", ctypes.c_int),
+                ("refine_precision", ctypes.c_int)]
 
 
 class CcsdReport(ctypes.Structure):
@@ -173,7 +174,7 @@ def default_opts(**kw):
     for k, v in kw.items():
         if k == "mixer" and isinstance(v, str):
             v = {"LinearMixer": MIXER_LINEAR, "DiisMixer": MIXER_DIIS}[v]
-        if k == "precision" and isinstance(v, str):
+        if k in ("precision", "refine_precision") and isinstance(v, str):
             v = PRECISIONS[v]
         setattr(o, k, v)
     return o

@@ -274,6 +274,26 @@ def test_refined_mixed_solve_matches_fp64():
     assert abs(rr["energy"] - r64["energy"]) <= 1e-6
 
 
+def test_refined_mixed_solve_with_bf16_refinement_matches_fp64():
+    """Refinement residuals in plain bf16 between FP64 anchors: their error
+    scales with the change since the anchor, so the run still converges to
+    the FP64 energy."""
+    from sisi4s_amd import engine
+    No, Nv, NG = 5, 20, 80
+    e, g, ei, ea = make_engine(7, No, Nv, NG)
+    kw = dict(mixer="DiisMixer", max_residua=6, max_iterations=150,
+              amplitudes_convergence=1e-8, energy_convergence=1e-10)
+    announce("fp64 vs bf16-refined mixed solve")
+    r64 = e.ccsd_solve(engine.default_opts(**kw))
+    rr = e.ccsd_solve(engine.default_opts(precision="bf16x3", switch_threshold=1e-5,
+                                          refine_interval=8, refine_precision="bf16", **kw))
+    print("bf16-refined mixed: %d iterations (%d fp64) vs fp64 %d, |dE| %.2e"
+          % (rr["iterations"], rr["fp64_iterations"], r64["iterations"],
+             abs(rr["energy"] - r64["energy"])), flush=True)
+    assert r64["converged"] and rr["converged"]
+    assert abs(rr["energy"] - r64["energy"]) <= 1e-6
+
+
 @pytest.mark.gpu
 def test_ueg_known_answers():
     """The reference's own known answers for the uniform electron gas, rs=1,

@@ -8,7 +8,7 @@ from conftest import announce
 pytestmark = pytest.mark.gpu
 
 # error bound relative to max_{mn} sum_k |A_mk||B_kn|
-TOL = {0: 1e-14, 1: 4e-7, 2: 4e-5}
+TOL = {0: 1e-14, 1: 4e-7, 2: 4e-5, 3: 2e-2}
 
 
 def _run(prec, ta, tb, M, N, K, alpha=1.0, beta=0.0, seed=0):
@@ -27,7 +27,7 @@ def _run(prec, ta, tb, M, N, K, alpha=1.0, beta=0.0, seed=0):
     assert err <= TOL[prec] * max(bound, 1.0) * max(1.0, np.sqrt(K) / 8), (err, bound)
 
 
-@pytest.mark.parametrize("prec", [0, 1, 2])
+@pytest.mark.parametrize("prec", [0, 1, 2, 3])
 @pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
 @pytest.mark.parametrize("M,N,K", [(1, 1, 1), (17, 33, 5), (130, 129, 70), (256, 384, 96)])
 def test_gemm_shapes(prec, ta, tb, M, N, K):
@@ -40,7 +40,7 @@ def test_gemm_alpha_beta_and_empty_k(prec):
     _run(prec, 1, 0, 40, 50, 0, alpha=1.0, beta=-3.0)
 
 
-@pytest.mark.parametrize("prec", [0, 1, 2])
+@pytest.mark.parametrize("prec", [0, 1, 2, 3])
 def test_gemm_split_k(prec):
     # few output tiles and a long k: the launcher splits k and reduces in FP64
     _run(prec, 1, 0, 64, 48, 40000, seed=3)
