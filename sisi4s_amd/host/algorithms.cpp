@@ -172,6 +172,7 @@ class CcsdEnergyFromCoulombIntegrals : public Algorithm {
     opts.precision = precision_argument(*this);
     opts.switch_threshold = getRealArgument("switchThreshold", 1e-5);
     opts.refine_interval = int(getIntegerArgument("refineInterval", 0));
+    opts.refine_precision = int(getIntegerArgument("refinePrecision", SISI_BF16));
     std::vector<double> t1, t2;
     if (isArgumentGiven("initialSinglesAmplitudes")) t1 = getTensorArgument("initialSinglesAmplitudes").values;
     if (isArgumentGiven("initialDoublesAmplitudes")) t2 = getTensorArgument("initialDoublesAmplitudes").values;
