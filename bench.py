@@ -343,7 +343,7 @@ def batched(args):
         Nv = int(rng.integers(30, 121))
         specs.append((args.seed + k, No, Nv, 4 * Nv))
     systems = [eng.generate_system(*sp) for sp in specs]
-    kw = dict(mixer="DiisMixer", max_residua=6, max_iterations=300,
+    kw = dict(mixer="DiisMixer", max_residua=6, max_iterations=300, level_shift=args.level_shift,
               amplitudes_convergence=1e-8, energy_convergence=1e-10)
     res = {}
     for name, prec in (("fp64", "fp64"), ("mixed", "bf16x3")):
@@ -354,11 +354,13 @@ def batched(args):
                                          workers=args.workers, strict=False)
         res[name] = dict(seconds=time.perf_counter() - t0, energies=E.tolist(),
                          failed=[k for k, r in enumerate(reps) if r["status"] != 0],
+                         conv=[bool(r["converged"]) and r["status"] == 0 for r in reps],
                          converged=int(sum(r["converged"] for r in reps)),
                          iterations=int(sum(r["iterations"] for r in reps)),
                          fp64_iterations=int(sum(r["fp64_iterations"] for r in reps)))
         print("[batched] %s %.1fs" % (name, res[name]["seconds"]), file=sys.stderr, flush=True)
-    ok = [k for k in range(len(specs)) if k not in res["fp64"]["failed"]]
+    # compare the systems that converged in both runs
+    ok = [k for k in range(len(specs)) if res["fp64"]["conv"][k] and res["mixed"]["conv"][k]]
     dE = np.abs(np.array(res["fp64"]["energies"])[ok] - np.array(res["mixed"]["energies"])[ok])
     out = {"metric": METRIC, "mode": "batched",
            "config": {"workload": "%d seeded systems, No U[4,16], Nv U[30,120], NG=4Nv, DIIS(6), "
@@ -369,8 +371,9 @@ def batched(args):
            "systems_compared": len(ok),
            "systems_within_1uEh": int((dE <= 1e-6).sum()),
            "fp64_diverged": res["fp64"]["failed"],
-           "fp64": {k: v for k, v in res["fp64"].items() if k != "energies"},
-           "mixed": {k: v for k, v in res["mixed"].items() if k != "energies"}}
+           "level_shift": args.level_shift,
+           "fp64": {k: v for k, v in res["fp64"].items() if k not in ("energies", "conv")},
+           "mixed": {k: v for k, v in res["mixed"].items() if k not in ("energies", "conv")}}
     print(json.dumps(out), flush=True)
 
 

@@ -294,6 +294,28 @@ def test_refined_mixed_solve_with_bf16_refinement_matches_fp64():
     assert abs(rr["energy"] - r64["energy"]) <= 1e-6
 
 
+def test_mixed_schedule_at_mid_size_matches_fp64():
+    """No=20, Nv=200, NG=800: large enough that the ladder and ring products
+    run in split-BF16x3 (bulk) and plain bf16 (refinement) rather than
+    falling back to FP32; the mixed run reaches the FP64 energy to 1 uEh
+    with FP64 only on the anchors."""
+    from sisi4s_amd import engine
+    No, Nv, NG = 20, 200, 800
+    e, g, ei, ea = make_engine(20261202, No, Nv, NG)
+    kw = dict(mixer="DiisMixer", max_residua=6, max_iterations=200, slice_size=32,
+              level_shift=2.0, amplitudes_convergence=1e-8, energy_convergence=1e-10)
+    announce("mid-size fp64 vs mixed (bf16x3 bulk, bf16 refinement)")
+    r64 = e.ccsd_solve(engine.default_opts(**kw))
+    rm = e.ccsd_solve(engine.default_opts(precision="bf16x3", switch_threshold=1e-5,
+                                          refine_interval=8, refine_precision="bf16", **kw))
+    print("mid-size: fp64 %d it, mixed %d it (%d fp64), |dE| %.2e"
+          % (r64["iterations"], rm["iterations"], rm["fp64_iterations"],
+             abs(rm["energy"] - r64["energy"])), flush=True)
+    assert r64["converged"] and rm["converged"]
+    assert rm["fp64_iterations"] <= rm["iterations"] // 4 + 2
+    assert abs(rm["energy"] - r64["energy"]) <= 1e-6
+
+
 @pytest.mark.gpu
 def test_ueg_known_answers():
     """The reference's own known answers for the uniform electron gas, rs=1,
