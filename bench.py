@@ -86,11 +86,13 @@ def main():
     ap.add_argument("--level-shift", type=float, default=0.0)
     ap.add_argument("--max-iterations", type=int, default=400)
     ap.add_argument("--precisions", default="fp64,mixed")
+    ap.add_argument("--switch-threshold", type=float, default=1e-5,
+                    help="mixed runs: |dT|/|T| at which the bulk precision ends")
     ap.add_argument("--refine-precision", default="bf16", choices=["fp32", "bf16x3", "bf16"],
                     help="mixed runs: precision of the refinement residuals between anchors")
     ap.add_argument("--triples-precision", default="bf16x3", choices=["fp32", "bf16x3"],
                     help="(T) precision of the mixed run")
-    ap.add_argument("--refine-interval", type=int, default=8,
+    ap.add_argument("--refine-interval", type=int, default=16,
                     help="mixed runs: FP64 anchor every k iterations after the switch (0: plain "
                          "FP64 iterations)")
     args = ap.parse_args()
@@ -270,7 +272,8 @@ def converge(args):
     for name, prec, tprec in todo:
         o = eng.default_opts(mixer="DiisMixer", max_residua=6, max_iterations=args.max_iterations,
                              slice_size=s, amplitudes_convergence=1e-8, energy_convergence=1e-10,
-                             precision=prec, switch_threshold=1e-5, level_shift=args.level_shift,
+                             precision=prec, switch_threshold=args.switch_threshold,
+                             level_shift=args.level_shift,
                              refine_interval=args.refine_interval if prec != "fp64" else 0,
                              refine_precision=args.refine_precision)
         t0 = time.perf_counter()
