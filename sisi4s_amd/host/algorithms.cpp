@@ -172,7 +172,13 @@ class CcsdEnergyFromCoulombIntegrals : public Algorithm {
     opts.precision = precision_argument(*this);
     opts.switch_threshold = getRealArgument("switchThreshold", 1e-5);
     opts.refine_interval = int(getIntegerArgument("refineInterval", 0));
-    opts.refine_precision = int(getIntegerArgument("refinePrecision", SISI_BF16));
+    {
+      const std::string rp = getTextArgument("refinePrecision", "bf16");
+      if (rp == "bf16" || rp == "3") opts.refine_precision = SISI_BF16;
+      else if (rp == "bf16x3" || rp == "2") opts.refine_precision = SISI_BF16X3;
+      else if (rp == "fp32" || rp == "1") opts.refine_precision = SISI_FP32;
+      else throw StepError("unknown refinePrecision " + rp + " (bf16, bf16x3, fp32)");
+    }
     std::vector<double> t1, t2;
     if (isArgumentGiven("initialSinglesAmplitudes")) t1 = getTensorArgument("initialSinglesAmplitudes").values;
     if (isArgumentGiven("initialDoublesAmplitudes")) t2 = getTensorArgument("initialDoublesAmplitudes").values;

@@ -1,0 +1,274 @@
+// Input readers on the caller's side of the path: the Coulomb vertex and
+// eigenenergies from the files the reference reads, and TensorNorm for
+// checking what was read.  The real CCSD path needs a real vertex, so a
+// complex one is accepted only when its imaginary part is zero.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+
+#include "algorithm.hpp"
+#include "yaml_lite.hpp"
+
+namespace sisi_host {
+
+namespace {
+
+std::shared_ptr<Tensor> new_tensor(std::vector<int64_t> lens) {
+  auto t = std::make_shared<Tensor>();
+  t->lens = std::move(lens);
+  t->values.assign(size_t(t->size()), 0.0);
+  return t;
+}
+
+std::string num(double v) {
+  std::ostringstream o;
+  o.precision(17);
+  o << v;
+  return o.str();
+}
+
+const YamlNode &require(const YamlNode &n, const std::string &key, const std::string &file) {
+  const YamlNode *c = n.type == YamlNode::Map ? n.get(key) : nullptr;
+  if (!c) throw StepError(file + ": missing key '" + key + "'");
+  return *c;
+}
+
+double to_double(const YamlNode &n, const std::string &what) {
+  if (n.type != YamlNode::Scalar) throw StepError(what + ": expected a number");
+  char *end = nullptr;
+  const double v = std::strtod(n.scalar.c_str(), &end);
+  if (!end || *end || n.scalar.empty()) throw StepError(what + ": '" + n.scalar + "' is not a number");
+  return v;
+}
+
+}  // namespace
+
+// CoulombVertexReader (CoulombVertexReader.cxx:26-111, format in
+// docs/manual.org "FTODDUMP"): a 32-byte header {magic[8], int32 No, Nv, NG,
+// NSpins, kPoints, reserved} and chunks {magic[8], int64 size of the whole
+// chunk, payload}: FTODreal / FTODimag hold Gamma[G,p,q] (G fastest),
+// FTODepsi holds eps_p (holes first); other chunks are skipped by size.
+// The reference compares the first 8 bytes of "sisi4sFTOD"; the documented
+// cc4s magic "cc4sFTOD" is accepted as well.
+class CoulombVertexReader : public Algorithm {
+ public:
+  using Algorithm::Algorithm;
+  std::string getName() override { return "CoulombVertexReader"; }
+  void run() override {
+    const std::string fileName = getTextArgument("file");
+    if (getIntegerArgument("unrestricted", 0) != 0)
+      throw StepError("CoulombVertexReader: unrestricted = 1 is not supported");
+    std::ifstream f(fileName, std::ios::binary);
+    if (!f) throw StepError("Failed to open file " + fileName);
+    f.seekg(0, std::ios::end);
+    const int64_t fileSize = int64_t(f.tellg());
+    f.seekg(0);
+    struct {
+      char magic[8];
+      int32_t No, Nv, NG, NSpins, kPoints, reserved;
+    } header;
+    static_assert(sizeof(header) == 32, "FTOD header is 32 bytes");
+    if (fileSize < int64_t(sizeof(header)) || !f.read(reinterpret_cast<char *>(&header), sizeof(header)))
+      throw StepError("Invalid file format: " + fileName + " is shorter than its header");
+    if (std::strncmp(header.magic, "sisi4sFT", 8) != 0 && std::strncmp(header.magic, "cc4sFTOD", 8) != 0)
+      throw StepError("Invalid file format: " + fileName);
+    const int64_t No = header.No, Nv = header.Nv, NG = header.NG, Np = No + Nv;
+    if (No <= 0 || Nv <= 0 || NG <= 0) throw StepError("Invalid file format: bad dimensions");
+    auto gamma = new_tensor({NG, Np, Np});
+    auto epsi = new_tensor({No}), epsa = new_tensor({Nv});
+    bool have_real = false, have_eps = false;
+    double imag_max = 0.0;
+    const int64_t nvert = NG * Np * Np;
+    int64_t offset = sizeof(header);
+    while (offset < fileSize) {
+      struct {
+        char magic[8];
+        int64_t size;
+      } chunk;
+      static_assert(sizeof(chunk) == 16, "FTOD chunk header is 16 bytes");
+      f.seekg(offset);
+      if (!f.read(reinterpret_cast<char *>(&chunk), sizeof(chunk)) || chunk.size < 16 ||
+          offset + chunk.size > fileSize)
+        throw StepError("Invalid file format: truncated chunk at offset " + std::to_string(offset));
+      const int64_t payload = chunk.size - 16;
+      if (std::strncmp(chunk.magic, "FTODreal", 8) == 0 ||
+          std::strncmp(chunk.magic, "FTODimag", 8) == 0) {
+        if (payload < 8 * nvert) throw StepError("Invalid file format: short vertex chunk");
+        const bool real = chunk.magic[4] == 'r';
+        std::vector<double> buf;
+        double *dst = gamma->values.data();
+        if (!real) {
+          buf.resize(size_t(nvert));
+          dst = buf.data();
+        }
+        if (!f.read(reinterpret_cast<char *>(dst), 8 * nvert))
+          throw StepError("Invalid file format: short vertex chunk");
+        if (real) have_real = true;
+        else
+          for (double x : buf) imag_max = std::max(imag_max, std::fabs(x));
+      } else if (std::strncmp(chunk.magic, "FTODepsi", 8) == 0) {
+        if (payload < 8 * Np) throw StepError("Invalid file format: short eigenenergy chunk");
+        if (!f.read(reinterpret_cast<char *>(epsi->values.data()), 8 * No) ||
+            !f.read(reinterpret_cast<char *>(epsa->values.data()), 8 * Nv))
+          throw StepError("Invalid file format: short eigenenergy chunk");
+        have_eps = true;
+      }
+      offset += chunk.size;
+    }
+    if (!have_real || !have_eps)
+      throw StepError("Invalid file format: " + fileName + " lacks the FTODreal or FTODepsi chunk");
+    if (imag_max != 0.0)
+      throw StepError("CoulombVertexReader: the vertex has an imaginary part (max |Im| = " +
+                      num(imag_max) + "); only the real path is implemented");
+    allocatedTensorArgument("CoulombVertex", gamma);
+    allocatedTensorArgument("HoleEigenEnergies", epsi);
+    allocatedTensorArgument("ParticleEigenEnergies", epsa);
+    emitted.scalars.push_back({"NG", std::to_string(NG)});
+    emitted.scalars.push_back({"No", std::to_string(No)});
+    emitted.scalars.push_back({"Nv", std::to_string(Nv)});
+  }
+};
+
+// Read (Read.cxx:24-92): a cc4s tensor header (version 100, type Tensor,
+// dimensions [{length, type}], scalarType Real64 | Complex64,
+// elements.type TextFile | IeeeBinaryFile, unit) with its values in the file
+// of the same name and extension ".elements", first index fastest: one value
+// per line (text) or raw doubles ((re, im) pairs for Complex64).
+class Read : public Algorithm {
+ public:
+  using Algorithm::Algorithm;
+  std::string getName() override { return "Read"; }
+  void run() override {
+    const std::string fileName = getTextArgument("fileName");
+    YamlNode h;
+    try {
+      h = parse_yaml_file(fileName);
+    } catch (const YamlError &e) {
+      throw StepError("Read: " + std::string(e.what()));
+    }
+    const YamlNode &version = require(h, "version", fileName);
+    if (version.type != YamlNode::Scalar || version.scalar != "100")
+      throw StepError(fileName + ": unsupported version");
+    const YamlNode &type = require(h, "type", fileName);
+    if (type.scalar != "Tensor") throw StepError(fileName + ": type must be Tensor");
+    require(h, "unit", fileName);
+    const std::string scalarType = require(h, "scalarType", fileName).scalar;
+    if (scalarType != "Real64" && scalarType != "Complex64")
+      throw StepError(fileName + ": scalarType must be Real64 or Complex64");
+    const std::string elements = require(require(h, "elements", fileName), "type", fileName).scalar;
+    if (elements != "TextFile" && elements != "IeeeBinaryFile")
+      throw StepError(fileName + ": elements type must be TextFile or IeeeBinaryFile");
+    const YamlNode &dims = require(h, "dimensions", fileName);
+    if (dims.type != YamlNode::Seq || dims.seq.empty())
+      throw StepError(fileName + ": dimensions must be a non-empty list");
+    std::vector<int64_t> lens;
+    for (auto &d : dims.seq) {
+      const double len = to_double(require(d, "length", fileName), fileName + " dimension length");
+      const std::string dt = require(d, "type", fileName).scalar;
+      if (dt != "AuxiliaryField" && dt != "State")
+        throw StepError(fileName + ": dimension type must be AuxiliaryField or State");
+      if (len < 1 || len != std::floor(len)) throw StepError(fileName + ": bad dimension length");
+      lens.push_back(int64_t(len));
+    }
+    std::string data = fileName;
+    const auto slash = data.find_last_of('/');
+    const auto dot = data.find_last_of('.');
+    if (dot != std::string::npos && (slash == std::string::npos || dot > slash)) data.resize(dot);
+    data += ".elements";
+    auto t = new_tensor(lens);
+    const int64_t count = t->size();
+    if (scalarType == "Real64" && elements == "TextFile") {
+      std::ifstream s(data);
+      if (!s) throw StepError("Read: cannot open " + data);
+      int64_t k = 0;
+      for (std::string line; std::getline(s, line);) {
+        if (line.find_first_not_of(" \t\r") == std::string::npos) continue;
+        if (k >= count) throw StepError("Read: " + data + " holds more than " + std::to_string(count) + " values");
+        t->values[size_t(k++)] = std::atof(line.c_str());
+      }
+      if (k != count) throw StepError("Read: " + data + " holds " + std::to_string(k) + " of " + std::to_string(count) + " values");
+    } else if (elements == "IeeeBinaryFile") {
+      std::ifstream s(data, std::ios::binary);
+      if (!s) throw StepError("Read: cannot open " + data);
+      if (scalarType == "Real64") {
+        if (!s.read(reinterpret_cast<char *>(t->values.data()), 8 * count))
+          throw StepError("Read: " + data + " is shorter than " + std::to_string(count) + " doubles");
+      } else {
+        std::vector<double> z(size_t(2 * count));
+        if (!s.read(reinterpret_cast<char *>(z.data()), 16 * count))
+          throw StepError("Read: " + data + " is shorter than " + std::to_string(count) + " complex values");
+        double imag_max = 0.0;
+        for (int64_t k = 0; k < count; ++k) {
+          t->values[size_t(k)] = z[size_t(2 * k)];
+          imag_max = std::max(imag_max, std::fabs(z[size_t(2 * k + 1)]));
+        }
+        if (imag_max != 0.0)
+          throw StepError("Read: " + fileName + " has an imaginary part (max |Im| = " + num(imag_max) +
+                          "); only the real path is implemented");
+      }
+    } else {
+      throw StepError("Read: Complex64 tensors are read from IeeeBinaryFile elements only");
+    }
+    allocatedTensorArgument("destination", t);
+  }
+};
+
+// DefineHolesAndParticles (cc4s/DefineHolesAndParticles.cxx:8-51):
+// metaData.energies split at metaData.fermiEnergy (e <= fermiEnergy are
+// holes), keeping the file's order within each part.
+class DefineHolesAndParticles : public Algorithm {
+ public:
+  using Algorithm::Algorithm;
+  std::string getName() override { return "DefineHolesAndParticles"; }
+  void run() override {
+    const std::string fileName = getTextArgument("fileName");
+    YamlNode h;
+    try {
+      h = parse_yaml_file(fileName);
+    } catch (const YamlError &e) {
+      throw StepError("DefineHolesAndParticles: " + std::string(e.what()));
+    }
+    const YamlNode &meta = require(h, "metaData", fileName);
+    const double fermi = to_double(require(meta, "fermiEnergy", fileName), "fermiEnergy");
+    const YamlNode &en = require(meta, "energies", fileName);
+    if (en.type != YamlNode::Seq) throw StepError(fileName + ": energies must be a list");
+    std::vector<double> holes, particles;
+    for (auto &e : en.seq) {
+      const double v = to_double(e, "energies");
+      (v <= fermi ? holes : particles).push_back(v);
+    }
+    if (holes.empty() || particles.empty())
+      throw StepError(fileName + ": no holes or no particles at this Fermi energy");
+    auto epsi = new_tensor({int64_t(holes.size())}), epsa = new_tensor({int64_t(particles.size())});
+    epsi->values = holes;
+    epsa->values = particles;
+    allocatedTensorArgument("HoleEigenEnergies", epsi);
+    allocatedTensorArgument("ParticleEigenEnergies", epsa);
+    emitted.scalars.push_back({"No", std::to_string(holes.size())});
+    emitted.scalars.push_back({"Nv", std::to_string(particles.size())});
+  }
+};
+
+// TensorNorm (TensorNorm.cxx): Frobenius norm of Data, published as Norm.
+class TensorNorm : public Algorithm {
+ public:
+  using Algorithm::Algorithm;
+  std::string getName() override { return "TensorNorm"; }
+  void run() override {
+    const Tensor &t = getTensorArgument("Data");
+    long double s = 0.0L;
+    for (double x : t.values) s += (long double)x * x;
+    const double norm = std::sqrt(double(s));
+    emitted.scalars.push_back({"norm", num(norm)});
+    if (isArgumentGiven("Norm")) setRealArgument("Norm", norm);
+  }
+};
+
+static Registrar<CoulombVertexReader> rr0("CoulombVertexReader");
+static Registrar<Read> rr1("Read");
+static Registrar<DefineHolesAndParticles> rr2("DefineHolesAndParticles");
+static Registrar<TensorNorm> rr3("TensorNorm");
+
+}  // namespace sisi_host

@@ -48,11 +48,19 @@ class Parser {
       std::string s = rtrim(strip_comment(raw));
       if (trim(s).empty()) continue;
       if (trim(s) == "---" || trim(s) == "...") continue;
+      // a flow collection ({...} or [...]) may continue on the next lines
+      const int first = n;
+      while (flow_depth(s) > 0 && std::getline(in, raw)) {
+        ++n;
+        s += " " + trim(strip_comment(raw));
+      }
+      if (flow_depth(s) != 0)
+        throw YamlError("line " + std::to_string(first) + ": unbalanced { } or [ ]");
       int ind = 0;
       while (ind < int(s.size()) && s[ind] == ' ') ++ind;
       if (ind < int(s.size()) && s[ind] == '\t')
         throw YamlError("line " + std::to_string(n) + ": tabs are not allowed for indentation");
-      lines_.push_back({ind, s.substr(ind), n});
+      lines_.push_back({ind, s.substr(ind), first});
     }
   }
 
@@ -70,6 +78,89 @@ class Parser {
   std::map<std::string, YamlNode> anchors_;
 
   static bool is_seq_item(const std::string &t) { return t == "-" || t.rfind("- ", 0) == 0; }
+
+  // open { and [ minus closed ones, outside quotes
+  static int flow_depth(const std::string &t) {
+    bool sq = false, dq = false;
+    int d = 0;
+    for (char c : t) {
+      if (c == '\'' && !dq) sq = !sq;
+      else if (c == '"' && !sq) dq = !dq;
+      else if (!sq && !dq && (c == '{' || c == '[')) ++d;
+      else if (!sq && !dq && (c == '}' || c == ']')) --d;
+    }
+    return d;
+  }
+
+  // Flow collections: [a, b], {k: v, k2: v2}, nested; plain items go through
+  // scalar() (quotes, anchors, aliases).
+  YamlNode flow(const std::string &s, size_t &i, int lineno) {
+    auto skip = [&]() {
+      while (i < s.size() && (s[i] == ' ' || s[i] == '\t')) ++i;
+    };
+    auto fail = [&](const std::string &m) -> YamlError {
+      return YamlError("line " + std::to_string(lineno) + ": " + m);
+    };
+    skip();
+    YamlNode n;
+    if (i < s.size() && (s[i] == '[' || s[i] == '{')) {
+      const bool is_map = s[i] == '{';
+      const char close = is_map ? '}' : ']';
+      n.type = is_map ? YamlNode::Map : YamlNode::Seq;
+      ++i;
+      skip();
+      if (i < s.size() && s[i] == close) {
+        ++i;
+        return n;
+      }
+      while (true) {
+        if (is_map) {
+          // key up to ':' at depth 0
+          size_t k0 = i;
+          bool sq = false, dq = false;
+          while (i < s.size() && !(s[i] == ':' && !sq && !dq)) {
+            if (s[i] == '\'' && !dq) sq = !sq;
+            else if (s[i] == '"' && !sq) dq = !dq;
+            else if (!sq && !dq && (s[i] == ',' || s[i] == close)) throw fail("expected 'key: value' in {}");
+            ++i;
+          }
+          if (i >= s.size()) throw fail("unterminated {");
+          std::string key = trim(s.substr(k0, i - k0));
+          if (key.size() >= 2 && (key[0] == '"' || key[0] == '\'')) key = key.substr(1, key.size() - 2);
+          ++i;  // ':'
+          YamlNode v = flow(s, i, lineno);
+          for (auto &kv : n.map)
+            if (kv.first == key) throw fail("duplicate key " + key);
+          n.map.push_back({key, v});
+        } else {
+          n.seq.push_back(flow(s, i, lineno));
+        }
+        skip();
+        if (i >= s.size()) throw fail(std::string("unterminated ") + (is_map ? "{" : "["));
+        if (s[i] == ',') {
+          ++i;
+          continue;
+        }
+        if (s[i] == close) {
+          ++i;
+          break;
+        }
+        throw fail(std::string("expected ',' or '") + close + "'");
+      }
+      return n;
+    }
+    // plain or quoted item up to ',' ']' '}' at depth 0
+    size_t b = i;
+    bool sq = false, dq = false;
+    while (i < s.size()) {
+      const char c = s[i];
+      if (c == '\'' && !dq) sq = !sq;
+      else if (c == '"' && !sq) dq = !dq;
+      else if (!sq && !dq && (c == ',' || c == ']' || c == '}')) break;
+      ++i;
+    }
+    return scalar(s.substr(b, i - b), lineno);
+  }
 
   // position of the "key:" separator (colon followed by space or end), -1 if none
   static int key_colon(const std::string &t) {
@@ -99,14 +190,11 @@ class Parser {
       anchor = s.substr(1, sp == std::string::npos ? std::string::npos : sp - 1);
       s = sp == std::string::npos ? "" : trim(s.substr(sp + 1));
     }
-    if (!s.empty() && s[0] == '[') {
-      if (s.back() != ']') throw YamlError("line " + std::to_string(lineno) + ": unterminated [");
-      n.type = YamlNode::Seq;
-      std::string body = s.substr(1, s.size() - 2);
-      std::stringstream ss(body);
-      std::string item;
-      while (std::getline(ss, item, ','))
-        if (!trim(item).empty()) n.seq.push_back(scalar(item, lineno));
+    if (!s.empty() && (s[0] == '[' || s[0] == '{')) {
+      size_t i = 0;
+      n = flow(s, i, lineno);
+      if (!trim(s.substr(i)).empty())
+        throw YamlError("line " + std::to_string(lineno) + ": text after a flow collection");
     } else if (!s.empty()) {
       n.type = YamlNode::Scalar;
       if ((s[0] == '"' || s[0] == '\'') && s.size() >= 2 && s.back() == s[0]) {

@@ -1,8 +1,8 @@
-// Minimal YAML reader for sisi4s input files: a sequence of step maps
-// ({name, in, out, note, fallible, disable}) whose values are scalars,
-// $Symbol references, nested maps, flow sequences of scalars, and
-// anchors/aliases on scalars (&x value / *x), as in
-// integration-tests/tests/cc4s/hf/in.yaml.  The reference parses the same
+// Minimal YAML reader for sisi4s input files and cc4s data headers: block
+// maps and sequences, flow collections ([...], {...}, nested, possibly over
+// several lines), scalars, $Symbol references and anchors/aliases (&x / *x),
+// as in integration-tests/tests/cc4s/hf/in.yaml and
+// integration-tests/tests/cc4s/ueg/rs1.0-7occ-26virt/in.yaml.  The reference parses the same
 // files with yaml-cpp (src/Parser.cxx:21-72).
 #pragma once
 

@@ -1,0 +1,198 @@
+"""Input readers of the C++ step runner: the FTOD binary vertex
+(CoulombVertexReader), cc4s tensor headers with text or binary elements
+(Read) and DefineHolesAndParticles, checked through TensorNorm against
+numpy on files written here in the documented layouts; on the GPU, the
+reference's uniform-electron-gas pipeline read from such files."""
+
+import os
+import re
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, announce
+from oracle import philox
+
+RUNNER = os.path.join(ROOT, "sisi4s_amd", "sisi4s-mi355x")
+
+
+def run_steps(tmp_path, text):
+    p = tmp_path / "in.yaml"
+    p.write_text(text)
+    r = subprocess.run([RUNNER, "-i", str(p), "-o", str(tmp_path / "out.yaml")],
+                       capture_output=True, text=True, timeout=600)
+    vals = {}
+    for line in r.stdout.splitlines():
+        m = re.match(r"\s+(\w+) = (\S+)", line)
+        if m:
+            vals.setdefault(m.group(1), []).append(float(m.group(2)))
+    return r, vals
+
+
+def write_ftod(path, gamma, eps, No, magic=b"cc4sFTOD", imag=None, extra_chunk=True,
+               truncate=0):
+    NG, Np, _ = gamma.shape
+    out = bytearray(magic[:8].ljust(8, b"\0"))
+    out += struct.pack("<6i", No, Np - No, NG, 1, 1, 0)
+
+    def chunk(m, payload):
+        return m + struct.pack("<q", 16 + len(payload)) + payload
+
+    if extra_chunk:  # unknown to the reader: skipped by its size
+        out += chunk(b"FTIAreal", np.arange(5, dtype="<f8").tobytes())
+    out += chunk(b"FTODreal", np.asarray(gamma, "<f8").ravel(order="F").tobytes())
+    out += chunk(b"FTODimag", (np.zeros_like(gamma) if imag is None else imag)
+                 .astype("<f8").ravel(order="F").tobytes())
+    out += chunk(b"FTODepsi", np.asarray(eps, "<f8").tobytes())
+    if truncate:
+        out = out[:-truncate]
+    path.write_bytes(bytes(out))
+
+
+NORMS = """
+- name: TensorNorm
+  in: {Data: $CoulombVertex}
+  out: {Norm: $NormG}
+- name: TensorNorm
+  in: {Data: $HoleEigenEnergies}
+  out: {Norm: $NormI}
+- name: TensorNorm
+  in: {Data: $ParticleEigenEnergies}
+  out: {Norm: $NormA}
+"""
+
+
+@pytest.mark.parametrize("magic", [b"cc4sFTOD", b"sisi4sFTOD"])
+def test_ftod_reader(tmp_path, magic):
+    g, ei, ea = philox.system(5, 3, 5, 7)
+    write_ftod(tmp_path / "v.bin", g, np.concatenate([ei, ea]), 3, magic=magic)
+    r, vals = run_steps(tmp_path, """
+- name: CoulombVertexReader
+  in: {file: "%s"}
+  out:
+    CoulombVertex: $CoulombVertex
+    HoleEigenEnergies: $HoleEigenEnergies
+    ParticleEigenEnergies: $ParticleEigenEnergies
+""" % (tmp_path / "v.bin") + NORMS)
+    assert r.returncode == 0, r.stderr
+    assert vals["Norm"] == pytest.approx([np.linalg.norm(g), np.linalg.norm(ei),
+                                          np.linalg.norm(ea)], rel=1e-15)
+
+
+@pytest.mark.parametrize("case", ["imaginary", "magic", "truncated"])
+def test_ftod_reader_rejects(tmp_path, case):
+    g, ei, ea = philox.system(5, 3, 5, 7)
+    kw = {}
+    if case == "imaginary":
+        kw["imag"] = 1e-3 * g
+    elif case == "magic":
+        kw["magic"] = b"FTODxxxx"
+    else:
+        kw["truncate"] = 9
+    write_ftod(tmp_path / "v.bin", g, np.concatenate([ei, ea]), 3, **kw)
+    r, _ = run_steps(tmp_path, """
+- name: CoulombVertexReader
+  in: {file: "%s"}
+  out: {CoulombVertex: $G, HoleEigenEnergies: $I, ParticleEigenEnergies: $A}
+""" % (tmp_path / "v.bin"))
+    assert r.returncode != 0
+    assert {"imaginary": "imaginary part", "magic": "Invalid file format",
+            "truncated": "Invalid file format"}[case] in r.stderr
+
+
+def write_cc4s_tensor(base, x, scalar="Real64", elements="IeeeBinaryFile", types=None):
+    dims = "\n".join("  - length: %d\n    type: %s" % (n, (types or ["State"] * x.ndim)[k])
+                     for k, n in enumerate(x.shape))
+    base.with_suffix(".yaml").write_text(
+        "version: 100\ntype: Tensor\nscalarType: %s\ndimensions:\n%s\nelements:\n  type: %s\nunit: 1.0\n"
+        % (scalar, dims, elements))
+    flat = x.ravel(order="F")
+    if elements == "TextFile":
+        base.with_suffix(".elements").write_text("".join("%.17g\n" % v for v in flat))
+    elif scalar == "Complex64":
+        base.with_suffix(".elements").write_bytes(flat.astype("<c16").tobytes())
+    else:
+        base.with_suffix(".elements").write_bytes(flat.astype("<f8").tobytes())
+
+
+@pytest.mark.parametrize("scalar,elements", [("Real64", "TextFile"), ("Real64", "IeeeBinaryFile"),
+                                             ("Complex64", "IeeeBinaryFile")])
+def test_read_and_define_holes_and_particles(tmp_path, scalar, elements):
+    g, ei, ea = philox.system(9, 4, 6, 11)
+    x = g.astype(complex) if scalar == "Complex64" else g
+    write_cc4s_tensor(tmp_path / "CoulombVertex", x, scalar, elements,
+                      ["AuxiliaryField", "State", "State"])
+    eps = np.concatenate([ei, ea])
+    (tmp_path / "EigenEnergies.yaml").write_text(
+        "metaData:\n  fermiEnergy: %.17g\n  energies:\n%s\n"
+        % (0.5 * (ei.max() + ea.min()), "\n".join("    - %.17g" % v for v in eps)))
+    r, vals = run_steps(tmp_path, """
+- name: DefineHolesAndParticles
+  in: {fileName: "%s"}
+  out: {HoleEigenEnergies: $HoleEigenEnergies, ParticleEigenEnergies: $ParticleEigenEnergies}
+- name: Read
+  in: {fileName: "%s"}
+  out: {destination: $CoulombVertex}
+""" % (tmp_path / "EigenEnergies.yaml", tmp_path / "CoulombVertex.yaml") + NORMS)
+    assert r.returncode == 0, r.stderr
+    assert vals["Norm"] == pytest.approx([np.linalg.norm(g), np.linalg.norm(ei),
+                                          np.linalg.norm(ea)], rel=1e-15)
+    assert "No: 4" in (tmp_path / "out.yaml").read_text()
+
+
+def test_read_rejects_complex_vertex_and_bad_headers(tmp_path):
+    g, _, _ = philox.system(9, 4, 6, 11)
+    write_cc4s_tensor(tmp_path / "Z", g + 1j * g, "Complex64", "IeeeBinaryFile")
+    r, _ = run_steps(tmp_path, "- name: Read\n  in: {fileName: \"%s\"}\n  out: {destination: $Z}\n"
+                     % (tmp_path / "Z.yaml"))
+    assert r.returncode != 0 and "imaginary part" in r.stderr
+    (tmp_path / "B.yaml").write_text("version: 200\ntype: Tensor\n")
+    r, _ = run_steps(tmp_path, "- name: Read\n  in: {fileName: \"%s\"}\n  out: {destination: $B}\n"
+                     % (tmp_path / "B.yaml"))
+    assert r.returncode != 0 and "version" in r.stderr
+
+
+@pytest.mark.gpu
+def test_ueg_pipeline_from_files(tmp_path):
+    """The reference's UEG test (integration-tests/tests/cc4s/ueg/
+    rs1.0-7occ-26virt/in.yaml: DefineHolesAndParticles, Read, integrals,
+    CCSD with DIIS(4), (T)) on cc4s-format files holding the oracle's
+    real-orbital UEG vertex: the committed MP2, CCSD and (T) energies."""
+    from oracle import ueg
+    g, ei, ea, _ = ueg.system(7, 26, 1.0)
+    write_cc4s_tensor(tmp_path / "CoulombVertex", g.astype(complex), "Complex64", "IeeeBinaryFile",
+                      ["AuxiliaryField", "State", "State"])
+    (tmp_path / "EigenEnergies.yaml").write_text(
+        "metaData:\n  fermiEnergy: %.17g\n  energies: [%s]\n"
+        % (0.5 * (ei.max() + ea.min()), ", ".join("%.17g" % v for v in np.concatenate([ei, ea]))))
+    announce("ueg pipeline from cc4s-format files")
+    r, vals = run_steps(tmp_path, """
+- name: DefineHolesAndParticles
+  in: {fileName: "%s"}
+  out: {HoleEigenEnergies: $HoleEigenEnergies, ParticleEigenEnergies: $ParticleEigenEnergies}
+- name: Read
+  in: {fileName: "%s"}
+  out: {destination: $CoulombVertex}
+- name: CoulombIntegralsFromVertex
+  in: {CoulombVertex: $CoulombVertex, HoleEigenEnergies: $HoleEigenEnergies,
+       ParticleEigenEnergies: $ParticleEigenEnergies, complex: 0}
+  out: {PPHHCoulombIntegrals: $PPHHCoulombIntegrals, HHHPCoulombIntegrals: $HHHPCoulombIntegrals}
+- name: CcsdEnergyFromCoulombIntegrals
+  in: {mixer: DiisMixer, maxResidua: 4, maxIterations: 10, energyConvergence: 1e-8,
+       amplitudesConvergence: 1e-8, HoleEigenEnergies: $HoleEigenEnergies,
+       ParticleEigenEnergies: $ParticleEigenEnergies, CoulombVertex: $CoulombVertex,
+       PPHHCoulombIntegrals: $PPHHCoulombIntegrals}
+  out: {CcsdEnergy: $CcsdEnergy, CcsdSinglesAmplitudes: $T1, CcsdDoublesAmplitudes: $T2}
+- name: CcsdPerturbativeTriples
+  in: {HoleEigenEnergies: $HoleEigenEnergies, ParticleEigenEnergies: $ParticleEigenEnergies,
+       PPHHCoulombIntegrals: $PPHHCoulombIntegrals, HHHPCoulombIntegrals: $HHHPCoulombIntegrals,
+       CoulombVertex: $CoulombVertex, CcsdEnergy: $CcsdEnergy, CcsdSinglesAmplitudes: $T1,
+       CcsdDoublesAmplitudes: $T2}
+  out: {CcsdPerturbativeTriplesEnergy: $CcsdT}
+""" % (tmp_path / "EigenEnergies.yaml", tmp_path / "CoulombVertex.yaml"))
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert abs(vals["CcsdEnergy"][0] - -0.39269658954585018) <= 1e-12
+    assert abs(vals["CcsdPerturbativeTriplesEnergy"][0]
+               - (-0.39269658954585018 - 0.0063019625641725016)) <= 1e-12
