@@ -186,6 +186,12 @@ int sisi_ccsd_solve(sisi_engine *e, const sisi_ccsd_opts *opts,
 /* storeAmplitudes (cxx:288-299): copy the current T1/T2 to the host. */
 int sisi_ccsd_get_amplitudes(sisi_engine *e, double *T1, double *T2);
 
+/* PairEnergy of getEnergy (ClusterSinglesDoublesAlgorithm.cxx:218-235,
+ * non-antisymmetrized branch) for the current amplitudes:
+ * pair[i + No*j] = sum_ab tau_abij (2 V_abij - V_baij), tau = T2 + T1 T1;
+ * the pairs sum to the correlation energy. */
+int sisi_ccsd_pair_energies(sisi_engine *e, double *pair);
+
 /* Per-iteration energies of the current run (energy.value per iteration,
  * cxx:185-213); *n receives the count (at most max). */
 int sisi_ccsd_get_energies(sisi_engine *e, double *energies, int max, int *n);

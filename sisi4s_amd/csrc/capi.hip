@@ -280,6 +280,18 @@ int sisi_ccsd_get_amplitudes(sisi_engine *h, double *T1, double *T2) {
   });
 }
 
+int sisi_ccsd_pair_energies(sisi_engine *h, double *pair) {
+  return guarded([&] {
+    ENGINE_REQUIRED(h);
+    Engine &e = h->e;
+    SISI_REQUIRE(pair, SISI_ERR_INVALID, "null pointer");
+    SISI_REQUIRE(e.run_active, SISI_ERR_STATE, "no CCSD run");
+    DBuf p(&e.pool, size_t(e.No) * e.No);
+    ccsd_pair_energies(e, e.T1.get(), e.T2.get(), p.get());
+    e.download(pair, p, size_t(e.No) * e.No);
+  });
+}
+
 int sisi_ccsd_get_energies(sisi_engine *h, double *energies, int max, int *n) {
   return guarded([&] {
     ENGINE_REQUIRED(h);

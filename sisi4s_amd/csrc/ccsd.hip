@@ -552,6 +552,16 @@ void ccsd_energy(Engine &e, const double *T1, const double *T2, double &E, doubl
   E = direct + exchange;
 }
 
+void ccsd_pair_energies(Engine &e, const double *T1, const double *T2, double *pair) {
+  SISI_REQUIRE(e.have_integrals, SISI_ERR_STATE, "pair energies: integrals not built");
+  const int64_t o = e.No, v = e.Nv;
+  DBuf Xb(&e.pool, v * v * o * o);
+  add_t1t1(e, Xb.get(), T2, T1, 1.0, int(v), int(o));  // tau = T2 + T1 T1
+  // the four terms of cxx:226-231 are tau_abij (2 V_abij - V_baij) = tau U
+  ctr(e, 1.0, "abij,abij->ij", V4(Xb.get(), v, v, o, o), V4(e.Uabij.get(), v, v, o, o), 0.0,
+      V2(pair, o, o), SISI_FP64);
+}
+
 // ---------------------------------------------------------------------------
 // Driver
 // ---------------------------------------------------------------------------

@@ -25,6 +25,11 @@ void ccsd_residual_delta(Engine &e, int slice_size, const double *T1, const doub
 void ccsd_energy(Engine &e, const double *T1, const double *T2, double &E, double &direct,
                  double &exchange);
 
+// PairEnergy of getEnergy (ClusterSinglesDoublesAlgorithm.cxx:218-235,
+// non-antisymmetrized): e_ij = sum_ab (T2 + T1 T1)_abij (2 V_abij - V_baij);
+// pair is a device [No, No] buffer (i fastest).
+void ccsd_pair_energies(Engine &e, const double *T1, const double *T2, double *pair);
+
 // Driver (ClusterSinglesDoublesAlgorithm.cxx:38-128).
 void ccsd_begin(Engine &e, const sisi_ccsd_opts &opts, const double *T1h, const double *T2h);
 void ccsd_iterate(Engine &e, int n);

@@ -102,6 +102,7 @@ SIGNATURES = [
       ctypes.POINTER(CcsdReport)]),
     ("sisi_ccsd_get_amplitudes", ctypes.c_int, [ctypes.c_void_p, _c_double_p, _c_double_p]),
     ("sisi_ccsd_get_energies", ctypes.c_int, [ctypes.c_void_p, _c_double_p, ctypes.c_int, _c_int_p]),
+    ("sisi_ccsd_pair_energies", ctypes.c_int, [ctypes.c_void_p, _c_double_p]),
     ("sisi_triples_energy", ctypes.c_int,
      [ctypes.c_void_p, _c_double_p, _c_double_p, ctypes.c_int, _c_double_p]),
     ("sisi_triples_energy_blocks", ctypes.c_int,
@@ -372,6 +373,12 @@ class Engine:
         t2 = np.empty(v * v * o * o)
         _check(self._lib.sisi_ccsd_get_amplitudes(self._h, _ptr(t1), _ptr(t2)))
         return _unbuf(t1, (v, o)), _unbuf(t2, (v, v, o, o))
+
+    def pair_energies(self):
+        """PairEnergy [No, No] of the current amplitudes."""
+        p = np.empty(self.No * self.No)
+        _check(self._lib.sisi_ccsd_pair_energies(self._h, _ptr(p)))
+        return _unbuf(p, (self.No, self.No))
 
     def energies(self):
         n = ctypes.c_int()

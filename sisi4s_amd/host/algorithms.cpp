@@ -149,7 +149,6 @@ class CcsdEnergyFromCoulombIntegrals : public Algorithm {
     if (getIntegerArgument("unrestricted", 0) != 0 || getIntegerArgument("antisymmetrize", 0) != 0)
       throw StepError("unrestricted / antisymmetrized energies are not supported");
     if (isArgumentGiven("HPFockMatrix")) throw StepError("HPFockMatrix is not supported");
-    if (isArgumentGiven("PairEnergy")) throw StepError("PairEnergy is not supported");
     sisi_engine *e = engine_of(*this);
     const int64_t o = getTensorArgument("HoleEigenEnergies").lens[0];
     const int64_t v = getTensorArgument("ParticleEigenEnergies").lens[0];
@@ -200,6 +199,16 @@ class CcsdEnergyFromCoulombIntegrals : public Algorithm {
     emitted.scalars.push_back({"converged", r.converged ? "true" : "false"});
     emitted.scalars.push_back({"fp64Iterations", std::to_string(r.fp64_iterations)});
     emitted.scalars.push_back({"energy", num(r.energy)});
+    // energy decomposition of getEnergy (ClusterSinglesDoublesAlgorithm.cxx:180-190)
+    emitted.scalars.push_back({"direct", num(r.direct)});
+    emitted.scalars.push_back({"exchange", num(r.exchange)});
+    emitted.scalars.push_back({"singlet", num(0.25 * r.direct - 0.5 * r.exchange)});
+    emitted.scalars.push_back({"triplet", num(0.75 * r.direct + 1.5 * r.exchange)});
+    if (isArgumentGiven("PairEnergy")) {  // cxx:218-235
+      auto P = make_tensor({o, o});
+      check(sisi_ccsd_pair_energies(e, P->values.data()), "sisi_ccsd_pair_energies");
+      allocatedTensorArgument("PairEnergy", P);
+    }
     if (isArgumentGiven("CcsdSinglesAmplitudes") || isArgumentGiven("CcsdDoublesAmplitudes")) {
       auto T1 = make_tensor({v, o}), T2 = make_tensor({v, v, o, o});
       check(sisi_ccsd_get_amplitudes(e, T1->values.data(), T2->values.data()),

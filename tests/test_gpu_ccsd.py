@@ -97,6 +97,21 @@ def test_config1_diis6_30_iterations():
     assert err <= 1e-8 * float(np.abs(ref["T2"]).max()), err
 
 
+def test_pair_energies_match_oracle_and_sum_to_energy():
+    """PairEnergy (ClusterSinglesDoublesAlgorithm.cxx:218-235) of the
+    converged config-1 amplitudes: the oracle's pair energies, summing to the
+    correlation energy."""
+    from sisi4s_amd import engine
+    e, g, ei, ea = make_engine(20261202, 10, 50, 300)
+    announce("pair energies config1")
+    rep = e.ccsd_solve(engine.default_opts(mixer="DiisMixer", max_residua=6, max_iterations=12))
+    T1, T2 = e.amplitudes()
+    P = e.pair_energies()
+    ref = ccsd.pair_energy(T1, T2, ccsd.integrals_from_vertex(g, 10)["PPHH"])
+    assert np.abs(P - ref).max() <= 1e-11 * np.abs(ref).max()
+    assert abs(P.sum() - rep["energy"]) <= 1e-10 * abs(rep["energy"])
+
+
 def test_linear_mixer_and_convergence_flag():
     from sisi4s_amd import engine
     No, Nv, NG = 4, 9, 20

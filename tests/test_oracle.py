@@ -119,6 +119,22 @@ def test_mp2_driver_matches_closed_form():
     assert abs(r["energies"][0] - e_cf) <= 1e-13 * abs(e_cf)
 
 
+def test_pair_energies_sum_to_energy():
+    # e_ij = sum_ab tau (2V - V') over pairs must add up to direct + exchange
+    g, ei, ea = philox.system(4, 4, 7, 12)
+    T1, T2 = random_amplitudes(7, 4, seed=1)
+    V = ccsd.integrals_from_vertex(g, 4)["PPHH"]
+    P = ccsd.pair_energy(T1, T2, V)
+    E = ccsd.energy(T1, T2, V)[0]
+    assert P.shape == (4, 4) and abs(P.sum() - E) <= 1e-12 * abs(E)
+    # a pair energy only sees its own (i,j) amplitudes
+    T2b = T2.copy()
+    T2b[:, :, 1, 2] += 0.1
+    Pb = ccsd.pair_energy(T1, T2b, V)
+    changed = np.argwhere(np.abs(Pb - P) > 0)
+    assert changed.tolist() == [[1, 2]]
+
+
 def test_diis_weights():
     rng = np.random.default_rng(0)
     mix = ccsd.DiisMixer(3)

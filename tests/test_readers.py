@@ -159,7 +159,8 @@ def test_ueg_pipeline_from_files(tmp_path):
     """The reference's UEG test (integration-tests/tests/cc4s/ueg/
     rs1.0-7occ-26virt/in.yaml: DefineHolesAndParticles, Read, integrals,
     CCSD with DIIS(4), (T)) on cc4s-format files holding the oracle's
-    real-orbital UEG vertex: the committed MP2, CCSD and (T) energies."""
+    real-orbital UEG vertex: the committed MP2, CCSD and (T) energies, and
+    the CCSD pair energies against the oracle's."""
     from oracle import ueg
     g, ei, ea, _ = ueg.system(7, 26, 1.0)
     write_cc4s_tensor(tmp_path / "CoulombVertex", g.astype(complex), "Complex64", "IeeeBinaryFile",
@@ -184,7 +185,11 @@ def test_ueg_pipeline_from_files(tmp_path):
        amplitudesConvergence: 1e-8, HoleEigenEnergies: $HoleEigenEnergies,
        ParticleEigenEnergies: $ParticleEigenEnergies, CoulombVertex: $CoulombVertex,
        PPHHCoulombIntegrals: $PPHHCoulombIntegrals}
-  out: {CcsdEnergy: $CcsdEnergy, CcsdSinglesAmplitudes: $T1, CcsdDoublesAmplitudes: $T2}
+  out: {CcsdEnergy: $CcsdEnergy, CcsdSinglesAmplitudes: $T1, CcsdDoublesAmplitudes: $T2,
+        PairEnergy: $PairEnergy}
+- name: TensorNorm
+  in: {Data: $PairEnergy}
+  out: {Norm: $NormP}
 - name: CcsdPerturbativeTriples
   in: {HoleEigenEnergies: $HoleEigenEnergies, ParticleEigenEnergies: $ParticleEigenEnergies,
        PPHHCoulombIntegrals: $PPHHCoulombIntegrals, HHHPCoulombIntegrals: $HHHPCoulombIntegrals,
@@ -196,3 +201,11 @@ def test_ueg_pipeline_from_files(tmp_path):
     assert abs(vals["CcsdEnergy"][0] - -0.39269658954585018) <= 1e-12
     assert abs(vals["CcsdPerturbativeTriplesEnergy"][0]
                - (-0.39269658954585018 - 0.0063019625641725016)) <= 1e-12
+    # PairEnergy of ClusterSinglesDoublesAlgorithm.cxx:218-235 from the same
+    # ten DIIS(4) iterations run by the oracle
+    from oracle import ccsd
+    ref = ccsd.solve(g, ei, ea, mixer="DiisMixer", max_residua=4, max_iterations=10,
+                     energy_convergence=1e-8, amplitudes_convergence=1e-8)
+    P = ccsd.pair_energy(ref["T1"], ref["T2"], ccsd.integrals_from_vertex(g, 7)["PPHH"])
+    assert abs(P.sum() - vals["CcsdEnergy"][0]) <= 1e-12
+    assert vals["Norm"][0] == pytest.approx(np.linalg.norm(P), rel=1e-10)
