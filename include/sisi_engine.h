@@ -86,6 +86,13 @@ typedef struct {
                                     SISI_BF16 (default), SISI_BF16X3 or
                                     SISI_FP32; their error scales with the
                                     change since the anchor               */
+  int ppl;                       /* ppl (CcsdEnergyFromCoulombIntegrals.cxx:
+                                    324-338), default 1; 0 drops the
+                                    particle-particle ladder              */
+  int distinguishable;           /* distinguishable (cxx:123-124), default
+                                    0; nonzero selects the distinguishable-
+                                    cluster terms (cxx:150-187, 244-246,
+                                    283-285, 316-320)                     */
 } sisi_ccsd_opts;
 
 typedef struct {
@@ -138,6 +145,12 @@ int sisi_get_integrals(sisi_engine *e, int which, double *out);
 
 /* Precision used by the dense contractions of the next residual call. */
 int sisi_engine_set_precision(sisi_engine *e, int precision);
+
+/* Residual variant of the next residual calls: the ppl and distinguishable
+ * arguments of CcsdEnergyFromCoulombIntegrals (cxx:123-124, 324-338);
+ * default ppl = 1, distinguishable = 0.  sisi_ccsd_begin / _solve set it
+ * from their opts. */
+int sisi_engine_set_residual_variant(sisi_engine *e, int ppl, int distinguishable);
 
 /* CcsdEnergyFromCoulombIntegrals::getResiduum, real path
  * (CcsdEnergyFromCoulombIntegrals.cxx:27-494) with the vertex-built ladder

@@ -95,9 +95,14 @@ def ppl_ladder(dGab, X, slice_size, R2):
             slice_into_residuum(Rxyij, xs, ys, R2)
 
 
-def residual(it, T1, T2, gamma, ints, slice_size=None, initial_doubles=False):
+def residual(it, T1, T2, gamma, ints, slice_size=None, initial_doubles=False,
+             ppl=True, distinguishable=False):
     """CcsdEnergyFromCoulombIntegrals::getResiduum, real path (cxx:27-494),
-    distinguishable = 0, ppl = 1, no CoulombFactors.  Returns (R1, R2)."""
+    no CoulombFactors.  distinguishable (cxx:123-124) selects the
+    distinguishable-cluster terms (cxx:150-160, 176-186, 244-246, 283-285,
+    316-320); ppl = False (cxx:324-333) skips the particle-particle ladder
+    and contracts the last hole-ladder term with T2 + T1 T1.  Returns
+    (R1, R2)."""
     Nv, No = T1.shape
     Vabij = ints["PPHH"]
     R1 = np.zeros_like(T1)
@@ -115,12 +120,22 @@ def residual(it, T1, T2, gamma, ints, slice_size=None, initial_doubles=False):
     # Kac, Lac, Kki, Lki (cxx:141-189)
     Kac = -2.0 * ein("cdkl,adkl->ac", Vabij, X)
     Kac += ein("dckl,adkl->ac", Vabij, X)
-    Lac = Kac.copy()
+    if distinguishable:                                       # cxx:152-160
+        Y = T2 + 2.0 * ein("ai,bj->abij", T1, T1)
+        Lac = -1.0 * ein("cdkl,adkl->ac", Vabij, Y)
+        Lac += 0.5 * ein("dckl,adkl->ac", Vabij, Y)
+    else:
+        Lac = Kac.copy()
     Lac += 2.0 * ein("Gca,Gdk,dk->ac", Gab, Gai, T1)
     Lac += -1.0 * ein("Gck,Gda,dk->ac", Gai, Gab, T1)
     Kki = 2.0 * ein("cdkl,cdil->ki", Vabij, X)
     Kki += -1.0 * ein("dckl,cdil->ki", Vabij, X)
-    Lki = Kki.copy()
+    if distinguishable:                                       # cxx:178-186
+        Y = T2 + 2.0 * ein("ai,bj->abij", T1, T1)
+        Lki = ein("cdkl,cdil->ki", Vabij, Y)
+        Lki += -0.5 * ein("dckl,cdil->ki", Vabij, Y)
+    else:
+        Lki = Kki.copy()
     Lki += 2.0 * ein("klic,cl->ki", Vijka, T1)
     Lki += -1.0 * ein("lkic,cl->ki", Vijka, T1)
     R2 += ein("ac,cbij->abij", Lac, T2)                       # cxx:192
@@ -138,7 +153,8 @@ def residual(it, T1, T2, gamma, ints, slice_size=None, initial_doubles=False):
     Y = T2 + 2.0 * ein("ai,bj->abij", T1, T1)
     Xakic += -0.5 * ein("dclk,dail->akic", Vabij, Y)
     Xakic += ein("dclk,adil->akic", Vabij, T2)
-    Xakic += -0.5 * ein("cdlk,adil->akic", Vabij, T2)
+    if not distinguishable:                                   # cxx:244-246
+        Xakic += -0.5 * ein("cdlk,adil->akic", Vabij, T2)
     Y = 2.0 * T2 - ein("bckj->cbkj", T2)                      # cxx:248-249
     R2 += ein("akic,cbkj->abij", Xakic, Y)
 
@@ -146,7 +162,8 @@ def residual(it, T1, T2, gamma, ints, slice_size=None, initial_doubles=False):
     dGab = Gab - ein("Gcl,al->Gac", Gai, T1)
     dGij = Gij + ein("Gdk,di->Gki", Gai, T1)
     Xakci = ein("Gac,Gki->akci", dGab, dGij)
-    Xakci += -0.5 * ein("cdlk,dail->akci", Vabij, T2)
+    if not distinguishable:                                   # cxx:283-285
+        Xakci += -0.5 * ein("cdlk,dail->akci", Vabij, T2)
     R2 += -1.0 * ein("akci,cbkj->abij", Xakci, T2)
     R2 += -1.0 * ein("bkci,ackj->abij", Xakci, T2)
     R2 += ein("baji->abij", R2)                               # cxx:292
@@ -158,12 +175,16 @@ def residual(it, T1, T2, gamma, ints, slice_size=None, initial_doubles=False):
     Xklij += ein("klic,cj->klij", Vijka, T1)
     Xklij += ein("lkjc,ci->klij", Vijka, T1)
     R2 += ein("klij,abkl->abij", Xklij, X)
-    Xklij = ein("cdkl,cdij->klij", Vabij, X)
-    R2 += ein("klij,abkl->abij", Xklij, T2)
+    if distinguishable:                                       # cxx:316-320
+        Xklij = ein("ci,cdkl,dj->klij", T1, Vabij, T1)
+    else:
+        Xklij = ein("cdkl,cdij->klij", Vabij, X)
+    R2 += ein("klij,abkl->abij", Xklij, T2 if ppl else X)     # cxx:326-333
 
-    # particle-particle ladder (cxx:386-453)
-    dGab = Gab - ein("Gbk,ak->Gab", Gai, T1)
-    ppl_ladder(dGab, X, slice_size, R2)
+    # particle-particle ladder (cxx:336-453)
+    if ppl:
+        dGab = Gab - ein("Gbk,ak->Gab", Gai, T1)
+        ppl_ladder(dGab, X, slice_size, R2)
 
     # T1 equations (cxx:457-492)
     R1 += ein("ac,ci->ai", Kac, T1)
@@ -293,7 +314,7 @@ class LinearMixer:
 def solve(gamma, eps_i, eps_a, *, mixer="DiisMixer", max_residua=4,
           mixing_ratio=1.0, max_iterations=16, amplitudes_convergence=1e-5,
           energy_convergence=1e-6, level_shift=0.0, slice_size=None,
-          initial=None, callback=None):
+          initial=None, callback=None, ppl=True, distinguishable=False):
     """ClusterSinglesDoublesAlgorithm::run<double> (cxx:38-128).
 
     Returns dict(energy, T1, T2, energies=[per iteration], iterations,
@@ -317,7 +338,8 @@ def solve(gamma, eps_i, eps_a, *, mixer="DiisMixer", max_residua=4,
     i = 0
     while i < max_iterations:
         R1, R2 = residual(i, T[0], T[1], gamma, ints, slice_size,
-                          initial_doubles=initial is not None)
+                          initial_doubles=initial is not None, ppl=ppl,
+                          distinguishable=distinguishable)
         E1, E2 = estimate_amplitudes(R1, R2, T[0], T[1], eps_i, eps_a, level_shift)
         dT = [E1 - T[0], E2 - T[1]]
         mix.append([E1, E2], dT)

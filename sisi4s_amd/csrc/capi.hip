@@ -79,6 +79,8 @@ void sisi_ccsd_opts_default(sisi_ccsd_opts *o) {
   o->switch_threshold = 1e-5;
   o->refine_interval = 0;
   o->refine_precision = SISI_BF16;
+  o->ppl = 1;
+  o->distinguishable = 0;
 }
 
 int sisi_engine_create(const sisi_dims *dims, sisi_engine **out) {
@@ -154,6 +156,14 @@ int sisi_engine_set_precision(sisi_engine *h, int precision) {
   return guarded([&] {
     ENGINE_REQUIRED(h);
     set_precision(h->e, precision);
+  });
+}
+
+int sisi_engine_set_residual_variant(sisi_engine *h, int ppl, int distinguishable) {
+  return guarded([&] {
+    ENGINE_REQUIRED(h);
+    h->e.ppl = ppl != 0;
+    h->e.distinguishable = distinguishable != 0;
   });
 }
 

@@ -98,7 +98,16 @@ void ccsd_residual(Engine &e, int it, bool initial_doubles, int slice_size, cons
   {
     // Kac = -2 V[cdkl] X[adkl] + V[dckl] X[adkl] = -U[cdkl] X[adkl] (cxx:146-147)
     ctr(e, -1.0, "cdkl,adkl->ac", U, X, 0.0, Kac, prec);
-    copy(e, Lac, Kac);  // cxx:151
+    // distinguishable: Lac = -V[cdkl] Y[adkl] + V[dckl] Y[adkl] / 2
+    // = -U[cdkl] Y[adkl] / 2 with Y = T2 + 2 T1 T1 (cxx:150-160)
+    DBuf Ydb;
+    if (e.distinguishable) {
+      Ydb = DBuf(&e.pool, n2);
+      add_t1t1(e, Ydb.get(), T2p, T1p, 2.0, int(v), int(o));
+      ctr(e, -0.5, "cdkl,adkl->ac", U, V4(Ydb.get(), v, v, o, o), 0.0, Lac, prec);
+    } else {
+      copy(e, Lac, Kac);  // cxx:151
+    }
     // + 2 Gamma_Gca Gamma_Gdk T_dk (cxx:162-165)
     DBuf Zb(&e.pool, G);
     ctr(e, 1.0, "Gdk,dk->G", Gai, T1, 0.0, V1(Zb.get(), G), prec);
@@ -109,7 +118,10 @@ void ccsd_residual(Engine &e, int it, bool initial_doubles, int slice_size, cons
     ctr(e, -1.0, "Gck,Gak->ac", Gai, V3(Yb.get(), G, v, o), 1.0, Lac, prec);
     // Kki = 2 V[cdkl] X[cdil] - V[dckl] X[cdil] = U[cdkl] X[cdil] (cxx:172-173)
     ctr(e, 1.0, "cdkl,cdil->ki", U, X, 0.0, Kki, prec);
-    copy(e, Lki, Kki);  // cxx:177
+    if (e.distinguishable)  // Lki = U[cdkl] Y[cdil] / 2 (cxx:176-186)
+      ctr(e, 0.5, "cdkl,cdil->ki", U, V4(Ydb.get(), v, v, o, o), 0.0, Lki, prec);
+    else
+      copy(e, Lki, Kki);  // cxx:177
     ctr(e, 2.0, "klic,cl->ki", Vijka, T1, 1.0, Lki, prec);   // cxx:188
     ctr(e, -1.0, "lkic,cl->ki", Vijka, T1, 1.0, Lki, prec);  // cxx:189
     e.record("Kac", Kacb.get(), v * v);
@@ -151,7 +163,8 @@ void ccsd_residual(Engine &e, int it, bool initial_doubles, int slice_size, cons
     add_t1t1(e, Yb.get(), T2p, T1p, 2.0, int(v), int(o));    // cxx:238-240
     ctr(e, -0.5, "dclk,dail->akic", V, Y, 1.0, Xakic, prec);   // cxx:242
     ctr(e, 1.0, "dclk,adil->akic", V, T2, 1.0, Xakic, prec);   // cxx:243
-    ctr(e, -0.5, "cdlk,adil->akic", V, T2, 1.0, Xakic, prec);  // cxx:245
+    if (!e.distinguishable)
+      ctr(e, -0.5, "cdlk,adil->akic", V, T2, 1.0, Xakic, prec);  // cxx:244-246
     e.record("Xakic", Xb2.get(), v * o * o * v);
     // Y = 2 T[cbkj] - T[bckj] (cxx:248-249)
     axpby(e, n2, 2.0, T2p, 0.0, Yb.get());
@@ -171,7 +184,8 @@ void ccsd_residual(Engine &e, int it, bool initial_doubles, int slice_size, cons
     DBuf Xb3(&e.pool, v * o * v * o);
     View Xakci = V4(Xb3.get(), v, o, v, o);
     ctr(e, 1.0, "Gac,Gki->akci", dGab, dGij, 0.0, Xakci, prec);  // cxx:277-278
-    ctr(e, -0.5, "cdlk,dail->akci", V, T2, 1.0, Xakci, prec);    // cxx:284
+    if (!e.distinguishable)
+      ctr(e, -0.5, "cdlk,dail->akci", V, T2, 1.0, Xakci, prec);  // cxx:283-285
     e.frozen.push_back(Xb3.get());
     e.record("Xakci", Xb3.get(), v * o * v * o);
     ctr(e, -1.0, "akci,cbkj->abij", Xakci, T2, 1.0, R2, prec);   // cxx:288
@@ -194,12 +208,21 @@ void ccsd_residual(Engine &e, int it, bool initial_doubles, int slice_size, cons
     ctr(e, 1.0, "lkjc,ci->klij", Vijka, T1, 1.0, Xklij, prec);  // cxx:309
     e.record("Xklij1", Xb4.get(), o * o * o * o);
     ctr(e, 1.0, "klij,abkl->abij", Xklij, X, 1.0, R2, prec);    // cxx:313
-    ctr(e, 1.0, "cdkl,cdij->klij", V, X, 0.0, Xklij, prec);     // cxx:317
+    if (e.distinguishable) {
+      // Xklij = T[ci] V[cdkl] T[dj] (cxx:319): V contracted with X - T2
+      DBuf Wb(&e.pool, n2);
+      axpby(e, n2, 1.0, Xb.get(), 0.0, Wb.get());
+      axpby(e, n2, -1.0, T2p, 1.0, Wb.get());
+      ctr(e, 1.0, "cdkl,cdij->klij", V, V4(Wb.get(), v, v, o, o), 0.0, Xklij, prec);
+    } else {
+      ctr(e, 1.0, "cdkl,cdij->klij", V, X, 0.0, Xklij, prec);   // cxx:317
+    }
     e.record("Xklij2", Xb4.get(), o * o * o * o);
-    ctr(e, 1.0, "klij,abkl->abij", Xklij, T2, 1.0, R2, prec);   // cxx:330 (ppl = 1)
+    // ppl = 1: the T1 part of this term comes with the ladder (cxx:326-333)
+    ctr(e, 1.0, "klij,abkl->abij", Xklij, e.ppl ? T2 : X, 1.0, R2, prec);
   }
-  {
-    // particle-particle ladder (cxx:374-453)
+  if (e.ppl) {
+    // particle-particle ladder (cxx:336-453)
     if (slice_size <= 0) slice_size = int(o);
     DBuf dGabb(&e.pool, G * v * v);
     View dGab = V3(dGabb.get(), G, v, v);
@@ -272,6 +295,8 @@ void ccsd_residual_delta(Engine &e, int slice_size, const double *T1c, const dou
                          double *R1p, double *R2p) {
   Anchor &an = e.anchor;
   SISI_REQUIRE(an.valid, SISI_ERR_STATE, "refinement residual without an anchor");
+  SISI_REQUIRE(an.ppl == e.ppl && an.distinguishable == e.distinguishable, SISI_ERR_STATE,
+               "refinement residual with another residual variant than its anchor");
   const int64_t o = e.No, v = e.Nv, G = e.NG;
   const int64_t n2 = v * v * o * o, n1 = v * o;
   const int prec = e.precision;
@@ -315,14 +340,27 @@ void ccsd_residual_delta(Engine &e, int slice_size, const double *T1c, const dou
        dLki = V2(dLkib.get(), o, o);
   {
     ctr(e, -1.0, "cdkl,adkl->ac", U, dX, 0.0, dKac, prec);
-    copy(e, dLac, dKac);
+    DBuf dYdb;
+    if (e.distinguishable) {  // change of Y = T2 + 2 T1 T1
+      dYdb = DBuf(&e.pool, n2);
+      DBuf YAb(&e.pool, n2);
+      add_t1t1(e, dYdb.get(), T2p, T1p, 2.0, int(v), int(o));
+      add_t1t1(e, YAb.get(), an.T2.get(), an.T1.get(), 2.0, int(v), int(o));
+      axpby(e, n2, -1.0, YAb.get(), 1.0, dYdb.get());
+      ctr(e, -0.5, "cdkl,adkl->ac", U, V4(dYdb.get(), v, v, o, o), 0.0, dLac, prec);
+    } else {
+      copy(e, dLac, dKac);
+    }
     DBuf Zb(&e.pool, G), Yb(&e.pool, G * v * o);
     ctr(e, 1.0, "Gdk,dk->G", Gai, d1, 0.0, V1(Zb.get(), G), prec);
     ctr(e, 2.0, "Gca,G->ac", Gab, V1(Zb.get(), G), 1.0, dLac, prec);
     ctr(e, 1.0, "Gda,dk->Gak", Gab, d1, 0.0, V3(Yb.get(), G, v, o), prec);
     ctr(e, -1.0, "Gck,Gak->ac", Gai, V3(Yb.get(), G, v, o), 1.0, dLac, prec);
     ctr(e, 1.0, "cdkl,cdil->ki", U, dX, 0.0, dKki, prec);
-    copy(e, dLki, dKki);
+    if (e.distinguishable)
+      ctr(e, 0.5, "cdkl,cdil->ki", U, V4(dYdb.get(), v, v, o, o), 0.0, dLki, prec);
+    else
+      copy(e, dLki, dKki);
     ctr(e, 2.0, "klic,cl->ki", Vijka, d1, 1.0, dLki, prec);
     ctr(e, -1.0, "lkic,cl->ki", Vijka, d1, 1.0, dLki, prec);
     // Lac T2, Lki T2
@@ -367,7 +405,7 @@ void ccsd_residual_delta(Engine &e, int slice_size, const double *T1c, const dou
     YAb.reset();
     ctr(e, -0.5, "dclk,dail->akic", V, V4(dYb.get(), v, v, o, o), 1.0, dXakic, prec);
     ctr(e, 1.0, "dclk,adil->akic", V, d2, 1.0, dXakic, prec);
-    ctr(e, -0.5, "cdlk,adil->akic", V, d2, 1.0, dXakic, prec);
+    if (!e.distinguishable) ctr(e, -0.5, "cdlk,adil->akic", V, d2, 1.0, dXakic, prec);
     // Y2 = 2 T2 - T2^(bc) at T and its change
     DBuf Y2b(&e.pool, n2);
     View Y2 = V4(Y2b.get(), v, v, o, o), dY2 = V4(dYb.get(), v, v, o, o);
@@ -391,7 +429,7 @@ void ccsd_residual_delta(Engine &e, int slice_size, const double *T1c, const dou
         prec);
     ctr(e, 1.0, "Gac,Gki->akci", V3(A("dGab4"), G, v, v), V3(ddGij.get(), G, o, o), 1.0, dXakci,
         prec);
-    ctr(e, -0.5, "cdlk,dail->akci", V, d2, 1.0, dXakci, prec);
+    if (!e.distinguishable) ctr(e, -0.5, "cdlk,dail->akci", V, d2, 1.0, dXakci, prec);
     View XakciA = V4(A("Xakci"), v, o, v, o);
     ctr(e, -1.0, "akci,cbkj->abij", dXakci, T2, 1.0, dR2, prec);
     ctr(e, -1.0, "akci,cbkj->abij", XakciA, d2, 1.0, dR2, prec);
@@ -411,11 +449,18 @@ void ccsd_residual_delta(Engine &e, int slice_size, const double *T1c, const dou
     ctr(e, 1.0, "lkjc,ci->klij", Vijka, d1, 1.0, dXklij, prec);
     ctr(e, 1.0, "klij,abkl->abij", dXklij, Xn, 1.0, dR2, prec);
     ctr(e, 1.0, "klij,abkl->abij", V4(A("Xklij1"), o, o, o, o), dX, 1.0, dR2, prec);
-    ctr(e, 1.0, "cdkl,cdij->klij", V, dX, 0.0, dXklij, prec);
-    ctr(e, 1.0, "klij,abkl->abij", dXklij, T2, 1.0, dR2, prec);
-    ctr(e, 1.0, "klij,abkl->abij", V4(A("Xklij2"), o, o, o, o), d2, 1.0, dR2, prec);
+    if (e.distinguishable) {  // change of T1 V T1: V contracted with dX - d2
+      DBuf dWb(&e.pool, n2);
+      axpby(e, n2, 1.0, dXb.get(), 0.0, dWb.get());
+      axpby(e, n2, -1.0, d2b.get(), 1.0, dWb.get());
+      ctr(e, 1.0, "cdkl,cdij->klij", V, V4(dWb.get(), v, v, o, o), 0.0, dXklij, prec);
+    } else {
+      ctr(e, 1.0, "cdkl,cdij->klij", V, dX, 0.0, dXklij, prec);
+    }
+    ctr(e, 1.0, "klij,abkl->abij", dXklij, e.ppl ? T2 : Xn, 1.0, dR2, prec);
+    ctr(e, 1.0, "klij,abkl->abij", V4(A("Xklij2"), o, o, o, o), e.ppl ? d2 : dX, 1.0, dR2, prec);
   }
-  {
+  if (e.ppl) {
     // ladder: V~(T) X(T) - V~(T_A) X(T_A) = dV~ X(T) + V~(T_A) dX per slice pair,
     // dV~ = dG[n]^T G(T)[m] + G_A[n]^T dG[m] for the dressed Gamma_ab
     if (slice_size <= 0) slice_size = int(o);
@@ -543,6 +588,8 @@ void ccsd_residual_anchor(Engine &e, int slice_size, const double *T1, const dou
   axpby(e, n2, 1.0, R2, 0.0, an.R2.get());
   an.valid = true;
   an.age = 0;
+  an.ppl = e.ppl;
+  an.distinguishable = e.distinguishable;
 }
 
 void ccsd_energy(Engine &e, const double *T1, const double *T2, double &E, double &direct,
@@ -610,6 +657,8 @@ void ccsd_begin(Engine &e, const sisi_ccsd_opts &opts, const double *T1h, const 
   SISI_REQUIRE(opts.mixer != SISI_MIXER_DIIS || opts.max_residua >= 1, SISI_ERR_INVALID,
                "maxResidua must be >= 1");
   e.opts = opts;
+  e.ppl = opts.ppl != 0;
+  e.distinguishable = opts.distinguishable != 0;
   e.T1 = DBuf(&e.pool, e.n1());
   e.T2 = DBuf(&e.pool, e.n2());
   e.run_initial = (T2h != nullptr);

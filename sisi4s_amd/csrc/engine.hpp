@@ -60,6 +60,7 @@ class Timed {
 struct Anchor {
   bool valid = false;
   int age = 0;  // refinement iterations since the anchor was set
+  bool ppl = true, distinguishable = false;  // residual variant it was recorded with
   DBuf T1, T2, R1, R2;
   std::map<std::string, DBuf> v;
 };
@@ -83,6 +84,10 @@ class Engine {
   // precision of general contractions and of the particle-particle ladder
   int precision = SISI_FP64;
   int prec_ladder = SISI_FP64;
+  // residual variant (CcsdEnergyFromCoulombIntegrals.cxx:123-124, 324-338):
+  // ppl = false drops the particle-particle ladder, distinguishable selects
+  // the distinguishable-cluster terms
+  bool ppl = true, distinguishable = false;
 
   // Operands marked constant for the duration of one residual evaluation:
   // the contraction planner caches their permuted dense copies.

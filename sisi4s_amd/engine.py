@@ -43,7 +43,8 @@ class CcsdOpts(ctypes.Structure):
                 ("energy_convergence", ctypes.c_double), ("level_shift", ctypes.c_double),
                 ("slice_size", ctypes.c_int), ("precision", ctypes.c_int),
                 ("switch_threshold", ctypes.c_double), ("refine_interval", ctypes.c_int),
-                ("refine_precision", ctypes.c_int)]
+                ("refine_precision", ctypes.c_int), ("ppl", ctypes.c_int),
+                ("distinguishable", ctypes.c_int)]
 
 
 class CcsdReport(ctypes.Structure):
@@ -85,6 +86,7 @@ SIGNATURES = [
     ("sisi_integrals_from_vertex", ctypes.c_int, [ctypes.c_void_p]),
     ("sisi_get_integrals", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _c_double_p]),
     ("sisi_engine_set_precision", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("sisi_engine_set_residual_variant", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     ("sisi_ccsd_residual", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_double_p, _c_double_p,
       _c_double_p, _c_double_p]),
@@ -310,6 +312,12 @@ class Engine:
         if isinstance(precision, str):
             precision = PRECISIONS[precision]
         _check(self._lib.sisi_engine_set_precision(self._h, int(precision)))
+
+    def set_residual_variant(self, ppl=True, distinguishable=False):
+        """ppl / distinguishable of the next residual calls
+        (CcsdEnergyFromCoulombIntegrals.cxx:123-124, 324-338)."""
+        _check(self._lib.sisi_engine_set_residual_variant(self._h, int(bool(ppl)),
+                                                          int(bool(distinguishable))))
 
     # --- residual / energies ---------------------------------------------
     def residual(self, iteration, T1, T2, slice_size=-1, initial_doubles=False):

@@ -142,9 +142,6 @@ class CcsdEnergyFromCoulombIntegrals : public Algorithm {
   using Algorithm::Algorithm;
   std::string getName() override { return "CcsdEnergyFromCoulombIntegrals"; }
   void run() override {
-    if (getIntegerArgument("distinguishable", 0) != 0)
-      throw StepError("distinguishable = 1 is not supported");
-    if (getIntegerArgument("ppl", 1) != 1) throw StepError("ppl = 0 is not supported");
     if (isArgumentGiven("CoulombFactors")) throw StepError("CoulombFactors is not supported");
     if (getIntegerArgument("unrestricted", 0) != 0 || getIntegerArgument("antisymmetrize", 0) != 0)
       throw StepError("unrestricted / antisymmetrized energies are not supported");
@@ -171,6 +168,8 @@ class CcsdEnergyFromCoulombIntegrals : public Algorithm {
     opts.precision = precision_argument(*this);
     opts.switch_threshold = getRealArgument("switchThreshold", 1e-5);
     opts.refine_interval = int(getIntegerArgument("refineInterval", 0));
+    opts.ppl = getIntegerArgument("ppl", 1) != 0;                          // cxx:324
+    opts.distinguishable = getIntegerArgument("distinguishable", 0) != 0;  // cxx:123-124
     {
       const std::string rp = getTextArgument("refinePrecision", "bf16");
       if (rp == "bf16" || rp == "3") opts.refine_precision = SISI_BF16;
@@ -196,6 +195,8 @@ class CcsdEnergyFromCoulombIntegrals : public Algorithm {
     for (int k = 0; k < n; ++k)
       emitted.iterations.push_back({{"iteration", std::to_string(k + 1)}, {"energy", num(es[k])}});
     emitted.scalars.push_back({"maxIterations", std::to_string(opts.max_iterations)});
+    if (isArgumentGiven("distinguishable"))  // ClusterSinglesDoublesAlgorithm.cxx:69-71
+      emitted.scalars.push_back({"distinguishable", std::to_string(getIntegerArgument("distinguishable"))});
     emitted.scalars.push_back({"converged", r.converged ? "true" : "false"});
     emitted.scalars.push_back({"fp64Iterations", std::to_string(r.fp64_iterations)});
     emitted.scalars.push_back({"energy", num(r.energy)});

@@ -112,6 +112,61 @@ def test_pair_energies_match_oracle_and_sum_to_energy():
     assert abs(P.sum() - rep["energy"]) <= 1e-10 * abs(rep["energy"])
 
 
+@pytest.mark.parametrize("ppl,dist", [(False, False), (True, True), (False, True)])
+def test_residual_variants_match_oracle(ppl, dist):
+    """ppl / distinguishable (CcsdEnergyFromCoulombIntegrals.cxx:123-124,
+    150-187, 244-246, 283-285, 316-338): the FP64 residual against the
+    oracle, the refinement residual's exactness in FP64, and the refusal of
+    a refinement under another variant than its anchor."""
+    from sisi4s_amd import engine
+    No, Nv, NG = 4, 13, 30
+    e, g, ei, ea = make_engine(71, No, Nv, NG)
+    ints = ccsd.integrals_from_vertex(g, No)
+    T1, T2 = random_amplitudes(Nv, No, seed=9)
+    e.set_residual_variant(ppl, dist)
+    for s in (3, 13):
+        announce("residual variant ppl=%d dist=%d slice=%d" % (ppl, dist, s))
+        R1, R2 = e.residual(1, T1, T2, slice_size=s)
+        O1, O2 = ccsd.residual(1, T1, T2, g, ints, slice_size=s, ppl=ppl, distinguishable=dist)
+        scale = float(np.abs(O2).max())
+        assert float(np.abs(R2 - O2).max()) <= 1e-11 * scale
+        assert float(np.abs(R1 - O1).max()) <= 1e-11 * float(np.abs(O1).max()) + 1e-13 * scale
+    D1, D2 = random_amplitudes(Nv, No, seed=10, scale=1e-3)
+    e.residual_anchor(T1, T2, slice_size=3)
+    e.set_precision("fp64")
+    Q1, Q2 = e.residual_refine(T1 + D1, T2 + D2, slice_size=3)
+    P1, P2 = e.residual(1, T1 + D1, T2 + D2, slice_size=3)
+    scale = float(np.abs(P2).max())
+    assert float(np.abs(Q2 - P2).max()) <= 1e-12 * scale
+    assert float(np.abs(Q1 - P1).max()) <= 1e-12 * scale
+    e.set_residual_variant(not ppl, dist)
+    with pytest.raises(engine.EngineError):
+        e.residual_refine(T1 + D1, T2 + D2, slice_size=3)
+
+
+@pytest.mark.parametrize("ppl,dist", [(True, True), (False, False)])
+def test_variant_solves_match_oracle_and_mixed_schedule(ppl, dist):
+    """Whole runs of the variants: per-iteration energies against the oracle's
+    driver (FP64), and the mixed schedule with refinement residuals within
+    1e-6 Eh of the FP64 run."""
+    from sisi4s_amd import engine
+    No, Nv, NG = 5, 20, 80
+    e, g, ei, ea = make_engine(7, No, Nv, NG)
+    kw = dict(mixer="DiisMixer", max_residua=6, max_iterations=150,
+              amplitudes_convergence=1e-8, energy_convergence=1e-10, slice_size=6)
+    announce("variant solve ppl=%d dist=%d fp64" % (ppl, dist))
+    r64 = e.ccsd_solve(engine.default_opts(ppl=int(ppl), distinguishable=int(dist), **kw))
+    ref = ccsd.solve(g, ei, ea, ppl=ppl, distinguishable=dist, **kw)
+    got = e.energies()
+    assert r64["converged"] and ref["converged"] and len(got) == len(ref["energies"])
+    assert np.abs(np.array(got) - ref["energies"]).max() <= 1e-10
+    announce("variant solve ppl=%d dist=%d mixed" % (ppl, dist))
+    rr = e.ccsd_solve(engine.default_opts(precision="bf16x3", switch_threshold=1e-5,
+                                          refine_interval=8, ppl=int(ppl),
+                                          distinguishable=int(dist), **kw))
+    assert rr["converged"] and abs(rr["energy"] - r64["energy"]) <= 1e-6
+
+
 def test_linear_mixer_and_convergence_flag():
     from sisi4s_amd import engine
     No, Nv, NG = 4, 9, 20

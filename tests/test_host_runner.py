@@ -108,3 +108,36 @@ def test_runner_ueg_known_answers(tmp_path):
                - (-0.39269658954585018 - 0.0063019625641725016)) <= 1e-12
     m = re.search(r"iteration: 1, energy: \{value: ([-+0-9.eE]+)\}", out.read_text())
     assert m and abs(float(m.group(1)) - -0.36143028565969504) <= 1e-12
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("args,ppl,dist", [("ppl: 0", False, False),
+                                           ("distinguishable: 1", True, True)])
+def test_runner_residual_variants(tmp_path, args, ppl, dist):
+    """The ppl and distinguishable arguments of CcsdEnergyFromCoulombIntegrals
+    (CcsdEnergyFromCoulombIntegrals.cxx:123-124, 324-338) through the runner,
+    against the oracle's driver."""
+    from oracle import ccsd, philox
+    p = tmp_path / "in.yaml"
+    p.write_text("""
+- name: GenerateSyntheticSystem
+  in: {seed: 7, No: 5, Nv: 20, NG: 80}
+  out: {CoulombVertex: $G, HoleEigenEnergies: $I, ParticleEigenEnergies: $A}
+- name: CoulombIntegralsFromVertex
+  in: {CoulombVertex: $G, HoleEigenEnergies: $I, ParticleEigenEnergies: $A}
+  out: {PPHHCoulombIntegrals: $V}
+- name: CcsdEnergyFromCoulombIntegrals
+  in: {mixer: DiisMixer, maxResidua: 6, maxIterations: 20, integralsSliceSize: 6, %s,
+       HoleEigenEnergies: $I, ParticleEigenEnergies: $A, PPHHCoulombIntegrals: $V,
+       CoulombVertex: $G}
+  out: {CcsdEnergy: $E}
+""" % args)
+    announce("runner variant %s" % args)
+    r = run(["-i", str(p), "-o", str(tmp_path / "out.yaml")])
+    assert r.returncode == 0, r.stdout + r.stderr
+    m = re.search(r"CcsdEnergy = (\S+)", r.stdout)
+    g, ei, ea = philox.system(7, 5, 20, 80)
+    ref = ccsd.solve(g, ei, ea, mixer="DiisMixer", max_residua=6, max_iterations=20,
+                     slice_size=6, ppl=ppl, distinguishable=dist)
+    assert m and abs(float(m.group(1)) - ref["energy"]) <= 1e-10
+    assert ("distinguishable: 1" in (tmp_path / "out.yaml").read_text()) == dist

@@ -135,6 +135,65 @@ def test_pair_energies_sum_to_energy():
     assert changed.tolist() == [[1, 2]]
 
 
+def test_distinguishable_cluster_is_exact_for_two_electrons():
+    """The distinguishable-cluster residual (distinguishable = 1) keeps CCSD's
+    exactness for two electrons: at No = 1 both converge to the same (full
+    CI) energy, while for more electrons the two models differ."""
+    kw = dict(mixer="DiisMixer", max_residua=6, max_iterations=80,
+              amplitudes_convergence=1e-13, energy_convergence=1e-15)
+    for Nv, NG in [(6, 20), (9, 30)]:
+        g, ei, ea = philox.system(5, 1, Nv, NG)
+        a = ccsd.solve(g, ei, ea, **kw)
+        b = ccsd.solve(g, ei, ea, distinguishable=True, **kw)
+        assert a["converged"] and b["converged"]
+        assert abs(a["energy"] - b["energy"]) <= 1e-13 * abs(a["energy"])
+    g, ei, ea = philox.system(5, 3, 8, 20)
+    a = ccsd.solve(g, ei, ea, **kw)
+    b = ccsd.solve(g, ei, ea, distinguishable=True, **kw)
+    assert abs(a["energy"] - b["energy"]) > 1e-3
+
+
+def test_distinguishable_changes_only_nonlinear_terms():
+    # R_dist(lam T) - R_ccsd(lam T) has no constant or linear part in lam
+    g, ei, ea = philox.system(8, 3, 7, 15)
+    ints = ccsd.integrals_from_vertex(g, 3)
+    T1, T2 = random_amplitudes(7, 3, seed=4)
+
+    def diff(lam):
+        a = ccsd.residual(1, lam * T1, lam * T2, g, ints, 2, distinguishable=True)
+        b = ccsd.residual(1, lam * T1, lam * T2, g, ints, 2)
+        return np.concatenate([(a[0] - b[0]).ravel(), (a[1] - b[1]).ravel()])
+
+    d1, d2 = diff(1e-3), diff(2e-3)
+    assert np.linalg.norm(d1) > 0
+    assert abs(np.linalg.norm(d2) / np.linalg.norm(d1) - 4.0) < 0.05
+
+
+@pytest.mark.parametrize("slice_size", [1, 3, 7])
+def test_ppl0_drops_exactly_the_ladder(slice_size):
+    """ppl = 0 leaves out the particle-particle ladder, written here with the
+    stored PPPP/PPPH integrals instead of sliced dressed vertices:
+    sum_cd (V_abcd - T_ak V_bkdc - T_bl V_adcl + T_ak T_bl V_cdkl) X_cdij,
+    and moves the T1 T1 part of the hole-ladder term to the residual."""
+    No, Nv, NG = 3, 7, 15
+    g, ei, ea = philox.system(9, No, Nv, NG)
+    ints = ccsd.integrals_from_vertex(g, No)
+    Vabcd, Vabci = ccsd.vabcd_from_vertex(g, No)
+    V = ints["PPHH"]
+    T1, T2 = random_amplitudes(Nv, No, seed=6)
+    X = T2 + np.einsum("ai,bj->abij", T1, T1)
+    dressed = (Vabcd - np.einsum("ak,bcdk->abcd", T1, Vabci)
+               - np.einsum("bl,adcl->abcd", T1, Vabci)
+               + np.einsum("ak,bl,cdkl->abcd", T1, T1, V))
+    ladder = np.einsum("abcd,cdij->abij", dressed, X)
+    hole = np.einsum("klij,ak,bl->abij", np.einsum("cdkl,cdij->klij", V, X), T1, T1)
+    R1a, R2a = ccsd.residual(1, T1, T2, g, ints, slice_size)
+    R1b, R2b = ccsd.residual(1, T1, T2, g, ints, slice_size, ppl=False)
+    assert np.abs(R1a - R1b).max() == 0.0
+    err = np.abs(R2a - R2b - ladder + hole).max()
+    assert err <= 1e-13 * np.abs(ladder).max(), err
+
+
 def test_diis_weights():
     rng = np.random.default_rng(0)
     mix = ccsd.DiisMixer(3)
