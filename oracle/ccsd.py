@@ -241,8 +241,10 @@ def pair_energy(T1, T2, Vabij):
 
 
 def fock_dot(x, y):
-    """FockVector::dot (FockVector.hpp:224-236) for real components."""
-    return sum(float(np.sum(a * b)) for a, b in zip(x, y))
+    """Real part of FockVector::dot (FockVector.hpp:224-236, dot(x,y) =
+    conj(x) y, MathFunctions.hpp:45-48); the DIIS overlaps take 2 Re of it
+    (DiisMixer.cxx:123) and |dT|^2/|T|^2 is real."""
+    return sum(float(np.real(np.vdot(a, b))) for a, b in zip(x, y))
 
 
 class DiisMixer:

@@ -275,3 +275,69 @@ def test_ueg_known_answers():
 def test_ueg_open_shell_is_rejected():
     with pytest.raises(ValueError):
         ueg.system(5, 28, 1.0)
+
+
+# ---------------------------------------------------------------------------
+# complex path (oracle/ccsd_complex.py)
+# ---------------------------------------------------------------------------
+
+def test_complex_path_equals_real_path_for_real_vertex():
+    """On a real vertex symmetric in (p,q) the complex residual
+    (CcsdEnergyFromCoulombIntegrals.cxx:496-891) and the real one
+    (cxx:27-494) are the same equations."""
+    from oracle import ccsd_complex as cz
+    g, ei, ea = philox.system(5, 3, 8, 20)
+    T1, T2 = random_amplitudes(8, 3, seed=3)
+    ints, zints = ccsd.integrals_from_vertex(g, 3), cz.integrals_from_vertex(g.astype(complex), 3)
+    for ppl, dist in [(True, False), (False, False), (True, True)]:
+        R1, R2 = ccsd.residual(1, T1, T2, g, ints, 3, ppl=ppl, distinguishable=dist)
+        Z1, Z2 = cz.residual(1, T1.astype(complex), T2.astype(complex), g.astype(complex),
+                             zints, 3, ppl=ppl, distinguishable=dist)
+        assert np.abs(Z2 - R2).max() <= 1e-13 * np.abs(R2).max()
+        assert np.abs(Z1 - R1).max() <= 1e-13 * np.abs(R2).max()
+
+
+def test_complex_path_is_gauge_covariant():
+    """A random phase per orbital and a unitary mix of the auxiliary index
+    make the vertex genuinely complex without changing the physics: the
+    residual transforms like the amplitudes (phase e^{-i(t_a+t_b-t_i-t_j)})
+    and every iteration's energy is the real path's, with zero imaginary
+    part.  A conjugate dropped or added anywhere in the term list breaks
+    this."""
+    from oracle import ccsd_complex as cz
+    No, Nv = 3, 8
+    g, ei, ea = philox.system(5, No, Nv, 20)
+    gz, t = cz.gauge_transform(g.astype(complex), No, 1)
+    T1, T2 = random_amplitudes(Nv, No, seed=3)
+    ph1 = np.exp(-1j * (t[No:, None] - t[None, :No]))
+    ph2 = np.exp(-1j * (t[No:, None, None, None] + t[None, No:, None, None]
+                        - t[None, None, :No, None] - t[None, None, None, :No]))
+    R1, R2 = ccsd.residual(1, T1, T2, g, ccsd.integrals_from_vertex(g, No), 3)
+    Z1, Z2 = cz.residual(1, ph1 * T1, ph2 * T2, gz, cz.integrals_from_vertex(gz, No), 3)
+    assert np.abs(Z2 - ph2 * R2).max() <= 1e-12 * np.abs(R2).max()
+    assert np.abs(Z1 - ph1 * R1).max() <= 1e-12 * np.abs(R2).max()
+    kw = dict(mixer="DiisMixer", max_residua=6, max_iterations=12, slice_size=3,
+              amplitudes_convergence=1e-10, energy_convergence=1e-12)
+    a = ccsd.solve(g, ei, ea, **kw)
+    b = cz.solve(gz, ei, ea, **kw)
+    assert np.abs(np.array(b["energies"]) - a["energies"]).max() <= 1e-12
+    assert np.abs(np.imag(b["energies"])).max() <= 1e-13
+
+
+@pytest.mark.parametrize("gauge", [False, True])
+def test_complex_path_ueg_plane_wave_known_answers(gauge):
+    """The plane-wave UEG vertex (not symmetric in p,q; complex after a gauge
+    transformation) through the complex path: the reference's MP2 and
+    ten-iteration CCSD energies (integration-tests/tests/cc4s/ueg/
+    rs1.0-7occ-26virt/cc4s.correct.out.yaml:128, 153)."""
+    from oracle import ccsd_complex as cz
+    pw = ueg.plane_wave(7, 26, 1.0)
+    G = pw["coeff"].astype(complex)
+    if gauge:
+        G, _ = cz.gauge_transform(G, 7, 2)
+    ei, ea = pw["eps"][:7], pw["eps"][7:]
+    r = cz.solve(G, ei, ea, mixer="DiisMixer", max_residua=4, max_iterations=10,
+                 energy_convergence=1e-8, amplitudes_convergence=1e-8)
+    assert abs(r["energies"][0] - -0.36143028565969504) <= 1e-13
+    assert abs(r["energy"] - -0.39269658954585018) <= 1e-13
+    assert abs(r["energy_complex"].imag) <= 1e-13
