@@ -19,6 +19,11 @@
  *  - The caller owns host arrays; the engine owns device memory and one HIP
  *    stream.  A handle is not thread-safe (the reference drives the path
  *    from a single host thread per rank).
+ *  - Complex mode: after sisi_engine_set_vertex_complex the engine runs the
+ *    reference's complex path (getResiduum<complex>); every tensor argument
+ *    of the CCSD entry points (amplitudes, residuals, integrals, pair
+ *    energies) is then complex128, interleaved (re, im), with the same
+ *    element order, i.e. twice as many doubles.  Eigenenergies stay real.
  *  - Precision: SISI_FP64 is the plain path.  SISI_FP32 and SISI_BF16X3 run
  *    the dense contractions on MFMA with FP32 accumulation; energies,
  *    denominators, the DIIS solve and all reductions stay FP64.
@@ -53,6 +58,8 @@ extern "C" {
 #define SISI_HHHH 2 /* V_ijkl  [No,No,No,No] */
 #define SISI_HHHP 3 /* V_ijka  [No,No,No,Nv] */
 #define SISI_HHPP 4 /* V_ijab  [No,No,Nv,Nv] */
+#define SISI_PHHP 5 /* V_aijb  [Nv,No,No,Nv] (complex mode) */
+#define SISI_PHHH 6 /* V_aijk  [Nv,No,No,No] (complex mode) */
 
 typedef struct sisi_engine sisi_engine;
 
@@ -105,6 +112,8 @@ typedef struct {
   int fp64_iterations;/* iterations run with FP64 contractions           */
   int precision;      /* precision of the last iteration                 */
   int status;         /* SISI_OK, or the error of this system in a batch */
+  double energy_imag; /* imaginary part of the energy (complex mode; the
+                         reference reports the real part, cxx:30-33)     */
 } sisi_ccsd_report;
 
 typedef struct {
@@ -135,12 +144,22 @@ int sisi_engine_set_vertex(sisi_engine *e, const double *gamma);
 int sisi_engine_set_eigenenergies(sisi_engine *e, const double *eps_i,
                                   const double *eps_a);
 
+/* Upload a complex vertex Gamma[NG,Np,Np] (complex128, interleaved) and
+ * switch the engine to complex mode: CoulombIntegralsFromVertex with
+ * complex: 1 (calculateComplexIntegrals, CoulombIntegralsFromVertex.cxx:
+ * 613-810) and the complex residual (CcsdEnergyFromCoulombIntegrals.cxx:
+ * 496-891, ClusterSinglesDoublesAlgorithm.cxx:490-565), which the
+ * reference selects when the integrals are complex (cxx:25-35).  No
+ * symmetry of Gamma is assumed.  sisi_engine_set_vertex switches back. */
+int sisi_engine_set_vertex_complex(sisi_engine *e, const double *gamma);
+
 /* CoulombIntegralsFromVertex::calculateRealIntegrals
  * (CoulombIntegralsFromVertex.cxx:203-571): builds PPHH, PHPH, HHHH, HHHP
  * (and HHPP as a transposed view) on the device. */
 int sisi_integrals_from_vertex(sisi_engine *e);
 
-/* Copy one integral class (SISI_PPHH ...) to host memory. */
+/* Copy one integral class (SISI_PPHH ...) to host memory (complex128 in
+ * complex mode, where HHPP, PHHP and PHHH are the reference's own classes). */
 int sisi_get_integrals(sisi_engine *e, int which, double *out);
 
 /* Precision used by the dense contractions of the next residual call. */

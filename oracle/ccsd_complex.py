@@ -182,6 +182,13 @@ def energy(T1, T2, Vijab):
     return direct + exchange, direct, exchange
 
 
+def pair_energy(T1, T2, Vijab):
+    """PairEnergy with F = complex (ClusterSinglesDoublesAlgorithm.cxx:
+    218-235, non-antisymmetrized branch)."""
+    tau = T2 + ein("ai,bj->abij", T1, T1)
+    return 2.0 * ein("abij,ijab->ij", tau, Vijab) - ein("abij,ijba->ij", tau, Vijab)
+
+
 def solve(gamma, eps_i, eps_a, *, mixer="DiisMixer", max_residua=4,
           mixing_ratio=1.0, max_iterations=16, amplitudes_convergence=1e-5,
           energy_convergence=1e-6, level_shift=0.0, slice_size=None,

@@ -102,9 +102,27 @@ int sisi_engine_set_vertex(sisi_engine *h, const double *gamma) {
     const size_t n = size_t(e.NG) * e.Np * e.Np;
     SISI_REQUIRE(gamma != nullptr, SISI_ERR_INVALID, "null CoulombVertex");
     check_finite(gamma, n, "CoulombVertex");
+    e.cplx = false;
     e.upload(e.gamma, gamma, n);
     e.have_vertex = true;
     e.have_integrals = false;
+    e.run_active = false;
+    e.drop_triples_cache();
+  });
+}
+
+int sisi_engine_set_vertex_complex(sisi_engine *h, const double *gamma) {
+  return guarded([&] {
+    ENGINE_REQUIRED(h);
+    Engine &e = h->e;
+    const size_t n = 2 * size_t(e.NG) * e.Np * e.Np;
+    SISI_REQUIRE(gamma != nullptr, SISI_ERR_INVALID, "null CoulombVertex");
+    check_finite(gamma, n, "CoulombVertex");
+    e.cplx = true;
+    e.upload_t(e.gamma, gamma, n);
+    e.have_vertex = true;
+    e.have_integrals = false;
+    e.run_active = false;
     e.drop_triples_cache();
   });
 }
@@ -135,6 +153,19 @@ int sisi_get_integrals(sisi_engine *h, int which, double *out) {
     Engine &e = h->e;
     SISI_REQUIRE(e.have_integrals, SISI_ERR_STATE, "integrals not built");
     const size_t o = size_t(e.No), v = size_t(e.Nv);
+    if (e.cplx) {  // complex128, interleaved
+      switch (which) {
+        case SISI_PPHH: e.download_t(out, e.Vabij, 2 * v * v * o * o); break;
+        case SISI_PHPH: e.download_t(out, e.Vaibj, 2 * v * o * v * o); break;
+        case SISI_HHHH: e.download_t(out, e.Vijkl, 2 * o * o * o * o); break;
+        case SISI_HHHP: e.download_t(out, e.Vijka, 2 * o * o * o * v); break;
+        case SISI_HHPP: e.download_t(out, e.Vijab, 2 * o * o * v * v); break;
+        case SISI_PHHP: e.download_t(out, e.Vaijb, 2 * v * o * o * v); break;
+        case SISI_PHHH: e.download_t(out, e.Vaijk, 2 * v * o * o * o); break;
+        default: throw Error(SISI_ERR_INVALID, "unknown integral class");
+      }
+      return;
+    }
     switch (which) {
       case SISI_PPHH: e.download(out, e.Vabij, v * v * o * o); break;
       case SISI_PHPH: e.download(out, e.Vaibj, v * o * v * o); break;
@@ -175,12 +206,12 @@ int sisi_ccsd_residual(sisi_engine *h, int iteration, int initial_doubles, int s
     SISI_REQUIRE(T1 && T2 && R1 && R2, SISI_ERR_INVALID, "null amplitude/residual pointer");
     SISI_REQUIRE(iteration >= 0, SISI_ERR_INVALID, "negative iteration");
     DBuf t1, t2, r1(&e.pool, e.n1()), r2(&e.pool, e.n2());
-    e.upload(t1, T1, e.n1());
-    e.upload(t2, T2, e.n2());
+    e.upload_t(t1, T1, e.n1());
+    e.upload_t(t2, T2, e.n2());
     ccsd_residual(e, iteration, initial_doubles != 0, slice_size, t1.get(), t2.get(), r1.get(),
                   r2.get());
-    e.download(R1, r1, e.n1());
-    e.download(R2, r2, e.n2());
+    e.download_t(R1, r1, e.n1());
+    e.download_t(R2, r2, e.n2());
   });
 }
 
@@ -217,8 +248,8 @@ int sisi_ccsd_energy(sisi_engine *h, const double *T1, const double *T2, double 
     Engine &e = h->e;
     SISI_REQUIRE(T1 && T2 && out, SISI_ERR_INVALID, "null pointer");
     DBuf t1, t2;
-    e.upload(t1, T1, e.n1());
-    e.upload(t2, T2, e.n2());
+    e.upload_t(t1, T1, e.n1());
+    e.upload_t(t2, T2, e.n2());
     ccsd_energy(e, t1.get(), t2.get(), out[0], out[1], out[2]);
   });
 }
@@ -271,7 +302,7 @@ int sisi_ccsd_solve(sisi_engine *h, const sisi_ccsd_opts *opts, const double *T1
     Engine &e = h->e;
     if (o.max_iterations == 0) {  // energy of the given amplitudes (cxx:116-119)
       double E, d, x;
-      ccsd_energy(e, e.T1.get(), e.T2.get(), E, d, x);
+      ccsd_energy(e, e.T1.get(), e.T2.get(), E, d, x, &e.last_energy_imag);
       e.last_energy = E;
       e.last_direct = d;
       e.last_exchange = x;
@@ -285,8 +316,8 @@ int sisi_ccsd_get_amplitudes(sisi_engine *h, double *T1, double *T2) {
     ENGINE_REQUIRED(h);
     Engine &e = h->e;
     SISI_REQUIRE(e.run_active, SISI_ERR_STATE, "no CCSD run");
-    if (T1) e.download(T1, e.T1, e.n1());
-    if (T2) e.download(T2, e.T2, e.n2());
+    if (T1) e.download_t(T1, e.T1, e.n1());
+    if (T2) e.download_t(T2, e.T2, e.n2());
   });
 }
 
@@ -296,9 +327,10 @@ int sisi_ccsd_pair_energies(sisi_engine *h, double *pair) {
     Engine &e = h->e;
     SISI_REQUIRE(pair, SISI_ERR_INVALID, "null pointer");
     SISI_REQUIRE(e.run_active, SISI_ERR_STATE, "no CCSD run");
-    DBuf p(&e.pool, size_t(e.No) * e.No);
+    const size_t n = size_t(e.No) * e.No * (e.cplx ? 2 : 1);
+    DBuf p(&e.pool, n);
     ccsd_pair_energies(e, e.T1.get(), e.T2.get(), p.get());
-    e.download(pair, p, size_t(e.No) * e.No);
+    e.download_t(pair, p, n);
   });
 }
 
@@ -319,6 +351,7 @@ static int triples_common(sisi_engine *h, const double *T1, const double *T2, in
     ENGINE_REQUIRED(h);
     Engine &e = h->e;
     SISI_REQUIRE(energy, SISI_ERR_INVALID, "null pointer");
+    SISI_REQUIRE(!e.cplx, SISI_ERR_INVALID, "(T) is not built for a complex vertex");
     SISI_REQUIRE(precision == SISI_FP64 || precision == SISI_FP32 || precision == SISI_BF16X3,
                  SISI_ERR_INVALID, "unknown precision");
     SISI_REQUIRE((T1 == nullptr) == (T2 == nullptr), SISI_ERR_INVALID,

@@ -28,6 +28,7 @@ void set_precision(Engine &e, int precision) {
 }
 
 void integrals_from_vertex(Engine &e) {
+  if (e.cplx) return zintegrals_from_vertex(e);
   e.require_inputs();
   const int64_t o = e.No, v = e.Nv, G = e.NG, P = e.Np;
   // slices (CoulombIntegralsFromVertex.cxx:121-136; holes are p < No)
@@ -64,6 +65,7 @@ void integrals_from_vertex(Engine &e) {
 
 void ccsd_residual(Engine &e, int it, bool initial_doubles, int slice_size, const double *T1c,
                    const double *T2c, double *R1p, double *R2p) {
+  if (e.cplx) return zccsd_residual(e, it, initial_doubles, slice_size, T1c, T2c, R1p, R2p);
   SISI_REQUIRE(e.have_integrals, SISI_ERR_STATE, "residual: integrals not built");
   const int64_t o = e.No, v = e.Nv, G = e.NG;
   const int64_t n2 = v * v * o * o;
@@ -293,6 +295,7 @@ void ccsd_residual(Engine &e, int it, bool initial_doubles, int slice_size, cons
 // FP64 residual at T_A with the same slice size.
 void ccsd_residual_delta(Engine &e, int slice_size, const double *T1c, const double *T2c,
                          double *R1p, double *R2p) {
+  SISI_REQUIRE(!e.cplx, SISI_ERR_INVALID, "refinement residuals are not built for complex amplitudes");
   Anchor &an = e.anchor;
   SISI_REQUIRE(an.valid, SISI_ERR_STATE, "refinement residual without an anchor");
   SISI_REQUIRE(an.ppl == e.ppl && an.distinguishable == e.distinguishable, SISI_ERR_STATE,
@@ -560,6 +563,7 @@ void ccsd_residual_delta(Engine &e, int slice_size, const double *T1c, const dou
 // FP64 residual at T that also becomes the new refinement anchor.
 void ccsd_residual_anchor(Engine &e, int slice_size, const double *T1, const double *T2,
                           double *R1, double *R2) {
+  SISI_REQUIRE(!e.cplx, SISI_ERR_INVALID, "refinement residuals are not built for complex amplitudes");
   const int64_t n1 = int64_t(e.n1()), n2 = int64_t(e.n2());
   const int p = e.precision, pl = e.prec_ladder;
   e.precision = SISI_FP64;
@@ -593,14 +597,25 @@ void ccsd_residual_anchor(Engine &e, int slice_size, const double *T1, const dou
 }
 
 void ccsd_energy(Engine &e, const double *T1, const double *T2, double &E, double &direct,
-                 double &exchange) {
+                 double &exchange, double *E_imag) {
   SISI_REQUIRE(e.have_integrals, SISI_ERR_STATE, "energy: integrals not built");
+  if (e.cplx) {  // F = complex: the real parts are reported (cxx:30-33, 185-189)
+    double Ez[2], dz[2], xz[2];
+    zccsd_energy(e, T1, T2, Ez, dz, xz);
+    E = Ez[0];
+    direct = dz[0];
+    exchange = xz[0];
+    if (E_imag) *E_imag = Ez[1];
+    return;
+  }
   energy_terms(e, T1, T2, direct, exchange);
   E = direct + exchange;
+  if (E_imag) *E_imag = 0.0;
 }
 
 void ccsd_pair_energies(Engine &e, const double *T1, const double *T2, double *pair) {
   SISI_REQUIRE(e.have_integrals, SISI_ERR_STATE, "pair energies: integrals not built");
+  if (e.cplx) return zccsd_pair_energies(e, T1, T2, pair);
   const int64_t o = e.No, v = e.Nv;
   DBuf Xb(&e.pool, v * v * o * o);
   add_t1t1(e, Xb.get(), T2, T1, 1.0, int(v), int(o));  // tau = T2 + T1 T1
@@ -656,20 +671,26 @@ void ccsd_begin(Engine &e, const sisi_ccsd_opts &opts, const double *T1h, const 
                SISI_ERR_INVALID, "Mixer not implemented");
   SISI_REQUIRE(opts.mixer != SISI_MIXER_DIIS || opts.max_residua >= 1, SISI_ERR_INVALID,
                "maxResidua must be >= 1");
+  SISI_REQUIRE(opts.refine_interval >= 0, SISI_ERR_INVALID, "refine_interval must be >= 0");
+  SISI_REQUIRE(!e.cplx || opts.precision == SISI_FP64 || opts.refine_interval == 0,
+               SISI_ERR_INVALID, "refinement residuals are not built for complex amplitudes");
+  SISI_REQUIRE(opts.refine_precision == SISI_FP32 || opts.refine_precision == SISI_BF16X3 ||
+                   opts.refine_precision == SISI_BF16,
+               SISI_ERR_INVALID, "refine_precision must be SISI_FP32, SISI_BF16X3 or SISI_BF16");
   e.opts = opts;
   e.ppl = opts.ppl != 0;
   e.distinguishable = opts.distinguishable != 0;
   e.T1 = DBuf(&e.pool, e.n1());
   e.T2 = DBuf(&e.pool, e.n2());
   e.run_initial = (T2h != nullptr);
-  if (T1h) e.upload(e.T1, T1h, e.n1());
+  if (T1h) e.upload_t(e.T1, T1h, e.n1());
   else zero(e, e.T1.get(), int64_t(e.n1()));
-  if (T2h) e.upload(e.T2, T2h, e.n2());
+  if (T2h) e.upload_t(e.T2, T2h, e.n2());
   else zero(e, e.T2.get(), int64_t(e.n2()));
   e.iteration = 0;
   e.fp64_iterations = 0;
-  e.prev_energy = 0.0;
-  e.last_energy = e.last_direct = e.last_exchange = 0.0;
+  e.prev_energy = e.prev_energy_imag = 0.0;
+  e.last_energy = e.last_direct = e.last_exchange = e.last_energy_imag = 0.0;
   e.last_delta = 1.0;
   e.converged = false;
   e.energies.clear();
@@ -680,10 +701,6 @@ void ccsd_begin(Engine &e, const sisi_ccsd_opts &opts, const double *T1h, const 
   e.lin1.reset();
   e.lin2.reset();
   e.anchor = Anchor();
-  SISI_REQUIRE(opts.refine_interval >= 0, SISI_ERR_INVALID, "refine_interval must be >= 0");
-  SISI_REQUIRE(opts.refine_precision == SISI_FP32 || opts.refine_precision == SISI_BF16X3 ||
-                   opts.refine_precision == SISI_BF16,
-               SISI_ERR_INVALID, "refine_precision must be SISI_FP32, SISI_BF16X3 or SISI_BF16");
   if (opts.mixer == SISI_MIXER_DIIS) {
     const int N = opts.max_residua;
     e.diis.resize(N);
@@ -782,12 +799,13 @@ void ccsd_iterate(Engine &e, int n) {
       const double tt = fock_dot(e, e.T1.get(), e.T2.get(), e.T1.get(), e.T2.get());
       e.last_delta = std::sqrt(std::fabs(dd / tt));
     }
-    double E, dir, exc;
-    ccsd_energy(e, e.T1.get(), e.T2.get(), E, dir, exc);
+    double E, dir, exc, Ei;
+    ccsd_energy(e, e.T1.get(), e.T2.get(), E, dir, exc, &Ei);
     SISI_REQUIRE(std::isfinite(E) && std::isfinite(e.last_delta), SISI_ERR_NUMERIC,
                  "CCSD iteration " + std::to_string(e.iteration + 1) +
                      " diverged (non-finite energy or amplitudes)");
     e.last_energy = E;
+    e.last_energy_imag = Ei;
     e.last_direct = dir;
     e.last_exchange = exc;
     e.energies.push_back(E);
@@ -799,13 +817,15 @@ void ccsd_iterate(Engine &e, int n) {
     // convergence test (ClusterSinglesDoublesAlgorithm.cxx:103-106), declared on
     // FP64 residuals only; a refinement iteration that meets it asks for an
     // FP64 anchor next, which then confirms it
-    const double de = std::fabs((E - e.prev_energy) / E);
+    // |(e - previousE) / e| of the complex energies in complex mode
+    const double de = std::hypot(E - e.prev_energy, Ei - e.prev_energy_imag) / std::hypot(E, Ei);
     const bool met = de < std::fabs(o.energy_convergence) &&
                      e.last_delta * e.last_delta <
                          std::fabs(o.amplitudes_convergence * o.amplitudes_convergence);
     if (met && prec == SISI_FP64) e.converged = true;
     else if (met && refine) e.anchor.age = o.refine_interval;
     e.prev_energy = E;
+    e.prev_energy_imag = Ei;
   }
 }
 
@@ -819,6 +839,7 @@ void fill_report(const Engine &e, sisi_ccsd_report *r) {
   r->fp64_iterations = e.fp64_iterations;
   r->precision = e.cur_precision;
   r->status = SISI_OK;
+  r->energy_imag = e.last_energy_imag;
 }
 
 }  // namespace sisi

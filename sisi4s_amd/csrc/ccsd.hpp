@@ -23,12 +23,25 @@ void ccsd_residual_delta(Engine &e, int slice_size, const double *T1, const doub
 
 // getEnergy (ClusterSinglesDoublesAlgorithm.cxx:131-213).
 void ccsd_energy(Engine &e, const double *T1, const double *T2, double &E, double &direct,
-                 double &exchange);
+                 double &exchange, double *E_imag = nullptr);
 
 // PairEnergy of getEnergy (ClusterSinglesDoublesAlgorithm.cxx:218-235,
 // non-antisymmetrized): e_ij = sum_ab (T2 + T1 T1)_abij (2 V_abij - V_baij);
-// pair is a device [No, No] buffer (i fastest).
+// pair is a device [No, No] buffer (i fastest; planar complex in complex mode).
 void ccsd_pair_energies(Engine &e, const double *T1, const double *T2, double *pair);
+
+// Complex path (complex mode, planar tensors):
+// CoulombIntegralsFromVertex::calculateComplexIntegrals (cxx:613-810) and
+// CcsdEnergyFromCoulombIntegrals::getResiduum for complex amplitudes
+// (cxx:496-891); getEnergy and PairEnergy with F = complex
+// (ClusterSinglesDoublesAlgorithm.cxx:131-235), E/direct/exchange as
+// (Re, Im) and pair as a planar [No, No] buffer.
+void zintegrals_from_vertex(Engine &e);
+void zccsd_residual(Engine &e, int iteration, bool initial_doubles, int slice_size,
+                    const double *T1, const double *T2, double *R1, double *R2);
+void zccsd_energy(Engine &e, const double *T1, const double *T2, double E[2], double direct[2],
+                  double exchange[2]);
+void zccsd_pair_energies(Engine &e, const double *T1, const double *T2, double *pair);
 
 // Driver (ClusterSinglesDoublesAlgorithm.cxx:38-128).
 void ccsd_begin(Engine &e, const sisi_ccsd_opts &opts, const double *T1h, const double *T2h);

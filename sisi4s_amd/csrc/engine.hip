@@ -142,6 +142,30 @@ void Engine::download(double *dst, const DBuf &src, size_t n) {
   SISI_HIP(hipStreamSynchronize(stream));
 }
 
+void Engine::upload_t(DBuf &dst, const double *src, size_t n) {
+  if (!cplx) return upload(dst, src, n);
+  SISI_REQUIRE(src != nullptr, SISI_ERR_INVALID, "null host pointer");
+  const size_t h = n / 2;
+  std::vector<double> planar(n);
+  for (size_t k = 0; k < h; ++k) {
+    planar[k] = src[2 * k];
+    planar[h + k] = src[2 * k + 1];
+  }
+  upload(dst, planar.data(), n);
+}
+
+void Engine::download_t(double *dst, const DBuf &src, size_t n) {
+  if (!cplx) return download(dst, src, n);
+  SISI_REQUIRE(dst != nullptr, SISI_ERR_INVALID, "null host pointer");
+  const size_t h = n / 2;
+  std::vector<double> planar(n);
+  download(planar.data(), src, n);
+  for (size_t k = 0; k < h; ++k) {
+    dst[2 * k] = planar[k];
+    dst[2 * k + 1] = planar[h + k];
+  }
+}
+
 void Engine::sync() { SISI_HIP(hipStreamSynchronize(stream)); }
 
 void Engine::record(const std::string &name, const double *p, size_t n) {

@@ -121,6 +121,13 @@ class Engine {
       }
   }
 
+  // Complex mode (vertex given by sisi_engine_set_vertex_complex): the
+  // vertex, its slices, the integrals, amplitudes and residuals are stored
+  // planar, every real part followed by the imaginary part of the same
+  // tensor, so sizes double and host arrays are converted from/to
+  // interleaved complex128 at the boundary.
+  bool cplx = false;
+
   // inputs
   DBuf gamma, eps_i, eps_a;
   DBuf Gij, Gai, Gab;  // dense slices [NG,No,No], [NG,Nv,No], [NG,Nv,Nv]
@@ -128,6 +135,11 @@ class Engine {
 
   // integrals
   DBuf Vabij, Vaibj, Vijkl, Vijka;
+  // complex mode only: Gamma_ia slice and the classes the complex residual
+  // reads besides the ones above (CoulombIntegralsFromVertex.cxx:762-810);
+  // U_klcd = 2 V_klcd - V_kldc of HHPP; HHPP as [a,b,i,j] and with a,b
+  // exchanged (energy terms)
+  DBuf Gia, Vaijb, Vijab, Vaijk, Uklcd, VijabT, VjiabT;
   DBuf Vbaij;  // V_abij with a,b exchanged (energy exchange term)
   DBuf Uabij;  // 2 V_abij - V_baij
   // (T) integrals, built on first use: V_bdck as [d,b,c,k] and as [d,c,b,k],
@@ -159,6 +171,7 @@ class Engine {
   int cur_precision = SISI_FP64;
   double prev_energy = 0.0, last_energy = 0.0, last_direct = 0.0,
          last_exchange = 0.0, last_delta = 0.0;
+  double prev_energy_imag = 0.0, last_energy_imag = 0.0;  // complex mode
   bool converged = false;
   DBuf T1, T2;
   std::vector<double> energies;
@@ -174,9 +187,17 @@ class Engine {
   void require_inputs() const;
   void upload(DBuf &dst, const double *src, size_t n);
   void download(double *dst, const DBuf &src, size_t n);
+  // n doubles of a tensor as the caller holds it: in complex mode the host
+  // array is interleaved complex128 and the device copy planar
+  void upload_t(DBuf &dst, const double *src, size_t n);
+  void download_t(double *dst, const DBuf &src, size_t n);
   void sync();
-  size_t n1() const { return size_t(Nv) * No; }
-  size_t n2() const { return size_t(Nv) * Nv * No * No; }
+  // doubles per singles / doubles amplitude tensor (both planes in complex
+  // mode) and elements per plane
+  size_t n1() const { return n1e() * (cplx ? 2 : 1); }
+  size_t n2() const { return n2e() * (cplx ? 2 : 1); }
+  size_t n1e() const { return size_t(Nv) * No; }
+  size_t n2e() const { return size_t(Nv) * Nv * No * No; }
 };
 
 }  // namespace sisi

@@ -175,8 +175,8 @@ void axpby(Engine &e, int64_t n, double a, const double *x, double b, double *y)
   SISI_HIP(hipGetLastError());
 }
 
-__global__ void add_t1t1_kernel(double *X, const double *T2, const double *T1, double f,
-                                int Nv, int No) {
+__global__ void add_t1t1_kernel(double *X, const double *T2, const double *T1,
+                                const double *T1b, double f, int Nv, int No) {
   const int64_t n = int64_t(Nv) * Nv * No * No;
   for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < n;
        e += int64_t(gridDim.x) * blockDim.x) {
@@ -187,17 +187,17 @@ __global__ void add_t1t1_kernel(double *X, const double *T2, const double *T1, d
     r /= Nv;
     const int i = int(r % No);
     const int j = int(r / No);
-    X[e] = T2[e] + f * T1[a + int64_t(Nv) * i] * T1[b + int64_t(Nv) * j];
+    X[e] = T2[e] + f * T1[a + int64_t(Nv) * i] * T1b[b + int64_t(Nv) * j];
   }
 }
 
 void add_t1t1(Engine &e, double *X, const double *T2, const double *T1, double f, int Nv,
-              int No) {
+              int No, const double *T1b) {
   const int64_t n = int64_t(Nv) * Nv * No * No;
   Timed t(*e.timers, Kind::Bandwidth, 16.0 * double(n));
   const int blocks = int(std::min<int64_t>((n + 255) / 256, 1 << 16));
-  hipLaunchKernelGGL(add_t1t1_kernel, dim3(blocks), dim3(256), 0, e.stream, X, T2, T1, f, Nv,
-                     No);
+  hipLaunchKernelGGL(add_t1t1_kernel, dim3(blocks), dim3(256), 0, e.stream, X, T2, T1,
+                     T1b ? T1b : T1, f, Nv, No);
   SISI_HIP(hipGetLastError());
 }
 
@@ -255,16 +255,17 @@ double dot(Engine &e, int64_t n, const double *x, const double *y) {
 
 __global__ void estimate_kernel(int order, int Nv, int No, double *R, const double *T,
                                 double *dT, const double *eps_i, const double *eps_a,
-                                double shift, int64_t n) {
+                                double shift, int64_t n, int64_t nelem) {
   for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < n;
        e += int64_t(gridDim.x) * blockDim.x) {
     double D;
+    const int64_t x = e % nelem;  // element within its (real or imaginary) plane
     if (order == 2) {
-      const int a = int(e % Nv);
-      const int i = int(e / Nv);
+      const int a = int(x % Nv);
+      const int i = int(x / Nv);
       D = eps_a[a] - eps_i[i];
     } else {
-      int64_t r = e;
+      int64_t r = x;
       const int a = int(r % Nv);
       r /= Nv;
       const int b = int(r % Nv);
@@ -283,10 +284,11 @@ __global__ void estimate_kernel(int order, int Nv, int No, double *R, const doub
 void estimate_amplitudes(Engine &e, int order, double *R, const double *T, double *dT,
                          double shift) {
   const int64_t n = order == 2 ? int64_t(e.n1()) : int64_t(e.n2());
+  const int64_t nelem = order == 2 ? int64_t(e.n1e()) : int64_t(e.n2e());
   Timed t(*e.timers, Kind::Bandwidth, 32.0 * double(n));
   const int blocks = int(std::min<int64_t>((n + 255) / 256, 1 << 16));
   hipLaunchKernelGGL(estimate_kernel, dim3(blocks), dim3(256), 0, e.stream, order, e.Nv, e.No,
-                     R, T, dT, e.eps_i.get(), e.eps_a.get(), shift, n);
+                     R, T, dT, e.eps_i.get(), e.eps_a.get(), shift, n, nelem);
   SISI_HIP(hipGetLastError());
 }
 

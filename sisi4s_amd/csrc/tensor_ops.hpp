@@ -18,15 +18,17 @@ void zero(Engine &e, double *p, int64_t n);
 // y = a*x + b*y on contiguous arrays.
 void axpby(Engine &e, int64_t n, double a, const double *x, double b, double *y);
 
-// X[a,b,i,j] = T2[a,b,i,j] + f * T1[a,i] * T1[b,j]  (X may alias T2).
+// X[a,b,i,j] = T2[a,b,i,j] + f * T1[a,i] * T1b[b,j]  (X may alias T2;
+// T1b == nullptr means T1b = T1).
 void add_t1t1(Engine &e, double *X, const double *T2, const double *T1,
-              double f, int Nv, int No);
+              double f, int Nv, int No, const double *T1b = nullptr);
 
 // Deterministic FP64 sum_k x[k]*y[k] (returned to the host).
 double dot(Engine &e, int64_t n, const double *x, const double *y);
 
 // Amplitude estimate of ClusterSinglesDoublesAlgorithm.cxx:302-331 on one
-// component (order 2 for T1, 4 for T2), fused with the amplitude change:
+// component (order 2 for T1, 4 for T2; both planes in complex mode, the
+// denominators being real), fused with the amplitude change:
 //   Test = -(R - shift*T) / (D + shift),  dT = Test - T,
 // D from the eigenenergies.  R is overwritten by Test.
 void estimate_amplitudes(Engine &e, int order, double *R, const double *T,

@@ -26,7 +26,7 @@ SISI_ERR_ALLOC = 5
 FP64, FP32, BF16X3, BF16 = 0, 1, 2, 3
 PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16x3": BF16X3, "bf16": BF16}
 MIXER_LINEAR, MIXER_DIIS = 0, 1
-PPHH, PHPH, HHHH, HHHP, HHPP = 0, 1, 2, 3, 4
+PPHH, PHPH, HHHH, HHHP, HHPP, PHHP, PHHH = 0, 1, 2, 3, 4, 5, 6
 
 _c_double_p = ctypes.POINTER(ctypes.c_double)
 _c_int_p = ctypes.POINTER(ctypes.c_int)
@@ -52,7 +52,7 @@ class CcsdReport(ctypes.Structure):
                 ("exchange", ctypes.c_double), ("delta", ctypes.c_double),
                 ("iterations", ctypes.c_int), ("converged", ctypes.c_int),
                 ("fp64_iterations", ctypes.c_int), ("precision", ctypes.c_int),
-                ("status", ctypes.c_int)]
+                ("status", ctypes.c_int), ("energy_imag", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -83,6 +83,7 @@ SIGNATURES = [
     ("sisi_engine_destroy", ctypes.c_int, [ctypes.c_void_p]),
     ("sisi_engine_set_vertex", ctypes.c_int, [ctypes.c_void_p, _c_double_p]),
     ("sisi_engine_set_eigenenergies", ctypes.c_int, [ctypes.c_void_p, _c_double_p, _c_double_p]),
+    ("sisi_engine_set_vertex_complex", ctypes.c_int, [ctypes.c_void_p, _c_double_p]),
     ("sisi_integrals_from_vertex", ctypes.c_int, [ctypes.c_void_p]),
     ("sisi_get_integrals", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _c_double_p]),
     ("sisi_engine_set_precision", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
@@ -279,11 +280,38 @@ class Engine:
     def __exit__(self, *exc):
         self.close()
 
+    # --- tensors at the boundary (complex128 interleaved in complex mode) --
+    complex = False
+
+    def _in(self, x):
+        if not self.complex:
+            return _buf(x)
+        b = np.ascontiguousarray(np.asarray(x, dtype=np.complex128).ravel(order="F"))
+        return b.view(np.float64)
+
+    def _out(self, shape):
+        n = int(np.prod(shape))
+        return np.empty(2 * n if self.complex else n)
+
+    def _shape(self, b, shape):
+        if self.complex:
+            return b.view(np.complex128).reshape(shape, order="F")
+        return _unbuf(b, shape)
+
     # --- inputs -----------------------------------------------------------
     def set_vertex(self, gamma):
         g = _buf(gamma)
         assert g.size == self.NG * self.Np * self.Np
         _check(self._lib.sisi_engine_set_vertex(self._h, _ptr(g)))
+        self.complex = False
+
+    def set_vertex_complex(self, gamma):
+        """Complex vertex: the engine switches to the reference's complex
+        path (CcsdEnergyFromCoulombIntegrals.cxx:496-891)."""
+        g = np.ascontiguousarray(np.asarray(gamma, dtype=np.complex128).ravel(order="F"))
+        assert g.size == self.NG * self.Np * self.Np
+        _check(self._lib.sisi_engine_set_vertex_complex(self._h, _ptr(g.view(np.float64))))
+        self.complex = True
 
     def set_eigenenergies(self, eps_i, eps_a):
         ei = _buf(eps_i)
@@ -291,7 +319,10 @@ class Engine:
         _check(self._lib.sisi_engine_set_eigenenergies(self._h, _ptr(ei), _ptr(ea)))
 
     def set_system(self, gamma, eps_i, eps_a, integrals=True):
-        self.set_vertex(gamma)
+        if np.iscomplexobj(gamma):
+            self.set_vertex_complex(gamma)
+        else:
+            self.set_vertex(gamma)
         self.set_eigenenergies(eps_i, eps_a)
         if integrals:
             self.integrals_from_vertex()
@@ -302,11 +333,12 @@ class Engine:
     def get_integrals(self, which):
         o, v = self.No, self.Nv
         shapes = {PPHH: (v, v, o, o), PHPH: (v, o, v, o), HHHH: (o, o, o, o),
-                  HHHP: (o, o, o, v), HHPP: (o, o, v, v)}
+                  HHHP: (o, o, o, v), HHPP: (o, o, v, v), PHHP: (v, o, o, v),
+                  PHHH: (v, o, o, o)}
         shape = shapes[which]
-        b = np.empty(int(np.prod(shape)))
+        b = self._out(shape)
         _check(self._lib.sisi_get_integrals(self._h, which, _ptr(b)))
-        return _unbuf(b, shape)
+        return self._shape(b, shape)
 
     def set_precision(self, precision):
         if isinstance(precision, str):
@@ -322,13 +354,12 @@ class Engine:
     # --- residual / energies ---------------------------------------------
     def residual(self, iteration, T1, T2, slice_size=-1, initial_doubles=False):
         v, o = self.Nv, self.No
-        t1, t2 = _buf(T1), _buf(T2)
-        r1 = np.empty(v * o)
-        r2 = np.empty(v * v * o * o)
+        t1, t2 = self._in(T1), self._in(T2)
+        r1, r2 = self._out((v, o)), self._out((v, v, o, o))
         _check(self._lib.sisi_ccsd_residual(self._h, iteration, int(initial_doubles),
                                             int(slice_size), _ptr(t1), _ptr(t2), _ptr(r1),
                                             _ptr(r2)))
-        return _unbuf(r1, (v, o)), _unbuf(r2, (v, v, o, o))
+        return self._shape(r1, (v, o)), self._shape(r2, (v, v, o, o))
 
     def residual_anchor(self, T1, T2, slice_size=-1):
         return self._residual2(self._lib.sisi_ccsd_residual_anchor, T1, T2, slice_size)
@@ -346,7 +377,8 @@ class Engine:
 
     def energy(self, T1, T2):
         out = np.empty(3)
-        _check(self._lib.sisi_ccsd_energy(self._h, _ptr(_buf(T1)), _ptr(_buf(T2)), _ptr(out)))
+        _check(self._lib.sisi_ccsd_energy(self._h, _ptr(self._in(T1)), _ptr(self._in(T2)),
+                                          _ptr(out)))
         return tuple(out)
 
     def mp2_energy(self):
@@ -357,8 +389,8 @@ class Engine:
     # --- driver ------------------------------------------------------------
     def ccsd_begin(self, opts=None, T1=None, T2=None):
         o = opts if opts is not None else default_opts()
-        t1 = _buf(T1) if T1 is not None else None
-        t2 = _buf(T2) if T2 is not None else None
+        t1 = self._in(T1) if T1 is not None else None
+        t2 = self._in(T2) if T2 is not None else None
         _check(self._lib.sisi_ccsd_begin(self._h, ctypes.byref(o), _ptr(t1), _ptr(t2)))
 
     def ccsd_iterate(self, n):
@@ -368,8 +400,8 @@ class Engine:
 
     def ccsd_solve(self, opts=None, T1=None, T2=None):
         o = opts if opts is not None else default_opts()
-        t1 = _buf(T1) if T1 is not None else None
-        t2 = _buf(T2) if T2 is not None else None
+        t1 = self._in(T1) if T1 is not None else None
+        t2 = self._in(T2) if T2 is not None else None
         r = CcsdReport()
         _check(self._lib.sisi_ccsd_solve(self._h, ctypes.byref(o), _ptr(t1), _ptr(t2),
                                          ctypes.byref(r)))
@@ -377,16 +409,15 @@ class Engine:
 
     def amplitudes(self):
         v, o = self.Nv, self.No
-        t1 = np.empty(v * o)
-        t2 = np.empty(v * v * o * o)
+        t1, t2 = self._out((v, o)), self._out((v, v, o, o))
         _check(self._lib.sisi_ccsd_get_amplitudes(self._h, _ptr(t1), _ptr(t2)))
-        return _unbuf(t1, (v, o)), _unbuf(t2, (v, v, o, o))
+        return self._shape(t1, (v, o)), self._shape(t2, (v, v, o, o))
 
     def pair_energies(self):
         """PairEnergy [No, No] of the current amplitudes."""
-        p = np.empty(self.No * self.No)
+        p = self._out((self.No, self.No))
         _check(self._lib.sisi_ccsd_pair_energies(self._h, _ptr(p)))
-        return _unbuf(p, (self.No, self.No))
+        return self._shape(p, (self.No, self.No))
 
     def energies(self):
         n = ctypes.c_int()
