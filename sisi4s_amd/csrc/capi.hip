@@ -351,7 +351,6 @@ static int triples_common(sisi_engine *h, const double *T1, const double *T2, in
     ENGINE_REQUIRED(h);
     Engine &e = h->e;
     SISI_REQUIRE(energy, SISI_ERR_INVALID, "null pointer");
-    SISI_REQUIRE(!e.cplx, SISI_ERR_INVALID, "(T) is not built for a complex vertex");
     SISI_REQUIRE(precision == SISI_FP64 || precision == SISI_FP32 || precision == SISI_BF16X3,
                  SISI_ERR_INVALID, "unknown precision");
     SISI_REQUIRE((T1 == nullptr) == (T2 == nullptr), SISI_ERR_INVALID,
@@ -359,8 +358,8 @@ static int triples_common(sisi_engine *h, const double *T1, const double *T2, in
     DBuf t1, t2;
     const double *d1, *d2;
     if (T1) {
-      e.upload(t1, T1, e.n1());
-      e.upload(t2, T2, e.n2());
+      e.upload_t(t1, T1, e.n1());
+      e.upload_t(t2, T2, e.n2());
       d1 = t1.get();
       d2 = t2.get();
     } else {

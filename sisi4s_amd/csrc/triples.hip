@@ -29,6 +29,7 @@
 
 #include "contract.hpp"
 #include "tensor_ops.hpp"
+#include "ztensor.hpp"
 
 namespace sisi {
 
@@ -57,6 +58,11 @@ struct EnergyArgs {
   const double *W;
   const double *T1;  // [Nv, No]
   const double *V;   // Vabij [Nv,Nv,No,No]
+  // optional second singles product sb * T1b V_b (complex amplitudes: the
+  // Re/Im part of T1 V needs two products of planes); T1b == nullptr: none
+  const double *T1b;
+  const double *Vb;
+  double sb;
   const double *eps_a;
   const int *orbits;  // X0 | X1 << 10 | X2 << 20, X0 <= X1 <= X2
   double *part;
@@ -145,8 +151,10 @@ __global__ __launch_bounds__(256) void triples_energy_kernel(EnergyArgs g) {
         const PermTerm &p = g.terms[t];
         // S(i.pi)[u] = 0.5 T1[u0, i_pi0] V[u1, u2, i_pi1, i_pi2] read at u_k = x_pi(k)
         const int u0 = x[p.pi[0]], u1 = x[p.pi[1]], u2 = x[p.pi[2]];
-        z += 0.5 * g.T1[u0 + int64_t(Nv) * p.idx[0]] *
-             g.V[u1 + s1 * u2 + s2 * (p.idx[1] + int64_t(g.No) * p.idx[2])];
+        const int64_t t1 = u0 + int64_t(Nv) * p.idx[0];
+        const int64_t vv = u1 + s1 * u2 + s2 * (p.idx[1] + int64_t(g.No) * p.idx[2]);
+        z += 0.5 * g.T1[t1] * g.V[vv];
+        if (g.T1b) z += 0.5 * g.sb * g.T1b[t1] * g.Vb[vv];
       }
       acc += f * z;
     }
@@ -170,11 +178,16 @@ struct TriplesOps {
 };
 
 // W[x] = sum over the given permutations of W(i.pi)[x.pi], three launches.
-void build_w(Engine &e, int prec, const TriplesOps &op, const std::array<int, 3> &ib,
-             const std::array<std::array<int, 3>, 6> &perms, const bool *use, double *W) {
+// The first factor of every product (T2 of the particle terms, VP of the hole
+// terms) comes from opA, the second from opB, the products are scaled by
+// `sign`, and W is overwritten unless `accumulate` is set (complex amplitudes
+// combine planes this way; the real path uses opA = opB, +1, overwrite).
+void build_w(Engine &e, int prec, const TriplesOps &opA, const TriplesOps &opB,
+             const std::array<int, 3> &ib, const std::array<std::array<int, 3>, 6> &perms,
+             const bool *use, double *W, double sign = 1.0, bool accumulate = false) {
   const int64_t o = e.No, v = e.Nv;
   const int64_t st[3] = {1, v, v * v};
-  bool first = true;
+  bool first = !accumulate;
   for (int j = 0; j < 3; ++j) {
     const int p = j == 0 ? 1 : 0, q = j == 2 ? 1 : 2;  // the other two positions, p < q
     SegGemmArgs a;
@@ -185,26 +198,26 @@ void build_w(Engine &e, int prec, const TriplesOps &op, const std::array<int, 3>
       if (pi[0] == j) {
         // + sum_d T2[x_j, d, x, y] V[d, x_pi1, x_pi2, z]
         GemmSeg &sg = a.seg[a.nseg++];
-        sg.A = op.T2 + (x + o * y) * v * v;
+        sg.A = opA.T2 + (x + o * y) * v * v;
         sg.sam = 1;
         sg.sak = v;
-        sg.B = (pi[1] == p ? op.Vt : op.Vt2) + z * v * v * v;
+        sg.B = (pi[1] == p ? opB.Vt : opB.Vt2) + z * v * v * v;
         sg.sbk = 1;
         sg.sbn = v;
         sg.K = int(v);
-        sg.scale = 1.0;
+        sg.scale = sign;
       }
       if (pi[2] == j) {
         // - sum_l V_P[l, x_j, y, z] T2[x_pi0, x_pi1, x, l]
         GemmSeg &sg = a.seg[a.nseg++];
-        sg.A = op.VP + (y + o * z) * o * v;
+        sg.A = opA.VP + (y + o * z) * o * v;
         sg.sam = o;
         sg.sak = 1;
-        sg.B = (pi[0] == p ? op.T2 : op.T2t) + x * v * v;
+        sg.B = (pi[0] == p ? opB.T2 : opB.T2t) + x * v * v;
         sg.sbk = v * v * o;
         sg.sbn = 1;
         sg.K = int(o);
-        sg.scale = -1.0;
+        sg.scale = -sign;
       }
     }
     if (a.nseg == 0) continue;
@@ -317,6 +330,151 @@ void build_w_b3(Engine &e, const TriplesPlanes &pl, const std::array<int, 3> &ib
   }
 }
 
+// The permutations of a block: tup[p] = i.pi_p and whether pi_p gives a
+// tuple no earlier permutation gave (CcsdPerturbativeTriples.cxx:161-176).
+void block_perms(const std::array<int, 3> &ib, const std::array<std::array<int, 3>, 6> &perms,
+                 std::array<int, 3> tup[6], bool distinct[6]) {
+  for (int p = 0; p < 6; ++p) {
+    for (int k = 0; k < 3; ++k) tup[p][k] = ib[perms[p][k]];  // i * pi (Permutation.hpp:93-98)
+    int q = 0;
+    for (; q < p; ++q)
+      if (tup[q] == tup[p]) break;
+    distinct[p] = (q == p);
+  }
+}
+
+// Block-dependent fields of the energy kernel: eps_i+eps_j+eps_k, the
+// sign factors sf' and the singles terms of the distinct permutations.
+void block_energy_args(EnergyArgs &g, const std::array<int, 3> &ib,
+                       const std::array<std::array<int, 3>, 6> &perms,
+                       const std::array<int, 3> tup[6], const bool distinct[6],
+                       const std::vector<double> &eps_i) {
+  g.eps_ijk = eps_i[ib[0]] + eps_i[ib[1]] + eps_i[ib[2]];
+  // tau: the swap of the equal positions (sig index 1 swaps 0,1; 3 swaps 1,2)
+  const int tau = ib[0] == ib[1] ? 1 : ib[1] == ib[2] ? 3 : -1;
+  for (int s = 0; s < 6; ++s) g.sfp[s] = hSf[s] + (tau >= 0 ? hSf[compose_idx(s, tau)] : 0.0);
+  g.nterms = 0;
+  for (int p = 0; p < 6; ++p) {
+    if (!distinct[p]) continue;
+    PermTerm &t = g.terms[g.nterms++];
+    for (int k = 0; k < 3; ++k) {
+      t.pi[k] = perms[p][k];
+      t.idx[k] = tup[p][k];
+    }
+  }
+}
+
+std::vector<std::array<int, 3>> checked_blocks(const Engine &e,
+                                               const std::vector<std::array<int, 3>> &in) {
+  const int o = e.No;
+  std::vector<std::array<int, 3>> blocks = in;
+  if (blocks.empty())
+    for (int a = 0; a < o; ++a)
+      for (int b = a; b < o; ++b)
+        for (int c = b; c < o; ++c) blocks.push_back({a, b, c});
+  for (auto &b : blocks)
+    SISI_REQUIRE(b[0] >= 0 && b[0] <= b[1] && b[1] <= b[2] && b[2] < o, SISI_ERR_INVALID,
+                 "triples: blocks must be sorted i0 <= i1 <= i2 < No");
+  return blocks;
+}
+
+// Cube orbits X0 <= X1 <= X2 of the energy kernel, uploaded.
+DBuf orbit_codes(Engine &e, int &norbits) {
+  const int ncube = int((e.Nv + 7) / 8);
+  std::vector<int> codes;
+  for (int a = 0; a < ncube; ++a)
+    for (int b = a; b < ncube; ++b)
+      for (int c = b; c < ncube; ++c) codes.push_back(a | (b << 10) | (c << 20));
+  norbits = int(codes.size());
+  DBuf buf(&e.pool, (norbits + 1) / 2);
+  SISI_HIP(hipMemcpyAsync(buf.get(), codes.data(), sizeof(int) * norbits, hipMemcpyHostToDevice,
+                          e.stream));
+  return buf;
+}
+
+// CcsdPerturbativeTriplesComplex::Calculator<complex>::calculate
+// (CcsdPerturbativeTriplesComplex.cxx:172-279) on planar complex tensors.
+// W = sum_d T2 V_t - sum_l T2 V_P is complex (four plane passes of build_w);
+// the energy sum_x conj(DV/D)[x] Tabc[x] has real part E(Re) + E(Im), the
+// real-path energy functional evaluated on the Re planes of W and the
+// singles term plus the same on the Im planes, since every step between W
+// and the energy is linear with real coefficients.
+double ztriples_energy(Engine &e, const double *T1, const double *T2, int prec,
+                       const std::vector<std::array<int, 3>> &blocks_in) {
+  const int64_t o = e.No, v = e.Nv, G = e.NG;
+  const int64_t n2 = v * v * o * o, nvt = v * v * v * o, nvp = o * v * o * o;
+  std::vector<std::array<int, 3>> blocks = checked_blocks(e, blocks_in);
+  if (!e.Vt.get()) {
+    // V_t[d,b,c,k] = sum_F conj(Gamma_Fdb) Gamma_Fck (cxx:373-375)
+    e.Vt = DBuf(&e.pool, 2 * nvt);
+    zctr(e, 1.0, "Fdb,Fck->dbck", zdense(e.Gab.get(), {G, v, v}), true,
+         zdense(e.Gai.get(), {G, v, o}), false, 0.0, zdense(e.Vt.get(), {v, v, v, o}), SISI_FP64);
+    e.Vt2 = DBuf(&e.pool, 2 * nvt);
+    zsum(e, 1.0, zdense(e.Vt.get(), {v, v, v, o}), "dbck", false, 0.0,
+         zdense(e.Vt2.get(), {v, v, v, o}), "dcbk");
+    // hole term V_PHHH[c,l,k,j] (cxx:157-162) as VP[l,c,j,k]
+    e.VP = DBuf(&e.pool, 2 * nvp);
+    zsum(e, 1.0, zdense(e.Vaijk.get(), {v, o, o, o}), "clkj", false, 0.0,
+         zdense(e.VP.get(), {o, v, o, o}), "lcjk");
+  }
+  DBuf T2t(&e.pool, 2 * n2);
+  zsum(e, 1.0, zdense(const_cast<double *>(T2), {v, v, o, o}), "abij", false, 0.0,
+       zdense(T2t.get(), {v, v, o, o}), "baij");
+  std::vector<double> eps_i(o);
+  e.download(eps_i.data(), e.eps_i, size_t(o));
+  const TriplesOps re{T2, T2t.get(), e.Vt.get(), e.Vt2.get(), e.VP.get()};
+  const TriplesOps im{T2 + n2, T2t.get() + n2, e.Vt.get() + nvt, e.Vt2.get() + nvt,
+                      e.VP.get() + nvp};
+  int norbits = 0;
+  DBuf orbits = orbit_codes(e, norbits);
+  const int64_t v3 = v * v * v;
+  DBuf W(&e.pool, 2 * v3);
+  DBuf part(&e.pool, norbits), Eb(&e.pool, 2 * blocks.size());
+  zero(e, Eb.get(), int64_t(2 * blocks.size()));
+  std::array<std::array<int, 3>, 6> perms;
+  for (int p = 0; p < 6; ++p) perms[p] = permutation3(p);
+  const double *Vr = e.Vabij.get(), *Vi = e.Vabij.get() + n2;
+  const double *T1r = T1, *T1i = T1 + v * o;
+  for (size_t bi = 0; bi < blocks.size(); ++bi) {
+    const auto &ib = blocks[bi];
+    if (ib[0] == ib[2]) continue;  // i0 = i1 = i2: sf' = 0
+    std::array<int, 3> tup[6];
+    bool distinct[6];
+    block_perms(ib, perms, tup, distinct);
+    // Re W = Re T Re V - Im T Im V, Im W = Re T Im V + Im T Re V (both terms)
+    build_w(e, prec, re, re, ib, perms, distinct, W.get(), 1.0, false);
+    build_w(e, prec, im, im, ib, perms, distinct, W.get(), -1.0, true);
+    build_w(e, prec, re, im, ib, perms, distinct, W.get() + v3, 1.0, false);
+    build_w(e, prec, im, re, ib, perms, distinct, W.get() + v3, 1.0, true);
+    for (int plane = 0; plane < 2; ++plane) {
+      EnergyArgs g{};
+      g.W = W.get() + plane * v3;
+      // singles: Re(T1 V) = T1r Vr - T1i Vi, Im(T1 V) = T1r Vi + T1i Vr
+      g.T1 = T1r;
+      g.V = plane ? Vi : Vr;
+      g.T1b = T1i;
+      g.Vb = plane ? Vr : Vi;
+      g.sb = plane ? 1.0 : -1.0;
+      g.eps_a = e.eps_a.get();
+      g.orbits = reinterpret_cast<const int *>(orbits.get());
+      g.part = part.get();
+      g.Nv = int(v);
+      g.No = int(o);
+      block_energy_args(g, ib, perms, tup, distinct, eps_i);
+      Timed tm(*e.timers, Kind::Bandwidth, 8.0 * double(v3));
+      hipLaunchKernelGGL(triples_energy_kernel, dim3(norbits), dim3(256), 0, e.stream, g);
+      hipLaunchKernelGGL(sum_into_kernel, dim3(1), dim3(256), 0, e.stream, part.get(), norbits,
+                         Eb.get() + 2 * bi + plane);
+      SISI_HIP(hipGetLastError());
+    }
+  }
+  std::vector<double> eb(2 * blocks.size());
+  e.download(eb.data(), Eb, eb.size());
+  double E = 0.0;
+  for (double x : eb) E += x;
+  return E;
+}
+
 }  // namespace
 
 double triples_energy(Engine &e, const double *T1, const double *T2, int prec,
@@ -325,16 +483,10 @@ double triples_energy(Engine &e, const double *T1, const double *T2, int prec,
   SISI_REQUIRE(prec == SISI_FP64 || prec == SISI_FP32 || prec == SISI_BF16X3, SISI_ERR_INVALID,
                "triples: unknown precision");
   e.thaw();
+  SISI_REQUIRE(e.Nv <= 8 * 1023, SISI_ERR_INVALID, "triples: Nv above 8184");
+  if (e.cplx) return ztriples_energy(e, T1, T2, prec, blocks_in);
   const int64_t o = e.No, v = e.Nv, G = e.NG;
-  SISI_REQUIRE(v <= 8 * 1023, SISI_ERR_INVALID, "triples: Nv above 8184");
-  std::vector<std::array<int, 3>> blocks = blocks_in;
-  if (blocks.empty())
-    for (int a = 0; a < o; ++a)
-      for (int b = a; b < o; ++b)
-        for (int c = b; c < o; ++c) blocks.push_back({a, b, c});
-  for (auto &b : blocks)
-    SISI_REQUIRE(b[0] >= 0 && b[0] <= b[1] && b[1] <= b[2] && b[2] < o, SISI_ERR_INVALID,
-                 "triples: blocks must be sorted i0 <= i1 <= i2 < No");
+  std::vector<std::array<int, 3>> blocks = checked_blocks(e, blocks_in);
   if (!e.Vt.get()) {
     // V_bdck = sum_F Gamma_Fbd Gamma_Fck stored as Vt[d,b,c,k] (cxx:88-91), FP64
     e.Vt = DBuf(&e.pool, v * v * v * o);
@@ -374,16 +526,8 @@ double triples_energy(Engine &e, const double *T1, const double *T2, int prec,
                  pl.T2th);
   }
 
-  // cube orbits X0 <= X1 <= X2
-  const int ncube = int((v + 7) / 8);
-  std::vector<int> orbit_codes;
-  for (int a = 0; a < ncube; ++a)
-    for (int b = a; b < ncube; ++b)
-      for (int c = b; c < ncube; ++c) orbit_codes.push_back(a | (b << 10) | (c << 20));
-  const int norbits = int(orbit_codes.size());
-  DBuf orbits_buf(&e.pool, (norbits + 1) / 2);
-  SISI_HIP(hipMemcpyAsync(orbits_buf.get(), orbit_codes.data(), sizeof(int) * norbits,
-                          hipMemcpyHostToDevice, e.stream));
+  int norbits = 0;
+  DBuf orbits_buf = orbit_codes(e, norbits);
   DBuf W(&e.pool, v * v * v);
   DBuf part(&e.pool, norbits), Eb(&e.pool, blocks.size());
   zero(e, Eb.get(), blocks.size());
@@ -395,15 +539,9 @@ double triples_energy(Engine &e, const double *T1, const double *T2, int prec,
     if (ib[0] == ib[2]) continue;  // i0 = i1 = i2: sf' = 0
     bool distinct[6];
     std::array<int, 3> tup[6];
-    for (int p = 0; p < 6; ++p) {
-      for (int k = 0; k < 3; ++k) tup[p][k] = ib[perms[p][k]];  // i * pi (Permutation.hpp:93-98)
-      int q = 0;
-      for (; q < p; ++q)
-        if (tup[q] == tup[p]) break;
-      distinct[p] = (q == p);
-    }
+    block_perms(ib, perms, tup, distinct);
     if (prec == SISI_BF16X3) build_w_b3(e, pl, ib, perms, distinct, W.get());
-    else build_w(e, prec, op, ib, perms, distinct, W.get());
+    else build_w(e, prec, op, op, ib, perms, distinct, W.get());
 
     EnergyArgs g{};
     g.W = W.get();
@@ -412,22 +550,9 @@ double triples_energy(Engine &e, const double *T1, const double *T2, int prec,
     g.eps_a = e.eps_a.get();
     g.orbits = reinterpret_cast<const int *>(orbits_buf.get());
     g.part = part.get();
-    g.eps_ijk = eps_i[ib[0]] + eps_i[ib[1]] + eps_i[ib[2]];
     g.Nv = int(v);
     g.No = int(o);
-    // tau: the swap of the equal positions (sig index 1 swaps 0,1; 3 swaps 1,2)
-    const int tau = ib[0] == ib[1] ? 1 : ib[1] == ib[2] ? 3 : -1;
-    for (int s = 0; s < 6; ++s)
-      g.sfp[s] = hSf[s] + (tau >= 0 ? hSf[compose_idx(s, tau)] : 0.0);
-    g.nterms = 0;
-    for (int p = 0; p < 6; ++p) {
-      if (!distinct[p]) continue;
-      PermTerm &t = g.terms[g.nterms++];
-      for (int k = 0; k < 3; ++k) {
-        t.pi[k] = perms[p][k];
-        t.idx[k] = tup[p][k];
-      }
-    }
+    block_energy_args(g, ib, perms, tup, distinct, eps_i);
     {
       Timed tm(*e.timers, Kind::Bandwidth, 8.0 * double(v * v * v));
       hipLaunchKernelGGL(triples_energy_kernel, dim3(norbits), dim3(256), 0, e.stream, g);

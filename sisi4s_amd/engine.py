@@ -431,8 +431,8 @@ class Engine:
         if isinstance(precision, str):
             precision = PRECISIONS[precision]
         e = ctypes.c_double()
-        t1 = _buf(T1) if T1 is not None else None
-        t2 = _buf(T2) if T2 is not None else None
+        t1 = self._in(T1) if T1 is not None else None
+        t2 = self._in(T2) if T2 is not None else None
         if blocks is None:
             _check(self._lib.sisi_triples_energy(self._h, _ptr(t1), _ptr(t2), int(precision),
                                                  ctypes.byref(e)))

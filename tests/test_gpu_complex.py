@@ -118,7 +118,7 @@ def test_complex_ueg_plane_wave_known_answers(gauge):
 
 def test_complex_mixed_precision_and_refusals():
     """Bulk BF16x3 iterations then FP64 on the complex path reach the FP64
-    energy; refinement residuals and (T) refuse a complex vertex."""
+    energy; refinement residuals refuse a complex vertex."""
     from sisi4s_amd import engine
     gz, ei, ea = complex_system(7, 5, 20, 80)
     e = make_engine(gz, ei, ea)
@@ -131,5 +131,35 @@ def test_complex_mixed_precision_and_refusals():
     assert abs(rm["energy"] - r64["energy"]) <= 1e-6
     with pytest.raises(engine.EngineError):
         e.ccsd_solve(engine.default_opts(precision="bf16x3", refine_interval=8, **kw))
-    with pytest.raises(engine.EngineError):
-        e.triples()
+
+
+@pytest.mark.parametrize("No,Nv,NG", [(3, 8, 20), (5, 13, 40)])
+def test_complex_triples_matches_oracle(No, Nv, NG):
+    """CcsdPerturbativeTriplesComplex on the GPU against the oracle's literal
+    restatement, with complex amplitudes on a complex vertex (all blocks,
+    including those with two equal indices), and its reduced-precision
+    contractions."""
+    gz, ei, ea = complex_system(13, No, Nv, NG)
+    e = make_engine(gz, ei, ea)
+    T1, T2 = complex_amplitudes(Nv, No, seed=17, scale=0.05)
+    from oracle import triples_complex as tz
+    announce("complex (T) No=%d Nv=%d" % (No, Nv))
+    ref = tz.triples_energy(T1, T2, gz, ei, ea).real
+    got = e.triples(T1, T2)
+    assert abs(got - ref) <= 1e-11 * abs(ref), (got, ref)
+    assert abs(e.triples(T1, T2, precision="fp32") - ref) <= 1e-5 * abs(ref)
+    assert abs(e.triples(T1, T2, precision="bf16x3") - ref) <= 1e-4 * abs(ref)
+
+
+def test_complex_ueg_plane_wave_triples_known_answer():
+    """(T) of the plane-wave UEG vertex on the engine's ten-iteration complex
+    amplitudes: the reference's value
+    (integration-tests/tests/cc4s/ueg/rs1.0-7occ-26virt/cc4s.correct.out.yaml:169)."""
+    from sisi4s_amd import engine
+    pw = ueg.plane_wave(7, 26, 1.0)
+    G, _ = cz.gauge_transform(pw["coeff"].astype(complex), 7, 4)
+    announce("complex ueg plane wave (T)")
+    e = make_engine(G, pw["eps"][:7], pw["eps"][7:])
+    e.ccsd_solve(engine.default_opts(mixer="DiisMixer", max_residua=4, max_iterations=10,
+                                     energy_convergence=1e-8, amplitudes_convergence=1e-8))
+    assert abs(e.triples() - -0.0063019625641725016) <= 1e-13

@@ -341,3 +341,37 @@ def test_complex_path_ueg_plane_wave_known_answers(gauge):
     assert abs(r["energies"][0] - -0.36143028565969504) <= 1e-13
     assert abs(r["energy"] - -0.39269658954585018) <= 1e-13
     assert abs(r["energy_complex"].imag) <= 1e-13
+
+
+def test_complex_triples_equals_real_for_real_vertex_and_is_gauge_invariant():
+    """CcsdPerturbativeTriplesComplex (CcsdPerturbativeTriplesComplex.cxx:
+    172-279): on a real symmetric vertex it is the real (T); on the
+    gauge-transformed vertex with that vertex's converged complex amplitudes
+    the energy is unchanged and real."""
+    from oracle import ccsd_complex as cz, triples_complex as tz
+    g, ei, ea = philox.system(5, 3, 8, 20)
+    kw = dict(mixer="DiisMixer", max_residua=6, max_iterations=40, slice_size=3,
+              amplitudes_convergence=1e-10, energy_convergence=1e-12)
+    a = ccsd.solve(g, ei, ea, **kw)
+    et = triples.triples_energy(a["T1"], a["T2"], g, ei, ea)
+    ez = tz.triples_energy(a["T1"].astype(complex), a["T2"].astype(complex), g.astype(complex),
+                           ei, ea)
+    assert abs(ez - et) <= 1e-14 * abs(et)
+    gz, _ = cz.gauge_transform(g.astype(complex), 3, 1)
+    c = cz.solve(gz, ei, ea, **kw)
+    eg = tz.triples_energy(c["T1"], c["T2"], gz, ei, ea)
+    assert abs(eg.real - et) <= 1e-12 * abs(et) and abs(eg.imag) <= 1e-13
+
+
+def test_complex_triples_ueg_plane_wave_known_answer():
+    """(T) of the plane-wave UEG vertex on the ten-iteration amplitudes: the
+    reference's value (integration-tests/tests/cc4s/ueg/rs1.0-7occ-26virt/
+    cc4s.correct.out.yaml:169)."""
+    from oracle import ccsd_complex as cz, triples_complex as tz
+    pw = ueg.plane_wave(7, 26, 1.0)
+    G = pw["coeff"].astype(complex)
+    ei, ea = pw["eps"][:7], pw["eps"][7:]
+    r = cz.solve(G, ei, ea, mixer="DiisMixer", max_residua=4, max_iterations=10,
+                 energy_convergence=1e-8, amplitudes_convergence=1e-8)
+    et = tz.triples_energy(r["T1"], r["T2"], G, ei, ea)
+    assert abs(et.real - -0.0063019625641725016) <= 1e-15
