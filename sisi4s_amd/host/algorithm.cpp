@@ -1,5 +1,8 @@
 #include "algorithm.hpp"
 
+#include <algorithm>
+#include <cmath>
+
 #include <cstdlib>
 
 namespace sisi_host {
@@ -113,6 +116,7 @@ void Algorithm::allocatedTensorArgument(const std::string &n, std::shared_ptr<Te
 namespace {
 struct CachedEngine {
   std::shared_ptr<Tensor> gamma, eps_i, eps_a;
+  bool complex_path = false;
   sisi_engine *e = nullptr;
 };
 std::vector<CachedEngine> &engines() {
@@ -122,9 +126,10 @@ std::vector<CachedEngine> &engines() {
 }  // namespace
 
 sisi_engine *engine_for(const std::shared_ptr<Tensor> &gamma, const std::shared_ptr<Tensor> &eps_i,
-                        const std::shared_ptr<Tensor> &eps_a) {
+                        const std::shared_ptr<Tensor> &eps_a, bool complex_path) {
   for (auto &c : engines())
-    if (c.gamma == gamma && c.eps_i == eps_i && c.eps_a == eps_a) return c.e;
+    if (c.gamma == gamma && c.eps_i == eps_i && c.eps_a == eps_a && c.complex_path == complex_path)
+      return c.e;
   if (gamma->lens.size() != 3 || gamma->lens[1] != gamma->lens[2])
     throw StepError("CoulombVertex must be a [NG, Np, Np] tensor");
   if (eps_i->lens.size() != 1 || eps_a->lens.size() != 1)
@@ -132,13 +137,32 @@ sisi_engine *engine_for(const std::shared_ptr<Tensor> &gamma, const std::shared_
   sisi_dims d{int(eps_i->lens[0]), int(eps_a->lens[0]), int(gamma->lens[0])};
   if (d.No + d.Nv != gamma->lens[1])
     throw StepError("CoulombVertex orbital length differs from No + Nv");
+  if (eps_i->complex || eps_a->complex) throw StepError("eigenenergies must be real");
+  const size_t n = size_t(gamma->size());
+  std::vector<double> host;  // the vertex in the form the chosen path takes
+  if (complex_path && !gamma->complex) {
+    host.assign(2 * n, 0.0);
+    for (size_t k = 0; k < n; ++k) host[2 * k] = gamma->values[k];
+  } else if (!complex_path && gamma->complex) {
+    host.resize(n);
+    double imag_max = 0.0;
+    for (size_t k = 0; k < n; ++k) {
+      host[k] = gamma->values[2 * k];
+      imag_max = std::max(imag_max, std::fabs(gamma->values[2 * k + 1]));
+    }
+    if (imag_max != 0.0)
+      throw StepError("the CoulombVertex has an imaginary part: the real path needs a real "
+                      "vertex (use complex: 1 in CoulombIntegralsFromVertex)");
+  }
+  const double *g = host.empty() ? gamma->values.data() : host.data();
   sisi_engine *e = nullptr;
   check(sisi_engine_create(&d, &e), "sisi_engine_create");
-  check(sisi_engine_set_vertex(e, gamma->values.data()), "sisi_engine_set_vertex");
+  if (complex_path) check(sisi_engine_set_vertex_complex(e, g), "sisi_engine_set_vertex_complex");
+  else check(sisi_engine_set_vertex(e, g), "sisi_engine_set_vertex");
   check(sisi_engine_set_eigenenergies(e, eps_i->values.data(), eps_a->values.data()),
         "sisi_engine_set_eigenenergies");
   check(sisi_integrals_from_vertex(e), "sisi_integrals_from_vertex");
-  engines().push_back({gamma, eps_i, eps_a, e});
+  engines().push_back({gamma, eps_i, eps_a, complex_path, e});
   return e;
 }
 

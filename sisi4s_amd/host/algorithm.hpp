@@ -19,10 +19,12 @@
 
 namespace sisi_host {
 
-// Dense real tensor, first index fastest (CTF global order).
+// Dense tensor, first index fastest (CTF global order); a complex tensor
+// (the reference's Tensor<complex>) holds interleaved (re, im) pairs.
 struct Tensor {
   std::vector<int64_t> lens;
   std::vector<double> values;
+  bool complex = false;
   int64_t size() const {
     int64_t n = 1;
     for (auto l : lens) n *= l;
@@ -115,9 +117,12 @@ struct Registrar {
 // Throws StepError with the engine's message on a non-zero status.
 void check(int rc, const char *what);
 
-// Engine shared by the steps that read the same vertex (uploaded once).
+// Engine shared by the steps that read the same vertex (uploaded once), on
+// the real path or, with complex_path, on the complex one
+// (sisi_engine_set_vertex_complex).  The real path needs a vertex without
+// imaginary part.
 sisi_engine *engine_for(const std::shared_ptr<Tensor> &gamma, const std::shared_ptr<Tensor> &eps_i,
-                        const std::shared_ptr<Tensor> &eps_a);
+                        const std::shared_ptr<Tensor> &eps_a, bool complex_path = false);
 void release_engines();
 
 }  // namespace sisi_host

@@ -16,16 +16,43 @@ std::string num(double v) {
   return o.str();
 }
 
-std::shared_ptr<Tensor> make_tensor(std::vector<int64_t> lens) {
+std::shared_ptr<Tensor> make_tensor(std::vector<int64_t> lens, bool complex = false) {
   auto t = std::make_shared<Tensor>();
   t->lens = std::move(lens);
-  t->values.assign(size_t(t->size()), 0.0);
+  t->complex = complex;
+  t->values.assign(size_t(t->size()) * (complex ? 2 : 1), 0.0);
   return t;
 }
 
-sisi_engine *engine_of(Algorithm &a) {
+sisi_engine *engine_of(Algorithm &a, bool complex_path = false) {
   return engine_for(a.getTensorPtr("CoulombVertex"), a.getTensorPtr("HoleEigenEnergies"),
-                    a.getTensorPtr("ParticleEigenEnergies"));
+                    a.getTensorPtr("ParticleEigenEnergies"), complex_path);
+}
+
+// The reference runs the complex path when the integrals it is given are
+// complex tensors (ClusterSinglesDoublesAlgorithm.cxx:25-35).
+bool complex_integrals(Algorithm &a) {
+  return a.isArgumentGiven("PPHHCoulombIntegrals") &&
+         a.getTensorArgument("PPHHCoulombIntegrals").complex;
+}
+
+// Amplitudes of the path's type: a real tensor given to the complex path
+// gets a zero imaginary part; a complex one given to the real path must
+// have none.
+std::vector<double> amplitudes_for(const Tensor &t, bool complex_path, const char *name) {
+  if (t.complex == complex_path) return t.values;
+  const size_t n = size_t(t.size());
+  std::vector<double> out(complex_path ? 2 * n : n, 0.0);
+  for (size_t k = 0; k < n; ++k) {
+    if (complex_path) {
+      out[2 * k] = t.values[k];
+    } else {
+      out[k] = t.values[2 * k];
+      if (t.values[2 * k + 1] != 0.0)
+        throw StepError(std::string(name) + " is complex but the integrals are real");
+    }
+  }
+  return out;
 }
 
 int precision_argument(Algorithm &a) {
@@ -95,16 +122,17 @@ class UegVertexGenerator : public Algorithm {
 };
 
 // CoulombIntegralsFromVertex (CoulombIntegralsFromVertex.cxx:23-148, real
-// integrals 203-571): publishes the requested classes.
+// integrals 203-571, complex: 1 calculateComplexIntegrals 613-810):
+// publishes the requested classes (complex tensors with complex: 1).
 class CoulombIntegralsFromVertex : public Algorithm {
  public:
   using Algorithm::Algorithm;
   std::string getName() override { return "CoulombIntegralsFromVertex"; }
   void run() override {
-    if (getIntegerArgument("complex", 0) != 0) throw StepError("complex integrals are not supported");
     if (getIntegerArgument("antisymmetrize", 0) != 0)
       throw StepError("antisymmetrized integrals are not supported");
-    sisi_engine *e = engine_of(*this);
+    const bool cplx = getIntegerArgument("complex", 0) != 0;
+    sisi_engine *e = engine_of(*this, cplx);
     const auto &ei = getTensorArgument("HoleEigenEnergies");
     const auto &ea = getTensorArgument("ParticleEigenEnergies");
     const int64_t o = ei.lens[0], v = ea.lens[0];
@@ -116,10 +144,14 @@ class CoulombIntegralsFromVertex : public Algorithm {
                    {"PHPHCoulombIntegrals", SISI_PHPH, {v, o, v, o}},
                    {"HHHHCoulombIntegrals", SISI_HHHH, {o, o, o, o}},
                    {"HHHPCoulombIntegrals", SISI_HHHP, {o, o, o, v}},
-                   {"HHPPCoulombIntegrals", SISI_HHPP, {o, o, v, v}}};
+                   {"HHPPCoulombIntegrals", SISI_HHPP, {o, o, v, v}},
+                   {"PHHPCoulombIntegrals", SISI_PHHP, {v, o, o, v}},
+                   {"PHHHCoulombIntegrals", SISI_PHHH, {v, o, o, o}}};
     for (auto &c : classes) {
       if (!out_.count(c.name)) continue;
-      auto t = make_tensor(c.lens);
+      if (!cplx && (c.cls == SISI_PHHP || c.cls == SISI_PHHH))
+        throw StepError(std::string(c.name) + " is built with complex: 1 only");
+      auto t = make_tensor(c.lens, cplx);
       check(sisi_get_integrals(e, c.cls, t->values.data()), "sisi_get_integrals");
       allocatedTensorArgument(c.name, t);
     }
@@ -130,6 +162,7 @@ class CoulombIntegralsFromVertex : public Algorithm {
     }
     emitted.scalars.push_back({"No", std::to_string(o)});
     emitted.scalars.push_back({"Nv", std::to_string(v)});
+    emitted.scalars.push_back({"complex", cplx ? "1" : "0"});
   }
 };
 
@@ -146,7 +179,8 @@ class CcsdEnergyFromCoulombIntegrals : public Algorithm {
     if (getIntegerArgument("unrestricted", 0) != 0 || getIntegerArgument("antisymmetrize", 0) != 0)
       throw StepError("unrestricted / antisymmetrized energies are not supported");
     if (isArgumentGiven("HPFockMatrix")) throw StepError("HPFockMatrix is not supported");
-    sisi_engine *e = engine_of(*this);
+    const bool cplx = complex_integrals(*this);
+    sisi_engine *e = engine_of(*this, cplx);
     const int64_t o = getTensorArgument("HoleEigenEnergies").lens[0];
     const int64_t v = getTensorArgument("ParticleEigenEnergies").lens[0];
     sisi_ccsd_opts opts;
@@ -178,12 +212,18 @@ class CcsdEnergyFromCoulombIntegrals : public Algorithm {
       else throw StepError("unknown refinePrecision " + rp + " (bf16, bf16x3, fp32)");
     }
     std::vector<double> t1, t2;
-    if (isArgumentGiven("initialSinglesAmplitudes")) t1 = getTensorArgument("initialSinglesAmplitudes").values;
-    if (isArgumentGiven("initialDoublesAmplitudes")) t2 = getTensorArgument("initialDoublesAmplitudes").values;
-    if (!t1.empty() && int64_t(t1.size()) != v * o) throw StepError("initialSinglesAmplitudes has the wrong size");
-    if (!t2.empty() && int64_t(t2.size()) != v * v * o * o)
+    if (isArgumentGiven("initialSinglesAmplitudes"))
+      t1 = amplitudes_for(getTensorArgument("initialSinglesAmplitudes"), cplx,
+                          "initialSinglesAmplitudes");
+    if (isArgumentGiven("initialDoublesAmplitudes"))
+      t2 = amplitudes_for(getTensorArgument("initialDoublesAmplitudes"), cplx,
+                          "initialDoublesAmplitudes");
+    const int64_t w = cplx ? 2 : 1;  // doubles per element
+    if (!t1.empty() && int64_t(t1.size()) != w * v * o)
+      throw StepError("initialSinglesAmplitudes has the wrong size");
+    if (!t2.empty() && int64_t(t2.size()) != w * v * v * o * o)
       throw StepError("initialDoublesAmplitudes has the wrong size");
-    if (!t2.empty() && t1.empty()) t1.assign(size_t(v * o), 0.0);
+    if (!t2.empty() && t1.empty()) t1.assign(size_t(w * v * o), 0.0);
     sisi_ccsd_report r;
     check(sisi_ccsd_solve(e, &opts, t1.empty() ? nullptr : t1.data(),
                           t2.empty() ? nullptr : t2.data(), &r),
@@ -206,12 +246,12 @@ class CcsdEnergyFromCoulombIntegrals : public Algorithm {
     emitted.scalars.push_back({"singlet", num(0.25 * r.direct - 0.5 * r.exchange)});
     emitted.scalars.push_back({"triplet", num(0.75 * r.direct + 1.5 * r.exchange)});
     if (isArgumentGiven("PairEnergy")) {  // cxx:218-235
-      auto P = make_tensor({o, o});
+      auto P = make_tensor({o, o}, cplx);
       check(sisi_ccsd_pair_energies(e, P->values.data()), "sisi_ccsd_pair_energies");
       allocatedTensorArgument("PairEnergy", P);
     }
     if (isArgumentGiven("CcsdSinglesAmplitudes") || isArgumentGiven("CcsdDoublesAmplitudes")) {
-      auto T1 = make_tensor({v, o}), T2 = make_tensor({v, v, o, o});
+      auto T1 = make_tensor({v, o}, cplx), T2 = make_tensor({v, v, o, o}, cplx);
       check(sisi_ccsd_get_amplitudes(e, T1->values.data(), T2->values.data()),
             "sisi_ccsd_get_amplitudes");
       allocatedTensorArgument("CcsdSinglesAmplitudes", T1);
@@ -228,9 +268,12 @@ class CcsdPerturbativeTriples : public Algorithm {
   using Algorithm::Algorithm;
   std::string getName() override { return "CcsdPerturbativeTriples"; }
   void run() override {
+    if (complex_integrals(*this))
+      throw StepError("complex integrals: (T) of the complex path is CcsdPerturbativeTriplesComplex");
     sisi_engine *e = engine_of(*this);
     const auto &T1 = getTensorArgument("CcsdSinglesAmplitudes");
     const auto &T2 = getTensorArgument("CcsdDoublesAmplitudes");
+    if (T1.complex || T2.complex) throw StepError("CcsdPerturbativeTriples needs real amplitudes");
     const int64_t o = getTensorArgument("HoleEigenEnergies").lens[0];
     const int64_t v = getTensorArgument("ParticleEigenEnergies").lens[0];
     if (T1.size() != v * o || T2.size() != v * v * o * o)
@@ -246,7 +289,38 @@ class CcsdPerturbativeTriples : public Algorithm {
   }
 };
 
+// CcsdPerturbativeTriplesComplex (CcsdPerturbativeTriplesComplex.cxx:32-85):
+// the complex calculator when the PPHH integrals are complex, else the real
+// one; output is CCSD energy + Re (T).
+class CcsdPerturbativeTriplesComplex : public Algorithm {
+ public:
+  using Algorithm::Algorithm;
+  std::string getName() override { return "CcsdPerturbativeTriplesComplex"; }
+  void run() override {
+    const bool cplx = complex_integrals(*this);
+    sisi_engine *e = engine_of(*this, cplx);
+    const int64_t o = getTensorArgument("HoleEigenEnergies").lens[0];
+    const int64_t v = getTensorArgument("ParticleEigenEnergies").lens[0];
+    const auto T1 = amplitudes_for(getTensorArgument("CcsdSinglesAmplitudes"), cplx,
+                                   "CcsdSinglesAmplitudes");
+    const auto T2 = amplitudes_for(getTensorArgument("CcsdDoublesAmplitudes"), cplx,
+                                   "CcsdDoublesAmplitudes");
+    const int64_t w = cplx ? 2 : 1;
+    if (int64_t(T1.size()) != w * v * o || int64_t(T2.size()) != w * v * v * o * o)
+      throw StepError("amplitudes do not match the eigenenergies");
+    double et = 0;
+    check(sisi_triples_energy(e, T1.data(), T2.data(), precision_argument(*this), &et),
+          "sisi_triples_energy");
+    const double ec = getRealArgument("CcsdEnergy");
+    emitted.scalars.push_back({"triples", num(et)});
+    emitted.scalars.push_back({"ccsd", num(ec)});
+    emitted.scalars.push_back({"energy", num(ec + et)});
+    setRealArgument("CcsdPerturbativeTriplesComplexEnergy", ec + et);
+  }
+};
+
 static Registrar<GenerateSyntheticSystem> r0("GenerateSyntheticSystem");
+static Registrar<CcsdPerturbativeTriplesComplex> r5("CcsdPerturbativeTriplesComplex");
 static Registrar<CoulombIntegralsFromVertex> r1("CoulombIntegralsFromVertex");
 static Registrar<UegVertexGenerator> r4("UegVertexGenerator");
 static Registrar<CcsdEnergyFromCoulombIntegrals> r2("CcsdEnergyFromCoulombIntegrals");

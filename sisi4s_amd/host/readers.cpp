@@ -1,7 +1,8 @@
 // Input readers on the caller's side of the path: the Coulomb vertex and
 // eigenenergies from the files the reference reads, and TensorNorm for
-// checking what was read.  The real CCSD path needs a real vertex, so a
-// complex one is accepted only when its imaginary part is zero.
+// checking what was read.  A complex vertex with a non-zero imaginary part
+// is kept complex (for the complex path, complex: 1); one whose imaginary
+// part is zero is stored real.
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -81,6 +82,7 @@ class CoulombVertexReader : public Algorithm {
     bool have_real = false, have_eps = false;
     double imag_max = 0.0;
     const int64_t nvert = NG * Np * Np;
+    std::vector<double> imag;
     int64_t offset = sizeof(header);
     while (offset < fileSize) {
       struct {
@@ -97,17 +99,16 @@ class CoulombVertexReader : public Algorithm {
           std::strncmp(chunk.magic, "FTODimag", 8) == 0) {
         if (payload < 8 * nvert) throw StepError("Invalid file format: short vertex chunk");
         const bool real = chunk.magic[4] == 'r';
-        std::vector<double> buf;
         double *dst = gamma->values.data();
         if (!real) {
-          buf.resize(size_t(nvert));
-          dst = buf.data();
+          imag.resize(size_t(nvert));
+          dst = imag.data();
         }
         if (!f.read(reinterpret_cast<char *>(dst), 8 * nvert))
           throw StepError("Invalid file format: short vertex chunk");
         if (real) have_real = true;
         else
-          for (double x : buf) imag_max = std::max(imag_max, std::fabs(x));
+          for (double x : imag) imag_max = std::max(imag_max, std::fabs(x));
       } else if (std::strncmp(chunk.magic, "FTODepsi", 8) == 0) {
         if (payload < 8 * Np) throw StepError("Invalid file format: short eigenenergy chunk");
         if (!f.read(reinterpret_cast<char *>(epsi->values.data()), 8 * No) ||
@@ -119,9 +120,17 @@ class CoulombVertexReader : public Algorithm {
     }
     if (!have_real || !have_eps)
       throw StepError("Invalid file format: " + fileName + " lacks the FTODreal or FTODepsi chunk");
-    if (imag_max != 0.0)
-      throw StepError("CoulombVertexReader: the vertex has an imaginary part (max |Im| = " +
-                      num(imag_max) + "); only the real path is implemented");
+    if (imag_max != 0.0) {  // complex vertex: interleave (re, im)
+      auto z = std::make_shared<Tensor>();
+      z->lens = gamma->lens;
+      z->complex = true;
+      z->values.resize(size_t(2 * nvert));
+      for (int64_t k = 0; k < nvert; ++k) {
+        z->values[size_t(2 * k)] = gamma->values[size_t(k)];
+        z->values[size_t(2 * k + 1)] = imag[size_t(k)];
+      }
+      gamma = z;
+    }
     allocatedTensorArgument("CoulombVertex", gamma);
     allocatedTensorArgument("HoleEigenEnergies", epsi);
     allocatedTensorArgument("ParticleEigenEnergies", epsa);
@@ -204,9 +213,10 @@ class Read : public Algorithm {
           t->values[size_t(k)] = z[size_t(2 * k)];
           imag_max = std::max(imag_max, std::fabs(z[size_t(2 * k + 1)]));
         }
-        if (imag_max != 0.0)
-          throw StepError("Read: " + fileName + " has an imaginary part (max |Im| = " + num(imag_max) +
-                          "); only the real path is implemented");
+        if (imag_max != 0.0) {  // keep it complex, (re, im) interleaved
+          t->complex = true;
+          t->values = std::move(z);
+        }
       }
     } else {
       throw StepError("Read: Complex64 tensors are read from IeeeBinaryFile elements only");
