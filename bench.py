@@ -78,7 +78,8 @@ def main():
     ap.add_argument("--mixed-steps", type=int, default=2)
     ap.add_argument("--triples-blocks", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--mode", choices=["iteration", "converge", "batched"], default="iteration",
+    ap.add_argument("--mode", choices=["iteration", "converge", "batched", "complex"],
+                    default="iteration",
                     help="iteration: timed FP64 iterations (default line); converge: FP64 and "
                          "mixed runs to |dT|/|T| <= 1e-8 with (T); batched: 64 small systems")
     ap.add_argument("--systems", type=int, default=64)
@@ -108,6 +109,8 @@ def main():
         return converge(args)
     if args.mode == "batched":
         return batched(args)
+    if args.mode == "complex":
+        return complex_path(args)
 
     No, Nv, NG, s = args.No, args.Nv, args.NG, args.slice
     log = lambda m: print("[bench] " + m, file=sys.stderr, flush=True)  # noqa: E731
@@ -253,6 +256,105 @@ def main():
             "mixed_bf16x3": mixed,
             "triples": triples,
         },
+    }
+    print(json.dumps(out), flush=True)
+
+
+def complex_vertex(g, No, seed):
+    """A genuinely complex vertex with the physics of g: a random phase per
+    auxiliary function and per orbital, Gamma[G,p,q] e^{i(f_G - t_p + t_q)};
+    every Coulomb integral only picks up orbital phases."""
+    rng = np.random.default_rng(seed)
+    NG, Np, _ = g.shape
+    f = np.exp(1j * rng.uniform(0, 2 * np.pi, NG))
+    t = np.exp(1j * rng.uniform(0, 2 * np.pi, Np))
+    z = g.astype(np.complex128)
+    z *= f[:, None, None]
+    z *= t.conj()[None, :, None]
+    z *= t[None, None, :]
+    return z
+
+
+def complex_path(args):
+    """Complex CCSD iterations (CcsdEnergyFromCoulombIntegrals.cxx:496-891)
+    and complex (T) blocks at the given size, FP64; flops counted as four
+    real products per complex product."""
+    from sisi4s_amd import engine as eng
+    No, Nv, NG, s = args.No, args.Nv, args.NG, args.slice
+    log = lambda m: print("[bench] " + m, file=sys.stderr, flush=True)  # noqa: E731
+    g, ei, ea = eng.generate_system(args.seed, No, Nv, NG)
+    z = complex_vertex(g, No, args.seed + 1)
+    del g
+    e = eng.Engine(No, Nv, NG)
+    e.set_vertex_complex(z)
+    e.set_eigenenergies(ei, ea)
+    del z
+    t0 = time.perf_counter()
+    e.integrals_from_vertex()
+    t_int = time.perf_counter() - t0
+    log("complex integrals %.2fs" % t_int)
+    opts = eng.default_opts(mixer="DiisMixer", max_residua=6, max_iterations=10000,
+                            slice_size=s, amplitudes_convergence=1e-8, energy_convergence=1e-10)
+    e.ccsd_begin(opts)
+    for k in range(args.warmup):
+        t0 = time.perf_counter()
+        rep = e.ccsd_iterate(1)
+        log("warmup iteration %d: %.2fs E=%.10f" % (k + 1, time.perf_counter() - t0, rep["energy"]))
+    e.enable_stats(True)
+    e.reset_stats()
+    times = []
+    for k in range(args.steps):
+        t0 = time.perf_counter()
+        rep = e.ccsd_iterate(1)
+        times.append(time.perf_counter() - t0)
+        log("timed iteration %d: %.3fs E=%.10f Im E=%.1e" % (k + 1, times[-1], rep["energy"],
+                                                              rep["energy_imag"]))
+    st = e.stats()
+    e.enable_stats(False)
+    t_iter = float(np.mean(times))
+    flops_it = 4.0 * ccsd_iteration_flops(No, Nv, NG, s)
+    g_sec, g_flops, g_n = st["gemm_seconds"][0], st["gemm_flops"][0], st["gemm_launches"][0]
+    achieved = g_flops / g_sec / 1e12 if g_sec > 0 else 0.0
+    triples = {}
+    if args.triples_blocks > 0:
+        rng = np.random.default_rng(1)
+        blocks = []
+        while len(blocks) < args.triples_blocks:
+            b = sorted(rng.choice(No, 3, replace=False).tolist())
+            if b not in blocks:
+                blocks.append(b)
+        e.triples(precision="fp64", blocks=[blocks[0]])  # builds the (T) integrals
+        nblk = No * (No + 1) * (No + 2) // 6
+        t0 = time.perf_counter()
+        e.triples(precision="fp64", blocks=blocks)
+        tb = (time.perf_counter() - t0) / len(blocks)
+        log("complex (T) %.4fs per block, %.0fs extrapolated" % (tb, tb * nblk))
+        triples = dict(seconds_per_block=tb, blocks=nblk, seconds_extrapolated=tb * nblk,
+                       tflops=4.0 * triples_flops(No, Nv, NG) / (tb * nblk) / 1e12,
+                       sample="%d random i<j<k blocks, FP64" % len(blocks))
+    out = {
+        "metric": "complex CCSD iteration time (s), FP64; complex (T) time",
+        "value": t_iter,
+        "unit": "s/iteration (complex CCSD, FP64)",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * t_iter,
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "c128",
+        "data": "synthetic (seeded Philox vertex with random auxiliary and orbital phases, "
+                "No=%d Nv=%d NG=%d)" % (No, Nv, NG),
+        "config": {"workload": "complex CCSD FP64 iteration, No=%d Nv=%d NG=%d, ladder slices of "
+                               "%d, DIIS(6), MP2 start" % (No, Nv, NG, s),
+                   "No": No, "Nv": Nv, "NG": NG, "slice": s},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "kernel": "gemm_k4_kernel<FP64>", "launches": g_n,
+                     "avg_launch_ms": 1e3 * g_sec / max(g_n, 1)},
+        "cpu_baseline": None,
+        "extra": {"iteration_tflops": flops_it / t_iter / 1e12,
+                  "iteration_pct_fp64_peak": 100.0 * flops_it / t_iter / 1e12 / PEAK_FP64_TFLOPS,
+                  "iteration_algorithmic_flops": flops_it, "integrals_seconds": t_int,
+                  "gemm_seconds_per_iteration": g_sec / args.steps,
+                  "bandwidth_kernels_seconds_per_iteration": st["other_seconds"] / args.steps,
+                  "energy_after_timed_iterations": rep["energy"],
+                  "energy_imag": rep["energy_imag"], "triples": triples},
     }
     print(json.dumps(out), flush=True)
 
