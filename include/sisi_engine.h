@@ -165,6 +165,18 @@ int sisi_get_integrals(sisi_engine *e, int which, double *out);
 /* Precision used by the dense contractions of the next residual call. */
 int sisi_engine_set_precision(sisi_engine *e, int precision);
 
+/* CoulombFactors Lambda[G,R] (complex128 [NG, NR]) and FactorOrbitals
+ * Pi[p,R] (complex128 [Np, NR]): while set (NR > 0), the particle-particle
+ * ladder of the real residual is the factor-based one,
+ * sliceAmplitudesFromCoupledCoulombFactors (ClusterSinglesDoublesAlgorithm.cxx:
+ * 567-672) over factor slices of factors_slice_size (<= 0: Nv, the
+ * factorsSliceSize default, CcsdEnergyFromCoulombIntegrals.cxx:340-372).  It
+ * equals the vertex ladder when Gamma[G,p,q] = sum_R Lambda_GR Pi_pR Pi_qR.
+ * NR = 0 removes the factors.  Not built for complex amplitudes or with the
+ * refinement residuals. */
+int sisi_engine_set_coulomb_factors(sisi_engine *e, int NR, int factors_slice_size,
+                                    const double *lambda, const double *pi);
+
 /* Residual variant of the next residual calls: the ppl and distinguishable
  * arguments of CcsdEnergyFromCoulombIntegrals (cxx:123-124, 324-338);
  * default ppl = 1, distinguishable = 0.  sisi_ccsd_begin / _solve set it

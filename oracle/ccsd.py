@@ -95,14 +95,36 @@ def ppl_ladder(dGab, X, slice_size, R2):
             slice_into_residuum(Rxyij, xs, ys, R2)
 
 
+def factor_ladder(T1, X, Lambda, Pi, slice_size, R2):
+    """sliceAmplitudesFromCoupledCoulombFactors for real amplitudes
+    (ClusterSinglesDoublesAlgorithm.cxx:567-672) over all pairs of factor
+    slices (CcsdEnergyFromCoulombIntegrals.cxx:357-369): Lambda[G,R] and
+    Pi[p,R] complex, X = T2 + T1 T1.  Equals the vertex ladder when
+    Gamma[G,p,q] = sum_R Lambda[G,R] Pi[p,R] Pi[q,R]."""
+    Nv, No = T1.shape
+    NR = Pi.shape[1]
+    PiaR, PiiR = Pi[No:, :], Pi[:No, :]
+    dPi = PiaR - ein("kR,ak->aR", PiiR, T1)                    # cxx:639-650
+    for b in range(0, NR, slice_size):
+        for a in range(0, NR, slice_size):
+            L, Rr = slice(a, min(a + slice_size, NR)), slice(b, min(b + slice_size, NR))
+            XRaij = ein("cdij,cR->Rdij", X, PiaR[:, L].conj())           # cxx:607-610
+            XRSij = ein("Rdij,dS->RSij", XRaij, PiaR[:, Rr])            # cxx:613-614
+            VRS = ein("GR,GS->RS", Lambda[:, L].conj(), Lambda[:, Rr])  # cxx:617-621
+            XRSij = XRSij * VRS[:, :, None, None]                        # cxx:623
+            XRbij = ein("RSij,bS->Rbij", XRSij, dPi[:, Rr])              # cxx:660
+            R2 += np.real(ein("Rbij,aR->abij", XRbij, dPi[:, L].conj()))  # cxx:666-668
+
+
 def residual(it, T1, T2, gamma, ints, slice_size=None, initial_doubles=False,
-             ppl=True, distinguishable=False):
+             ppl=True, distinguishable=False, factors=None):
     """CcsdEnergyFromCoulombIntegrals::getResiduum, real path (cxx:27-494),
     no CoulombFactors.  distinguishable (cxx:123-124) selects the
     distinguishable-cluster terms (cxx:150-160, 176-186, 244-246, 283-285,
     316-320); ppl = False (cxx:324-333) skips the particle-particle ladder
     and contracts the last hole-ladder term with T2 + T1 T1.  Returns
-    (R1, R2)."""
+    (R1, R2).  factors = (Lambda, Pi, factors_slice_size) selects the
+    factor-based ladder (cxx:340-372) in place of the vertex ladder."""
     Nv, No = T1.shape
     Vabij = ints["PPHH"]
     R1 = np.zeros_like(T1)
@@ -182,7 +204,9 @@ def residual(it, T1, T2, gamma, ints, slice_size=None, initial_doubles=False,
     R2 += ein("klij,abkl->abij", Xklij, T2 if ppl else X)     # cxx:326-333
 
     # particle-particle ladder (cxx:336-453)
-    if ppl:
+    if ppl and factors is not None:
+        factor_ladder(T1, X, factors[0], factors[1], factors[2], R2)
+    elif ppl:
         dGab = Gab - ein("Gbk,ak->Gab", Gai, T1)
         ppl_ladder(dGab, X, slice_size, R2)
 
@@ -316,7 +340,7 @@ class LinearMixer:
 def solve(gamma, eps_i, eps_a, *, mixer="DiisMixer", max_residua=4,
           mixing_ratio=1.0, max_iterations=16, amplitudes_convergence=1e-5,
           energy_convergence=1e-6, level_shift=0.0, slice_size=None,
-          initial=None, callback=None, ppl=True, distinguishable=False):
+          initial=None, callback=None, ppl=True, distinguishable=False, factors=None):
     """ClusterSinglesDoublesAlgorithm::run<double> (cxx:38-128).
 
     Returns dict(energy, T1, T2, energies=[per iteration], iterations,
@@ -341,7 +365,7 @@ def solve(gamma, eps_i, eps_a, *, mixer="DiisMixer", max_residua=4,
     while i < max_iterations:
         R1, R2 = residual(i, T[0], T[1], gamma, ints, slice_size,
                           initial_doubles=initial is not None, ppl=ppl,
-                          distinguishable=distinguishable)
+                          distinguishable=distinguishable, factors=factors)
         E1, E2 = estimate_amplitudes(R1, R2, T[0], T[1], eps_i, eps_a, level_shift)
         dT = [E1 - T[0], E2 - T[1]]
         mix.append([E1, E2], dT)

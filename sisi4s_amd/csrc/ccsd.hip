@@ -223,7 +223,10 @@ void ccsd_residual(Engine &e, int it, bool initial_doubles, int slice_size, cons
     // ppl = 1: the T1 part of this term comes with the ladder (cxx:326-333)
     ctr(e, 1.0, "klij,abkl->abij", Xklij, e.ppl ? T2 : X, 1.0, R2, prec);
   }
-  if (e.ppl) {
+  if (e.ppl && e.NR > 0) {
+    // factor-based ladder (cxx:340-372)
+    factor_ladder(e, T1p, Xb.get(), R2p, e.prec_ladder);
+  } else if (e.ppl) {
     // particle-particle ladder (cxx:336-453)
     if (slice_size <= 0) slice_size = int(o);
     DBuf dGabb(&e.pool, G * v * v);
@@ -296,6 +299,7 @@ void ccsd_residual(Engine &e, int it, bool initial_doubles, int slice_size, cons
 void ccsd_residual_delta(Engine &e, int slice_size, const double *T1c, const double *T2c,
                          double *R1p, double *R2p) {
   SISI_REQUIRE(!e.cplx, SISI_ERR_INVALID, "refinement residuals are not built for complex amplitudes");
+  SISI_REQUIRE(e.NR == 0, SISI_ERR_INVALID, "refinement residuals are not built with CoulombFactors");
   Anchor &an = e.anchor;
   SISI_REQUIRE(an.valid, SISI_ERR_STATE, "refinement residual without an anchor");
   SISI_REQUIRE(an.ppl == e.ppl && an.distinguishable == e.distinguishable, SISI_ERR_STATE,
@@ -564,6 +568,7 @@ void ccsd_residual_delta(Engine &e, int slice_size, const double *T1c, const dou
 void ccsd_residual_anchor(Engine &e, int slice_size, const double *T1, const double *T2,
                           double *R1, double *R2) {
   SISI_REQUIRE(!e.cplx, SISI_ERR_INVALID, "refinement residuals are not built for complex amplitudes");
+  SISI_REQUIRE(e.NR == 0, SISI_ERR_INVALID, "refinement residuals are not built with CoulombFactors");
   const int64_t n1 = int64_t(e.n1()), n2 = int64_t(e.n2());
   const int p = e.precision, pl = e.prec_ladder;
   e.precision = SISI_FP64;
@@ -674,6 +679,8 @@ void ccsd_begin(Engine &e, const sisi_ccsd_opts &opts, const double *T1h, const 
   SISI_REQUIRE(opts.refine_interval >= 0, SISI_ERR_INVALID, "refine_interval must be >= 0");
   SISI_REQUIRE(!e.cplx || opts.precision == SISI_FP64 || opts.refine_interval == 0,
                SISI_ERR_INVALID, "refinement residuals are not built for complex amplitudes");
+  SISI_REQUIRE(e.NR == 0 || opts.precision == SISI_FP64 || opts.refine_interval == 0,
+               SISI_ERR_INVALID, "refinement residuals are not built with CoulombFactors");
   SISI_REQUIRE(opts.refine_precision == SISI_FP32 || opts.refine_precision == SISI_BF16X3 ||
                    opts.refine_precision == SISI_BF16,
                SISI_ERR_INVALID, "refine_precision must be SISI_FP32, SISI_BF16X3 or SISI_BF16");

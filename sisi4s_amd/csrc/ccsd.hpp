@@ -43,6 +43,13 @@ void zccsd_energy(Engine &e, const double *T1, const double *T2, double E[2], do
                   double exchange[2]);
 void zccsd_pair_energies(Engine &e, const double *T1, const double *T2, double *pair);
 
+// Coulomb factors (host arrays, interleaved complex128: lambda [NG, NR],
+// pi [Np, NR]) for the factor-based ladder; NR = 0 removes them.  slice <= 0
+// means Nv (factorsSliceSize default).
+void set_coulomb_factors(Engine &e, int NR, int slice, const double *lambda, const double *pi);
+// R2 += the factor-based ladder of X = T2 + T1 T1 (real amplitudes).
+void factor_ladder(Engine &e, const double *T1, const double *X, double *R2, int precision);
+
 // Driver (ClusterSinglesDoublesAlgorithm.cxx:38-128).
 void ccsd_begin(Engine &e, const sisi_ccsd_opts &opts, const double *T1h, const double *T2h);
 void ccsd_iterate(Engine &e, int n);

@@ -104,6 +104,8 @@ void zintegrals_from_vertex(Engine &e) {
 void zccsd_residual(Engine &e, int it, bool initial_doubles, int slice_size, const double *T1c,
                     const double *T2c, double *R1p, double *R2p) {
   SISI_REQUIRE(e.have_integrals && e.cplx, SISI_ERR_STATE, "residual: integrals not built");
+  SISI_REQUIRE(e.NR == 0 || !e.ppl, SISI_ERR_INVALID,
+               "the factor-based ladder is built for real amplitudes only");
   const int64_t o = e.No, v = e.Nv, G = e.NG;
   const int64_t n2 = v * v * o * o;
   double *T1p = const_cast<double *>(T1c), *T2p = const_cast<double *>(T2c);

@@ -140,6 +140,12 @@ class Engine {
   // U_klcd = 2 V_klcd - V_kldc of HHPP; HHPP as [a,b,i,j] and with a,b
   // exchanged (energy terms)
   DBuf Gia, Vaijb, Vijab, Vaijk, Uklcd, VijabT, VjiabT;
+  // Coulomb factors (CoulombFactors Lambda[G,R], FactorOrbitals Pi[p,R],
+  // planar complex) and V_RS = sum_G conj(Lambda_GR) Lambda_GS; NR > 0 makes
+  // the ladder the factor-based one (CcsdEnergyFromCoulombIntegrals.cxx:340-372)
+  int NR = 0, factors_slice = 0;
+  bool factors_real = true;  // imaginary parts all zero: real products only
+  DBuf Lam, PiF, VRS;
   DBuf Vbaij;  // V_abij with a,b exchanged (energy exchange term)
   DBuf Uabij;  // 2 V_abij - V_baij
   // (T) integrals, built on first use: V_bdck as [d,b,c,k] and as [d,c,b,k],

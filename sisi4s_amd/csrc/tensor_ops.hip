@@ -158,6 +158,37 @@ void zero(Engine &e, const View &dst) {
   copy(e, dst, dst, 0.0, 0.0);
 }
 
+// C[R,S,ij] *= V[R0 + R, S0 + S] over planar complex C (Rx x Ry x nij per
+// plane) and V (ldV rows, plane size nV); real == 1: real parts only.
+__global__ void scale_pairs_kernel(double *C, int64_t Rx, int64_t Ry, int64_t nij, const double *V,
+                                   int64_t ldV, int64_t nV, int64_t R0, int64_t S0, int real) {
+  const int64_t n = Rx * Ry * nij;
+  for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < n;
+       e += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t R = e % Rx, S = (e / Rx) % Ry;
+    const int64_t w = (R0 + R) + ldV * (S0 + S);
+    const double vr = V[w];
+    if (real) {
+      C[e] *= vr;
+    } else {
+      const double vi = V[nV + w], cr = C[e], ci = C[n + e];
+      C[e] = cr * vr - ci * vi;
+      C[n + e] = cr * vi + ci * vr;
+    }
+  }
+}
+
+void scale_pairs(Engine &e, double *C, int64_t Rx, int64_t Ry, int64_t nij, const double *V,
+                 int64_t ldV, int64_t nV, int64_t R0, int64_t S0, bool real) {
+  const int64_t n = Rx * Ry * nij;
+  if (n <= 0) return;
+  Timed t(*e.timers, Kind::Bandwidth, (real ? 16.0 : 32.0) * double(n));
+  const int blocks = int(std::min<int64_t>((n + 255) / 256, 1 << 16));
+  hipLaunchKernelGGL(scale_pairs_kernel, dim3(blocks), dim3(256), 0, e.stream, C, Rx, Ry, nij, V,
+                     ldV, nV, R0, S0, real ? 1 : 0);
+  SISI_HIP(hipGetLastError());
+}
+
 __global__ void axpby_kernel(int64_t n, double a, const double *x, double b, double *y) {
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += int64_t(gridDim.x) * blockDim.x) {

@@ -23,6 +23,12 @@ void axpby(Engine &e, int64_t n, double a, const double *x, double b, double *y)
 void add_t1t1(Engine &e, double *X, const double *T2, const double *T1,
               double f, int Nv, int No, const double *T1b = nullptr);
 
+// C[R,S,ij] *= V[R0 + R, S0 + S]: C planar complex (Rx x Ry x nij per plane),
+// V planar complex with leading dimension ldV and plane size nV; real: only
+// the real planes (both operands real).
+void scale_pairs(Engine &e, double *C, int64_t Rx, int64_t Ry, int64_t nij, const double *V,
+                 int64_t ldV, int64_t nV, int64_t R0, int64_t S0, bool real);
+
 // Deterministic FP64 sum_k x[k]*y[k] (returned to the host).
 double dot(Engine &e, int64_t n, const double *x, const double *y);
 

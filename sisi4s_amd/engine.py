@@ -88,6 +88,8 @@ SIGNATURES = [
     ("sisi_get_integrals", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _c_double_p]),
     ("sisi_engine_set_precision", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("sisi_engine_set_residual_variant", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    ("sisi_engine_set_coulomb_factors", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _c_double_p, _c_double_p]),
     ("sisi_ccsd_residual", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_double_p, _c_double_p,
       _c_double_p, _c_double_p]),
@@ -344,6 +346,20 @@ class Engine:
         if isinstance(precision, str):
             precision = PRECISIONS[precision]
         _check(self._lib.sisi_engine_set_precision(self._h, int(precision)))
+
+    def set_coulomb_factors(self, Lambda=None, Pi=None, slice_size=-1):
+        """CoulombFactors Lambda[G,R] and FactorOrbitals Pi[p,R]: the ladder
+        becomes the factor-based one (CcsdEnergyFromCoulombIntegrals.cxx:
+        340-372); no arguments remove them."""
+        if Lambda is None:
+            _check(self._lib.sisi_engine_set_coulomb_factors(self._h, 0, 0, None, None))
+            return
+        L = np.ascontiguousarray(np.asarray(Lambda, dtype=np.complex128).ravel(order="F"))
+        P = np.ascontiguousarray(np.asarray(Pi, dtype=np.complex128).ravel(order="F"))
+        NR = int(np.shape(Lambda)[1])
+        assert L.size == self.NG * NR and P.size == self.Np * NR
+        _check(self._lib.sisi_engine_set_coulomb_factors(
+            self._h, NR, int(slice_size), _ptr(L.view(np.float64)), _ptr(P.view(np.float64))))
 
     def set_residual_variant(self, ppl=True, distinguishable=False):
         """ppl / distinguishable of the next residual calls

@@ -175,7 +175,6 @@ class CcsdEnergyFromCoulombIntegrals : public Algorithm {
   using Algorithm::Algorithm;
   std::string getName() override { return "CcsdEnergyFromCoulombIntegrals"; }
   void run() override {
-    if (isArgumentGiven("CoulombFactors")) throw StepError("CoulombFactors is not supported");
     if (getIntegerArgument("unrestricted", 0) != 0 || getIntegerArgument("antisymmetrize", 0) != 0)
       throw StepError("unrestricted / antisymmetrized energies are not supported");
     if (isArgumentGiven("HPFockMatrix")) throw StepError("HPFockMatrix is not supported");
@@ -210,6 +209,27 @@ class CcsdEnergyFromCoulombIntegrals : public Algorithm {
       else if (rp == "bf16x3" || rp == "2") opts.refine_precision = SISI_BF16X3;
       else if (rp == "fp32" || rp == "1") opts.refine_precision = SISI_FP32;
       else throw StepError("unknown refinePrecision " + rp + " (bf16, bf16x3, fp32)");
+    }
+    if (isArgumentGiven("CoulombFactors")) {
+      // factor-based ladder (cxx:340-372): CoulombFactors [NG, NR] and
+      // FactorOrbitals [Np, NR], complex or real
+      if (cplx) throw StepError("CoulombFactors: the factor-based ladder is built for real amplitudes");
+      const auto &L = getTensorArgument("CoulombFactors");
+      const auto &P = getTensorArgument("FactorOrbitals");
+      if (L.lens.size() != 2 || P.lens.size() != 2 || L.lens[1] != P.lens[1] ||
+          P.lens[0] != o + v)
+        throw StepError("CoulombFactors [NG, NR] / FactorOrbitals [Np, NR] do not match");
+      const int64_t NR = L.lens[1];
+      int fs = int(getIntegerArgument("factorsSliceSize", -1));
+      if (fs == -1 && isArgumentGiven("factorsSliceFactor"))
+        fs = int(double(NR) * getRealArgument("factorsSliceFactor"));
+      const auto Lz = amplitudes_for(L, true, "CoulombFactors");
+      const auto Pz = amplitudes_for(P, true, "FactorOrbitals");
+      check(sisi_engine_set_coulomb_factors(e, int(NR), fs, Lz.data(), Pz.data()),
+            "sisi_engine_set_coulomb_factors");
+    } else {
+      check(sisi_engine_set_coulomb_factors(e, 0, 0, nullptr, nullptr),
+            "sisi_engine_set_coulomb_factors");
     }
     std::vector<double> t1, t2;
     if (isArgumentGiven("initialSinglesAmplitudes"))

@@ -406,3 +406,54 @@ def test_ueg_known_answers():
         assert abs(e.triples() - -0.0063019625641725016) <= 1e-14
     finally:
         e.close()
+
+
+def factor_system(seed, No, Nv, NG, NR, complex_factors):
+    rng = np.random.default_rng(seed)
+    Lam = rng.normal(size=(NG, NR)) / np.sqrt(NR * NG)
+    Pi = rng.normal(size=(No + Nv, NR)) / 2
+    g = np.einsum("GR,pR,qR->Gpq", Lam, Pi, Pi)
+    if complex_factors:
+        Lam = Lam + 0.3j * rng.normal(size=Lam.shape) / np.sqrt(NR * NG)
+        Pi = Pi + 0.3j * rng.normal(size=Pi.shape) / 2
+    ea = np.sort(rng.uniform(0.2, 3.0, Nv))
+    ei = np.sort(rng.uniform(-2.0, -0.5, No))
+    return g, ei, ea, Lam.astype(complex), Pi.astype(complex)
+
+
+@pytest.mark.parametrize("complex_factors", [False, True])
+def test_factor_ladder_matches_oracle_and_vertex_ladder(complex_factors):
+    """CoulombFactors / FactorOrbitals (ClusterSinglesDoublesAlgorithm.cxx:
+    567-672): the residual with the factor-based ladder against the oracle
+    for several factor slice sizes; with real factors of the vertex it is the
+    vertex-ladder residual; a whole run with the factors; removing them
+    restores the vertex ladder."""
+    from sisi4s_amd import engine
+    No, Nv, NG, NR = 4, 12, 16, 30
+    g, ei, ea, Lam, Pi = factor_system(5, No, Nv, NG, NR, complex_factors)
+    e = engine.Engine(No, Nv, NG)
+    e.set_system(g, ei, ea)
+    ints = ccsd.integrals_from_vertex(g, No)
+    T1, T2 = random_amplitudes(Nv, No, seed=6, scale=0.05)
+    V1, V2 = e.residual(1, T1, T2, slice_size=5)
+    for fs in (7, 30):
+        announce("factor ladder complex=%d slice=%d" % (complex_factors, fs))
+        e.set_coulomb_factors(Lam, Pi, slice_size=fs)
+        R1, R2 = e.residual(1, T1, T2, slice_size=5)
+        O1, O2 = ccsd.residual(1, T1, T2, g, ints, 5, factors=(Lam, Pi, fs))
+        scale = float(np.abs(O2).max())
+        assert float(np.abs(R2 - O2).max()) <= 1e-11 * scale
+        assert float(np.abs(R1 - O1).max()) <= 1e-11 * scale
+        if not complex_factors:
+            assert float(np.abs(R2 - V2).max()) <= 1e-11 * scale
+    kw = dict(mixer="DiisMixer", max_residua=6, max_iterations=60, slice_size=5,
+              amplitudes_convergence=1e-9, energy_convergence=1e-11)
+    rep = e.ccsd_solve(engine.default_opts(**kw))
+    ref = ccsd.solve(g, ei, ea, factors=(Lam, Pi, 30), **kw)
+    assert rep["converged"] and ref["converged"]
+    assert np.abs(e.energies() - np.array(ref["energies"])).max() <= 1e-10
+    with pytest.raises(engine.EngineError):
+        e.ccsd_solve(engine.default_opts(precision="bf16x3", refine_interval=8, **kw))
+    e.set_coulomb_factors()
+    R1, R2 = e.residual(1, T1, T2, slice_size=5)
+    assert np.array_equal(R2, V2)

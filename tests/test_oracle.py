@@ -375,3 +375,31 @@ def test_complex_triples_ueg_plane_wave_known_answer():
                  energy_convergence=1e-8, amplitudes_convergence=1e-8)
     et = tz.triples_energy(r["T1"], r["T2"], G, ei, ea)
     assert abs(et.real - -0.0063019625641725016) <= 1e-15
+
+
+@pytest.mark.parametrize("fs", [1, 7, 20])
+def test_factor_ladder_equals_vertex_ladder(fs):
+    """The factor-based ladder (ClusterSinglesDoublesAlgorithm.cxx:567-672)
+    equals the vertex ladder when Gamma[G,p,q] = sum_R Lambda_GR Pi_pR Pi_qR,
+    for every factor slice size; with complex factors it equals the real
+    path's Re sum_G conj(dGamma_G,ac) dGamma_G,bd X_cdij written densely."""
+    rng = np.random.default_rng(3)
+    No, Nv, NG, NR = 3, 8, 12, 20
+    Lam = rng.normal(size=(NG, NR)) / np.sqrt(NR)
+    Pi = rng.normal(size=(No + Nv, NR)) / 2
+    g = np.einsum("GR,pR,qR->Gpq", Lam, Pi, Pi)
+    ints = ccsd.integrals_from_vertex(g, No)
+    T1, T2 = random_amplitudes(Nv, No, seed=4)
+    R1, R2 = ccsd.residual(1, T1, T2, g, ints, 3)
+    F1, F2 = ccsd.residual(1, T1, T2, g, ints, 3, factors=(Lam.astype(complex),
+                                                           Pi.astype(complex), fs))
+    assert np.abs(F2 - R2).max() <= 1e-13 * np.abs(R2).max() and np.abs(F1 - R1).max() == 0.0
+    Lz = Lam + 1j * rng.normal(size=Lam.shape) / np.sqrt(NR)
+    Pz = Pi + 1j * rng.normal(size=Pi.shape) / 2
+    gz = np.einsum("GR,pR,qR->Gpq", Lz, Pz, Pz)
+    X = T2 + np.einsum("ai,bj->abij", T1, T1)
+    dG = gz[:, No:, No:] - np.einsum("Gbk,ak->Gab", gz[:, No:, :No], T1)
+    ref = np.real(np.einsum("Gac,Gbd,cdij->abij", dG.conj(), dG, X))
+    out = np.zeros_like(X)
+    ccsd.factor_ladder(T1, X, Lz, Pz, fs, out)
+    assert np.abs(out - ref).max() <= 1e-13 * np.abs(ref).max()

@@ -278,3 +278,56 @@ def test_ueg_plane_wave_complex_pipeline_from_files(tmp_path):
                - (-0.39269658954585018 - 0.0063019625641725016)) <= 1e-12
     assert "CcsdPerturbativeTriplesComplex" in r.stderr + r.stdout
     assert "CcsdPerturbativeTriplesEnergy" not in vals
+
+
+@pytest.mark.gpu
+def test_runner_factor_ladder_from_files(tmp_path):
+    """CcsdEnergyFromCoulombIntegrals with CoulombFactors / FactorOrbitals
+    (CcsdEnergyFromCoulombIntegrals.cxx:340-372) read from cc4s files, complex
+    factors, factorsSliceFactor: the oracle's energy with the factor-based
+    ladder."""
+    from oracle import ccsd
+    rng = np.random.default_rng(11)
+    No, Nv, NG, NR = 4, 12, 16, 30
+    Lam = rng.normal(size=(NG, NR)) / np.sqrt(NR * NG)
+    Pi = rng.normal(size=(No + Nv, NR)) / 2
+    g = np.einsum("GR,pR,qR->Gpq", Lam, Pi, Pi)
+    Lz = Lam + 0.3j * rng.normal(size=Lam.shape) / np.sqrt(NR * NG)
+    Pz = Pi + 0.3j * rng.normal(size=Pi.shape) / 2
+    ei = np.sort(rng.uniform(-2.0, -0.5, No))
+    ea = np.sort(rng.uniform(0.2, 3.0, Nv))
+    write_cc4s_tensor(tmp_path / "CoulombVertex", g, "Real64", "IeeeBinaryFile")
+    write_cc4s_tensor(tmp_path / "Lambda", Lz, "Complex64", "IeeeBinaryFile")
+    write_cc4s_tensor(tmp_path / "Pi", Pz, "Complex64", "IeeeBinaryFile")
+    (tmp_path / "EigenEnergies.yaml").write_text(
+        "metaData:\n  fermiEnergy: %.17g\n  energies: [%s]\n"
+        % (0.5 * (ei.max() + ea.min()), ", ".join("%.17g" % v for v in np.concatenate([ei, ea]))))
+    announce("runner factor ladder from files")
+    r, vals = run_steps(tmp_path, """
+- name: DefineHolesAndParticles
+  in: {fileName: "%s"}
+  out: {HoleEigenEnergies: $I, ParticleEigenEnergies: $A}
+- name: Read
+  in: {fileName: "%s"}
+  out: {destination: $CoulombVertex}
+- name: Read
+  in: {fileName: "%s"}
+  out: {destination: $Lambda}
+- name: Read
+  in: {fileName: "%s"}
+  out: {destination: $Pi}
+- name: CoulombIntegralsFromVertex
+  in: {CoulombVertex: $CoulombVertex, HoleEigenEnergies: $I, ParticleEigenEnergies: $A}
+  out: {PPHHCoulombIntegrals: $V}
+- name: CcsdEnergyFromCoulombIntegrals
+  in: {mixer: DiisMixer, maxResidua: 6, maxIterations: 15, integralsSliceSize: 5,
+       CoulombFactors: $Lambda, FactorOrbitals: $Pi, factorsSliceFactor: 0.4,
+       HoleEigenEnergies: $I, ParticleEigenEnergies: $A, PPHHCoulombIntegrals: $V,
+       CoulombVertex: $CoulombVertex}
+  out: {CcsdEnergy: $E}
+""" % tuple(tmp_path / f for f in ("EigenEnergies.yaml", "CoulombVertex.yaml", "Lambda.yaml",
+                                    "Pi.yaml")))
+    assert r.returncode == 0, r.stdout + r.stderr
+    ref = ccsd.solve(g, ei, ea, mixer="DiisMixer", max_residua=6, max_iterations=15, slice_size=5,
+                     factors=(Lz, Pz, int(NR * 0.4)))
+    assert abs(vals["CcsdEnergy"][0] - ref["energy"]) <= 1e-10
