@@ -76,6 +76,9 @@ def main():
     ap.add_argument("--slice", type=int, default=64)
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--mixed-steps", type=int, default=2)
+    ap.add_argument("--factors", type=int, default=0,
+                    help="iteration mode: NR > 0 runs the factor-based ladder with NR random "
+                         "complex CoulombFactors / FactorOrbitals (factor slices of Nv)")
     ap.add_argument("--triples-blocks", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mode", choices=["iteration", "converge", "batched", "complex"],
@@ -125,6 +128,13 @@ def main():
     e.integrals_from_vertex()
     t_int = time.perf_counter() - t0
     log("integrals %.2fs" % t_int)
+    if args.factors > 0:
+        rng = np.random.default_rng(args.seed + 2)
+        NR = args.factors
+        lam = (rng.standard_normal((NG, NR)) + 1j * rng.standard_normal((NG, NR))) / np.sqrt(NR * NG)
+        pi = (rng.standard_normal((No + Nv, NR)) + 1j * rng.standard_normal((No + Nv, NR))) / 2
+        e.set_coulomb_factors(lam, pi, slice_size=Nv)
+        log("factor-based ladder with NR=%d complex factors" % NR)
 
     opts = eng.default_opts(mixer="DiisMixer", max_residua=6, max_iterations=10000,
                             slice_size=s, amplitudes_convergence=1e-8, energy_convergence=1e-10)
@@ -146,6 +156,12 @@ def main():
     e.enable_stats(False)
     t_iter = float(np.sum(times)) / max(len(times), 1)
     flops_it = ccsd_iteration_flops(No, Nv, NG, s)
+    if args.factors > 0:
+        # the vertex ladder's term replaced by the factor ladder's products per
+        # factor slice pair, complex (four real products each)
+        NR = args.factors
+        flops_it += (4.0 * (4.0 * NR * Nv ** 2 * No ** 2 + 4.0 * NR ** 2 * Nv * No ** 2)
+                     - 2.0 * slice_pairs(Nv, s) * Nv ** 2 * (NG + No ** 2))
     g_sec, g_flops, g_n = st["gemm_seconds"][0], st["gemm_flops"][0], st["gemm_launches"][0]
     achieved = g_flops / g_sec / 1e12 if g_sec > 0 else 0.0
     other_bw = st["other_bytes"] / st["other_seconds"] / 1e9 if st["other_seconds"] > 0 else 0.0
@@ -237,7 +253,9 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded Philox vertex, No=%d Nv=%d NG=%d)" % (No, Nv, NG),
         "config": {"workload": "CCSD FP64 iteration, No=%d Nv=%d NG=%d, ladder slices of %d, "
-                               "DIIS(6), MP2 start" % (No, Nv, NG, s),
+                               "DIIS(6), MP2 start%s" % (No, Nv, NG, s,
+                                                         ", factor-based ladder NR=%d" % args.factors
+                                                         if args.factors > 0 else ""),
                    "No": No, "Nv": Nv, "NG": NG, "slice": s, "mixer": "DiisMixer(6)"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
