@@ -60,6 +60,13 @@ extern "C" {
 #define SISI_HHPP 4 /* V_ijab  [No,No,Nv,Nv] */
 #define SISI_PHHP 5 /* V_aijb  [Nv,No,No,Nv] (complex mode) */
 #define SISI_PHHH 6 /* V_aijk  [Nv,No,No,No] (complex mode) */
+#define SISI_PPPP 7 /* V_abcd  [Nv,Nv,Nv,Nv] (full-integral formulation) */
+#define SISI_PPPH 8 /* V_abci  [Nv,Nv,Nv,No] (full-integral formulation) */
+
+/* Residual formulations. */
+#define SISI_FORMULATION_VERTEX 0    /* CcsdEnergyFromCoulombIntegrals       */
+#define SISI_FORMULATION_INTEGRALS 1 /* CcsdEnergyFromCoulombIntegralsReference */
+#define SISI_FORMULATION_ONLY_PPL 2  /* ... its onlyPPL branch                */
 
 typedef struct sisi_engine sisi_engine;
 
@@ -159,7 +166,9 @@ int sisi_engine_set_vertex_complex(sisi_engine *e, const double *gamma);
 int sisi_integrals_from_vertex(sisi_engine *e);
 
 /* Copy one integral class (SISI_PPHH ...) to host memory (complex128 in
- * complex mode, where HHPP, PHHP and PHHH are the reference's own classes). */
+ * complex mode, where HHPP, PHHP and PHHH are the reference's own classes).
+ * PPPP and PPPH (CoulombIntegralsFromVertex.cxx:423, 430) are built from the
+ * real vertex on request. */
 int sisi_get_integrals(sisi_engine *e, int which, double *out);
 
 /* Precision used by the dense contractions of the next residual call. */
@@ -176,6 +185,20 @@ int sisi_engine_set_precision(sisi_engine *e, int precision);
  * refinement residuals. */
 int sisi_engine_set_coulomb_factors(sisi_engine *e, int NR, int factors_slice_size,
                                     const double *lambda, const double *pi);
+
+/* Upload one real integral class (SISI_PPHH, _PHPH, _HHHH, _HHHP, _PPPP,
+ * _PPPH) instead of building it from a vertex: the inputs of
+ * CcsdEnergyFromCoulombIntegralsReference, which reads integrals only.
+ * With PPHH, PHPH, HHHH and HHHP given the engine counts as having its
+ * integrals (no vertex needed for the full-integral formulation). */
+int sisi_engine_set_integrals(sisi_engine *e, int which, const double *V);
+
+/* Residual formulation of the next residual calls and solves:
+ * SISI_FORMULATION_VERTEX (default), SISI_FORMULATION_INTEGRALS
+ * (CcsdEnergyFromCoulombIntegralsReference::getResiduum, cxx:29-295, stored
+ * V_abcd and V_abci) or SISI_FORMULATION_ONLY_PPL (its onlyPPL branch,
+ * cxx:56-75: R2 = V_abcd (T2 + T1 T1), R1 = 0, every iteration). */
+int sisi_engine_set_formulation(sisi_engine *e, int formulation);
 
 /* Residual variant of the next residual calls: the ppl and distinguishable
  * arguments of CcsdEnergyFromCoulombIntegrals (cxx:123-124, 324-338);

@@ -4,7 +4,8 @@ These pin the restatement in ``ccsd.py`` / ``triples.py``: a mistake there (a
 dropped term, a wrong sign, a transposed operand) makes them disagree.
 
 * ``residual_full_integrals`` -- CcsdEnergyFromCoulombIntegralsReference.cxx:29-295
-  (stored V_abcd / V_abci, no vertex, no slicing).
+  (stored V_abcd / V_abci, no vertex, no slicing); ``residual_only_ppl`` its
+  onlyPPL branch (cxx:56-73).
 * ``triples_unsliced``       -- PerturbativeTriples.cxx:19-97 (runInMemory:
   6-index T_abcijk, Z_ai / Z_abij intermediates).
 * ``mp2_closed_form``        -- ParticleHoleCoulombVertexDecomposition.cxx:279-323,
@@ -68,6 +69,13 @@ def residual_full_integrals(it, T1, T2, gamma, No, initial_doubles=False):
     R1 += -2.0 * ein("klic,ackl->ai", Vijka, T2) + ein("lkic,ackl->ai", Vijka, T2)
     R1 += -2.0 * ein("klic,ak,cl->ai", Vijka, T1, T1) + ein("lkic,ak,cl->ai", Vijka, T1, T1)
     return R1, R2
+
+
+def residual_only_ppl(T1, T2, gamma, No):
+    """onlyPPL branch of CcsdEnergyFromCoulombIntegralsReference::getResiduum
+    (cxx:56-73): R2 = V_abcd (T2 + T1 T1), R1 = 0, at every iteration."""
+    Vabcd, _ = vabcd_from_vertex(gamma, No)
+    return np.zeros_like(T1), ein("abcd,cdij->abij", Vabcd, T2 + ein("ai,bj->abij", T1, T1))
 
 
 def triples_unsliced(T1, T2, gamma, eps_i, eps_a):

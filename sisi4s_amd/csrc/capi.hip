@@ -171,6 +171,14 @@ int sisi_get_integrals(sisi_engine *h, int which, double *out) {
       case SISI_PHPH: e.download(out, e.Vaibj, v * o * v * o); break;
       case SISI_HHHH: e.download(out, e.Vijkl, o * o * o * o); break;
       case SISI_HHHP: e.download(out, e.Vijka, o * o * o * v); break;
+      case SISI_PPPP:
+        if (!e.Vabcd.get()) build_vabcd(e);
+        e.download(out, e.Vabcd, v * v * v * v);
+        break;
+      case SISI_PPPH:
+        if (!e.Vabci.get()) build_vabcd(e);
+        e.download(out, e.Vabci, v * v * v * o);
+        break;
       case SISI_HHPP: {
         DBuf t(&e.pool, o * o * v * v);
         sum(e, 1.0, dense(e.Vabij.get(), {int64_t(v), int64_t(v), int64_t(o), int64_t(o)}), "abij",
@@ -196,6 +204,25 @@ int sisi_engine_set_coulomb_factors(sisi_engine *h, int NR, int factors_slice_si
     ENGINE_REQUIRED(h);
     SISI_REQUIRE(NR >= 0, SISI_ERR_INVALID, "negative NR");
     set_coulomb_factors(h->e, NR, factors_slice_size, lambda, pi);
+  });
+}
+
+int sisi_engine_set_integrals(sisi_engine *h, int which, const double *V) {
+  return guarded([&] {
+    ENGINE_REQUIRED(h);
+    set_integrals(h->e, which, V);
+  });
+}
+
+int sisi_engine_set_formulation(sisi_engine *h, int formulation) {
+  return guarded([&] {
+    ENGINE_REQUIRED(h);
+    SISI_REQUIRE(formulation == SISI_FORMULATION_VERTEX || formulation == SISI_FORMULATION_INTEGRALS ||
+                     formulation == SISI_FORMULATION_ONLY_PPL,
+                 SISI_ERR_INVALID, "unknown formulation");
+    SISI_REQUIRE(formulation == SISI_FORMULATION_VERTEX || !h->e.cplx, SISI_ERR_INVALID,
+                 "the full-integral formulation is built for the real path");
+    h->e.formulation = formulation;
   });
 }
 

@@ -66,6 +66,8 @@ void integrals_from_vertex(Engine &e) {
 void ccsd_residual(Engine &e, int it, bool initial_doubles, int slice_size, const double *T1c,
                    const double *T2c, double *R1p, double *R2p) {
   if (e.cplx) return zccsd_residual(e, it, initial_doubles, slice_size, T1c, T2c, R1p, R2p);
+  if (e.formulation != SISI_FORMULATION_VERTEX)
+    return ccsd_residual_full(e, it, initial_doubles, T1c, T2c, R1p, R2p);
   SISI_REQUIRE(e.have_integrals, SISI_ERR_STATE, "residual: integrals not built");
   const int64_t o = e.No, v = e.Nv, G = e.NG;
   const int64_t n2 = v * v * o * o;
@@ -84,6 +86,9 @@ void ccsd_residual(Engine &e, int it, bool initial_doubles, int slice_size, cons
     return;
   }
   zero(e, R2p, n2);
+  SISI_REQUIRE(e.Gij.get() && e.Gai.get() && e.Gab.get() && e.Vaibj.get() && e.Vijkl.get() &&
+                   e.Vijka.get(),
+               SISI_ERR_STATE, "the vertex residual needs the vertex and its integrals");
   const int prec = e.precision;
   e.thaw();
   e.frozen = {T1p, T2p, e.Vabij.get(), e.Uabij.get(), e.Vaibj.get(), e.Vijka.get(),
@@ -300,6 +305,8 @@ void ccsd_residual_delta(Engine &e, int slice_size, const double *T1c, const dou
                          double *R1p, double *R2p) {
   SISI_REQUIRE(!e.cplx, SISI_ERR_INVALID, "refinement residuals are not built for complex amplitudes");
   SISI_REQUIRE(e.NR == 0, SISI_ERR_INVALID, "refinement residuals are not built with CoulombFactors");
+  SISI_REQUIRE(e.formulation == SISI_FORMULATION_VERTEX, SISI_ERR_INVALID,
+               "refinement residuals are built for the vertex formulation only");
   Anchor &an = e.anchor;
   SISI_REQUIRE(an.valid, SISI_ERR_STATE, "refinement residual without an anchor");
   SISI_REQUIRE(an.ppl == e.ppl && an.distinguishable == e.distinguishable, SISI_ERR_STATE,
@@ -621,6 +628,7 @@ void ccsd_energy(Engine &e, const double *T1, const double *T2, double &E, doubl
 void ccsd_pair_energies(Engine &e, const double *T1, const double *T2, double *pair) {
   SISI_REQUIRE(e.have_integrals, SISI_ERR_STATE, "pair energies: integrals not built");
   if (e.cplx) return zccsd_pair_energies(e, T1, T2, pair);
+  SISI_REQUIRE(e.Uabij.get(), SISI_ERR_STATE, "pair energies: integrals not built");
   const int64_t o = e.No, v = e.Nv;
   DBuf Xb(&e.pool, v * v * o * o);
   add_t1t1(e, Xb.get(), T2, T1, 1.0, int(v), int(o));  // tau = T2 + T1 T1
@@ -681,6 +689,9 @@ void ccsd_begin(Engine &e, const sisi_ccsd_opts &opts, const double *T1h, const 
                SISI_ERR_INVALID, "refinement residuals are not built for complex amplitudes");
   SISI_REQUIRE(e.NR == 0 || opts.precision == SISI_FP64 || opts.refine_interval == 0,
                SISI_ERR_INVALID, "refinement residuals are not built with CoulombFactors");
+  SISI_REQUIRE(e.formulation == SISI_FORMULATION_VERTEX || opts.precision == SISI_FP64 ||
+                   opts.refine_interval == 0,
+               SISI_ERR_INVALID, "refinement residuals are built for the vertex formulation only");
   SISI_REQUIRE(opts.refine_precision == SISI_FP32 || opts.refine_precision == SISI_BF16X3 ||
                    opts.refine_precision == SISI_BF16,
                SISI_ERR_INVALID, "refine_precision must be SISI_FP32, SISI_BF16X3 or SISI_BF16");

@@ -50,6 +50,14 @@ void set_coulomb_factors(Engine &e, int NR, int slice, const double *lambda, con
 // R2 += the factor-based ladder of X = T2 + T1 T1 (real amplitudes).
 void factor_ladder(Engine &e, const double *T1, const double *X, double *R2, int precision);
 
+// Full-integral formulation (CcsdEnergyFromCoulombIntegralsReference.cxx:
+// 29-295): integral classes given directly, PPPP/PPPH built from the vertex
+// on request, and the residual from stored integrals.
+void set_integrals(Engine &e, int which, const double *V);
+void build_vabcd(Engine &e);
+void ccsd_residual_full(Engine &e, int iteration, bool initial_doubles, const double *T1,
+                        const double *T2, double *R1, double *R2);
+
 // Driver (ClusterSinglesDoublesAlgorithm.cxx:38-128).
 void ccsd_begin(Engine &e, const sisi_ccsd_opts &opts, const double *T1h, const double *T2h);
 void ccsd_iterate(Engine &e, int n);

@@ -143,6 +143,13 @@ class Engine {
   // Coulomb factors (CoulombFactors Lambda[G,R], FactorOrbitals Pi[p,R],
   // planar complex) and V_RS = sum_G conj(Lambda_GR) Lambda_GS; NR > 0 makes
   // the ladder the factor-based one (CcsdEnergyFromCoulombIntegrals.cxx:340-372)
+  // Full-integral formulation (CcsdEnergyFromCoulombIntegralsReference):
+  // PPPP V_abcd and PPPH V_abci, and which classes were given directly
+  // (sisi_engine_set_integrals, no vertex needed); formulation 0: vertex
+  // residual, 1: full-integral residual, 2: its onlyPPL branch
+  DBuf Vabcd, Vabci;
+  unsigned given_integrals = 0;
+  int formulation = 0;
   int NR = 0, factors_slice = 0;
   bool factors_real = true;  // imaginary parts all zero: real products only
   DBuf Lam, PiF, VRS;

@@ -484,6 +484,8 @@ double triples_energy(Engine &e, const double *T1, const double *T2, int prec,
                "triples: unknown precision");
   e.thaw();
   SISI_REQUIRE(e.Nv <= 8 * 1023, SISI_ERR_INVALID, "triples: Nv above 8184");
+  SISI_REQUIRE(e.Gab.get() && e.Gai.get() && e.Vabij.get() && (e.cplx ? e.Vaijk.get() : e.Vijka.get()),
+               SISI_ERR_STATE, "triples: needs the vertex and its integrals");
   if (e.cplx) return ztriples_energy(e, T1, T2, prec, blocks_in);
   const int64_t o = e.No, v = e.Nv, G = e.NG;
   std::vector<std::array<int, 3>> blocks = checked_blocks(e, blocks_in);

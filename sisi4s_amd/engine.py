@@ -26,7 +26,8 @@ SISI_ERR_ALLOC = 5
 FP64, FP32, BF16X3, BF16 = 0, 1, 2, 3
 PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16x3": BF16X3, "bf16": BF16}
 MIXER_LINEAR, MIXER_DIIS = 0, 1
-PPHH, PHPH, HHHH, HHHP, HHPP, PHHP, PHHH = 0, 1, 2, 3, 4, 5, 6
+PPHH, PHPH, HHHH, HHHP, HHPP, PHHP, PHHH, PPPP, PPPH = 0, 1, 2, 3, 4, 5, 6, 7, 8
+FORMULATION_VERTEX, FORMULATION_INTEGRALS, FORMULATION_ONLY_PPL = 0, 1, 2
 
 _c_double_p = ctypes.POINTER(ctypes.c_double)
 _c_int_p = ctypes.POINTER(ctypes.c_int)
@@ -88,6 +89,8 @@ SIGNATURES = [
     ("sisi_get_integrals", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _c_double_p]),
     ("sisi_engine_set_precision", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("sisi_engine_set_residual_variant", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    ("sisi_engine_set_integrals", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _c_double_p]),
+    ("sisi_engine_set_formulation", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("sisi_engine_set_coulomb_factors", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _c_double_p, _c_double_p]),
     ("sisi_ccsd_residual", ctypes.c_int,
@@ -336,7 +339,7 @@ class Engine:
         o, v = self.No, self.Nv
         shapes = {PPHH: (v, v, o, o), PHPH: (v, o, v, o), HHHH: (o, o, o, o),
                   HHHP: (o, o, o, v), HHPP: (o, o, v, v), PHHP: (v, o, o, v),
-                  PHHH: (v, o, o, o)}
+                  PHHH: (v, o, o, o), PPPP: (v, v, v, v), PPPH: (v, v, v, o)}
         shape = shapes[which]
         b = self._out(shape)
         _check(self._lib.sisi_get_integrals(self._h, which, _ptr(b)))
@@ -360,6 +363,15 @@ class Engine:
         assert L.size == self.NG * NR and P.size == self.Np * NR
         _check(self._lib.sisi_engine_set_coulomb_factors(
             self._h, NR, int(slice_size), _ptr(L.view(np.float64)), _ptr(P.view(np.float64))))
+
+    def set_integrals(self, which, V):
+        """Upload one real integral class (no vertex needed for the
+        full-integral formulation)."""
+        b = _buf(V)
+        _check(self._lib.sisi_engine_set_integrals(self._h, int(which), _ptr(b)))
+
+    def set_formulation(self, formulation):
+        _check(self._lib.sisi_engine_set_formulation(self._h, int(formulation)))
 
     def set_residual_variant(self, ppl=True, distinguishable=False):
         """ppl / distinguishable of the next residual calls

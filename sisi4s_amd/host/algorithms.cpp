@@ -146,11 +146,15 @@ class CoulombIntegralsFromVertex : public Algorithm {
                    {"HHHPCoulombIntegrals", SISI_HHHP, {o, o, o, v}},
                    {"HHPPCoulombIntegrals", SISI_HHPP, {o, o, v, v}},
                    {"PHHPCoulombIntegrals", SISI_PHHP, {v, o, o, v}},
-                   {"PHHHCoulombIntegrals", SISI_PHHH, {v, o, o, o}}};
+                   {"PHHHCoulombIntegrals", SISI_PHHH, {v, o, o, o}},
+                   {"PPPPCoulombIntegrals", SISI_PPPP, {v, v, v, v}},
+                   {"PPPHCoulombIntegrals", SISI_PPPH, {v, v, v, o}}};
     for (auto &c : classes) {
       if (!out_.count(c.name)) continue;
       if (!cplx && (c.cls == SISI_PHHP || c.cls == SISI_PHHH))
         throw StepError(std::string(c.name) + " is built with complex: 1 only");
+      if (cplx && (c.cls == SISI_PPPP || c.cls == SISI_PPPH))
+        throw StepError(std::string(c.name) + " is built for real integrals only");
       auto t = make_tensor(c.lens, cplx);
       check(sisi_get_integrals(e, c.cls, t->values.data()), "sisi_get_integrals");
       allocatedTensorArgument(c.name, t);
@@ -175,11 +179,19 @@ class CcsdEnergyFromCoulombIntegrals : public Algorithm {
   using Algorithm::Algorithm;
   std::string getName() override { return "CcsdEnergyFromCoulombIntegrals"; }
   void run() override {
+    const bool cplx = complex_integrals(*this);
+    sisi_engine *e = engine_of(*this, cplx);
+    check(sisi_engine_set_formulation(e, SISI_FORMULATION_VERTEX), "sisi_engine_set_formulation");
+    solve(e, cplx, true);
+  }
+
+ protected:
+  // ClusterSinglesDoublesAlgorithm::run on engine e; vertex_arguments: the
+  // CoulombFactors / ppl / distinguishable arguments of the vertex residual
+  void solve(sisi_engine *e, bool cplx, bool vertex_arguments) {
     if (getIntegerArgument("unrestricted", 0) != 0 || getIntegerArgument("antisymmetrize", 0) != 0)
       throw StepError("unrestricted / antisymmetrized energies are not supported");
     if (isArgumentGiven("HPFockMatrix")) throw StepError("HPFockMatrix is not supported");
-    const bool cplx = complex_integrals(*this);
-    sisi_engine *e = engine_of(*this, cplx);
     const int64_t o = getTensorArgument("HoleEigenEnergies").lens[0];
     const int64_t v = getTensorArgument("ParticleEigenEnergies").lens[0];
     sisi_ccsd_opts opts;
@@ -201,8 +213,10 @@ class CcsdEnergyFromCoulombIntegrals : public Algorithm {
     opts.precision = precision_argument(*this);
     opts.switch_threshold = getRealArgument("switchThreshold", 1e-5);
     opts.refine_interval = int(getIntegerArgument("refineInterval", 0));
-    opts.ppl = getIntegerArgument("ppl", 1) != 0;                          // cxx:324
-    opts.distinguishable = getIntegerArgument("distinguishable", 0) != 0;  // cxx:123-124
+    if (vertex_arguments) {
+      opts.ppl = getIntegerArgument("ppl", 1) != 0;                          // cxx:324
+      opts.distinguishable = getIntegerArgument("distinguishable", 0) != 0;  // cxx:123-124
+    }
     {
       const std::string rp = getTextArgument("refinePrecision", "bf16");
       if (rp == "bf16" || rp == "3") opts.refine_precision = SISI_BF16;
@@ -210,7 +224,7 @@ class CcsdEnergyFromCoulombIntegrals : public Algorithm {
       else if (rp == "fp32" || rp == "1") opts.refine_precision = SISI_FP32;
       else throw StepError("unknown refinePrecision " + rp + " (bf16, bf16x3, fp32)");
     }
-    if (isArgumentGiven("CoulombFactors")) {
+    if (vertex_arguments && isArgumentGiven("CoulombFactors")) {
       // factor-based ladder (cxx:340-372): CoulombFactors [NG, NR] and
       // FactorOrbitals [Np, NR], complex or real
       if (cplx) throw StepError("CoulombFactors: the factor-based ladder is built for real amplitudes");
@@ -281,6 +295,56 @@ class CcsdEnergyFromCoulombIntegrals : public Algorithm {
   }
 };
 
+// CcsdEnergyFromCoulombIntegralsReference (CcsdEnergyFromCoulombIntegralsReference.cxx:
+// 29-295): the same driver on the full-integral residual from the stored
+// PPHH, PHPH, HHHH, HHHP, PPPP and PPPH integrals (no vertex); onlyPPL = 1
+// keeps only the particle-particle ladder (cxx:56-73).
+class CcsdEnergyFromCoulombIntegralsReference : public CcsdEnergyFromCoulombIntegrals {
+ public:
+  using CcsdEnergyFromCoulombIntegrals::CcsdEnergyFromCoulombIntegrals;
+  std::string getName() override { return "CcsdEnergyFromCoulombIntegralsReference"; }
+  void run() override {
+    const auto &ei = getTensorArgument("HoleEigenEnergies");
+    const auto &ea = getTensorArgument("ParticleEigenEnergies");
+    if (complex_integrals(*this))
+      throw StepError("CcsdEnergyFromCoulombIntegralsReference is built for real integrals");
+    const int64_t o = ei.lens[0], v = ea.lens[0];
+    sisi_dims d{int(o), int(v), 1};
+    struct Owned {
+      sisi_engine *e = nullptr;
+      ~Owned() { if (e) sisi_engine_destroy(e); }
+    } owned;
+    check(sisi_engine_create(&d, &owned.e), "sisi_engine_create");
+    sisi_engine *e = owned.e;
+    check(sisi_engine_set_eigenenergies(e, ei.values.data(), ea.values.data()),
+          "sisi_engine_set_eigenenergies");
+    const struct {
+      const char *name;
+      int cls;
+      std::vector<int64_t> lens;
+    } classes[] = {{"PPHHCoulombIntegrals", SISI_PPHH, {v, v, o, o}},
+                   {"PHPHCoulombIntegrals", SISI_PHPH, {v, o, v, o}},
+                   {"HHHHCoulombIntegrals", SISI_HHHH, {o, o, o, o}},
+                   {"HHHPCoulombIntegrals", SISI_HHHP, {o, o, o, v}},
+                   {"PPPPCoulombIntegrals", SISI_PPPP, {v, v, v, v}},
+                   {"PPPHCoulombIntegrals", SISI_PPPH, {v, v, v, o}}};
+    const bool only_ppl = getIntegerArgument("onlyPPL", 0) == 1;
+    for (auto &c : classes) {
+      if (!isArgumentGiven(c.name)) {
+        if (only_ppl && c.cls != SISI_PPPP) continue;
+        throw StepError(std::string(c.name) + " is needed");
+      }
+      const auto &t = getTensorArgument(c.name);
+      if (t.complex || t.lens != c.lens) throw StepError(std::string(c.name) + " has the wrong shape");
+      check(sisi_engine_set_integrals(e, c.cls, t.values.data()), "sisi_engine_set_integrals");
+    }
+    check(sisi_engine_set_formulation(
+              e, only_ppl ? SISI_FORMULATION_ONLY_PPL : SISI_FORMULATION_INTEGRALS),
+          "sisi_engine_set_formulation");
+    solve(e, false, false);
+  }
+};
+
 // CcsdPerturbativeTriples (CcsdPerturbativeTriples.cxx:119-248): output is
 // CCSD energy + (T).
 class CcsdPerturbativeTriples : public Algorithm {
@@ -341,6 +405,7 @@ class CcsdPerturbativeTriplesComplex : public Algorithm {
 
 static Registrar<GenerateSyntheticSystem> r0("GenerateSyntheticSystem");
 static Registrar<CcsdPerturbativeTriplesComplex> r5("CcsdPerturbativeTriplesComplex");
+static Registrar<CcsdEnergyFromCoulombIntegralsReference> r6("CcsdEnergyFromCoulombIntegralsReference");
 static Registrar<CoulombIntegralsFromVertex> r1("CoulombIntegralsFromVertex");
 static Registrar<UegVertexGenerator> r4("UegVertexGenerator");
 static Registrar<CcsdEnergyFromCoulombIntegrals> r2("CcsdEnergyFromCoulombIntegrals");

@@ -457,3 +457,44 @@ def test_factor_ladder_matches_oracle_and_vertex_ladder(complex_factors):
     e.set_coulomb_factors()
     R1, R2 = e.residual(1, T1, T2, slice_size=5)
     assert np.array_equal(R2, V2)
+
+
+def test_full_integral_formulation_matches_oracle_and_vertex_path():
+    """CcsdEnergyFromCoulombIntegralsReference (cxx:29-295) on an engine that
+    only holds uploaded integrals: PPPP/PPPH built from the vertex match the
+    oracle, the full-integral residual and its onlyPPL branch match the
+    oracle's restatements, and a whole run gives the vertex formulation's
+    energies."""
+    from sisi4s_amd import engine
+    from oracle import formulations
+    No, Nv, NG = 4, 11, 30
+    ev, g, ei, ea = make_engine(41, No, Nv, NG)
+    Vabcd, Vabci = ccsd.vabcd_from_vertex(g, No)
+    announce("full-integral formulation No=4 Nv=11")
+    assert np.abs(ev.get_integrals(engine.PPPP) - Vabcd).max() <= 1e-13 * np.abs(Vabcd).max()
+    assert np.abs(ev.get_integrals(engine.PPPH) - Vabci).max() <= 1e-13 * np.abs(Vabci).max()
+    ints = ccsd.integrals_from_vertex(g, No)
+    ef = engine.Engine(No, Nv, 1)
+    ef.set_eigenenergies(ei, ea)
+    for cls, V in [(engine.PPHH, ints["PPHH"]), (engine.PHPH, ints["PHPH"]),
+                   (engine.HHHH, ints["HHHH"]), (engine.HHHP, ints["HHHP"]),
+                   (engine.PPPP, Vabcd), (engine.PPPH, Vabci)]:
+        ef.set_integrals(cls, V)
+    ef.set_formulation(engine.FORMULATION_INTEGRALS)
+    T1, T2 = random_amplitudes(Nv, No, seed=12)
+    R1, R2 = ef.residual(1, T1, T2)
+    O1, O2 = formulations.residual_full_integrals(1, T1, T2, g, No)
+    scale = float(np.abs(O2).max())
+    assert float(np.abs(R2 - O2).max()) <= 1e-11 * scale
+    assert float(np.abs(R1 - O1).max()) <= 1e-11 * scale
+    ef.set_formulation(engine.FORMULATION_ONLY_PPL)
+    R1, R2 = ef.residual(0, T1, T2)
+    _, P2 = formulations.residual_only_ppl(T1, T2, g, No)
+    assert float(np.abs(R2 - P2).max()) <= 1e-11 * float(np.abs(P2).max()) and not R1.any()
+    ef.set_formulation(engine.FORMULATION_INTEGRALS)
+    kw = dict(mixer="DiisMixer", max_residua=6, max_iterations=120, amplitudes_convergence=1e-9,
+              energy_convergence=1e-11)
+    rf = ef.ccsd_solve(engine.default_opts(**kw))
+    rv = ev.ccsd_solve(engine.default_opts(**kw))
+    assert rf["converged"] and rv["converged"]
+    assert np.abs(ef.energies() - ev.energies()).max() <= 1e-10

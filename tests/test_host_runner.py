@@ -141,3 +141,34 @@ def test_runner_residual_variants(tmp_path, args, ppl, dist):
                      slice_size=6, ppl=ppl, distinguishable=dist)
     assert m and abs(float(m.group(1)) - ref["energy"]) <= 1e-10
     assert ("distinguishable: 1" in (tmp_path / "out.yaml").read_text()) == dist
+
+
+@pytest.mark.gpu
+def test_runner_full_integral_formulation(tmp_path):
+    """CcsdEnergyFromCoulombIntegralsReference fed by CoulombIntegralsFromVertex
+    (PPPP/PPPH included) gives CcsdEnergyFromCoulombIntegrals' energy."""
+    p = tmp_path / "in.yaml"
+    p.write_text("""
+- name: GenerateSyntheticSystem
+  in: {seed: 3, No: 4, Nv: 14, NG: 40}
+  out: {CoulombVertex: $G, HoleEigenEnergies: $I, ParticleEigenEnergies: $A}
+- name: CoulombIntegralsFromVertex
+  in: {CoulombVertex: $G, HoleEigenEnergies: $I, ParticleEigenEnergies: $A}
+  out: {PPHHCoulombIntegrals: $PPHH, PHPHCoulombIntegrals: $PHPH, HHHHCoulombIntegrals: $HHHH,
+        HHHPCoulombIntegrals: $HHHP, PPPPCoulombIntegrals: $PPPP, PPPHCoulombIntegrals: $PPPH}
+- name: CcsdEnergyFromCoulombIntegralsReference
+  in: {mixer: DiisMixer, maxResidua: 6, maxIterations: 20, HoleEigenEnergies: $I,
+       ParticleEigenEnergies: $A, PPHHCoulombIntegrals: $PPHH, PHPHCoulombIntegrals: $PHPH,
+       HHHHCoulombIntegrals: $HHHH, HHHPCoulombIntegrals: $HHHP, PPPPCoulombIntegrals: $PPPP,
+       PPPHCoulombIntegrals: $PPPH}
+  out: {CcsdEnergy: $Eref}
+- name: CcsdEnergyFromCoulombIntegrals
+  in: {mixer: DiisMixer, maxResidua: 6, maxIterations: 20, HoleEigenEnergies: $I,
+       ParticleEigenEnergies: $A, PPHHCoulombIntegrals: $PPHH, CoulombVertex: $G}
+  out: {CcsdEnergy: $E}
+""")
+    announce("runner full-integral formulation")
+    r = run(["-i", str(p), "-o", str(tmp_path / "out.yaml")])
+    assert r.returncode == 0, r.stdout + r.stderr
+    e = [float(m) for m in re.findall(r"CcsdEnergy = (\S+)", r.stdout)]
+    assert len(e) == 2 and abs(e[0] - e[1]) <= 1e-10

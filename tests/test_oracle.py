@@ -403,3 +403,19 @@ def test_factor_ladder_equals_vertex_ladder(fs):
     out = np.zeros_like(X)
     ccsd.factor_ladder(T1, X, Lz, Pz, fs, out)
     assert np.abs(out - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+def test_only_ppl_is_the_ladder_of_the_vertex_path_without_singles():
+    """onlyPPL (CcsdEnergyFromCoulombIntegralsReference.cxx:56-73) with T1 = 0
+    is the particle-particle ladder the vertex path adds (its residual with
+    ppl = 1 minus with ppl = 0), independent of the slice size."""
+    No, Nv = 3, 8
+    g, ei, ea = philox.system(9, No, Nv, 15)
+    ints = ccsd.integrals_from_vertex(g, No)
+    _, T2 = random_amplitudes(Nv, No, seed=6)
+    T1 = np.zeros((Nv, No))
+    _, P2 = formulations.residual_only_ppl(T1, T2, g, No)
+    for s in (2, 8):
+        _, A = ccsd.residual(1, T1, T2, g, ints, s)
+        _, B = ccsd.residual(1, T1, T2, g, ints, s, ppl=False)
+        assert np.abs((A - B) - P2).max() <= 1e-13 * np.abs(P2).max()
