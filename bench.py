@@ -96,6 +96,8 @@ def main():
                     help="mixed runs: precision of the refinement residuals between anchors")
     ap.add_argument("--triples-precision", default="bf16x3", choices=["fp32", "bf16x3"],
                     help="(T) precision of the mixed run")
+    ap.add_argument("--coarse-threshold", type=float, default=0.0,
+                    help="mixed runs: plain-bf16 coarse stage until |dT|/|T| reaches this value")
     ap.add_argument("--refine-interval", type=int, default=16,
                     help="mixed runs: FP64 anchor every k iterations after the switch (0: plain "
                          "FP64 iterations)")
@@ -395,7 +397,8 @@ def converge(args):
                              precision=prec, switch_threshold=args.switch_threshold,
                              level_shift=args.level_shift,
                              refine_interval=args.refine_interval if prec != "fp64" else 0,
-                             refine_precision=args.refine_precision)
+                             refine_precision=args.refine_precision,
+                             coarse_threshold=args.coarse_threshold if prec != "fp64" else 0.0)
         t0 = time.perf_counter()
         e.ccsd_begin(o)
         rep = None

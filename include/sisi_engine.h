@@ -107,6 +107,12 @@ typedef struct {
                                     0; nonzero selects the distinguishable-
                                     cluster terms (cxx:150-187, 244-246,
                                     283-285, 316-320)                     */
+  double coarse_threshold;       /* mixed runs: a coarse stage in
+                                    coarse_precision runs before the bulk
+                                    precision until |dT|/|T| <= this value
+                                    (default 0: no coarse stage)          */
+  int coarse_precision;          /* SISI_BF16 (default), SISI_FP32 or
+                                    SISI_BF16X3                            */
 } sisi_ccsd_opts;
 
 typedef struct {

@@ -81,6 +81,8 @@ void sisi_ccsd_opts_default(sisi_ccsd_opts *o) {
   o->refine_precision = SISI_BF16;
   o->ppl = 1;
   o->distinguishable = 0;
+  o->coarse_threshold = 0.0;
+  o->coarse_precision = SISI_BF16;
 }
 
 int sisi_engine_create(const sisi_dims *dims, sisi_engine **out) {

@@ -726,6 +726,9 @@ void ccsd_begin(Engine &e, const sisi_ccsd_opts &opts, const double *T1h, const 
     for (int k = 1; k <= N; ++k) e.B[0][k] = e.B[k][0] = -1.0;  // DiisMixer.cxx:83-96
   }
   e.cur_precision = opts.precision;
+  SISI_REQUIRE(opts.coarse_threshold <= 0.0 || is_precision(opts.coarse_precision), SISI_ERR_INVALID,
+               "coarse_precision must be a precision");
+  e.coarse_active = opts.precision != SISI_FP64 && opts.coarse_threshold > 0.0;
   e.run_active = true;
 }
 
@@ -733,11 +736,12 @@ void ccsd_iterate(Engine &e, int n) {
   SISI_REQUIRE(e.run_active, SISI_ERR_STATE, "ccsd: no run (call begin first)");
   const sisi_ccsd_opts &o = e.opts;
   for (int step = 0; step < n && !e.converged && e.iteration < o.max_iterations; ++step) {
-    // precision schedule: bulk precision until |dT|/|T| <= switch_threshold,
-    // FP64 afterwards (and FP64 throughout for the plain path)
+    // precision schedule: coarse precision until |dT|/|T| <= coarse_threshold
+    // (if set), bulk precision until |dT|/|T| <= switch_threshold, FP64
+    // afterwards (and FP64 throughout for the plain path)
     const bool bulk = o.precision != SISI_FP64 && e.cur_precision != SISI_FP64;
     const bool refine = !bulk && o.precision != SISI_FP64 && o.refine_interval > 0;
-    int prec = bulk ? o.precision : SISI_FP64;
+    int prec = bulk ? (e.coarse_active ? o.coarse_precision : o.precision) : SISI_FP64;
     DBuf R1(&e.pool, e.n1()), R2(&e.pool, e.n2());
     if (refine) {
       // refinement phase: FP64 anchor every refine_interval iterations,
@@ -829,8 +833,12 @@ void ccsd_iterate(Engine &e, int n) {
     e.energies.push_back(E);
     if (prec == SISI_FP64) e.fp64_iterations++;
     e.iteration++;
-    // hand over to FP64 once the bulk iterations reached the threshold
-    if (prec != SISI_FP64 && e.last_delta <= o.switch_threshold && e.iteration > 1)
+    // hand over from the coarse stage, and to FP64 once the bulk iterations
+    // reached their threshold
+    if (e.coarse_active && e.last_delta <= o.coarse_threshold && e.iteration > 1)
+      e.coarse_active = false;
+    if (prec != SISI_FP64 && !e.coarse_active && e.last_delta <= o.switch_threshold &&
+        e.iteration > 1)
       e.cur_precision = SISI_FP64;
     // convergence test (ClusterSinglesDoublesAlgorithm.cxx:103-106), declared on
     // FP64 residuals only; a refinement iteration that meets it asks for an
@@ -855,7 +863,7 @@ void fill_report(const Engine &e, sisi_ccsd_report *r) {
   r->iterations = e.iteration;
   r->converged = e.converged ? 1 : 0;
   r->fp64_iterations = e.fp64_iterations;
-  r->precision = e.cur_precision;
+  r->precision = e.coarse_active ? e.opts.coarse_precision : e.cur_precision;
   r->status = SISI_OK;
   r->energy_imag = e.last_energy_imag;
 }

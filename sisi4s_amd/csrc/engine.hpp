@@ -182,6 +182,7 @@ class Engine {
   int iteration = 0;
   int fp64_iterations = 0;
   int cur_precision = SISI_FP64;
+  bool coarse_active = false;  // mixed run still in its coarse stage
   double prev_energy = 0.0, last_energy = 0.0, last_direct = 0.0,
          last_exchange = 0.0, last_delta = 0.0;
   double prev_energy_imag = 0.0, last_energy_imag = 0.0;  // complex mode
@@ -211,6 +212,10 @@ class Engine {
   size_t n2() const { return n2e() * (cplx ? 2 : 1); }
   size_t n1e() const { return size_t(Nv) * No; }
   size_t n2e() const { return size_t(Nv) * Nv * No * No; }
+
+  // hipBLASLt state of the split-BF16x3 library backend (handle, workspace,
+  // algorithms per shape); declared last so it is released before the pool
+  std::shared_ptr<void> blaslt;
 };
 
 }  // namespace sisi

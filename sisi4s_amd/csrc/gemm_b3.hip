@@ -7,7 +7,10 @@
 // v_mfma_f32_16x16x32_bf16 (FP32 accumulation); the FP64 epilogue applies
 // alpha/beta and the split output addressing of the FP64 kernels.
 #include <cstdlib>
+#include <cstring>
+#include <tuple>
 
+#include "blaslt.hpp"
 #include "engine.hpp"
 #include "gemm.hpp"
 
@@ -29,6 +32,8 @@ struct SplitArgs {
   u16 *hi, *lo;
   int64_t bsrc = 0, bdst = 0;  // batch strides (grid z) of source and planes
   double scale = 1.0;          // applied before the split (exact for +-1)
+  int64_t ld = 0;              // row stride of hi/lo/hi2 in bf16 (0: Kp)
+  u16 *hi2 = nullptr;          // second copy of hi (tripled rows), or none
 };
 
 __device__ __forceinline__ u16 bf16_bits(float f) {
@@ -71,12 +76,18 @@ __global__ __launch_bounds__(256) void split_bf16_kernel(SplitArgs a) {
     __bf16 hb = *reinterpret_cast<__bf16 *>(&h[j]);
     l[j] = bf16_bits(x - float(hb));
   }
-  U128 *ph = reinterpret_cast<U128 *>(a.hi + int64_t(r) * a.Kp + k);
-  U128 *pl = reinterpret_cast<U128 *>(a.lo + int64_t(r) * a.Kp + k);
+  const int64_t row = int64_t(r) * (a.ld ? a.ld : a.Kp) + k;
+  U128 *ph = reinterpret_cast<U128 *>(a.hi + row);
+  U128 *pl = reinterpret_cast<U128 *>(a.lo + row);
   ph[0] = *reinterpret_cast<U128 *>(&h[0]);
   ph[1] = *reinterpret_cast<U128 *>(&h[8]);
   pl[0] = *reinterpret_cast<U128 *>(&l[0]);
   pl[1] = *reinterpret_cast<U128 *>(&l[8]);
+  if (a.hi2) {
+    U128 *p2 = reinterpret_cast<U128 *>(a.hi2 + row);
+    p2[0] = *reinterpret_cast<U128 *>(&h[0]);
+    p2[1] = *reinterpret_cast<U128 *>(&h[8]);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -403,55 +414,137 @@ __global__ __launch_bounds__(256) void gemm_b3p_seg_kernel(B3SegArgs p) {
 namespace {
 
 struct Planes {
-  DBuf store;  // hi and lo planes in one allocation (doubles as raw storage)
+  DBuf store;  // the planes in one allocation (doubles as raw storage)
   const u16 *hi = nullptr, *lo = nullptr;
+  const u16 *base = nullptr;  // start of row 0 (tripled layouts)
+  int64_t ld = 0;             // row stride in bf16
 };
+
+// Plane layouts: 0 = separate [Rp][Kp] hi and lo planes (hand-written
+// kernels); 1 = rows [hi | hi | lo] and 2 = rows [hi | lo | hi] of 3 Kp bf16,
+// so that sum_k' A'[m,k'] B'[n,k'] over k' < 3 Kp is hi*hi + hi*lo + lo*hi
+// with A in layout 1 and B in layout 2: one library GEMM with K tripled.
+void plane_pointers(u16 *base, int64_t Rp, int64_t Kp, int layout, u16 *&hi, u16 *&lo,
+                    u16 *&hi2, int64_t &ld) {
+  hi = base;
+  hi2 = nullptr;
+  if (layout == 0) {
+    lo = base + Rp * Kp;
+    ld = Kp;
+  } else {
+    ld = 3 * Kp;
+    lo = base + (layout == 1 ? 2 * Kp : Kp);
+    hi2 = base + (layout == 1 ? Kp : 2 * Kp);
+  }
+}
 
 // Split one (rows x K) operand view; frozen sources are split once per
 // residual evaluation and reused.
 const Planes &split_operand(Engine &e, const double *src, int64_t sr, int64_t sk, int R, int K,
-                            int Rp, int Kp, Planes &local) {
+                            int Rp, int Kp, Planes &local, int layout = 0) {
   std::string key;
   const bool cached = e.is_frozen(src);
+  const int64_t elems = int64_t(Rp) * Kp;  // bf16 per plane
+  const int nplanes = layout == 0 ? 2 : 3;
+  auto fill = [&](u16 *base) {
+    u16 *hi, *lo, *hi2;
+    int64_t ld;
+    plane_pointers(base, Rp, Kp, layout, hi, lo, hi2, ld);
+    local.base = base;
+    local.hi = hi;
+    local.lo = lo;
+    local.ld = ld;
+    return std::make_tuple(hi, lo, hi2, ld);
+  };
   if (cached) {
     key = "b3:" + std::to_string(reinterpret_cast<uintptr_t>(src)) + ":" + std::to_string(sr) +
           ":" + std::to_string(sk) + ":" + std::to_string(R) + ":" + std::to_string(K) + ":" +
-          std::to_string(Rp) + ":" + std::to_string(Kp);
+          std::to_string(Rp) + ":" + std::to_string(Kp) + ":" + std::to_string(layout);
     auto it = e.split_cache.find(key);
     if (it != e.split_cache.end()) {
-      local.hi = reinterpret_cast<const u16 *>(it->second.get());
-      local.lo = local.hi + int64_t(Rp) * Kp;
+      fill(reinterpret_cast<u16 *>(it->second.get()));
       return local;
     }
   }
-  const int64_t elems = int64_t(Rp) * Kp;        // bf16 per plane
-  const size_t doubles = size_t((2 * elems * 2 + 7) / 8);
+  const size_t doubles = size_t((nplanes * elems * 2 + 7) / 8);
   local.store = DBuf(&e.pool, doubles);
-  u16 *hi = reinterpret_cast<u16 *>(local.store.get());
-  u16 *lo = hi + elems;
+  auto [hi, lo, hi2, ld] = fill(reinterpret_cast<u16 *>(local.store.get()));
   SplitArgs a{src, sr, sk, R, K, Rp, Kp, hi, lo};
+  a.ld = ld;
+  a.hi2 = hi2;
   {
-    Timed t(*e.timers, Kind::Bandwidth, 8.0 * double(R) * K + 4.0 * double(elems));
+    Timed t(*e.timers, Kind::Bandwidth, 8.0 * double(R) * K + 2.0 * nplanes * double(elems));
     dim3 grid(unsigned((Kp + 63) / 64), unsigned((Rp + 63) / 64));
     hipLaunchKernelGGL(split_bf16_kernel, grid, dim3(256), 0, e.stream, a);
     SISI_HIP(hipGetLastError());
   }
-  local.hi = hi;
-  local.lo = lo;
   if (cached) {
     e.split_cache[key] = std::move(local.store);
-    local.hi = reinterpret_cast<const u16 *>(e.split_cache[key].get());
-    local.lo = local.hi + elems;
+    fill(reinterpret_cast<u16 *>(e.split_cache[key].get()));
   }
   return local;
 }
 
 }  // namespace
 
+// C[off(m, n)] = alpha * C32[m + M n] + beta * C[off(m, n)]: the FP64
+// epilogue of a library product (same output addressing as the kernels).
+__global__ void b3_epilogue_kernel(const float *C32, GemmArgs g) {
+  const int64_t n = int64_t(g.M) * g.N;
+  for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < n;
+       e += int64_t(gridDim.x) * blockDim.x) {
+    const int m = int(e % g.M), c = int(e / g.M);
+    const int64_t o = b3_off_m(g, m) + b3_off_n(g, c);
+    double v = g.alpha * double(C32[e]);
+    if (g.beta != 0.0) v += g.beta * g.C[o];
+    g.C[o] = v;
+  }
+}
+
+// Library backend of one split-BF16x3 (passes = 3: A in layout 1, B in
+// layout 2, one bf16 GEMM over 3 Kp) or plain bf16 (passes = 1: hi rows of
+// any layout) product: FP32 accumulation through hipBLASLt, then the FP64
+// epilogue.  False: no library algorithm for the shape.
+static bool b3_library(Engine &e, const GemmArgs &g, const Planes &A, const Planes &B, int Kp,
+                       int passes) {
+  DBuf c32(&e.pool, (size_t(g.M) * g.N + 1) / 2);
+  float *C = reinterpret_cast<float *>(c32.get());
+  Timed t(*e.timers, Kind::GemmB3, gemm_flops(g));
+  const int K = passes == 3 ? 3 * Kp : Kp;
+  if (!blaslt_bf16_gemm(e, g.M, g.N, K, A.hi, A.ld, B.hi, B.ld, C, 1.0f, 0.0f)) return false;
+  const int64_t n = int64_t(g.M) * g.N;
+  const int blocks = int(std::min<int64_t>((n + 255) / 256, 1 << 16));
+  hipLaunchKernelGGL(b3_epilogue_kernel, dim3(blocks), dim3(256), 0, e.stream, C, g);
+  SISI_HIP(hipGetLastError());
+  return true;
+}
+
+// Backend of the batch-1 split-BF16x3 and plain bf16 products: the
+// hand-written kernel (default; deterministic) or hipBLASLt on the same
+// split planes (SISI_B3_BACKEND=library: 1.3x the kernel's rate on the
+// ladder and ring shapes including the epilogue, run-to-run rounding not
+// reproducible).  Segmented and batched products always use the
+// hand-written kernels.
+static bool b3_use_library() {
+  static const bool lib = [] {
+    const char *v = getenv("SISI_B3_BACKEND");
+    return v != nullptr && std::strcmp(v, "library") == 0;
+  }();
+  return lib;
+}
+
 template <int BM, int BN, int WM, int WN, int NBUF, int MINB = 1, int PASSES = 3>
 static void run_b3(Engine &e, GemmArgs g) {
   const int Kp = ((g.K + 31) / 32) * 32;
   const int Mp = ((g.M + BM - 1) / BM) * BM, Np = ((g.N + BN - 1) / BN) * BN;
+  if (b3_use_library()) {
+    g.split_out = !(g.dm0 >= g.M && g.dn0 >= g.N);
+    Planes la, lb;
+    const int lay_a = PASSES == 3 ? 1 : 0, lay_b = PASSES == 3 ? 2 : 0;
+    const Planes &A = split_operand(e, g.A + 0, g.sam, g.sak, g.M, g.K, g.M, Kp, la, lay_a);
+    const Planes &B = split_operand(e, g.B + 0, g.sbn, g.sbk, g.N, g.K, g.N, Kp, lb, lay_b);
+    if (b3_library(e, g, A, B, Kp, PASSES)) return;
+  }
   Planes pa, pb;
   const Planes &A = split_operand(e, g.A + 0, g.sam, g.sak, g.M, g.K, Mp, Kp, pa);
   const Planes &B = split_operand(e, g.B + 0, g.sbn, g.sbk, g.N, g.K, Np, Kp, pb);

@@ -45,7 +45,8 @@ class CcsdOpts(ctypes.Structure):
                 ("slice_size", ctypes.c_int), ("precision", ctypes.c_int),
                 ("switch_threshold", ctypes.c_double), ("refine_interval", ctypes.c_int),
                 ("refine_precision", ctypes.c_int), ("ppl", ctypes.c_int),
-                ("distinguishable", ctypes.c_int)]
+                ("distinguishable", ctypes.c_int), ("coarse_threshold", ctypes.c_double),
+                ("coarse_precision", ctypes.c_int)]
 
 
 class CcsdReport(ctypes.Structure):
@@ -183,7 +184,7 @@ def default_opts(**kw):
     for k, v in kw.items():
         if k == "mixer" and isinstance(v, str):
             v = {"LinearMixer": MIXER_LINEAR, "DiisMixer": MIXER_DIIS}[v]
-        if k in ("precision", "refine_precision") and isinstance(v, str):
+        if k in ("precision", "refine_precision", "coarse_precision") and isinstance(v, str):
             v = PRECISIONS[v]
         setattr(o, k, v)
     return o

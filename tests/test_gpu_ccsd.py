@@ -384,6 +384,14 @@ def test_mixed_schedule_at_mid_size_matches_fp64():
     assert r64["converged"] and rm["converged"]
     assert rm["fp64_iterations"] <= rm["iterations"] // 4 + 2
     assert abs(rm["energy"] - r64["energy"]) <= 1e-6
+    # with a plain-bf16 coarse stage down to |dT|/|T| = 1e-3 before the bulk
+    rc = e.ccsd_solve(engine.default_opts(precision="bf16x3", switch_threshold=1e-5,
+                                          refine_interval=8, refine_precision="bf16",
+                                          coarse_threshold=1e-3, **kw))
+    print("mid-size with coarse stage: %d it (%d fp64), |dE| %.2e"
+          % (rc["iterations"], rc["fp64_iterations"], abs(rc["energy"] - r64["energy"])),
+          flush=True)
+    assert rc["converged"] and abs(rc["energy"] - r64["energy"]) <= 1e-6
 
 
 @pytest.mark.gpu
