@@ -320,6 +320,14 @@ int sisi_generate_system(uint64_t seed, int No, int Nv, int NG, double *gamma,
 int sisi_generate_ueg(int No, int Nv, double rs, int hartree_fock, double madelung, int *NG,
                       double *gamma, double *eps_i, double *eps_a);
 
+/* Backend of the split-BF16x3 and bf16 products of one operand pair (batch
+ * 1) for the whole process: 0 = the engine's own MFMA kernel (default), 1 =
+ * plain bf16 GEMMs of hipBLASLt on the engine's operand planes (K tripled,
+ * FP32 accumulation) followed by the engine's FP64 epilogue.  The
+ * environment variable SISI_B3_BACKEND=library sets 1 at start-up.  No
+ * counterpart in the reference. */
+int sisi_set_bf16_backend(int library);
+
 /* One MFMA GEMM C = alpha*op(A)*op(B) + beta*C on host arrays (column-major,
  * op = transpose when trans* != 0), in the given precision.  No counterpart
  * in the reference (CTF calls BLAS internally); exposed so the kernel family

@@ -209,6 +209,13 @@ int sisi_engine_set_coulomb_factors(sisi_engine *h, int NR, int factors_slice_si
   });
 }
 
+int sisi_set_bf16_backend(int library) {
+  return guarded([&] {
+    SISI_REQUIRE(library == 0 || library == 1, SISI_ERR_INVALID, "backend must be 0 or 1");
+    set_b3_backend(library);
+  });
+}
+
 int sisi_engine_set_integrals(sisi_engine *h, int which, const double *V) {
   return guarded([&] {
     ENGINE_REQUIRED(h);

@@ -106,6 +106,10 @@ void gemm(Engine &eng, int precision, GemmArgs g, bool exact = false);
 // passes = 1 keeps the hi*hi product only (plain bf16).
 void gemm_b3(Engine &eng, GemmArgs g, int passes = 3);
 
+// Backend of the batch-1 split-BF16x3 / bf16 products for the whole process:
+// 0 = hand-written kernel (default), 1 = hipBLASLt on the split planes.
+void set_b3_backend(int library);
+
 // Split-K reduction of g.work into C (deterministic order).
 void gemm_reduce(Engine &eng, const GemmArgs &g);
 

@@ -7,6 +7,7 @@
 // v_mfma_f32_16x16x32_bf16 (FP32 accumulation); the FP64 epilogue applies
 // alpha/beta and the split output addressing of the FP64 kernels.
 #include <cstdlib>
+#include <atomic>
 #include <cstring>
 #include <tuple>
 
@@ -525,13 +526,16 @@ static bool b3_library(Engine &e, const GemmArgs &g, const Planes &A, const Plan
 // ladder and ring shapes including the epilogue, run-to-run rounding not
 // reproducible).  Segmented and batched products always use the
 // hand-written kernels.
-static bool b3_use_library() {
-  static const bool lib = [] {
+static std::atomic<int> &b3_backend() {
+  static std::atomic<int> backend{[] {
     const char *v = getenv("SISI_B3_BACKEND");
-    return v != nullptr && std::strcmp(v, "library") == 0;
-  }();
-  return lib;
+    return v != nullptr && std::strcmp(v, "library") == 0 ? 1 : 0;
+  }()};
+  return backend;
 }
+static bool b3_use_library() { return b3_backend().load() == 1; }
+
+void set_b3_backend(int library) { b3_backend().store(library ? 1 : 0); }
 
 template <int BM, int BN, int WM, int WN, int NBUF, int MINB = 1, int PASSES = 3>
 static void run_b3(Engine &e, GemmArgs g) {

@@ -91,6 +91,7 @@ SIGNATURES = [
     ("sisi_engine_set_precision", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("sisi_engine_set_residual_variant", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     ("sisi_engine_set_integrals", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _c_double_p]),
+    ("sisi_set_bf16_backend", ctypes.c_int, [ctypes.c_int]),
     ("sisi_engine_set_formulation", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("sisi_engine_set_coulomb_factors", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _c_double_p, _c_double_p]),
@@ -176,6 +177,12 @@ def _ptr(b):
 
 def _unbuf(b, shape):
     return b.reshape(shape, order="F")
+
+
+def set_bf16_backend(library):
+    """Process-wide backend of the split-BF16x3 / bf16 products: False = the
+    engine's MFMA kernel, True = hipBLASLt on the engine's operand planes."""
+    _check(load().sisi_set_bf16_backend(int(bool(library))))
 
 
 def default_opts(**kw):

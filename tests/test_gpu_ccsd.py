@@ -290,6 +290,33 @@ def test_batched_reports_failure_of_one_system():
     assert "system 1" in str(ex.value)
 
 
+def test_library_backend_residuals():
+    """With the hipBLASLt backend for the split-BF16x3 / bf16 products (the
+    planner's permuted outputs go through its FP64 epilogue), the split-BF16x3
+    residual and a plain-bf16 refinement residual keep their error bounds."""
+    from sisi4s_amd import engine
+    No, Nv, NG = 10, 50, 300
+    e, g, ei, ea = make_engine(20261202, No, Nv, NG)
+    TA1, TA2 = random_amplitudes(Nv, No, seed=2, scale=0.02)
+    D1, D2 = random_amplitudes(Nv, No, seed=3, scale=2e-6)
+    P1, P2 = e.residual(1, TA1 + D1, TA2 + D2)
+    scale = float(np.abs(P2).max())
+    engine.set_bf16_backend(True)
+    try:
+        announce("library backend residuals")
+        e.set_precision("bf16x3")
+        Q1, Q2 = e.residual(1, TA1 + D1, TA2 + D2)
+        assert float(np.abs(Q2 - P2).max()) <= 1e-4 * scale
+        e.set_precision("fp64")
+        e.residual_anchor(TA1, TA2)
+        e.set_precision("bf16")
+        R1, R2 = e.residual_refine(TA1 + D1, TA2 + D2)
+        assert float(np.abs(R2 - P2).max()) <= 1e-8 * scale
+    finally:
+        engine.set_bf16_backend(False)
+        e.set_precision("fp64")
+
+
 def test_refinement_residual_is_exact_in_fp64():
     """Anchor at T_A, refinement at T_A + d (FP64): equals the plain residual
     at T_A + d (the expansion is exact algebra)."""

@@ -44,3 +44,17 @@ def test_gemm_alpha_beta_and_empty_k(prec):
 def test_gemm_split_k(prec):
     # few output tiles and a long k: the launcher splits k and reduces in FP64
     _run(prec, 1, 0, 64, 48, 40000, seed=3)
+
+
+@pytest.mark.parametrize("prec", [2, 3])
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 1)])
+@pytest.mark.parametrize("M,N,K", [(17, 33, 5), (256, 384, 96), (300, 200, 3000)])
+def test_gemm_library_backend(prec, ta, tb, M, N, K):
+    """The hipBLASLt backend of the split-BF16x3 / bf16 products (operand
+    planes with K tripled, FP64 epilogue) meets the same error bounds."""
+    from sisi4s_amd import engine
+    engine.set_bf16_backend(True)
+    try:
+        _run(prec, ta, tb, M, N, K, alpha=0.75, beta=0.5, seed=M + K)
+    finally:
+        engine.set_bf16_backend(False)
