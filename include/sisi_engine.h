@@ -180,17 +180,19 @@ int sisi_get_integrals(sisi_engine *e, int which, double *out);
 /* Precision used by the dense contractions of the next residual call. */
 int sisi_engine_set_precision(sisi_engine *e, int precision);
 
-/* CoulombFactors Lambda[G,R] (complex128 [NG, NR]) and FactorOrbitals
- * Pi[p,R] (complex128 [Np, NR]): while set (NR > 0), the particle-particle
- * ladder of the real residual is the factor-based one,
- * sliceAmplitudesFromCoupledCoulombFactors (ClusterSinglesDoublesAlgorithm.cxx:
- * 567-672) over factor slices of factors_slice_size (<= 0: Nv, the
- * factorsSliceSize default, CcsdEnergyFromCoulombIntegrals.cxx:340-372).  It
- * equals the vertex ladder when Gamma[G,p,q] = sum_R Lambda_GR Pi_pR Pi_qR.
- * NR = 0 removes the factors.  Not built for complex amplitudes or with the
- * refinement residuals. */
+/* CoulombFactors Lambda[G,R] (complex128 [NG, NR]), FactorOrbitals
+ * Pi[p,R] and OutgoingFactorOrbitals Po[p,R] (complex128 [Np, NR]; NULL
+ * means Pi): while set (NR > 0), the particle-particle ladder is the
+ * factor-based one, sliceAmplitudesFromCoupledCoulombFactors
+ * (ClusterSinglesDoublesAlgorithm.cxx:567-672 for real amplitudes, which
+ * reads Pi only; 745-880 for complex amplitudes) over factor slices of
+ * factors_slice_size (<= 0: Nv, the factorsSliceSize default,
+ * CcsdEnergyFromCoulombIntegrals.cxx:340-372, 786-813).  It equals the
+ * vertex ladder when Gamma[G,q,r] = sum_R Lambda_GR Po_qR Pi_rR.  NR = 0
+ * removes the factors.  Not combined with the refinement residuals. */
 int sisi_engine_set_coulomb_factors(sisi_engine *e, int NR, int factors_slice_size,
-                                    const double *lambda, const double *pi);
+                                    const double *lambda, const double *pi,
+                                    const double *pi_out);
 
 /* Upload one real integral class (SISI_PPHH, _PHPH, _HHHH, _HHHP, _PPPP,
  * _PPPH) instead of building it from a vertex: the inputs of

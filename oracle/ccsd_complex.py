@@ -64,10 +64,33 @@ def ppl_ladder(left, right, X, slice_size, R2):
             ccsd.slice_into_residuum(Rxyij, a, b, R2)
 
 
+def factor_ladder(T1, X, Lambda, Pi, PiOut, slice_size, R2):
+    """sliceAmplitudesFromCoupledCoulombFactors for complex amplitudes
+    (ClusterSinglesDoublesAlgorithm.cxx:745-880) over all pairs of factor
+    slices: Lambda[G,R], Pi[p,R] (FactorOrbitals), PiOut[p,R]
+    (OutgoingFactorOrbitals), X = T2 + T1 T1."""
+    Nv, No = T1.shape
+    NR = Pi.shape[1]
+    PiaR, PiiR = Pi[No:, :], Pi[:No, :]
+    PicR, PijR = PiOut[No:, :], PiOut[:No, :]
+    dressed = PicR - ein("kR,bk->bR", PijR, T1)                       # cxx:851-853
+    conj_dressed = PiaR.conj() - ein("kR,ak->aR", PiiR.conj(), T1)    # cxx:855-857
+    for b in range(0, NR, slice_size):
+        for a in range(0, NR, slice_size):
+            L, Rr = slice(a, min(a + slice_size, NR)), slice(b, min(b + slice_size, NR))
+            XRaij = ein("cdij,cR->Rdij", X, PicR[:, L].conj())        # cxx:827
+            XRSij = ein("Rdij,dS->RSij", XRaij, PiaR[:, Rr])          # cxx:830
+            VRS = ein("GR,GS->RS", Lambda[:, L].conj(), Lambda[:, Rr])
+            XRSij = XRSij * VRS[:, :, None, None]                      # cxx:834
+            XRbij = ein("RSij,bS->Rbij", XRSij, dressed[:, Rr])        # cxx:867
+            R2 += ein("Rbij,aR->abij", XRbij, conj_dressed[:, L])      # cxx:874
+
+
 def residual(it, T1, T2, gamma, ints, slice_size=None, initial_doubles=False,
-             ppl=True, distinguishable=False):
+             ppl=True, distinguishable=False, factors=None):
     """CcsdEnergyFromCoulombIntegrals::getResiduum for complex amplitudes
-    (cxx:496-891), no CoulombFactors.  Returns (R1, R2), complex."""
+    (cxx:496-891).  factors = (Lambda, Pi, PiOut, factors_slice_size) selects
+    the factor-based ladder (cxx:786-813).  Returns (R1, R2), complex."""
     Nv, No = T1.shape
     Vabij = ints["PPHH"]
     R1 = np.zeros((Nv, No), complex)
@@ -150,7 +173,9 @@ def residual(it, T1, T2, gamma, ints, slice_size=None, initial_doubles=False,
 
     # particle-particle ladder from the dressed vertex
     # (ClusterSinglesDoublesAlgorithm.cxx:531-557)
-    if ppl:
+    if ppl and factors is not None:
+        factor_ladder(T1, X, factors[0], factors[1], factors[2], factors[3], R2)
+    elif ppl:
         right = Gab - ein("Gkb,ak->Gab", Gia, T1)
         left = cGab - ein("Gkb,ak->Gab", cGia, T1)
         ppl_ladder(left, right, X, slice_size, R2)
@@ -192,7 +217,7 @@ def pair_energy(T1, T2, Vijab):
 def solve(gamma, eps_i, eps_a, *, mixer="DiisMixer", max_residua=4,
           mixing_ratio=1.0, max_iterations=16, amplitudes_convergence=1e-5,
           energy_convergence=1e-6, level_shift=0.0, slice_size=None,
-          initial=None, ppl=True, distinguishable=False):
+          initial=None, ppl=True, distinguishable=False, factors=None):
     """ClusterSinglesDoublesAlgorithm::run<complex> (cxx:38-128); energies
     are complex, the reported energy is their real part (cxx:30-33)."""
     No, Nv = eps_i.shape[0], eps_a.shape[0]
@@ -214,7 +239,7 @@ def solve(gamma, eps_i, eps_a, *, mixer="DiisMixer", max_residua=4,
     while i < max_iterations:
         R1, R2 = residual(i, T[0], T[1], gamma, ints, slice_size,
                           initial_doubles=initial is not None, ppl=ppl,
-                          distinguishable=distinguishable)
+                          distinguishable=distinguishable, factors=factors)
         E1, E2 = ccsd.estimate_amplitudes(R1, R2, T[0], T[1], eps_i, eps_a, level_shift)
         dT = [E1 - T[0], E2 - T[1]]
         mix.append([E1, E2], dT)

@@ -201,11 +201,12 @@ int sisi_engine_set_precision(sisi_engine *h, int precision) {
 }
 
 int sisi_engine_set_coulomb_factors(sisi_engine *h, int NR, int factors_slice_size,
-                                    const double *lambda, const double *pi) {
+                                    const double *lambda, const double *pi,
+                                    const double *pi_out) {
   return guarded([&] {
     ENGINE_REQUIRED(h);
     SISI_REQUIRE(NR >= 0, SISI_ERR_INVALID, "negative NR");
-    set_coulomb_factors(h->e, NR, factors_slice_size, lambda, pi);
+    set_coulomb_factors(h->e, NR, factors_slice_size, lambda, pi, pi_out);
   });
 }
 

@@ -46,9 +46,12 @@ void zccsd_pair_energies(Engine &e, const double *T1, const double *T2, double *
 // Coulomb factors (host arrays, interleaved complex128: lambda [NG, NR],
 // pi [Np, NR]) for the factor-based ladder; NR = 0 removes them.  slice <= 0
 // means Nv (factorsSliceSize default).
-void set_coulomb_factors(Engine &e, int NR, int slice, const double *lambda, const double *pi);
-// R2 += the factor-based ladder of X = T2 + T1 T1 (real amplitudes).
+void set_coulomb_factors(Engine &e, int NR, int slice, const double *lambda, const double *pi,
+                         const double *pi_out);
+// R2 += the factor-based ladder of X = T2 + T1 T1 (real amplitudes, and
+// planar complex amplitudes with the outgoing factor orbitals).
 void factor_ladder(Engine &e, const double *T1, const double *X, double *R2, int precision);
+void zfactor_ladder(Engine &e, const double *T1, const double *X, double *R2, int precision);
 
 // Full-integral formulation (CcsdEnergyFromCoulombIntegralsReference.cxx:
 // 29-295): integral classes given directly, PPPP/PPPH built from the vertex

@@ -104,8 +104,6 @@ void zintegrals_from_vertex(Engine &e) {
 void zccsd_residual(Engine &e, int it, bool initial_doubles, int slice_size, const double *T1c,
                     const double *T2c, double *R1p, double *R2p) {
   SISI_REQUIRE(e.have_integrals && e.cplx, SISI_ERR_STATE, "residual: integrals not built");
-  SISI_REQUIRE(e.NR == 0 || !e.ppl, SISI_ERR_INVALID,
-               "the factor-based ladder is built for real amplitudes only");
   const int64_t o = e.No, v = e.Nv, G = e.NG;
   const int64_t n2 = v * v * o * o;
   double *T1p = const_cast<double *>(T1c), *T2p = const_cast<double *>(T2c);
@@ -250,7 +248,10 @@ void zccsd_residual(Engine &e, int it, bool initial_doubles, int slice_size, con
     }
     zctr(e, 1.0, "klij,abkl->abij", Xklij, false, e.ppl ? T2 : X, false, 1.0, R2, prec);  // :763-767
   }
-  if (e.ppl) {
+  if (e.ppl && e.NR > 0) {
+    // factor-based ladder for complex amplitudes (cxx:786-813)
+    zfactor_ladder(e, T1p, Xb.get(), R2p, e.prec_ladder);
+  } else if (e.ppl) {
     // particle-particle ladder (cxx:814-858) with the dressed vertices of
     // sliceCoupledCoulombIntegrals (ClusterSinglesDoublesAlgorithm.cxx:531-557):
     // right = Gamma_Gab - Gamma_Gkb T_ak, left = conj(Gamma_Gba) - conj(Gamma_Gbk) T_ak

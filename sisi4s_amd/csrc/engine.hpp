@@ -152,7 +152,7 @@ class Engine {
   int formulation = 0;
   int NR = 0, factors_slice = 0;
   bool factors_real = true;  // imaginary parts all zero: real products only
-  DBuf Lam, PiF, VRS;
+  DBuf Lam, PiF, PiO, VRS;  // PiO: OutgoingFactorOrbitals (complex amplitudes)
   DBuf Vbaij;  // V_abij with a,b exchanged (energy exchange term)
   DBuf Uabij;  // 2 V_abij - V_baij
   // (T) integrals, built on first use: V_bdck as [d,b,c,k] and as [d,c,b,k],

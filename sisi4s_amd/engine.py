@@ -94,7 +94,7 @@ SIGNATURES = [
     ("sisi_set_bf16_backend", ctypes.c_int, [ctypes.c_int]),
     ("sisi_engine_set_formulation", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("sisi_engine_set_coulomb_factors", ctypes.c_int,
-     [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _c_double_p, _c_double_p]),
+     [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _c_double_p, _c_double_p, _c_double_p]),
     ("sisi_ccsd_residual", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_double_p, _c_double_p,
       _c_double_p, _c_double_p]),
@@ -358,19 +358,24 @@ class Engine:
             precision = PRECISIONS[precision]
         _check(self._lib.sisi_engine_set_precision(self._h, int(precision)))
 
-    def set_coulomb_factors(self, Lambda=None, Pi=None, slice_size=-1):
-        """CoulombFactors Lambda[G,R] and FactorOrbitals Pi[p,R]: the ladder
-        becomes the factor-based one (CcsdEnergyFromCoulombIntegrals.cxx:
-        340-372); no arguments remove them."""
+    def set_coulomb_factors(self, Lambda=None, Pi=None, slice_size=-1, PiOut=None):
+        """CoulombFactors Lambda[G,R], FactorOrbitals Pi[p,R] and (complex
+        amplitudes) OutgoingFactorOrbitals PiOut[p,R]: the ladder becomes the
+        factor-based one (CcsdEnergyFromCoulombIntegrals.cxx:340-372,
+        786-813); no arguments remove them."""
         if Lambda is None:
-            _check(self._lib.sisi_engine_set_coulomb_factors(self._h, 0, 0, None, None))
+            _check(self._lib.sisi_engine_set_coulomb_factors(self._h, 0, 0, None, None, None))
             return
-        L = np.ascontiguousarray(np.asarray(Lambda, dtype=np.complex128).ravel(order="F"))
-        P = np.ascontiguousarray(np.asarray(Pi, dtype=np.complex128).ravel(order="F"))
+
+        def z(a):
+            return np.ascontiguousarray(np.asarray(a, dtype=np.complex128).ravel(order="F"))
+        L, P = z(Lambda), z(Pi)
+        O = z(PiOut) if PiOut is not None else None
         NR = int(np.shape(Lambda)[1])
         assert L.size == self.NG * NR and P.size == self.Np * NR
         _check(self._lib.sisi_engine_set_coulomb_factors(
-            self._h, NR, int(slice_size), _ptr(L.view(np.float64)), _ptr(P.view(np.float64))))
+            self._h, NR, int(slice_size), _ptr(L.view(np.float64)), _ptr(P.view(np.float64)),
+            _ptr(O.view(np.float64)) if O is not None else None))
 
     def set_integrals(self, which, V):
         """Upload one real integral class (no vertex needed for the

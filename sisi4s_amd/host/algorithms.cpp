@@ -227,7 +227,6 @@ class CcsdEnergyFromCoulombIntegrals : public Algorithm {
     if (vertex_arguments && isArgumentGiven("CoulombFactors")) {
       // factor-based ladder (cxx:340-372): CoulombFactors [NG, NR] and
       // FactorOrbitals [Np, NR], complex or real
-      if (cplx) throw StepError("CoulombFactors: the factor-based ladder is built for real amplitudes");
       const auto &L = getTensorArgument("CoulombFactors");
       const auto &P = getTensorArgument("FactorOrbitals");
       if (L.lens.size() != 2 || P.lens.size() != 2 || L.lens[1] != P.lens[1] ||
@@ -239,10 +238,18 @@ class CcsdEnergyFromCoulombIntegrals : public Algorithm {
         fs = int(double(NR) * getRealArgument("factorsSliceFactor"));
       const auto Lz = amplitudes_for(L, true, "CoulombFactors");
       const auto Pz = amplitudes_for(P, true, "FactorOrbitals");
-      check(sisi_engine_set_coulomb_factors(e, int(NR), fs, Lz.data(), Pz.data()),
+      // complex amplitudes read OutgoingFactorOrbitals as well (cxx:751-752)
+      std::vector<double> Oz;
+      if (cplx) {
+        const auto &O = getTensorArgument("OutgoingFactorOrbitals");
+        if (O.lens != P.lens) throw StepError("OutgoingFactorOrbitals [Np, NR] does not match");
+        Oz = amplitudes_for(O, true, "OutgoingFactorOrbitals");
+      }
+      check(sisi_engine_set_coulomb_factors(e, int(NR), fs, Lz.data(), Pz.data(),
+                                            Oz.empty() ? nullptr : Oz.data()),
             "sisi_engine_set_coulomb_factors");
     } else {
-      check(sisi_engine_set_coulomb_factors(e, 0, 0, nullptr, nullptr),
+      check(sisi_engine_set_coulomb_factors(e, 0, 0, nullptr, nullptr, nullptr),
             "sisi_engine_set_coulomb_factors");
     }
     std::vector<double> t1, t2;

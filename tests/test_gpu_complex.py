@@ -163,3 +163,43 @@ def test_complex_ueg_plane_wave_triples_known_answer():
     e.ccsd_solve(engine.default_opts(mixer="DiisMixer", max_residua=4, max_iterations=10,
                                      energy_convergence=1e-8, amplitudes_convergence=1e-8))
     assert abs(e.triples() - -0.0063019625641725016) <= 1e-13
+
+
+def test_complex_factor_ladder_matches_oracle_and_vertex_ladder():
+    """Factor-based ladder for complex amplitudes with OutgoingFactorOrbitals
+    (ClusterSinglesDoublesAlgorithm.cxx:745-880): against the oracle and the
+    engine's own vertex ladder on Gamma = sum_R Lambda Po Pi (Po = conj(Pi),
+    Lambda real), and a whole run."""
+    from sisi4s_amd import engine
+    rng = np.random.default_rng(21)
+    No, Nv, NG, NR = 4, 12, 16, 30
+
+    def cn(*s):
+        return rng.normal(size=s) + 1j * rng.normal(size=s)
+
+    # a vertex Hermitian in (q,r), as the vertex ladder's slicing assumes
+    Lam = rng.normal(size=(NG, NR)) / np.sqrt(2 * NR * NG)
+    Pi = cn(No + Nv, NR) / 3
+    Po = Pi.conj()
+    g = np.einsum("GR,qR,rR->Gqr", Lam, Po, Pi)
+    ei = np.sort(rng.uniform(-2.0, -0.5, No))
+    ea = np.sort(rng.uniform(0.2, 3.0, Nv))
+    e = make_engine(g, ei, ea)
+    ints = cz.integrals_from_vertex(g, No)
+    T1, T2 = complex_amplitudes(Nv, No, seed=22, scale=0.05)
+    V1, V2 = e.residual(1, T1, T2, slice_size=5)
+    for fs in (7, 30):
+        announce("complex factor ladder slice=%d" % fs)
+        e.set_coulomb_factors(Lam, Pi, slice_size=fs, PiOut=Po)
+        R1, R2 = e.residual(1, T1, T2, slice_size=5)
+        O1, O2 = cz.residual(1, T1, T2, g, ints, 5, factors=(Lam, Pi, Po, fs))
+        scale = float(np.abs(O2).max())
+        assert float(np.abs(R2 - O2).max()) <= 1e-11 * scale
+        assert float(np.abs(R1 - O1).max()) <= 1e-11 * scale
+        assert float(np.abs(R2 - V2).max()) <= 1e-11 * scale
+    kw = dict(mixer="DiisMixer", max_residua=6, max_iterations=80, slice_size=5,
+              amplitudes_convergence=1e-9, energy_convergence=1e-11)
+    rep = e.ccsd_solve(engine.default_opts(**kw))
+    ref = cz.solve(g, ei, ea, factors=(Lam, Pi, Po, 30), **kw)
+    assert rep["converged"] and ref["converged"]
+    assert np.abs(e.energies() - np.real(ref["energies"])).max() <= 1e-10

@@ -419,3 +419,40 @@ def test_only_ppl_is_the_ladder_of_the_vertex_path_without_singles():
         _, A = ccsd.residual(1, T1, T2, g, ints, s)
         _, B = ccsd.residual(1, T1, T2, g, ints, s, ppl=False)
         assert np.abs((A - B) - P2).max() <= 1e-13 * np.abs(P2).max()
+
+
+def test_complex_factor_ladder_equals_complex_vertex_ladder():
+    """The factor-based ladder for complex amplitudes (ClusterSinglesDoublesAlgorithm.cxx:
+    745-880) with FactorOrbitals Pi and OutgoingFactorOrbitals Po equals the
+    complex vertex ladder when Gamma[G,q,r] = sum_R Lambda_GR Po_qR Pi_rR: on
+    the ladder alone for any factors, and on the whole residual (whose vertex
+    ladder evaluates slice pairs a >= b and adds their transposes, i.e.
+    assumes a vertex Hermitian in (q,r)) for Po = conj(Pi), Lambda real, for
+    several factor slice sizes."""
+    from oracle import ccsd_complex as cz
+    rng = np.random.default_rng(3)
+    No, Nv, NG, NR = 3, 7, 10, 16
+
+    def cn(*s):
+        return rng.normal(size=s) + 1j * rng.normal(size=s)
+
+    T1, T2 = 0.1 * cn(Nv, No), 0.1 * cn(Nv, Nv, No, No)
+    T2 = T2 + T2.transpose(1, 0, 3, 2)
+    X = T2 + np.einsum("ai,bj->abij", T1, T1)
+    Lam, Pi, Po = cn(NG, NR) / np.sqrt(NR * NG), cn(No + Nv, NR) / 2, cn(No + Nv, NR) / 2
+    g = np.einsum("GR,qR,rR->Gqr", Lam, Po, Pi)
+    out = np.zeros_like(X)
+    cz.factor_ladder(T1, X, Lam, Pi, Po, 5, out)
+    right = g[:, No:, No:] - np.einsum("Gkb,ak->Gab", g[:, :No, No:], T1)
+    left = (np.einsum("Gba->Gab", g[:, No:, No:].conj())
+            - np.einsum("Gkb,ak->Gab", np.einsum("Gai->Gia", g[:, No:, :No].conj()), T1))
+    full = np.einsum("Gxc,Gyd,cdij->xyij", left, right, X)
+    assert np.abs(out - full).max() <= 1e-13 * np.abs(full).max()
+    Lam = Lam.real
+    Po = Pi.conj()
+    g = np.einsum("GR,qR,rR->Gqr", Lam, Po, Pi)
+    ints = cz.integrals_from_vertex(g, No)
+    R1, R2 = cz.residual(1, T1, T2, g, ints, 3)
+    for fs in (1, 5, 16):
+        F1, F2 = cz.residual(1, T1, T2, g, ints, 3, factors=(Lam, Pi, Po, fs))
+        assert np.abs(F2 - R2).max() <= 1e-13 * np.abs(R2).max() and np.abs(F1 - R1).max() == 0.0
