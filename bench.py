@@ -87,6 +87,8 @@ def main():
                          "mixed runs to |dT|/|T| <= 1e-8 with (T); batched: 64 small systems")
     ap.add_argument("--systems", type=int, default=64)
     ap.add_argument("--workers", type=int, default=8)
+    ap.add_argument("--graphs", type=int, default=0,
+                    help="batched: replay each system's residual from a captured hipGraph")
     ap.add_argument("--level-shift", type=float, default=0.0)
     ap.add_argument("--max-iterations", type=int, default=400)
     ap.add_argument("--precisions", default="fp64,mixed")
@@ -476,7 +478,8 @@ def batched(args):
         t0 = time.perf_counter()
         ri = args.refine_interval if prec != "fp64" else 0
         E, reps = eng.ccsd_solve_batched(systems, eng.default_opts(precision=prec,
-                                                                   refine_interval=ri, **kw),
+                                                                   refine_interval=ri,
+                                                                   graphs=args.graphs, **kw),
                                          workers=args.workers, strict=False)
         res[name] = dict(seconds=time.perf_counter() - t0, energies=E.tolist(),
                          failed=[k for k, r in enumerate(reps) if r["status"] != 0],
@@ -490,7 +493,8 @@ def batched(args):
     dE = np.abs(np.array(res["fp64"]["energies"])[ok] - np.array(res["mixed"]["energies"])[ok])
     out = {"metric": METRIC, "mode": "batched",
            "config": {"workload": "%d seeded systems, No U[4,16], Nv U[30,120], NG=4Nv, DIIS(6), "
-                                  "to 1e-8" % args.systems, "workers": args.workers},
+                                  "to 1e-8" % args.systems, "workers": args.workers,
+                      "graphs": args.graphs},
            "fp64_seconds": res["fp64"]["seconds"], "mixed_seconds": res["mixed"]["seconds"],
            "speedup": res["fp64"]["seconds"] / res["mixed"]["seconds"],
            "max_dE": float(np.nanmax(dE)) if len(dE) else None,

@@ -113,6 +113,10 @@ typedef struct {
                                     (default 0: no coarse stage)          */
   int coarse_precision;          /* SISI_BF16 (default), SISI_FP32 or
                                     SISI_BF16X3                            */
+  int graphs;                    /* nonzero: the plain residual is captured
+                                    once per precision in a hipGraph and
+                                    replayed (small, launch-bound systems);
+                                    default 0                             */
 } sisi_ccsd_opts;
 
 typedef struct {

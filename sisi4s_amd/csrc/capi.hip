@@ -83,6 +83,7 @@ void sisi_ccsd_opts_default(sisi_ccsd_opts *o) {
   o->distinguishable = 0;
   o->coarse_threshold = 0.0;
   o->coarse_precision = SISI_BF16;
+  o->graphs = 0;
 }
 
 int sisi_engine_create(const sisi_dims *dims, sisi_engine **out) {
@@ -110,6 +111,7 @@ int sisi_engine_set_vertex(sisi_engine *h, const double *gamma) {
     e.have_integrals = false;
     e.run_active = false;
     e.drop_triples_cache();
+    e.drop_residual_graph();
   });
 }
 
@@ -126,6 +128,7 @@ int sisi_engine_set_vertex_complex(sisi_engine *h, const double *gamma) {
     e.have_integrals = false;
     e.run_active = false;
     e.drop_triples_cache();
+    e.drop_residual_graph();
   });
 }
 

@@ -6,6 +6,7 @@
 #include <cmath>
 
 #include "contract.hpp"
+#include "gemm.hpp"
 #include "tensor_ops.hpp"
 
 namespace sisi {
@@ -59,6 +60,7 @@ void integrals_from_vertex(Engine &e) {
   axpby(e, v * v * o * o, 2.0, e.Vabij.get(), 0.0, e.Uabij.get());
   axpby(e, v * v * o * o, -1.0, e.Vbaij.get(), 1.0, e.Uabij.get());
   e.drop_triples_cache();
+  e.drop_residual_graph();
   e.have_integrals = true;
   e.sync();
 }
@@ -676,6 +678,71 @@ double fock_dot(Engine &e, const double *x1, const double *x2, const double *y1,
   return dot(e, int64_t(e.n1()), x1, y1) + dot(e, int64_t(e.n2()), x2, y2);
 }
 
+// Residual replayed from a hipGraph.  The plain residual issues only kernels
+// and device copies on e.stream (no host synchronisation, no memsets), so
+// one capture per configuration serves every later iteration: T is copied
+// into the graph's fixed input buffers, the graph is launched and R copied
+// out.  For small systems the residual is bound by launch overhead (a few
+// hundred launches of a few microseconds each), which the replay removes.
+// Every pool buffer the captured launches touch stays reserved until the
+// graph is dropped.  Returns false (and the caller runs the residual
+// eagerly) when the stream cannot be captured.
+bool graph_residual(Engine &e, int slice, const double *T1, const double *T2, double *R1,
+                    double *R2) {
+  Engine::ResidualGraph &g = e.rgraph;
+  const std::vector<int> key = {e.precision,       slice,   e.formulation,  int(e.ppl),
+                                int(e.distinguishable), e.NR, int(e.cplx), int(e.run_initial),
+                                get_b3_backend()};
+  if (g.exec && g.key != key) e.drop_residual_graph();
+  const int64_t n1 = int64_t(e.n1()), n2 = int64_t(e.n2());
+  if (!g.exec) {
+    e.drop_residual_graph();
+    g.T1 = DBuf(&e.pool, size_t(n1));
+    g.T2 = DBuf(&e.pool, size_t(n2));
+    g.R1 = DBuf(&e.pool, size_t(n1));
+    g.R2 = DBuf(&e.pool, size_t(n2));
+    copy(e, V1(g.T1.get(), n1), V1(const_cast<double *>(T1), n1));
+    copy(e, V1(g.T2.get(), n2), V1(const_cast<double *>(T2), n2));
+    // eager pass first: library handles, algorithm choices and lazily
+    // created state are set up outside the capture
+    ccsd_residual(e, 1, e.run_initial, slice, g.T1.get(), g.T2.get(), g.R1.get(), g.R2.get());
+    e.sync();
+    hipGraph_t graph = nullptr;
+    e.pool.begin_capture();
+    hipError_t err = hipStreamBeginCapture(e.stream, hipStreamCaptureModeRelaxed);
+    if (err == hipSuccess) {
+      try {
+        ccsd_residual(e, 1, e.run_initial, slice, g.T1.get(), g.T2.get(), g.R1.get(), g.R2.get());
+      } catch (...) {
+        (void)hipStreamEndCapture(e.stream, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        (void)hipGetLastError();
+        e.pool.release(e.pool.end_capture());
+        e.drop_residual_graph();
+        throw;
+      }
+      err = hipStreamEndCapture(e.stream, &graph);
+    }
+    g.held = e.pool.end_capture();
+    if (err == hipSuccess) err = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (err != hipSuccess) {
+      (void)hipGetLastError();
+      g.exec = nullptr;
+      e.drop_residual_graph();
+      return false;
+    }
+    g.key = key;
+  } else {
+    copy(e, V1(g.T1.get(), n1), V1(const_cast<double *>(T1), n1));
+    copy(e, V1(g.T2.get(), n2), V1(const_cast<double *>(T2), n2));
+  }
+  SISI_HIP(hipGraphLaunch(g.exec, e.stream));
+  copy(e, V1(R1, n1), V1(g.R1.get(), n1));
+  copy(e, V1(R2, n2), V1(g.R2.get(), n2));
+  return true;
+}
+
 }  // namespace
 
 void ccsd_begin(Engine &e, const sisi_ccsd_opts &opts, const double *T1h, const double *T2h) {
@@ -695,6 +762,7 @@ void ccsd_begin(Engine &e, const sisi_ccsd_opts &opts, const double *T1h, const 
   SISI_REQUIRE(opts.refine_precision == SISI_FP32 || opts.refine_precision == SISI_BF16X3 ||
                    opts.refine_precision == SISI_BF16,
                SISI_ERR_INVALID, "refine_precision must be SISI_FP32, SISI_BF16X3 or SISI_BF16");
+  e.drop_residual_graph();
   e.opts = opts;
   e.ppl = opts.ppl != 0;
   e.distinguishable = opts.distinguishable != 0;
@@ -758,8 +826,15 @@ void ccsd_iterate(Engine &e, int n) {
       }
     } else {
       set_precision(e, prec);
-      ccsd_residual(e, e.iteration, e.run_initial, o.slice_size, e.T1.get(), e.T2.get(), R1.get(),
-                    R2.get());
+      // graphed residual after the MP2 start (which is a copy) and with
+      // launch timing off
+      const bool graphed = o.graphs != 0 && (e.iteration > 0 || e.run_initial) &&
+                           !e.timers->enabled &&
+                           graph_residual(e, o.slice_size, e.T1.get(), e.T2.get(), R1.get(),
+                                          R2.get());
+      if (!graphed)
+        ccsd_residual(e, e.iteration, e.run_initial, o.slice_size, e.T1.get(), e.T2.get(),
+                      R1.get(), R2.get());
     }
     DBuf d1(&e.pool, e.n1()), d2(&e.pool, e.n2());
     estimate_amplitudes(e, 2, R1.get(), e.T1.get(), d1.get(), o.level_shift);

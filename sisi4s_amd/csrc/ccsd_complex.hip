@@ -97,6 +97,7 @@ void zintegrals_from_vertex(Engine &e) {
   e.Vbaij.reset();
   e.Uabij.reset();
   e.drop_triples_cache();
+  e.drop_residual_graph();
   e.have_integrals = true;
   e.sync();
 }

@@ -49,6 +49,7 @@ void set_integrals(Engine &e, int which, const double *V) {
   const unsigned core = (1u << SISI_PPHH) | (1u << SISI_PHPH) | (1u << SISI_HHHH) | (1u << SISI_HHHP);
   if ((e.given_integrals & core) == core) e.have_integrals = true;
   e.drop_triples_cache();
+  e.drop_residual_graph();
   e.sync();
 }
 

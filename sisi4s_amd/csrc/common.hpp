@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <map>
+#include <set>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -52,6 +53,8 @@ class Pool {
     if (it != free_.end() && !it->second.empty()) {
       void *p = it->second.back();
       it->second.pop_back();
+      live_.insert(p);
+      if (capturing_) captured_.push_back(p);
       return p;
     }
     void *p = nullptr;
@@ -68,13 +71,50 @@ class Pool {
       }
     }
     size_[p] = bytes;
+    live_.insert(p);
+    if (capturing_) captured_.push_back(p);
     return p;
   }
   void put(void *p) {
     if (!p) return;
     auto it = size_.find(p);
     if (it == size_.end()) return;
+    live_.erase(p);
+    if (reserved_.count(p)) return;  // held by a captured graph
     free_[it->second].push_back(p);
+  }
+  // Graph capture: every buffer handed out between begin_capture and
+  // end_capture is referenced by the captured graph, so end_capture takes
+  // them out of circulation until release() (when the graph is destroyed);
+  // a buffer that still has an owner at release() returns to the free list
+  // when that owner gives it back.
+  void begin_capture() {
+    capturing_ = true;
+    captured_.clear();
+  }
+  std::vector<void *> end_capture() {
+    capturing_ = false;
+    std::vector<void *> held;
+    for (void *p : captured_) {
+      if (reserved_.count(p)) continue;
+      reserved_.insert(p);
+      held.push_back(p);
+      auto &fl = free_[size_[p]];
+      for (size_t k = 0; k < fl.size(); ++k)
+        if (fl[k] == p) {
+          fl.erase(fl.begin() + k);
+          break;
+        }
+    }
+    captured_.clear();
+    return held;
+  }
+  void release(const std::vector<void *> &held) {
+    for (void *p : held) {
+      reserved_.erase(p);
+      auto it = size_.find(p);
+      if (it != size_.end() && !live_.count(p)) free_[it->second].push_back(p);
+    }
   }
   void trim() {
     (void)hipDeviceSynchronize();
@@ -90,11 +130,17 @@ class Pool {
     for (auto &kv : size_) (void)hipFree(kv.first);
     size_.clear();
     free_.clear();
+    reserved_.clear();
+    live_.clear();
   }
 
  private:
   std::map<size_t, std::vector<void *>> free_;
   std::map<void *, size_t> size_;
+  std::set<void *> reserved_;
+  std::set<void *> live_;  // handed out and not yet given back
+  std::vector<void *> captured_;
+  bool capturing_ = false;
 };
 
 // Owning device buffer of doubles, returned to the pool on destruction.

@@ -94,8 +94,24 @@ Engine::Engine(const sisi_dims &d) {
   timers.reset(new Timers(stream));
 }
 
+void Engine::drop_residual_graph() {
+  if (rgraph.exec) {
+    (void)hipStreamSynchronize(stream);
+    (void)hipGraphExecDestroy(rgraph.exec);
+    rgraph.exec = nullptr;
+  }
+  pool.release(rgraph.held);
+  rgraph.held.clear();
+  rgraph.key.clear();
+  rgraph.T1.reset();
+  rgraph.T2.reset();
+  rgraph.R1.reset();
+  rgraph.R2.reset();
+}
+
 Engine::~Engine() {
   (void)hipStreamSynchronize(stream);
+  drop_residual_graph();
   timers.reset();
   // device buffers return to the pool before it is released
   T1.reset();

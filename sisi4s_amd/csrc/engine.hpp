@@ -216,6 +216,17 @@ class Engine {
   // hipBLASLt state of the split-BF16x3 library backend (handle, workspace,
   // algorithms per shape); declared last so it is released before the pool
   std::shared_ptr<void> blaslt;
+
+  // Residual captured in a hipGraph (opts.graphs): fixed input/output
+  // buffers, the pool buffers the graph references, and what it was captured
+  // for (precision, slice size, formulation, variants, factors)
+  struct ResidualGraph {
+    hipGraphExec_t exec = nullptr;
+    DBuf T1, T2, R1, R2;
+    std::vector<void *> held;
+    std::vector<int> key;
+  } rgraph;
+  void drop_residual_graph();
 };
 
 }  // namespace sisi

@@ -24,6 +24,7 @@ namespace sisi {
 
 void set_coulomb_factors(Engine &e, int NR, int slice, const double *lambda, const double *pi,
                          const double *pi_out) {
+  e.drop_residual_graph();
   if (NR <= 0) {
     e.NR = 0;
     e.Lam.reset();

@@ -109,6 +109,7 @@ void gemm_b3(Engine &eng, GemmArgs g, int passes = 3);
 // Backend of the batch-1 split-BF16x3 / bf16 products for the whole process:
 // 0 = hand-written kernel (default), 1 = hipBLASLt on the split planes.
 void set_b3_backend(int library);
+int get_b3_backend();
 
 // Split-K reduction of g.work into C (deterministic order).
 void gemm_reduce(Engine &eng, const GemmArgs &g);

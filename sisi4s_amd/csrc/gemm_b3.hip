@@ -536,6 +536,7 @@ static std::atomic<int> &b3_backend() {
 static bool b3_use_library() { return b3_backend().load() == 1; }
 
 void set_b3_backend(int library) { b3_backend().store(library ? 1 : 0); }
+int get_b3_backend() { return b3_backend().load(); }
 
 template <int BM, int BN, int WM, int WN, int NBUF, int MINB = 1, int PASSES = 3>
 static void run_b3(Engine &e, GemmArgs g) {
