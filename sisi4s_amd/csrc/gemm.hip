@@ -421,23 +421,36 @@ __global__ __launch_bounds__(256, 2) void gemm_bf16x3_kernel(GemmArgs g) {
   }
 }
 
-// Split-K reduction: C(m,n) = alpha * sum_s W[s](m,n) + beta * C(m,n), the
-// partial sums added in split order.
-__global__ void gemm_reduce_kernel(GemmArgs g) {
+// Split-K reduction: C(m,n) = alpha * sum_s W[s](m,n) + beta * C(m,n).  A
+// block takes 64 consecutive outputs; its four waves sum the partials
+// s = q, q + 4, ... (q = wave) and the four sums are added in a fixed order,
+// so the result does not depend on timing.
+__global__ __launch_bounds__(256) void gemm_reduce_kernel(GemmArgs g) {
   const int64_t MN = int64_t(g.M) * g.N;
   const int bz = blockIdx.y;
   double *C = g.C + int64_t(bz) * g.bsc;
   const double *W = g.work + int64_t(bz) * g.ksplit * MN;
-  for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < MN;
-       e += int64_t(gridDim.x) * blockDim.x) {
-    const int m = int(e % g.M);
-    const int n = int(e / g.M);
+  __shared__ double part[4][64];
+  const int t = threadIdx.x & 63, q = threadIdx.x >> 6;
+  for (int64_t e0 = int64_t(blockIdx.x) * 64; e0 < MN; e0 += int64_t(gridDim.x) * 64) {
+    const int64_t e = e0 + t;
     double s = 0.0;
-    for (int k = 0; k < g.ksplit; ++k) s += W[k * MN + e];
-    const int64_t o = out_off_m(g, m) + out_off_n(g, n);
-    double v = g.alpha * s;
-    if (g.beta != 0.0) v += g.beta * C[o];
-    C[o] = v;
+    if (e < MN) {
+#pragma unroll 4
+      for (int k = q; k < g.ksplit; k += 4) s += W[k * MN + e];
+    }
+    part[q][t] = s;
+    __syncthreads();
+    if (q == 0 && e < MN) {
+      const int m = int(e % g.M);
+      const int n = int(e / g.M);
+      const double sum = (part[0][t] + part[1][t]) + (part[2][t] + part[3][t]);
+      const int64_t o = out_off_m(g, m) + out_off_n(g, n);
+      double v = g.alpha * sum;
+      if (g.beta != 0.0) v += g.beta * C[o];
+      C[o] = v;
+    }
+    __syncthreads();
   }
 }
 
@@ -463,9 +476,14 @@ static void launch_b3(GemmArgs &g, dim3 grid, hipStream_t s, bool AK, bool BNF) 
 void gemm_reduce(Engine &eng, const GemmArgs &g) {
   Timed t(*eng.timers, Kind::Bandwidth, 8.0 * (g.ksplit + 2) * double(g.M) * g.N * g.batch);
   const int64_t MN = int64_t(g.M) * g.N;
-  int blocks = int(std::min<int64_t>((MN + 255) / 256, 4096));
+  int blocks = int(std::min<int64_t>((MN + 63) / 64, 8192));
   hipLaunchKernelGGL(gemm_reduce_kernel, dim3(blocks, g.batch), dim3(256), 0, eng.stream, g);
   SISI_HIP(hipGetLastError());
+}
+
+static int64_t env_int(const char *name, int64_t fallback) {
+  const char *v = getenv(name);
+  return v != nullptr && *v != 0 ? int64_t(atoll(v)) : fallback;
 }
 
 void gemm(Engine &eng, int precision, GemmArgs g, bool exact) {
@@ -504,13 +522,17 @@ void gemm(Engine &eng, int precision, GemmArgs g, bool exact) {
     g.group = env_group > 0 ? env_group : 8;
   }
   const int kq = (precision == SISI_BF16X3) ? 32 : 16;
-  // split-K when the tile grid alone cannot fill the 256 CUs
+  // split-K when the tile grid alone cannot fill the 256 CUs: about `want`
+  // workgroups, each with at least `minsteps` k-chunks, at most `maxsplit`
+  // partial sums
+  static const int64_t want = env_int("SISI_GEMM_SPLIT_WANT", 1024);
+  static const int64_t minsteps = std::max<int64_t>(1, env_int("SISI_GEMM_SPLIT_MINSTEPS", 8));
+  static const int64_t maxsplit = std::max<int64_t>(1, env_int("SISI_GEMM_SPLIT_MAX", 64));
   int ksplit = 1;
-  const int64_t want = 1024;
   if (g.K > 4 * kq && tiles * g.batch < want) {
     int64_t s = (want + tiles * g.batch - 1) / (tiles * g.batch);
-    int64_t smax = std::max<int64_t>(1, g.K / (8 * kq));
-    ksplit = int(std::min<int64_t>(std::min<int64_t>(s, smax), 64));
+    int64_t smax = std::max<int64_t>(1, g.K / (minsteps * kq));
+    ksplit = int(std::min<int64_t>(std::min<int64_t>(s, smax), maxsplit));
   }
   int kchunk = (g.K + ksplit - 1) / ksplit;
   kchunk = ((kchunk + kq - 1) / kq) * kq;
