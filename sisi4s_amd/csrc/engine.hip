@@ -90,6 +90,8 @@ Engine::Engine(const sisi_dims &d) {
   SISI_HIP(hipGetDeviceCount(&count));
   SISI_REQUIRE(count > 0, SISI_ERR_HIP, "no HIP device");
   SISI_HIP(hipGetDevice(&device));
+  SISI_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+  if (cus <= 0) cus = 256;
   SISI_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   timers.reset(new Timers(stream));
 }

@@ -77,6 +77,7 @@ class Engine {
 
   int No, Nv, NG, Np;
   int device = 0;
+  int cus = 256;  // compute units of the device
   hipStream_t stream = nullptr;
   Pool pool;
   std::unique_ptr<Timers> timers;

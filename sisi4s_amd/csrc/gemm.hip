@@ -481,6 +481,23 @@ void gemm_reduce(Engine &eng, const GemmArgs &g) {
   SISI_HIP(hipGetLastError());
 }
 
+int round_filling_split(int64_t groups, int64_t s, int64_t smax, int cus) {
+  // workgroups of equal length run in rounds of 2 per CU: from s up to 4 s,
+  // the smallest split whose last round is at least 95 % full (else the
+  // fullest one found)
+  const int64_t slots = 2 * int64_t(cus);
+  auto fill = [&](int64_t ks) {
+    const int64_t blocks = groups * ks;
+    return double(blocks) / double(((blocks + slots - 1) / slots) * slots);
+  };
+  if (s < 1) s = 1;
+  if (groups * s < slots) return int(s);
+  int64_t best = s;
+  for (int64_t ks = s; ks <= std::min(4 * s, smax) && fill(best) < 0.95; ++ks)
+    if (fill(ks) > fill(best) + 0.02) best = ks;
+  return int(best);
+}
+
 static int64_t env_int(const char *name, int64_t fallback) {
   const char *v = getenv(name);
   return v != nullptr && *v != 0 ? int64_t(atoll(v)) : fallback;
@@ -530,9 +547,10 @@ void gemm(Engine &eng, int precision, GemmArgs g, bool exact) {
   static const int64_t maxsplit = std::max<int64_t>(1, env_int("SISI_GEMM_SPLIT_MAX", 64));
   int ksplit = 1;
   if (g.K > 4 * kq && tiles * g.batch < want) {
-    int64_t s = (want + tiles * g.batch - 1) / (tiles * g.batch);
-    int64_t smax = std::max<int64_t>(1, g.K / (minsteps * kq));
-    ksplit = int(std::min<int64_t>(std::min<int64_t>(s, smax), maxsplit));
+    const int64_t tb = tiles * g.batch;
+    int64_t s = (want + tb - 1) / tb;
+    const int64_t smax = std::min<int64_t>(std::max<int64_t>(1, g.K / (minsteps * kq)), maxsplit);
+    ksplit = round_filling_split(tb, std::min(s, smax), smax, eng.cus);
   }
   int kchunk = (g.K + ksplit - 1) / ksplit;
   kchunk = ((kchunk + kq - 1) / kq) * kq;

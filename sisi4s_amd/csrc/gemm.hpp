@@ -111,6 +111,10 @@ void gemm_b3(Engine &eng, GemmArgs g, int passes = 3);
 void set_b3_backend(int library);
 int get_b3_backend();
 
+// Split-K factor from s (at most smax) chosen so that groups * split
+// workgroups fill their last round on a device of `cus` compute units.
+int round_filling_split(int64_t groups, int64_t s, int64_t smax, int cus);
+
 // Split-K reduction of g.work into C (deterministic order).
 void gemm_reduce(Engine &eng, const GemmArgs &g);
 

@@ -560,8 +560,8 @@ static void run_b3(Engine &e, GemmArgs g) {
   int ksplit = 1;
   if (Kp > 4 * 32 && tiles < 512) {
     const int64_t want = (512 + tiles - 1) / tiles;
-    ksplit = int(std::min<int64_t>(std::min<int64_t>(want, Kp / 256), 64));
-    if (ksplit < 1) ksplit = 1;
+    const int64_t smax = std::max<int64_t>(1, std::min<int64_t>(Kp / 256, 64));
+    ksplit = round_filling_split(tiles, std::min(want, smax), smax, e.cus);
   }
   int kchunk = ((Kp + ksplit - 1) / ksplit + 31) / 32 * 32;
   ksplit = (Kp + kchunk - 1) / kchunk;
