@@ -67,6 +67,7 @@ extern "C" {
 #define SISI_FORMULATION_VERTEX 0    /* CcsdEnergyFromCoulombIntegrals       */
 #define SISI_FORMULATION_INTEGRALS 1 /* CcsdEnergyFromCoulombIntegralsReference */
 #define SISI_FORMULATION_ONLY_PPL 2  /* ... its onlyPPL branch                */
+#define SISI_FORMULATION_DRCCD 3     /* DrccdEnergyFromCoulombIntegrals       */
 
 typedef struct sisi_engine sisi_engine;
 
@@ -199,7 +200,8 @@ int sisi_engine_set_coulomb_factors(sisi_engine *e, int NR, int factors_slice_si
                                     const double *pi_out);
 
 /* Upload one real integral class (SISI_PPHH, _PHPH, _HHHH, _HHHP, _PPPP,
- * _PPPH) instead of building it from a vertex: the inputs of
+ * _PPPH; _PHHP and _HHPP for the direct-ring CCD formulation) instead of
+ * building it from a vertex: the inputs of
  * CcsdEnergyFromCoulombIntegralsReference, which reads integrals only.
  * With PPHH, PHPH, HHHH and HHHP given the engine counts as having its
  * integrals (no vertex needed for the full-integral formulation). */
@@ -209,8 +211,20 @@ int sisi_engine_set_integrals(sisi_engine *e, int which, const double *V);
  * SISI_FORMULATION_VERTEX (default), SISI_FORMULATION_INTEGRALS
  * (CcsdEnergyFromCoulombIntegralsReference::getResiduum, cxx:29-295, stored
  * V_abcd and V_abci) or SISI_FORMULATION_ONLY_PPL (its onlyPPL branch,
- * cxx:56-75: R2 = V_abcd (T2 + T1 T1), R1 = 0, every iteration). */
+ * cxx:56-75: R2 = V_abcd (T2 + T1 T1), R1 = 0, every iteration) or
+ * SISI_FORMULATION_DRCCD (DrccdEnergyFromCoulombIntegrals::getResiduum,
+ * DrccdEnergyFromCoulombIntegrals.cxx:31-80: doubles only, from PPHH, PHHP
+ * and HHPP; needs only PPHH uploaded or a vertex). */
 int sisi_engine_set_formulation(sisi_engine *e, int formulation);
+
+/* Arguments of DrccdEnergyFromCoulombIntegrals (cxx:41-65): unrestricted
+ * (spins = 1 instead of 2, also in the energy of
+ * ClusterSinglesDoublesAlgorithm.cxx:133,163-170), linearized (no quadratic
+ * term) and adjacentPairsExchange (W_ijab = spins V_ijab - V_jiab).  Applies
+ * to SISI_FORMULATION_DRCCD; defaults 0, 0, 0.  Replaces the integer
+ * arguments read in DrccdEnergyFromCoulombIntegrals.cxx:41,47,63. */
+int sisi_engine_set_drccd(sisi_engine *e, int unrestricted, int linearized,
+                          int adjacent_pairs_exchange);
 
 /* Residual variant of the next residual calls: the ppl and distinguishable
  * arguments of CcsdEnergyFromCoulombIntegrals (cxx:123-124, 324-338);

@@ -191,6 +191,13 @@ int sisi_get_integrals(sisi_engine *h, int which, double *out) {
         e.download(out, t, o * o * v * v);
         break;
       }
+      case SISI_PHHP: {  // V_aijb = V_abji (CoulombIntegralsFromVertex.cxx:515)
+        DBuf t(&e.pool, v * o * o * v);
+        sum(e, 1.0, dense(e.Vabij.get(), {int64_t(v), int64_t(v), int64_t(o), int64_t(o)}), "abji",
+            0.0, dense(t.get(), {int64_t(v), int64_t(o), int64_t(o), int64_t(v)}), "aijb");
+        e.download(out, t, v * o * o * v);
+        break;
+      }
       default: throw Error(SISI_ERR_INVALID, "unknown integral class");
     }
   });
@@ -231,11 +238,20 @@ int sisi_engine_set_formulation(sisi_engine *h, int formulation) {
   return guarded([&] {
     ENGINE_REQUIRED(h);
     SISI_REQUIRE(formulation == SISI_FORMULATION_VERTEX || formulation == SISI_FORMULATION_INTEGRALS ||
-                     formulation == SISI_FORMULATION_ONLY_PPL,
+                     formulation == SISI_FORMULATION_ONLY_PPL ||
+                     formulation == SISI_FORMULATION_DRCCD,
                  SISI_ERR_INVALID, "unknown formulation");
     SISI_REQUIRE(formulation == SISI_FORMULATION_VERTEX || !h->e.cplx, SISI_ERR_INVALID,
                  "the full-integral formulation is built for the real path");
     h->e.formulation = formulation;
+  });
+}
+
+int sisi_engine_set_drccd(sisi_engine *h, int unrestricted, int linearized,
+                          int adjacent_pairs_exchange) {
+  return guarded([&] {
+    ENGINE_REQUIRED(h);
+    set_drccd(h->e, unrestricted, linearized, adjacent_pairs_exchange);
   });
 }
 

@@ -67,6 +67,8 @@ void integrals_from_vertex(Engine &e) {
 
 void ccsd_residual(Engine &e, int it, bool initial_doubles, int slice_size, const double *T1c,
                    const double *T2c, double *R1p, double *R2p) {
+  if (e.formulation == SISI_FORMULATION_DRCCD)
+    return drccd_residual(e, it, initial_doubles, T2c, R1p, R2p);
   if (e.cplx) return zccsd_residual(e, it, initial_doubles, slice_size, T1c, T2c, R1p, R2p);
   if (e.formulation != SISI_FORMULATION_VERTEX)
     return ccsd_residual_full(e, it, initial_doubles, T1c, T2c, R1p, R2p);
@@ -612,7 +614,7 @@ void ccsd_residual_anchor(Engine &e, int slice_size, const double *T1, const dou
 
 void ccsd_energy(Engine &e, const double *T1, const double *T2, double &E, double &direct,
                  double &exchange, double *E_imag) {
-  SISI_REQUIRE(e.have_integrals, SISI_ERR_STATE, "energy: integrals not built");
+  SISI_REQUIRE(integrals_ready(e), SISI_ERR_STATE, "energy: integrals not built");
   if (e.cplx) {  // F = complex: the real parts are reported (cxx:30-33, 185-189)
     double Ez[2], dz[2], xz[2];
     zccsd_energy(e, T1, T2, Ez, dz, xz);
@@ -623,12 +625,17 @@ void ccsd_energy(Engine &e, const double *T1, const double *T2, double &E, doubl
     return;
   }
   energy_terms(e, T1, T2, direct, exchange);
+  // direct 1/2 spins^2 tau V_ijab, exchange -1/2 spins tau V_ijba
+  // (ClusterSinglesDoublesAlgorithm.cxx:163-170); energy_terms uses spins = 2
+  direct *= 0.25 * e.spins * e.spins;
+  exchange *= 0.5 * e.spins;
   E = direct + exchange;
   if (E_imag) *E_imag = 0.0;
 }
 
 void ccsd_pair_energies(Engine &e, const double *T1, const double *T2, double *pair) {
-  SISI_REQUIRE(e.have_integrals, SISI_ERR_STATE, "pair energies: integrals not built");
+  SISI_REQUIRE(integrals_ready(e), SISI_ERR_STATE, "pair energies: integrals not built");
+  SISI_REQUIRE(e.spins == 2.0, SISI_ERR_INVALID, "pair energies are built for spins = 2");
   if (e.cplx) return zccsd_pair_energies(e, T1, T2, pair);
   SISI_REQUIRE(e.Uabij.get(), SISI_ERR_STATE, "pair energies: integrals not built");
   const int64_t o = e.No, v = e.Nv;
@@ -746,7 +753,7 @@ bool graph_residual(Engine &e, int slice, const double *T1, const double *T2, do
 }  // namespace
 
 void ccsd_begin(Engine &e, const sisi_ccsd_opts &opts, const double *T1h, const double *T2h) {
-  SISI_REQUIRE(e.have_integrals, SISI_ERR_STATE, "ccsd: integrals not built");
+  SISI_REQUIRE(integrals_ready(e), SISI_ERR_STATE, "ccsd: integrals not built");
   SISI_REQUIRE(opts.mixer == SISI_MIXER_LINEAR || opts.mixer == SISI_MIXER_DIIS,
                SISI_ERR_INVALID, "Mixer not implemented");
   SISI_REQUIRE(opts.mixer != SISI_MIXER_DIIS || opts.max_residua >= 1, SISI_ERR_INVALID,

@@ -57,6 +57,12 @@ void zfactor_ladder(Engine &e, const double *T1, const double *X, double *R2, in
 // 29-295): integral classes given directly, PPPP/PPPH built from the vertex
 // on request, and the residual from stored integrals.
 void set_integrals(Engine &e, int which, const double *V);
+// Direct-ring CCD (DrccdEnergyFromCoulombIntegrals.cxx:31-80)
+void set_drccd(Engine &e, int unrestricted, int linearized, int adjacent_pairs_exchange);
+void drccd_residual(Engine &e, int iteration, bool initial_doubles, const double *T2, double *R1,
+                    double *R2);
+// Integrals the current formulation needs are present.
+bool integrals_ready(const Engine &e);
 void build_vabcd(Engine &e);
 void ccsd_residual_full(Engine &e, int iteration, bool initial_doubles, const double *T1,
                         const double *T2, double *R1, double *R2);

@@ -30,6 +30,8 @@ void set_integrals(Engine &e, int which, const double *V) {
     case SISI_HHHP: dst = &e.Vijka; n = o * o * o * v; break;
     case SISI_PPPP: dst = &e.Vabcd; n = v * v * v * v; break;
     case SISI_PPPH: dst = &e.Vabci; n = v * v * v * o; break;
+    case SISI_PHHP: dst = &e.ring_Vaijb; n = v * o * o * v; break;
+    case SISI_HHPP: dst = &e.ring_Vijab; n = o * o * v * v; break;
     default: throw Error(SISI_ERR_INVALID, "unknown integral class");
   }
   for (int64_t k = 0; k < n; ++k)

@@ -151,6 +151,12 @@ class Engine {
   DBuf Vabcd, Vabci;
   unsigned given_integrals = 0;
   int formulation = 0;
+  // Direct-ring CCD (formulation SISI_FORMULATION_DRCCD): spins = 1 for
+  // `unrestricted`, the linearized and adjacentPairsExchange arguments, and
+  // uploaded PHHP / HHPP integrals (derived from PPHH when absent)
+  double spins = 2.0;
+  bool ring_linearized = false, ring_adjacent = false;
+  DBuf ring_Vaijb, ring_Vijab;
   int NR = 0, factors_slice = 0;
   bool factors_real = true;  // imaginary parts all zero: real products only
   DBuf Lam, PiF, PiO, VRS;  // PiO: OutgoingFactorOrbitals (complex amplitudes)

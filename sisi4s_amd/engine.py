@@ -27,7 +27,7 @@ FP64, FP32, BF16X3, BF16 = 0, 1, 2, 3
 PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16x3": BF16X3, "bf16": BF16}
 MIXER_LINEAR, MIXER_DIIS = 0, 1
 PPHH, PHPH, HHHH, HHHP, HHPP, PHHP, PHHH, PPPP, PPPH = 0, 1, 2, 3, 4, 5, 6, 7, 8
-FORMULATION_VERTEX, FORMULATION_INTEGRALS, FORMULATION_ONLY_PPL = 0, 1, 2
+FORMULATION_VERTEX, FORMULATION_INTEGRALS, FORMULATION_ONLY_PPL, FORMULATION_DRCCD = 0, 1, 2, 3
 
 _c_double_p = ctypes.POINTER(ctypes.c_double)
 _c_int_p = ctypes.POINTER(ctypes.c_int)
@@ -93,6 +93,7 @@ SIGNATURES = [
     ("sisi_engine_set_integrals", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _c_double_p]),
     ("sisi_set_bf16_backend", ctypes.c_int, [ctypes.c_int]),
     ("sisi_engine_set_formulation", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("sisi_engine_set_drccd", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     ("sisi_engine_set_coulomb_factors", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _c_double_p, _c_double_p, _c_double_p]),
     ("sisi_ccsd_residual", ctypes.c_int,
@@ -385,6 +386,13 @@ class Engine:
 
     def set_formulation(self, formulation):
         _check(self._lib.sisi_engine_set_formulation(self._h, int(formulation)))
+
+    def set_drccd(self, unrestricted=False, linearized=False, adjacent_pairs_exchange=False):
+        """Direct-ring CCD residual (DrccdEnergyFromCoulombIntegrals.cxx:31-80)
+        for the next residual calls and solves, with its arguments."""
+        _check(self._lib.sisi_engine_set_drccd(self._h, int(bool(unrestricted)), int(bool(linearized)),
+                                               int(bool(adjacent_pairs_exchange))))
+        self.set_formulation(FORMULATION_DRCCD)
 
     def set_residual_variant(self, ppl=True, distinguishable=False):
         """ppl / distinguishable of the next residual calls

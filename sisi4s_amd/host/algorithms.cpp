@@ -151,7 +151,7 @@ class CoulombIntegralsFromVertex : public Algorithm {
                    {"PPPHCoulombIntegrals", SISI_PPPH, {v, v, v, o}}};
     for (auto &c : classes) {
       if (!out_.count(c.name)) continue;
-      if (!cplx && (c.cls == SISI_PHHP || c.cls == SISI_PHHH))
+      if (!cplx && c.cls == SISI_PHHH)
         throw StepError(std::string(c.name) + " is built with complex: 1 only");
       if (cplx && (c.cls == SISI_PPPP || c.cls == SISI_PPPH))
         throw StepError(std::string(c.name) + " is built for real integrals only");
@@ -188,8 +188,12 @@ class CcsdEnergyFromCoulombIntegrals : public Algorithm {
  protected:
   // ClusterSinglesDoublesAlgorithm::run on engine e; vertex_arguments: the
   // CoulombFactors / ppl / distinguishable arguments of the vertex residual
-  void solve(sisi_engine *e, bool cplx, bool vertex_arguments) {
-    if (getIntegerArgument("unrestricted", 0) != 0 || getIntegerArgument("antisymmetrize", 0) != 0)
+  // abbreviation: prefix of the output names (getDataName, "Ccsd" / "Drccd");
+  // ring: the direct-ring CCD residual, which takes `unrestricted`
+  void solve(sisi_engine *e, bool cplx, bool vertex_arguments, const std::string &abbreviation = "Ccsd",
+             bool ring = false) {
+    if ((!ring && getIntegerArgument("unrestricted", 0) != 0) ||
+        getIntegerArgument("antisymmetrize", 0) != 0)
       throw StepError("unrestricted / antisymmetrized energies are not supported");
     if (isArgumentGiven("HPFockMatrix")) throw StepError("HPFockMatrix is not supported");
     const int64_t o = getTensorArgument("HoleEigenEnergies").lens[0];
@@ -291,14 +295,16 @@ class CcsdEnergyFromCoulombIntegrals : public Algorithm {
       check(sisi_ccsd_pair_energies(e, P->values.data()), "sisi_ccsd_pair_energies");
       allocatedTensorArgument("PairEnergy", P);
     }
-    if (isArgumentGiven("CcsdSinglesAmplitudes") || isArgumentGiven("CcsdDoublesAmplitudes")) {
+    const std::string singles = abbreviation + "SinglesAmplitudes",
+                      doubles = abbreviation + "DoublesAmplitudes";
+    if (isArgumentGiven(singles) || isArgumentGiven(doubles)) {
       auto T1 = make_tensor({v, o}, cplx), T2 = make_tensor({v, v, o, o}, cplx);
       check(sisi_ccsd_get_amplitudes(e, T1->values.data(), T2->values.data()),
             "sisi_ccsd_get_amplitudes");
-      allocatedTensorArgument("CcsdSinglesAmplitudes", T1);
-      allocatedTensorArgument("CcsdDoublesAmplitudes", T2);
+      allocatedTensorArgument(singles, T1);
+      allocatedTensorArgument(doubles, T2);
     }
-    setRealArgument("CcsdEnergy", r.energy);
+    setRealArgument(abbreviation + "Energy", r.energy);
   }
 };
 
@@ -349,6 +355,51 @@ class CcsdEnergyFromCoulombIntegralsReference : public CcsdEnergyFromCoulombInte
               e, only_ppl ? SISI_FORMULATION_ONLY_PPL : SISI_FORMULATION_INTEGRALS),
           "sisi_engine_set_formulation");
     solve(e, false, false);
+  }
+};
+
+// DrccdEnergyFromCoulombIntegrals (DrccdEnergyFromCoulombIntegrals.cxx:31-80)
+// on the same driver: the direct-ring CCD residual from the real PPHH, PHHP
+// and HHPP integrals, arguments unrestricted, linearized and
+// adjacentPairsExchange; outputs DrccdEnergy, DrccdSinglesAmplitudes and
+// DrccdDoublesAmplitudes.
+class DrccdEnergyFromCoulombIntegrals : public CcsdEnergyFromCoulombIntegrals {
+ public:
+  using CcsdEnergyFromCoulombIntegrals::CcsdEnergyFromCoulombIntegrals;
+  std::string getName() override { return "DrccdEnergyFromCoulombIntegrals"; }
+  void run() override {
+    const auto &ei = getTensorArgument("HoleEigenEnergies");
+    const auto &ea = getTensorArgument("ParticleEigenEnergies");
+    if (complex_integrals(*this))
+      throw StepError("DrccdEnergyFromCoulombIntegrals is built for real integrals");
+    const int64_t o = ei.lens[0], v = ea.lens[0];
+    sisi_dims d{int(o), int(v), 1};
+    struct Owned {
+      sisi_engine *e = nullptr;
+      ~Owned() { if (e) sisi_engine_destroy(e); }
+    } owned;
+    check(sisi_engine_create(&d, &owned.e), "sisi_engine_create");
+    sisi_engine *e = owned.e;
+    check(sisi_engine_set_eigenenergies(e, ei.values.data(), ea.values.data()),
+          "sisi_engine_set_eigenenergies");
+    const struct {
+      const char *name;
+      int cls;
+      std::vector<int64_t> lens;
+    } classes[] = {{"PPHHCoulombIntegrals", SISI_PPHH, {v, v, o, o}},   // cxx:36-38
+                   {"PHHPCoulombIntegrals", SISI_PHHP, {v, o, o, v}},
+                   {"HHPPCoulombIntegrals", SISI_HHPP, {o, o, v, v}}};
+    for (auto &c : classes) {
+      const auto &t = getTensorArgument(c.name);
+      if (t.complex || t.lens != c.lens) throw StepError(std::string(c.name) + " has the wrong shape");
+      check(sisi_engine_set_integrals(e, c.cls, t.values.data()), "sisi_engine_set_integrals");
+    }
+    check(sisi_engine_set_drccd(e, int(getIntegerArgument("unrestricted", 0)),
+                                int(getIntegerArgument("linearized", 0)),
+                                int(getIntegerArgument("adjacentPairsExchange", 0))),
+          "sisi_engine_set_drccd");
+    check(sisi_engine_set_formulation(e, SISI_FORMULATION_DRCCD), "sisi_engine_set_formulation");
+    solve(e, false, false, "Drccd", true);
   }
 };
 
@@ -417,5 +468,6 @@ static Registrar<CoulombIntegralsFromVertex> r1("CoulombIntegralsFromVertex");
 static Registrar<UegVertexGenerator> r4("UegVertexGenerator");
 static Registrar<CcsdEnergyFromCoulombIntegrals> r2("CcsdEnergyFromCoulombIntegrals");
 static Registrar<CcsdPerturbativeTriples> r3("CcsdPerturbativeTriples");
+static Registrar<DrccdEnergyFromCoulombIntegrals> r7("DrccdEnergyFromCoulombIntegrals");
 
 }  // namespace sisi_host

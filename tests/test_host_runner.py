@@ -172,3 +172,34 @@ def test_runner_full_integral_formulation(tmp_path):
     assert r.returncode == 0, r.stdout + r.stderr
     e = [float(m) for m in re.findall(r"CcsdEnergy = (\S+)", r.stdout)]
     assert len(e) == 2 and abs(e[0] - e[1]) <= 1e-10
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("unrestricted", [0, 1])
+def test_runner_drccd(tmp_path, unrestricted):
+    """DrccdEnergyFromCoulombIntegrals fed by CoulombIntegralsFromVertex's
+    PPHH, PHHP and HHPP: the oracle's direct-ring CCD energy."""
+    from oracle import ccsd, drccd, philox
+    p = tmp_path / "in.yaml"
+    p.write_text("""
+- name: GenerateSyntheticSystem
+  in: {seed: 5, No: 4, Nv: 12, NG: 36}
+  out: {CoulombVertex: $G, HoleEigenEnergies: $I, ParticleEigenEnergies: $A}
+- name: CoulombIntegralsFromVertex
+  in: {CoulombVertex: $G, HoleEigenEnergies: $I, ParticleEigenEnergies: $A}
+  out: {PPHHCoulombIntegrals: $PPHH, PHHPCoulombIntegrals: $PHHP, HHPPCoulombIntegrals: $HHPP}
+- name: DrccdEnergyFromCoulombIntegrals
+  in: {mixer: DiisMixer, maxResidua: 6, maxIterations: 30, unrestricted: %d,
+       adjacentPairsExchange: 1, HoleEigenEnergies: $I, ParticleEigenEnergies: $A,
+       PPHHCoulombIntegrals: $PPHH, PHHPCoulombIntegrals: $PHHP, HHPPCoulombIntegrals: $HHPP}
+  out: {DrccdEnergy: $E, DrccdDoublesAmplitudes: $T2}
+""" % unrestricted)
+    announce("runner drccd unrestricted=%d" % unrestricted)
+    r = run(["-i", str(p), "-o", str(tmp_path / "out.yaml")])
+    assert r.returncode == 0, r.stdout + r.stderr
+    m = re.search(r"DrccdEnergy = (\S+)", r.stdout)
+    g, ei, ea = philox.system(5, 4, 12, 36)
+    V = ccsd.integrals_from_vertex(g, 4)["PPHH"]
+    ref = drccd.solve(V, ei, ea, spins=1.0 if unrestricted else 2.0, adjacent_pairs_exchange=True,
+                      mixer="DiisMixer", max_residua=6, max_iterations=30)
+    assert m and abs(float(m.group(1)) - ref["energy"]) <= 1e-10, (r.stdout, ref["energy"])

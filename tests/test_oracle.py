@@ -456,3 +456,89 @@ def test_complex_factor_ladder_equals_complex_vertex_ladder():
     for fs in (1, 5, 16):
         F1, F2 = cz.residual(1, T1, T2, g, ints, 3, factors=(Lam, Pi, Po, fs))
         assert np.abs(F2 - R2).max() <= 1e-13 * np.abs(R2).max() and np.abs(F1 - R1).max() == 0.0
+
+
+# --- direct-ring CCD (DrccdEnergyFromCoulombIntegrals) ------------------------
+
+def _drccd_system(seed=31, No=3, Nv=8, NG=24):
+    from oracle import drccd
+    g, ei, ea = philox.system(seed, No, Nv, NG)
+    V = ccsd.integrals_from_vertex(g, No)["PPHH"]
+    return drccd, V, ei, ea
+
+
+def _ring_matrices(V, ei, ea):
+    """K[(a,i),(b,j)] = V_abij and the orbital-energy differences (a,i)."""
+    Nv, No = V.shape[0], V.shape[2]
+    K = np.transpose(V, (0, 2, 1, 3)).reshape(Nv * No, Nv * No)
+    delta = (ea[:, None] - ei[None, :]).reshape(-1)
+    return K, delta
+
+
+@pytest.mark.parametrize("spins", [2.0, 1.0])
+def test_drccd_direct_energy_is_the_rpa_correlation_energy(spins):
+    """Converged ring CCD gives the direct-RPA correlation energy
+    1/2 Tr(Omega - A), Omega^2 = eig((A-B)^1/2 (A+B) (A-B)^1/2), with
+    A = Delta + s K, B = s K (s = 2 restricted, 1 unrestricted)."""
+    drccd, V, ei, ea = _drccd_system()
+    r = drccd.solve(V, ei, ea, spins=spins, max_residua=6, max_iterations=200,
+                    amplitudes_convergence=1e-12, energy_convergence=1e-14)
+    assert r["converged"]
+    K, delta = _ring_matrices(V, ei, ea)
+    A = np.diag(delta) + spins * K
+    sq = np.sqrt(delta)
+    M = sq[:, None] * (np.diag(delta) + 2.0 * spins * K) * sq[None, :]
+    omega = np.sqrt(np.linalg.eigvalsh(M))
+    e_rpa = 0.5 * (omega.sum() - np.trace(A))
+    assert abs(r["direct"] - e_rpa) <= 1e-10 * abs(e_rpa), (r["direct"], e_rpa)
+    # a planted error in the quadratic term moves the energy off the closed form
+    Vb = V.copy()
+    r2 = drccd.solve(Vb, ei, ea, spins=spins, max_residua=6, max_iterations=200,
+                     amplitudes_convergence=1e-12, energy_convergence=1e-14,
+                     adjacent_pairs_exchange=True)
+    assert abs(r2["direct"] - e_rpa) > 1e-6 * abs(e_rpa)
+
+
+def test_linearized_drccd_solves_the_sylvester_equation():
+    """linearized = 1: K + A T + T A = 0 with A = Delta + 2K (matrix form over
+    compound indices (a,i)), solved independently by scipy."""
+    import scipy.linalg
+    drccd, V, ei, ea = _drccd_system(seed=32)
+    r = drccd.solve(V, ei, ea, linearized=True, max_residua=6, max_iterations=200,
+                    amplitudes_convergence=1e-12, energy_convergence=1e-14)
+    assert r["converged"]
+    K, delta = _ring_matrices(V, ei, ea)
+    A = np.diag(delta) + 2.0 * K
+    T = scipy.linalg.solve_sylvester(A, A, -K)
+    Nv, No = V.shape[0], V.shape[2]
+    Tm = np.transpose(r["T2"], (0, 2, 1, 3)).reshape(Nv * No, Nv * No)
+    assert np.abs(Tm - T).max() <= 1e-10 * np.abs(T).max()
+
+
+def test_drccd_adjacent_pairs_exchange_term_by_loops_and_mp2_start():
+    drccd, V, ei, ea = _drccd_system(seed=33, No=2, Nv=3, NG=7)
+    _, Vaijb, Vijab = drccd.integrals(V)
+    T1, T2 = random_amplitudes(3, 2, seed=4)
+    T2 = 0.5 * (T2 + np.transpose(T2, (1, 0, 3, 2)))
+    for spins in (2.0, 1.0):
+        _, Ra = drccd.residual(1, T2, V, Vaijb, Vijab, spins=spins, adjacent_pairs_exchange=True)
+        _, Rn = drccd.residual(1, T2, V, Vaijb, Vijab, spins=spins)
+        ref = np.zeros_like(Rn)
+        Nv, No = 3, 2
+        for a in range(Nv):
+            for b in range(Nv):
+                for i in range(No):
+                    for j in range(No):
+                        s = 0.0
+                        for k in range(No):
+                            for l in range(No):
+                                for c in range(Nv):
+                                    for d in range(Nv):
+                                        s += V[c, d, l, k] * T2[a, c, i, k] * T2[d, b, l, j]
+                        ref[a, b, i, j] = -spins * s
+        assert np.abs((Ra - Rn) - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max())
+    # iteration 0: MP2 start, energy = closed-form MP2
+    g, ei2, ea2 = philox.system(34, 3, 8, 24)
+    V2 = ccsd.integrals_from_vertex(g, 3)["PPHH"]
+    r = drccd.solve(V2, ei2, ea2, max_iterations=1)
+    assert abs(r["energy"] - formulations.mp2_closed_form(g, ei2, ea2)) <= 1e-12
