@@ -68,6 +68,7 @@ extern "C" {
 #define SISI_FORMULATION_INTEGRALS 1 /* CcsdEnergyFromCoulombIntegralsReference */
 #define SISI_FORMULATION_ONLY_PPL 2  /* ... its onlyPPL branch                */
 #define SISI_FORMULATION_DRCCD 3     /* DrccdEnergyFromCoulombIntegrals       */
+#define SISI_FORMULATION_UCCSD 4     /* UccsdAmplitudesFromCoulombIntegrals   */
 
 typedef struct sisi_engine sisi_engine;
 
@@ -214,7 +215,13 @@ int sisi_engine_set_integrals(sisi_engine *e, int which, const double *V);
  * cxx:56-75: R2 = V_abcd (T2 + T1 T1), R1 = 0, every iteration) or
  * SISI_FORMULATION_DRCCD (DrccdEnergyFromCoulombIntegrals::getResiduum,
  * DrccdEnergyFromCoulombIntegrals.cxx:31-80: doubles only, from PPHH, PHHP
- * and HHPP; needs only PPHH uploaded or a vertex). */
+ * and HHPP; needs only PPHH uploaded or a vertex) or SISI_FORMULATION_UCCSD
+ * (UccsdAmplitudesFromCoulombIntegrals::getResiduumTemplate,
+ * UccsdAmplitudesFromCoulombIntegrals.cxx:107-241: spin-orbital CCSD on an
+ * engine whose No, Nv count spin orbitals, from the antisymmetrized classes
+ * <pq||rs> PPHH, HHHH, PPPP, HHHP, PPPH and PHPH uploaded with
+ * sisi_engine_set_integrals; energy 1/4 T2 V + 1/2 T1 T1 V, reported as
+ * `direct`, exchange 0). */
 int sisi_engine_set_formulation(sisi_engine *e, int formulation);
 
 /* Arguments of DrccdEnergyFromCoulombIntegrals (cxx:41-65): unrestricted

@@ -69,6 +69,8 @@ void ccsd_residual(Engine &e, int it, bool initial_doubles, int slice_size, cons
                    const double *T2c, double *R1p, double *R2p) {
   if (e.formulation == SISI_FORMULATION_DRCCD)
     return drccd_residual(e, it, initial_doubles, T2c, R1p, R2p);
+  if (e.formulation == SISI_FORMULATION_UCCSD)
+    return uccsd_residual(e, it, initial_doubles, T1c, T2c, R1p, R2p);
   if (e.cplx) return zccsd_residual(e, it, initial_doubles, slice_size, T1c, T2c, R1p, R2p);
   if (e.formulation != SISI_FORMULATION_VERTEX)
     return ccsd_residual_full(e, it, initial_doubles, T1c, T2c, R1p, R2p);
@@ -624,6 +626,12 @@ void ccsd_energy(Engine &e, const double *T1, const double *T2, double &E, doubl
     if (E_imag) *E_imag = Ez[1];
     return;
   }
+  if (e.formulation == SISI_FORMULATION_UCCSD) {  // antisymmetrized (cxx:163-167)
+    E = direct = uccsd_energy(e, T1, T2);
+    exchange = 0.0;
+    if (E_imag) *E_imag = 0.0;
+    return;
+  }
   energy_terms(e, T1, T2, direct, exchange);
   // direct 1/2 spins^2 tau V_ijab, exchange -1/2 spins tau V_ijba
   // (ClusterSinglesDoublesAlgorithm.cxx:163-170); energy_terms uses spins = 2
@@ -635,7 +643,8 @@ void ccsd_energy(Engine &e, const double *T1, const double *T2, double &E, doubl
 
 void ccsd_pair_energies(Engine &e, const double *T1, const double *T2, double *pair) {
   SISI_REQUIRE(integrals_ready(e), SISI_ERR_STATE, "pair energies: integrals not built");
-  SISI_REQUIRE(e.spins == 2.0, SISI_ERR_INVALID, "pair energies are built for spins = 2");
+  SISI_REQUIRE(e.spins == 2.0 && e.formulation != SISI_FORMULATION_UCCSD, SISI_ERR_INVALID,
+               "pair energies are built for the restricted closed-shell energy");
   if (e.cplx) return zccsd_pair_energies(e, T1, T2, pair);
   SISI_REQUIRE(e.Uabij.get(), SISI_ERR_STATE, "pair energies: integrals not built");
   const int64_t o = e.No, v = e.Nv;

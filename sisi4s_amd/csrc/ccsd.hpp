@@ -63,6 +63,11 @@ void drccd_residual(Engine &e, int iteration, bool initial_doubles, const double
                     double *R2);
 // Integrals the current formulation needs are present.
 bool integrals_ready(const Engine &e);
+// Spin-orbital CCSD (UccsdAmplitudesFromCoulombIntegrals.cxx:107-241)
+bool uccsd_ready(const Engine &e);
+void uccsd_residual(Engine &e, int iteration, bool initial_doubles, const double *T1,
+                    const double *T2, double *R1, double *R2);
+double uccsd_energy(Engine &e, const double *T1, const double *T2);
 void build_vabcd(Engine &e);
 void ccsd_residual_full(Engine &e, int iteration, bool initial_doubles, const double *T1,
                         const double *T2, double *R1, double *R2);

@@ -20,6 +20,7 @@ void set_drccd(Engine &e, int unrestricted, int linearized, int adjacent_pairs_e
 }
 
 bool integrals_ready(const Engine &e) {
+  if (e.formulation == SISI_FORMULATION_UCCSD) return uccsd_ready(e);
   if (e.formulation == SISI_FORMULATION_DRCCD)
     return !e.cplx && e.Vabij.get() != nullptr && e.Uabij.get() != nullptr;
   return e.have_integrals;

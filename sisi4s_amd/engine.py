@@ -28,6 +28,7 @@ PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16x3": BF16X3, "bf16": BF16}
 MIXER_LINEAR, MIXER_DIIS = 0, 1
 PPHH, PHPH, HHHH, HHHP, HHPP, PHHP, PHHH, PPPP, PPPH = 0, 1, 2, 3, 4, 5, 6, 7, 8
 FORMULATION_VERTEX, FORMULATION_INTEGRALS, FORMULATION_ONLY_PPL, FORMULATION_DRCCD = 0, 1, 2, 3
+FORMULATION_UCCSD = 4
 
 _c_double_p = ctypes.POINTER(ctypes.c_double)
 _c_int_p = ctypes.POINTER(ctypes.c_int)

@@ -542,3 +542,63 @@ def test_drccd_adjacent_pairs_exchange_term_by_loops_and_mp2_start():
     V2 = ccsd.integrals_from_vertex(g, 3)["PPHH"]
     r = drccd.solve(V2, ei2, ea2, max_iterations=1)
     assert abs(r["energy"] - formulations.mp2_closed_form(g, ei2, ea2)) <= 1e-12
+
+
+# --- spin-orbital CCSD (UccsdAmplitudesFromCoulombIntegrals) ------------------
+
+def test_uccsd_closed_shell_equals_restricted_ccsd():
+    """A closed-shell system written in spin orbitals: iteration 1 is MP2 and
+    the converged energy is the restricted CCSD energy of the pinned real path."""
+    from oracle import uccsd
+    No, Nv, NG = 2, 5, 14
+    g, ei, ea = philox.system(41, No, Nv, NG)
+    ints = uccsd.spin_orbital_integrals(g, No, Nv)
+    kw = dict(max_residua=6, max_iterations=200, amplitudes_convergence=1e-11,
+              energy_convergence=1e-13)
+    ru = uccsd.solve(ints, np.repeat(ei, 2), np.repeat(ea, 2), **kw)
+    rr = ccsd.solve(g, ei, ea, **kw)
+    assert ru["converged"] and rr["converged"]
+    assert abs(ru["energies"][0] - formulations.mp2_closed_form(g, ei, ea)) <= 1e-13
+    assert abs(ru["energy"] - rr["energy"]) <= 1e-10 * abs(rr["energy"])
+
+
+def _two_electron_fci(asym, eps, no=2):
+    """Full-CI correlation energy of a two-electron Hamiltonian whose Fock
+    matrix with respect to the first `no` = 2 spin orbitals is diag(eps)."""
+    n = asym.shape[0]
+    f = np.diag(eps)
+    h = f - np.einsum("pkqk->pq", asym[:, :no, :, :no])
+    dets = [(p, q) for p in range(n) for q in range(p + 1, n)]
+    H = np.zeros((len(dets), len(dets)))
+    for I, (p, q) in enumerate(dets):
+        for J, (r, s) in enumerate(dets):
+            H[I, J] = (h[p, r] * (q == s) - h[p, s] * (q == r) - h[q, r] * (p == s)
+                       + h[q, s] * (p == r) + asym[p, q, r, s])
+    e_ref = h[0, 0] + h[1, 1] + asym[0, 1, 0, 1]
+    return np.linalg.eigvalsh(H)[0] - e_ref
+
+
+def test_uccsd_is_exact_for_two_electrons():
+    """Generic (spin-symmetry-broken) real spin-orbital integrals, two
+    electrons: the converged energy equals full CI from Slater-Condon rules
+    and an exact diagonalisation; a planted error in one term is detected."""
+    from oracle import uccsd
+    rng = np.random.default_rng(7)
+    n, NG = 8, 12
+    g = rng.normal(size=(NG, n, n)) * 0.25
+    g = 0.5 * (g + np.transpose(g, (0, 2, 1)))
+    eri = np.einsum("Gpr,Gqs->pqrs", g, g)
+    asym = eri - np.swapaxes(eri, 2, 3)
+    eps = np.concatenate([[-1.1, -0.8], np.sort(rng.uniform(0.3, 2.0, n - 2))])
+    ints = uccsd.classes(asym, 2)
+    r = uccsd.solve(ints, eps[:2], eps[2:], max_residua=6, max_iterations=300,
+                    amplitudes_convergence=1e-11, energy_convergence=1e-13)
+    assert r["converged"]
+    e_fci = _two_electron_fci(asym, eps)
+    assert abs(r["energy"] - e_fci) <= 1e-10 * abs(e_fci), (r["energy"], e_fci)
+    # plant: drop the T1 <na||if> term's sign (a plausible index mistake)
+    bad = dict(ints)
+    bad["PHPH"] = -ints["PHPH"]
+    rb = uccsd.solve(bad, eps[:2], eps[2:], max_residua=6, max_iterations=300,
+                     amplitudes_convergence=1e-11, energy_convergence=1e-13)
+    assert abs(rb["energy"] - e_fci) > 1e-6
