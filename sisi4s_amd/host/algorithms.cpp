@@ -188,12 +188,13 @@ class CcsdEnergyFromCoulombIntegrals : public Algorithm {
  protected:
   // ClusterSinglesDoublesAlgorithm::run on engine e; vertex_arguments: the
   // CoulombFactors / ppl / distinguishable arguments of the vertex residual
-  // abbreviation: prefix of the output names (getDataName, "Ccsd" / "Drccd");
-  // ring: the direct-ring CCD residual, which takes `unrestricted`
+  // abbreviation: prefix of the output names (getDataName: "Ccsd", "Drccd",
+  // "Uccsd"); kind: 0 restricted CCSD, 1 direct-ring CCD (takes
+  // `unrestricted`), 2 spin-orbital CCSD (antisymmetrized energy)
   void solve(sisi_engine *e, bool cplx, bool vertex_arguments, const std::string &abbreviation = "Ccsd",
-             bool ring = false) {
-    if ((!ring && getIntegerArgument("unrestricted", 0) != 0) ||
-        getIntegerArgument("antisymmetrize", 0) != 0)
+             int kind = 0) {
+    if ((kind == 0 && getIntegerArgument("unrestricted", 0) != 0) ||
+        (kind != 2 && getIntegerArgument("antisymmetrize", 0) != 0))
       throw StepError("unrestricted / antisymmetrized energies are not supported");
     if (isArgumentGiven("HPFockMatrix")) throw StepError("HPFockMatrix is not supported");
     const int64_t o = getTensorArgument("HoleEigenEnergies").lens[0];
@@ -285,11 +286,14 @@ class CcsdEnergyFromCoulombIntegrals : public Algorithm {
     emitted.scalars.push_back({"converged", r.converged ? "true" : "false"});
     emitted.scalars.push_back({"fp64Iterations", std::to_string(r.fp64_iterations)});
     emitted.scalars.push_back({"energy", num(r.energy)});
-    // energy decomposition of getEnergy (ClusterSinglesDoublesAlgorithm.cxx:180-190)
-    emitted.scalars.push_back({"direct", num(r.direct)});
-    emitted.scalars.push_back({"exchange", num(r.exchange)});
-    emitted.scalars.push_back({"singlet", num(0.25 * r.direct - 0.5 * r.exchange)});
-    emitted.scalars.push_back({"triplet", num(0.75 * r.direct + 1.5 * r.exchange)});
+    // energy decomposition of getEnergy (ClusterSinglesDoublesAlgorithm.cxx:180-190),
+    // not emitted by its antisymmetrized branch
+    if (kind != 2) {
+      emitted.scalars.push_back({"direct", num(r.direct)});
+      emitted.scalars.push_back({"exchange", num(r.exchange)});
+      emitted.scalars.push_back({"singlet", num(0.25 * r.direct - 0.5 * r.exchange)});
+      emitted.scalars.push_back({"triplet", num(0.75 * r.direct + 1.5 * r.exchange)});
+    }
     if (isArgumentGiven("PairEnergy")) {  // cxx:218-235
       auto P = make_tensor({o, o}, cplx);
       check(sisi_ccsd_pair_energies(e, P->values.data()), "sisi_ccsd_pair_energies");
@@ -399,7 +403,56 @@ class DrccdEnergyFromCoulombIntegrals : public CcsdEnergyFromCoulombIntegrals {
                                 int(getIntegerArgument("adjacentPairsExchange", 0))),
           "sisi_engine_set_drccd");
     check(sisi_engine_set_formulation(e, SISI_FORMULATION_DRCCD), "sisi_engine_set_formulation");
-    solve(e, false, false, "Drccd", true);
+    solve(e, false, false, "Drccd", 1);
+  }
+};
+
+// UccsdAmplitudesFromCoulombIntegrals (UccsdAmplitudesFromCoulombIntegrals.cxx:
+// 107-241): spin-orbital CCSD on the same driver from antisymmetrized real
+// integrals, canonical orbitals (HoleEigenEnergies / ParticleEigenEnergies
+// of the spin orbitals); reads the PPHH, HHHH, PPPP, HHHP, PPPH and PHPH
+// classes (the reference's other classes are these with indices reordered);
+// outputs UccsdEnergy, UccsdSinglesAmplitudes, UccsdDoublesAmplitudes.
+class UccsdAmplitudesFromCoulombIntegrals : public CcsdEnergyFromCoulombIntegrals {
+ public:
+  using CcsdEnergyFromCoulombIntegrals::CcsdEnergyFromCoulombIntegrals;
+  std::string getName() override { return "UccsdAmplitudesFromCoulombIntegrals"; }
+  void run() override {
+    if (getIntegerArgument("onlyPPL", 0) == 1) throw StepError("onlyPPL is not supported for UCCSD");
+    if (isArgumentGiven("HPFockMatrix") || isArgumentGiven("HHFockMatrix") ||
+        isArgumentGiven("PPFockMatrix"))
+      throw StepError("non-canonical orbitals (Fock matrices) are not supported");
+    const auto &ei = getTensorArgument("HoleEigenEnergies");
+    const auto &ea = getTensorArgument("ParticleEigenEnergies");
+    if (complex_integrals(*this))
+      throw StepError("UccsdAmplitudesFromCoulombIntegrals is built for real integrals");
+    const int64_t o = ei.lens[0], v = ea.lens[0];
+    sisi_dims d{int(o), int(v), 1};
+    struct Owned {
+      sisi_engine *e = nullptr;
+      ~Owned() { if (e) sisi_engine_destroy(e); }
+    } owned;
+    check(sisi_engine_create(&d, &owned.e), "sisi_engine_create");
+    sisi_engine *e = owned.e;
+    check(sisi_engine_set_eigenenergies(e, ei.values.data(), ea.values.data()),
+          "sisi_engine_set_eigenenergies");
+    const struct {
+      const char *name;
+      int cls;
+      std::vector<int64_t> lens;
+    } classes[] = {{"PPHHCoulombIntegrals", SISI_PPHH, {v, v, o, o}},   // cxx:138, 167-180
+                   {"HHHHCoulombIntegrals", SISI_HHHH, {o, o, o, o}},
+                   {"PPPPCoulombIntegrals", SISI_PPPP, {v, v, v, v}},
+                   {"HHHPCoulombIntegrals", SISI_HHHP, {o, o, o, v}},
+                   {"PPPHCoulombIntegrals", SISI_PPPH, {v, v, v, o}},
+                   {"PHPHCoulombIntegrals", SISI_PHPH, {v, o, v, o}}};
+    for (auto &c : classes) {
+      const auto &t = getTensorArgument(c.name);
+      if (t.complex || t.lens != c.lens) throw StepError(std::string(c.name) + " has the wrong shape");
+      check(sisi_engine_set_integrals(e, c.cls, t.values.data()), "sisi_engine_set_integrals");
+    }
+    check(sisi_engine_set_formulation(e, SISI_FORMULATION_UCCSD), "sisi_engine_set_formulation");
+    solve(e, false, false, "Uccsd", 2);
   }
 };
 
@@ -469,5 +522,6 @@ static Registrar<UegVertexGenerator> r4("UegVertexGenerator");
 static Registrar<CcsdEnergyFromCoulombIntegrals> r2("CcsdEnergyFromCoulombIntegrals");
 static Registrar<CcsdPerturbativeTriples> r3("CcsdPerturbativeTriples");
 static Registrar<DrccdEnergyFromCoulombIntegrals> r7("DrccdEnergyFromCoulombIntegrals");
+static Registrar<UccsdAmplitudesFromCoulombIntegrals> r8("UccsdAmplitudesFromCoulombIntegrals");
 
 }  // namespace sisi_host

@@ -331,3 +331,41 @@ def test_runner_factor_ladder_from_files(tmp_path):
     ref = ccsd.solve(g, ei, ea, mixer="DiisMixer", max_residua=6, max_iterations=15, slice_size=5,
                      factors=(Lz, Pz, int(NR * 0.4)))
     assert abs(vals["CcsdEnergy"][0] - ref["energy"]) <= 1e-10
+
+
+@pytest.mark.gpu
+def test_uccsd_from_files(tmp_path):
+    """UccsdAmplitudesFromCoulombIntegrals on antisymmetrized spin-orbital
+    integral classes read from cc4s-format files (a closed-shell system in
+    spin orbitals): the oracle's per-iteration energies, and at convergence
+    the restricted CCSD energy."""
+    from oracle import ccsd, uccsd
+    No, Nv = 2, 5
+    g, ei, ea = philox.system(17, No, Nv, 14)
+    ints = uccsd.spin_orbital_integrals(g, No, Nv)
+    eps = np.concatenate([np.repeat(ei, 2), np.repeat(ea, 2)])
+    (tmp_path / "EigenEnergies.yaml").write_text(
+        "metaData:\n  fermiEnergy: %.17g\n  energies:\n%s\n"
+        % (0.5 * (ei.max() + ea.min()), "\n".join("    - %.17g" % v for v in eps)))
+    steps = ["- name: DefineHolesAndParticles\n  in: {fileName: \"%s\"}\n"
+             "  out: {HoleEigenEnergies: $I, ParticleEigenEnergies: $A}\n"
+             % (tmp_path / "EigenEnergies.yaml")]
+    for c in ("PPHH", "HHHH", "PPPP", "HHHP", "PPPH", "PHPH"):
+        write_cc4s_tensor(tmp_path / c, ints[c])
+        steps.append("- name: Read\n  in: {fileName: \"%s\"}\n  out: {destination: $%s}\n"
+                     % (tmp_path / (c + ".yaml"), c))
+    steps.append("""- name: UccsdAmplitudesFromCoulombIntegrals
+  in: {mixer: DiisMixer, maxResidua: 6, maxIterations: 100, amplitudesConvergence: 1e-10,
+       energyConvergence: 1e-12, HoleEigenEnergies: $I, ParticleEigenEnergies: $A,
+       PPHHCoulombIntegrals: $PPHH, HHHHCoulombIntegrals: $HHHH, PPPPCoulombIntegrals: $PPPP,
+       HHHPCoulombIntegrals: $HHHP, PPPHCoulombIntegrals: $PPPH, PHPHCoulombIntegrals: $PHPH}
+  out: {UccsdEnergy: $E, UccsdDoublesAmplitudes: $T2}
+""")
+    r, vals = run_steps(tmp_path, "".join(steps))
+    assert r.returncode == 0, r.stdout + r.stderr
+    kw = dict(max_residua=6, max_iterations=100, amplitudes_convergence=1e-10,
+              energy_convergence=1e-12)
+    ref = uccsd.solve(ints, np.repeat(ei, 2), np.repeat(ea, 2), **kw)
+    rr = ccsd.solve(g, ei, ea, **kw)
+    E = vals["UccsdEnergy"][-1]
+    assert abs(E - ref["energy"]) <= 1e-10 and abs(E - rr["energy"]) <= 1e-9 * abs(rr["energy"])
