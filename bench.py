@@ -81,7 +81,11 @@ def main():
                          "complex CoulombFactors / FactorOrbitals (factor slices of Nv)")
     ap.add_argument("--triples-blocks", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--mode", choices=["iteration", "converge", "batched", "complex"],
+    ap.add_argument("--uccsd-no", type=int, default=10,
+                    help="consumers: occupied spatial orbitals of the spin-orbital CCSD system")
+    ap.add_argument("--uccsd-nv", type=int, default=60,
+                    help="consumers: virtual spatial orbitals of the spin-orbital CCSD system")
+    ap.add_argument("--mode", choices=["iteration", "converge", "batched", "complex", "consumers"],
                     default="iteration",
                     help="iteration: timed FP64 iterations (default line); converge: FP64 and "
                          "mixed runs to |dT|/|T| <= 1e-8 with (T); batched: 64 small systems")
@@ -116,6 +120,8 @@ def main():
         return converge(args)
     if args.mode == "batched":
         return batched(args)
+    if args.mode == "consumers":
+        return consumers(args)
     if args.mode == "complex":
         return complex_path(args)
 
@@ -295,6 +301,97 @@ def complex_vertex(g, No, seed):
     z *= t.conj()[None, :, None]
     z *= t[None, None, :]
     return z
+
+
+def spin_orbital_classes(g, No, Nv):
+    """Antisymmetrized spin-orbital classes <pq||rs> (PPHH, HHHH, PPPP, HHHP,
+    PPPH, PHPH) of a closed-shell real vertex; spin orbital 2 P + s inside
+    the hole and particle blocks."""
+    h = np.repeat(np.arange(No), 2)
+    p = np.repeat(np.arange(No, No + Nv), 2)
+    sh, sp = np.tile([0, 1], No), np.tile([0, 1], Nv)
+    idx = {"H": (h, sh), "P": (p, sp)}
+
+    def block(cls):
+        (a, sa), (b, sb), (c, sc), (d, sd) = (idx[x] for x in cls)
+
+        def coulomb(i, j, k, l):  # <ij|kl> = (ik|jl) on spatial indices
+            ui, uj, uk, ul = (np.unique(x) for x in (i, j, k, l))
+            S = np.einsum("Gpr,Gqs->pqrs", g[:, ui][:, :, uk], g[:, uj][:, :, ul], optimize=True)
+            return S[np.ix_(np.searchsorted(ui, i), np.searchsorted(uj, j),
+                            np.searchsorted(uk, k), np.searchsorted(ul, l))]
+        direct = coulomb(a, b, c, d) * ((sa[:, None, None, None] == sc[None, None, :, None])
+                                        & (sb[None, :, None, None] == sd[None, None, None, :]))
+        exch = coulomb(a, b, d, c).transpose(0, 1, 3, 2) * (
+            (sa[:, None, None, None] == sd[None, None, None, :])
+            & (sb[None, :, None, None] == sc[None, None, :, None]))
+        return direct - exch
+    return {c: block(c) for c in ("PPHH", "HHHH", "PPPP", "HHHP", "PPPH", "PHPH")}
+
+
+def consumers(args):
+    """Direct-ring CCD (DrccdEnergyFromCoulombIntegrals) at the given size and
+    spin-orbital CCSD (UccsdAmplitudesFromCoulombIntegrals) on a closed-shell
+    system in spin orbitals: FP64 iteration time and the GEMM family's rate."""
+    from sisi4s_amd import engine as eng
+    log = lambda m: print("[bench] " + m, file=sys.stderr, flush=True)  # noqa: E731
+    out = {"metric": "iteration time (s), FP64: direct-ring CCD and spin-orbital CCSD",
+           "mode": "consumers"}
+
+    def timed(e, label, flops):
+        opts = eng.default_opts(mixer="DiisMixer", max_residua=6, max_iterations=10000,
+                                amplitudes_convergence=1e-8, energy_convergence=1e-10)
+        e.ccsd_begin(opts)
+        for k in range(args.warmup):
+            e.ccsd_iterate(1)
+        e.enable_stats(True)
+        e.reset_stats()
+        times = []
+        for k in range(args.steps):
+            t0 = time.perf_counter()
+            rep = e.ccsd_iterate(1)
+            times.append(time.perf_counter() - t0)
+            log("%s iteration %d: %.3fs E=%.10f" % (label, k + 1, times[-1], rep["energy"]))
+        st = e.stats()
+        e.enable_stats(False)
+        t = float(np.mean(times))
+        gs, gf, gn = st["gemm_seconds"][0], st["gemm_flops"][0], st["gemm_launches"][0]
+        return dict(seconds_per_iteration=t, algorithmic_flops=flops,
+                    tflops=flops / t / 1e12 if flops else None,
+                    gemm_tflops=gf / gs / 1e12 if gs > 0 else None,
+                    gemm_frac_fp64_peak=(gf / gs / 1e12) / PEAK_FP64_TFLOPS if gs > 0 else None,
+                    gemm_launches_per_iteration=gn / args.steps,
+                    gemm_avg_launch_ms=1e3 * gs / max(gn, 1),
+                    energy=rep["energy"])
+
+    No, Nv, NG = args.No, args.Nv, args.NG
+    g, ei, ea = eng.generate_system(args.seed, No, Nv, NG)
+    e = eng.Engine(No, Nv, NG)
+    e.set_system(g, ei, ea)
+    del g
+    e.set_drccd()
+    nvo = float(Nv * No)
+    out["drccd"] = dict(config="No=%d Nv=%d NG=%d, vertex integrals" % (No, Nv, NG),
+                        **timed(e, "drccd", 4 * 2.0 * nvo ** 3))
+    del e
+    o, v = args.uccsd_no, args.uccsd_nv
+    gs_, ei_, ea_ = eng.generate_system(args.seed + 1, o, v, 4 * v)
+    t0 = time.perf_counter()
+    ints = spin_orbital_classes(gs_, o, v)
+    log("spin-orbital integrals %.1fs" % (time.perf_counter() - t0))
+    u = eng.Engine(2 * o, 2 * v, 1)
+    u.set_eigenenergies(np.repeat(ei_, 2), np.repeat(ea_, 2))
+    for c, V in ints.items():
+        u.set_integrals(getattr(eng, c), V)
+    del ints
+    u.set_formulation(eng.FORMULATION_UCCSD)
+    O, W = 2 * o, 2 * v
+    # dominant products: particle ladder 2 V^4 O^2, ring 2 O^3 V^3 (W_mbej T2),
+    # and the tau <am||ef> product 2 V^3 O^3
+    out["uccsd"] = dict(config="spin orbitals No=%d Nv=%d (closed shell No=%d Nv=%d, NG=%d)"
+                        % (O, W, o, v, 4 * v),
+                        **timed(u, "uccsd", 2.0 * W ** 4 * O ** 2 + 4.0 * O ** 3 * W ** 3))
+    print(json.dumps(out), flush=True)
 
 
 def complex_path(args):
