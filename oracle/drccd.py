@@ -46,21 +46,31 @@ def residual(it, T2, Vabij, Vaijb, Vijab, *, spins=2.0, linearized=False,
     return R1, R2
 
 
-def energy(T2, Vabij, spins=2.0):
+def energy(T2, Vabij, spins=2.0, Vijab=None):
     """getEnergy (cxx:131-190) with T1 = 0: direct 1/2 spins^2 T V_ijab,
-    exchange -1/2 spins T V_ijba.  Returns (E, direct, exchange)."""
-    direct = 0.5 * spins * spins * np.sum(T2 * Vabij)
-    exchange = -0.5 * spins * np.sum(T2 * ein("baij->abij", Vabij))
+    exchange -1/2 spins T V_ijba, V_ijab = HHPP (V_abij^(ijab) for a real
+    vertex).  Returns (E, direct, exchange), complex for complex integrals."""
+    if Vijab is None:
+        Vijab = ein("abij->ijab", Vabij)
+    direct = 0.5 * spins * spins * ein("abij,ijab->", T2, Vijab)
+    exchange = -0.5 * spins * ein("abij,ijba->", T2, Vijab)
     return direct + exchange, direct, exchange
 
 
 def solve(Vabij, eps_i, eps_a, *, spins=2.0, linearized=False, adjacent_pairs_exchange=False,
           mixer="DiisMixer", max_residua=4, mixing_ratio=1.0, max_iterations=16,
-          amplitudes_convergence=1e-5, energy_convergence=1e-6, level_shift=0.0):
-    """ClusterSinglesDoublesAlgorithm::run (cxx:38-128) on the DRCCD residual."""
+          amplitudes_convergence=1e-5, energy_convergence=1e-6, level_shift=0.0,
+          complex_integrals=None):
+    """ClusterSinglesDoublesAlgorithm::run (cxx:38-128) on the DRCCD residual;
+    complex_integrals = (V_abij, V_aijb, V_ijab) of the complex path (F =
+    complex: complex energies, |dE/E| of the complex energies)."""
     No, Nv = eps_i.shape[0], eps_a.shape[0]
-    V, Vaijb, Vijab = integrals(Vabij)
-    T = [np.zeros((Nv, No)), np.zeros((Nv, Nv, No, No))]
+    if complex_integrals is not None:
+        V, Vaijb, Vijab = complex_integrals
+    else:
+        V, Vaijb, Vijab = integrals(Vabij)
+    dt = complex if np.iscomplexobj(V) else float
+    T = [np.zeros((Nv, No), dt), np.zeros((Nv, Nv, No, No), dt)]
     mix = ccsd.DiisMixer(max_residua) if mixer == "DiisMixer" else ccsd.LinearMixer(mixing_ratio)
     prev, energies, converged, i = 0.0, [], False, 0
     parts = (0.0, 0.0, 0.0)
@@ -71,7 +81,7 @@ def solve(Vabij, eps_i, eps_a, *, spins=2.0, linearized=False, adjacent_pairs_ex
         dT = [E1 - T[0], E2 - T[1]]
         mix.append([E1, E2], dT)
         T = mix.get()
-        parts = energy(T[1], V, spins)
+        parts = energy(T[1], V, spins, Vijab)
         e = parts[0]
         ratio = ccsd.fock_dot(dT, dT) / ccsd.fock_dot(T, T)
         energies.append(e)
@@ -81,5 +91,6 @@ def solve(Vabij, eps_i, eps_a, *, spins=2.0, linearized=False, adjacent_pairs_ex
             converged = True
             break
         prev = e
-    return dict(energy=parts[0], direct=parts[1], exchange=parts[2], T2=T[1],
+    return dict(energy=np.real(parts[0]), energy_complex=parts[0], direct=np.real(parts[1]),
+                exchange=np.real(parts[2]), T2=T[1],
                 energies=energies, iterations=i, converged=converged)

@@ -241,8 +241,9 @@ int sisi_engine_set_formulation(sisi_engine *h, int formulation) {
                      formulation == SISI_FORMULATION_ONLY_PPL ||
                      formulation == SISI_FORMULATION_DRCCD || formulation == SISI_FORMULATION_UCCSD,
                  SISI_ERR_INVALID, "unknown formulation");
-    SISI_REQUIRE(formulation == SISI_FORMULATION_VERTEX || !h->e.cplx, SISI_ERR_INVALID,
-                 "the full-integral formulation is built for the real path");
+    SISI_REQUIRE(formulation == SISI_FORMULATION_VERTEX || formulation == SISI_FORMULATION_DRCCD ||
+                     !h->e.cplx,
+                 SISI_ERR_INVALID, "this formulation is built for the real path");
     h->e.formulation = formulation;
   });
 }

@@ -620,6 +620,12 @@ void ccsd_energy(Engine &e, const double *T1, const double *T2, double &E, doubl
   if (e.cplx) {  // F = complex: the real parts are reported (cxx:30-33, 185-189)
     double Ez[2], dz[2], xz[2];
     zccsd_energy(e, T1, T2, Ez, dz, xz);
+    // spins (direct-ring CCD's unrestricted) as in the real branch
+    for (int k = 0; k < 2; ++k) {
+      dz[k] *= 0.25 * e.spins * e.spins;
+      xz[k] *= 0.5 * e.spins;
+      Ez[k] = dz[k] + xz[k];
+    }
     E = Ez[0];
     direct = dz[0];
     exchange = xz[0];

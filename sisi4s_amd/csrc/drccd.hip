@@ -9,6 +9,7 @@
 #include "ccsd.hpp"
 #include "contract.hpp"
 #include "tensor_ops.hpp"
+#include "ztensor.hpp"
 
 namespace sisi {
 
@@ -21,15 +22,55 @@ void set_drccd(Engine &e, int unrestricted, int linearized, int adjacent_pairs_e
 
 bool integrals_ready(const Engine &e) {
   if (e.formulation == SISI_FORMULATION_UCCSD) return uccsd_ready(e);
-  if (e.formulation == SISI_FORMULATION_DRCCD)
-    return !e.cplx && e.Vabij.get() != nullptr && e.Uabij.get() != nullptr;
+  if (e.formulation == SISI_FORMULATION_DRCCD) {
+    if (e.cplx) return e.have_integrals && e.Vaijb.get() != nullptr && e.Vijab.get() != nullptr;
+    return e.Vabij.get() != nullptr && e.Uabij.get() != nullptr;
+  }
   return e.have_integrals;
 }
 
+namespace {
+
+// F = complex (the reference's template with complex PPHH, PHHP and HHPP of
+// CoulombIntegralsFromVertex's complex path): the same products, no
+// conjugations, on planar complex amplitudes.
+void zdrccd_residual(Engine &e, int it, bool initial_doubles, const double *T2c, double *R1p,
+                     double *R2p) {
+  const int64_t o = e.No, v = e.Nv;
+  const int64_t n2 = v * v * o * o;
+  double *T2p = const_cast<double *>(T2c);
+  ZV T2 = zdense(T2p, {v, v, o, o}), R2 = zdense(R2p, {v, v, o, o});
+  ZV V = zdense(e.Vabij.get(), {v, v, o, o}), Vaijb = zdense(e.Vaijb.get(), {v, o, o, v}),
+     Vijab = zdense(e.Vijab.get(), {o, o, v, v});
+  zero(e, R1p, int64_t(e.n1()));
+  zcopy(e, R2, V);  // cxx:57, 77
+  if (it == 0 && !initial_doubles) return;
+  const int prec = e.precision;
+  const double s = e.spins;
+  e.thaw();
+  for (const double *p : {T2p, e.Vaijb.get(), e.Vijab.get()}) {
+    e.frozen.push_back(p);
+    e.frozen.push_back(p + n2);
+  }
+  zctr(e, s, "akic,cbkj->abij", Vaijb, false, T2, false, 1.0, R2, prec);  // cxx:58
+  zctr(e, s, "bkjc,acik->abij", Vaijb, false, T2, false, 1.0, R2, prec);  // cxx:59
+  if (!e.ring_linearized) {
+    DBuf Wb(&e.pool, size_t(2 * n2)), Cb(&e.pool, size_t(2 * n2));
+    ZV W = zdense(Wb.get(), {o, o, v, v}), C = zdense(Cb.get(), {v, o, o, v});
+    zcopy(e, W, Vijab, false, s);                                            // cxx:61-62
+    if (e.ring_adjacent) zsum(e, -1.0, Vijab, "jiab", false, 1.0, W, "ijab");  // cxx:63-65
+    zctr(e, s, "klcd,acik->alid", W, false, T2, false, 0.0, C, prec);          // cxx:68
+    zctr(e, 1.0, "alid,dblj->abij", C, false, T2, false, 1.0, R2, prec);       // cxx:69
+  }
+  e.thaw();
+}
+
+}  // namespace
+
 void drccd_residual(Engine &e, int it, bool initial_doubles, const double *T2c, double *R1p,
                     double *R2p) {
-  SISI_REQUIRE(!e.cplx, SISI_ERR_INVALID, "the direct-ring CCD residual is built for real amplitudes");
   SISI_REQUIRE(integrals_ready(e), SISI_ERR_STATE, "direct-ring CCD needs the PPHH integrals");
+  if (e.cplx) return zdrccd_residual(e, it, initial_doubles, T2c, R1p, R2p);
   const int64_t o = e.No, v = e.Nv;
   const int64_t n2 = v * v * o * o;
   double *T2p = const_cast<double *>(T2c);

@@ -107,3 +107,33 @@ def test_drccd_mixed_precision_reaches_fp64_energy():
     assert abs(rmx["energy"] - r64["energy"]) <= 1e-8
     ref = drccd.solve(V, ei, ea, adjacent_pairs_exchange=True, **kw)
     assert abs(r64["energy"] - ref["energy"]) <= 1e-10
+
+
+def test_complex_drccd_matches_oracle_and_real_path():
+    """F = complex: residual against the oracle's complex restatement, and a
+    whole run on a gauge-transformed vertex gives the real path's energies."""
+    from sisi4s_amd import engine
+    from oracle import ccsd_complex as cz
+    g, ei, ea, V = system(24, 4, 12, 36)
+    No, Nv = len(ei), len(ea)
+    gz, _ = cz.gauge_transform(g.astype(complex), No, 25)
+    iz = cz.integrals_from_vertex(gz, No)
+    ez = engine.Engine(No, Nv, g.shape[0])
+    ez.set_system(gz, ei, ea)
+    assert ez.complex
+    ez.set_drccd(adjacent_pairs_exchange=True)
+    T2 = symmetric_amplitudes(Nv, No, 6) * (1 + 0.3j)
+    _, O2 = drccd.residual(1, T2, iz["PPHH"], iz["PHHP"], iz["HHPP"], adjacent_pairs_exchange=True)
+    announce("complex drccd residual and solve")
+    R1, R2 = ez.residual(1, np.zeros((Nv, No), complex), T2)
+    assert float(np.abs(R2 - O2).max()) <= 1e-11 * float(np.abs(O2).max()) and not R1.any()
+    er = engine.Engine(No, Nv, g.shape[0])
+    er.set_system(g, ei, ea)
+    er.set_drccd(adjacent_pairs_exchange=True)
+    kw = dict(mixer="DiisMixer", max_residua=6, max_iterations=60, amplitudes_convergence=1e-10,
+              energy_convergence=1e-12)
+    rz = ez.ccsd_solve(engine.default_opts(**kw))
+    rr = er.ccsd_solve(engine.default_opts(**kw))
+    assert rz["converged"] and rr["converged"] and rz["iterations"] == rr["iterations"]
+    assert np.abs(ez.energies() - er.energies()).max() <= 1e-10
+    assert abs(rz["energy_imag"]) <= 1e-10

@@ -602,3 +602,23 @@ def test_uccsd_is_exact_for_two_electrons():
     rb = uccsd.solve(bad, eps[:2], eps[2:], max_residua=6, max_iterations=300,
                      amplitudes_convergence=1e-11, energy_convergence=1e-13)
     assert abs(rb["energy"] - e_fci) > 1e-6
+
+
+def test_complex_drccd_is_gauge_covariant():
+    """F = complex direct-ring CCD on a gauge-transformed vertex (random
+    orbital phases and a unitary on the auxiliary index): every iteration's
+    energy equals the real path's, with zero imaginary part."""
+    from oracle import ccsd_complex as cz, drccd
+    No, Nv, NG = 3, 8, 24
+    g, ei, ea = philox.system(44, No, Nv, NG)
+    V = ccsd.integrals_from_vertex(g, No)["PPHH"]
+    gz, _ = cz.gauge_transform(g.astype(complex), No, 45)
+    iz = cz.integrals_from_vertex(gz, No)
+    kw = dict(max_residua=6, max_iterations=40, amplitudes_convergence=1e-10,
+              energy_convergence=1e-12, adjacent_pairs_exchange=True)
+    rr = drccd.solve(V, ei, ea, **kw)
+    rz = drccd.solve(None, ei, ea, complex_integrals=(iz["PPHH"], iz["PHHP"], iz["HHPP"]), **kw)
+    ez = np.array(rz["energies"])
+    assert rr["iterations"] == rz["iterations"]
+    assert np.abs(ez.real - np.array(rr["energies"])).max() <= 1e-12
+    assert np.abs(ez.imag).max() <= 1e-12
