@@ -6,6 +6,9 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <algorithm>
+#include <map>
+#include <set>
 #include <fstream>
 #include <sstream>
 
@@ -276,6 +279,129 @@ class TensorNorm : public Algorithm {
   }
 };
 
+// TensorUnrestricter (TensorUnrestricter.cxx:15-79): spatial orbitals to
+// spin orbitals, spin orbital 2 p + s.  Order 1 and 2 tensors fill the
+// diagonal spin blocks; order 4 tensors <pq|rs> the blocks where indices 0
+// and 2, and 1 and 3, carry the same spin.  Real or complex.
+class TensorUnrestricter : public Algorithm {
+ public:
+  using Algorithm::Algorithm;
+  std::string getName() override { return "TensorUnrestricter"; }
+  void run() override {
+    const Tensor &t = getTensorArgument("Data");
+    const size_t order = t.lens.size();
+    if (order != 1 && order != 2 && order != 4) throw StepError("Tensor size not supported");
+    auto out = std::make_shared<Tensor>();
+    out->complex = t.complex;
+    for (auto l : t.lens) out->lens.push_back(2 * l);
+    const int w = t.complex ? 2 : 1;
+    out->values.assign(size_t(out->size()) * w, 0.0);
+    std::vector<int64_t> idx(order, 0);
+    const int64_t n = t.size();
+    for (int64_t e = 0; e < n; ++e) {
+      int64_t r = e;
+      for (size_t k = 0; k < order; ++k) {
+        idx[k] = r % t.lens[k];
+        r /= t.lens[k];
+      }
+      for (int s = 0; s < 2; ++s)
+        for (int u = 0; u < (order == 4 ? 2 : 1); ++u) {
+          // spins of the indices: order 1/2 all s; order 4 (s, u, s, u)
+          int64_t o = 0, stride = 1;
+          for (size_t k = 0; k < order; ++k) {
+            const int spin = (order == 4 && (k == 1 || k == 3)) ? u : s;
+            o += (2 * idx[k] + spin) * stride;
+            stride *= out->lens[k];
+          }
+          for (int c = 0; c < w; ++c) out->values[size_t(o * w + c)] = t.values[size_t(e * w + c)];
+        }
+    }
+    allocatedTensorArgument("Out", out);
+  }
+};
+
+// TensorAntisymmetrizer (TensorAntisymmetrizer.cxx:14-124, Integrals.hpp:
+// 52-84): every given class V_pqrs becomes V_pqrs - V_pqsr, the exchange
+// value read from the first given class among the equivalent forms of
+// <qp|rs> and <pq|sr> (the class itself first), always from the values
+// before antisymmetrisation.
+class TensorAntisymmetrizer : public Algorithm {
+ public:
+  using Algorithm::Algorithm;
+  std::string getName() override { return "TensorAntisymmetrizer"; }
+  void run() override {
+    using Info = std::pair<std::string, std::string>;  // class letters, index labels
+    static const Info classes[] = {
+        {"HHHH", "ijkl"}, {"HHHP", "ijka"}, {"HHPH", "ijak"}, {"HHPP", "ijab"},
+        {"HPHH", "iajk"}, {"HPHP", "iajb"}, {"HPPH", "iabj"}, {"HPPP", "iabc"},
+        {"PHHH", "aijk"}, {"PHHP", "aijb"}, {"PHPH", "aibj"}, {"PHPP", "aibc"},
+        {"PPHH", "abij"}, {"PPHP", "abic"}, {"PPPH", "abci"}, {"PPPP", "abcd"}};
+    auto swap2 = [](Info x, int i, int j) {
+      std::swap(x.first[size_t(i)], x.first[size_t(j)]);
+      std::swap(x.second[size_t(i)], x.second[size_t(j)]);
+      return x;
+    };
+    auto antisymmetrizers = [&](const Info &x) {
+      std::vector<Info> res;
+      std::set<std::string> names;
+      for (const Info &t : {swap2(x, 0, 1), swap2(x, 2, 3)})
+        for (const Info &eq : {swap2(swap2(t, 0, 1), 2, 3), swap2(swap2(t, 0, 2), 1, 3),
+                               swap2(t, 0, 2)})
+          if (names.insert(eq.first).second) res.push_back(eq);
+      std::stable_partition(res.begin(), res.end(),
+                            [&](const Info &a) { return a.first == x.first; });
+      return res;
+    };
+    std::map<std::string, Tensor> original;
+    for (const auto &c : classes)
+      if (isArgumentGiven(c.first + "CoulombIntegrals"))
+        original[c.first] = getTensorArgument(c.first + "CoulombIntegrals");
+    if (original.empty()) throw StepError("TensorAntisymmetrizer: no integrals given");
+    for (const auto &c : classes) {
+      if (!original.count(c.first)) continue;
+      const Info *use = nullptr;
+      const auto cands = antisymmetrizers(c);
+      for (const auto &a : cands)
+        if (original.count(a.first)) {
+          use = &a;
+          break;
+        }
+      if (!use) throw StepError(c.first + "CoulombIntegrals could not be antisymmetrized");
+      Tensor &V = *getTensorPtr(c.first + "CoulombIntegrals");
+      const Tensor &A = original.at(use->first);
+      if (V.lens.size() != 4 || A.lens.size() != 4 || V.complex != A.complex)
+        throw StepError("TensorAntisymmetrizer: integrals of different kinds");
+      // position in A of each of V's index labels
+      int pos[4];
+      for (int k = 0; k < 4; ++k) {
+        const auto at = use->second.find(c.second[size_t(k)]);
+        if (at == std::string::npos) throw StepError("TensorAntisymmetrizer: label mismatch");
+        pos[k] = int(at);
+      }
+      int64_t astride[4], s = 1;
+      for (int k = 0; k < 4; ++k) {
+        astride[k] = s;
+        s *= A.lens[size_t(k)];
+      }
+      for (int k = 0; k < 4; ++k)
+        if (A.lens[size_t(pos[k])] != V.lens[size_t(k)])
+          throw StepError("TensorAntisymmetrizer: " + use->first + " does not match " + c.first);
+      const int w = V.complex ? 2 : 1;
+      const int64_t n = V.size();
+      for (int64_t e = 0; e < n; ++e) {
+        int64_t r = e, ao = 0;
+        for (int k = 0; k < 4; ++k) {
+          ao += (r % V.lens[size_t(k)]) * astride[pos[k]];
+          r /= V.lens[size_t(k)];
+        }
+        for (int cc = 0; cc < w; ++cc) V.values[size_t(e * w + cc)] -= A.values[size_t(ao * w + cc)];
+      }
+    }
+  }
+};
+
+static Registrar<TensorUnrestricter> rr4("TensorUnrestricter");
+static Registrar<TensorAntisymmetrizer> rr5("TensorAntisymmetrizer");
 static Registrar<CoulombVertexReader> rr0("CoulombVertexReader");
 static Registrar<Read> rr1("Read");
 static Registrar<DefineHolesAndParticles> rr2("DefineHolesAndParticles");

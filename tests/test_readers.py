@@ -369,3 +369,68 @@ def test_uccsd_from_files(tmp_path):
     rr = ccsd.solve(g, ei, ea, **kw)
     E = vals["UccsdEnergy"][-1]
     assert abs(E - ref["energy"]) <= 1e-10 and abs(E - rr["energy"]) <= 1e-9 * abs(rr["energy"])
+
+
+SPIN_CLASSES = ("PPHH", "HHHH", "PPPP", "HHHP", "PPPH", "PHPH")
+
+
+def test_unrestricter_and_antisymmetrizer(tmp_path):
+    """TensorUnrestricter (spin orbital 2p + s) then TensorAntisymmetrizer
+    (V_pqrs - V_pqsr) on spatial integral classes read from files give the
+    antisymmetrized spin-orbital classes (compared through their norms and
+    the unrestricted eigenenergies)."""
+    from oracle import ccsd, uccsd
+    No, Nv = 2, 3
+    g, ei, ea = philox.system(23, No, Nv, 9)
+    sp = ccsd.integrals_from_vertex(g, No)
+    sp["PPPP"], sp["PPPH"] = ccsd.vabcd_from_vertex(g, No)
+    write_cc4s_tensor(tmp_path / "eps", ei)
+    steps = ["- name: Read\n  in: {fileName: \"%s\"}\n  out: {destination: $eps}\n"
+             "- name: TensorUnrestricter\n  in: {Data: $eps}\n  out: {Out: $eps}\n"
+             "- name: TensorNorm\n  in: {Data: $eps}\n  out: {Norm: $N}\n" % (tmp_path / "eps.yaml")]
+    for c in SPIN_CLASSES:
+        write_cc4s_tensor(tmp_path / c, sp[c])
+        steps.append("- name: Read\n  in: {fileName: \"%s\"}\n  out: {destination: $%s}\n"
+                     "- name: TensorUnrestricter\n  in: {Data: $%s}\n  out: {Out: $%s}\n"
+                     % (tmp_path / (c + ".yaml"), c, c, c))
+    steps.append("- name: TensorAntisymmetrizer\n  in: {%s}\n"
+                 % ", ".join("%sCoulombIntegrals: $%s" % (c, c) for c in SPIN_CLASSES))
+    for c in SPIN_CLASSES:
+        steps.append("- name: TensorNorm\n  in: {Data: $%s}\n  out: {Norm: $N}\n" % c)
+    r, vals = run_steps(tmp_path, "".join(steps))
+    assert r.returncode == 0, r.stdout + r.stderr
+    ref = uccsd.spin_orbital_integrals(g, No, Nv)
+    expect = [np.linalg.norm(np.repeat(ei, 2))] + [np.linalg.norm(ref[c]) for c in SPIN_CLASSES]
+    assert vals["Norm"] == pytest.approx(expect, rel=1e-14)
+
+
+@pytest.mark.gpu
+def test_uccsd_pipeline_from_vertex_matches_restricted_ccsd(tmp_path):
+    """The reference's spin-orbital workflow (integrals, TensorUnrestricter,
+    TensorAntisymmetrizer, UccsdAmplitudesFromCoulombIntegrals) on a
+    closed-shell system converges to CcsdEnergyFromCoulombIntegrals' energy."""
+    conv = "mixer: DiisMixer, maxResidua: 6, maxIterations: 100, amplitudesConvergence: 1e-10, " \
+           "energyConvergence: 1e-12"
+    steps = ["""- name: GenerateSyntheticSystem
+  in: {seed: 31, No: 3, Nv: 9, NG: 30}
+  out: {CoulombVertex: $G, HoleEigenEnergies: $I, ParticleEigenEnergies: $A}
+- name: CoulombIntegralsFromVertex
+  in: {CoulombVertex: $G, HoleEigenEnergies: $I, ParticleEigenEnergies: $A}
+  out: {%s}
+- name: CcsdEnergyFromCoulombIntegrals
+  in: {%s, HoleEigenEnergies: $I, ParticleEigenEnergies: $A, PPHHCoulombIntegrals: $PPHH,
+       CoulombVertex: $G}
+  out: {CcsdEnergy: $Er}
+""" % (", ".join("%sCoulombIntegrals: $%s" % (c, c) for c in SPIN_CLASSES), conv)]
+    for x in ["I", "A"] + list(SPIN_CLASSES):
+        steps.append("- name: TensorUnrestricter\n  in: {Data: $%s}\n  out: {Out: $%s}\n" % (x, x))
+    steps.append("- name: TensorAntisymmetrizer\n  in: {%s}\n"
+                 % ", ".join("%sCoulombIntegrals: $%s" % (c, c) for c in SPIN_CLASSES))
+    steps.append("- name: UccsdAmplitudesFromCoulombIntegrals\n  in: {%s, HoleEigenEnergies: $I, "
+                 "ParticleEigenEnergies: $A, %s}\n  out: {UccsdEnergy: $Eu}\n"
+                 % (conv, ", ".join("%sCoulombIntegrals: $%s" % (c, c) for c in SPIN_CLASSES)))
+    announce("uccsd pipeline from the vertex")
+    r, vals = run_steps(tmp_path, "".join(steps))
+    assert r.returncode == 0, r.stdout + r.stderr
+    er, eu = vals["CcsdEnergy"][-1], vals["UccsdEnergy"][-1]
+    assert abs(eu - er) <= 1e-9 * abs(er), (eu, er)
