@@ -434,3 +434,36 @@ def test_uccsd_pipeline_from_vertex_matches_restricted_ccsd(tmp_path):
     assert r.returncode == 0, r.stdout + r.stderr
     er, eu = vals["CcsdEnergy"][-1], vals["UccsdEnergy"][-1]
     assert abs(eu - er) <= 1e-9 * abs(er), (eu, er)
+
+
+def test_consumer_steps_reject_what_they_do_not_build(tmp_path):
+    """Errors of the direct-ring / spin-orbital steps surface as step failures
+    before any device work: an order-3 tensor for TensorUnrestricter, a class
+    with no exchange partner for TensorAntisymmetrizer, onlyPPL for UCCSD and
+    complex integrals for the direct-ring step."""
+    from oracle import ccsd
+    g, ei, ea = philox.system(3, 2, 3, 5)
+    write_cc4s_tensor(tmp_path / "G", g)
+    r, _ = run_steps(tmp_path, "- name: Read\n  in: {fileName: \"%s\"}\n  out: {destination: $G}\n"
+                     "- name: TensorUnrestricter\n  in: {Data: $G}\n  out: {Out: $U}\n"
+                     % (tmp_path / "G.yaml"))
+    assert r.returncode != 0 and "not supported" in r.stderr
+    V = ccsd.integrals_from_vertex(g, 2)["PHPH"]
+    write_cc4s_tensor(tmp_path / "X", V)
+    r, _ = run_steps(tmp_path, "- name: Read\n  in: {fileName: \"%s\"}\n  out: {destination: $X}\n"
+                     "- name: TensorAntisymmetrizer\n  in: {PHPHCoulombIntegrals: $X}\n"
+                     % (tmp_path / "X.yaml"))
+    assert r.returncode != 0 and "could not be antisymmetrized" in r.stderr
+    r, _ = run_steps(tmp_path, "- name: UccsdAmplitudesFromCoulombIntegrals\n  in: {onlyPPL: 1}\n")
+    assert r.returncode != 0 and "onlyPPL" in r.stderr
+    P = ccsd.integrals_from_vertex(g, 2)["PPHH"]
+    write_cc4s_tensor(tmp_path / "Z", P + 0.3j * P, "Complex64")  # non-zero imaginary part
+    write_cc4s_tensor(tmp_path / "I", ei)
+    write_cc4s_tensor(tmp_path / "A", ea)
+    r, _ = run_steps(tmp_path, "".join(
+        "- name: Read\n  in: {fileName: \"%s\"}\n  out: {destination: $%s}\n" % (tmp_path / (n + ".yaml"), n)
+        for n in ("Z", "I", "A")) +
+        "- name: DrccdEnergyFromCoulombIntegrals\n  in: {HoleEigenEnergies: $I, ParticleEigenEnergies: $A, "
+        "PPHHCoulombIntegrals: $Z, PHHPCoulombIntegrals: $Z, HHPPCoulombIntegrals: $Z}\n"
+        "  out: {DrccdEnergy: $E}\n")
+    assert r.returncode != 0 and "real integrals" in r.stderr
