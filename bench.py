@@ -85,7 +85,8 @@ def main():
                     help="consumers: occupied spatial orbitals of the spin-orbital CCSD system")
     ap.add_argument("--uccsd-nv", type=int, default=60,
                     help="consumers: virtual spatial orbitals of the spin-orbital CCSD system")
-    ap.add_argument("--mode", choices=["iteration", "converge", "batched", "complex", "consumers"],
+    ap.add_argument("--mode", choices=["iteration", "converge", "batched", "complex", "consumers",
+                                       "config1"],
                     default="iteration",
                     help="iteration: timed FP64 iterations (default line); converge: FP64 and "
                          "mixed runs to |dT|/|T| <= 1e-8 with (T); batched: 64 small systems")
@@ -122,6 +123,8 @@ def main():
         return batched(args)
     if args.mode == "consumers":
         return consumers(args)
+    if args.mode == "config1":
+        return config1(args)
     if args.mode == "complex":
         return complex_path(args)
 
@@ -301,6 +304,37 @@ def complex_vertex(g, No, seed):
     z *= t.conj()[None, :, None]
     z *= t[None, None, :]
     return z
+
+
+def config1(args):
+    """The reference's CPU-runnable case: No=10, Nv=50, NG=300, MP2 start,
+    DIIS(6), 30 iterations, slices of No; the whole run on the GPU (FP64)
+    beside the whole run of the FP64 CPU oracle on the host cores, and the
+    largest energy difference over the 30 iterations."""
+    from sisi4s_amd import engine as eng
+    from oracle import ccsd as oc
+    No, Nv, NG = 10, 50, 300
+    g, ei, ea = eng.generate_system(args.seed, No, Nv, NG)
+    kw = dict(mixer="DiisMixer", max_residua=6, max_iterations=30, amplitudes_convergence=0.0,
+              energy_convergence=0.0, slice_size=No)
+    e = eng.Engine(No, Nv, NG)
+    e.set_system(g, ei, ea)
+    e.ccsd_solve(eng.default_opts(**kw))  # warm-up run (allocations, split caches)
+    times = []
+    for _ in range(max(args.steps, 1)):
+        t0 = time.perf_counter()
+        rep = e.ccsd_solve(eng.default_opts(**kw))
+        times.append(time.perf_counter() - t0)
+    gpu = float(np.median(times))
+    t0 = time.perf_counter()
+    ref = oc.solve(g, ei, ea, **kw)
+    cpu = time.perf_counter() - t0
+    dE = float(np.abs(e.energies() - np.array(ref["energies"])).max())
+    print(json.dumps({"mode": "config1", "workload": "No=10 Nv=50 NG=300, MP2 start, DIIS(6), "
+                      "30 iterations, slices of No", "gpu_seconds": gpu,
+                      "gpu_ms_per_iteration": 1e3 * gpu / 30, "cpu_seconds": cpu,
+                      "cpu_kind": "port (numpy FP64 oracle)", "cpu_threads": os.cpu_count(),
+                      "energy": rep["energy"], "max_energy_difference": dE}), flush=True)
 
 
 def spin_orbital_classes(g, No, Nv):
