@@ -256,6 +256,20 @@ void ccsd_residual(Engine &e, int it, bool initial_doubles, int slice_size, cons
     }
     const int64_t smax = std::min<int64_t>(slice_size, v);
     DBuf Vxy(&e.pool, smax * smax * v * v), Rxy(&e.pool, smax * smax * o * o);
+    // an anchor keeps the bf16 planes of its V~ slices for the refinement
+    // residuals when they fit in half the free device memory
+    bool keep_planes = false;
+    if (e.recording) {
+      e.anchor.ladder.clear();
+      e.anchor.ladder_slice = slice_size;
+      const int64_t Kp = ((v * v + 31) / 32) * 32;
+      double bytes = 0.0;
+      for (int m = 0; m < nslices; ++m)
+        for (int n = m; n < nslices; ++n)
+          bytes += 4.0 * double(((hi[n] - lo[n]) * (hi[m] - lo[m]) + 127) / 128 * 128) * double(Kp);
+      size_t free_b = 0, total_b = 0;
+      keep_planes = hipMemGetInfo(&free_b, &total_b) == hipSuccess && bytes <= 0.5 * double(free_b);
+    }
     for (int m = 0; m < nslices; ++m)
       for (int n = m; n < nslices; ++n) {  // cxx:418-451
         const int64_t Nx = hi[n] - lo[n], Ny = hi[m] - lo[m];
@@ -263,6 +277,11 @@ void ccsd_residual(Engine &e, int it, bool initial_doubles, int slice_size, cons
         View Rxyij = V4(Rxy.get(), Nx, Ny, o, o);
         ctr(e, 1.0, "Gxc,Gyd->xycd", V3(sl[n].get(), G, Nx, v), V3(sl[m].get(), G, Ny, v), 0.0,
             Vxycd, e.prec_ladder);  // cxx:431-434
+        if (keep_planes) {
+          e.anchor.ladder.emplace_back();
+          if (2.0 * double(Nx * Ny) * double(o * o) * double(v * v) >= kB3MinFlops)
+            b3_freeze_a(e, Vxy.get(), 1, Nx * Ny, int(Nx * Ny), int(v * v), e.anchor.ladder.back());
+        }
         ctr(e, 1.0, "xycd,cdij->xyij", Vxycd, X, 0.0, Rxyij, e.prec_ladder);  // cxx:443
         // sliceIntoResiduum (ClusterSinglesDoublesAlgorithm.cxx:887-915)
         const int64_t a = lo[n], b = lo[m];
@@ -511,19 +530,46 @@ void ccsd_residual_delta(Engine &e, int slice_size, const double *T1c, const dou
     DBuf VAb(&e.pool, smax * smax * v * v), dVb(&e.pool, smax * smax * v * v),
         Rxy(&e.pool, smax * smax * o * o);
     const int pl = e.prec_ladder;
+    // V~(T_A) of each pair from the anchor's bf16 planes when the ladder runs
+    // in bf16 or split-BF16x3 (no rebuild, no split), else rebuilt here
+    const bool planes = (pl == SISI_BF16 || pl == SISI_BF16X3) && an.ladder_slice == slice_size &&
+                        an.ladder.size() == size_t(nslices * (nslices + 1) / 2);
+    int pair = 0;
     for (int m = 0; m < nslices; ++m)
-      for (int n = m; n < nslices; ++n) {
+      for (int n = m; n < nslices; ++n, ++pair) {
         const int64_t Nx = hi[n] - lo[n], Ny = hi[m] - lo[m];
         View VA = V4(VAb.get(), Nx, Ny, v, v), dV = V4(dVb.get(), Nx, Ny, v, v);
         View Rxyij = V4(Rxy.get(), Nx, Ny, o, o);
-        ctr(e, 1.0, "Gxc,Gyd->xycd", V3(sA[n].get(), G, Nx, v), V3(sA[m].get(), G, Ny, v), 0.0,
-            VA, pl);
+        const B3FrozenA *fa =
+            planes && an.ladder[size_t(pair)].store.get() ? &an.ladder[size_t(pair)] : nullptr;
+        if (!fa)
+          ctr(e, 1.0, "Gxc,Gyd->xycd", V3(sA[n].get(), G, Nx, v), V3(sA[m].get(), G, Ny, v), 0.0,
+              VA, pl);
         ctr(e, 1.0, "Gxc,Gyd->xycd", V3(sD[n].get(), G, Nx, v), V3(sN[m].get(), G, Ny, v), 0.0,
             dV, pl);
         ctr(e, 1.0, "Gxc,Gyd->xycd", V3(sA[n].get(), G, Nx, v), V3(sD[m].get(), G, Ny, v), 1.0,
             dV, pl);
         ctr(e, 1.0, "xycd,cdij->xyij", dV, Xn, 0.0, Rxyij, pl);
-        ctr(e, 1.0, "xycd,cdij->xyij", VA, dX, 1.0, Rxyij, pl);
+        if (fa) {
+          // Rxyij(m = x + Nx y, n = i + o j) += sum_k V~_A(m, k = c + v d) dX(k, n)
+          GemmArgs g;
+          g.B = dXb.get();
+          g.sbk = 1;
+          g.sbn = v * v;
+          g.C = Rxy.get();
+          g.M = int(Nx * Ny);
+          g.N = int(o * o);
+          g.K = int(v * v);
+          g.cm0 = 1;
+          g.cn0 = Nx * Ny;
+          g.dm0 = g.M;
+          g.dn0 = g.N;
+          g.alpha = 1.0;
+          g.beta = 1.0;
+          gemm_b3_frozen_a(e, g, *fa, pl == SISI_BF16 ? 1 : 3);
+        } else {
+          ctr(e, 1.0, "xycd,cdij->xyij", VA, dX, 1.0, Rxyij, pl);
+        }
         const int64_t a = lo[n], b = lo[m];
         copy(e, subview(dR2, {a, b, 0, 0}, {a + Nx, b + Ny, o, o}), Rxyij, 1.0, 1.0);
         if (a > b)
@@ -587,6 +633,7 @@ void ccsd_residual_anchor(Engine &e, int slice_size, const double *T1, const dou
   e.precision = SISI_FP64;
   e.prec_ladder = SISI_FP64;
   e.anchor.valid = false;
+  e.anchor.ladder.clear();
   e.recording = true;
   try {
     ccsd_residual(e, 1, true, slice_size, T1, T2, R1, R2);

@@ -63,6 +63,11 @@ struct Anchor {
   bool ppl = true, distinguishable = false;  // residual variant it was recorded with
   DBuf T1, T2, R1, R2;
   std::map<std::string, DBuf> v;
+  // bf16 planes of the ladder's V~(T_A) slice pairs (pair order m <= n),
+  // split from the FP64 slices the anchor computes; an empty store marks a
+  // pair too small for the bf16 kernels
+  std::vector<B3FrozenA> ladder;
+  int ladder_slice = 0;
 };
 
 struct DiisSlot {

@@ -115,6 +115,20 @@ int get_b3_backend();
 // workgroups fill their last round on a device of `cus` compute units.
 int round_filling_split(int64_t groups, int64_t s, int64_t smax, int cus);
 
+// An A operand (M x K, element (m,k) at A[m*sam + k*sak]) split once into the
+// bf16 hi and lo planes of the 128-row split-BF16x3 / bf16 kernels and kept
+// across residual evaluations (the refinement anchor's ladder slices);
+// `store` owns both planes.
+struct B3FrozenA {
+  DBuf store;
+  int M = 0, K = 0, Mp = 0, Kp = 0;
+};
+void b3_freeze_a(Engine &e, const double *A, int64_t sam, int64_t sak, int M, int K,
+                 B3FrozenA &out);
+// C = alpha * A B + beta * C with A taken from frozen planes (g.A unused; g.M
+// and g.K must match them); passes = 1 (bf16) or 3 (split-BF16x3).
+void gemm_b3_frozen_a(Engine &e, GemmArgs g, const B3FrozenA &A, int passes);
+
 // Split-K reduction of g.work into C (deterministic order).
 void gemm_reduce(Engine &eng, const GemmArgs &g);
 

@@ -442,9 +442,10 @@ void plane_pointers(u16 *base, int64_t Rp, int64_t Kp, int layout, u16 *&hi, u16
 // Split one (rows x K) operand view; frozen sources are split once per
 // residual evaluation and reused.
 const Planes &split_operand(Engine &e, const double *src, int64_t sr, int64_t sk, int R, int K,
-                            int Rp, int Kp, Planes &local, int layout = 0) {
+                            int Rp, int Kp, Planes &local, int layout = 0,
+                            bool allow_cache = true) {
   std::string key;
-  const bool cached = e.is_frozen(src);
+  const bool cached = allow_cache && e.is_frozen(src);
   const int64_t elems = int64_t(Rp) * Kp;  // bf16 per plane
   const int nplanes = layout == 0 ? 2 : 3;
   auto fill = [&](u16 *base) {
@@ -539,10 +540,10 @@ void set_b3_backend(int library) { b3_backend().store(library ? 1 : 0); }
 int get_b3_backend() { return b3_backend().load(); }
 
 template <int BM, int BN, int WM, int WN, int NBUF, int MINB = 1, int PASSES = 3>
-static void run_b3(Engine &e, GemmArgs g) {
+static void run_b3(Engine &e, GemmArgs g, const B3FrozenA *fa = nullptr) {
   const int Kp = ((g.K + 31) / 32) * 32;
   const int Mp = ((g.M + BM - 1) / BM) * BM, Np = ((g.N + BN - 1) / BN) * BN;
-  if (b3_use_library()) {
+  if (b3_use_library() && !fa) {
     g.split_out = !(g.dm0 >= g.M && g.dn0 >= g.N);
     Planes la, lb;
     const int lay_a = PASSES == 3 ? 1 : 0, lay_b = PASSES == 3 ? 2 : 0;
@@ -551,7 +552,15 @@ static void run_b3(Engine &e, GemmArgs g) {
     if (b3_library(e, g, A, B, Kp, PASSES)) return;
   }
   Planes pa, pb;
-  const Planes &A = split_operand(e, g.A + 0, g.sam, g.sak, g.M, g.K, Mp, Kp, pa);
+  if (fa) {  // planes split earlier in the layout split_operand gives (layout 0)
+    SISI_REQUIRE(fa->store.get() && fa->M == g.M && fa->K == g.K && fa->Mp == Mp && fa->Kp == Kp,
+                 SISI_ERR_INVALID, "frozen A planes do not match the product");
+    pa.hi = reinterpret_cast<const u16 *>(fa->store.get());
+    pa.lo = pa.hi + int64_t(Mp) * Kp;
+    pa.base = pa.hi;
+    pa.ld = Kp;
+  }
+  const Planes &A = fa ? pa : split_operand(e, g.A + 0, g.sam, g.sak, g.M, g.K, Mp, Kp, pa);
   const Planes &B = split_operand(e, g.B + 0, g.sbn, g.sbk, g.N, g.K, Np, Kp, pb);
   g.tiles_m = Mp / BM;
   g.tiles_n = Np / BN;
@@ -631,6 +640,27 @@ void gemm_b3_segments(Engine &e, B3SegArgs a) {
   Timed t(*e.timers, Kind::GemmB3, flops);
   hipLaunchKernelGGL(gemm_b3p_seg_kernel, dim3(unsigned(tiles)), dim3(256), 0, e.stream, a);
   SISI_HIP(hipGetLastError());
+}
+
+void b3_freeze_a(Engine &e, const double *A, int64_t sam, int64_t sak, int M, int K,
+                 B3FrozenA &out) {
+  SISI_REQUIRE(A && M > 0 && K > 0, SISI_ERR_INVALID, "b3_freeze_a: empty operand");
+  const int Kp = ((K + 31) / 32) * 32, Mp = ((M + 127) / 128) * 128;
+  Planes local;
+  split_operand(e, A, sam, sak, M, K, Mp, Kp, local, 0, false);
+  out.store = std::move(local.store);
+  out.M = M;
+  out.K = K;
+  out.Mp = Mp;
+  out.Kp = Kp;
+}
+
+void gemm_b3_frozen_a(Engine &e, GemmArgs g, const B3FrozenA &A, int passes) {
+  if (g.M <= 0 || g.N <= 0) return;
+  SISI_REQUIRE(g.batch == 1, SISI_ERR_INVALID, "gemm_b3_frozen_a: batch 1 only");
+  // the 128-row kernels whose plane layout b3_freeze_a produced
+  if (passes == 1) run_b3<128, 128, 64, 64, 1, 1, 1>(e, g, &A);
+  else run_b3<128, 128, 64, 64, 1>(e, g, &A);
 }
 
 void gemm_b3(Engine &e, GemmArgs g, int passes) {

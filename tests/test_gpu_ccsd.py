@@ -353,6 +353,34 @@ def test_refinement_residual_reduced_precision_error_scales_with_change():
     assert err_ref <= 1e-3 * err_plain and err_ref <= 1e-9 * scale
 
 
+@pytest.mark.parametrize("prec", ["bf16", "bf16x3"])
+def test_refinement_residual_with_anchor_ladder_planes(prec):
+    """Ladder slice pairs large enough for the bf16 kernels take V~(T_A) from
+    the bf16 planes the FP64 anchor kept (No=16, Nv=160, slices of 40): the
+    refinement residual matches the FP64 residual to 1e-9 of its max norm in
+    split-BF16x3 for a change of 2e-6 (a wrong plane layout would leave an
+    error of the size of the change times |V~|), and in plain bf16 to 1e-6
+    and a thousandth of a plain bf16 residual's error."""
+    No, Nv, NG = 16, 160, 640
+    e, g, ei, ea = make_engine(11, No, Nv, NG)
+    TA1, TA2 = random_amplitudes(Nv, No, seed=4, scale=0.02)
+    D1, D2 = random_amplitudes(Nv, No, seed=5, scale=2e-6)
+    announce("refinement residual with anchor ladder planes, %s" % prec)
+    P1, P2 = e.residual(1, TA1 + D1, TA2 + D2, slice_size=40)
+    e.residual_anchor(TA1, TA2, slice_size=40)
+    e.set_precision(prec)
+    R1, R2 = e.residual_refine(TA1 + D1, TA2 + D2, slice_size=40)
+    Q1, Q2 = e.residual(1, TA1 + D1, TA2 + D2, slice_size=40)  # plain reduced precision
+    e.set_precision("fp64")
+    scale = float(np.abs(P2).max())
+    err, err_plain = float(np.abs(R2 - P2).max()), float(np.abs(Q2 - P2).max())
+    print("%s refinement with planes: err %.2e, plain %.2e of |R| %.2e"
+          % (prec, err / scale, err_plain / scale, scale), flush=True)
+    tol = 1e-9 if prec == "bf16x3" else 1e-6
+    assert err <= tol * scale and err <= 1e-3 * err_plain
+    assert float(np.abs(R1 - P1).max()) <= tol * scale
+
+
 def test_refined_mixed_solve_matches_fp64():
     from sisi4s_amd import engine
     No, Nv, NG = 5, 20, 80
