@@ -86,7 +86,7 @@ def main():
     ap.add_argument("--uccsd-nv", type=int, default=60,
                     help="consumers: virtual spatial orbitals of the spin-orbital CCSD system")
     ap.add_argument("--mode", choices=["iteration", "converge", "batched", "complex", "consumers",
-                                       "config1"],
+                                       "small"],
                     default="iteration",
                     help="iteration: timed FP64 iterations (default line); converge: FP64 and "
                          "mixed runs to |dT|/|T| <= 1e-8 with (T); batched: 64 small systems")
@@ -123,8 +123,8 @@ def main():
         return batched(args)
     if args.mode == "consumers":
         return consumers(args)
-    if args.mode == "config1":
-        return config1(args)
+    if args.mode == "small":
+        return small_case(args)
     if args.mode == "complex":
         return complex_path(args)
 
@@ -306,7 +306,7 @@ def complex_vertex(g, No, seed):
     return z
 
 
-def config1(args):
+def small_case(args):
     """The reference's CPU-runnable case: No=10, Nv=50, NG=300, MP2 start,
     DIIS(6), 30 iterations, slices of No; the whole run on the GPU (FP64)
     beside the whole run of the FP64 CPU oracle on the host cores, and the
@@ -330,7 +330,7 @@ def config1(args):
     ref = oc.solve(g, ei, ea, **kw)
     cpu = time.perf_counter() - t0
     dE = float(np.abs(e.energies() - np.array(ref["energies"])).max())
-    print(json.dumps({"mode": "config1", "workload": "No=10 Nv=50 NG=300, MP2 start, DIIS(6), "
+    print(json.dumps({"mode": "small", "workload": "No=10 Nv=50 NG=300, MP2 start, DIIS(6), "
                       "30 iterations, slices of No", "gpu_seconds": gpu,
                       "gpu_ms_per_iteration": 1e3 * gpu / 30, "cpu_seconds": cpu,
                       "cpu_kind": "port (numpy FP64 oracle)", "cpu_threads": os.cpu_count(),
@@ -592,7 +592,7 @@ def converge(args):
 
 
 def batched(args):
-    """Config 5: independent seeded systems, No in [4,16], Nv in [30,120],
+    """Independent seeded systems (64 by default), No in [4,16], Nv in [30,120],
     NG = 4 Nv, FP64 and mixed runs to 1e-8; per-system |dE|."""
     from sisi4s_amd import engine as eng
     rng = np.random.default_rng(args.seed)

@@ -309,8 +309,8 @@ int sisi_triples_energy_blocks(sisi_engine *e, const double *T1,
                                const double *T2, int precision,
                                const int *blocks, int nblocks, double *energy);
 
-/* Many independent CCSD runs (config-5 style batches of small systems of
- * uneven No/Nv/NG) on one GPU: systems are taken from a shared queue by
+/* Many independent CCSD runs (batches of small systems of uneven
+ * No/Nv/NG) on one GPU: systems are taken from a shared queue by
  * `workers` host threads, each driving its own engine and HIP stream, so the
  * small kernels of different systems overlap on the device.  Each system is
  * ClusterSinglesDoublesAlgorithm::run (cxx:38-128) with the same opts.

@@ -1,5 +1,5 @@
 """CCSD parity: the HIP path through the C ABI against the FP64 oracle on the
-same seeded inputs.  Tolerances: FP64 path energies to 1e-8 Eh (config 1
+same seeded inputs.  Tolerances: FP64 path energies to 1e-8 Eh (the No=10, Nv=50, NG=300
 target), residual elements to 1e-11 relative to the residual's max norm."""
 
 import numpy as np
@@ -74,22 +74,22 @@ def test_residual_initial_doubles_and_energy():
         formulations.mp2_closed_form(g, ei, ea))
 
 
-def test_config1_diis6_30_iterations():
-    """Config 1: No=10, Nv=50, NG=300, MP2 start, DIIS(6), 30 iterations;
+def test_no10_nv50_ng300_diis6_30_iterations():
+    """No=10, Nv=50, NG=300, MP2 start, DIIS(6), 30 iterations;
     every iteration's energy within 1e-8 Eh of the oracle."""
     from sisi4s_amd import engine
     No, Nv, NG = 10, 50, 300
     e, g, ei, ea = make_engine(20261202, No, Nv, NG)
     opts = engine.default_opts(mixer="DiisMixer", max_residua=6, max_iterations=30,
                                amplitudes_convergence=1e-12, energy_convergence=1e-14)
-    announce("config1 DIIS(6) 30 iterations")
+    announce("No=10 Nv=50 NG=300 DIIS(6) 30 iterations")
     rep = e.ccsd_solve(opts)
     gpu = e.energies()
     ref = ccsd.solve(g, ei, ea, mixer="DiisMixer", max_residua=6, max_iterations=30,
                      amplitudes_convergence=1e-12, energy_convergence=1e-14)
     assert rep["iterations"] == 30 and len(gpu) == 30
     diff = np.abs(gpu - np.array(ref["energies"]))
-    print("config1 energies gpu/oracle max |dE| = %.3e, final %.12f / %.12f"
+    print("No=10 Nv=50 energies gpu/oracle max |dE| = %.3e, final %.12f / %.12f"
           % (diff.max(), gpu[-1], ref["energies"][-1]), flush=True)
     assert float(diff.max()) <= 1e-8, list(diff)
     T1, T2 = e.amplitudes()
@@ -99,11 +99,11 @@ def test_config1_diis6_30_iterations():
 
 def test_pair_energies_match_oracle_and_sum_to_energy():
     """PairEnergy (ClusterSinglesDoublesAlgorithm.cxx:218-235) of the
-    converged config-1 amplitudes: the oracle's pair energies, summing to the
+    converged No=10, Nv=50, NG=300 amplitudes: the oracle's pair energies, summing to the
     correlation energy."""
     from sisi4s_amd import engine
     e, g, ei, ea = make_engine(20261202, 10, 50, 300)
-    announce("pair energies config1")
+    announce("pair energies No=10 Nv=50")
     rep = e.ccsd_solve(engine.default_opts(mixer="DiisMixer", max_residua=6, max_iterations=12))
     T1, T2 = e.amplitudes()
     P = e.pair_energies()
@@ -235,7 +235,7 @@ def test_mixed_precision_solve_reaches_fp64_energy():
     assert abs(rmx["energy"] - r64["energy"]) <= 1e-6
 
 
-def config5_like_systems(n, base=20261202, no=(3, 6), nv=(10, 24)):
+def uneven_small_systems(n, base=20261202, no=(3, 6), nv=(10, 24)):
     rng = np.random.default_rng(base)
     out = []
     for s in range(n):
@@ -246,10 +246,10 @@ def config5_like_systems(n, base=20261202, no=(3, 6), nv=(10, 24)):
 
 
 def test_batched_ccsd_matches_per_system_oracle():
-    """Batched CCSD over uneven systems (config-5 style): each system's
+    """Batched CCSD over uneven small systems: each system's
     10-iteration DIIS(6) energy equals the oracle's to 1e-8 Eh."""
     from sisi4s_amd import engine
-    specs = config5_like_systems(8)
+    specs = uneven_small_systems(8)
     systems = [philox.system(seed, No, Nv, NG) for seed, No, Nv, NG in specs]
     opts = engine.default_opts(mixer="DiisMixer", max_residua=6, max_iterations=10,
                                amplitudes_convergence=0.0, energy_convergence=0.0)
@@ -264,7 +264,7 @@ def test_batched_ccsd_matches_per_system_oracle():
 
 def test_batched_mixed_precision_within_1uEh():
     from sisi4s_amd import engine
-    specs = config5_like_systems(6, base=99)
+    specs = uneven_small_systems(6, base=99)
     systems = [philox.system(seed, No, Nv, NG) for seed, No, Nv, NG in specs]
     kw = dict(mixer="DiisMixer", max_residua=6, max_iterations=200,
               amplitudes_convergence=1e-8, energy_convergence=1e-10)

@@ -56,12 +56,12 @@ def test_triples_reduced_precision():
     assert abs(eb3 - e64) <= 1e-4 * abs(e64)
 
 
-def test_triples_config1_after_ccsd():
-    """(T) on config-1 amplitudes from the engine's own CCSD run."""
+def test_triples_after_no10_nv50_ccsd():
+    """(T) on No=10, Nv=50, NG=300 amplitudes from the engine's own CCSD run."""
     from sisi4s_amd import engine
     No, Nv, NG = 10, 50, 300
     e, g, ei, ea = make(20261202, No, Nv, NG)
-    announce("config1 ccsd 10 iterations + (T)")
+    announce("No=10 Nv=50 ccsd 10 iterations + (T)")
     e.ccsd_solve(engine.default_opts(mixer="DiisMixer", max_residua=6, max_iterations=10))
     T1, T2 = e.amplitudes()
     et = e.triples()

@@ -1,5 +1,5 @@
 """The C++ step runner (host mirror of the plugin surface): input parsing on
-the CPU; end-to-end config-1 CCSD + (T) on the GPU against the oracle."""
+the CPU; end-to-end CCSD + (T) at No=10, Nv=50, NG=300 on the GPU against the oracle."""
 
 import os
 import re
@@ -11,7 +11,7 @@ import pytest
 from conftest import ROOT, announce
 
 RUNNER = os.path.join(ROOT, "sisi4s_amd", "sisi4s-mi355x")
-INPUT = os.path.join(ROOT, "tests", "inputs", "config1_ccsd_t.yaml")
+INPUT = os.path.join(ROOT, "tests", "inputs", "no10_nv50_ccsd_t.yaml")
 UEG_INPUT = os.path.join(ROOT, "tests", "inputs", "ueg_rs1_no7_nv26.yaml")
 
 
@@ -67,10 +67,10 @@ def test_runner_rejects_malformed_yaml(tmp_path):
 
 
 @pytest.mark.gpu
-def test_runner_config1_ccsd_and_triples(tmp_path):
+def test_runner_no10_nv50_ccsd_and_triples(tmp_path):
     from oracle import ccsd, philox, triples
     out = tmp_path / "out.yaml"
-    announce("runner config1 ccsd + (T)")
+    announce("runner No=10 Nv=50 ccsd + (T)")
     r = run(["-i", INPUT, "-o", str(out)])
     assert r.returncode == 0, r.stdout + r.stderr
     vals = {}
