@@ -92,6 +92,8 @@ def main():
                          "mixed runs to |dT|/|T| <= 1e-8 with (T); batched: 64 small systems")
     ap.add_argument("--systems", type=int, default=64)
     ap.add_argument("--workers", type=int, default=8)
+    ap.add_argument("--batched-refine-interval", type=int, default=0,
+                    help="batched: FP64 anchor interval of the mixed runs (0: plain FP64 after the switch)")
     ap.add_argument("--graphs", type=int, default=0,
                     help="batched: replay each system's residual from a captured hipGraph")
     ap.add_argument("--level-shift", type=float, default=0.0)
@@ -607,7 +609,10 @@ def batched(args):
     res = {}
     for name, prec in (("fp64", "fp64"), ("mixed", "bf16x3")):
         t0 = time.perf_counter()
-        ri = args.refine_interval if prec != "fp64" else 0
+        # FP32 bulk then plain FP64 iterations: at these sizes no product is
+        # large enough for the bf16 kernels, so refinement residuals (more
+        # products per iteration at FP32 speed) would cost more than FP64
+        ri = args.batched_refine_interval if prec != "fp64" else 0
         E, reps = eng.ccsd_solve_batched(systems, eng.default_opts(precision=prec,
                                                                    refine_interval=ri,
                                                                    graphs=args.graphs, **kw),
@@ -625,6 +630,7 @@ def batched(args):
     out = {"metric": METRIC, "mode": "batched",
            "config": {"workload": "%d seeded systems, No U[4,16], Nv U[30,120], NG=4Nv, DIIS(6), "
                                   "to 1e-8" % args.systems, "workers": args.workers,
+                      "mixed_refine_interval": args.batched_refine_interval,
                       "graphs": args.graphs},
            "fp64_seconds": res["fp64"]["seconds"], "mixed_seconds": res["mixed"]["seconds"],
            "speedup": res["fp64"]["seconds"] / res["mixed"]["seconds"],
