@@ -622,3 +622,22 @@ def test_complex_drccd_is_gauge_covariant():
     assert rr["iterations"] == rz["iterations"]
     assert np.abs(ez.real - np.array(rr["energies"])).max() <= 1e-12
     assert np.abs(ez.imag).max() <= 1e-12
+
+
+def test_linear_mixer_fixed_point_and_unit_ratio():
+    """LinearMixer (LinearMixer.cxx:30-55): with ratio 1 it is the unmixed
+    update, i.e. DIIS with one vector (weight 1) iteration for iteration; with
+    ratio 0.6 on a weakly correlated system it converges to the same energy
+    as DIIS (mixing changes the path, not the fixed point)."""
+    g, ei, ea = philox.system(51, 3, 10, 30)
+    g = 0.5 * g   # weak coupling: the plain update converges
+    kw = dict(max_iterations=12, amplitudes_convergence=0.0, energy_convergence=0.0)
+    r1 = ccsd.solve(g, ei, ea, mixer="LinearMixer", mixing_ratio=1.0, **kw)
+    d1 = ccsd.solve(g, ei, ea, mixer="DiisMixer", max_residua=1, **kw)
+    assert np.abs(np.array(r1["energies"]) - np.array(d1["energies"])).max() <= 1e-14
+    conv = dict(amplitudes_convergence=1e-11, energy_convergence=1e-13)
+    rl = ccsd.solve(g, ei, ea, mixer="LinearMixer", mixing_ratio=0.6, max_iterations=300, **conv)
+    rd = ccsd.solve(g, ei, ea, mixer="DiisMixer", max_residua=6, max_iterations=300, **conv)
+    assert rl["converged"] and rd["converged"]
+    assert abs(rl["energy"] - rd["energy"]) <= 1e-10 * abs(rd["energy"])
+    assert rl["iterations"] != rd["iterations"]
