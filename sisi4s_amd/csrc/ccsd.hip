@@ -777,6 +777,8 @@ bool graph_residual(Engine &e, int slice, const double *T1, const double *T2, do
     ccsd_residual(e, 1, e.run_initial, slice, g.T1.get(), g.T2.get(), g.R1.get(), g.R2.get());
     e.sync();
     hipGraph_t graph = nullptr;
+    // no device-wide operation of another thread may run during the capture
+    std::lock_guard<std::recursive_mutex> lock(device_wide_mutex());
     e.pool.begin_capture();
     hipError_t err = hipStreamBeginCapture(e.stream, hipStreamCaptureModeRelaxed);
     if (err == hipSuccess) {

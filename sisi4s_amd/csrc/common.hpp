@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <map>
+#include <mutex>
 #include <set>
 #include <stdexcept>
 #include <string>
@@ -43,6 +44,14 @@ void set_last_error(const std::string &msg);
 // Device memory pool: buffers are recycled by exact byte size, so the
 // allocation pattern of one CCSD iteration is served without hipMalloc after
 // the first iteration.  Not thread-safe (one engine, one host thread).
+// Device-wide operations (hipDeviceSynchronize, hipFree, stream creation and
+// destruction) are not allowed while another thread captures its stream into
+// a hipGraph: they take this process-wide lock, and so does a capture.
+inline std::recursive_mutex &device_wide_mutex() {
+  static std::recursive_mutex m;
+  return m;
+}
+
 class Pool {
  public:
   ~Pool() { release_all(); }
@@ -117,6 +126,7 @@ class Pool {
     }
   }
   void trim() {
+    std::lock_guard<std::recursive_mutex> lock(device_wide_mutex());
     (void)hipDeviceSynchronize();
     for (auto &kv : free_)
       for (void *p : kv.second) {
@@ -126,6 +136,7 @@ class Pool {
     free_.clear();
   }
   void release_all() {
+    std::lock_guard<std::recursive_mutex> lock(device_wide_mutex());
     (void)hipDeviceSynchronize();
     for (auto &kv : size_) (void)hipFree(kv.first);
     size_.clear();

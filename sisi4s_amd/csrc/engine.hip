@@ -92,7 +92,10 @@ Engine::Engine(const sisi_dims &d) {
   SISI_HIP(hipGetDevice(&device));
   SISI_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
   if (cus <= 0) cus = 256;
-  SISI_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  {
+    std::lock_guard<std::recursive_mutex> lock(device_wide_mutex());
+    SISI_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  }
   timers.reset(new Timers(stream));
 }
 
@@ -112,6 +115,7 @@ void Engine::drop_residual_graph() {
 }
 
 Engine::~Engine() {
+  std::lock_guard<std::recursive_mutex> lock(device_wide_mutex());
   (void)hipStreamSynchronize(stream);
   drop_residual_graph();
   timers.reset();
@@ -137,6 +141,7 @@ Engine::~Engine() {
   Vbaij.reset();
   Uabij.reset();
   drop_triples_cache();
+  blaslt.reset();  // library handle and workspace, under the device-wide lock
   pool.release_all();
   if (stream) (void)hipStreamDestroy(stream);
 }

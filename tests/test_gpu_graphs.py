@@ -148,3 +148,30 @@ def test_graphed_batched_solve_matches_eager():
     E1, r1 = engine.ccsd_solve_batched(systems, engine.default_opts(graphs=1, **kw), workers=3)
     assert [r["iterations"] for r in r0] == [r["iterations"] for r in r1]
     assert np.abs(E1 - E0).max() <= TOL
+
+
+def test_graphed_direct_ring_and_spin_orbital_residuals():
+    from sisi4s_amd import engine
+    from oracle import uccsd
+    No, Nv, NG = 4, 12, 40
+    g, ei, ea = philox.system(17, No, Nv, NG)
+    kw = fixed_iterations(8)
+    e = engine.Engine(No, Nv, NG)
+    e.set_system(g, ei, ea)
+    e.set_drccd(adjacent_pairs_exchange=True)
+    announce("graphs direct-ring CCD")
+    _, A = run(e, **kw)
+    _, B = run(e, graphs=1, **kw)
+    assert np.abs(A - B).max() <= TOL
+    o, v = 2, 5
+    gs, eis, eas = philox.system(18, o, v, 14)
+    ints = uccsd.spin_orbital_integrals(gs, o, v)
+    u = engine.Engine(2 * o, 2 * v, 1)
+    u.set_eigenenergies(np.repeat(eis, 2), np.repeat(eas, 2))
+    for c, V in ints.items():
+        u.set_integrals(getattr(engine, c), V)
+    u.set_formulation(engine.FORMULATION_UCCSD)
+    announce("graphs spin-orbital CCSD")
+    _, A = run(u, **kw)
+    _, B = run(u, graphs=1, **kw)
+    assert np.abs(A - B).max() <= TOL
