@@ -467,3 +467,40 @@ def test_consumer_steps_reject_what_they_do_not_build(tmp_path):
         "PPHHCoulombIntegrals: $Z, PHHPCoulombIntegrals: $Z, HHPPCoulombIntegrals: $Z}\n"
         "  out: {DrccdEnergy: $E}\n")
     assert r.returncode != 0 and "real integrals" in r.stderr
+
+
+def test_antisymmetrizer_with_all_sixteen_classes(tmp_path):
+    """The reference's full set of sixteen classes (as in its spin-orbital
+    workflows), unrestricted and antisymmetrized in the runner, equals the
+    blocks of the antisymmetrized spin-orbital tensor <pq||rs> (norms, and
+    the class picks among several exchange partners)."""
+    from oracle import ccsd
+    No, Nv = 2, 3
+    g, ei, ea = philox.system(29, No, Nv, 9)
+    eri = ccsd.ein("Gpr,Gqs->pqrs", g, g)           # spatial <pq|rs>
+    H, P = slice(0, No), slice(No, No + Nv)
+    names = ["HHHH", "HHHP", "HHPH", "HHPP", "HPHH", "HPHP", "HPPH", "HPPP",
+             "PHHH", "PHHP", "PHPH", "PHPP", "PPHH", "PPHP", "PPPH", "PPPP"]
+    blk = lambda c: eri[tuple(H if x == "H" else P for x in c)]
+    # spin-orbital antisymmetrized reference, 2p + s inside each block
+    n = 2 * (No + Nv)
+    so = [(p, s) for p in range(No) for s in (0, 1)] + [(p, s) for p in range(No, No + Nv) for s in (0, 1)]
+    sp = np.array([p for p, _ in so]); sg = np.array([s for _, s in so])
+    full = eri[np.ix_(sp, sp, sp, sp)] * ((sg[:, None, None, None] == sg[None, None, :, None])
+                                          & (sg[None, :, None, None] == sg[None, None, None, :]))
+    asym = full - np.swapaxes(full, 2, 3)
+    h2, p2 = slice(0, 2 * No), slice(2 * No, n)
+    steps = []
+    for c in names:
+        write_cc4s_tensor(tmp_path / c, blk(c))
+        steps.append("- name: Read\n  in: {fileName: \"%s\"}\n  out: {destination: $%s}\n"
+                     "- name: TensorUnrestricter\n  in: {Data: $%s}\n  out: {Out: $%s}\n"
+                     % (tmp_path / (c + ".yaml"), c, c, c))
+    steps.append("- name: TensorAntisymmetrizer\n  in: {%s}\n"
+                 % ", ".join("%sCoulombIntegrals: $%s" % (c, c) for c in names))
+    for c in names:
+        steps.append("- name: TensorNorm\n  in: {Data: $%s}\n  out: {Norm: $N}\n" % c)
+    r, vals = run_steps(tmp_path, "".join(steps))
+    assert r.returncode == 0, r.stdout + r.stderr
+    expect = [np.linalg.norm(asym[tuple(h2 if x == "H" else p2 for x in c)]) for c in names]
+    assert vals["Norm"] == pytest.approx(expect, rel=1e-13)
