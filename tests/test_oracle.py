@@ -641,3 +641,46 @@ def test_linear_mixer_fixed_point_and_unit_ratio():
     assert rl["converged"] and rd["converged"]
     assert abs(rl["energy"] - rd["energy"]) <= 1e-10 * abs(rd["energy"])
     assert rl["iterations"] != rd["iterations"]
+
+
+# --- IP-EOM-CCSD right-apply (UCcsdIPEquationOfMotionDavidson) ---------------
+
+def test_ip_eom_is_exact_for_two_electrons():
+    """Two electrons, generic spin-orbital integrals: every IP-EOM-CCSD
+    eigenvalue equals E(N-1) - E(N) from full CI."""
+    from oracle import eom, uccsd
+    rng = np.random.default_rng(7)
+    n, NG = 8, 12
+    g = rng.normal(size=(NG, n, n)) * 0.25
+    g = 0.5 * (g + np.transpose(g, (0, 2, 1)))
+    eri = np.einsum("Gpr,Gqs->pqrs", g, g)
+    asym = eri - np.swapaxes(eri, 2, 3)
+    eps = np.concatenate([[-1.1, -0.8], np.sort(rng.uniform(0.3, 2.0, n - 2))])
+    ints = uccsd.classes(asym, 2)
+    r = uccsd.solve(ints, eps[:2], eps[2:], max_residua=6, max_iterations=300,
+                    amplitudes_convergence=1e-12, energy_convergence=1e-14)
+    assert r["converged"]
+    ev = eom.ip_eigenvalues(ints, eps[:2], eps[2:], r["T1"], r["T2"])
+    h = np.diag(eps) - np.einsum("pkqk->pq", asym[:, :2, :, :2])
+    dets = [(p, q) for p in range(n) for q in range(p + 1, n)]
+    H = np.array([[h[p, s] * (q == t) - h[p, t] * (q == s) - h[q, s] * (p == t)
+                   + h[q, t] * (p == s) + asym[p, q, s, t] for (s, t) in dets] for (p, q) in dets])
+    exact = np.sort(np.linalg.eigvalsh(h) - np.linalg.eigvalsh(H)[0])
+    assert np.abs(ev - exact).max() <= 1e-11
+
+
+def test_ip_eom_closed_shell_spin_degeneracy():
+    """A closed-shell system in spin orbitals: the IP spectrum comes in
+    degenerate spin pairs (doublets), and the lowest root lies near minus the
+    highest occupied orbital energy (Koopmans) for a weak interaction."""
+    from oracle import eom, uccsd
+    g, ei, ea = philox.system(61, 2, 4, 12)
+    g = 0.3 * g
+    ints = uccsd.spin_orbital_integrals(g, 2, 4)
+    epi, epa = np.repeat(ei, 2), np.repeat(ea, 2)
+    r = uccsd.solve(ints, epi, epa, max_residua=6, max_iterations=200,
+                    amplitudes_convergence=1e-12, energy_convergence=1e-14)
+    ev = eom.ip_eigenvalues(ints, epi, epa, r["T1"], r["T2"])
+    low = ev[:4]
+    assert abs(low[0] - low[1]) <= 1e-9 and abs(low[2] - low[3]) <= 1e-9
+    assert abs(low[0] + ei.max()) <= 0.1 * abs(ei.max())
