@@ -224,6 +224,16 @@ int sisi_engine_set_integrals(sisi_engine *e, int which, const double *V);
  * `direct`, exchange 0). */
 int sisi_engine_set_formulation(sisi_engine *e, int formulation);
 
+/* IP-EOM-CCSD right-apply, the product UCcsdIPEquationOfMotionDavidson
+ * (UCcsdIPEquationOfMotionDavidson.cxx:43-205) repeats in its Davidson
+ * iterations: SimilarityTransformedHamiltonian::right_apply_CCSD_IP
+ * (SimilarityTransformedHamiltonian.cxx:64-68) with CCSD dressing, FP64, on an
+ * engine in SISI_FORMULATION_UCCSD (spin orbitals, canonical orbitals).
+ * T1[Nv,No], T2[Nv,Nv,No,No]: the CCSD amplitudes; r1[No], r2[No,No,Nv]
+ * (antisymmetric in the two holes), first index fastest; s1, s2 alike. */
+int sisi_eom_ip_sigma(sisi_engine *e, const double *T1, const double *T2, const double *r1,
+                      const double *r2, double *s1, double *s2);
+
 /* Arguments of DrccdEnergyFromCoulombIntegrals (cxx:41-65): unrestricted
  * (spins = 1 instead of 2, also in the energy of
  * ClusterSinglesDoublesAlgorithm.cxx:133,163-170), linearized (no quadratic

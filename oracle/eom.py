@@ -13,7 +13,7 @@ Pinned by a closed form: for two electrons CCSD and IP-EOM-CCSD are exact,
 so the eigenvalues equal E(N-1) - E(N) from full CI (the one-electron
 Hamiltonian's eigenvalues minus the two-electron ground state from
 Slater-Condon rules); a sign error in one ring term of W_mbij moves them by
-2e-3, so the check detects such mistakes.  The device path is not built yet.
+2e-3, so the check detects such mistakes.
 """
 
 import numpy as np

@@ -68,6 +68,10 @@ bool uccsd_ready(const Engine &e);
 void uccsd_residual(Engine &e, int iteration, bool initial_doubles, const double *T1,
                     const double *T2, double *R1, double *R2);
 double uccsd_energy(Engine &e, const double *T1, const double *T2);
+// IP-EOM-CCSD right-apply (SimilarityTransformedHamiltonian.cxx:64-68): s =
+// Hbar r on the 1h + 2h1p space, r1[i], r2[i,j,a], spin-orbital engine
+void eom_ip_sigma(Engine &e, const double *T1, const double *T2, const double *r1,
+                  const double *r2, double *s1, double *s2);
 void build_vabcd(Engine &e);
 void ccsd_residual_full(Engine &e, int iteration, bool initial_doubles, const double *T1,
                         const double *T2, double *R1, double *R2);

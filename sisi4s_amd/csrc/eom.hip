@@ -1,0 +1,137 @@
+// IP-EOM-CCSD right-apply on the device: SimilarityTransformedHamiltonian::
+// right_apply_CCSD_IP (SimilarityTransformedHamiltonian.cxx:64-68), the
+// product that UCcsdIPEquationOfMotionDavidson (cxx:43-205) repeats inside
+// its Davidson iterations.  Spin orbitals, canonical orbitals, the six
+// antisymmetrized classes of the spin-orbital CCSD engine (PPHH V_abij, HHHH
+// V_ijkl, HHHP V_ijka, PPPH V_abci, PHPH V_aibj), amplitudes T1[a,i],
+// T2[a,b,i,j]; R = (r1[i], r2[i,j,a]) with r2 antisymmetric in i, j.  The
+// similarity-transformed blocks follow the intermediates of Stanton and Gauss
+// (J. Chem. Phys. 101, 8938 (1994)); every product runs in FP64 on the
+// contraction planner.
+#include "ccsd.hpp"
+#include "contract.hpp"
+#include "tensor_ops.hpp"
+
+namespace sisi {
+
+namespace {
+
+// diagonal of an n x n matrix += eps
+void add_diagonal(Engine &e, double *M, int64_t n, const double *eps) {
+  View d;
+  d.p = M;
+  d.order = 1;
+  d.len[0] = n;
+  d.stride[0] = n + 1;
+  copy(e, d, dense(const_cast<double *>(eps), {n}), 1.0, 1.0);
+}
+
+}  // namespace
+
+void eom_ip_sigma(Engine &e, const double *T1c, const double *T2c, const double *r1c,
+                  const double *r2c, double *s1p, double *s2p) {
+  SISI_REQUIRE(e.formulation == SISI_FORMULATION_UCCSD && uccsd_ready(e), SISI_ERR_STATE,
+               "the IP right-apply needs the spin-orbital CCSD integrals (SISI_FORMULATION_UCCSD)");
+  SISI_REQUIRE(e.eps_i.get() && e.eps_a.get(), SISI_ERR_STATE, "eigenenergies not set");
+  const int64_t o = e.No, v = e.Nv, n2 = v * v * o * o;
+  const int p = SISI_FP64;
+  double *T1p = const_cast<double *>(T1c), *T2p = const_cast<double *>(T2c);
+  View T1 = dense(T1p, {v, o}), T2 = dense(T2p, {v, v, o, o});
+  View r1 = dense(const_cast<double *>(r1c), {o}), r2 = dense(const_cast<double *>(r2c), {o, o, v});
+  View s1 = dense(s1p, {o}), s2 = dense(s2p, {o, o, v});
+  View V = dense(e.Vabij.get(), {v, v, o, o}), W4o = dense(e.Vijkl.get(), {o, o, o, o}),
+       Q = dense(e.Vijka.get(), {o, o, o, v}), P3 = dense(e.Vabci.get(), {v, v, v, o}),
+       X = dense(e.Vaibj.get(), {v, o, v, o});
+  e.thaw();
+
+  // tau~ = T2 + (T_ai T_bj - T_bi T_aj) / 2, tau = T2 + T_ai T_bj - T_bi T_aj
+  DBuf O1b(&e.pool, n2), taub(&e.pool, n2), tautb(&e.pool, n2);
+  View O1 = dense(O1b.get(), {v, v, o, o}), tau = dense(taub.get(), {v, v, o, o}),
+       taut = dense(tautb.get(), {v, v, o, o});
+  zero(e, O1b.get(), n2);
+  add_t1t1(e, O1b.get(), O1b.get(), T1p, 1.0, int(v), int(o));
+  copy(e, tau, T2);
+  sum(e, 1.0, O1, "abij", 1.0, tau, "abij");
+  sum(e, -1.0, O1, "baij", 1.0, tau, "abij");
+  copy(e, taut, T2);
+  sum(e, 0.5, O1, "abij", 1.0, taut, "abij");
+  sum(e, -0.5, O1, "baij", 1.0, taut, "abij");
+
+  // one-body blocks
+  DBuf Fmeb(&e.pool, o * v), Foob(&e.pool, o * o), Fvvb(&e.pool, v * v);
+  View Fme = dense(Fmeb.get(), {o, v}), Foo = dense(Foob.get(), {o, o}), Fvv = dense(Fvvb.get(), {v, v});
+  ctr(e, 1.0, "fn,efmn->me", T1, V, 0.0, Fme, p);        // t_nf <mn||ef>
+  ctr(e, 1.0, "en,mnie->mi", T1, Q, 0.0, Foo, p);        // t_ne <mn||ie>
+  ctr(e, 0.5, "efin,efmn->mi", taut, V, 1.0, Foo, p);    // tau~_inef <mn||ef> / 2
+  ctr(e, 0.5, "ei,me->mi", T1, Fme, 1.0, Foo, p);        // t_ie F_me / 2
+  add_diagonal(e, Foob.get(), o, e.eps_i.get());          // f_mi
+  ctr(e, -1.0, "fm,feam->ae", T1, P3, 0.0, Fvv, p);      // t_mf <ma||fe>
+  ctr(e, -0.5, "afmn,efmn->ae", taut, V, 1.0, Fvv, p);   // -tau~_mnaf <mn||ef> / 2
+  ctr(e, -0.5, "am,me->ae", T1, Fme, 1.0, Fvv, p);       // -t_ma F_me / 2
+  add_diagonal(e, Fvvb.get(), v, e.eps_a.get());          // f_ae
+
+  // W_mnij = <mn||ij> + P(ij) t_je <mn||ie> + tau_ijef <mn||ef> / 2
+  DBuf Woob(&e.pool, o * o * o * o), Ab(&e.pool, o * o * o * o);
+  View Woooo = dense(Woob.get(), {o, o, o, o}), A = dense(Ab.get(), {o, o, o, o});
+  copy(e, Woooo, W4o);
+  ctr(e, 1.0, "ej,mnie->mnij", T1, Q, 0.0, A, p);
+  sum(e, 1.0, A, "mnij", 1.0, Woooo, "mnij");
+  sum(e, -1.0, A, "mnji", 1.0, Woooo, "mnij");
+  ctr(e, 0.5, "efij,efmn->mnij", tau, V, 1.0, Woooo, p);
+
+  // W_mbej = <mb||ej> + t_jf <mb||ef> - t_nb <mn||ej> - (t_jnfb + t_jf t_nb) <mn||ef>
+  DBuf Wvob(&e.pool, o * v * v * o), Sb(&e.pool, n2);
+  View Wovvo = dense(Wvob.get(), {o, v, v, o}), S = dense(Sb.get(), {v, v, o, o});
+  sum(e, -1.0, X, "bmej", 0.0, Wovvo, "mbej");
+  ctr(e, -1.0, "fj,efbm->mbej", T1, P3, 1.0, Wovvo, p);
+  ctr(e, 1.0, "bn,mnje->mbej", T1, Q, 1.0, Wovvo, p);
+  add_t1t1(e, Sb.get(), T2p, T1p, 1.0, int(v), int(o));  // S_fbjn = t_jnfb + t_jf t_nb
+  ctr(e, -1.0, "fbjn,efmn->mbej", S, V, 1.0, Wovvo, p);
+
+  // W_mnie = <mn||ie> + t_if <mn||fe>
+  DBuf Wooob(&e.pool, o * o * o * v);
+  View Wooov = dense(Wooob.get(), {o, o, o, v});
+  copy(e, Wooov, Q);
+  ctr(e, 1.0, "fi,femn->mnie", T1, V, 1.0, Wooov, p);
+
+  // W_mbij = <mb||ij> - F_me t_ijbe - t_nb W_mnij + <mb||ef> tau_ijef / 2
+  //          + P(ij) <mn||ie> t_jnbe + P(ij) t_ie (<mb||ej> - t_njbf <mn||ef>)
+  DBuf Wovoob(&e.pool, o * v * o * o), tmpb(&e.pool, o * v * o * o), Yb(&e.pool, o * v * v * o);
+  View Wovoo = dense(Wovoob.get(), {o, v, o, o}), tmp = dense(tmpb.get(), {o, v, o, o}),
+       Y = dense(Yb.get(), {o, v, v, o});
+  sum(e, 1.0, Q, "ijmb", 0.0, Wovoo, "mbij");
+  ctr(e, -1.0, "me,beij->mbij", Fme, T2, 1.0, Wovoo, p);
+  ctr(e, -1.0, "bn,mnij->mbij", T1, Woooo, 1.0, Wovoo, p);
+  ctr(e, -0.5, "efbm,efij->mbij", P3, tau, 1.0, Wovoo, p);
+  ctr(e, 1.0, "mnie,bejn->mbij", Q, T2, 0.0, tmp, p);
+  sum(e, 1.0, tmp, "mbij", 1.0, Wovoo, "mbij");
+  sum(e, -1.0, tmp, "mbji", 1.0, Wovoo, "mbij");
+  ctr(e, -1.0, "ei,bmej->mbij", T1, X, 0.0, tmp, p);
+  ctr(e, 1.0, "bfnj,efmn->mbej", T2, V, 0.0, Y, p);     // t_njbf <mn||ef>
+  ctr(e, -1.0, "ei,mbej->mbij", T1, Y, 1.0, tmp, p);
+  sum(e, 1.0, tmp, "mbij", 1.0, Wovoo, "mbij");
+  sum(e, -1.0, tmp, "mbji", 1.0, Wovoo, "mbij");
+
+  // sigma_1 = -F_mi r_m + F_me r_mie - W_nmie r_mne / 2
+  ctr(e, -1.0, "mi,m->i", Foo, r1, 0.0, s1, p);
+  ctr(e, 1.0, "me,mie->i", Fme, r2, 1.0, s1, p);
+  ctr(e, -0.5, "nmie,mne->i", Wooov, r2, 1.0, s1, p);
+  // sigma_2 = -W_maji r_m + F_ae r_ije - P(ij) F_mi r_mja + W_mnij r_mna / 2
+  //           + P(ij) W_maei r_mje + (<mn||ef> r_mnf / 2) t_ijae
+  DBuf t3b(&e.pool, o * o * v), gb(&e.pool, v);
+  View t3 = dense(t3b.get(), {o, o, v}), gv = dense(gb.get(), {v});
+  ctr(e, -1.0, "maji,m->ija", Wovoo, r1, 0.0, s2, p);
+  ctr(e, 1.0, "ae,ije->ija", Fvv, r2, 1.0, s2, p);
+  ctr(e, 1.0, "mi,mja->ija", Foo, r2, 0.0, t3, p);
+  sum(e, -1.0, t3, "ija", 1.0, s2, "ija");
+  sum(e, 1.0, t3, "jia", 1.0, s2, "ija");
+  ctr(e, 0.5, "mnij,mna->ija", Woooo, r2, 1.0, s2, p);
+  ctr(e, 1.0, "maei,mje->ija", Wovvo, r2, 0.0, t3, p);
+  sum(e, 1.0, t3, "ija", 1.0, s2, "ija");
+  sum(e, -1.0, t3, "jia", 1.0, s2, "ija");
+  ctr(e, 0.5, "efmn,mnf->e", V, r2, 0.0, gv, p);
+  ctr(e, 1.0, "e,aeij->ija", gv, T2, 1.0, s2, p);
+  e.thaw();
+}
+
+}  // namespace sisi

@@ -95,6 +95,8 @@ SIGNATURES = [
     ("sisi_set_bf16_backend", ctypes.c_int, [ctypes.c_int]),
     ("sisi_engine_set_formulation", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("sisi_engine_set_drccd", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    ("sisi_eom_ip_sigma", ctypes.c_int, [ctypes.c_void_p, _c_double_p, _c_double_p, _c_double_p,
+                                         _c_double_p, _c_double_p, _c_double_p]),
     ("sisi_engine_set_coulomb_factors", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _c_double_p, _c_double_p, _c_double_p]),
     ("sisi_ccsd_residual", ctypes.c_int,
@@ -394,6 +396,15 @@ class Engine:
         _check(self._lib.sisi_engine_set_drccd(self._h, int(bool(unrestricted)), int(bool(linearized)),
                                                int(bool(adjacent_pairs_exchange))))
         self.set_formulation(FORMULATION_DRCCD)
+
+    def eom_ip_sigma(self, T1, T2, r1, r2):
+        """IP-EOM-CCSD right-apply (SimilarityTransformedHamiltonian.cxx:64-68)
+        on a spin-orbital engine: returns (s1[i], s2[i,j,a])."""
+        o, v = self.No, self.Nv
+        b1, b2 = np.empty(o), np.empty(o * o * v)
+        _check(self._lib.sisi_eom_ip_sigma(self._h, _ptr(_buf(T1)), _ptr(_buf(T2)), _ptr(_buf(r1)),
+                                           _ptr(_buf(r2)), _ptr(b1), _ptr(b2)))
+        return _unbuf(b1, (o,)), _unbuf(b2, (o, o, v))
 
     def set_residual_variant(self, ppl=True, distinguishable=False):
         """ppl / distinguishable of the next residual calls
