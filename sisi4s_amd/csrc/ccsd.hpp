@@ -65,6 +65,8 @@ void drccd_residual(Engine &e, int iteration, bool initial_doubles, const double
 bool integrals_ready(const Engine &e);
 // Spin-orbital CCSD (UccsdAmplitudesFromCoulombIntegrals.cxx:107-241)
 bool uccsd_ready(const Engine &e);
+// the classes of the IP right-apply: those of uccsd_ready but PPPP
+bool ip_eom_ready(const Engine &e);
 void uccsd_residual(Engine &e, int iteration, bool initial_doubles, const double *T1,
                     const double *T2, double *R1, double *R2);
 double uccsd_energy(Engine &e, const double *T1, const double *T2);

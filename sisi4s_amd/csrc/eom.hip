@@ -30,8 +30,9 @@ void add_diagonal(Engine &e, double *M, int64_t n, const double *eps) {
 
 void eom_ip_sigma(Engine &e, const double *T1c, const double *T2c, const double *r1c,
                   const double *r2c, double *s1p, double *s2p) {
-  SISI_REQUIRE(e.formulation == SISI_FORMULATION_UCCSD && uccsd_ready(e), SISI_ERR_STATE,
-               "the IP right-apply needs the spin-orbital CCSD integrals (SISI_FORMULATION_UCCSD)");
+  SISI_REQUIRE(e.formulation == SISI_FORMULATION_UCCSD && ip_eom_ready(e), SISI_ERR_STATE,
+               "the IP right-apply needs the antisymmetrized PPHH, HHHH, HHHP, PPPH and PHPH "
+               "integrals (SISI_FORMULATION_UCCSD)");
   SISI_REQUIRE(e.eps_i.get() && e.eps_a.get(), SISI_ERR_STATE, "eigenenergies not set");
   const int64_t o = e.No, v = e.Nv, n2 = v * v * o * o;
   const int p = SISI_FP64;

@@ -1,9 +1,11 @@
 // The hot-path algorithms with the reference's names, arguments, defaults
 // and outputs, evaluated by the engine through the C ABI.
 #include <cmath>
+#include <fstream>
 #include <sstream>
 
 #include "algorithm.hpp"
+#include "davidson.hpp"
 
 namespace sisi_host {
 
@@ -514,6 +516,136 @@ class CcsdPerturbativeTriplesComplex : public Algorithm {
   }
 };
 
+// UCcsdIPEquationOfMotionDavidson (UCcsdIPEquationOfMotionDavidson.cxx:25-206):
+// the lowest IP-EOM-CCSD roots of spin-orbital CCSD amplitudes by the host
+// Davidson driver (EigenSystemDavidsonMono), each right-apply on the device
+// (sisi_eom_ip_sigma).  Inputs as the reference's: the eigenenergies, the
+// SinglesAmplitudes [Nv,No] and DoublesAmplitudes [Nv,Nv,No,No], and the
+// antisymmetrized HHPP, HHHH, HHHP, HPPP and HPHP classes (the reference's
+// PHPP, HPHH, HPPH and PHHP are these with indices reordered and are not
+// read).  The vector space is r1[i] and r2[i,j,a] for i < j; the
+// preconditioner diagonal -f_ii and -f_ii - f_jj + f_aa is
+// IPCcsdPreconditioner's (CcsdPreconditioner.cxx:546-560).  Emits the
+// eigenvalues and writes IPEomEnergies.dat ("k re im" per root) as the
+// reference does (cxx:199-205).
+class UCcsdIPEquationOfMotionDavidson : public Algorithm {
+ public:
+  using Algorithm::Algorithm;
+  std::string getName() override { return "UCcsdIPEquationOfMotionDavidson"; }
+  void run() override {
+    if (isArgumentGiven("HPFockMatrix") || isArgumentGiven("HHFockMatrix") ||
+        isArgumentGiven("PPFockMatrix"))
+      throw StepError("non-canonical orbitals (Fock matrices) are not supported");
+    for (const char *unsupported : {"refreshIterations", "oneBodyRdmRange", "printEigenvectorsRange"})
+      if (!getTextArgument(unsupported, "").empty())
+        throw StepError(std::string(unsupported) + " is not supported");
+    const auto &ei = getTensorArgument("HoleEigenEnergies");
+    const auto &ea = getTensorArgument("ParticleEigenEnergies");
+    const int64_t o = ei.lens[0], v = ea.lens[0];
+    const auto &T1 = getTensorArgument("SinglesAmplitudes");
+    const auto &T2 = getTensorArgument("DoublesAmplitudes");
+    if (T1.complex || T2.complex || T1.lens != std::vector<int64_t>{v, o} ||
+        T2.lens != std::vector<int64_t>{v, v, o, o})
+      throw StepError("the amplitudes must be real with shapes [Nv,No] and [Nv,Nv,No,No]");
+    DavidsonOptions opt;
+    opt.nroots = int(getIntegerArgument("eigenstates", 1));
+    opt.tolerance_energy = getRealArgument("energyConvergence", 1e-6);
+    opt.tolerance_vectors = getRealArgument("amplitudesConvergence", 1e-6);
+    opt.max_iterations = int(getIntegerArgument("maxIterations", 32));
+    opt.min_iterations = int(getIntegerArgument("minIterations", 1));
+    opt.max_basis = int(getIntegerArgument("maxBasisSize", o * v + (o * (o - 1) / 2) * (v * (v - 1) / 2)));
+    opt.refresh_on_max_basis = getIntegerArgument("refreshOnMaxBasisSize", 0) == 1;
+
+    sisi_dims d{int(o), int(v), 1};
+    struct Owned {
+      sisi_engine *e = nullptr;
+      ~Owned() { if (e) sisi_engine_destroy(e); }
+    } owned;
+    check(sisi_engine_create(&d, &owned.e), "sisi_engine_create");
+    sisi_engine *e = owned.e;
+    check(sisi_engine_set_eigenenergies(e, ei.values.data(), ea.values.data()),
+          "sisi_engine_set_eigenenergies");
+    // reference class -> engine class, real antisymmetrized integrals:
+    // V_abij = V_ijab, V_abci = -V_icab, V_aibj = V_iajb
+    auto in = [&](const char *name, std::vector<int64_t> lens) -> const Tensor & {
+      const auto &t = getTensorArgument(name);
+      if (t.complex || t.lens != lens) throw StepError(std::string(name) + " has the wrong shape");
+      return t;
+    };
+    const Tensor &hhpp = in("HHPPCoulombIntegrals", {o, o, v, v});
+    const Tensor &hhhh = in("HHHHCoulombIntegrals", {o, o, o, o});
+    const Tensor &hhhp = in("HHHPCoulombIntegrals", {o, o, o, v});
+    const Tensor &hppp = in("HPPPCoulombIntegrals", {o, v, v, v});
+    const Tensor &hphp = in("HPHPCoulombIntegrals", {o, v, o, v});
+    std::vector<double> buf(static_cast<size_t>(std::max(v * v * v * o, v * v * o * o)));
+    for (int64_t j = 0; j < o; ++j)
+      for (int64_t i = 0; i < o; ++i)
+        for (int64_t b = 0; b < v; ++b)
+          for (int64_t a = 0; a < v; ++a)
+            buf[size_t(a + v * (b + v * (i + o * j)))] = hhpp.values[size_t(i + o * (j + o * (a + v * b)))];
+    check(sisi_engine_set_integrals(e, SISI_PPHH, buf.data()), "sisi_engine_set_integrals");
+    check(sisi_engine_set_integrals(e, SISI_HHHH, hhhh.values.data()), "sisi_engine_set_integrals");
+    check(sisi_engine_set_integrals(e, SISI_HHHP, hhhp.values.data()), "sisi_engine_set_integrals");
+    for (int64_t i = 0; i < o; ++i)
+      for (int64_t c = 0; c < v; ++c)
+        for (int64_t b = 0; b < v; ++b)
+          for (int64_t a = 0; a < v; ++a)
+            buf[size_t(a + v * (b + v * (c + v * i)))] = -hppp.values[size_t(i + o * (c + v * (a + v * b)))];
+    check(sisi_engine_set_integrals(e, SISI_PPPH, buf.data()), "sisi_engine_set_integrals");
+    for (int64_t j = 0; j < o; ++j)
+      for (int64_t b = 0; b < v; ++b)
+        for (int64_t i = 0; i < o; ++i)
+          for (int64_t a = 0; a < v; ++a)
+            buf[size_t(a + v * (i + o * (b + v * j)))] = hphp.values[size_t(i + o * (a + v * (j + o * b)))];
+    check(sisi_engine_set_integrals(e, SISI_PHPH, buf.data()), "sisi_engine_set_integrals");
+    check(sisi_engine_set_formulation(e, SISI_FORMULATION_UCCSD), "sisi_engine_set_formulation");
+
+    // packed vector: r1[i], then r2[i,j,a] for the pairs i < j (a fastest)
+    std::vector<std::pair<int64_t, int64_t>> pairs;
+    for (int64_t j = 0; j < o; ++j)
+      for (int64_t i = 0; i < j; ++i) pairs.emplace_back(i, j);
+    const size_t n = size_t(o) + pairs.size() * size_t(v);
+    if (opt.nroots < 1 || size_t(opt.nroots) > n) throw StepError("eigenstates must lie in [1, dimension]");
+    std::vector<double> diag(n);
+    for (int64_t i = 0; i < o; ++i) diag[size_t(i)] = -ei.values[size_t(i)];
+    for (size_t p = 0; p < pairs.size(); ++p)
+      for (int64_t a = 0; a < v; ++a)
+        diag[size_t(o) + p * size_t(v) + size_t(a)] =
+            -ei.values[size_t(pairs[p].first)] - ei.values[size_t(pairs[p].second)] + ea.values[size_t(a)];
+    const size_t no = static_cast<size_t>(o), nr2 = static_cast<size_t>(o * o * v);
+    std::vector<double> r1(no), r2(nr2), s1(no), s2(nr2);
+    auto apply = [&](const std::vector<double> &x) {
+      std::fill(r2.begin(), r2.end(), 0.0);
+      for (int64_t i = 0; i < o; ++i) r1[size_t(i)] = x[size_t(i)];
+      for (size_t p = 0; p < pairs.size(); ++p)
+        for (int64_t a = 0; a < v; ++a) {
+          const double val = x[size_t(o) + p * size_t(v) + size_t(a)];
+          const int64_t i = pairs[p].first, j = pairs[p].second;
+          r2[size_t(i + o * (j + o * a))] = val;
+          r2[size_t(j + o * (i + o * a))] = -val;
+        }
+      check(sisi_eom_ip_sigma(e, T1.values.data(), T2.values.data(), r1.data(), r2.data(), s1.data(),
+                              s2.data()),
+            "sisi_eom_ip_sigma");
+      std::vector<double> y(n);
+      for (int64_t i = 0; i < o; ++i) y[size_t(i)] = s1[size_t(i)];
+      for (size_t p = 0; p < pairs.size(); ++p)
+        for (int64_t a = 0; a < v; ++a)
+          y[size_t(o) + p * size_t(v) + size_t(a)] =
+              s2[size_t(pairs[p].first + o * (pairs[p].second + o * a))];
+      return y;
+    };
+    const DavidsonResult res = davidson(apply, diag, lowest_diagonal_basis(diag, opt.nroots), opt);
+    std::ofstream dat("IPEomEnergies.dat");
+    for (size_t k = 0; k < res.values.size(); ++k) {
+      dat << (k + 1) << " " << num(res.values[k].real()) << " " << num(res.values[k].imag()) << "\n";
+      emitted.scalars.push_back({"eigenvalue" + std::to_string(k + 1), num(res.values[k].real())});
+    }
+    emitted.scalars.push_back({"davidsonIterations", std::to_string(res.iterations)});
+    emitted.scalars.push_back({"converged", res.converged ? "true" : "false"});
+  }
+};
+
 static Registrar<GenerateSyntheticSystem> r0("GenerateSyntheticSystem");
 static Registrar<CcsdPerturbativeTriplesComplex> r5("CcsdPerturbativeTriplesComplex");
 static Registrar<CcsdEnergyFromCoulombIntegralsReference> r6("CcsdEnergyFromCoulombIntegralsReference");
@@ -523,5 +655,6 @@ static Registrar<CcsdEnergyFromCoulombIntegrals> r2("CcsdEnergyFromCoulombIntegr
 static Registrar<CcsdPerturbativeTriples> r3("CcsdPerturbativeTriples");
 static Registrar<DrccdEnergyFromCoulombIntegrals> r7("DrccdEnergyFromCoulombIntegrals");
 static Registrar<UccsdAmplitudesFromCoulombIntegrals> r8("UccsdAmplitudesFromCoulombIntegrals");
+static Registrar<UCcsdIPEquationOfMotionDavidson> r9("UCcsdIPEquationOfMotionDavidson");
 
 }  // namespace sisi_host

@@ -504,3 +504,91 @@ def test_antisymmetrizer_with_all_sixteen_classes(tmp_path):
     assert r.returncode == 0, r.stdout + r.stderr
     expect = [np.linalg.norm(asym[tuple(h2 if x == "H" else p2 for x in c)]) for c in names]
     assert vals["Norm"] == pytest.approx(expect, rel=1e-13)
+
+
+def _ip_eom_system(no, n, seed):
+    """Generic real spin-orbital integrals <pq||rs> from a random vertex, `no`
+    electrons, and their converged spin-orbital CCSD amplitudes (oracle)."""
+    from oracle import uccsd
+    rng = np.random.default_rng(seed)
+    g = rng.normal(size=(12, n, n)) * 0.25
+    g = 0.5 * (g + np.transpose(g, (0, 2, 1)))
+    eri = np.einsum("Gpr,Gqs->pqrs", g, g)
+    asym = eri - np.swapaxes(eri, 2, 3)
+    eps = np.concatenate([np.linspace(-1.3, -0.8, no), np.sort(rng.uniform(0.3, 2.0, n - no))])
+    ints = uccsd.classes(asym, no)
+    r = uccsd.solve(ints, eps[:no], eps[no:], max_residua=6, max_iterations=300,
+                    amplitudes_convergence=1e-12, energy_convergence=1e-14)
+    assert r["converged"]
+    return asym, eps, ints, r["T1"], r["T2"]
+
+
+def _ip_eom_steps(tmp_path, asym, eps, no, T1, T2, args):
+    h, p = slice(0, no), slice(no, asym.shape[0])
+    tensors = dict(I=eps[:no], A=eps[no:], T1=T1, T2=T2, HHPP=asym[h, h, p, p],
+                   HHHH=asym[h, h, h, h], HHHP=asym[h, h, h, p], HPPP=asym[h, p, p, p],
+                   HPHP=asym[h, p, h, p])
+    steps = []
+    for name, x in tensors.items():
+        write_cc4s_tensor(tmp_path / name, np.ascontiguousarray(x))
+        steps.append("- name: Read\n  in: {fileName: \"%s\"}\n  out: {destination: $%s}\n"
+                     % (tmp_path / (name + ".yaml"), name))
+    steps.append("- name: UCcsdIPEquationOfMotionDavidson\n  in: {HoleEigenEnergies: $I, "
+                 "ParticleEigenEnergies: $A, SinglesAmplitudes: $T1, DoublesAmplitudes: $T2, %s, %s}\n"
+                 % (", ".join("%sCoulombIntegrals: $%s" % (c, c)
+                              for c in ("HHPP", "HHHH", "HHHP", "HPPP", "HPHP")), args))
+    p = tmp_path / "in.yaml"
+    p.write_text("".join(steps))
+    return subprocess.run([RUNNER, "-i", str(p), "-o", str(tmp_path / "out.yaml")],
+                          capture_output=True, text=True, timeout=600, cwd=str(tmp_path))
+
+
+def test_ip_eom_step_rejects_what_it_does_not_build(tmp_path):
+    """UCcsdIPEquationOfMotionDavidson: non-canonical orbitals, refreshIterations
+    and the eigenvector / density printing ranges fail the step before any
+    device work; amplitudes of the wrong shape too."""
+    r, _ = run_steps(tmp_path, "- name: UCcsdIPEquationOfMotionDavidson\n  in: {HHFockMatrix: 1}\n")
+    assert r.returncode != 0 and "Fock" in r.stderr
+    r, _ = run_steps(tmp_path, "- name: UCcsdIPEquationOfMotionDavidson\n"
+                     "  in: {refreshIterations: \"2,4\"}\n")
+    assert r.returncode != 0 and "refreshIterations is not supported" in r.stderr
+    ei, ea = np.array([-1.0, -0.5]), np.array([0.5, 1.0, 1.5])
+    for n, x in dict(I=ei, A=ea, T1=np.zeros((2, 3))).items():  # T1 must be [Nv, No]
+        write_cc4s_tensor(tmp_path / n, x)
+    r, _ = run_steps(tmp_path, "".join(
+        "- name: Read\n  in: {fileName: \"%s\"}\n  out: {destination: $%s}\n" % (tmp_path / (n + ".yaml"), n)
+        for n in ("I", "A", "T1")) +
+        "- name: UCcsdIPEquationOfMotionDavidson\n  in: {HoleEigenEnergies: $I, "
+        "ParticleEigenEnergies: $A, SinglesAmplitudes: $T1, DoublesAmplitudes: $T1}\n")
+    assert r.returncode != 0 and "amplitudes must be real" in r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.skip(reason="the UCcsdIPEquationOfMotionDavidson step has not yet been run on a GPU")
+def test_ip_eom_davidson_step_matches_full_ci_and_the_oracle(tmp_path):
+    """The IP-EOM-CCSD Davidson step from files (Read of the reference's
+    classes and amplitudes): two electrons reproduce full CI's ionization
+    energies; four electrons match the oracle's dense spectrum."""
+    from oracle import eom
+    asym, eps, ints, T1, T2 = _ip_eom_system(2, 8, 7)
+    h = np.diag(eps) - np.einsum("pkqk->pq", asym[:, :2, :, :2])
+    dets = [(p, q) for p in range(8) for q in range(p + 1, 8)]
+    H = np.array([[h[p, s] * (q == t) - h[p, t] * (q == s) - h[q, s] * (p == t)
+                   + h[q, t] * (p == s) + asym[p, q, s, t] for (s, t) in dets] for (p, q) in dets])
+    exact = np.sort(np.linalg.eigvalsh(h) - np.linalg.eigvalsh(H)[0])
+    announce("ip-eom davidson step, 2 electrons in 8 spin orbitals")
+    r = _ip_eom_steps(tmp_path, asym, eps, 2, T1, T2,
+                      "eigenstates: 2, energyConvergence: 0.0, amplitudesConvergence: 1e-20, "
+                      "maxIterations: 40")
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = np.loadtxt(tmp_path / "IPEomEnergies.dat", ndmin=2)
+    assert np.abs(got[:, 1] - exact[:2]).max() <= 1e-9 and np.abs(got[:, 2]).max() == 0.0
+    asym, eps, ints, T1, T2 = _ip_eom_system(4, 10, 8)
+    ref = eom.ip_eigenvalues(ints, eps[:4], eps[4:], T1, T2)
+    announce("ip-eom davidson step, 4 electrons in 10 spin orbitals")
+    r = _ip_eom_steps(tmp_path, asym, eps, 4, T1, T2,
+                      "eigenstates: 3, energyConvergence: 0.0, amplitudesConvergence: 1e-20, "
+                      "maxIterations: 60")
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = np.loadtxt(tmp_path / "IPEomEnergies.dat", ndmin=2)
+    assert np.abs(got[:, 1] - ref[:3]).max() <= 1e-9
