@@ -5,6 +5,8 @@ numpy's Philox, the reference's full-integral CCSD formulation
 symmetries and limits, and the reference's own known answers for the
 uniform electron gas (rs=1, No=7, Nv=26)."""
 
+import itertools
+
 import numpy as np
 import pytest
 
@@ -684,3 +686,105 @@ def test_ip_eom_closed_shell_spin_degeneracy():
     low = ev[:4]
     assert abs(low[0] - low[1]) <= 1e-9 and abs(low[2] - low[3]) <= 1e-9
     assert abs(low[0] + ei.max()) <= 0.1 * abs(ei.max())
+
+
+# --- EA-EOM-CCSD right-apply (UCcsdEAEquationOfMotionDavidson) ---------------
+
+def _fci_energies(h, asym, k):
+    """Full CI in the k-electron determinant space of H = sum h_pq p+ q +
+    1/4 sum <pq||rs> p+ q+ s r, by applying the operators to occupation bit
+    strings (the sign of each operator: (-1) to the number of occupied
+    orbitals below it).  Small n only."""
+    n = h.shape[0]
+    dets = [sum(1 << p for p in c) for c in itertools.combinations(range(n), k)]
+    index = {d: i for i, d in enumerate(dets)}
+
+    def op(d, p, create):
+        if ((d >> p) & 1) == create:
+            return None, 0
+        return d ^ (1 << p), (-1) ** bin(d & ((1 << p) - 1)).count("1")
+
+    H = np.zeros((len(dets), len(dets)))
+    for j, d in enumerate(dets):
+        for p in range(n):
+            for q in range(n):
+                d1, s1 = op(d, q, 0)
+                d2, s2 = op(d1, p, 1) if d1 is not None else (None, 0)
+                if d2 is not None:
+                    H[index[d2], j] += s1 * s2 * h[p, q]
+        for r in range(n):
+            for s in range(r + 1, n):
+                d1, s1 = op(d, r, 0)
+                d2, s2 = op(d1, s, 0) if d1 is not None else (None, 0)
+                if d2 is None:
+                    continue
+                for p in range(n):
+                    for q in range(p + 1, n):
+                        d3, s3 = op(d2, q, 1)
+                        d4, s4 = op(d3, p, 1) if d3 is not None else (None, 0)
+                        if d4 is not None:
+                            H[index[d4], j] += s1 * s2 * s3 * s4 * asym[p, q, r, s]
+    return np.linalg.eigvalsh(H)
+
+
+def _generic_spin_orbital_system(no, n, seed):
+    rng = np.random.default_rng(seed)
+    g = rng.normal(size=(12, n, n)) * 0.25
+    g = 0.5 * (g + np.transpose(g, (0, 2, 1)))
+    eri = np.einsum("Gpr,Gqs->pqrs", g, g)
+    asym = eri - np.swapaxes(eri, 2, 3)
+    eps = np.concatenate([np.linspace(-1.3, -0.8, no), np.sort(rng.uniform(0.3, 2.0, n - no))])
+    h = np.diag(eps) - np.einsum("pkqk->pq", asym[:, :no, :, :no])  # Fock matrix = diag(eps)
+    return asym, eps, h
+
+
+def test_fci_helper_closed_forms():
+    """The determinant full CI used as the EOM pin: one electron gives the
+    eigenvalues of h; two electrons give the Slater-Condon matrix of the IP
+    test; the Hartree-Fock determinant's energy is E_ref = sum h_ii +
+    1/2 sum <ij||ij> when it is an eigenvector (no interaction)."""
+    asym, eps, h = _generic_spin_orbital_system(2, 6, 3)
+    assert np.abs(_fci_energies(h, asym, 1) - np.linalg.eigvalsh(h)).max() <= 1e-12
+    dets = [(p, q) for p in range(6) for q in range(p + 1, 6)]
+    H = np.array([[h[p, s] * (q == t) - h[p, t] * (q == s) - h[q, s] * (p == t)
+                   + h[q, t] * (p == s) + asym[p, q, s, t] for (s, t) in dets] for (p, q) in dets])
+    assert np.abs(_fci_energies(h, asym, 2) - np.linalg.eigvalsh(H)).max() <= 1e-12
+    e = np.array([-1.0, -0.5, -0.2, 0.4, 0.9])
+    assert abs(_fci_energies(np.diag(e), np.zeros((5,) * 4), 3)[0] - e[:3].sum()) <= 1e-15
+
+
+@pytest.mark.parametrize("no", [2, 3, 4])
+def test_ea_eom_is_exact_with_two_virtual_spin_orbitals(no):
+    """With two virtual spin orbitals CCSD is exact for any electron count and
+    the EA space (1p + 2p1h) spans every (N+1)-electron determinant, so every
+    EA-EOM-CCSD eigenvalue equals E(N+1) - E(N) from full CI.  Scaling any
+    one block of Hbar by 1 % moves the eigenvalues by 2e-5 to 2e-2."""
+    from oracle import eom, uccsd
+    asym, eps, h = _generic_spin_orbital_system(no, no + 2, 5)
+    ints = uccsd.classes(asym, no)
+    r = uccsd.solve(ints, eps[:no], eps[no:], max_residua=6, max_iterations=300,
+                    amplitudes_convergence=1e-12, energy_convergence=1e-14)
+    assert r["converged"]
+    E0 = _fci_energies(h, asym, no)[0]
+    e_ref = np.trace(h[:no, :no]) + 0.5 * np.einsum("ijij->", asym[:no, :no, :no, :no])
+    assert abs(e_ref + r["energy"] - E0) <= 1e-11  # CCSD exact
+    ev = eom.ea_eigenvalues(ints, eps[:no], eps[no:], r["T1"], r["T2"])
+    exact = np.sort(_fci_energies(h, asym, no + 1) - E0)
+    assert len(ev) == len(exact) == no + 2
+    assert np.abs(ev - exact).max() <= 1e-11
+
+
+def test_ea_eom_closed_shell_spin_degeneracy():
+    """Closed shell in spin orbitals: the EA roots come in spin doublets and
+    the lowest lies near the lowest virtual orbital energy for a weak
+    interaction (Koopmans)."""
+    from oracle import eom, uccsd
+    g, ei, ea = philox.system(61, 2, 3, 12)
+    g = 0.3 * g
+    ints = uccsd.spin_orbital_integrals(g, 2, 3)
+    epi, epa = np.repeat(ei, 2), np.repeat(ea, 2)
+    r = uccsd.solve(ints, epi, epa, max_residua=6, max_iterations=200,
+                    amplitudes_convergence=1e-12, energy_convergence=1e-14)
+    ev = eom.ea_eigenvalues(ints, epi, epa, r["T1"], r["T2"])
+    assert abs(ev[0] - ev[1]) <= 1e-9 and abs(ev[2] - ev[3]) <= 1e-9
+    assert abs(ev[0] - ea.min()) <= 0.1 * abs(ea.min())
