@@ -234,6 +234,16 @@ int sisi_engine_set_formulation(sisi_engine *e, int formulation);
 int sisi_eom_ip_sigma(sisi_engine *e, const double *T1, const double *T2, const double *r1,
                       const double *r2, double *s1, double *s2);
 
+/* EA-EOM-CCSD right-apply, the product UCcsdEAEquationOfMotionDavidson
+ * (UCcsdEAEquationOfMotionDavidson.cxx:43-351) repeats in its Davidson
+ * iterations: SimilarityTransformedHamiltonian::right_apply_CCSD_EA
+ * (SimilarityTransformedHamiltonian.cxx:289-294), FP64, on an engine in
+ * SISI_FORMULATION_UCCSD with PPHH, PPPP, HHHP, PPPH and PHPH uploaded.
+ * T1, T2 as for sisi_eom_ip_sigma; r1[Nv], r2[No,Nv,Nv] (antisymmetric in the
+ * two particles), first index fastest; s1, s2 alike. */
+int sisi_eom_ea_sigma(sisi_engine *e, const double *T1, const double *T2, const double *r1,
+                      const double *r2, double *s1, double *s2);
+
 /* Arguments of DrccdEnergyFromCoulombIntegrals (cxx:41-65): unrestricted
  * (spins = 1 instead of 2, also in the energy of
  * ClusterSinglesDoublesAlgorithm.cxx:133,163-170), linearized (no quadratic

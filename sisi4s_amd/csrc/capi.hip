@@ -267,6 +267,25 @@ int sisi_eom_ip_sigma(sisi_engine *h, const double *T1, const double *T2, const 
   });
 }
 
+int sisi_eom_ea_sigma(sisi_engine *h, const double *T1, const double *T2, const double *r1,
+                      const double *r2, double *s1, double *s2) {
+  return guarded([&] {
+    ENGINE_REQUIRED(h);
+    Engine &e = h->e;
+    SISI_REQUIRE(T1 && T2 && r1 && r2 && s1 && s2, SISI_ERR_INVALID, "null argument");
+    SISI_REQUIRE(!e.cplx, SISI_ERR_INVALID, "the EA right-apply is built for real amplitudes");
+    const size_t o = size_t(e.No), v = size_t(e.Nv);
+    DBuf t1, t2, q1, q2, w1(&e.pool, v), w2(&e.pool, o * v * v);
+    e.upload(t1, T1, v * o);
+    e.upload(t2, T2, v * v * o * o);
+    e.upload(q1, r1, v);
+    e.upload(q2, r2, o * v * v);
+    eom_ea_sigma(e, t1.get(), t2.get(), q1.get(), q2.get(), w1.get(), w2.get());
+    e.download(s1, w1, v);
+    e.download(s2, w2, o * v * v);
+  });
+}
+
 int sisi_engine_set_drccd(sisi_engine *h, int unrestricted, int linearized,
                           int adjacent_pairs_exchange) {
   return guarded([&] {

@@ -67,12 +67,18 @@ bool integrals_ready(const Engine &e);
 bool uccsd_ready(const Engine &e);
 // the classes of the IP right-apply: those of uccsd_ready but PPPP
 bool ip_eom_ready(const Engine &e);
+// the classes of the EA right-apply: those of uccsd_ready but HHHH
+bool ea_eom_ready(const Engine &e);
 void uccsd_residual(Engine &e, int iteration, bool initial_doubles, const double *T1,
                     const double *T2, double *R1, double *R2);
 double uccsd_energy(Engine &e, const double *T1, const double *T2);
 // IP-EOM-CCSD right-apply (SimilarityTransformedHamiltonian.cxx:64-68): s =
 // Hbar r on the 1h + 2h1p space, r1[i], r2[i,j,a], spin-orbital engine
 void eom_ip_sigma(Engine &e, const double *T1, const double *T2, const double *r1,
+                  const double *r2, double *s1, double *s2);
+// EA-EOM-CCSD right-apply (SimilarityTransformedHamiltonian.cxx:289-294): s =
+// Hbar r on the 1p + 2p1h space, r1[a], r2[j,a,b], spin-orbital engine
+void eom_ea_sigma(Engine &e, const double *T1, const double *T2, const double *r1,
                   const double *r2, double *s1, double *s2);
 void build_vabcd(Engine &e);
 void ccsd_residual_full(Engine &e, int iteration, bool initial_doubles, const double *T1,

@@ -135,4 +135,117 @@ void eom_ip_sigma(Engine &e, const double *T1c, const double *T2c, const double 
   e.thaw();
 }
 
+// EA-EOM-CCSD right-apply: SimilarityTransformedHamiltonian::right_apply_CCSD_EA
+// (SimilarityTransformedHamiltonian.cxx:289-294), repeated by
+// UCcsdEAEquationOfMotionDavidson (UCcsdEAEquationOfMotionDavidson.cxx:43-351).
+// R = (r1[a], r2[j,a,b]) with r2 antisymmetric in a, b.  The particle-hole
+// counterpart of the IP blocks (Stanton and Gauss, J. Chem. Phys. 103, 1064
+// (1995)); needs PPPP for W_abef (Nv^4 doubles, as the reference's stored
+// Vabcd), HHHH is not read.
+void eom_ea_sigma(Engine &e, const double *T1c, const double *T2c, const double *r1c,
+                  const double *r2c, double *s1p, double *s2p) {
+  SISI_REQUIRE(e.formulation == SISI_FORMULATION_UCCSD && ea_eom_ready(e), SISI_ERR_STATE,
+               "the EA right-apply needs the antisymmetrized PPHH, PPPP, HHHP, PPPH and PHPH "
+               "integrals (SISI_FORMULATION_UCCSD)");
+  SISI_REQUIRE(e.eps_i.get() && e.eps_a.get(), SISI_ERR_STATE, "eigenenergies not set");
+  const int64_t o = e.No, v = e.Nv, n2 = v * v * o * o;
+  const int p = SISI_FP64;
+  double *T1p = const_cast<double *>(T1c), *T2p = const_cast<double *>(T2c);
+  View T1 = dense(T1p, {v, o}), T2 = dense(T2p, {v, v, o, o});
+  View r1 = dense(const_cast<double *>(r1c), {v}), r2 = dense(const_cast<double *>(r2c), {o, v, v});
+  View s1 = dense(s1p, {v}), s2 = dense(s2p, {o, v, v});
+  View V = dense(e.Vabij.get(), {v, v, o, o}), W4v = dense(e.Vabcd.get(), {v, v, v, v}),
+       Q = dense(e.Vijka.get(), {o, o, o, v}), P3 = dense(e.Vabci.get(), {v, v, v, o}),
+       X = dense(e.Vaibj.get(), {v, o, v, o});
+  e.thaw();
+
+  // tau~ = T2 + (T_ai T_bj - T_bi T_aj) / 2, tau = T2 + T_ai T_bj - T_bi T_aj
+  DBuf O1b(&e.pool, n2), taub(&e.pool, n2), tautb(&e.pool, n2);
+  View O1 = dense(O1b.get(), {v, v, o, o}), tau = dense(taub.get(), {v, v, o, o}),
+       taut = dense(tautb.get(), {v, v, o, o});
+  zero(e, O1b.get(), n2);
+  add_t1t1(e, O1b.get(), O1b.get(), T1p, 1.0, int(v), int(o));
+  copy(e, tau, T2);
+  sum(e, 1.0, O1, "abij", 1.0, tau, "abij");
+  sum(e, -1.0, O1, "baij", 1.0, tau, "abij");
+  copy(e, taut, T2);
+  sum(e, 0.5, O1, "abij", 1.0, taut, "abij");
+  sum(e, -0.5, O1, "baij", 1.0, taut, "abij");
+
+  // one-body blocks, as for IP
+  DBuf Fmeb(&e.pool, o * v), Foob(&e.pool, o * o), Fvvb(&e.pool, v * v);
+  View Fme = dense(Fmeb.get(), {o, v}), Foo = dense(Foob.get(), {o, o}), Fvv = dense(Fvvb.get(), {v, v});
+  ctr(e, 1.0, "fn,efmn->me", T1, V, 0.0, Fme, p);
+  ctr(e, 1.0, "en,mnie->mi", T1, Q, 0.0, Foo, p);
+  ctr(e, 0.5, "efin,efmn->mi", taut, V, 1.0, Foo, p);
+  ctr(e, 0.5, "ei,me->mi", T1, Fme, 1.0, Foo, p);
+  add_diagonal(e, Foob.get(), o, e.eps_i.get());
+  ctr(e, -1.0, "fm,feam->ae", T1, P3, 0.0, Fvv, p);
+  ctr(e, -0.5, "afmn,efmn->ae", taut, V, 1.0, Fvv, p);
+  ctr(e, -0.5, "am,me->ae", T1, Fme, 1.0, Fvv, p);
+  add_diagonal(e, Fvvb.get(), v, e.eps_a.get());
+
+  // W_mbej, as for IP
+  DBuf Wvob(&e.pool, o * v * v * o), Sb(&e.pool, n2);
+  View Wovvo = dense(Wvob.get(), {o, v, v, o}), S = dense(Sb.get(), {v, v, o, o});
+  sum(e, -1.0, X, "bmej", 0.0, Wovvo, "mbej");
+  ctr(e, -1.0, "fj,efbm->mbej", T1, P3, 1.0, Wovvo, p);
+  ctr(e, 1.0, "bn,mnje->mbej", T1, Q, 1.0, Wovvo, p);
+  add_t1t1(e, Sb.get(), T2p, T1p, 1.0, int(v), int(o));
+  ctr(e, -1.0, "fbjn,efmn->mbej", S, V, 1.0, Wovvo, p);
+
+  // W_abef = <ab||ef> - P(ab) t_mb <am||ef> + tau_mnab <mn||ef> / 2, <mb||ef> = -V_efbm
+  DBuf Wvvvvb(&e.pool, v * v * v * v);
+  View Wvvvv = dense(Wvvvvb.get(), {v, v, v, v});
+  copy(e, Wvvvv, W4v);
+  ctr(e, 1.0, "am,efbm->abef", T1, P3, 1.0, Wvvvv, p);
+  ctr(e, -1.0, "bm,efam->abef", T1, P3, 1.0, Wvvvv, p);
+  ctr(e, 0.5, "abmn,efmn->abef", tau, V, 1.0, Wvvvv, p);
+
+  // W_amef = <am||ef> - t_na <nm||ef>
+  DBuf Wvovvb(&e.pool, v * o * v * v);
+  View Wvovv = dense(Wvovvb.get(), {v, o, v, v});
+  sum(e, 1.0, P3, "efam", 0.0, Wvovv, "amef");
+  ctr(e, -1.0, "an,efnm->amef", T1, V, 1.0, Wvovv, p);
+
+  // W_abei = <ab||ei> - F_me t_miab + t_if W_abef + <mn||ei> tau_mnab / 2
+  //          - P(ab) <mb||ef> t_miaf - P(ab) t_ma (<mb||ei> - t_nibf <mn||ef>)
+  DBuf Wvvvob(&e.pool, v * v * v * o), tmpb(&e.pool, v * v * v * o), Yb(&e.pool, o * v * v * o);
+  View Wvvvo = dense(Wvvvob.get(), {v, v, v, o}), tmp = dense(tmpb.get(), {v, v, v, o}),
+       Y = dense(Yb.get(), {o, v, v, o});
+  copy(e, Wvvvo, P3);
+  ctr(e, -1.0, "me,abmi->abei", Fme, T2, 1.0, Wvvvo, p);
+  ctr(e, 1.0, "fi,abef->abei", T1, Wvvvv, 1.0, Wvvvo, p);
+  ctr(e, -0.5, "mnie,abmn->abei", Q, tau, 1.0, Wvvvo, p);
+  ctr(e, -1.0, "efbm,afmi->abei", P3, T2, 0.0, tmp, p);  // <mb||ef> t_miaf
+  sum(e, -1.0, tmp, "abei", 1.0, Wvvvo, "abei");
+  sum(e, 1.0, tmp, "baei", 1.0, Wvvvo, "abei");
+  sum(e, -1.0, X, "bmei", 0.0, Y, "mbei");                 // <mb||ei> = -V_bmei
+  ctr(e, -1.0, "bfni,efmn->mbei", T2, V, 1.0, Y, p);       // - t_nibf <mn||ef>
+  ctr(e, 1.0, "am,mbei->abei", T1, Y, 0.0, tmp, p);
+  sum(e, -1.0, tmp, "abei", 1.0, Wvvvo, "abei");
+  sum(e, 1.0, tmp, "baei", 1.0, Wvvvo, "abei");
+
+  // sigma_1 = F_ac r_c + F_ld r_lad + W_alcd r_lcd / 2
+  ctr(e, 1.0, "ac,c->a", Fvv, r1, 0.0, s1, p);
+  ctr(e, 1.0, "ld,lad->a", Fme, r2, 1.0, s1, p);
+  ctr(e, 0.5, "alcd,lcd->a", Wvovv, r2, 1.0, s1, p);
+  // sigma_2 = W_abcj r_c + P(ab) F_ac r_jcb - F_lj r_lab + P(ab) W_lbdj r_lad
+  //           + W_abcd r_jcd / 2 - (<kl||cd> r_lcd / 2) t_kjab
+  DBuf t3b(&e.pool, o * v * v), gb(&e.pool, o);
+  View t3 = dense(t3b.get(), {o, v, v}), gk = dense(gb.get(), {o});
+  ctr(e, 1.0, "abcj,c->jab", Wvvvo, r1, 0.0, s2, p);
+  ctr(e, 1.0, "ac,jcb->jab", Fvv, r2, 0.0, t3, p);
+  sum(e, 1.0, t3, "jab", 1.0, s2, "jab");
+  sum(e, -1.0, t3, "jba", 1.0, s2, "jab");
+  ctr(e, -1.0, "lj,lab->jab", Foo, r2, 1.0, s2, p);
+  ctr(e, 1.0, "lbdj,lad->jab", Wovvo, r2, 0.0, t3, p);
+  sum(e, 1.0, t3, "jab", 1.0, s2, "jab");
+  sum(e, -1.0, t3, "jba", 1.0, s2, "jab");
+  ctr(e, 0.5, "abcd,jcd->jab", Wvvvv, r2, 1.0, s2, p);
+  ctr(e, 0.5, "cdkl,lcd->k", V, r2, 0.0, gk, p);
+  ctr(e, -1.0, "k,abkj->jab", gk, T2, 1.0, s2, p);
+  e.thaw();
+}
+
 }  // namespace sisi

@@ -41,6 +41,11 @@ bool ip_eom_ready(const Engine &e) {
   return !e.cplx && (e.given_integrals & kIp) == kIp;
 }
 
+bool ea_eom_ready(const Engine &e) {
+  constexpr unsigned kEa = kUccsdClasses & ~(1u << SISI_HHHH);
+  return !e.cplx && (e.given_integrals & kEa) == kEa;
+}
+
 void uccsd_residual(Engine &e, int it, bool initial_doubles, const double *T1c, const double *T2c,
                     double *R1p, double *R2p) {
   SISI_REQUIRE(!e.cplx, SISI_ERR_INVALID, "the spin-orbital CCSD residual is built for real amplitudes");

@@ -97,6 +97,8 @@ SIGNATURES = [
     ("sisi_engine_set_drccd", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     ("sisi_eom_ip_sigma", ctypes.c_int, [ctypes.c_void_p, _c_double_p, _c_double_p, _c_double_p,
                                          _c_double_p, _c_double_p, _c_double_p]),
+    ("sisi_eom_ea_sigma", ctypes.c_int, [ctypes.c_void_p, _c_double_p, _c_double_p, _c_double_p,
+                                         _c_double_p, _c_double_p, _c_double_p]),
     ("sisi_engine_set_coulomb_factors", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _c_double_p, _c_double_p, _c_double_p]),
     ("sisi_ccsd_residual", ctypes.c_int,
@@ -405,6 +407,15 @@ class Engine:
         _check(self._lib.sisi_eom_ip_sigma(self._h, _ptr(_buf(T1)), _ptr(_buf(T2)), _ptr(_buf(r1)),
                                            _ptr(_buf(r2)), _ptr(b1), _ptr(b2)))
         return _unbuf(b1, (o,)), _unbuf(b2, (o, o, v))
+
+    def eom_ea_sigma(self, T1, T2, r1, r2):
+        """EA-EOM-CCSD right-apply (SimilarityTransformedHamiltonian.cxx:289-294)
+        on a spin-orbital engine with PPPP uploaded: returns (s1[a], s2[j,a,b])."""
+        o, v = self.No, self.Nv
+        b1, b2 = np.empty(v), np.empty(o * v * v)
+        _check(self._lib.sisi_eom_ea_sigma(self._h, _ptr(_buf(T1)), _ptr(_buf(T2)), _ptr(_buf(r1)),
+                                           _ptr(_buf(r2)), _ptr(b1), _ptr(b2)))
+        return _unbuf(b1, (v,)), _unbuf(b2, (o, v, v))
 
     def set_residual_variant(self, ppl=True, distinguishable=False):
         """ppl / distinguishable of the next residual calls
