@@ -516,22 +516,26 @@ class CcsdPerturbativeTriplesComplex : public Algorithm {
   }
 };
 
-// UCcsdIPEquationOfMotionDavidson (UCcsdIPEquationOfMotionDavidson.cxx:25-206):
-// the lowest IP-EOM-CCSD roots of spin-orbital CCSD amplitudes by the host
-// Davidson driver (EigenSystemDavidsonMono), each right-apply on the device
-// (sisi_eom_ip_sigma).  Inputs as the reference's: the eigenenergies, the
-// SinglesAmplitudes [Nv,No] and DoublesAmplitudes [Nv,Nv,No,No], and the
-// antisymmetrized HHPP, HHHH, HHHP, HPPP and HPHP classes (the reference's
-// PHPP, HPHH, HPPH and PHHP are these with indices reordered and are not
-// read).  The vector space is r1[i] and r2[i,j,a] for i < j; the
-// preconditioner diagonal -f_ii and -f_ii - f_jj + f_aa is
-// IPCcsdPreconditioner's (CcsdPreconditioner.cxx:546-560).  Emits the
-// eigenvalues and writes IPEomEnergies.dat ("k re im" per root) as the
-// reference does (cxx:199-205).
-class UCcsdIPEquationOfMotionDavidson : public Algorithm {
+// UCcsdIPEquationOfMotionDavidson (UCcsdIPEquationOfMotionDavidson.cxx:25-206)
+// and UCcsdEAEquationOfMotionDavidson (UCcsdEAEquationOfMotionDavidson.cxx:
+// 25-351): the lowest IP- / EA-EOM-CCSD roots of spin-orbital CCSD amplitudes
+// by the host Davidson driver (EigenSystemDavidsonMono), each right-apply on
+// the device (sisi_eom_ip_sigma / sisi_eom_ea_sigma).  Inputs as the
+// reference's: the eigenenergies, SinglesAmplitudes [Nv,No],
+// DoublesAmplitudes [Nv,Nv,No,No] and antisymmetrized classes; IP reads
+// HHPP, HHHH, HHHP, HPPP and HPHP, EA reads HHPP, PPPP, HHHP, PPPH and PHPH
+// (the other classes the reference reads are these with indices reordered).
+// Vector spaces: IP r1[i] and r2[i,j,a] for i < j, EA r1[a] and r2[j,a,b]
+// for a < b; preconditioner diagonals -f_ii, -f_ii - f_jj + f_aa (IP) and
+// f_aa, -f_jj + f_aa + f_bb (EA) as IPCcsdPreconditioner's and
+// EACcsdPreconditioner's (CcsdPreconditioner.cxx:546-560, 374-397).  Emits the
+// eigenvalues and writes IPEomEnergies.dat / EAEomEnergies.dat ("k re im" per
+// root) as the reference does (IP cxx:199-205, EA cxx:343-350).
+class EomCcsdDavidson : public Algorithm {
  public:
-  using Algorithm::Algorithm;
-  std::string getName() override { return "UCcsdIPEquationOfMotionDavidson"; }
+  EomCcsdDavidson(std::map<std::string, Argument> in, std::map<std::string, Argument> out, DataMap &d,
+                  bool ea)
+      : Algorithm(std::move(in), std::move(out), d), ea_(ea) {}
   void run() override {
     if (isArgumentGiven("HPFockMatrix") || isArgumentGiven("HHFockMatrix") ||
         isArgumentGiven("PPFockMatrix"))
@@ -565,78 +569,54 @@ class UCcsdIPEquationOfMotionDavidson : public Algorithm {
     sisi_engine *e = owned.e;
     check(sisi_engine_set_eigenenergies(e, ei.values.data(), ea.values.data()),
           "sisi_engine_set_eigenenergies");
-    // reference class -> engine class, real antisymmetrized integrals:
-    // V_abij = V_ijab, V_abci = -V_icab, V_aibj = V_iajb
-    auto in = [&](const char *name, std::vector<int64_t> lens) -> const Tensor & {
-      const auto &t = getTensorArgument(name);
-      if (t.complex || t.lens != lens) throw StepError(std::string(name) + " has the wrong shape");
-      return t;
-    };
-    const Tensor &hhpp = in("HHPPCoulombIntegrals", {o, o, v, v});
-    const Tensor &hhhh = in("HHHHCoulombIntegrals", {o, o, o, o});
-    const Tensor &hhhp = in("HHHPCoulombIntegrals", {o, o, o, v});
-    const Tensor &hppp = in("HPPPCoulombIntegrals", {o, v, v, v});
-    const Tensor &hphp = in("HPHPCoulombIntegrals", {o, v, o, v});
-    std::vector<double> buf(static_cast<size_t>(std::max(v * v * v * o, v * v * o * o)));
-    for (int64_t j = 0; j < o; ++j)
-      for (int64_t i = 0; i < o; ++i)
-        for (int64_t b = 0; b < v; ++b)
-          for (int64_t a = 0; a < v; ++a)
-            buf[size_t(a + v * (b + v * (i + o * j)))] = hhpp.values[size_t(i + o * (j + o * (a + v * b)))];
-    check(sisi_engine_set_integrals(e, SISI_PPHH, buf.data()), "sisi_engine_set_integrals");
-    check(sisi_engine_set_integrals(e, SISI_HHHH, hhhh.values.data()), "sisi_engine_set_integrals");
-    check(sisi_engine_set_integrals(e, SISI_HHHP, hhhp.values.data()), "sisi_engine_set_integrals");
-    for (int64_t i = 0; i < o; ++i)
-      for (int64_t c = 0; c < v; ++c)
-        for (int64_t b = 0; b < v; ++b)
-          for (int64_t a = 0; a < v; ++a)
-            buf[size_t(a + v * (b + v * (c + v * i)))] = -hppp.values[size_t(i + o * (c + v * (a + v * b)))];
-    check(sisi_engine_set_integrals(e, SISI_PPPH, buf.data()), "sisi_engine_set_integrals");
-    for (int64_t j = 0; j < o; ++j)
-      for (int64_t b = 0; b < v; ++b)
-        for (int64_t i = 0; i < o; ++i)
-          for (int64_t a = 0; a < v; ++a)
-            buf[size_t(a + v * (i + o * (b + v * j)))] = hphp.values[size_t(i + o * (a + v * (j + o * b)))];
-    check(sisi_engine_set_integrals(e, SISI_PHPH, buf.data()), "sisi_engine_set_integrals");
+    upload_classes(e, o, v);
     check(sisi_engine_set_formulation(e, SISI_FORMULATION_UCCSD), "sisi_engine_set_formulation");
 
-    // packed vector: r1[i], then r2[i,j,a] for the pairs i < j (a fastest)
+    // packed vector: r1, then r2 over (hole pairs i < j) x a for IP, or
+    // j x (particle pairs a < b) for EA
     std::vector<std::pair<int64_t, int64_t>> pairs;
-    for (int64_t j = 0; j < o; ++j)
-      for (int64_t i = 0; i < j; ++i) pairs.emplace_back(i, j);
-    const size_t n = size_t(o) + pairs.size() * size_t(v);
+    const int64_t np = ea_ ? v : o;
+    for (int64_t y = 0; y < np; ++y)
+      for (int64_t x = 0; x < y; ++x) pairs.emplace_back(x, y);
+    const int64_t n1 = ea_ ? v : o, nk = ea_ ? o : v;  // r1 length, unpaired index range
+    const size_t n = size_t(n1) + pairs.size() * size_t(nk);
     if (opt.nroots < 1 || size_t(opt.nroots) > n) throw StepError("eigenstates must lie in [1, dimension]");
+    // offset of r2 element (pair p, unpaired index k) in the packed vector, and
+    // of the two antisymmetric entries in the dense r2 (first index fastest)
+    auto packed = [&](size_t p, int64_t k) { return size_t(n1) + p * size_t(nk) + size_t(k); };
+    auto dense_at = [&](int64_t x, int64_t y, int64_t k) {
+      return ea_ ? size_t(k + o * (x + v * y)) : size_t(x + o * (y + o * k));  // r2[j,a,b] / r2[i,j,a]
+    };
     std::vector<double> diag(n);
-    for (int64_t i = 0; i < o; ++i) diag[size_t(i)] = -ei.values[size_t(i)];
+    for (int64_t i = 0; i < n1; ++i) diag[size_t(i)] = ea_ ? ea.values[size_t(i)] : -ei.values[size_t(i)];
     for (size_t p = 0; p < pairs.size(); ++p)
-      for (int64_t a = 0; a < v; ++a)
-        diag[size_t(o) + p * size_t(v) + size_t(a)] =
-            -ei.values[size_t(pairs[p].first)] - ei.values[size_t(pairs[p].second)] + ea.values[size_t(a)];
-    const size_t no = static_cast<size_t>(o), nr2 = static_cast<size_t>(o * o * v);
-    std::vector<double> r1(no), r2(nr2), s1(no), s2(nr2);
-    auto apply = [&](const std::vector<double> &x) {
+      for (int64_t k = 0; k < nk; ++k) {
+        const size_t x = size_t(pairs[p].first), y = size_t(pairs[p].second);
+        diag[packed(p, k)] = ea_ ? -ei.values[size_t(k)] + ea.values[x] + ea.values[y]
+                                 : -ei.values[x] - ei.values[y] + ea.values[size_t(k)];
+      }
+    const size_t nr1 = static_cast<size_t>(n1), nr2 = static_cast<size_t>(o * v * (ea_ ? v : o));
+    std::vector<double> r1(nr1), r2(nr2), s1(nr1), s2(nr2);
+    auto apply = [&](const std::vector<double> &xv) {
       std::fill(r2.begin(), r2.end(), 0.0);
-      for (int64_t i = 0; i < o; ++i) r1[size_t(i)] = x[size_t(i)];
+      for (size_t i = 0; i < nr1; ++i) r1[i] = xv[i];
       for (size_t p = 0; p < pairs.size(); ++p)
-        for (int64_t a = 0; a < v; ++a) {
-          const double val = x[size_t(o) + p * size_t(v) + size_t(a)];
-          const int64_t i = pairs[p].first, j = pairs[p].second;
-          r2[size_t(i + o * (j + o * a))] = val;
-          r2[size_t(j + o * (i + o * a))] = -val;
+        for (int64_t k = 0; k < nk; ++k) {
+          const double val = xv[packed(p, k)];
+          r2[dense_at(pairs[p].first, pairs[p].second, k)] = val;
+          r2[dense_at(pairs[p].second, pairs[p].first, k)] = -val;
         }
-      check(sisi_eom_ip_sigma(e, T1.values.data(), T2.values.data(), r1.data(), r2.data(), s1.data(),
-                              s2.data()),
-            "sisi_eom_ip_sigma");
+      check((ea_ ? sisi_eom_ea_sigma : sisi_eom_ip_sigma)(e, T1.values.data(), T2.values.data(), r1.data(),
+                                                         r2.data(), s1.data(), s2.data()),
+            ea_ ? "sisi_eom_ea_sigma" : "sisi_eom_ip_sigma");
       std::vector<double> y(n);
-      for (int64_t i = 0; i < o; ++i) y[size_t(i)] = s1[size_t(i)];
+      for (size_t i = 0; i < nr1; ++i) y[i] = s1[i];
       for (size_t p = 0; p < pairs.size(); ++p)
-        for (int64_t a = 0; a < v; ++a)
-          y[size_t(o) + p * size_t(v) + size_t(a)] =
-              s2[size_t(pairs[p].first + o * (pairs[p].second + o * a))];
+        for (int64_t k = 0; k < nk; ++k) y[packed(p, k)] = s2[dense_at(pairs[p].first, pairs[p].second, k)];
       return y;
     };
     const DavidsonResult res = davidson(apply, diag, lowest_diagonal_basis(diag, opt.nroots), opt);
-    std::ofstream dat("IPEomEnergies.dat");
+    std::ofstream dat(ea_ ? "EAEomEnergies.dat" : "IPEomEnergies.dat");
     for (size_t k = 0; k < res.values.size(); ++k) {
       dat << (k + 1) << " " << num(res.values[k].real()) << " " << num(res.values[k].imag()) << "\n";
       emitted.scalars.push_back({"eigenvalue" + std::to_string(k + 1), num(res.values[k].real())});
@@ -644,6 +624,73 @@ class UCcsdIPEquationOfMotionDavidson : public Algorithm {
     emitted.scalars.push_back({"davidsonIterations", std::to_string(res.iterations)});
     emitted.scalars.push_back({"converged", res.converged ? "true" : "false"});
   }
+
+ private:
+  bool ea_;
+
+  const Tensor &real_class(const char *name, std::vector<int64_t> lens) {
+    const auto &t = getTensorArgument(name);
+    if (t.complex || t.lens != lens) throw StepError(std::string(name) + " has the wrong shape");
+    return t;
+  }
+
+  // reference class -> engine class, real antisymmetrized integrals:
+  // V_abij = V_ijab, V_abci = -V_icab, V_aibj = V_iajb
+  void upload_classes(sisi_engine *e, int64_t o, int64_t v) {
+    const Tensor &hhpp = real_class("HHPPCoulombIntegrals", {o, o, v, v});
+    const Tensor &hhhp = real_class("HHHPCoulombIntegrals", {o, o, o, v});
+    std::vector<double> buf(static_cast<size_t>(v * v * o * o));
+    for (int64_t j = 0; j < o; ++j)
+      for (int64_t i = 0; i < o; ++i)
+        for (int64_t b = 0; b < v; ++b)
+          for (int64_t a = 0; a < v; ++a)
+            buf[size_t(a + v * (b + v * (i + o * j)))] = hhpp.values[size_t(i + o * (j + o * (a + v * b)))];
+    check(sisi_engine_set_integrals(e, SISI_PPHH, buf.data()), "sisi_engine_set_integrals");
+    check(sisi_engine_set_integrals(e, SISI_HHHP, hhhp.values.data()), "sisi_engine_set_integrals");
+    if (ea_) {
+      check(sisi_engine_set_integrals(e, SISI_PPPP, real_class("PPPPCoulombIntegrals", {v, v, v, v}).values.data()),
+            "sisi_engine_set_integrals");
+      check(sisi_engine_set_integrals(e, SISI_PPPH, real_class("PPPHCoulombIntegrals", {v, v, v, o}).values.data()),
+            "sisi_engine_set_integrals");
+      check(sisi_engine_set_integrals(e, SISI_PHPH, real_class("PHPHCoulombIntegrals", {v, o, v, o}).values.data()),
+            "sisi_engine_set_integrals");
+      return;
+    }
+    check(sisi_engine_set_integrals(e, SISI_HHHH, real_class("HHHHCoulombIntegrals", {o, o, o, o}).values.data()),
+          "sisi_engine_set_integrals");
+    const Tensor &hppp = real_class("HPPPCoulombIntegrals", {o, v, v, v});
+    const Tensor &hphp = real_class("HPHPCoulombIntegrals", {o, v, o, v});
+    buf.assign(static_cast<size_t>(v * v * v * o), 0.0);
+    for (int64_t i = 0; i < o; ++i)
+      for (int64_t c = 0; c < v; ++c)
+        for (int64_t b = 0; b < v; ++b)
+          for (int64_t a = 0; a < v; ++a)
+            buf[size_t(a + v * (b + v * (c + v * i)))] = -hppp.values[size_t(i + o * (c + v * (a + v * b)))];
+    check(sisi_engine_set_integrals(e, SISI_PPPH, buf.data()), "sisi_engine_set_integrals");
+    buf.assign(static_cast<size_t>(v * o * v * o), 0.0);
+    for (int64_t j = 0; j < o; ++j)
+      for (int64_t b = 0; b < v; ++b)
+        for (int64_t i = 0; i < o; ++i)
+          for (int64_t a = 0; a < v; ++a)
+            buf[size_t(a + v * (i + o * (b + v * j)))] = hphp.values[size_t(i + o * (a + v * (j + o * b)))];
+    check(sisi_engine_set_integrals(e, SISI_PHPH, buf.data()), "sisi_engine_set_integrals");
+  }
+};
+
+class UCcsdIPEquationOfMotionDavidson : public EomCcsdDavidson {
+ public:
+  UCcsdIPEquationOfMotionDavidson(std::map<std::string, Argument> in, std::map<std::string, Argument> out,
+                                  DataMap &d)
+      : EomCcsdDavidson(std::move(in), std::move(out), d, false) {}
+  std::string getName() override { return "UCcsdIPEquationOfMotionDavidson"; }
+};
+
+class UCcsdEAEquationOfMotionDavidson : public EomCcsdDavidson {
+ public:
+  UCcsdEAEquationOfMotionDavidson(std::map<std::string, Argument> in, std::map<std::string, Argument> out,
+                                  DataMap &d)
+      : EomCcsdDavidson(std::move(in), std::move(out), d, true) {}
+  std::string getName() override { return "UCcsdEAEquationOfMotionDavidson"; }
 };
 
 static Registrar<GenerateSyntheticSystem> r0("GenerateSyntheticSystem");
@@ -656,5 +703,6 @@ static Registrar<CcsdPerturbativeTriples> r3("CcsdPerturbativeTriples");
 static Registrar<DrccdEnergyFromCoulombIntegrals> r7("DrccdEnergyFromCoulombIntegrals");
 static Registrar<UccsdAmplitudesFromCoulombIntegrals> r8("UccsdAmplitudesFromCoulombIntegrals");
 static Registrar<UCcsdIPEquationOfMotionDavidson> r9("UCcsdIPEquationOfMotionDavidson");
+static Registrar<UCcsdEAEquationOfMotionDavidson> r10("UCcsdEAEquationOfMotionDavidson");
 
 }  // namespace sisi_host

@@ -523,20 +523,22 @@ def _ip_eom_system(no, n, seed):
     return asym, eps, ints, r["T1"], r["T2"]
 
 
-def _ip_eom_steps(tmp_path, asym, eps, no, T1, T2, args):
+def _ip_eom_steps(tmp_path, asym, eps, no, T1, T2, args, ea=False):
     h, p = slice(0, no), slice(no, asym.shape[0])
-    tensors = dict(I=eps[:no], A=eps[no:], T1=T1, T2=T2, HHPP=asym[h, h, p, p],
-                   HHHH=asym[h, h, h, h], HHHP=asym[h, h, h, p], HPPP=asym[h, p, p, p],
-                   HPHP=asym[h, p, h, p])
+    tensors = dict(I=eps[:no], A=eps[no:], T1=T1, T2=T2, HHPP=asym[h, h, p, p], HHHP=asym[h, h, h, p])
+    if ea:
+        tensors.update(PPPP=asym[p, p, p, p], PPPH=asym[p, p, p, h], PHPH=asym[p, h, p, h])
+    else:
+        tensors.update(HHHH=asym[h, h, h, h], HPPP=asym[h, p, p, p], HPHP=asym[h, p, h, p])
+    classes = [c for c in tensors if len(c) == 4]
     steps = []
     for name, x in tensors.items():
         write_cc4s_tensor(tmp_path / name, np.ascontiguousarray(x))
         steps.append("- name: Read\n  in: {fileName: \"%s\"}\n  out: {destination: $%s}\n"
                      % (tmp_path / (name + ".yaml"), name))
-    steps.append("- name: UCcsdIPEquationOfMotionDavidson\n  in: {HoleEigenEnergies: $I, "
+    steps.append("- name: UCcsd%sEquationOfMotionDavidson\n  in: {HoleEigenEnergies: $I, "
                  "ParticleEigenEnergies: $A, SinglesAmplitudes: $T1, DoublesAmplitudes: $T2, %s, %s}\n"
-                 % (", ".join("%sCoulombIntegrals: $%s" % (c, c)
-                              for c in ("HHPP", "HHHH", "HHHP", "HPPP", "HPHP")), args))
+                 % ("EA" if ea else "IP", ", ".join("%sCoulombIntegrals: $%s" % (c, c) for c in classes), args))
     p = tmp_path / "in.yaml"
     p.write_text("".join(steps))
     return subprocess.run([RUNNER, "-i", str(p), "-o", str(tmp_path / "out.yaml")],
@@ -561,6 +563,9 @@ def test_ip_eom_step_rejects_what_it_does_not_build(tmp_path):
         "- name: UCcsdIPEquationOfMotionDavidson\n  in: {HoleEigenEnergies: $I, "
         "ParticleEigenEnergies: $A, SinglesAmplitudes: $T1, DoublesAmplitudes: $T1}\n")
     assert r.returncode != 0 and "amplitudes must be real" in r.stderr
+    r, _ = run_steps(tmp_path, "- name: UCcsdEAEquationOfMotionDavidson\n"
+                     "  in: {printEigenvectorsRange: \"1\"}\n")
+    assert r.returncode != 0 and "printEigenvectorsRange is not supported" in r.stderr
 
 
 @pytest.mark.gpu
@@ -592,3 +597,22 @@ def test_ip_eom_davidson_step_matches_full_ci_and_the_oracle(tmp_path):
     assert r.returncode == 0, r.stdout + r.stderr
     got = np.loadtxt(tmp_path / "IPEomEnergies.dat", ndmin=2)
     assert np.abs(got[:, 1] - ref[:3]).max() <= 1e-9
+
+
+@pytest.mark.gpu
+@pytest.mark.skip(reason="the UCcsdEAEquationOfMotionDavidson step has not yet been run on a GPU")
+def test_ea_eom_davidson_step_matches_full_ci(tmp_path):
+    """The EA-EOM-CCSD Davidson step from files: with two virtual spin
+    orbitals its roots equal the oracle's dense EA spectrum, which equals full
+    CI there; with four virtuals the lowest roots match the oracle."""
+    from oracle import eom
+    for no, n, k in ((3, 5, 3), (3, 7, 3)):
+        asym, eps, ints, T1, T2 = _ip_eom_system(no, n, 11)
+        ref = eom.ea_eigenvalues(ints, eps[:no], eps[no:], T1, T2)
+        announce("ea-eom davidson step, %d electrons in %d spin orbitals" % (no, n))
+        r = _ip_eom_steps(tmp_path, asym, eps, no, T1, T2,
+                          "eigenstates: %d, energyConvergence: 0.0, amplitudesConvergence: 1e-20, "
+                          "maxIterations: 60" % k, ea=True)
+        assert r.returncode == 0, r.stdout + r.stderr
+        got = np.loadtxt(tmp_path / "EAEomEnergies.dat", ndmin=2)
+        assert np.abs(got[:, 1] - ref[:k]).max() <= 1e-9
