@@ -85,6 +85,8 @@ def main():
                     help="consumers: occupied spatial orbitals of the spin-orbital CCSD system")
     ap.add_argument("--uccsd-nv", type=int, default=60,
                     help="consumers: virtual spatial orbitals of the spin-orbital CCSD system")
+    ap.add_argument("--eom", type=int, default=0,
+                    help="consumers: also time the IP- and EA-EOM-CCSD right-applies (needs PPPP)")
     ap.add_argument("--mode", choices=["iteration", "converge", "batched", "complex", "consumers",
                                        "small"],
                     default="iteration",
@@ -427,6 +429,25 @@ def consumers(args):
     out["uccsd"] = dict(config="spin orbitals No=%d Nv=%d (closed shell No=%d Nv=%d, NG=%d)"
                         % (O, W, o, v, 4 * v),
                         **timed(u, "uccsd", 2.0 * W ** 4 * O ** 2 + 4.0 * O ** 3 * W ** 3))
+    if args.eom:
+        # EOM-CCSD right-applies on the same engine and amplitudes: wall time
+        # per call through the C ABI (host buffers in and out, Hbar rebuilt
+        # each call)
+        T1, T2 = u.amplitudes()
+        rng = np.random.default_rng(args.seed)
+        r1, r2 = rng.normal(size=O), rng.normal(size=(O, O, W))
+        r2 = r2 - r2.transpose(1, 0, 2)
+        q1, q2 = rng.normal(size=W), rng.normal(size=(O, W, W))
+        q2 = q2 - q2.transpose(0, 2, 1)
+        for label, call, x1, x2 in (("ip", u.eom_ip_sigma, r1, r2), ("ea", u.eom_ea_sigma, q1, q2)):
+            call(T1, T2, x1, x2)
+            times = []
+            for k in range(args.steps):
+                t0 = time.perf_counter()
+                call(T1, T2, x1, x2)
+                times.append(time.perf_counter() - t0)
+                log("%s-eom right-apply %d: %.3fs" % (label, k + 1, times[-1]))
+            out["%s_eom_right_apply_seconds" % label] = float(np.mean(times))
     print(json.dumps(out), flush=True)
 
 
