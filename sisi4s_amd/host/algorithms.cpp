@@ -6,6 +6,7 @@
 
 #include "algorithm.hpp"
 #include "davidson.hpp"
+#include "eom_space.hpp"
 
 namespace sisi_host {
 
@@ -572,48 +573,17 @@ class EomCcsdDavidson : public Algorithm {
     upload_classes(e, o, v);
     check(sisi_engine_set_formulation(e, SISI_FORMULATION_UCCSD), "sisi_engine_set_formulation");
 
-    // packed vector: r1, then r2 over (hole pairs i < j) x a for IP, or
-    // j x (particle pairs a < b) for EA
-    std::vector<std::pair<int64_t, int64_t>> pairs;
-    const int64_t np = ea_ ? v : o;
-    for (int64_t y = 0; y < np; ++y)
-      for (int64_t x = 0; x < y; ++x) pairs.emplace_back(x, y);
-    const int64_t n1 = ea_ ? v : o, nk = ea_ ? o : v;  // r1 length, unpaired index range
-    const size_t n = size_t(n1) + pairs.size() * size_t(nk);
-    if (opt.nroots < 1 || size_t(opt.nroots) > n) throw StepError("eigenstates must lie in [1, dimension]");
-    // offset of r2 element (pair p, unpaired index k) in the packed vector, and
-    // of the two antisymmetric entries in the dense r2 (first index fastest)
-    auto packed = [&](size_t p, int64_t k) { return size_t(n1) + p * size_t(nk) + size_t(k); };
-    auto dense_at = [&](int64_t x, int64_t y, int64_t k) {
-      return ea_ ? size_t(k + o * (x + v * y)) : size_t(x + o * (y + o * k));  // r2[j,a,b] / r2[i,j,a]
-    };
-    std::vector<double> diag(n);
-    for (int64_t i = 0; i < n1; ++i) diag[size_t(i)] = ea_ ? ea.values[size_t(i)] : -ei.values[size_t(i)];
-    for (size_t p = 0; p < pairs.size(); ++p)
-      for (int64_t k = 0; k < nk; ++k) {
-        const size_t x = size_t(pairs[p].first), y = size_t(pairs[p].second);
-        diag[packed(p, k)] = ea_ ? -ei.values[size_t(k)] + ea.values[x] + ea.values[y]
-                                 : -ei.values[x] - ei.values[y] + ea.values[size_t(k)];
-      }
-    const size_t nr1 = static_cast<size_t>(n1), nr2 = static_cast<size_t>(o * v * (ea_ ? v : o));
-    std::vector<double> r1(nr1), r2(nr2), s1(nr1), s2(nr2);
+    const EomSpace space(ea_, o, v);
+    if (opt.nroots < 1 || size_t(opt.nroots) > space.n)
+      throw StepError("eigenstates must lie in [1, dimension]");
+    const std::vector<double> diag = space.diagonal(ei.values.data(), ea.values.data());
+    std::vector<double> r1, r2, s1(space.r1_size()), s2(space.r2_size());
     auto apply = [&](const std::vector<double> &xv) {
-      std::fill(r2.begin(), r2.end(), 0.0);
-      for (size_t i = 0; i < nr1; ++i) r1[i] = xv[i];
-      for (size_t p = 0; p < pairs.size(); ++p)
-        for (int64_t k = 0; k < nk; ++k) {
-          const double val = xv[packed(p, k)];
-          r2[dense_at(pairs[p].first, pairs[p].second, k)] = val;
-          r2[dense_at(pairs[p].second, pairs[p].first, k)] = -val;
-        }
+      space.unpack(xv, r1, r2);
       check((ea_ ? sisi_eom_ea_sigma : sisi_eom_ip_sigma)(e, T1.values.data(), T2.values.data(), r1.data(),
                                                          r2.data(), s1.data(), s2.data()),
             ea_ ? "sisi_eom_ea_sigma" : "sisi_eom_ip_sigma");
-      std::vector<double> y(n);
-      for (size_t i = 0; i < nr1; ++i) y[i] = s1[i];
-      for (size_t p = 0; p < pairs.size(); ++p)
-        for (int64_t k = 0; k < nk; ++k) y[packed(p, k)] = s2[dense_at(pairs[p].first, pairs[p].second, k)];
-      return y;
+      return space.pack(s1, s2);
     };
     const DavidsonResult res = davidson(apply, diag, lowest_diagonal_basis(diag, opt.nroots), opt);
     std::ofstream dat(ea_ ? "EAEomEnergies.dat" : "IPEomEnergies.dat");
@@ -634,46 +604,21 @@ class EomCcsdDavidson : public Algorithm {
     return t;
   }
 
-  // reference class -> engine class, real antisymmetrized integrals:
-  // V_abij = V_ijab, V_abci = -V_icab, V_aibj = V_iajb
+  // reference classes -> engine classes (pphh_from_hhpp, ppph_from_hppp,
+  // phph_from_hphp); EA's PPPH and PHPH come in the engine's order
   void upload_classes(sisi_engine *e, int64_t o, int64_t v) {
-    const Tensor &hhpp = real_class("HHPPCoulombIntegrals", {o, o, v, v});
-    const Tensor &hhhp = real_class("HHHPCoulombIntegrals", {o, o, o, v});
-    std::vector<double> buf(static_cast<size_t>(v * v * o * o));
-    for (int64_t j = 0; j < o; ++j)
-      for (int64_t i = 0; i < o; ++i)
-        for (int64_t b = 0; b < v; ++b)
-          for (int64_t a = 0; a < v; ++a)
-            buf[size_t(a + v * (b + v * (i + o * j)))] = hhpp.values[size_t(i + o * (j + o * (a + v * b)))];
-    check(sisi_engine_set_integrals(e, SISI_PPHH, buf.data()), "sisi_engine_set_integrals");
-    check(sisi_engine_set_integrals(e, SISI_HHHP, hhhp.values.data()), "sisi_engine_set_integrals");
+    auto set = [&](int cls, const double *p) { check(sisi_engine_set_integrals(e, cls, p), "sisi_engine_set_integrals"); };
+    set(SISI_PPHH, pphh_from_hhpp(real_class("HHPPCoulombIntegrals", {o, o, v, v}).values.data(), o, v).data());
+    set(SISI_HHHP, real_class("HHHPCoulombIntegrals", {o, o, o, v}).values.data());
     if (ea_) {
-      check(sisi_engine_set_integrals(e, SISI_PPPP, real_class("PPPPCoulombIntegrals", {v, v, v, v}).values.data()),
-            "sisi_engine_set_integrals");
-      check(sisi_engine_set_integrals(e, SISI_PPPH, real_class("PPPHCoulombIntegrals", {v, v, v, o}).values.data()),
-            "sisi_engine_set_integrals");
-      check(sisi_engine_set_integrals(e, SISI_PHPH, real_class("PHPHCoulombIntegrals", {v, o, v, o}).values.data()),
-            "sisi_engine_set_integrals");
+      set(SISI_PPPP, real_class("PPPPCoulombIntegrals", {v, v, v, v}).values.data());
+      set(SISI_PPPH, real_class("PPPHCoulombIntegrals", {v, v, v, o}).values.data());
+      set(SISI_PHPH, real_class("PHPHCoulombIntegrals", {v, o, v, o}).values.data());
       return;
     }
-    check(sisi_engine_set_integrals(e, SISI_HHHH, real_class("HHHHCoulombIntegrals", {o, o, o, o}).values.data()),
-          "sisi_engine_set_integrals");
-    const Tensor &hppp = real_class("HPPPCoulombIntegrals", {o, v, v, v});
-    const Tensor &hphp = real_class("HPHPCoulombIntegrals", {o, v, o, v});
-    buf.assign(static_cast<size_t>(v * v * v * o), 0.0);
-    for (int64_t i = 0; i < o; ++i)
-      for (int64_t c = 0; c < v; ++c)
-        for (int64_t b = 0; b < v; ++b)
-          for (int64_t a = 0; a < v; ++a)
-            buf[size_t(a + v * (b + v * (c + v * i)))] = -hppp.values[size_t(i + o * (c + v * (a + v * b)))];
-    check(sisi_engine_set_integrals(e, SISI_PPPH, buf.data()), "sisi_engine_set_integrals");
-    buf.assign(static_cast<size_t>(v * o * v * o), 0.0);
-    for (int64_t j = 0; j < o; ++j)
-      for (int64_t b = 0; b < v; ++b)
-        for (int64_t i = 0; i < o; ++i)
-          for (int64_t a = 0; a < v; ++a)
-            buf[size_t(a + v * (i + o * (b + v * j)))] = hphp.values[size_t(i + o * (a + v * (j + o * b)))];
-    check(sisi_engine_set_integrals(e, SISI_PHPH, buf.data()), "sisi_engine_set_integrals");
+    set(SISI_HHHH, real_class("HHHHCoulombIntegrals", {o, o, o, o}).values.data());
+    set(SISI_PPPH, ppph_from_hppp(real_class("HPPPCoulombIntegrals", {o, v, v, v}).values.data(), o, v).data());
+    set(SISI_PHPH, phph_from_hphp(real_class("HPHPCoulombIntegrals", {o, v, o, v}).values.data(), o, v).data());
   }
 };
 

@@ -1,4 +1,6 @@
-// CPU harness for sisi4s_amd/host/davidson.hpp, driven by tests/test_davidson.py.
+// CPU harness for sisi4s_amd/host/davidson.hpp and eom_space.hpp, driven by
+// tests/test_davidson.py.  Modes "unpack", "pack", "diag" and "classes" read
+// "ea o v" (classes: "o v") and then the vectors described at each branch.
 // Reads from stdin: mode ("eig" or "davidson"), n, then for "davidson" the
 // root count, tolerance_vectors, tolerance_energy, max_basis, max_iterations,
 // min_iterations and refresh flag, then the n x n matrix (row-major) and, for
@@ -9,11 +11,69 @@
 #include <string>
 
 #include "../sisi4s_amd/host/davidson.hpp"
+#include "../sisi4s_amd/host/eom_space.hpp"
+
+static void print_line(const std::vector<double> &x) {
+  for (size_t i = 0; i < x.size(); ++i) std::printf(i ? " %.17g" : "%.17g", x[i]);
+  std::printf("\n");
+}
+
+static std::vector<double> read_vector(size_t n) {
+  std::vector<double> x(n);
+  for (auto &y : x) std::cin >> y;
+  return x;
+}
+
+// EomSpace and the class reorderings: returns -1 if mode is not one of them
+static int eom_modes(const std::string &mode) {
+  if (mode == "classes") {  // o v, then HHPP, HPPP, HPHP (first index fastest)
+    int64_t o = 0, v = 0;
+    std::cin >> o >> v;
+    const auto hhpp = read_vector(size_t(o * o * v * v)), hppp = read_vector(size_t(o * v * v * v)),
+               hphp = read_vector(size_t(o * v * o * v));
+    if (!std::cin) return 2;
+    print_line(sisi_host::pphh_from_hhpp(hhpp.data(), o, v));
+    print_line(sisi_host::ppph_from_hppp(hppp.data(), o, v));
+    print_line(sisi_host::phph_from_hphp(hphp.data(), o, v));
+    return 0;
+  }
+  if (mode != "unpack" && mode != "pack" && mode != "diag") return -1;
+  int ea = 0;
+  int64_t o = 0, v = 0;
+  std::cin >> ea >> o >> v;
+  const sisi_host::EomSpace space(ea != 0, o, v);
+  if (mode == "diag") {  // eps_i[o], eps_a[v] -> packed diagonal
+    const auto ei = read_vector(size_t(o)), eav = read_vector(size_t(v));
+    if (!std::cin) return 2;
+    print_line(space.diagonal(ei.data(), eav.data()));
+    return 0;
+  }
+  int m = 0;
+  std::cin >> m;
+  for (int k = 0; k < m; ++k) {
+    if (mode == "unpack") {  // packed vector -> line: r1 then r2
+      const auto x = read_vector(space.n);
+      if (!std::cin) return 2;
+      std::vector<double> r1, r2;
+      space.unpack(x, r1, r2);
+      r1.insert(r1.end(), r2.begin(), r2.end());
+      print_line(r1);
+    } else {  // r1 then r2 -> packed line
+      const auto s1 = read_vector(space.r1_size()), s2 = read_vector(space.r2_size());
+      if (!std::cin) return 2;
+      print_line(space.pack(s1, s2));
+    }
+  }
+  return 0;
+}
 
 int main() {
   std::string mode;
+  std::cin >> mode;
+  const int rc = eom_modes(mode);
+  if (rc >= 0) return rc;
   int n = 0;
-  std::cin >> mode >> n;
+  std::cin >> n;
   sisi_host::DavidsonOptions opt;
   int refresh = 0;
   if (mode == "davidson")

@@ -169,3 +169,70 @@ def test_davidson_ip_eom_closed_shell(harness):
     assert np.abs(vals.real - ref[:4]).max() <= 1e-9
     assert abs(vals[0].real - vals[1].real) <= 1e-9 and abs(vals[2].real - vals[3].real) <= 1e-9
     assert resid.max() <= 1e-7
+
+
+def _lines(exe, text):
+    return [np.array([float(x) for x in l.split()]) for l in _run(exe, text)]
+
+
+def _fmt(x):
+    return " ".join("%.17g" % y for y in np.asarray(x).ravel(order="F"))
+
+
+@pytest.mark.parametrize("ea", [False, True])
+def test_eom_space_packing_reproduces_the_oracle_spectrum(harness, ea):
+    """The EOM steps' packed space (sisi4s_amd/host/eom_space.hpp): unpacking
+    the unit vectors, applying the oracle's sigma to the dense r1 / r2 arrays
+    in the layout the C ABI takes, and packing the result gives a matrix with
+    the oracle's IP / EA spectrum; unpack then pack is the identity."""
+    from oracle import eom
+    from test_eom_program import _system
+    ints, ei, ea_, T1, T2, _ = _system(3, 8, 31)
+    o, v = 3, 5
+    n1, r2shape = (v, (o, v, v)) if ea else (o, (o, o, v))
+    n = n1 + (v * (v - 1) // 2 * o if ea else o * (o - 1) // 2 * v)
+    head = "%d %d %d" % (int(ea), o, v)
+    dense = _lines(harness, "unpack %s %d\n%s\n" % (head, n, _fmt(np.eye(n))))
+    assert len(dense) == n
+    H = (eom.hbar_ea if ea else eom.hbar_ip)(ints, ei, ea_, T1, T2)
+    sigma = eom.ea_sigma if ea else eom.ip_sigma
+    cols = []
+    for x in dense:
+        r1, r2 = x[:n1], x[n1:].reshape(r2shape, order="F")
+        assert np.abs(r2 + (r2.transpose(0, 2, 1) if ea else r2.transpose(1, 0, 2))).max() == 0.0
+        s1, s2 = sigma(H, r1, r2)
+        cols.append(_fmt(s1) + " " + _fmt(s2))
+    M = np.array(_lines(harness, "pack %s %d\n%s\n" % (head, n, "\n".join(cols)))).T
+    ref = (eom.ea_eigenvalues if ea else eom.ip_eigenvalues)(ints, ei, ea_, T1, T2)
+    assert np.abs(np.sort(np.linalg.eigvals(M).real) - ref).max() <= 1e-11
+    back = np.array(_lines(harness, "pack %s %d\n%s\n" % (head, n, "\n".join(_fmt(x) for x in dense))))
+    assert np.array_equal(back, np.eye(n))
+    d = _lines(harness, "diag %s\n%s %s\n" % (head, _fmt(ei), _fmt(ea_)))[0]
+    # the diagonal of the zeroth-order (Fock) part of Hbar: sigma with T = 0 and no integrals
+    zero = {k: np.zeros_like(x) for k, x in ints.items()}
+    H0 = (eom.hbar_ea if ea else eom.hbar_ip)(zero, ei, ea_, np.zeros_like(T1), np.zeros_like(T2))
+    D = []
+    for x in dense:
+        s1, s2 = sigma(H0, x[:n1], x[n1:].reshape(r2shape, order="F"))
+        D.append(np.concatenate([s1, s2.ravel(order="F")]) @ x / (x @ x))
+    # each unit vector x is an eigenvector of H0; its Rayleigh quotient in the
+    # dense metric equals the packed diagonal element
+    assert np.abs(np.array(D) - d).max() <= 1e-14
+
+
+def test_class_reorderings_of_the_eom_steps(harness):
+    """HHPP -> PPHH, HPPP -> PPPH, HPHP -> PHPH as the steps upload them,
+    against slices of one antisymmetrized tensor."""
+    rng = np.random.default_rng(5)
+    n, o = 7, 3
+    g = rng.normal(size=(6, n, n))
+    g = 0.5 * (g + np.transpose(g, (0, 2, 1)))
+    eri = np.einsum("Gpr,Gqs->pqrs", g, g)
+    a = eri - np.swapaxes(eri, 2, 3)
+    h, p = slice(0, o), slice(o, n)
+    v = n - o
+    out = _lines(harness, "classes %d %d\n%s\n%s\n%s\n" % (o, v, _fmt(a[h, h, p, p]), _fmt(a[h, p, p, p]),
+                                                          _fmt(a[h, p, h, p])))
+    assert np.abs(out[0] - a[p, p, h, h].ravel(order="F")).max() == 0.0
+    assert np.abs(out[1] - a[p, p, p, h].ravel(order="F")).max() <= 1e-15
+    assert np.abs(out[2] - a[p, h, p, h].ravel(order="F")).max() == 0.0
