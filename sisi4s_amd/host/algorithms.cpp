@@ -558,7 +558,8 @@ class EomCcsdDavidson : public Algorithm {
     opt.tolerance_vectors = getRealArgument("amplitudesConvergence", 1e-6);
     opt.max_iterations = int(getIntegerArgument("maxIterations", 32));
     opt.min_iterations = int(getIntegerArgument("minIterations", 1));
-    opt.max_basis = int(getIntegerArgument("maxBasisSize", o * v + (o * (o - 1) / 2) * (v * (v - 1) / 2)));
+    opt.max_basis = int(std::min<int64_t>(
+        getIntegerArgument("maxBasisSize", o * v + (o * (o - 1) / 2) * (v * (v - 1) / 2)), INT32_MAX));
     opt.refresh_on_max_basis = getIntegerArgument("refreshOnMaxBasisSize", 0) == 1;
 
     sisi_dims d{int(o), int(v), 1};
