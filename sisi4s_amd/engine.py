@@ -181,6 +181,18 @@ def _ptr(b):
     return b.ctypes.data_as(_c_double_p) if b is not None else None
 
 
+def _sized(x, n, name, cplx=False):
+    """_buf(x) after checking that x holds exactly n logical elements, so
+    the library never reads past a caller's array."""
+    if np.size(x) != n:
+        raise ValueError("%s: expected %d elements, got %d (shape %s)"
+                         % (name, n, np.size(x), np.shape(x)))
+    if cplx:
+        b = np.ascontiguousarray(np.asarray(x, dtype=np.complex128).ravel(order="F"))
+        return b.view(np.float64)
+    return _buf(x)
+
+
 def _unbuf(b, shape):
     return b.reshape(shape, order="F")
 
@@ -257,8 +269,9 @@ def ccsd_solve_batched(systems, opts=None, workers=8, strict=True):
     ap = (_c_double_p * max(n, 1))()
     for s, (g, ei, ea) in enumerate(systems):
         No, Nv = len(ei), len(ea)
-        dims[s] = Dims(No, Nv, int(np.shape(g)[0]))
-        b = (_buf(g), _buf(ei), _buf(ea))
+        NG = int(np.shape(g)[0])
+        dims[s] = Dims(No, Nv, NG)
+        b = (_sized(g, NG * (No + Nv) ** 2, "system %d gamma[NG,Np,Np]" % s), _buf(ei), _buf(ea))
         bufs.append(b)
         gp[s], ip[s], ap[s] = _ptr(b[0]), _ptr(b[1]), _ptr(b[2])
     o = opts if opts is not None else default_opts()
@@ -302,11 +315,19 @@ class Engine:
     # --- tensors at the boundary (complex128 interleaved in complex mode) --
     complex = False
 
-    def _in(self, x):
+    def _in(self, x, n=None, name="tensor"):
+        if n is not None:
+            return _sized(x, n, name, self.complex)
         if not self.complex:
             return _buf(x)
         b = np.ascontiguousarray(np.asarray(x, dtype=np.complex128).ravel(order="F"))
         return b.view(np.float64)
+
+    def _t1(self, T1):
+        return self._in(T1, self.Nv * self.No, "T1[Nv,No]")
+
+    def _t2(self, T2):
+        return self._in(T2, self.Nv * self.Nv * self.No * self.No, "T2[Nv,Nv,No,No]")
 
     def _out(self, shape):
         n = int(np.prod(shape))
@@ -319,22 +340,20 @@ class Engine:
 
     # --- inputs -----------------------------------------------------------
     def set_vertex(self, gamma):
-        g = _buf(gamma)
-        assert g.size == self.NG * self.Np * self.Np
+        g = _sized(gamma, self.NG * self.Np * self.Np, "gamma[NG,Np,Np]")
         _check(self._lib.sisi_engine_set_vertex(self._h, _ptr(g)))
         self.complex = False
 
     def set_vertex_complex(self, gamma):
         """Complex vertex: the engine switches to the reference's complex
         path (CcsdEnergyFromCoulombIntegrals.cxx:496-891)."""
-        g = np.ascontiguousarray(np.asarray(gamma, dtype=np.complex128).ravel(order="F"))
-        assert g.size == self.NG * self.Np * self.Np
-        _check(self._lib.sisi_engine_set_vertex_complex(self._h, _ptr(g.view(np.float64))))
+        g = _sized(gamma, self.NG * self.Np * self.Np, "gamma[NG,Np,Np]", cplx=True)
+        _check(self._lib.sisi_engine_set_vertex_complex(self._h, _ptr(g)))
         self.complex = True
 
     def set_eigenenergies(self, eps_i, eps_a):
-        ei = _buf(eps_i)
-        ea = _buf(eps_a)
+        ei = _sized(eps_i, self.No, "eps_i[No]")
+        ea = _sized(eps_a, self.Nv, "eps_a[Nv]")
         _check(self._lib.sisi_engine_set_eigenenergies(self._h, _ptr(ei), _ptr(ea)))
 
     def set_system(self, gamma, eps_i, eps_a, integrals=True):
@@ -378,7 +397,8 @@ class Engine:
         L, P = z(Lambda), z(Pi)
         O = z(PiOut) if PiOut is not None else None
         NR = int(np.shape(Lambda)[1])
-        assert L.size == self.NG * NR and P.size == self.Np * NR
+        if L.size != self.NG * NR or P.size != self.Np * NR or (O is not None and O.size != P.size):
+            raise ValueError("set_coulomb_factors: Lambda must be [NG,NR], Pi and PiOut [Np,NR]")
         _check(self._lib.sisi_engine_set_coulomb_factors(
             self._h, NR, int(slice_size), _ptr(L.view(np.float64)), _ptr(P.view(np.float64)),
             _ptr(O.view(np.float64)) if O is not None else None))
@@ -386,7 +406,13 @@ class Engine:
     def set_integrals(self, which, V):
         """Upload one real integral class (no vertex needed for the
         full-integral formulation)."""
-        b = _buf(V)
+        o, v = self.No, self.Nv
+        sizes = {PPHH: v * v * o * o, PHPH: v * o * v * o, HHHH: o ** 4, HHHP: o ** 3 * v,
+                 HHPP: o * o * v * v, PHHP: v * o * o * v, PHHH: v * o ** 3, PPPP: v ** 4,
+                 PPPH: v ** 3 * o}
+        if int(which) not in sizes:
+            raise ValueError("set_integrals: unknown integral class %r" % (which,))
+        b = _sized(V, sizes[int(which)], "integrals class %d" % int(which))
         _check(self._lib.sisi_engine_set_integrals(self._h, int(which), _ptr(b)))
 
     def set_formulation(self, formulation):
@@ -404,8 +430,11 @@ class Engine:
         on a spin-orbital engine: returns (s1[i], s2[i,j,a])."""
         o, v = self.No, self.Nv
         b1, b2 = np.empty(o), np.empty(o * o * v)
-        _check(self._lib.sisi_eom_ip_sigma(self._h, _ptr(_buf(T1)), _ptr(_buf(T2)), _ptr(_buf(r1)),
-                                           _ptr(_buf(r2)), _ptr(b1), _ptr(b2)))
+        t1 = _sized(T1, v * o, "T1[Nv,No]")
+        t2 = _sized(T2, v * v * o * o, "T2[Nv,Nv,No,No]")
+        x1, x2 = _sized(r1, o, "r1[No]"), _sized(r2, o * o * v, "r2[No,No,Nv]")
+        _check(self._lib.sisi_eom_ip_sigma(self._h, _ptr(t1), _ptr(t2), _ptr(x1), _ptr(x2),
+                                           _ptr(b1), _ptr(b2)))
         return _unbuf(b1, (o,)), _unbuf(b2, (o, o, v))
 
     def eom_ea_sigma(self, T1, T2, r1, r2):
@@ -413,8 +442,11 @@ class Engine:
         on a spin-orbital engine with PPPP uploaded: returns (s1[a], s2[j,a,b])."""
         o, v = self.No, self.Nv
         b1, b2 = np.empty(v), np.empty(o * v * v)
-        _check(self._lib.sisi_eom_ea_sigma(self._h, _ptr(_buf(T1)), _ptr(_buf(T2)), _ptr(_buf(r1)),
-                                           _ptr(_buf(r2)), _ptr(b1), _ptr(b2)))
+        t1 = _sized(T1, v * o, "T1[Nv,No]")
+        t2 = _sized(T2, v * v * o * o, "T2[Nv,Nv,No,No]")
+        x1, x2 = _sized(r1, v, "r1[Nv]"), _sized(r2, o * v * v, "r2[No,Nv,Nv]")
+        _check(self._lib.sisi_eom_ea_sigma(self._h, _ptr(t1), _ptr(t2), _ptr(x1), _ptr(x2),
+                                           _ptr(b1), _ptr(b2)))
         return _unbuf(b1, (v,)), _unbuf(b2, (o, v, v))
 
     def set_residual_variant(self, ppl=True, distinguishable=False):
@@ -426,7 +458,7 @@ class Engine:
     # --- residual / energies ---------------------------------------------
     def residual(self, iteration, T1, T2, slice_size=-1, initial_doubles=False):
         v, o = self.Nv, self.No
-        t1, t2 = self._in(T1), self._in(T2)
+        t1, t2 = self._t1(T1), self._t2(T2)
         r1, r2 = self._out((v, o)), self._out((v, v, o, o))
         _check(self._lib.sisi_ccsd_residual(self._h, iteration, int(initial_doubles),
                                             int(slice_size), _ptr(t1), _ptr(t2), _ptr(r1),
@@ -441,7 +473,8 @@ class Engine:
 
     def _residual2(self, fn, T1, T2, slice_size):
         v, o = self.Nv, self.No
-        t1, t2 = _buf(T1), _buf(T2)
+        t1 = _sized(T1, v * o, "T1[Nv,No]")
+        t2 = _sized(T2, v * v * o * o, "T2[Nv,Nv,No,No]")
         r1 = np.empty(v * o)
         r2 = np.empty(v * v * o * o)
         _check(fn(self._h, int(slice_size), _ptr(t1), _ptr(t2), _ptr(r1), _ptr(r2)))
@@ -449,7 +482,7 @@ class Engine:
 
     def energy(self, T1, T2):
         out = np.empty(3)
-        _check(self._lib.sisi_ccsd_energy(self._h, _ptr(self._in(T1)), _ptr(self._in(T2)),
+        _check(self._lib.sisi_ccsd_energy(self._h, _ptr(self._t1(T1)), _ptr(self._t2(T2)),
                                           _ptr(out)))
         return tuple(out)
 
@@ -461,8 +494,8 @@ class Engine:
     # --- driver ------------------------------------------------------------
     def ccsd_begin(self, opts=None, T1=None, T2=None):
         o = opts if opts is not None else default_opts()
-        t1 = self._in(T1) if T1 is not None else None
-        t2 = self._in(T2) if T2 is not None else None
+        t1 = self._t1(T1) if T1 is not None else None
+        t2 = self._t2(T2) if T2 is not None else None
         _check(self._lib.sisi_ccsd_begin(self._h, ctypes.byref(o), _ptr(t1), _ptr(t2)))
 
     def ccsd_iterate(self, n):
@@ -472,8 +505,8 @@ class Engine:
 
     def ccsd_solve(self, opts=None, T1=None, T2=None):
         o = opts if opts is not None else default_opts()
-        t1 = self._in(T1) if T1 is not None else None
-        t2 = self._in(T2) if T2 is not None else None
+        t1 = self._t1(T1) if T1 is not None else None
+        t2 = self._t2(T2) if T2 is not None else None
         r = CcsdReport()
         _check(self._lib.sisi_ccsd_solve(self._h, ctypes.byref(o), _ptr(t1), _ptr(t2),
                                          ctypes.byref(r)))
@@ -503,8 +536,8 @@ class Engine:
         if isinstance(precision, str):
             precision = PRECISIONS[precision]
         e = ctypes.c_double()
-        t1 = self._in(T1) if T1 is not None else None
-        t2 = self._in(T2) if T2 is not None else None
+        t1 = self._t1(T1) if T1 is not None else None
+        t2 = self._t2(T2) if T2 is not None else None
         if blocks is None:
             _check(self._lib.sisi_triples_energy(self._h, _ptr(t1), _ptr(t2), int(precision),
                                                  ctypes.byref(e)))

@@ -152,29 +152,35 @@ inline std::vector<std::complex<double>> small_eigenvalues(std::vector<double> A
   return ev;
 }
 
-// Right eigenvector of A (row-major n x n) for the real eigenvalue lambda by
-// inverse iteration (LU with partial pivoting of A - (lambda + delta) I).
-inline std::vector<double> small_eigenvector(const std::vector<double> &A, int n, double lambda) {
-  const double delta = 1e-10 * (1.0 + std::fabs(lambda));
-  std::vector<double> M(A);
+// Right eigenvector of A (row-major n x n) for the eigenvalue lambda (real,
+// or one of a complex-conjugate pair) by inverse iteration: LU with partial
+// pivoting of A - (lambda + delta) I in complex arithmetic, three sweeps from
+// the all-ones vector, normalised to unit 2-norm.  For a complex lambda the
+// real and imaginary parts of the result span the pair's real invariant
+// subspace: A Re y = Re(lambda) Re y - Im(lambda) Im y.
+inline std::vector<std::complex<double>> small_eigenvector_complex(const std::vector<double> &A, int n,
+                                                                   std::complex<double> lambda) {
+  using cd = std::complex<double>;
+  const double delta = 1e-10 * (1.0 + std::abs(lambda));
+  std::vector<cd> M(A.begin(), A.end());
   for (int i = 0; i < n; ++i) M[size_t(i) * n + i] -= lambda + delta;
   std::vector<int> piv(static_cast<size_t>(n));
   for (int k = 0; k < n; ++k) {
     int p = k;
     for (int i = k + 1; i < n; ++i)
-      if (std::fabs(M[size_t(i) * n + k]) > std::fabs(M[size_t(p) * n + k])) p = i;
+      if (std::abs(M[size_t(i) * n + k]) > std::abs(M[size_t(p) * n + k])) p = i;
     piv[size_t(k)] = p;
     if (p != k)
       for (int j = 0; j < n; ++j) std::swap(M[size_t(k) * n + j], M[size_t(p) * n + j]);
-    double pk = M[size_t(k) * n + k];
-    if (pk == 0.0) pk = M[size_t(k) * n + k] = 1e-300;
+    cd pk = M[size_t(k) * n + k];
+    if (pk == cd(0.0)) pk = M[size_t(k) * n + k] = 1e-300;
     for (int i = k + 1; i < n; ++i) {
-      const double f = M[size_t(i) * n + k] / pk;
+      const cd f = M[size_t(i) * n + k] / pk;
       M[size_t(i) * n + k] = f;
       for (int j = k + 1; j < n; ++j) M[size_t(i) * n + j] -= f * M[size_t(k) * n + j];
     }
   }
-  std::vector<double> x(size_t(n), 1.0);
+  std::vector<cd> x(size_t(n), cd(1.0));
   for (int it = 0; it < 3; ++it) {
     for (int k = 0; k < n; ++k) std::swap(x[size_t(k)], x[size_t(piv[size_t(k)])]);
     for (int i = 0; i < n; ++i)
@@ -184,10 +190,25 @@ inline std::vector<double> small_eigenvector(const std::vector<double> &A, int n
       x[size_t(i)] /= M[size_t(i) * n + i];
     }
     double nrm = 0.0;
-    for (double v : x) nrm += v * v;
+    for (const cd &v : x) nrm += std::norm(v);
     nrm = std::sqrt(nrm);
-    for (double &v : x) v /= nrm;
+    for (cd &v : x) v /= nrm;
   }
+  return x;
+}
+
+// Real right eigenvector for a real eigenvalue lambda (the real part of the
+// complex inverse iteration's result, which is real up to rounding).
+inline std::vector<double> small_eigenvector(const std::vector<double> &A, int n, double lambda) {
+  const auto z = small_eigenvector_complex(A, n, std::complex<double>(lambda, 0.0));
+  std::vector<double> x(static_cast<size_t>(n));
+  double nrm = 0.0;
+  for (int i = 0; i < n; ++i) {
+    x[size_t(i)] = z[size_t(i)].real();
+    nrm += x[size_t(i)] * x[size_t(i)];
+  }
+  nrm = std::sqrt(nrm);
+  for (double &v : x) v /= nrm;
   return x;
 }
 
@@ -270,10 +291,15 @@ inline DavidsonResult davidson(const std::function<std::vector<double>(const std
   std::vector<std::complex<double>> prev(size_t(nroots), 0.0);
   for (int it = 1;; ++it) {
     if (opt.refresh_on_max_basis && int(V.size()) >= opt.max_basis && !res.vectors.empty()) {
-      std::vector<std::vector<double>> keep = res.vectors;
+      // restart from the Ritz vectors; if they span fewer than nroots
+      // directions, refill from the old basis in its order
+      std::vector<std::vector<double>> keep = res.vectors, old = V;
       V.clear();
       W.clear();
       for (auto &x : keep) append(x, 1e-12);
+      for (size_t j = 0; j < old.size() && int(V.size()) < nroots; ++j) append(old[j], 1e-8);
+      if (int(V.size()) < nroots)
+        throw std::runtime_error("davidson: restarted basis spans fewer vectors than eigenstates");
     }
     while (W.size() < V.size()) {
       W.push_back(apply(V[W.size()]));
@@ -287,31 +313,53 @@ inline DavidsonResult davidson(const std::function<std::vector<double>(const std
     std::stable_sort(ev.begin(), ev.end(), [](const std::complex<double> &a, const std::complex<double> &b) {
       return a.real() < b.real();
     });
+    if (int(ev.size()) < nroots) throw std::runtime_error("davidson: basis smaller than the number of eigenstates");
     res.values.assign(ev.begin(), ev.begin() + nroots);
     res.vectors.assign(size_t(nroots), std::vector<double>(size_t(n), 0.0));
     res.rms = 0.0;
     res.energy_difference = 0.0;
-    std::vector<std::vector<double>> corrections;
     for (int k = 0; k < nroots; ++k) {
-      const double theta = ev[size_t(k)].real();
-      const std::vector<double> y = small_eigenvector(Hs, m, theta);
-      std::vector<double> x(size_t(n), 0.0), r(size_t(n), 0.0);
+      res.energy_difference += std::abs(res.values[size_t(k)] - prev[size_t(k)]);
+      prev[size_t(k)] = res.values[size_t(k)];
+    }
+    std::vector<std::vector<double>> corrections;
+    // Ritz vectors and residuals.  A real theta gives one real Ritz vector.
+    // A complex-conjugate pair (theta, conj theta) gives the complex Ritz
+    // vector y; Re y and Im y span the pair's invariant subspace and serve as
+    // the two roots' vectors (one shift for both would give the same vector
+    // twice), with the complex residual r = H y - theta y split the same way
+    // and the corrections Re and Im of r / (theta - diag).  A pair cut by the
+    // root count keeps Re y as the last root and still adds both corrections.
+    for (int k = 0; k < nroots;) {
+      const std::complex<double> theta = ev[size_t(k)];
+      const bool cplx = theta.imag() != 0.0;
+      const auto y = small_eigenvector_complex(Hs, m, theta);
+      std::vector<std::complex<double>> x(size_t(n), 0.0), r(size_t(n), 0.0);
       for (int j = 0; j < m; ++j)
         for (int i = 0; i < n; ++i) {
           x[size_t(i)] += y[size_t(j)] * V[size_t(j)][size_t(i)];
           r[size_t(i)] += y[size_t(j)] * W[size_t(j)][size_t(i)];
         }
       for (int i = 0; i < n; ++i) r[size_t(i)] -= theta * x[size_t(i)];
-      res.rms += dot(r, r) / dot(x, x);
-      res.energy_difference += std::abs(res.values[size_t(k)] - prev[size_t(k)]);
-      prev[size_t(k)] = res.values[size_t(k)];
-      std::vector<double> t(static_cast<size_t>(n));
-      for (int i = 0; i < n; ++i) {
-        const double den = theta - diag[size_t(i)];
-        t[size_t(i)] = std::fabs(den) < 1e-4 ? 0.0 : r[size_t(i)] / den;
+      const int parts = cplx ? 2 : 1;
+      for (int part = 0; part < parts; ++part) {
+        std::vector<double> xr(static_cast<size_t>(n)), rr(static_cast<size_t>(n)),
+            t(static_cast<size_t>(n));
+        for (int i = 0; i < n; ++i) {
+          xr[size_t(i)] = part ? x[size_t(i)].imag() : x[size_t(i)].real();
+          rr[size_t(i)] = part ? r[size_t(i)].imag() : r[size_t(i)].real();
+          const std::complex<double> den = theta - diag[size_t(i)];
+          const std::complex<double> c = std::abs(den) < 1e-4 ? 0.0 : r[size_t(i)] / den;
+          t[size_t(i)] = part ? c.imag() : c.real();
+        }
+        corrections.push_back(std::move(t));
+        if (k + part < nroots) {
+          const double xx = dot(xr, xr);
+          res.rms += xx > 0.0 ? dot(rr, rr) / xx : 0.0;
+          res.vectors[size_t(k + part)] = std::move(xr);
+        }
       }
-      corrections.push_back(std::move(t));
-      res.vectors[size_t(k)] = std::move(x);
+      k += parts;
     }
     res.iterations = it;
     int appended = 0;

@@ -100,14 +100,38 @@ int main() {
   };
   const auto res = sisi_host::davidson(apply, diag, sisi_host::lowest_diagonal_basis(diag, opt.nroots), opt);
   for (const auto &z : res.values) std::printf("%.17g %.17g\n", z.real(), z.imag());
-  // residual check of the returned vectors, |A x - theta x| / |x| per root
+  // residual check of the returned vectors per root: |A x - theta x| / |x|
+  // for a real theta; for one of a complex pair, the distance of A x from the
+  // span of the pair's two vectors (their invariant subspace), / |x|
   for (size_t k = 0; k < res.vectors.size(); ++k) {
-    const auto y = apply(res.vectors[k]);
-    double r = 0.0, xx = 0.0;
-    for (int i = 0; i < n; ++i) {
-      const double d = y[size_t(i)] - res.values[k].real() * res.vectors[k][size_t(i)];
-      r += d * d;
-      xx += res.vectors[k][size_t(i)] * res.vectors[k][size_t(i)];
+    const auto &x = res.vectors[k];
+    const auto y = apply(x);
+    auto dot = [&](const std::vector<double> &a, const std::vector<double> &b) {
+      double s = 0.0;
+      for (int i = 0; i < n; ++i) s += a[size_t(i)] * b[size_t(i)];
+      return s;
+    };
+    double r = 0.0;
+    const double xx = dot(x, x);
+    size_t partner = k;
+    if (res.values[k].imag() != 0.0)
+      for (size_t q = 0; q < res.vectors.size(); ++q)
+        if (q != k && std::abs(res.values[q] - std::conj(res.values[k])) <= 1e-9 * (1.0 + std::abs(res.values[k])))
+          partner = q;
+    if (partner == k) {
+      for (int i = 0; i < n; ++i) {
+        const double d = y[size_t(i)] - res.values[k].real() * x[size_t(i)];
+        r += d * d;
+      }
+    } else {  // least squares of y on [x, z]
+      const auto &z = res.vectors[partner];
+      const double a11 = xx, a12 = dot(x, z), a22 = dot(z, z), b1 = dot(x, y), b2 = dot(z, y);
+      const double det = a11 * a22 - a12 * a12;
+      const double c1 = (b1 * a22 - b2 * a12) / det, c2 = (a11 * b2 - a12 * b1) / det;
+      for (int i = 0; i < n; ++i) {
+        const double d = y[size_t(i)] - c1 * x[size_t(i)] - c2 * z[size_t(i)];
+        r += d * d;
+      }
     }
     std::printf("residual %.17g\n", std::sqrt(r / xx));
   }

@@ -132,6 +132,36 @@ def test_davidson_stops_at_the_iteration_limit_and_basis_size(harness):
     assert np.abs(vals.real - ref[:2]).max() <= 1e-10
 
 
+def _complex_lowest_pair(n, seed):
+    """A non-symmetric matrix whose two lowest eigenvalues (by real part) are
+    the complex pair -2 +- 0.05i (a 2 x 2 rotation block), the others real and
+    well separated above, under a similarity close to the identity so that
+    the diagonal is a useful preconditioner."""
+    rng = np.random.default_rng(seed)
+    B = np.diag(np.linspace(-1.0, 6.0, n))
+    B[:2, :2] = [[-2.0, 0.05], [-0.05, -2.0]]
+    S = np.eye(n) + 0.01 * rng.normal(size=(n, n))
+    A = S @ B @ np.linalg.inv(S)
+    ref = np.sort_complex(np.linalg.eigvals(A))
+    assert abs(ref[0] - (-2.0 - 0.05j)) <= 1e-10 and abs(ref[1] - (-2.0 + 0.05j)) <= 1e-10
+    return A, np.diag(A).copy()
+
+
+@pytest.mark.parametrize("refresh,max_basis", [(0, 1000), (1, 6)])
+def test_davidson_complex_lowest_pair(harness, refresh, max_basis):
+    """A complex-conjugate lowest pair: the two roots must come from the
+    pair's two-dimensional invariant subspace (Re and Im of the complex Ritz
+    vector), not from one real shift used twice, with and without restarts
+    on a small basis."""
+    A, d = _complex_lowest_pair(60, 21)
+    vals, resid, info = davidson(harness, A, d, 2, tol_vectors=1e-22, tol_energy=0.0,
+                                 max_basis=max_basis, max_iterations=400, refresh=refresh)
+    assert info["converged"], info
+    assert abs(vals[0].real + 2.0) <= 1e-9 and abs(vals[1].real + 2.0) <= 1e-9
+    assert abs(abs(vals[0].imag) - 0.05) <= 1e-9 and abs(vals[0].imag + vals[1].imag) <= 1e-12
+    assert resid.max() <= 1e-9
+
+
 def test_davidson_min_iterations(harness):
     n = 30
     d = np.arange(n, dtype=float)

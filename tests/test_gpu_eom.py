@@ -86,8 +86,6 @@ def test_ip_eigenvalues_exact_for_two_electrons():
     assert np.abs(ev - exact).max() <= 1e-10
 
 
-@pytest.mark.skip(reason="the EA right-apply has not yet been run on a GPU; its contraction "
-                         "program is checked against the oracle on the CPU (test_eom_program.py)")
 def test_ea_sigma_matches_oracle():
     """EA-EOM-CCSD right-apply (SimilarityTransformedHamiltonian.cxx:289-294)
     through the C ABI against the oracle's sigma, and for two virtual spin

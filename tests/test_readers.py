@@ -569,7 +569,6 @@ def test_ip_eom_step_rejects_what_it_does_not_build(tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.skip(reason="the UCcsdIPEquationOfMotionDavidson step has not yet been run on a GPU")
 def test_ip_eom_davidson_step_matches_full_ci_and_the_oracle(tmp_path):
     """The IP-EOM-CCSD Davidson step from files (Read of the reference's
     classes and amplitudes): two electrons reproduce full CI's ionization
@@ -600,7 +599,6 @@ def test_ip_eom_davidson_step_matches_full_ci_and_the_oracle(tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.skip(reason="the UCcsdEAEquationOfMotionDavidson step has not yet been run on a GPU")
 def test_ea_eom_davidson_step_matches_full_ci(tmp_path):
     """The EA-EOM-CCSD Davidson step from files: with two virtual spin
     orbitals its roots equal the oracle's dense EA spectrum, which equals full
