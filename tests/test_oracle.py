@@ -217,6 +217,36 @@ def test_diis_weights():
     np.testing.assert_allclose(mix.weights, w, rtol=1e-10, atol=1e-12)
 
 
+def test_level_shift_keeps_the_fixed_point(monkeypatch):
+    """estimateAmplitudesFromResiduum with a level shift s
+    (ClusterSinglesDoublesAlgorithm.cxx:302-331) maps T to
+    -(R - sT)/(D + s), whose fixed point T D = -R does not depend on s: the
+    shifted and unshifted runs converge to the same energy, and the shifted
+    update leaves the converged amplitudes unchanged.  A shift of the wrong
+    sign (R + sT) has the fixed point T (D + 2s) = -R and is detected."""
+    g, ei, ea = philox.system(7, 4, 12, 48)
+    kw = dict(mixer="DiisMixer", max_residua=6, max_iterations=200,
+              amplitudes_convergence=1e-11, energy_convergence=1e-13)
+    r0 = ccsd.solve(g, ei, ea, level_shift=0.0, **kw)
+    r2 = ccsd.solve(g, ei, ea, level_shift=2.0, **kw)
+    assert r0["converged"] and r2["converged"]
+    assert abs(r0["energy"] - r2["energy"]) <= 1e-10 * abs(r0["energy"])
+    T1, T2 = r0["T1"], r0["T2"]
+    R1, R2 = ccsd.residual(5, T1, T2, g, r0["integrals"])
+    E1, E2 = ccsd.estimate_amplitudes(R1, R2, T1, T2, ei, ea, level_shift=2.0)
+    assert np.abs(E2 - T2).max() <= 1e-8 * np.abs(T2).max()
+    assert np.abs(E1 - T1).max() <= 1e-8 * np.abs(T2).max()
+
+    def wrong_sign(R1, R2, T1, T2, eps_i, eps_a, level_shift=0.0):
+        D1, D2 = ccsd.denominators(eps_i, eps_a)
+        return (-(R1 + level_shift * T1) / (D1 + level_shift),
+                -(R2 + level_shift * T2) / (D2 + level_shift))
+
+    monkeypatch.setattr(ccsd, "estimate_amplitudes", wrong_sign)
+    rw = ccsd.solve(g, ei, ea, level_shift=2.0, **kw)
+    assert abs(rw["energy"] - r0["energy"]) > 1e-4 * abs(r0["energy"])
+
+
 def test_golden_fixture_regression():
     d = np.load(__import__("os").path.join(__import__("os").path.dirname(__file__), "golden",
                                            "tiny_system.npz"))
