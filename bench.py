@@ -79,7 +79,12 @@ def main():
     ap.add_argument("--factors", type=int, default=0,
                     help="iteration mode: NR > 0 runs the factor-based ladder with NR random "
                          "complex CoulombFactors / FactorOrbitals (factor slices of Nv)")
-    ap.add_argument("--triples-blocks", type=int, default=4)
+    ap.add_argument("--triples-blocks", type=int, default=64,
+                    help="(T) blocks timed in FP64 and compared with split-BF16x3")
+    ap.add_argument("--triples-full", type=int, default=1,
+                    help="time one full split-BF16x3 (T) pass over every i<=j<=k block")
+    ap.add_argument("--refine-steps", type=int, default=4,
+                    help="refinement iterations timed after the FP64 anchor")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--uccsd-no", type=int, default=10,
                     help="consumers: occupied spatial orbitals of the spin-orbital CCSD system")
@@ -98,7 +103,9 @@ def main():
                     help="batched: FP64 anchor interval of the mixed runs (0: plain FP64 after the switch)")
     ap.add_argument("--graphs", type=int, default=0,
                     help="batched: replay each system's residual from a captured hipGraph")
-    ap.add_argument("--level-shift", type=float, default=0.0)
+    ap.add_argument("--level-shift", type=float, default=2.0,
+                    help="levelShift of every CCSD run (reference argument; 2.0 keeps the "
+                         "No=40 Nv=400 NG=1600 trajectory converging)")
     ap.add_argument("--max-iterations", type=int, default=400)
     ap.add_argument("--precisions", default="fp64,mixed")
     ap.add_argument("--switch-threshold", type=float, default=1e-5,
@@ -153,13 +160,38 @@ def main():
         e.set_coulomb_factors(lam, pi, slice_size=Nv)
         log("factor-based ladder with NR=%d complex factors" % NR)
 
-    opts = eng.default_opts(mixer="DiisMixer", max_residua=6, max_iterations=10000,
-                            slice_size=s, amplitudes_convergence=1e-8, energy_convergence=1e-10)
-    e.ccsd_begin(opts)
-    for k in range(args.warmup):
+    # MP2 closed form sum V (2V - V') / D from the engine's PPHH, to check the
+    # engine's MP2 energy and the driver's first iteration before anything is
+    # reported: the MP2 start with a level shift s gives T = -V / (D + s), so
+    # iteration 1 is the same sum over D + s
+    V = e.get_integrals(eng.PPHH)
+    D2 = (ea[:, None, None, None] + ea[None, :, None, None]
+          - ei[None, None, :, None] - ei[None, None, None, :])
+    W = V * (2.0 * V - V.transpose(1, 0, 2, 3))
+    e_mp2_closed = float(np.sum(W / (-D2)))
+    e_it1_closed = float(np.sum(W / (-(D2 + args.level_shift))))
+    del V, D2, W
+    e_mp2 = e.mp2_energy()
+    log("MP2 %.10f vs closed form %.10f" % (e_mp2, e_mp2_closed))
+    if abs(e_mp2 - e_mp2_closed) > 1e-9 * abs(e_mp2_closed):
+        raise SystemExit("MP2 energy differs from the closed form by %.3e" % abs(e_mp2 - e_mp2_closed))
+
+    kw = dict(mixer="DiisMixer", max_residua=6, max_iterations=10000, slice_size=s,
+              amplitudes_convergence=1e-8, energy_convergence=1e-10, level_shift=args.level_shift)
+    e.ccsd_begin(eng.default_opts(**kw))
+    trajectory = []
+    for k in range(max(args.warmup, 1)):
         t0 = time.perf_counter()
         rep = e.ccsd_iterate(1)
-        log("warmup iteration %d: %.2fs E=%.10f" % (k + 1, time.perf_counter() - t0, rep["energy"]))
+        trajectory.append((rep["energy"], rep["delta"]))
+        log("warmup iteration %d: %.2fs E=%.10f dT/T=%.2e" % (k + 1, time.perf_counter() - t0,
+                                                             rep["energy"], rep["delta"]))
+        if k == 0:
+            d1 = abs(rep["energy"] - e_it1_closed)
+            log("iteration 1 (MP2 start, levelShift %g) %.10f vs closed form %.10f: |d| %.1e"
+                % (args.level_shift, rep["energy"], e_it1_closed, d1))
+            if d1 > 1e-9 * abs(e_it1_closed):
+                raise SystemExit("iteration-1 energy differs from the closed form by %.3e" % d1)
     e.enable_stats(True)
     e.reset_stats()
     times = []
@@ -167,11 +199,13 @@ def main():
         t0 = time.perf_counter()
         rep = e.ccsd_iterate(1)
         times.append(time.perf_counter() - t0)
+        trajectory.append((rep["energy"], rep["delta"]))
         log("timed iteration %d: %.3fs E=%.10f dT/T=%.2e" % (k + 1, times[-1], rep["energy"],
                                                              rep["delta"]))
     st = e.stats()
     e.enable_stats(False)
     t_iter = float(np.sum(times)) / max(len(times), 1)
+    T1, T2 = e.amplitudes()  # FP64 amplitudes after the timed iterations, for (T)
     flops_it = ccsd_iteration_flops(No, Nv, NG, s)
     if args.factors > 0:
         # the vertex ladder's term replaced by the factor ladder's products per
@@ -183,12 +217,12 @@ def main():
     achieved = g_flops / g_sec / 1e12 if g_sec > 0 else 0.0
     other_bw = st["other_bytes"] / st["other_seconds"] / 1e9 if st["other_seconds"] > 0 else 0.0
 
-    # split-BF16x3 ladder + FP32 iterations of the mixed schedule (timed the same way)
+    # mixed schedule: split-BF16x3 bulk iterations, then refinement iterations
+    # (an FP64 anchor followed by plain-bf16 residual changes), timed the same
+    # way on the same trajectory
     mixed = {}
     if args.mixed_steps > 0:
-        e.set_precision("bf16x3")
-        om = eng.default_opts(mixer="DiisMixer", max_residua=6, max_iterations=10000,
-                              slice_size=s, precision="bf16x3", switch_threshold=0.0)
+        om = eng.default_opts(precision="bf16x3", switch_threshold=0.0, **kw)
         e.ccsd_begin(om)
         e.ccsd_iterate(2)
         e.enable_stats(True)
@@ -206,36 +240,63 @@ def main():
                      gemm_tflops_bf16x3=(sm["gemm_flops"][2] / sm["gemm_seconds"][2] / 1e12
                                          if sm["gemm_seconds"][2] > 0 else None),
                      speedup_per_iteration=t_iter / float(np.mean(mt)))
-        log("mixed iteration %.3fs" % mixed["seconds_per_iteration"])
+        log("mixed bulk iteration %.3fs" % mixed["seconds_per_iteration"])
+    refine = {}
+    if args.refine_steps > 0:
+        # switch at once (threshold 1): iteration 1 bulk, then an FP64 anchor,
+        # then plain-bf16 refinement iterations until the next anchor
+        orf = eng.default_opts(precision="bf16x3", switch_threshold=1.0,
+                               refine_interval=args.refine_interval,
+                               refine_precision=args.refine_precision, **kw)
+        e.ccsd_begin(orf)
+        rep_r = e.ccsd_iterate(2)
+        anchor_t, ref_t = [], []
+        for k in range(args.refine_steps + 1):
+            n64 = rep_r["fp64_iterations"]
+            t0 = time.perf_counter()
+            rep_r = e.ccsd_iterate(1)
+            dt = time.perf_counter() - t0
+            (anchor_t if rep_r["fp64_iterations"] > n64 else ref_t).append(dt)
+        refine = dict(refinement_seconds_per_iteration=float(np.mean(ref_t)) if ref_t else None,
+                      anchor_seconds=float(np.mean(anchor_t)) if anchor_t else None,
+                      refinement_iterations_timed=len(ref_t), anchors_timed=len(anchor_t),
+                      refine_precision=args.refine_precision,
+                      refine_interval=args.refine_interval)
+        log("refinement iteration %s s, anchor %s s" % (refine["refinement_seconds_per_iteration"],
+                                                        refine["anchor_seconds"]))
 
-    # (T): sample of distinct-index blocks at the workload size, extrapolated
+    # (T) on the FP64 amplitudes: FP64 and split-BF16x3 on a sample of blocks
+    # (their energy difference), then one full split-BF16x3 pass
     triples = {}
+    nblk = No * (No + 1) * (No + 2) // 6
     if args.triples_blocks > 0:
         rng = np.random.default_rng(1)
-        blocks = []
-        while len(blocks) < args.triples_blocks:
-            b = sorted(rng.choice(No, 3, replace=False).tolist())
-            if b not in blocks:
-                blocks.append(b)
-        e.triples(precision="fp64", blocks=[blocks[0]])  # builds the (T) integrals
-        nblk = No * (No + 1) * (No + 2) // 6
+        allb = [(a, b, c) for a in range(No) for b in range(a, No) for c in range(b, No)]
+        pick = rng.choice(len(allb), min(args.triples_blocks, len(allb)), replace=False)
+        blocks = [allb[k] for k in sorted(pick)]
+        e.triples(T1, T2, precision="fp64", blocks=[blocks[0]])  # builds the (T) integrals
         per = {}
-        for tp in ("fp64", "fp32", "bf16x3"):
-            e.triples(precision=tp, blocks=[blocks[0]])
+        for tp in ("fp64", "bf16x3"):
+            e.triples(T1, T2, precision=tp, blocks=[blocks[0]])
             t0 = time.perf_counter()
-            et = e.triples(precision=tp, blocks=blocks)
+            et = e.triples(T1, T2, precision=tp, blocks=blocks)
             per[tp] = ((time.perf_counter() - t0) / len(blocks), et)
-            log("(T) %s %.4fs per block, %.0fs extrapolated" % (tp, per[tp][0], per[tp][0] * nblk))
-        tb = per["fp64"][0]
-        triples = dict(seconds_per_block=tb, blocks=nblk,
-                       seconds_extrapolated=tb * nblk,
-                       tflops=triples_flops(No, Nv, NG) / (tb * nblk) / 1e12,
-                       sample="%d random i<j<k blocks, FP64" % len(blocks),
-                       reduced={tp: dict(seconds_per_block=per[tp][0],
-                                         seconds_extrapolated=per[tp][0] * nblk,
-                                         rel_error_sample=abs(per[tp][1] - per["fp64"][1])
-                                         / abs(per["fp64"][1]))
-                                for tp in ("fp32", "bf16x3")})
+            log("(T) %s %.4fs per block on %d blocks" % (tp, per[tp][0], len(blocks)))
+        triples = dict(sample_blocks=len(blocks),
+                       fp64_seconds_per_block=per["fp64"][0],
+                       bf16x3_seconds_per_block=per["bf16x3"][0],
+                       fp64_tflops=(triples_flops(No, Nv, NG) / nblk) / per["fp64"][0] / 1e12,
+                       sample_energy_fp64=per["fp64"][1],
+                       sample_energy_bf16x3=per["bf16x3"][1],
+                       sample_abs_difference=abs(per["fp64"][1] - per["bf16x3"][1]))
+        if args.triples_full:
+            t0 = time.perf_counter()
+            et = e.triples(T1, T2, precision="bf16x3")
+            tt = time.perf_counter() - t0
+            triples.update(full_pass_bf16x3_seconds=tt, full_pass_blocks=nblk,
+                           full_pass_energy_bf16x3=et,
+                           full_pass_tflops_bf16x3=triples_flops(No, Nv, NG) / tt / 1e12)
+            log("(T) full split-BF16x3 pass %.1fs over %d blocks" % (tt, nblk))
 
     cpu = None
     if not args.no_cpu:
@@ -243,7 +304,7 @@ def main():
         rate = fl / dt
         cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         cpu = dict(value=flops_it / rate, unit="s/iteration (extrapolated)", cores=cores,
-                   kind="port",
+                   kind="port", cpu_model=cpu_model(),
                    sample=("oracle ladder slice pair %dx%d at Nv=%d NG=%d No=%d: %.1f s, "
                            "%.1f GFLOP/s; iteration = algorithmic flops / rate"
                            % (s, s, Nv, NG, No, dt, rate / 1e9)))
@@ -270,10 +331,12 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded Philox vertex, No=%d Nv=%d NG=%d)" % (No, Nv, NG),
         "config": {"workload": "CCSD FP64 iteration, No=%d Nv=%d NG=%d, ladder slices of %d, "
-                               "DIIS(6), MP2 start%s" % (No, Nv, NG, s,
-                                                         ", factor-based ladder NR=%d" % args.factors
-                                                         if args.factors > 0 else ""),
-                   "No": No, "Nv": Nv, "NG": NG, "slice": s, "mixer": "DiisMixer(6)"},
+                               "DIIS(6), MP2 start, levelShift %g%s"
+                               % (No, Nv, NG, s, args.level_shift,
+                                  ", factor-based ladder NR=%d" % args.factors
+                                  if args.factors > 0 else ""),
+                   "No": No, "Nv": Nv, "NG": NG, "slice": s, "mixer": "DiisMixer(6)",
+                   "level_shift": args.level_shift},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
                      "kernel": "gemm_k4_kernel<FP64>", "launches": g_n,
@@ -287,12 +350,30 @@ def main():
             "bandwidth_kernels_seconds_per_iteration": st["other_seconds"] / args.steps,
             "bandwidth_kernels_GBps": other_bw,
             "integrals_seconds": t_int,
-            "energy_after_timed_iterations": rep["energy"],
-            "mixed_bf16x3": mixed,
+            "mp2_energy": e_mp2,
+            "mp2_closed_form": e_mp2_closed,
+            "iteration1_energy": trajectory[0][0],
+            "iteration1_closed_form": e_it1_closed,
+            "trajectory_energy_delta": trajectory,
+            "mixed_bf16x3_bulk": mixed,
+            "refinement": refine,
             "triples": triples,
+            "gemm_variants": {"bf16x3": eng.gemm_variants(3), "bf16": eng.gemm_variants(1)},
+            "tuning_records": [l for l in eng.tune_dump().splitlines() if l and l[0] != "#"],
         },
     }
     print(json.dumps(out), flush=True)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def complex_vertex(g, No, seed):

@@ -224,13 +224,21 @@ int sisi_engine_set_integrals(sisi_engine *e, int which, const double *V);
  * `direct`, exchange 0). */
 int sisi_engine_set_formulation(sisi_engine *e, int formulation);
 
+/* CCSD amplitudes T1[Nv,No], T2[Nv,Nv,No,No] of the EOM right-applies below
+ * (SimilarityTransformedHamiltonian's amplitudes, cxx:64-68): the
+ * similarity-transformed Hamiltonian's blocks are built from them once, on
+ * the first IP / EA right-apply, and kept on the engine until other
+ * amplitudes, integrals or eigenenergies are set. */
+int sisi_eom_set_amplitudes(sisi_engine *e, const double *T1, const double *T2);
+
 /* IP-EOM-CCSD right-apply, the product UCcsdIPEquationOfMotionDavidson
  * (UCcsdIPEquationOfMotionDavidson.cxx:43-205) repeats in its Davidson
  * iterations: SimilarityTransformedHamiltonian::right_apply_CCSD_IP
  * (SimilarityTransformedHamiltonian.cxx:64-68) with CCSD dressing, FP64, on an
  * engine in SISI_FORMULATION_UCCSD (spin orbitals, canonical orbitals).
- * T1[Nv,No], T2[Nv,Nv,No,No]: the CCSD amplitudes; r1[No], r2[No,No,Nv]
- * (antisymmetric in the two holes), first index fastest; s1, s2 alike. */
+ * T1[Nv,No], T2[Nv,Nv,No,No]: the CCSD amplitudes (as sisi_eom_set_amplitudes;
+ * both NULL: the amplitudes set before); r1[No], r2[No,No,Nv] (antisymmetric
+ * in the two holes), first index fastest; s1, s2 alike. */
 int sisi_eom_ip_sigma(sisi_engine *e, const double *T1, const double *T2, const double *r1,
                       const double *r2, double *s1, double *s2);
 
@@ -252,6 +260,22 @@ int sisi_eom_ea_sigma(sisi_engine *e, const double *T1, const double *T2, const 
  * arguments read in DrccdEnergyFromCoulombIntegrals.cxx:41,47,63. */
 int sisi_engine_set_drccd(sisi_engine *e, int unrestricted, int linearized,
                           int adjacent_pairs_exchange);
+
+/* Energy variants of getEnergy (ClusterSinglesDoublesAlgorithm.cxx:131-213)
+ * for the next energies of this engine (driver, sisi_ccsd_energy, pair
+ * energies): unrestricted = 1 sets spins = 1 (cxx:133) in the direct and
+ * exchange terms; antisymmetrize = 1 evaluates 0.25 T_abkl V_klab +
+ * 0.5 T_aj T_cl V_jlac (cxx:163-167) and its pair energies (cxx:222-226),
+ * real amplitudes only.  Replaces the `unrestricted` and `antisymmetrize`
+ * arguments of the CCSD plugins. */
+int sisi_engine_set_energy_variant(sisi_engine *e, int unrestricted, int antisymmetrize);
+
+/* HPFockMatrix f[i,a] ([No, Nv], real vertex; NULL clears):
+ * every energy adds the non-canonical term spins * sum_ai T_ai f_ia
+ * (ClusterSinglesDoublesAlgorithm.cxx:199-211); its last value is returned
+ * by sisi_engine_get_noncanonical_energy. */
+int sisi_engine_set_hp_fock(sisi_engine *e, const double *fia);
+int sisi_engine_get_noncanonical_energy(sisi_engine *e, double *out);
 
 /* Residual variant of the next residual calls: the ppl and distinguishable
  * arguments of CcsdEnergyFromCoulombIntegrals (cxx:123-124, 324-338);
@@ -374,6 +398,33 @@ int sisi_generate_ueg(int No, int Nv, double rs, int hartree_fock, double madelu
  * environment variable SISI_B3_BACKEND=library sets 1 at start-up.  No
  * counterpart in the reference. */
 int sisi_set_bf16_backend(int library);
+
+/* Kernel variant of the split-BF16x3 (passes = 3) or bf16 (passes = 1)
+ * products for the whole process: 0 .. sisi_gemm_variant_count(passes)-1
+ * forces one, -1 returns to the automatic choice (tuning cache, else a size
+ * rule).  The environment variables SISI_B3_VARIANT / SISI_B1_VARIANT set it
+ * at start-up.  No counterpart in the reference (CTF picks BLAS kernels). */
+int sisi_gemm_set_variant(int passes, int variant);
+int sisi_gemm_variant_count(int passes);
+const char *sisi_gemm_variant_name(int passes, int variant);
+
+/* Autotuner of the GEMM kernel variants: with tuning on (or SISI_TUNE=1),
+ * the first product of each (family, log2 M, log2 N, log2 K) class times
+ * every variant on the device and records the fastest in the process and
+ * in the cache file (SISI_TUNE_CACHE, else tuning/gfx950.txt beside the
+ * library); recorded classes are never timed again.  sisi_tune_dump writes
+ * the records as text (needed = bytes including the terminating 0).  No
+ * counterpart in the reference. */
+int sisi_tune_enable(int on);
+int sisi_tune_dump(char *buf, int len, int *needed);
+
+/* Timing of one GEMM shape on device-resident operands (A m x k and B k x n,
+ * both k-contiguous, C column-major): average microseconds per call of the
+ * GEMM kernels (HIP events around each launch) and of the whole call
+ * (operand splits included), over `reps` calls after one warm-up.  No
+ * counterpart in the reference. */
+int sisi_gemm_bench(int precision, int M, int N, int K, int reps, double *kernel_us,
+                    double *total_us);
 
 /* One MFMA GEMM C = alpha*op(A)*op(B) + beta*C on host arrays (column-major,
  * op = transpose when trans* != 0), in the given precision.  No counterpart

@@ -258,7 +258,36 @@ def energy(T1, T2, Vabij):
     return direct + exchange, direct, exchange
 
 
-def pair_energy(T1, T2, Vabij):
+def energy_of(T1, T2, Vabij, spins=2.0, antisymmetrized=False, fia=None):
+    """getEnergy with its arguments (ClusterSinglesDoublesAlgorithm.cxx:
+    131-213): spins = 1 for `unrestricted` (cxx:133) in the direct and
+    exchange terms (cxx:170-178); `antisymmetrize` evaluates
+    0.25 T_abkl V_klab + 0.5 T_aj T_cl V_jlac (cxx:163-167) on
+    V_ijab = V_abij (cxx:141-152); an HPFockMatrix f[i,a] adds
+    spins * T_ai f_ia (cxx:199-211)."""
+    Vijab = ein("abij->ijab", Vabij)
+    if antisymmetrized:
+        e = 0.25 * ein("abkl,klab->", T2, Vijab) + 0.5 * ein("aj,cl,jlac->", T1, T1, Vijab)
+    else:
+        dire = 0.5 * spins * spins * (ein("abij,ijab->", T2, Vijab)
+                                      + ein("ai,bj,ijab->", T1, T1, Vijab))
+        exce = -0.5 * spins * (ein("abij,ijba->", T2, Vijab) + ein("ai,bj,ijba->", T1, T1, Vijab))
+        e = dire + exce
+    if fia is not None:
+        e += spins * ein("ai,ia->", T1, fia)
+    return float(e)
+
+
+def pair_energy(T1, T2, Vabij, antisymmetrized=False):
+    """PairEnergy (cxx:218-235)."""
+    if antisymmetrized:
+        Vijab = ein("abij->ijab", Vabij)
+        return (0.25 * ein("abij,ijab->ij", T2, Vijab)
+                + 0.5 * ein("ai,bj,ijab->ij", T1, T1, Vijab))
+    return pair_energy_restricted(T1, T2, Vabij)
+
+
+def pair_energy_restricted(T1, T2, Vabij):
     """PairEnergy (cxx:218-235), non-antisymmetrized branch."""
     tau = T2 + ein("ai,bj->abij", T1, T1)
     return 2.0 * ein("abij,abij->ij", tau, Vabij) - ein("abij,baij->ij", tau, Vabij)
@@ -340,7 +369,8 @@ class LinearMixer:
 def solve(gamma, eps_i, eps_a, *, mixer="DiisMixer", max_residua=4,
           mixing_ratio=1.0, max_iterations=16, amplitudes_convergence=1e-5,
           energy_convergence=1e-6, level_shift=0.0, slice_size=None,
-          initial=None, callback=None, ppl=True, distinguishable=False, factors=None):
+          initial=None, callback=None, ppl=True, distinguishable=False, factors=None,
+          spins=2.0, antisymmetrized=False, fia=None):
     """ClusterSinglesDoublesAlgorithm::run<double> (cxx:38-128).
 
     Returns dict(energy, T1, T2, energies=[per iteration], iterations,
@@ -370,7 +400,7 @@ def solve(gamma, eps_i, eps_a, *, mixer="DiisMixer", max_residua=4,
         dT = [E1 - T[0], E2 - T[1]]
         mix.append([E1, E2], dT)
         T = mix.get()
-        e = energy(T[0], T[1], ints["PPHH"])[0]
+        e = energy_of(T[0], T[1], ints["PPHH"], spins, antisymmetrized, fia)
         ratio = fock_dot(dT, dT) / fock_dot(T, T)
         energies.append(e)
         deltas.append(np.sqrt(abs(ratio)))
@@ -384,7 +414,7 @@ def solve(gamma, eps_i, eps_a, *, mixer="DiisMixer", max_residua=4,
         prev = e
         i += 1
     if max_iterations == 0:
-        e = energy(T[0], T[1], ints["PPHH"])[0]
+        e = energy_of(T[0], T[1], ints["PPHH"], spins, antisymmetrized, fia)
     return dict(energy=e, T1=T[0], T2=T[1], energies=energies, deltas=deltas,
                 iterations=i, converged=converged, integrals=ints)
 

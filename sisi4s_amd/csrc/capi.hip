@@ -2,6 +2,7 @@
 // host <-> device staging.  See include/sisi_engine.h for the contract.
 #include <atomic>
 #include <cmath>
+#include <cstring>
 #include <string>
 #include <thread>
 #include <vector>
@@ -9,6 +10,8 @@
 #include "ccsd.hpp"
 #include "contract.hpp"
 #include "engine.hpp"
+#include "gemm.hpp"
+#include "tune.hpp"
 #include "tensor_ops.hpp"
 #include "triples.hpp"
 
@@ -142,6 +145,7 @@ int sisi_engine_set_eigenenergies(sisi_engine *h, const double *eps_i, const dou
     e.upload(e.eps_i, eps_i, size_t(e.No));
     e.upload(e.eps_a, eps_a, size_t(e.Nv));
     e.have_eps = true;
+    e.drop_eom();  // F_mi and F_ae hold the eigenenergies
   });
 }
 
@@ -227,6 +231,38 @@ int sisi_set_bf16_backend(int library) {
   });
 }
 
+int sisi_gemm_set_variant(int passes, int variant) {
+  return guarded([&] {
+    SISI_REQUIRE(passes == 1 || passes == 3, SISI_ERR_INVALID, "passes must be 1 or 3");
+    set_b3_variant(passes, variant);
+  });
+}
+
+int sisi_gemm_variant_count(int passes) {
+  return passes == 1 || passes == 3 ? b3_variant_count(passes) : 0;
+}
+
+const char *sisi_gemm_variant_name(int passes, int variant) {
+  return passes == 1 || passes == 3 ? b3_variant_name(passes, variant) : "";
+}
+
+int sisi_tune_enable(int on) {
+  return guarded([&] { tune_set_enabled(on != 0); });
+}
+
+int sisi_tune_dump(char *buf, int len, int *needed) {
+  return guarded([&] {
+    std::string all = "# cache: " + tune_cache_path() + "\n";
+    for (const auto &l : tune_dump()) all += l + "\n";
+    if (needed) *needed = int(all.size()) + 1;
+    if (buf && len > 0) {
+      const size_t n = std::min<size_t>(all.size(), size_t(len - 1));
+      std::memcpy(buf, all.data(), n);
+      buf[n] = 0;
+    }
+  });
+}
+
 int sisi_engine_set_integrals(sisi_engine *h, int which, const double *V) {
   return guarded([&] {
     ENGINE_REQUIRED(h);
@@ -248,20 +284,30 @@ int sisi_engine_set_formulation(sisi_engine *h, int formulation) {
   });
 }
 
+int sisi_eom_set_amplitudes(sisi_engine *h, const double *T1, const double *T2) {
+  return guarded([&] {
+    ENGINE_REQUIRED(h);
+    SISI_REQUIRE(T1 && T2, SISI_ERR_INVALID, "null amplitudes");
+    SISI_REQUIRE(!h->e.cplx, SISI_ERR_INVALID, "the EOM right-applies are built for real amplitudes");
+    eom_set_amplitudes(h->e, T1, T2);
+  });
+}
+
 int sisi_eom_ip_sigma(sisi_engine *h, const double *T1, const double *T2, const double *r1,
                       const double *r2, double *s1, double *s2) {
   return guarded([&] {
     ENGINE_REQUIRED(h);
     Engine &e = h->e;
-    SISI_REQUIRE(T1 && T2 && r1 && r2 && s1 && s2, SISI_ERR_INVALID, "null argument");
+    SISI_REQUIRE(r1 && r2 && s1 && s2, SISI_ERR_INVALID, "null argument");
+    SISI_REQUIRE((T1 == nullptr) == (T2 == nullptr), SISI_ERR_INVALID,
+                 "T1 and T2 are given together (or both null: the amplitudes set)");
     SISI_REQUIRE(!e.cplx, SISI_ERR_INVALID, "the IP right-apply is built for real amplitudes");
     const size_t o = size_t(e.No), v = size_t(e.Nv);
-    DBuf t1, t2, q1, q2, w1(&e.pool, o), w2(&e.pool, o * o * v);
-    e.upload(t1, T1, v * o);
-    e.upload(t2, T2, v * v * o * o);
+    if (T1) eom_set_amplitudes(e, T1, T2);
+    DBuf q1, q2, w1(&e.pool, o), w2(&e.pool, o * o * v);
     e.upload(q1, r1, o);
     e.upload(q2, r2, o * o * v);
-    eom_ip_sigma(e, t1.get(), t2.get(), q1.get(), q2.get(), w1.get(), w2.get());
+    eom_ip_sigma(e, q1.get(), q2.get(), w1.get(), w2.get());
     e.download(s1, w1, o);
     e.download(s2, w2, o * o * v);
   });
@@ -272,15 +318,16 @@ int sisi_eom_ea_sigma(sisi_engine *h, const double *T1, const double *T2, const 
   return guarded([&] {
     ENGINE_REQUIRED(h);
     Engine &e = h->e;
-    SISI_REQUIRE(T1 && T2 && r1 && r2 && s1 && s2, SISI_ERR_INVALID, "null argument");
+    SISI_REQUIRE(r1 && r2 && s1 && s2, SISI_ERR_INVALID, "null argument");
+    SISI_REQUIRE((T1 == nullptr) == (T2 == nullptr), SISI_ERR_INVALID,
+                 "T1 and T2 are given together (or both null: the amplitudes set)");
     SISI_REQUIRE(!e.cplx, SISI_ERR_INVALID, "the EA right-apply is built for real amplitudes");
     const size_t o = size_t(e.No), v = size_t(e.Nv);
-    DBuf t1, t2, q1, q2, w1(&e.pool, v), w2(&e.pool, o * v * v);
-    e.upload(t1, T1, v * o);
-    e.upload(t2, T2, v * v * o * o);
+    if (T1) eom_set_amplitudes(e, T1, T2);
+    DBuf q1, q2, w1(&e.pool, v), w2(&e.pool, o * v * v);
     e.upload(q1, r1, v);
     e.upload(q2, r2, o * v * v);
-    eom_ea_sigma(e, t1.get(), t2.get(), q1.get(), q2.get(), w1.get(), w2.get());
+    eom_ea_sigma(e, q1.get(), q2.get(), w1.get(), w2.get());
     e.download(s1, w1, v);
     e.download(s2, w2, o * v * v);
   });
@@ -291,6 +338,44 @@ int sisi_engine_set_drccd(sisi_engine *h, int unrestricted, int linearized,
   return guarded([&] {
     ENGINE_REQUIRED(h);
     set_drccd(h->e, unrestricted, linearized, adjacent_pairs_exchange);
+  });
+}
+
+int sisi_engine_set_energy_variant(sisi_engine *h, int unrestricted, int antisymmetrize) {
+  return guarded([&] {
+    ENGINE_REQUIRED(h);
+    h->e.spins = unrestricted ? 1.0 : 2.0;  // ClusterSinglesDoublesAlgorithm.cxx:133
+    h->e.anti_energy = antisymmetrize != 0;  // cxx:134, 163-167
+  });
+}
+
+int sisi_engine_set_hp_fock(sisi_engine *h, const double *fia) {
+  return guarded([&] {
+    ENGINE_REQUIRED(h);
+    Engine &e = h->e;
+    if (fia == nullptr) {
+      e.fai = DBuf();
+      e.noncanonical = 0.0;
+      return;
+    }
+    SISI_REQUIRE(!e.cplx, SISI_ERR_INVALID, "HPFockMatrix: built for a real vertex");
+    const int64_t o = e.No, v = e.Nv, w = 1;
+    check_finite(fia, size_t(w * o * v), "HPFockMatrix");
+    // f[i,a] at i + No a (complex128 interleaved in complex mode) -> f[a,i]
+    std::vector<double> t(size_t(w * o * v));
+    for (int64_t i = 0; i < o; ++i)
+      for (int64_t a = 0; a < v; ++a)
+        for (int64_t c = 0; c < w; ++c) t[size_t(w * (a + v * i) + c)] = fia[w * (i + o * a) + c];
+    e.upload_t(e.fai, t.data(), t.size());
+    e.fai_cplx = e.cplx;
+  });
+}
+
+int sisi_engine_get_noncanonical_energy(sisi_engine *h, double *out) {
+  return guarded([&] {
+    ENGINE_REQUIRED(h);
+    SISI_REQUIRE(out != nullptr, SISI_ERR_INVALID, "null output");
+    *out = h->e.noncanonical;
   });
 }
 
@@ -639,6 +724,55 @@ int sisi_gemm_host(int precision, int transA, int transB, int M, int N, int K, d
     g.beta = beta;
     gemm(e, precision, g, true);  // the precision asked for, whatever the size
     if (nc > 0) e.download(C, c, size_t(nc));
+  });
+}
+
+int sisi_gemm_bench(int precision, int M, int N, int K, int reps, double *kernel_us,
+                    double *total_us) {
+  return guarded([&] {
+    SISI_REQUIRE(M > 0 && N > 0 && K > 0 && reps > 0, SISI_ERR_INVALID, "bad bench shape");
+    SISI_REQUIRE(is_precision(precision), SISI_ERR_INVALID, "unknown precision");
+    sisi_dims d{1, 1, 1};
+    Engine e(d);
+    DBuf a(&e.pool, size_t(M) * K), b(&e.pool, size_t(K) * N), c(&e.pool, size_t(M) * N);
+    fill_pattern(e, a.get(), int64_t(M) * K, 1);
+    fill_pattern(e, b.get(), int64_t(K) * N, 2);
+    GemmArgs g;
+    g.A = a.get();
+    g.B = b.get();
+    g.C = c.get();
+    g.M = M;
+    g.N = N;
+    g.K = K;
+    g.sam = K;  // A(m,k) k-contiguous, B(k,n) k-contiguous: the planner's usual layouts
+    g.sak = 1;
+    g.sbk = 1;
+    g.sbn = K;
+    g.cm0 = 1;
+    g.cn0 = M;
+    g.dm0 = M;
+    g.dn0 = N;
+    gemm(e, precision, g, true);  // warm-up (and tuning, when enabled)
+    e.sync();
+    e.timers->enabled = true;
+    e.timers->reset();
+    hipEvent_t t0, t1;
+    SISI_HIP(hipEventCreate(&t0));
+    SISI_HIP(hipEventCreate(&t1));
+    SISI_HIP(hipEventRecord(t0, e.stream));
+    for (int r = 0; r < reps; ++r) gemm(e, precision, g, true);
+    SISI_HIP(hipEventRecord(t1, e.stream));
+    SISI_HIP(hipEventSynchronize(t1));
+    float ms = 0.f;
+    SISI_HIP(hipEventElapsedTime(&ms, t0, t1));
+    SISI_HIP(hipEventDestroy(t0));
+    SISI_HIP(hipEventDestroy(t1));
+    sisi_stats st{};
+    e.timers->collect(st);
+    e.timers->enabled = false;
+    const int kind = precision == SISI_FP64 ? 0 : precision == SISI_FP32 ? 1 : 2;
+    if (kernel_us) *kernel_us = 1e6 * st.gemm_seconds[kind] / reps;
+    if (total_us) *total_us = 1e3 * double(ms) / reps;
   });
 }
 

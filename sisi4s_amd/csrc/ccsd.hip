@@ -661,13 +661,25 @@ void ccsd_residual_anchor(Engine &e, int slice_size, const double *T1, const dou
   an.distinguishable = e.distinguishable;
 }
 
+// The HPFockMatrix term spins * sum_ai T_ai f_ia (cxx:199-211; uploaded for
+// a real vertex only).
+static double noncanonical_term(Engine &e, const double *T1) {
+  if (!e.fai.get()) return 0.0;
+  SISI_REQUIRE(e.fai_cplx == e.cplx, SISI_ERR_STATE,
+               "HPFockMatrix was set for the other (real / complex) mode of the vertex");
+  const int64_t n1 = int64_t(e.Nv) * e.No;
+  return e.spins * dot(e, n1, T1, e.fai.get());
+}
+
 void ccsd_energy(Engine &e, const double *T1, const double *T2, double &E, double &direct,
                  double &exchange, double *E_imag) {
   SISI_REQUIRE(integrals_ready(e), SISI_ERR_STATE, "energy: integrals not built");
   if (e.cplx) {  // F = complex: the real parts are reported (cxx:30-33, 185-189)
+    SISI_REQUIRE(!e.anti_energy, SISI_ERR_INVALID,
+                 "energy: the antisymmetrized energy is built for real amplitudes");
     double Ez[2], dz[2], xz[2];
     zccsd_energy(e, T1, T2, Ez, dz, xz);
-    // spins (direct-ring CCD's unrestricted) as in the real branch
+    // spins (unrestricted) as in the real branch
     for (int k = 0; k < 2; ++k) {
       dz[k] *= 0.25 * e.spins * e.spins;
       xz[k] *= 0.5 * e.spins;
@@ -677,33 +689,61 @@ void ccsd_energy(Engine &e, const double *T1, const double *T2, double &E, doubl
     direct = dz[0];
     exchange = xz[0];
     if (E_imag) *E_imag = Ez[1];
+    e.noncanonical = noncanonical_term(e, T1);
+    E += e.noncanonical;
     return;
   }
   if (e.formulation == SISI_FORMULATION_UCCSD) {  // antisymmetrized (cxx:163-167)
     E = direct = uccsd_energy(e, T1, T2);
     exchange = 0.0;
     if (E_imag) *E_imag = 0.0;
+    e.noncanonical = 0.0;
     return;
   }
-  energy_terms(e, T1, T2, direct, exchange);
-  // direct 1/2 spins^2 tau V_ijab, exchange -1/2 spins tau V_ijba
-  // (ClusterSinglesDoublesAlgorithm.cxx:163-170); energy_terms uses spins = 2
-  direct *= 0.25 * e.spins * e.spins;
-  exchange *= 0.5 * e.spins;
-  E = direct + exchange;
+  if (e.anti_energy) {
+    // 0.25 T_abkl V_klab + 0.5 T_aj T_cl V_jlac (cxx:163-167) with V_ijab =
+    // V_abij: 0.25 sum_abij (T2 + 2 T1 T1)_abij V_abij
+    const int64_t n2 = e.n2();
+    DBuf X(&e.pool, n2);
+    add_t1t1(e, X.get(), T2, T1, 2.0, e.Nv, e.No);
+    E = direct = 0.25 * dot(e, n2, X.get(), e.Vabij.get());
+    exchange = 0.0;
+  } else {
+    energy_terms(e, T1, T2, direct, exchange);
+    // direct 1/2 spins^2 tau V_ijab, exchange -1/2 spins tau V_ijba
+    // (ClusterSinglesDoublesAlgorithm.cxx:163-170); energy_terms uses spins = 2
+    direct *= 0.25 * e.spins * e.spins;
+    exchange *= 0.5 * e.spins;
+    E = direct + exchange;
+  }
   if (E_imag) *E_imag = 0.0;
+  e.noncanonical = noncanonical_term(e, T1);
+  E += e.noncanonical;
 }
 
 void ccsd_pair_energies(Engine &e, const double *T1, const double *T2, double *pair) {
   SISI_REQUIRE(integrals_ready(e), SISI_ERR_STATE, "pair energies: integrals not built");
-  SISI_REQUIRE(e.spins == 2.0 && e.formulation != SISI_FORMULATION_UCCSD, SISI_ERR_INVALID,
-               "pair energies are built for the restricted closed-shell energy");
-  if (e.cplx) return zccsd_pair_energies(e, T1, T2, pair);
-  SISI_REQUIRE(e.Uabij.get(), SISI_ERR_STATE, "pair energies: integrals not built");
+  SISI_REQUIRE(e.formulation != SISI_FORMULATION_UCCSD, SISI_ERR_INVALID,
+               "pair energies are built for the vertex and full-integral CCSD energies");
+  if (e.cplx) {
+    SISI_REQUIRE(!e.anti_energy, SISI_ERR_INVALID,
+                 "pair energies: the antisymmetrized form is built for real amplitudes");
+    return zccsd_pair_energies(e, T1, T2, pair);
+  }
   const int64_t o = e.No, v = e.Nv;
   DBuf Xb(&e.pool, v * v * o * o);
+  if (e.anti_energy) {
+    // 0.25 T_abij V_ijab + 0.5 T_ai T_bj V_ijab (cxx:222-226)
+    SISI_REQUIRE(e.Vabij.get(), SISI_ERR_STATE, "pair energies: integrals not built");
+    add_t1t1(e, Xb.get(), T2, T1, 2.0, int(v), int(o));
+    ctr(e, 0.25, "abij,abij->ij", V4(Xb.get(), v, v, o, o), V4(e.Vabij.get(), v, v, o, o), 0.0,
+        V2(pair, o, o), SISI_FP64);
+    return;
+  }
+  SISI_REQUIRE(e.Uabij.get(), SISI_ERR_STATE, "pair energies: integrals not built");
   add_t1t1(e, Xb.get(), T2, T1, 1.0, int(v), int(o));  // tau = T2 + T1 T1
   // the four terms of cxx:226-231 are tau_abij (2 V_abij - V_baij) = tau U
+  // (fixed factors: the reference's pair energies do not read `unrestricted`)
   ctr(e, 1.0, "abij,abij->ij", V4(Xb.get(), v, v, o, o), V4(e.Uabij.get(), v, v, o, o), 0.0,
       V2(pair, o, o), SISI_FP64);
 }

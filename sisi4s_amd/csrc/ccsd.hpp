@@ -72,14 +72,18 @@ bool ea_eom_ready(const Engine &e);
 void uccsd_residual(Engine &e, int iteration, bool initial_doubles, const double *T1,
                     const double *T2, double *R1, double *R2);
 double uccsd_energy(Engine &e, const double *T1, const double *T2);
+// Amplitudes (host T1[a,i], T2[a,b,i,j]) of the similarity-transformed
+// Hamiltonian of the EOM right-applies; the same amplitudes again keep the
+// blocks already built, others drop them.
+void eom_set_amplitudes(Engine &e, const double *T1, const double *T2);
 // IP-EOM-CCSD right-apply (SimilarityTransformedHamiltonian.cxx:64-68): s =
-// Hbar r on the 1h + 2h1p space, r1[i], r2[i,j,a], spin-orbital engine
-void eom_ip_sigma(Engine &e, const double *T1, const double *T2, const double *r1,
-                  const double *r2, double *s1, double *s2);
+// Hbar r on the 1h + 2h1p space, r1[i], r2[i,j,a] (device), spin-orbital
+// engine; builds the IP blocks on first use
+void eom_ip_sigma(Engine &e, const double *r1, const double *r2, double *s1, double *s2);
 // EA-EOM-CCSD right-apply (SimilarityTransformedHamiltonian.cxx:289-294): s =
-// Hbar r on the 1p + 2p1h space, r1[a], r2[j,a,b], spin-orbital engine
-void eom_ea_sigma(Engine &e, const double *T1, const double *T2, const double *r1,
-                  const double *r2, double *s1, double *s2);
+// Hbar r on the 1p + 2p1h space, r1[a], r2[j,a,b] (device), spin-orbital
+// engine; builds the EA blocks on first use
+void eom_ea_sigma(Engine &e, const double *r1, const double *r2, double *s1, double *s2);
 void build_vabcd(Engine &e);
 void ccsd_residual_full(Engine &e, int iteration, bool initial_doubles, const double *T1,
                         const double *T2, double *R1, double *R2);

@@ -160,6 +160,14 @@ class Engine {
   // `unrestricted`, the linearized and adjacentPairsExchange arguments, and
   // uploaded PHHP / HHPP integrals (derived from PPHH when absent)
   double spins = 2.0;
+  // getEnergy variants (ClusterSinglesDoublesAlgorithm.cxx:131-213): the
+  // antisymmetrized energy (antisymmetrize: 1) and the HPFockMatrix term
+  // spins * sum_ai T_ai f_ia, f stored transposed as fai[a,i] (planar
+  // complex in complex mode); `noncanonical` is its last value
+  bool anti_energy = false;
+  DBuf fai;
+  bool fai_cplx = false;
+  double noncanonical = 0.0;
   bool ring_linearized = false, ring_adjacent = false;
   DBuf ring_Vaijb, ring_Vijab;
   int NR = 0, factors_slice = 0;
@@ -173,7 +181,18 @@ class Engine {
   // the same operands as split-BF16x3 planes (V_jklc with the sign of its
   // term folded in), built on the first reduced-precision (T) call
   B3Planes Vt_b3, Vt2_b3, VP_b3;
+  // EOM-CCSD: the amplitudes the similarity-transformed Hamiltonian is built
+  // from and its blocks, built once per amplitude set (IP and EA blocks on
+  // their first right-apply) and dropped with the integrals
+  struct EomHbar {
+    bool have_t = false, ip = false, ea = false;
+    std::vector<double> hT1, hT2;  // host copies of the amplitudes set
+    DBuf T1, T2, Fme, Foo, Fvv, Woooo, Wovvo, Wooov, Wovoo, Wvvvv, Wvovv, Wvvvo;
+  } eom;
+  void drop_eom() { eom = EomHbar(); }
+
   void drop_triples_cache() {
+    drop_eom();
     Vt.reset();
     Vt2.reset();
     VP.reset();

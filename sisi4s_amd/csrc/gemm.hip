@@ -473,12 +473,11 @@ static void launch_b3(GemmArgs &g, dim3 grid, hipStream_t s, bool AK, bool BNF) 
   else hipLaunchKernelGGL((gemm_bf16x3_kernel<true, true>), grid, dim3(256), 0, s, g);
 }
 
+static void launch_reduce(const GemmArgs &g, hipStream_t s);
+
 void gemm_reduce(Engine &eng, const GemmArgs &g) {
   Timed t(*eng.timers, Kind::Bandwidth, 8.0 * (g.ksplit + 2) * double(g.M) * g.N * g.batch);
-  const int64_t MN = int64_t(g.M) * g.N;
-  int blocks = int(std::min<int64_t>((MN + 63) / 64, 8192));
-  hipLaunchKernelGGL(gemm_reduce_kernel, dim3(blocks, g.batch), dim3(256), 0, eng.stream, g);
-  SISI_HIP(hipGetLastError());
+  launch_reduce(g, eng.stream);
 }
 
 int round_filling_split(int64_t groups, int64_t s, int64_t smax, int cus) {
@@ -503,21 +502,11 @@ static int64_t env_int(const char *name, int64_t fallback) {
   return v != nullptr && *v != 0 ? int64_t(atoll(v)) : fallback;
 }
 
-void gemm(Engine &eng, int precision, GemmArgs g, bool exact) {
-  if (g.M <= 0 || g.N <= 0 || g.batch <= 0) return;
-  // split-BF16x3 splits both operands first (a bandwidth pass over each); a
-  // product too small to amortise that runs on FP32 MFMA instead
-  if ((precision == SISI_BF16X3 || precision == SISI_BF16) && !exact &&
-      gemm_flops(g) < kB3MinFlops)
-    precision = SISI_FP32;
-  if ((precision == SISI_BF16X3 || precision == SISI_BF16) && g.batch == 1 && g.K > 0) {
-    static const int env_b3 = getenv("SISI_B3_PRESPLIT") ? atoi(getenv("SISI_B3_PRESPLIT")) : 1;
-    if (env_b3) {
-      gemm_b3(eng, g, precision == SISI_BF16 ? 1 : 3);
-      return;
-    }
-  }
-  if (precision == SISI_BF16) precision = SISI_BF16X3;  // batched: the three-product kernel
+static void set_ksplit(GemmArgs &g, int ksplit, int kq);
+
+// Tile grid, load mappings and split-K plan of one FP64 / FP32 / batched
+// split-BF16x3 product (fields of g filled in); returns the load mappings.
+static void plan_k4(GemmArgs &g, int precision, int cus, bool &AKf, bool &BNFf) {
   if (g.K < 0) g.K = 0;  // K == 0 gives C = beta * C through the epilogue
   SISI_REQUIRE(g.sam == 1 || g.sak == 1 || g.K <= 1 || g.M <= 1, SISI_ERR_INVALID,
                "gemm: A needs a unit stride");
@@ -525,8 +514,8 @@ void gemm(Engine &eng, int precision, GemmArgs g, bool exact) {
                "gemm: B needs a unit stride");
   // The address formula is the same for every load mapping; the mapping only
   // decides which index consecutive threads walk (coalescing).
-  const bool AKf = (g.sak == 1) && (g.sam != 1);
-  const bool BNFf = (g.sbn == 1) && (g.sbk != 1);
+  AKf = (g.sak == 1) && (g.sam != 1);
+  BNFf = (g.sbn == 1) && (g.sbk != 1);
   g.tiles_m = (g.M + BM - 1) / BM;
   g.tiles_n = (g.N + BN - 1) / BN;
   const int64_t tiles = int64_t(g.tiles_m) * g.tiles_n;
@@ -550,8 +539,13 @@ void gemm(Engine &eng, int precision, GemmArgs g, bool exact) {
     const int64_t tb = tiles * g.batch;
     int64_t s = (want + tb - 1) / tb;
     const int64_t smax = std::min<int64_t>(std::max<int64_t>(1, g.K / (minsteps * kq)), maxsplit);
-    ksplit = round_filling_split(tb, std::min(s, smax), smax, eng.cus);
+    ksplit = round_filling_split(tb, std::min(s, smax), smax, cus);
   }
+  set_ksplit(g, ksplit, kq);
+}
+
+// Chunking of K for a split of (at most) ksplit partial sums.
+static void set_ksplit(GemmArgs &g, int ksplit, int kq) {
   int kchunk = (g.K + ksplit - 1) / ksplit;
   kchunk = ((kchunk + kq - 1) / kq) * kq;
   if (kchunk == 0) kchunk = kq;
@@ -559,23 +553,77 @@ void gemm(Engine &eng, int precision, GemmArgs g, bool exact) {
   if (ksplit < 1) ksplit = 1;
   g.ksplit = ksplit;
   g.kchunk = kchunk;
+}
+
+// Launch of a planned product; the split-K partial sums go to g.work.
+static void launch_planned(GemmArgs &g, int precision, bool AKf, bool BNFf, hipStream_t s) {
+  SISI_REQUIRE(g.batch <= 65535 && g.ksplit <= 65535, SISI_ERR_INVALID, "gemm: grid too large");
+  dim3 grid(unsigned(int64_t(g.tiles_m) * g.tiles_n), unsigned(g.ksplit), unsigned(g.batch));
+  if (precision == SISI_FP64) launch_k4<SISI_FP64>(g, grid, s, AKf, BNFf);
+  else if (precision == SISI_FP32) launch_k4<SISI_FP32>(g, grid, s, AKf, BNFf);
+  else launch_b3(g, grid, s, AKf, BNFf);
+  SISI_HIP(hipGetLastError());
+}
+
+static void launch_reduce(const GemmArgs &g, hipStream_t s) {
+  const int64_t MN = int64_t(g.M) * g.N;
+  int blocks = int(std::min<int64_t>((MN + 63) / 64, 8192));
+  hipLaunchKernelGGL(gemm_reduce_kernel, dim3(blocks, g.batch), dim3(256), 0, s, g);
+  SISI_HIP(hipGetLastError());
+}
+
+void gemm(Engine &eng, int precision, GemmArgs g, bool exact) {
+  if (g.M <= 0 || g.N <= 0 || g.batch <= 0) return;
+  // split-BF16x3 splits both operands first (a bandwidth pass over each); a
+  // product too small to amortise that runs on FP32 MFMA instead
+  if ((precision == SISI_BF16X3 || precision == SISI_BF16) && !exact &&
+      gemm_flops(g) < kB3MinFlops)
+    precision = SISI_FP32;
+  if ((precision == SISI_BF16X3 || precision == SISI_BF16) && g.batch == 1 && g.K > 0) {
+    static const int env_b3 = getenv("SISI_B3_PRESPLIT") ? atoi(getenv("SISI_B3_PRESPLIT")) : 1;
+    if (env_b3) {
+      gemm_b3(eng, g, precision == SISI_BF16 ? 1 : 3);
+      return;
+    }
+  }
+  if (precision == SISI_BF16) precision = SISI_BF16X3;  // batched: the three-product kernel
+  bool AKf, BNFf;
+  plan_k4(g, precision, eng.cus, AKf, BNFf);
   DBuf work;
-  if (ksplit > 1) {
-    work = DBuf(&eng.pool, size_t(ksplit) * g.batch * g.M * g.N);
+  if (g.ksplit > 1) {
+    work = DBuf(&eng.pool, size_t(g.ksplit) * g.batch * g.M * g.N);
     g.work = work.get();
   }
-  SISI_REQUIRE(g.batch <= 65535 && ksplit <= 65535, SISI_ERR_INVALID, "gemm: grid too large");
-  dim3 grid(unsigned(tiles), unsigned(ksplit), unsigned(g.batch));
   const Kind kind = precision == SISI_FP64 ? Kind::Gemm64
                     : precision == SISI_FP32 ? Kind::Gemm32 : Kind::GemmB3;
   {
     Timed t(*eng.timers, kind, gemm_flops(g));
-    if (precision == SISI_FP64) launch_k4<SISI_FP64>(g, grid, eng.stream, AKf, BNFf);
-    else if (precision == SISI_FP32) launch_k4<SISI_FP32>(g, grid, eng.stream, AKf, BNFf);
-    else launch_b3(g, grid, eng.stream, AKf, BNFf);
-    SISI_HIP(hipGetLastError());
+    launch_planned(g, precision, AKf, BNFf, eng.stream);
   }
-  if (ksplit > 1) gemm_reduce(eng, g);
+  if (g.ksplit > 1) gemm_reduce(eng, g);
+}
+
+void gemm_workspace(hipStream_t stream, int cus, int precision, GemmArgs g, double *work,
+                    int64_t work_doubles, int ksplit) {
+  SISI_REQUIRE(precision == SISI_FP64 || precision == SISI_FP32, SISI_ERR_INVALID,
+               "gemm_workspace: FP64 or FP32 only");
+  if (g.M <= 0 || g.N <= 0 || g.batch <= 0) return;
+  SISI_REQUIRE(ksplit >= 0, SISI_ERR_INVALID, "gemm_workspace: negative split");
+  bool AKf, BNFf;
+  plan_k4(g, precision, cus, AKf, BNFf);
+  const int64_t per = int64_t(g.batch) * g.M * g.N;  // doubles per partial sum
+  if (ksplit > 0) {
+    set_ksplit(g, ksplit, 16);
+    SISI_REQUIRE(g.ksplit == 1 || int64_t(g.ksplit) * per <= work_doubles, SISI_ERR_INVALID,
+                 "gemm_workspace: " + std::to_string(g.ksplit) + " partial sums of " +
+                     std::to_string(per) + " doubles exceed the workspace of " +
+                     std::to_string(work_doubles) + " doubles");
+  } else if (g.ksplit > 1 && int64_t(g.ksplit) * per > work_doubles) {
+    set_ksplit(g, int(std::max<int64_t>(1, work_doubles / per)), 16);  // fewer partial sums
+  }
+  if (g.ksplit > 1) g.work = work;
+  launch_planned(g, precision, AKf, BNFf, stream);
+  if (g.ksplit > 1) launch_reduce(g, stream);
 }
 
 }  // namespace sisi

@@ -102,6 +102,13 @@ void gemm_b3_segments(Engine &e, B3SegArgs a);
 // Products too small for the bf16 split run in FP32 unless `exact` is set.
 void gemm(Engine &eng, int precision, GemmArgs g, bool exact = false);
 
+// FP64 / FP32 product on `stream` with the split-K partial sums in the
+// caller's workspace (work_doubles doubles), no allocation.  ksplit = 0:
+// the split gemm() would choose, reduced to what the workspace holds;
+// ksplit > 0: that split, or an error when its partial sums do not fit.
+void gemm_workspace(hipStream_t stream, int cus, int precision, GemmArgs g, double *work,
+                    int64_t work_doubles, int ksplit);
+
 // Split-BF16x3 GEMM on operands split into bf16 planes (batch == 1);
 // passes = 1 keeps the hi*hi product only (plain bf16).
 void gemm_b3(Engine &eng, GemmArgs g, int passes = 3);
@@ -111,12 +118,20 @@ void gemm_b3(Engine &eng, GemmArgs g, int passes = 3);
 void set_b3_backend(int library);
 int get_b3_backend();
 
+// Kernel variants of the batch-1 split-BF16x3 (passes = 3) and bf16 (passes
+// = 1) products: count, name, and a process-wide forced choice (-1: the
+// tuning cache, else the size rule).
+int b3_variant_count(int passes);
+const char *b3_variant_name(int passes, int variant);
+void set_b3_variant(int passes, int variant);
+
 // Split-K factor from s (at most smax) chosen so that groups * split
 // workgroups fill their last round on a device of `cus` compute units.
 int round_filling_split(int64_t groups, int64_t s, int64_t smax, int cus);
 
 // An A operand (M x K, element (m,k) at A[m*sam + k*sak]) split once into the
-// bf16 hi and lo planes of the 128-row split-BF16x3 / bf16 kernels and kept
+// bf16 hi and lo planes of the split-BF16x3 / bf16 kernels (rows padded to
+// 256, so every kernel variant reads them) and kept
 // across residual evaluations (the refinement anchor's ladder slices);
 // `store` owns both planes.
 struct B3FrozenA {

@@ -14,6 +14,7 @@
 #include "blaslt.hpp"
 #include "engine.hpp"
 #include "gemm.hpp"
+#include "tune.hpp"
 
 namespace sisi {
 
@@ -270,6 +271,228 @@ __global__ __launch_bounds__(256, MINB) void gemm_b3p_kernel(B3Args p) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int n = n0 + wn + j * 16 + 4 * lk + r;
+          if (n < g.N) W[m + int64_t(n) * g.M] = double(acc[i][j][r]);
+        }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Large-tile GEMM on split planes with direct global-to-LDS loads
+// ---------------------------------------------------------------------------
+//
+// BM x BN block of WGM x WGN wave64s, k-chunks of 32 bf16 (64-byte rows),
+// STAGES chunks resident in LDS as a ring.  Each chunk is fetched with
+// global_load_lds_dwordx4: one instruction moves 16 rows x 64 bytes (64
+// lanes x 16 bytes) of one plane straight into LDS, no registers involved,
+// so a chunk's fetch runs while the MFMAs of earlier chunks execute.  Lane l
+// of an instruction writes 16-byte slot l & 3 of row l >> 2; it reads the
+// k-chunk (l & 3) ^ swz(row), so the rows are stored XOR-swizzled and the
+// fragment reads (32 rows of one 16-byte chunk per half wave) spread over
+// the LDS banks.
+//
+// Per chunk: wait for this thread's loads of the chunk (vmcnt), barrier
+// (every wave's loads landed, every wave done with the slot about to be
+// refilled), issue the loads of chunk c + STAGES - 1 into the slot freed by
+// chunk c - 1, multiply chunk c.  One barrier per chunk.
+//
+// MFMA: v_mfma_f32_32x32x16_bf16 with the B plane in the instruction's A
+// role and the A plane in its B role, so lane l, register r of a 32x32 tile
+// holds C(m = l % 32, n = 8 (r / 4) + 4 (l / 32) + r % 4): 32 lanes store
+// 32 consecutive m (the output's unit-stride index).
+
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+typedef float f16v __attribute__((ext_vector_type(16)));
+
+#define SISI_LDS_PTR(p) ((__attribute__((address_space(3))) void *)(p))
+
+// s_waitcnt with vmcnt(V), lgkmcnt(0), expcnt untouched (gfx9 encoding:
+// vmcnt bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8).
+template <int V>
+__device__ __forceinline__ void wait_vm() {
+  static_assert(V >= 0 && V < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((V & 15) | ((V >> 4) << 14) | (7 << 4));
+}
+
+// Workgroup barrier without the memory fence of __syncthreads (whose release
+// half waits for every outstanding global load, including the prefetches);
+// the asm statements keep the compiler from moving memory accesses across.
+__device__ __forceinline__ void lds_barrier() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
+// 16 bytes per lane from global memory straight into LDS: lane l's bytes land
+// at dst + 16 l (dst uniform across the wave).
+__device__ __forceinline__ void dma16(const u16 *src, unsigned char *dst) {
+  __builtin_amdgcn_global_load_lds(src, SISI_LDS_PTR(dst), 16, 0, 0);
+}
+
+template <int BM, int BN, int PASSES, int STAGES>
+constexpr int b3d_lds_bytes() {
+  return STAGES * (PASSES == 3 ? 2 : 1) * (BM + BN) * 64;
+}
+
+template <int BM, int BN, int WGM, int WGN, int PASSES, int STAGES, int SWZ>
+__global__ __launch_bounds__(64 * WGM * WGN, 1) void gemm_b3d_kernel(B3Args p) {
+  constexpr int NW = WGM * WGN;
+  constexpr int WM = BM / WGM, WN = BN / WGN, TI = WM / 32, TJ = WN / 32;
+  constexpr int PL = PASSES == 3 ? 2 : 1;
+  constexpr int A_BYTES = BM * 64, B_BYTES = BN * 64;
+  constexpr int STAGE_BYTES = PL * (A_BYTES + B_BYTES);
+  constexpr int IA = BM / (16 * NW), IB = BN / (16 * NW);
+  static_assert(IA * 16 * NW == BM && IB * 16 * NW == BN, "16-row blocks per wave");
+  static_assert(TI * 32 == WM && TJ * 32 == WN, "32x32 tiles per wave");
+  constexpr int LPS = PL * (IA + IB);  // loads per chunk per thread
+  static_assert(LPS * (STAGES - 2 > 0 ? STAGES - 2 : 0) < 64, "vmcnt range");
+  extern __shared__ __attribute__((aligned(1024))) unsigned char smem[];
+
+  const GemmArgs &g = p.g;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  int pid = blockIdx.x;
+  {
+    const int tiles = g.tiles_m * g.tiles_n, per = tiles >> 3;
+    if (g.xcd_map && pid < (per << 3)) pid = (pid & 7) * per + (pid >> 3);
+  }
+  const int per_group = 8 * g.tiles_n, gid = pid / per_group, first_m = gid * 8;
+  const int gsize = min(g.tiles_m - first_m, 8), rr = pid % per_group;
+  const int tm = first_m + rr % gsize, tn = rr / gsize;
+  const int m0 = tm * BM, n0 = tn * BN;
+  const int ks = blockIdx.y, bz = blockIdx.z;
+  const int kbeg = ks * g.kchunk;
+  const int kend = min(int(p.Kp), kbeg + g.kchunk);
+  const int nk = kend > kbeg ? (kend - kbeg) / 32 : 0;
+
+  // this lane's source row (within a 16-row block) and k-chunk
+  const int lr = lane >> 2;
+  const int lc = (lane & 3) ^ ((lr >> SWZ) & 3);
+  const int64_t Kp = p.Kp;
+  const u16 *srcA[PL], *srcB[PL];
+  srcA[0] = p.Ah + (int64_t(m0) + lr) * Kp + 8 * lc;
+  srcB[0] = p.Bh + (int64_t(n0) + lr) * Kp + 8 * lc;
+  if constexpr (PL == 2) {
+    srcA[PL - 1] = p.Al + (int64_t(m0) + lr) * Kp + 8 * lc;
+    srcB[PL - 1] = p.Bl + (int64_t(n0) + lr) * Kp + 8 * lc;
+  }
+
+  auto fetch = [&](int chunk, int slot) {
+    const int kb = kbeg + chunk * 32;
+    unsigned char *st = smem + slot * STAGE_BYTES;
+#pragma unroll
+    for (int q = 0; q < PL; ++q) {
+#pragma unroll
+      for (int i = 0; i < IA; ++i) {
+        const int row = 16 * (wave + NW * i);
+        dma16(srcA[q] + int64_t(row) * Kp + kb, st + q * A_BYTES + row * 64);
+      }
+#pragma unroll
+      for (int i = 0; i < IB; ++i) {
+        const int row = 16 * (wave + NW * i);
+        dma16(srcB[q] + int64_t(row) * Kp + kb, st + PL * A_BYTES + q * B_BYTES + row * 64);
+      }
+    }
+  };
+
+  f16v acc[TI][TJ];
+#pragma unroll
+  for (int i = 0; i < TI; ++i)
+#pragma unroll
+    for (int j = 0; j < TJ; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
+  const int wm = (wave / WGN) * WM, wn = (wave % WGN) * WN;
+  const int l32 = lane & 31, lh = lane >> 5;
+
+  // byte offset of (row, 16-byte chunk c) in a plane
+  auto at = [](int row, int c) { return row * 64 + ((c ^ ((row >> SWZ) & 3)) << 4); };
+
+  auto compute = [&](int slot) {
+    const unsigned char *st = smem + slot * STAGE_BYTES;
+#pragma unroll
+    for (int kk = 0; kk < 2; ++kk) {
+      const int c = 2 * kk + lh;
+      bf8v bh[TJ], bl[TJ];
+#pragma unroll
+      for (int j = 0; j < TJ; ++j) {
+        const int o = at(wn + 32 * j + l32, c);
+        bh[j] = *reinterpret_cast<const bf8v *>(st + PL * A_BYTES + o);
+        if constexpr (PASSES == 3)
+          bl[j] = *reinterpret_cast<const bf8v *>(st + PL * A_BYTES + B_BYTES + o);
+      }
+#pragma unroll
+      for (int i = 0; i < TI; ++i) {
+        const int o = at(wm + 32 * i + l32, c);
+        const bf8v ah = *reinterpret_cast<const bf8v *>(st + o);
+        if constexpr (PASSES == 3) {
+          const bf8v al = *reinterpret_cast<const bf8v *>(st + A_BYTES + o);
+#pragma unroll
+          for (int j = 0; j < TJ; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bl[j], ah, acc[i][j], 0, 0, 0);
+#pragma unroll
+          for (int j = 0; j < TJ; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bh[j], al, acc[i][j], 0, 0, 0);
+        }
+#pragma unroll
+        for (int j = 0; j < TJ; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bh[j], ah, acc[i][j], 0, 0, 0);
+      }
+    }
+  };
+
+#pragma unroll
+  for (int s = 0; s < STAGES - 1; ++s)
+    if (s < nk) fetch(s, s);
+  for (int c = 0; c < nk; ++c) {
+    // loads issued after chunk c's that may stay in flight
+    const int ahead = min(STAGES - 2, nk - 1 - c);
+    if constexpr (STAGES >= 4) {
+      if (ahead >= 2) wait_vm<2 * LPS>();
+      else if (ahead == 1) wait_vm<LPS>();
+      else wait_vm<0>();
+    } else if constexpr (STAGES == 3) {
+      if (ahead >= 1) wait_vm<LPS>();
+      else wait_vm<0>();
+    } else {
+      wait_vm<0>();
+    }
+    lds_barrier();
+    if (c + STAGES - 1 < nk) fetch(c + STAGES - 1, (c + STAGES - 1) % STAGES);
+    compute(c % STAGES);
+  }
+
+  if (g.ksplit == 1) {
+    double *C = g.C + int64_t(bz) * g.bsc;
+#pragma unroll
+    for (int i = 0; i < TI; ++i) {
+      const int m = m0 + wm + 32 * i + l32;
+      if (m >= g.M) continue;
+      const int64_t om = b3_off_m(g, m);
+#pragma unroll
+      for (int j = 0; j < TJ; ++j)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int n = n0 + wn + 32 * j + 8 * (r >> 2) + 4 * lh + (r & 3);
+          if (n < g.N) {
+            const int64_t o = om + b3_off_n(g, n);
+            double v = g.alpha * double(acc[i][j][r]);
+            if (g.beta != 0.0) v += g.beta * C[o];
+            C[o] = v;
+          }
+        }
+    }
+  } else {
+    double *W = g.work + (int64_t(bz) * g.ksplit + ks) * int64_t(g.M) * g.N;
+#pragma unroll
+    for (int i = 0; i < TI; ++i) {
+      const int m = m0 + wm + 32 * i + l32;
+      if (m >= g.M) continue;
+#pragma unroll
+      for (int j = 0; j < TJ; ++j)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int n = n0 + wn + 32 * j + 8 * (r >> 2) + 4 * lh + (r & 3);
           if (n < g.N) W[m + int64_t(n) * g.M] = double(acc[i][j][r]);
         }
     }
@@ -535,15 +758,13 @@ static std::atomic<int> &b3_backend() {
   return backend;
 }
 static bool b3_use_library() { return b3_backend().load() == 1; }
-// The plain bf16 products (the refinement residuals' changes) run on
-// hipBLASLt by default: 17 % faster refinement iterations at No=40, Nv=400,
-// NG=1600 with the same iteration count; their rounding, already at the bf16
-// level and corrected by the FP64 anchors, is not reproducible bit for bit.
-// SISI_B1_BACKEND=kernel keeps them on the hand-written kernel.
+// The plain bf16 products (the refinement residuals' changes) run on the
+// hand-written kernels; SISI_B1_BACKEND=library sends them to hipBLASLt on
+// the same planes (kept for comparison; rounding not reproducible).
 static bool b1_use_library() {
   static const bool lib = [] {
     const char *v = getenv("SISI_B1_BACKEND");
-    return v == nullptr || std::strcmp(v, "kernel") != 0;
+    return v != nullptr && std::strcmp(v, "library") == 0;
   }();
   return lib;
 }
@@ -551,46 +772,103 @@ static bool b1_use_library() {
 void set_b3_backend(int library) { b3_backend().store(library ? 1 : 0); }
 int get_b3_backend() { return b3_backend().load(); }
 
-template <int BM, int BN, int WM, int WN, int NBUF, int MINB = 1, int PASSES = 3>
-static void run_b3(Engine &e, GemmArgs g, const B3FrozenA *fa = nullptr) {
-  const int Kp = ((g.K + 31) / 32) * 32;
-  const int Mp = ((g.M + BM - 1) / BM) * BM, Np = ((g.N + BN - 1) / BN) * BN;
-  if (PASSES == 1 && fa && (b3_use_library() || b1_use_library())) {
-    // frozen hi plane: rows of Kp bf16, the first M rows used
-    g.split_out = !(g.dm0 >= g.M && g.dn0 >= g.N);
-    Planes la, lb;
-    la.hi = reinterpret_cast<const u16 *>(fa->store.get());
-    la.ld = Kp;
-    SISI_REQUIRE(fa->M == g.M && fa->Kp == Kp, SISI_ERR_INVALID, "frozen A planes do not match");
-    const Planes &B = split_operand(e, g.B + 0, g.sbn, g.sbk, g.N, g.K, g.N, Kp, lb, 0);
-    if (b3_library(e, g, la, B, Kp, 1)) return;
-  }
-  if ((b3_use_library() || (PASSES == 1 && b1_use_library())) && !fa) {
-    g.split_out = !(g.dm0 >= g.M && g.dn0 >= g.N);
-    Planes la, lb;
-    const int lay_a = PASSES == 3 ? 1 : 0, lay_b = PASSES == 3 ? 2 : 0;
-    const Planes &A = split_operand(e, g.A + 0, g.sam, g.sak, g.M, g.K, g.M, Kp, la, lay_a);
-    const Planes &B = split_operand(e, g.B + 0, g.sbn, g.sbk, g.N, g.K, g.N, Kp, lb, lay_b);
-    if (b3_library(e, g, A, B, Kp, PASSES)) return;
-  }
-  Planes pa, pb;
-  if (fa) {  // planes split earlier in the layout split_operand gives (layout 0)
-    SISI_REQUIRE(fa->store.get() && fa->M == g.M && fa->K == g.K && fa->Mp == Mp && fa->Kp == Kp,
-                 SISI_ERR_INVALID, "frozen A planes do not match the product");
-    pa.hi = reinterpret_cast<const u16 *>(fa->store.get());
-    pa.lo = pa.hi + int64_t(Mp) * Kp;
-    pa.base = pa.hi;
-    pa.ld = Kp;
-  }
-  const Planes &A = fa ? pa : split_operand(e, g.A + 0, g.sam, g.sak, g.M, g.K, Mp, Kp, pa);
-  const Planes &B = split_operand(e, g.B + 0, g.sbn, g.sbk, g.N, g.K, Np, Kp, pb);
-  g.tiles_m = Mp / BM;
-  g.tiles_n = Np / BN;
+// ---------------------------------------------------------------------------
+// Kernel variants of the batch-1 products and their launchers
+// ---------------------------------------------------------------------------
+
+template <int BM, int BN, int WM, int WN, int NBUF, int MINB, int PASSES>
+static void launch_b3p(const B3Args &p, dim3 grid, hipStream_t s) {
+  hipLaunchKernelGGL((gemm_b3p_kernel<BM, BN, WM, WN, NBUF, MINB, PASSES>), grid, dim3(256), 0, s,
+                     p);
+}
+
+template <int BM, int BN, int WGM, int WGN, int PASSES, int STAGES, int SWZ>
+static void launch_b3d(const B3Args &p, dim3 grid, hipStream_t s) {
+  constexpr int lds = b3d_lds_bytes<BM, BN, PASSES, STAGES>();
+  static_assert(lds <= 160 * 1024, "LDS of one CU");
+  static const hipError_t attr = hipFuncSetAttribute(
+      reinterpret_cast<const void *>(&gemm_b3d_kernel<BM, BN, WGM, WGN, PASSES, STAGES, SWZ>),
+      hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+  SISI_HIP(attr);
+  hipLaunchKernelGGL((gemm_b3d_kernel<BM, BN, WGM, WGN, PASSES, STAGES, SWZ>), grid,
+                     dim3(64 * WGM * WGN), lds, s, p);
+}
+
+// Every variant reads planes whose rows are padded to kB3RowPad, so planes
+// split once (frozen or cached) serve all of them.
+constexpr int kB3RowPad = 256;
+
+struct B3Variant {
+  int bm, bn;
+  void (*launch)(const B3Args &, dim3, hipStream_t);
+  const char *name;
+};
+
+// Variant tables per pass count; the index is the variant id of
+// sisi_gemm_set_variant, the tuning cache and the environment overrides
+// SISI_B3_VARIANT / SISI_B1_VARIANT.
+static const B3Variant kB3Variants[] = {
+    {128, 128, launch_b3p<128, 128, 64, 64, 1, 1, 3>, "lds128x128"},
+    {128, 128, launch_b3p<128, 128, 64, 64, 1, 3, 3>, "lds128x128_occ3"},
+    {256, 256, launch_b3d<256, 256, 2, 4, 3, 2, 2>, "dma256x256_w8_s2_x2"},
+    {256, 256, launch_b3d<256, 256, 2, 4, 3, 2, 1>, "dma256x256_w8_s2_x1"},
+    {256, 128, launch_b3d<256, 128, 2, 2, 3, 3, 2>, "dma256x128_w4_s3_x2"},
+    {256, 256, launch_b3d<256, 256, 2, 2, 3, 2, 2>, "dma256x256_w4_s2_x2"},
+};
+static const B3Variant kB1Variants[] = {
+    {128, 128, launch_b3p<128, 128, 64, 64, 1, 1, 1>, "lds128x128"},
+    {128, 128, launch_b3p<128, 128, 64, 64, 1, 2, 1>, "lds128x128_occ2"},
+    {256, 256, launch_b3d<256, 256, 2, 4, 1, 4, 2>, "dma256x256_w8_s4_x2"},
+    {256, 256, launch_b3d<256, 256, 2, 4, 1, 4, 1>, "dma256x256_w8_s4_x1"},
+    {256, 128, launch_b3d<256, 128, 2, 2, 1, 4, 2>, "dma256x128_w4_s4_x2"},
+    {256, 256, launch_b3d<256, 256, 2, 2, 1, 4, 2>, "dma256x256_w4_s4_x2"},
+};
+constexpr int kNumB3Variants = int(sizeof(kB3Variants) / sizeof(kB3Variants[0]));
+constexpr int kNumB1Variants = int(sizeof(kB1Variants) / sizeof(kB1Variants[0]));
+
+static std::atomic<int> &forced_variant(int passes) {
+  static std::atomic<int> v3{[] {
+    const char *v = getenv("SISI_B3_VARIANT");
+    return v != nullptr && *v ? atoi(v) : -1;
+  }()};
+  static std::atomic<int> v1{[] {
+    const char *v = getenv("SISI_B1_VARIANT");
+    return v != nullptr && *v ? atoi(v) : -1;
+  }()};
+  return passes == 1 ? v1 : v3;
+}
+
+int b3_variant_count(int passes) { return passes == 1 ? kNumB1Variants : kNumB3Variants; }
+const char *b3_variant_name(int passes, int v) {
+  if (v < 0 || v >= b3_variant_count(passes)) return "";
+  return (passes == 1 ? kB1Variants : kB3Variants)[v].name;
+}
+void set_b3_variant(int passes, int v) {
+  SISI_REQUIRE(v >= -1 && v < b3_variant_count(passes), SISI_ERR_INVALID, "unknown GEMM variant");
+  forced_variant(passes).store(v);
+}
+
+// Default choice: the 256-row DMA kernel when its tile grid fills the chip,
+// else the 128-row kernel (with split-K where the grid stays small).
+static int default_b3_variant(const GemmArgs &g, int passes) {
+  const int64_t t256 = int64_t((g.M + 255) / 256) * ((g.N + 255) / 256);
+  (void)passes;
+  return t256 >= 240 ? 2 : 0;
+}
+
+// Tile grid, split-K and workspace of one variant; launches the kernel (and
+// the split-K reduction) on the engine's stream.
+static void launch_variant(Engine &e, GemmArgs g, const B3Variant &v, int xcd, const Planes &A,
+                           const Planes &B, int Kp) {
+  g.tiles_m = (g.M + v.bm - 1) / v.bm;
+  g.tiles_n = (g.N + v.bn - 1) / v.bn;
+  g.xcd_map = xcd;
   const int64_t tiles = int64_t(g.tiles_m) * g.tiles_n;
   g.split_out = !(g.dm0 >= g.M && g.dn0 >= g.N);
   int ksplit = 1;
-  if (Kp > 4 * 32 && tiles < 512) {
-    const int64_t want = (512 + tiles - 1) / tiles;
+  const int64_t fill = v.bm >= 256 ? 256 : 512;  // workgroups that fill the chip
+  if (Kp > 4 * 32 && tiles < fill) {
+    const int64_t want = (fill + tiles - 1) / tiles;
     const int64_t smax = std::max<int64_t>(1, std::min<int64_t>(Kp / 256, 64));
     ksplit = round_filling_split(tiles, std::min(want, smax), smax, e.cus);
   }
@@ -607,11 +885,118 @@ static void run_b3(Engine &e, GemmArgs g, const B3FrozenA *fa = nullptr) {
   dim3 grid(unsigned(tiles), unsigned(ksplit), 1);
   {
     Timed t(*e.timers, Kind::GemmB3, gemm_flops(g));
-    hipLaunchKernelGGL((gemm_b3p_kernel<BM, BN, WM, WN, NBUF, MINB, PASSES>), grid, dim3(256), 0,
-                       e.stream, p);
+    v.launch(p, grid, e.stream);
     SISI_HIP(hipGetLastError());
   }
   if (ksplit > 1) gemm_reduce(e, g);
+}
+
+// Time every variant (both tile walks) of one product class on its real
+// planes into a scratch output; the fastest is recorded for the class.
+static TuneChoice tune_b3(Engine &e, const GemmArgs &g0, int passes, const Planes &A,
+                          const Planes &B, int Kp, const TuneKey &key) {
+  const B3Variant *table = passes == 1 ? kB1Variants : kB3Variants;
+  const int count = b3_variant_count(passes);
+  GemmArgs g = g0;
+  DBuf scratch(&e.pool, size_t(g.M) * g.N);
+  g.C = scratch.get();
+  g.cm0 = 1;
+  g.cn0 = g.M;
+  g.cm1 = g.cm2 = g.cn1 = g.cn2 = 0;
+  g.dm0 = g.M;
+  g.dn0 = g.N;
+  g.dm1 = g.dn1 = 1;
+  g.alpha = 1.0;
+  g.beta = 0.0;
+  const bool stats = e.timers->enabled;
+  e.timers->enabled = false;
+  hipEvent_t t0, t1;
+  SISI_HIP(hipEventCreate(&t0));
+  SISI_HIP(hipEventCreate(&t1));
+  TuneChoice best;
+  double best_us = 1e300;
+  for (int v = 0; v < count; ++v) {
+    for (int xcd = 1; xcd >= 0; --xcd) {
+      launch_variant(e, g, table[v], xcd, A, B, Kp);  // warm-up
+      const int reps = 3;
+      SISI_HIP(hipEventRecord(t0, e.stream));
+      for (int r = 0; r < reps; ++r) launch_variant(e, g, table[v], xcd, A, B, Kp);
+      SISI_HIP(hipEventRecord(t1, e.stream));
+      SISI_HIP(hipEventSynchronize(t1));
+      float ms = 0.f;
+      SISI_HIP(hipEventElapsedTime(&ms, t0, t1));
+      const double us = 1e3 * double(ms) / reps;
+      if (us < best_us) {
+        best_us = us;
+        best.variant = v;
+        best.xcd = xcd;
+        best.us = us;
+      }
+    }
+  }
+  SISI_HIP(hipEventDestroy(t0));
+  SISI_HIP(hipEventDestroy(t1));
+  e.timers->enabled = stats;
+  tune_record(key, best, table[best.variant].name);
+  return best;
+}
+
+static bool stream_capturing(hipStream_t s) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  return hipStreamIsCapturing(s, &st) == hipSuccess && st != hipStreamCaptureStatusNone;
+}
+
+static void run_b3(Engine &e, GemmArgs g, int passes, const B3FrozenA *fa = nullptr) {
+  const int Kp = ((g.K + 31) / 32) * 32;
+  const int Mp = ((g.M + kB3RowPad - 1) / kB3RowPad) * kB3RowPad;
+  const int Np = ((g.N + kB3RowPad - 1) / kB3RowPad) * kB3RowPad;
+  if (passes == 1 && fa && (b3_use_library() || b1_use_library())) {
+    // frozen hi plane: rows of Kp bf16, the first M rows used
+    g.split_out = !(g.dm0 >= g.M && g.dn0 >= g.N);
+    Planes la, lb;
+    la.hi = reinterpret_cast<const u16 *>(fa->store.get());
+    la.ld = Kp;
+    SISI_REQUIRE(fa->M == g.M && fa->Kp == Kp, SISI_ERR_INVALID, "frozen A planes do not match");
+    const Planes &B = split_operand(e, g.B + 0, g.sbn, g.sbk, g.N, g.K, g.N, Kp, lb, 0);
+    if (b3_library(e, g, la, B, Kp, 1)) return;
+  }
+  if ((b3_use_library() || (passes == 1 && b1_use_library())) && !fa) {
+    g.split_out = !(g.dm0 >= g.M && g.dn0 >= g.N);
+    Planes la, lb;
+    const int lay_a = passes == 3 ? 1 : 0, lay_b = passes == 3 ? 2 : 0;
+    const Planes &A = split_operand(e, g.A + 0, g.sam, g.sak, g.M, g.K, g.M, Kp, la, lay_a);
+    const Planes &B = split_operand(e, g.B + 0, g.sbn, g.sbk, g.N, g.K, g.N, Kp, lb, lay_b);
+    if (b3_library(e, g, A, B, Kp, passes)) return;
+  }
+  Planes pa, pb;
+  if (fa) {  // planes split earlier in the layout split_operand gives (layout 0)
+    SISI_REQUIRE(fa->store.get() && fa->M == g.M && fa->K == g.K && fa->Mp == Mp && fa->Kp == Kp,
+                 SISI_ERR_INVALID, "frozen A planes do not match the product");
+    pa.hi = reinterpret_cast<const u16 *>(fa->store.get());
+    pa.lo = pa.hi + int64_t(Mp) * Kp;
+    pa.base = pa.hi;
+    pa.ld = Kp;
+  }
+  const Planes &A = fa ? pa : split_operand(e, g.A + 0, g.sam, g.sak, g.M, g.K, Mp, Kp, pa);
+  const Planes &B = split_operand(e, g.B + 0, g.sbn, g.sbk, g.N, g.K, Np, Kp, pb);
+  const int count = b3_variant_count(passes);
+  int vid = forced_variant(passes).load(), xcd = 1;
+  if (vid < 0) {
+    const TuneKey key = tune_key(passes == 1 ? TuneFamily::B1 : TuneFamily::B3, g.M, g.N, g.K);
+    TuneChoice c = tune_lookup(key);
+    if (c.variant < 0 && tune_enabled() && !stream_capturing(e.stream))
+      c = tune_b3(e, g, passes, A, B, Kp, key);
+    if (c.variant >= 0 && c.variant < count) {
+      vid = c.variant;
+      xcd = c.xcd;
+    }
+  }
+  if (vid < 0) vid = default_b3_variant(g, passes);
+  const B3Variant &v = (passes == 1 ? kB1Variants : kB3Variants)[vid];
+  SISI_REQUIRE(int64_t((g.M + v.bm - 1) / v.bm) * v.bm <= Mp &&
+                   int64_t((g.N + v.bn - 1) / v.bn) * v.bn <= Np,
+               SISI_ERR_INVALID, "gemm_b3: tile grid exceeds the planes");
+  launch_variant(e, g, v, xcd, A, B, Kp);
 }
 
 void split_planes(Engine &e, const double *src, int64_t sr, int64_t sk, int R, int K, int Rp,
@@ -667,7 +1052,7 @@ void gemm_b3_segments(Engine &e, B3SegArgs a) {
 void b3_freeze_a(Engine &e, const double *A, int64_t sam, int64_t sak, int M, int K,
                  B3FrozenA &out) {
   SISI_REQUIRE(A && M > 0 && K > 0, SISI_ERR_INVALID, "b3_freeze_a: empty operand");
-  const int Kp = ((K + 31) / 32) * 32, Mp = ((M + 127) / 128) * 128;
+  const int Kp = ((K + 31) / 32) * 32, Mp = ((M + kB3RowPad - 1) / kB3RowPad) * kB3RowPad;
   Planes local;
   split_operand(e, A, sam, sak, M, K, Mp, Kp, local, 0, false);
   out.store = std::move(local.store);
@@ -680,26 +1065,9 @@ void b3_freeze_a(Engine &e, const double *A, int64_t sam, int64_t sak, int M, in
 void gemm_b3_frozen_a(Engine &e, GemmArgs g, const B3FrozenA &A, int passes) {
   if (g.M <= 0 || g.N <= 0) return;
   SISI_REQUIRE(g.batch == 1, SISI_ERR_INVALID, "gemm_b3_frozen_a: batch 1 only");
-  // the 128-row kernels whose plane layout b3_freeze_a produced
-  if (passes == 1) run_b3<128, 128, 64, 64, 1, 1, 1>(e, g, &A);
-  else run_b3<128, 128, 64, 64, 1>(e, g, &A);
+  run_b3(e, g, passes == 1 ? 1 : 3, &A);
 }
 
-void gemm_b3(Engine &e, GemmArgs g, int passes) {
-  // 128x128 blocks of four 64x64 waves, two workgroups per CU, measured
-  // fastest on the engine's shapes; the others are kept for measurement
-  static const int cfg = getenv("SISI_B3_CFG") ? atoi(getenv("SISI_B3_CFG")) : 2;
-  if (passes == 1) {
-    run_b3<128, 128, 64, 64, 1, 1, 1>(e, g);
-    return;
-  }
-  switch (cfg) {
-    case 0: run_b3<256, 128, 128, 64, 2>(e, g); break;
-    case 1: run_b3<256, 128, 128, 64, 1>(e, g); break;
-    case 3: run_b3<128, 128, 64, 64, 2>(e, g); break;
-    case 5: run_b3<128, 128, 64, 64, 1, 3>(e, g); break;
-    default: run_b3<128, 128, 64, 64, 1>(e, g); break;
-  }
-}
+void gemm_b3(Engine &e, GemmArgs g, int passes) { run_b3(e, g, passes == 1 ? 1 : 3); }
 
 }  // namespace sisi

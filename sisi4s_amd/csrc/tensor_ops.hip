@@ -153,6 +153,26 @@ void zero(Engine &e, double *p, int64_t n) {
   SISI_HIP(hipGetLastError());
 }
 
+// p[i] = a value in [-1, 1) from an integer hash of (i, seed): operands for
+// timing runs (no denormals, no structure a kernel could exploit).
+__global__ void fill_pattern_kernel(double *p, int64_t n, uint32_t seed) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    uint64_t x = uint64_t(i) * 0x9E3779B97F4A7C15ull + seed;
+    x ^= x >> 31;
+    x *= 0xBF58476D1CE4E5B9ull;
+    x ^= x >> 29;
+    p[i] = double(x >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+  }
+}
+
+void fill_pattern(Engine &e, double *p, int64_t n, uint32_t seed) {
+  if (n <= 0) return;
+  const int blocks = int(std::min<int64_t>((n + 255) / 256, 16384));
+  hipLaunchKernelGGL(fill_pattern_kernel, dim3(blocks), dim3(256), 0, e.stream, p, n, seed);
+  SISI_HIP(hipGetLastError());
+}
+
 void zero(Engine &e, const View &dst) {
   // alpha = 0, beta = 0 writes zeros through the copy kernel's addressing
   copy(e, dst, dst, 0.0, 0.0);

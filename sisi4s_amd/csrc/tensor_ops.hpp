@@ -15,6 +15,9 @@ void copy(Engine &e, const View &dst, const View &src, double alpha = 1.0,
 void zero(Engine &e, const View &dst);
 void zero(Engine &e, double *p, int64_t n);
 
+// p[i] = deterministic pseudo-random values in [-1, 1) (timing operands).
+void fill_pattern(Engine &e, double *p, int64_t n, uint32_t seed);
+
 // y = a*x + b*y on contiguous arrays.
 void axpby(Engine &e, int64_t n, double a, const double *x, double b, double *y);
 

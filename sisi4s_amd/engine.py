@@ -95,6 +95,7 @@ SIGNATURES = [
     ("sisi_set_bf16_backend", ctypes.c_int, [ctypes.c_int]),
     ("sisi_engine_set_formulation", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("sisi_engine_set_drccd", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    ("sisi_eom_set_amplitudes", ctypes.c_int, [ctypes.c_void_p, _c_double_p, _c_double_p]),
     ("sisi_eom_ip_sigma", ctypes.c_int, [ctypes.c_void_p, _c_double_p, _c_double_p, _c_double_p,
                                          _c_double_p, _c_double_p, _c_double_p]),
     ("sisi_eom_ea_sigma", ctypes.c_int, [ctypes.c_void_p, _c_double_p, _c_double_p, _c_double_p,
@@ -141,6 +142,17 @@ SIGNATURES = [
      [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
       ctypes.c_double, _c_double_p, ctypes.c_int, _c_double_p, ctypes.c_int, ctypes.c_double,
       _c_double_p, ctypes.c_int]),
+    ("sisi_engine_set_energy_variant", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    ("sisi_engine_set_hp_fock", ctypes.c_int, [ctypes.c_void_p, _c_double_p]),
+    ("sisi_engine_get_noncanonical_energy", ctypes.c_int, [ctypes.c_void_p, _c_double_p]),
+    ("sisi_gemm_set_variant", ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
+    ("sisi_gemm_variant_count", ctypes.c_int, [ctypes.c_int]),
+    ("sisi_gemm_variant_name", ctypes.c_char_p, [ctypes.c_int, ctypes.c_int]),
+    ("sisi_tune_enable", ctypes.c_int, [ctypes.c_int]),
+    ("sisi_tune_dump", ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, _c_int_p]),
+    ("sisi_gemm_bench", ctypes.c_int,
+     [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_double_p,
+      _c_double_p]),
 ]
 
 _lib = None
@@ -201,6 +213,47 @@ def set_bf16_backend(library):
     """Process-wide backend of the split-BF16x3 / bf16 products: False = the
     engine's MFMA kernel, True = hipBLASLt on the engine's operand planes."""
     _check(load().sisi_set_bf16_backend(int(bool(library))))
+
+
+def set_gemm_variant(passes, variant):
+    """Force kernel variant `variant` of the split-BF16x3 (passes=3) or bf16
+    (passes=1) products for the whole process; -1 restores the automatic
+    choice (tuning cache, else the size rule)."""
+    _check(load().sisi_gemm_set_variant(int(passes), int(variant)))
+
+
+def gemm_variants(passes):
+    """Names of the kernel variants of a pass count, by variant id."""
+    lib = load()
+    return [lib.sisi_gemm_variant_name(int(passes), v).decode()
+            for v in range(lib.sisi_gemm_variant_count(int(passes)))]
+
+
+def tune_enable(on=True):
+    """Time unknown product classes on first use and record the fastest
+    variant (process-wide)."""
+    _check(load().sisi_tune_enable(int(bool(on))))
+
+
+def tune_dump():
+    """The tuning records of this process as text lines."""
+    lib = load()
+    n = ctypes.c_int(0)
+    _check(lib.sisi_tune_dump(None, 0, ctypes.byref(n)))
+    buf = ctypes.create_string_buffer(max(n.value, 1))
+    _check(lib.sisi_tune_dump(buf, len(buf), ctypes.byref(n)))
+    return buf.value.decode()
+
+
+def gemm_bench(precision, M, N, K, reps=5):
+    """(kernel microseconds, whole-call microseconds) per GEMM call on
+    device-resident operands of the given shape."""
+    if isinstance(precision, str):
+        precision = PRECISIONS[precision]
+    k, t = ctypes.c_double(), ctypes.c_double()
+    _check(load().sisi_gemm_bench(int(precision), int(M), int(N), int(K), int(reps),
+                                  ctypes.byref(k), ctypes.byref(t)))
+    return k.value, t.value
 
 
 def default_opts(**kw):
@@ -448,6 +501,26 @@ class Engine:
         _check(self._lib.sisi_eom_ea_sigma(self._h, _ptr(t1), _ptr(t2), _ptr(x1), _ptr(x2),
                                            _ptr(b1), _ptr(b2)))
         return _unbuf(b1, (v,)), _unbuf(b2, (o, v, v))
+
+    def set_energy_variant(self, unrestricted=False, antisymmetrize=False):
+        """`unrestricted` (spins = 1) and `antisymmetrize` energies of getEnergy
+        (ClusterSinglesDoublesAlgorithm.cxx:131-226)."""
+        _check(self._lib.sisi_engine_set_energy_variant(self._h, int(bool(unrestricted)),
+                                                        int(bool(antisymmetrize))))
+
+    def set_hp_fock(self, fia=None):
+        """HPFockMatrix f[i,a] (No x Nv): energies add spins * sum T_ai f_ia
+        (ClusterSinglesDoublesAlgorithm.cxx:199-211); None clears it."""
+        if fia is None:
+            _check(self._lib.sisi_engine_set_hp_fock(self._h, None))
+            return
+        b = self._in(fia, self.No * self.Nv, "HPFockMatrix[No,Nv]")
+        _check(self._lib.sisi_engine_set_hp_fock(self._h, _ptr(b)))
+
+    def noncanonical_energy(self):
+        out = ctypes.c_double()
+        _check(self._lib.sisi_engine_get_noncanonical_energy(self._h, ctypes.byref(out)))
+        return out.value
 
     def set_residual_variant(self, ppl=True, distinguishable=False):
         """ppl / distinguishable of the next residual calls

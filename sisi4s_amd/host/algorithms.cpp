@@ -196,12 +196,25 @@ class CcsdEnergyFromCoulombIntegrals : public Algorithm {
   // `unrestricted`), 2 spin-orbital CCSD (antisymmetrized energy)
   void solve(sisi_engine *e, bool cplx, bool vertex_arguments, const std::string &abbreviation = "Ccsd",
              int kind = 0) {
-    if ((kind == 0 && getIntegerArgument("unrestricted", 0) != 0) ||
-        (kind != 2 && getIntegerArgument("antisymmetrize", 0) != 0))
-      throw StepError("unrestricted / antisymmetrized energies are not supported");
-    if (isArgumentGiven("HPFockMatrix")) throw StepError("HPFockMatrix is not supported");
     const int64_t o = getTensorArgument("HoleEigenEnergies").lens[0];
     const int64_t v = getTensorArgument("ParticleEigenEnergies").lens[0];
+    // getEnergy's arguments (ClusterSinglesDoublesAlgorithm.cxx:133-134,
+    // 199-211); the spin-orbital CCSD energy is the antisymmetrized one
+    const bool antisymmetrize = getIntegerArgument("antisymmetrize", 0) != 0;
+    if (kind != 2)  // (direct-ring CCD's residual reads the same `unrestricted`)
+      check(sisi_engine_set_energy_variant(e, getIntegerArgument("unrestricted", 0) != 0,
+                                           antisymmetrize),
+            "sisi_engine_set_energy_variant");
+    if (isArgumentGiven("HPFockMatrix")) {
+      if (kind == 2) throw StepError("HPFockMatrix is not read by the spin-orbital CCSD step");
+      const auto &F = getTensorArgument("HPFockMatrix");
+      if (F.lens != std::vector<int64_t>{o, v})
+        throw StepError("HPFockMatrix must be [No, Nv]");
+      const auto f = amplitudes_for(F, cplx, "HPFockMatrix");
+      check(sisi_engine_set_hp_fock(e, f.data()), "sisi_engine_set_hp_fock");
+    } else if (kind != 2) {
+      check(sisi_engine_set_hp_fock(e, nullptr), "sisi_engine_set_hp_fock");
+    }
     sisi_ccsd_opts opts;
     sisi_ccsd_opts_default(&opts);
     const std::string mixer = getTextArgument("mixer", "LinearMixer");
@@ -291,7 +304,12 @@ class CcsdEnergyFromCoulombIntegrals : public Algorithm {
     emitted.scalars.push_back({"energy", num(r.energy)});
     // energy decomposition of getEnergy (ClusterSinglesDoublesAlgorithm.cxx:180-190),
     // not emitted by its antisymmetrized branch
-    if (kind != 2) {
+    if (isArgumentGiven("HPFockMatrix")) {  // cxx:206-210
+      double nc = 0.0;
+      check(sisi_engine_get_noncanonical_energy(e, &nc), "sisi_engine_get_noncanonical_energy");
+      emitted.scalars.push_back({"noncanonical", num(nc)});
+    }
+    if (kind != 2 && !antisymmetrize) {
       emitted.scalars.push_back({"direct", num(r.direct)});
       emitted.scalars.push_back({"exchange", num(r.exchange)});
       emitted.scalars.push_back({"singlet", num(0.25 * r.direct - 0.5 * r.exchange)});

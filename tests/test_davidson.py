@@ -22,7 +22,11 @@ def harness(tmp_path_factory):
     if gxx is None:
         pytest.skip("g++ not found")
     exe = str(tmp_path_factory.mktemp("davidson") / "davidson_check")
-    subprocess.run([gxx, "-O2", "-std=c++17", "-Wall", "-Werror", SRC, "-o", exe], check=True)
+    # SISI_SANITIZE=1: the harness (and the host headers it includes) built
+    # with AddressSanitizer and UndefinedBehaviorSanitizer
+    san = (["-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined",
+            "-fno-sanitize-recover=undefined"] if os.environ.get("SISI_SANITIZE") == "1" else ["-O2"])
+    subprocess.run([gxx] + san + ["-std=c++17", "-Wall", "-Werror", SRC, "-o", exe], check=True)
     return exe
 
 

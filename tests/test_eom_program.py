@@ -1,5 +1,7 @@
 """The contraction programs of the device EOM right-applies (eom_ip_sigma and
-eom_ea_sigma in sisi4s_amd/csrc/eom.hip) replayed on the CPU: each ctr / sum /
+eom_ea_sigma in sisi4s_amd/csrc/eom.hip, after the block builds they call:
+make_taus, build_common_blocks and build_ip_blocks / build_ea_blocks)
+replayed on the CPU: each ctr / sum /
 copy / zero / add_t1t1 / add_diagonal statement of the function body runs as
 the numpy operation the engine's planner documents (ctr: C = alpha einsum(A, B)
 + beta C; sum: C[ic] = alpha A[ia] + beta C[ic]; first index fastest), on
@@ -22,7 +24,7 @@ SRC = os.path.join(ROOT, "sisi4s_amd", "csrc", "eom.hip")
 
 def _body(name):
     text = open(SRC).read()
-    start = text.index("void %s(" % name)
+    start = text.index(" %s(Engine &e" % name)
     i = text.index("{", start)
     depth, j = 0, i
     while True:
@@ -103,15 +105,18 @@ class Replay:
             st = " ".join(st.split())
             if not st:
                 continue
-            if st.startswith(("SISI_REQUIRE", "e.thaw()", "const int64_t", "const int p", "DBuf ")):
+            if st.startswith(("SISI_REQUIRE", "e.thaw()", "const int64_t", "const int p", "DBuf ",
+                              "if (", "return", "Taus ", "build_", "e.eom.ip = ", "e.eom.ea = ")):
                 continue
+            if re.fullmatch(r"e\.eom\.\w+ = DBuf\(.*\)", st):
+                continue  # a block's allocation
             if st.startswith("double *"):
                 for m in re.finditer(r"\*(\w+) = const_cast<double \*>\((\w+)\)", st):
                     self.alias[m.group(1)] = m.group(2)
                 continue
-            if st.startswith("View "):
-                for d in _split_top(st[5:]):
-                    m = re.fullmatch(r"(\w+) = dense\((.*), \{(.*)\}\)", d)
+            if st.startswith("View ") or re.fullmatch(r"[\w.]+ = dense\(.*\)", st):
+                for d in _split_top(st[5:] if st.startswith("View ") else st):
+                    m = re.fullmatch(r"([\w.]+) = dense\((.*), \{(.*)\}\)", d)
                     assert m, d
                     dims = [self.num(x) for x in m.group(3).split(",")]
                     self.views[m.group(1)] = (self.key(m.group(2)), dims)
@@ -174,7 +179,7 @@ def _system(no, n, seed):
 
 
 def _buffers(ints, ei, ea, T1, T2):
-    return {"T1c": T1, "T2c": T2, "e.Vabij.get()": ints["PPHH"], "e.Vijkl.get()": ints["HHHH"],
+    return {"e.eom.T1.get()": T1, "e.eom.T2.get()": T2, "e.Vabij.get()": ints["PPHH"], "e.Vijkl.get()": ints["HHHH"],
             "e.Vijka.get()": ints["HHHP"], "e.Vabci.get()": ints["PPPH"], "e.Vaibj.get()": ints["PHPH"],
             "e.Vabcd.get()": ints["PPPP"], "e.eps_i.get()": ei, "e.eps_a.get()": ea}
 
@@ -188,7 +193,8 @@ def test_ip_program_replays_to_the_oracle(no, n):
     r2 = rng.normal(size=(no, no, nv))
     r2 = r2 - r2.transpose(1, 0, 2)
     rp = Replay(no, nv, dict(_buffers(ints, ei, ea, T1, T2), r1c=r1, r2c=r2))
-    rp.run(_body("eom_ip_sigma"))
+    for f in ("make_taus", "build_common_blocks", "build_ip_blocks", "eom_ip_sigma"):
+        rp.run(_body(f))
     s1 = rp.mem["s1p"][:no]
     s2 = rp.mem["s2p"][:no * no * nv].reshape((no, no, nv), order="F")
     H = eom.hbar_ip(ints, ei, ea, T1, T2)
@@ -206,7 +212,8 @@ def test_ea_program_replays_to_the_oracle(no, n):
     r2 = rng.normal(size=(no, nv, nv))
     r2 = r2 - r2.transpose(0, 2, 1)
     rp = Replay(no, nv, dict(_buffers(ints, ei, ea, T1, T2), r1c=r1, r2c=r2))
-    rp.run(_body("eom_ea_sigma"))
+    for f in ("make_taus", "build_common_blocks", "build_ea_blocks", "eom_ea_sigma"):
+        rp.run(_body(f))
     s1 = rp.mem["s1p"][:nv]
     s2 = rp.mem["s2p"][:no * nv * nv].reshape((no, nv, nv), order="F")
     H = eom.hbar_ea(ints, ei, ea, T1, T2)
@@ -222,11 +229,14 @@ def test_replay_catches_a_planted_sign_error():
     ints, ei, ea, T1, T2, rng = _system(3, 8, 23)
     r1, r2 = rng.normal(size=3), rng.normal(size=(3, 3, 5))
     r2 = r2 - r2.transpose(1, 0, 2)
-    body = _body("eom_ip_sigma")
+    body = _body("build_ip_blocks")
     planted = body.replace('ctr(e, 1.0, "mnie,bejn->mbij"', 'ctr(e, -1.0, "mnie,bejn->mbij"')
     assert planted != body
     rp = Replay(3, 5, dict(_buffers(ints, ei, ea, T1, T2), r1c=r1, r2c=r2))
+    for f in ("make_taus", "build_common_blocks"):
+        rp.run(_body(f))
     rp.run(planted)
+    rp.run(_body("eom_ip_sigma"))
     s2 = rp.mem["s2p"][:45].reshape((3, 3, 5), order="F")
     _, w2 = eom.ip_sigma(eom.hbar_ip(ints, ei, ea, T1, T2), r1, r2)
     assert np.abs(s2 - w2).max() > 1e-4

@@ -561,3 +561,39 @@ def test_full_integral_formulation_matches_oracle_and_vertex_path():
     rv = ev.ccsd_solve(engine.default_opts(**kw))
     assert rf["converged"] and rv["converged"]
     assert np.abs(ef.energies() - ev.energies()).max() <= 1e-10
+
+
+@pytest.mark.parametrize("unrestricted,anti,fock", [(True, False, False), (False, True, False),
+                                                    (False, False, True), (True, False, True)])
+def test_energy_variants_match_oracle(unrestricted, anti, fock):
+    """getEnergy's arguments (ClusterSinglesDoublesAlgorithm.cxx:131-226):
+    `unrestricted` (spins = 1), `antisymmetrize`, and the HPFockMatrix term:
+    every iteration's energy within 1e-10 Eh of the oracle, pair energies to
+    1e-11 relative, and the non-canonical term itself."""
+    from sisi4s_amd import engine
+    No, Nv, NG = 5, 20, 80
+    e, g, ei, ea = make_engine(7, No, Nv, NG)
+    f = 0.05 * np.random.default_rng(2).standard_normal((No, Nv)) if fock else None
+    kw = dict(mixer="DiisMixer", max_residua=6, max_iterations=12, amplitudes_convergence=0.0,
+              energy_convergence=0.0)
+    announce("energy variants unrestricted=%d antisymmetrize=%d fock=%d" % (unrestricted, anti, fock))
+    try:
+        e.set_energy_variant(unrestricted, anti)
+        if fock:
+            e.set_hp_fock(f)
+        e.ccsd_solve(engine.default_opts(**kw))
+        got = e.energies()
+        spins = 1.0 if unrestricted else 2.0
+        ref = ccsd.solve(g, ei, ea, spins=spins, antisymmetrized=anti, fia=f, **kw)
+        assert float(np.abs(got - np.array(ref["energies"])).max()) <= 1e-10
+        T1, T2 = e.amplitudes()
+        V = ccsd.integrals_from_vertex(g, No)["PPHH"]
+        P = e.pair_energies()
+        Pr = ccsd.pair_energy(T1, T2, V, antisymmetrized=anti)
+        assert np.abs(P - Pr).max() <= 1e-11 * np.abs(Pr).max()
+        if fock:
+            nc = spins * float(np.sum(T1 * f.T))
+            assert abs(e.noncanonical_energy() - nc) <= 1e-12 * max(abs(nc), 1e-300)
+    finally:
+        e.set_energy_variant(False, False)
+        e.set_hp_fock(None)

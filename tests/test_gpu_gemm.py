@@ -58,3 +58,41 @@ def test_gemm_library_backend(prec, ta, tb, M, N, K):
         _run(prec, ta, tb, M, N, K, alpha=0.75, beta=0.5, seed=M + K)
     finally:
         engine.set_bf16_backend(False)
+
+
+def _variant_ids(passes):
+    from sisi4s_amd import engine
+    return list(range(len(engine.gemm_variants(passes))))
+
+
+@pytest.mark.parametrize("passes,prec", [(3, 2), (1, 3)])
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 1)])
+@pytest.mark.parametrize("M,N,K", [(17, 33, 5), (256, 384, 96), (300, 200, 3000),
+                                   (520, 270, 700)])
+def test_gemm_every_kernel_variant(passes, prec, ta, tb, M, N, K):
+    """Every kernel variant of the split-BF16x3 and bf16 products (the
+    128-row register-staged kernels and the 256-row direct-to-LDS kernels,
+    with their split-K paths) meets the precision's error bound."""
+    from sisi4s_amd import engine
+    for v in _variant_ids(passes):
+        engine.set_gemm_variant(passes, v)
+        try:
+            _run(prec, ta, tb, M, N, K, alpha=0.75, beta=0.5, seed=M + N + K + v)
+        finally:
+            engine.set_gemm_variant(passes, -1)
+
+
+def test_gemm_bench_and_tuning_cache():
+    """The timing entry point returns positive times, and a tuning pass on a
+    fresh cache records one winner for the class it met."""
+    from sisi4s_amd import engine
+    announce("gemm bench + tuning")
+    k, t = engine.gemm_bench("bf16x3", 1024, 1024, 512, reps=2)
+    assert 0.0 < k <= t
+    engine.tune_enable(True)
+    try:
+        engine.gemm_bench("bf16", 2048, 1536, 256, reps=1)
+    finally:
+        engine.tune_enable(False)
+    dump = engine.tune_dump()
+    assert any(line.startswith("1 11 11 8 ") for line in dump.splitlines()), dump

@@ -10,7 +10,7 @@ import pytest
 
 from conftest import ROOT, announce
 
-RUNNER = os.path.join(ROOT, "sisi4s_amd", "sisi4s-mi355x")
+RUNNER = os.environ.get("SISI_RUNNER") or os.path.join(ROOT, "sisi4s_amd", "sisi4s-mi355x")
 INPUT = os.path.join(ROOT, "tests", "inputs", "no10_nv50_ccsd_t.yaml")
 UEG_INPUT = os.path.join(ROOT, "tests", "inputs", "ueg_rs1_no7_nv26.yaml")
 

@@ -818,3 +818,60 @@ def test_ea_eom_closed_shell_spin_degeneracy():
     ev = eom.ea_eigenvalues(ints, epi, epa, r["T1"], r["T2"])
     assert abs(ev[0] - ev[1]) <= 1e-9 and abs(ev[2] - ev[3]) <= 1e-9
     assert abs(ev[0] - ea.min()) <= 0.1 * abs(ea.min())
+
+
+def test_energy_variants_closed_forms():
+    """getEnergy's arguments (ClusterSinglesDoublesAlgorithm.cxx:131-213) on a
+    closed shell written in spin orbitals, against the restricted energy of
+    the pinned real path: the antisymmetrized energy of the spin-orbital
+    amplitudes and <ij||ab> equals it; the unrestricted (spins = 1) direct
+    minus exchange energy of the spin-orbital amplitudes and the plain <ij|ab>
+    equals twice its doubles part plus its singles part (the spin sums give
+    4 and 2 for the doubles' direct and exchange terms, 4 and 2 against 2
+    and 1 for the singles').  The HPFockMatrix term is the element sum
+    spins * T_ai f_ia.  Planted mistakes (spins^2 in the exchange term, the
+    singles term of the antisymmetrized energy dropped) are detected."""
+    from oracle import uccsd
+    No, Nv, NG = 2, 5, 14
+    g, ei, ea = philox.system(41, No, Nv, NG)
+    r = ccsd.solve(g, ei, ea, max_residua=6, max_iterations=8, amplitudes_convergence=0.0,
+                   energy_convergence=0.0)
+    T1, T2 = r["T1"], r["T2"]
+    V = r["integrals"]["PPHH"]
+    E = ccsd.energy(T1, T2, V)[0]
+    assert abs(ccsd.energy_of(T1, T2, V) - E) <= 1e-13 * abs(E)
+    # spin orbital P = 2 p + s (alpha 0, beta 1) inside each block
+    sp_o, sp_v = np.tile([0, 1], No), np.tile([0, 1], Nv)
+    o2, v2 = np.repeat(np.arange(No), 2), np.repeat(np.arange(Nv), 2)
+    d = lambda x, y: (x[:, None] == y[None, :]).astype(float)  # noqa: E731
+    S1 = T1[np.ix_(v2, o2)] * d(sp_v, sp_o)
+    Tdir = T2[np.ix_(v2, v2, o2, o2)] * (d(sp_v, sp_o)[:, None, :, None]
+                                         * d(sp_v, sp_o)[None, :, None, :])
+    S2 = Tdir - Tdir.transpose(1, 0, 2, 3)
+    Vplain = V[np.ix_(v2, v2, o2, o2)] * (d(sp_v, sp_o)[:, None, :, None]
+                                          * d(sp_v, sp_o)[None, :, None, :])
+    Vanti = uccsd.spin_orbital_integrals(g, No, Nv)["PPHH"]
+    assert np.abs(Vanti - (Vplain - Vplain.transpose(1, 0, 2, 3))).max() <= 1e-14
+    Ea = ccsd.energy_of(S1, S2, Vanti, antisymmetrized=True)
+    assert abs(Ea - E) <= 1e-12 * abs(E)
+    Eu = ccsd.energy_of(S1, S2, Vplain, spins=1.0)
+    E2 = ccsd.energy_of(0.0 * T1, T2, V)
+    assert abs(Eu - (2.0 * E2 + (E - E2))) <= 1e-12 * abs(E)
+    assert abs(E - E2) > 1e-6 * abs(E)  # the singles part is not negligible here
+    # planted: the antisymmetrized energy without its singles term
+    Vij = ccsd.ein("abij->ijab", Vanti)
+    assert abs(0.25 * ccsd.ein("abkl,klab->", S2, Vij) - E) > 1e-6 * abs(E)
+    # planted: spins^2 in the exchange term of the unrestricted energy (equals
+    # the correct one at spins = 1, so compare at spins = 2 with the restricted E)
+    Vr = ccsd.ein("abij->ijab", V)
+    tau = T2 + ccsd.ein("ai,bj->abij", T1, T1)
+    wrong = 2.0 * ccsd.ein("abij,ijab->", tau, Vr) - 2.0 * ccsd.ein("abij,ijba->", tau, Vr)
+    assert abs(wrong - E) > 1e-6 * abs(E)
+    rng = np.random.default_rng(3)
+    f = rng.standard_normal((No, Nv))
+    nc = sum(T1[a, i] * f[i, a] for a in range(Nv) for i in range(No))
+    for spins in (1.0, 2.0):
+        base = ccsd.energy_of(T1, T2, V, spins=spins)
+        assert abs(ccsd.energy_of(T1, T2, V, spins=spins, fia=f) - base - spins * nc) <= 1e-13
+    P = ccsd.pair_energy(S1, S2, Vanti, antisymmetrized=True)
+    assert abs(P.sum() - Ea) <= 1e-13 * abs(Ea)

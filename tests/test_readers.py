@@ -15,7 +15,7 @@ import pytest
 from conftest import ROOT, announce
 from oracle import philox
 
-RUNNER = os.path.join(ROOT, "sisi4s_amd", "sisi4s-mi355x")
+RUNNER = os.environ.get("SISI_RUNNER") or os.path.join(ROOT, "sisi4s_amd", "sisi4s-mi355x")
 
 
 def run_steps(tmp_path, text):
