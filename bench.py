@@ -93,12 +93,24 @@ def main():
     ap.add_argument("--eom", type=int, default=0,
                     help="consumers: also time the IP- and EA-EOM-CCSD right-applies (needs PPPP)")
     ap.add_argument("--mode", choices=["iteration", "converge", "batched", "complex", "consumers",
-                                       "small"],
+                                       "small", "gemm"],
                     default="iteration",
                     help="iteration: timed FP64 iterations (default line); converge: FP64 and "
                          "mixed runs to |dT|/|T| <= 1e-8 with (T); batched: 64 small systems")
+    ap.add_argument("--gemm-shape", default="16000,16000,16000",
+                    help="gemm mode: M,N,K of the timed product")
+    ap.add_argument("--gemm-precision", default="bf16x3", choices=["fp64", "fp32", "bf16x3", "bf16"])
+    ap.add_argument("--gemm-variant", type=int, default=-1,
+                    help="gemm mode: kernel variant of the bf16 / split-BF16x3 products (-1: "
+                         "automatic)")
+    ap.add_argument("--gemm-reps", type=int, default=2)
     ap.add_argument("--systems", type=int, default=64)
     ap.add_argument("--workers", type=int, default=8)
+    ap.add_argument("--batch-solver", choices=["lockstep", "threads"], default="lockstep",
+                    help="batched mode: grouped launches over all systems, or one engine per "
+                         "system on --workers host threads")
+    ap.add_argument("--batched-slice", type=int, default=0,
+                    help="batched mode: integralsSliceSize (0: No, the reference default)")
     ap.add_argument("--batched-refine-interval", type=int, default=0,
                     help="batched: FP64 anchor interval of the mixed runs (0: plain FP64 after the switch)")
     ap.add_argument("--graphs", type=int, default=0,
@@ -132,6 +144,8 @@ def main():
         return converge(args)
     if args.mode == "batched":
         return batched(args)
+    if args.mode == "gemm":
+        return gemm_shape(args)
     if args.mode == "consumers":
         return consumers(args)
     if args.mode == "small":
@@ -695,6 +709,24 @@ def converge(args):
     print(json.dumps(out), flush=True)
 
 
+def gemm_shape(args):
+    """One product shape on device-resident operands (a warm-up call, then
+    --gemm-reps timed calls): the GEMM microbenchmark the kernel profiles use."""
+    from sisi4s_amd import engine as eng
+    M, N, K = (int(x) for x in args.gemm_shape.split(","))
+    passes = {"bf16": 1, "bf16x3": 3}.get(args.gemm_precision)
+    name = None
+    if passes is not None:
+        eng.set_gemm_variant(passes, args.gemm_variant)
+        if args.gemm_variant >= 0:
+            name = eng.gemm_variants(passes)[args.gemm_variant]
+    k_us, t_us = eng.gemm_bench(args.gemm_precision, M, N, K, reps=args.gemm_reps)
+    out = {"mode": "gemm", "precision": args.gemm_precision, "M": M, "N": N, "K": K,
+           "variant": name, "kernel_us": k_us, "call_us": t_us,
+           "tflops": 2.0 * M * N * K / k_us / 1e6}
+    print(json.dumps(out), flush=True)
+
+
 def batched(args):
     """Independent seeded systems (64 by default), No in [4,16], Nv in [30,120],
     NG = 4 Nv, FP64 and mixed runs to 1e-8; per-system |dE|."""
@@ -709,17 +741,21 @@ def batched(args):
     kw = dict(mixer="DiisMixer", max_residua=6, max_iterations=300, level_shift=args.level_shift,
               amplitudes_convergence=1e-8, energy_convergence=1e-10)
     res = {}
-    for name, prec in (("fp64", "fp64"), ("mixed", "bf16x3")):
+    lockstep = args.batch_solver == "lockstep"
+    for name, prec in (("fp64", "fp64"), ("mixed", "fp32" if lockstep else "bf16x3")):
         t0 = time.perf_counter()
         # FP32 bulk then plain FP64 iterations: at these sizes no product is
         # large enough for the bf16 kernels, so refinement residuals (more
         # products per iteration at FP32 speed) would cost more than FP64
         ri = args.batched_refine_interval if prec != "fp64" else 0
-        E, reps = eng.ccsd_solve_batched(systems, eng.default_opts(precision=prec,
-                                                                   refine_interval=ri,
-                                                                   graphs=args.graphs, **kw),
-                                         workers=args.workers, strict=False)
-        res[name] = dict(seconds=time.perf_counter() - t0, energies=E.tolist(),
+        opts = eng.default_opts(precision=prec, refine_interval=ri, graphs=args.graphs,
+                                slice_size=args.batched_slice, **kw)
+        st = None
+        if lockstep:
+            E, reps, st = eng.ccsd_solve_lockstep(systems, opts, strict=False)
+        else:
+            E, reps = eng.ccsd_solve_batched(systems, opts, workers=args.workers, strict=False)
+        res[name] = dict(seconds=time.perf_counter() - t0, energies=E.tolist(), lockstep=st,
                          failed=[k for k, r in enumerate(reps) if r["status"] != 0],
                          conv=[bool(r["converged"]) and r["status"] == 0 for r in reps],
                          converged=int(sum(r["converged"] for r in reps)),
@@ -731,7 +767,8 @@ def batched(args):
     dE = np.abs(np.array(res["fp64"]["energies"])[ok] - np.array(res["mixed"]["energies"])[ok])
     out = {"metric": METRIC, "mode": "batched",
            "config": {"workload": "%d seeded systems, No U[4,16], Nv U[30,120], NG=4Nv, DIIS(6), "
-                                  "to 1e-8" % args.systems, "workers": args.workers,
+                                  "to 1e-8" % args.systems, "solver": args.batch_solver,
+                      "workers": args.workers, "slice": args.batched_slice,
                       "mixed_refine_interval": args.batched_refine_interval,
                       "graphs": args.graphs},
            "fp64_seconds": res["fp64"]["seconds"], "mixed_seconds": res["mixed"]["seconds"],

@@ -366,6 +366,40 @@ int sisi_ccsd_solve_batched(int nsys, const sisi_dims *dims, const double *const
                             const sisi_ccsd_opts *opts, int workers, double *energies,
                             sisi_ccsd_report *reports);
 
+/* Reduced-precision ladder with plane output (default on; no counterpart
+ * in the reference): the split-BF16x3 / bf16 V~_xycd slice-pair product
+ * (CcsdEnergyFromCoulombIntegrals.cxx:431-434) writes the bf16 hi/lo planes
+ * the X contraction (cxx:443) reads, with the rounding an operand split of
+ * the FP64 slice pair would give; off: the FP64 slice pair is stored and
+ * split by the next product.  Results are the same either way. */
+int sisi_engine_set_ladder_planes(sisi_engine *e, int enable);
+
+/* Counters of a lockstep batched run (sisi_ccsd_solve_lockstep). */
+typedef struct {
+  double iterations; /* lockstep iterations (the longest system's count)  */
+  double launches;   /* grouped kernel launches, all iterations            */
+  double records;    /* per-system launch records those launches covered   */
+  double plans;      /* residual schedules built (precision switches and
+                        systems leaving the batch rebuild it)             */
+} sisi_lockstep_stats;
+
+/* The batch of sisi_ccsd_solve_batched iterated in lockstep instead of on
+ * host threads (no counterpart in the reference, which runs one system per
+ * driver invocation, ClusterSinglesDoublesAlgorithm.cxx:38-128): each
+ * system's residual is recorded once per precision as a list of launches,
+ * and every launch of an iteration (residual, amplitude estimate, mixer
+ * update, norms, energy) is one grouped launch over all systems, replayed
+ * from a hipGraph while the set of systems and their precisions stay the
+ * same.  The number of launches per iteration does not grow with nsys.
+ * Per system the iteration is ClusterSinglesDoublesAlgorithm::run with the
+ * same opts; reduced precisions (bulk and coarse stages of a mixed run)
+ * compute their products on FP32 MFMA, and the report's precision says so.
+ * Requires refine_interval == 0 and max_residua <= 30.  stats may be NULL. */
+int sisi_ccsd_solve_lockstep(int nsys, const sisi_dims *dims, const double *const *gammas,
+                             const double *const *eps_i, const double *const *eps_a,
+                             const sisi_ccsd_opts *opts, double *energies,
+                             sisi_ccsd_report *reports, sisi_lockstep_stats *stats);
+
 /* Device timing of kernel families since the last reset (collected only
  * while enabled; enabling adds one event pair per launch). */
 int sisi_engine_enable_stats(sisi_engine *e, int enable);

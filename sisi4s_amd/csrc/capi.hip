@@ -638,6 +638,44 @@ int sisi_ccsd_solve_batched(int nsys, const sisi_dims *dims, const double *const
   });
 }
 
+int sisi_engine_set_ladder_planes(sisi_engine *h, int enable) {
+  return guarded([&] {
+    ENGINE_REQUIRED(h);
+    h->e.ladder_planes = enable != 0;
+    h->e.drop_residual_graph();
+  });
+}
+
+int sisi_ccsd_solve_lockstep(int nsys, const sisi_dims *dims, const double *const *gammas,
+                             const double *const *eps_i, const double *const *eps_a,
+                             const sisi_ccsd_opts *opts, double *energies,
+                             sisi_ccsd_report *reports, sisi_lockstep_stats *stats) {
+  return guarded([&] {
+    SISI_REQUIRE(nsys >= 0 && dims && gammas && eps_i && eps_a && energies, SISI_ERR_INVALID,
+                 "null argument");
+    sisi_ccsd_opts o;
+    if (opts) o = *opts;
+    else sisi_ccsd_opts_default(&o);
+    if (stats) *stats = sisi_lockstep_stats{};
+    for (int s = 0; s < nsys; ++s) {
+      SISI_REQUIRE(dims[s].No > 0 && dims[s].Nv > 0 && dims[s].NG > 0, SISI_ERR_INVALID,
+                   "system " + std::to_string(s) + ": No, Nv, NG must be positive");
+      SISI_REQUIRE(gammas[s] && eps_i[s] && eps_a[s], SISI_ERR_INVALID,
+                   "system " + std::to_string(s) + ": null input");
+      const size_t np = size_t(dims[s].No) + dims[s].Nv;
+      try {
+        check_finite(gammas[s], size_t(dims[s].NG) * np * np, "CoulombVertex");
+        check_finite(eps_i[s], size_t(dims[s].No), "HoleEigenEnergies");
+        check_finite(eps_a[s], size_t(dims[s].Nv), "ParticleEigenEnergies");
+      } catch (const Error &err) {
+        throw Error(err.code, "system " + std::to_string(s) + ": " + err.what());
+      }
+    }
+    if (nsys == 0) return;
+    ccsd_solve_lockstep(nsys, dims, gammas, eps_i, eps_a, o, energies, reports, stats);
+  });
+}
+
 int sisi_engine_enable_stats(sisi_engine *h, int enable) {
   return guarded([&] {
     ENGINE_REQUIRED(h);

@@ -270,7 +270,86 @@ void ccsd_residual(Engine &e, int it, bool initial_doubles, int slice_size, cons
       size_t free_b = 0, total_b = 0;
       keep_planes = hipMemGetInfo(&free_b, &total_b) == hipSuccess && bytes <= 0.5 * double(free_b);
     }
-    for (int m = 0; m < nslices; ++m)
+    // reduced-precision ladder with plane output: the slices are re-laid as
+    // [G, c, x] so that the V~ product's rows m = c + v x run along c, the
+    // unit-stride index of the planes' rows (coalesced bf16 stores)
+    // (only when every pair's two products are large enough for the bf16
+    // kernels, so the precision of each product is the one the stored-pair
+    // ladder would use)
+    const int pl = e.prec_ladder;
+    const int64_t smin = hi[nslices - 1] - lo[nslices - 1];
+    const bool planes = e.ladder_planes && !e.tape && !keep_planes &&
+                        (pl == SISI_BF16X3 || pl == SISI_BF16) && get_b3_backend() == 0 &&
+                        2.0 * double(smin * smin) * double(v * v) * double(std::min(G, o * o)) >=
+                            kB3MinFlops;
+    if (planes) {
+      const int64_t Kp = (v * v + 31) / 32 * 32;
+      const int64_t Mmax = (smax * smax + 255) / 256 * 256;
+      std::vector<DBuf> slt(nslices);
+      for (int s2 = 0; s2 < nslices; ++s2) {
+        const int64_t Ns = hi[s2] - lo[s2];
+        slt[s2] = DBuf(&e.pool, G * Ns * v);
+        // element (G, x, c) at G + NG c + NG v x
+        copy(e, dense_strided(slt[s2].get(), {G, Ns, v}, {1, G * v, G}),
+             subview(dGab, {0, lo[s2], 0}, {G, hi[s2], v}));
+        e.frozen.push_back(slt[s2].get());  // split into planes once, used by every pair
+      }
+      B3FrozenA fa;
+      fa.store = DBuf(&e.pool, size_t((2 * Mmax * Kp * 2 + 7) / 8));
+      // the plane columns past v^2 (k padding) are never written: zero once
+      if (Kp != v * v) zero(e, fa.store.get(), int64_t(fa.store.size()));
+      for (int m = 0; m < nslices; ++m)
+        for (int n = m; n < nslices; ++n) {  // cxx:418-451
+          const int64_t Nx = hi[n] - lo[n], Ny = hi[m] - lo[m];
+          const int64_t Mp = (Nx * Ny + 255) / 256 * 256;
+          View Rxyij = V4(Rxy.get(), Nx, Ny, o, o);
+          // V~[x,y,c,d] = sum_G slice_n[G,x,c] slice_m[G,y,d] (cxx:431-434) as
+          // plane row r = x + Nx y, column c + v d
+          GemmArgs g1;
+          g1.A = slt[n].get();
+          g1.sam = G;
+          g1.sak = 1;
+          g1.B = slt[m].get();
+          g1.sbk = 1;
+          g1.sbn = G;
+          g1.M = int(v * Nx);
+          g1.N = int(v * Ny);
+          g1.K = int(G);
+          g1.dm0 = int(v);
+          g1.dm1 = int(Nx);
+          g1.cm0 = 1;
+          g1.cm1 = Kp;
+          g1.dn0 = int(v);
+          g1.dn1 = int(Ny);
+          g1.cn0 = v;
+          g1.cn1 = Nx * Kp;
+          g1.C = fa.store.get();
+          g1.out_planes = 1;
+          g1.plane_lo = Mp * Kp;
+          gemm_b3(e, g1, pl == SISI_BF16 ? 1 : 3);
+          fa.M = int(Nx * Ny);
+          fa.K = int(v * v);
+          fa.Mp = int(Mp);
+          fa.Kp = int(Kp);
+          // R~[x,y,i,j] = sum_cd V~[x,y,c,d] X[c,d,i,j] (cxx:443)
+          GemmArgs g2;
+          g2.B = Xb.get();
+          g2.sbk = 1;
+          g2.sbn = v * v;
+          g2.M = int(Nx * Ny);
+          g2.N = int(o * o);
+          g2.K = int(v * v);
+          g2.C = Rxy.get();
+          g2.cm0 = 1;
+          g2.cn0 = Nx * Ny;
+          gemm_b3_frozen_a(e, g2, fa, pl == SISI_BF16 ? 1 : 3);
+          const int64_t a = lo[n], b = lo[m];
+          copy(e, subview(R2, {a, b, 0, 0}, {a + Nx, b + Ny, o, o}), Rxyij, 1.0, 1.0);
+          if (a > b) sum(e, 1.0, Rxyij, "xyij", 1.0, subview(R2, {b, a, 0, 0}, {b + Ny, a + Nx, o, o}), "yxji");
+        }
+      for (auto &b : slt) e.unfreeze(b.get());
+    }
+    for (int m = 0; m < nslices && !planes; ++m)
       for (int n = m; n < nslices; ++n) {  // cxx:418-451
         const int64_t Nx = hi[n] - lo[n], Ny = hi[m] - lo[m];
         View Vxycd = V4(Vxy.get(), Nx, Ny, v, v);
@@ -752,11 +831,9 @@ void ccsd_pair_energies(Engine &e, const double *T1, const double *T2, double *p
 // Driver
 // ---------------------------------------------------------------------------
 
-namespace {
-
 // Solve the bordered DIIS system (dsysv_ in DiisMixer.cxx:16-40) in FP64 by
 // Gaussian elimination with partial pivoting.
-std::vector<double> solve_small(std::vector<std::vector<double>> A, std::vector<double> b) {
+std::vector<double> diis_solve(std::vector<std::vector<double>> A, std::vector<double> b) {
   const int n = int(b.size());
   for (int c = 0; c < n; ++c) {
     int p = c;
@@ -781,6 +858,8 @@ std::vector<double> solve_small(std::vector<std::vector<double>> A, std::vector<
   }
   return x;
 }
+
+namespace {
 
 double fock_dot(Engine &e, const double *x1, const double *x2, const double *y1,
                 const double *y2) {
@@ -972,7 +1051,7 @@ void ccsd_iterate(Engine &e, int n) {
         for (int c = 0; c < dim; ++c) Bm[r][c] = e.B[r][c];
       std::vector<double> rhs(dim, 0.0);
       rhs[0] = -1.0;
-      std::vector<double> col = solve_small(Bm, rhs);
+      std::vector<double> col = diis_solve(Bm, rhs);
       DBuf n1(&e.pool, e.n1()), n2(&e.pool, e.n2());
       zero(e, n1.get(), int64_t(e.n1()));
       zero(e, n2.get(), int64_t(e.n2()));

@@ -90,6 +90,15 @@ void ccsd_residual_full(Engine &e, int iteration, bool initial_doubles, const do
 
 // Driver (ClusterSinglesDoublesAlgorithm.cxx:38-128).
 void ccsd_begin(Engine &e, const sisi_ccsd_opts &opts, const double *T1h, const double *T2h);
+// Solution of the bordered DIIS system A x = b (DiisMixer.cxx:16-40) in FP64.
+std::vector<double> diis_solve(std::vector<std::vector<double>> A, std::vector<double> b);
+// Lockstep batched CCSD of nsys systems (inputs validated by the caller):
+// every system's residual and update run in grouped launches shared by all
+// systems.
+void ccsd_solve_lockstep(int nsys, const sisi_dims *dims, const double *const *gammas,
+                         const double *const *eps_i, const double *const *eps_a,
+                         const sisi_ccsd_opts &o, double *energies, sisi_ccsd_report *reports,
+                         sisi_lockstep_stats *stats);
 void ccsd_iterate(Engine &e, int n);
 void fill_report(const Engine &e, sisi_ccsd_report *r);
 

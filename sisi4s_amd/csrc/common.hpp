@@ -221,6 +221,21 @@ inline View dense(double *p, std::initializer_list<int64_t> lens) {
   return v;
 }
 
+// View with explicit strides (element x at p[sum_k x_k * strides_k]).
+inline View dense_strided(double *p, std::initializer_list<int64_t> lens,
+                          std::initializer_list<int64_t> strides) {
+  View v;
+  v.p = p;
+  v.order = 0;
+  auto s = strides.begin();
+  for (int64_t l : lens) {
+    v.len[v.order] = l;
+    v.stride[v.order] = *s++;
+    v.order++;
+  }
+  return v;
+}
+
 // Sub-block [lo_k, hi_k) of a view (same order).
 inline View subview(const View &v, std::initializer_list<int64_t> lo,
                     std::initializer_list<int64_t> hi) {

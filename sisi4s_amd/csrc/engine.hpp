@@ -70,6 +70,8 @@ struct Anchor {
   int ladder_slice = 0;
 };
 
+struct Tape;
+
 struct DiisSlot {
   DBuf t1, t2;   // estimated amplitudes
   DBuf d1, d2;   // amplitude change
@@ -87,6 +89,10 @@ class Engine {
   Pool pool;
   std::unique_ptr<Timers> timers;
 
+  // Deferred launches (lockstep batched solver): while set, the launch
+  // helpers append records to the tape instead of launching
+  Tape *tape = nullptr;
+
   // precision of general contractions and of the particle-particle ladder
   int precision = SISI_FP64;
   int prec_ladder = SISI_FP64;
@@ -94,6 +100,11 @@ class Engine {
   // ppl = false drops the particle-particle ladder, distinguishable selects
   // the distinguishable-cluster terms
   bool ppl = true, distinguishable = false;
+  // Reduced-precision ladder (split-BF16x3 / bf16): the V~_xycd product
+  // writes the bf16 planes the X contraction reads (no FP64 V~ slice pair
+  // and no operand split of it); off: the FP64 slice pair as in the FP64
+  // ladder, split by the next product
+  bool ladder_planes = true;
 
   // Operands marked constant for the duration of one residual evaluation:
   // the contraction planner caches their permuted dense copies.

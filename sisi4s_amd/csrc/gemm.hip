@@ -23,6 +23,7 @@
 #include "engine.hpp"
 #include "gemm.hpp"
 #include "gemm_device.hpp"
+#include "tape.hpp"
 
 namespace sisi {
 
@@ -33,8 +34,12 @@ namespace sisi {
 template <typename LT>
 __device__ __forceinline__ void mfma_k4(d4 &acc, double a, double b);
 
+// Body of the FP64 / FP32 kernel for output tile `pid` (of tiles_m x
+// tiles_n), k-split `ks` and batch entry `bz` (the eager kernel passes its
+// block indices, the grouped kernel of the batched solver the indices of one
+// record's blocks).
 template <int PREC, bool AK, bool BNF>
-__global__ __launch_bounds__(256, 2) void gemm_k4_kernel(GemmArgs g) {
+__device__ __forceinline__ void gemm_k4_body(const GemmArgs &g, int pid, int ks, int bz) {
   using LT = typename std::conditional<PREC == SISI_FP64, double, float>::type;
   using AT = typename std::conditional<PREC == SISI_FP64, d4, f4>::type;
   constexpr int BK = 16;
@@ -50,9 +55,7 @@ __global__ __launch_bounds__(256, 2) void gemm_k4_kernel(GemmArgs g) {
   const int lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: scalar branches
   int tm, tn;
-  tile_coords(g, blockIdx.x, tm, tn);
-  const int ks = blockIdx.y;
-  const int bz = blockIdx.z;
+  tile_coords(g, pid, tm, tn);
   const int m0 = tm * BM, n0 = tn * BN;
   const int kbeg = ks * g.kchunk;
   const int kend = min(g.K, kbeg + g.kchunk);
@@ -217,6 +220,29 @@ __global__ __launch_bounds__(256, 2) void gemm_k4_kernel(GemmArgs g) {
         }
     }
   }
+}
+
+template <int PREC, bool AK, bool BNF>
+__global__ __launch_bounds__(256, 2) void gemm_k4_kernel(GemmArgs g) {
+  gemm_k4_body<PREC, AK, BNF>(g, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// Grouped launch of n products (batched solver): record p owns the blocks
+// [start[p], start[p + 1]), tiles_m * tiles_n per batch entry, no split-K.
+template <int PREC, bool AK, bool BNF>
+__global__ __launch_bounds__(256, 2) void gemm_k4_grouped(const GemmArgs *const *args,
+                                                          const unsigned *__restrict__ start,
+                                                          int n) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= blockIdx.x) lo = mid;
+    else hi = mid - 1;
+  }
+  const GemmArgs g = *args[lo];
+  const int tiles = g.tiles_m * g.tiles_n;
+  const int local = int(blockIdx.x - start[lo]);
+  gemm_k4_body<PREC, AK, BNF>(g, local % tiles, 0, local / tiles);
 }
 
 // ---------------------------------------------------------------------------
@@ -572,8 +598,54 @@ static void launch_reduce(const GemmArgs &g, hipStream_t s) {
   SISI_HIP(hipGetLastError());
 }
 
+// Sub-kind of a taped product: precision (0 FP64, 1 FP32) * 4 + AK * 2 + BNF.
+static int tape_gemm_sub(int precision, bool AKf, bool BNFf) {
+  return (precision == SISI_FP64 ? 0 : 4) + (AKf ? 2 : 0) + (BNFf ? 1 : 0);
+}
+
+void launch_gemm_grouped(int sub, const GemmArgs *const *args, const unsigned *start, int n,
+                         unsigned total, hipStream_t s) {
+  switch (sub) {
+#define SISI_GROUPED_CASE(S, P, A, B)                                                            \
+  case S:                                                                                      \
+    hipLaunchKernelGGL((gemm_k4_grouped<P, A, B>), dim3(total), dim3(256), 0, s, args, start, n); \
+    break;
+    SISI_GROUPED_CASE(0, SISI_FP64, false, false)
+    SISI_GROUPED_CASE(1, SISI_FP64, false, true)
+    SISI_GROUPED_CASE(2, SISI_FP64, true, false)
+    SISI_GROUPED_CASE(3, SISI_FP64, true, true)
+    SISI_GROUPED_CASE(4, SISI_FP32, false, false)
+    SISI_GROUPED_CASE(5, SISI_FP32, false, true)
+    SISI_GROUPED_CASE(6, SISI_FP32, true, false)
+    SISI_GROUPED_CASE(7, SISI_FP32, true, true)
+#undef SISI_GROUPED_CASE
+    default: throw Error(SISI_ERR_INVALID, "grouped gemm: unknown sub-kind");
+  }
+  SISI_HIP(hipGetLastError());
+}
+
+// A product on a taped engine: recorded for the grouped launch, without
+// split-K (the systems of the group fill the device together) and with the
+// reduced precisions on FP32 MFMA.
+static void tape_gemm(Engine &eng, int precision, GemmArgs g) {
+  if (precision != SISI_FP64) precision = SISI_FP32;
+  bool AKf, BNFf;
+  plan_k4(g, precision, eng.cus, AKf, BNFf);
+  set_ksplit(g, 1, 16);
+  SISI_REQUIRE(int64_t(g.tiles_m) * g.tiles_n * g.batch < (int64_t(1) << 31), SISI_ERR_INVALID,
+               "gemm: too many tiles");
+  TapeOp op;
+  op.kind = OpKind::Gemm;
+  op.sub = tape_gemm_sub(precision, AKf, BNFf);
+  op.gx = unsigned(g.tiles_m * g.tiles_n);
+  op.gz = unsigned(g.batch);
+  op.g = g;
+  eng.tape->ops.push_back(op);
+}
+
 void gemm(Engine &eng, int precision, GemmArgs g, bool exact) {
   if (g.M <= 0 || g.N <= 0 || g.batch <= 0) return;
+  if (eng.tape) return tape_gemm(eng, precision, g);
   // split-BF16x3 splits both operands first (a bandwidth pass over each); a
   // product too small to amortise that runs on FP32 MFMA instead
   if ((precision == SISI_BF16X3 || precision == SISI_BF16) && !exact &&

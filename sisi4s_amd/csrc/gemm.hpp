@@ -38,6 +38,13 @@ struct GemmArgs {
   int xcd_map = 1;    // 1: give each of the 8 XCDs a contiguous tile range
   int group = 8;      // m-tiles per group in the tile walk
   double alpha = 1.0, beta = 0.0;
+  // split-BF16x3 / bf16 kernels only: out_planes = 1 writes the result as
+  // bf16 hi/lo planes instead of FP64 (C is then the hi plane as bf16, the
+  // output offsets count bf16 elements, lo = hi + plane_lo; beta must be 0):
+  // hi = bf16(x), lo = bf16(x - hi) of x = float(alpha * acc), the split an
+  // operand split of the FP64 result would give
+  int out_planes = 0;
+  int64_t plane_lo = 0;
 };
 
 // One product of a segmented GEMM: A(m,k) = A[m*sam + k*sak],

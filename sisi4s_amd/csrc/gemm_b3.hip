@@ -43,6 +43,23 @@ __device__ __forceinline__ u16 bf16_bits(float f) {
   return *reinterpret_cast<u16 *>(&b);
 }
 
+// Epilogue store of one output element: FP64 (alpha, beta), or the bf16
+// hi/lo planes of float(alpha * acc) (out_planes).
+__device__ __forceinline__ void b3_store(const GemmArgs &g, double *C, int64_t o, float acc) {
+  if (g.out_planes) {
+    const float x = float(g.alpha * double(acc));
+    const u16 h = bf16_bits(x);
+    const __bf16 hb = *reinterpret_cast<const __bf16 *>(&h);
+    u16 *P = reinterpret_cast<u16 *>(C);
+    P[o] = h;
+    P[o + g.plane_lo] = bf16_bits(x - float(hb));
+  } else {
+    double v = g.alpha * double(acc);
+    if (g.beta != 0.0) v += g.beta * C[o];
+    C[o] = v;
+  }
+}
+
 // 64 rows x 64 k per block: read along the unit-stride index into LDS, then
 // each thread converts 16 consecutive k of one row.
 __global__ __launch_bounds__(256) void split_bf16_kernel(SplitArgs a) {
@@ -252,12 +269,7 @@ __global__ __launch_bounds__(256, MINB) void gemm_b3p_kernel(B3Args p) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int n = n0 + wn + j * 16 + 4 * lk + r;
-          if (n < g.N) {
-            const int64_t o = om + b3_off_n(g, n);
-            double v = g.alpha * double(acc[i][j][r]);
-            if (g.beta != 0.0) v += g.beta * C[o];
-            C[o] = v;
-          }
+          if (n < g.N) b3_store(g, C, om + b3_off_n(g, n), acc[i][j][r]);
         }
     }
   } else {
@@ -334,7 +346,7 @@ constexpr int b3d_lds_bytes() {
   return STAGES * (PASSES == 3 ? 2 : 1) * (BM + BN) * 64;
 }
 
-template <int BM, int BN, int WGM, int WGN, int PASSES, int STAGES, int SWZ>
+template <int BM, int BN, int WGM, int WGN, int PASSES, int STAGES, int SWZ, int SCHED>
 __global__ __launch_bounds__(64 * WGM * WGN, 1) void gemm_b3d_kernel(B3Args p) {
   constexpr int NW = WGM * WGN;
   constexpr int WM = BM / WGM, WN = BN / WGN, TI = WM / 32, TJ = WN / 32;
@@ -408,36 +420,60 @@ __global__ __launch_bounds__(64 * WGM * WGN, 1) void gemm_b3d_kernel(B3Args p) {
   // byte offset of (row, 16-byte chunk c) in a plane
   auto at = [](int row, int c) { return row * 64 + ((c ^ ((row >> SWZ) & 3)) << 4); };
 
+  // The fragments of both k16 steps of a chunk are read from LDS before its
+  // MFMAs, so the reads of the second step are in flight while the first
+  // step's MFMAs issue (the compiler waits only for the set it uses).
   auto compute = [&](int slot) {
     const unsigned char *st = smem + slot * STAGE_BYTES;
+    bf8v bh[2][TJ], bl[2][PASSES == 3 ? TJ : 1], ah[2][TI], al[2][PASSES == 3 ? TI : 1];
 #pragma unroll
     for (int kk = 0; kk < 2; ++kk) {
       const int c = 2 * kk + lh;
-      bf8v bh[TJ], bl[TJ];
 #pragma unroll
       for (int j = 0; j < TJ; ++j) {
         const int o = at(wn + 32 * j + l32, c);
-        bh[j] = *reinterpret_cast<const bf8v *>(st + PL * A_BYTES + o);
+        bh[kk][j] = *reinterpret_cast<const bf8v *>(st + PL * A_BYTES + o);
         if constexpr (PASSES == 3)
-          bl[j] = *reinterpret_cast<const bf8v *>(st + PL * A_BYTES + B_BYTES + o);
+          bl[kk][j] = *reinterpret_cast<const bf8v *>(st + PL * A_BYTES + B_BYTES + o);
       }
 #pragma unroll
       for (int i = 0; i < TI; ++i) {
         const int o = at(wm + 32 * i + l32, c);
-        const bf8v ah = *reinterpret_cast<const bf8v *>(st + o);
+        ah[kk][i] = *reinterpret_cast<const bf8v *>(st + o);
+        if constexpr (PASSES == 3) al[kk][i] = *reinterpret_cast<const bf8v *>(st + A_BYTES + o);
+      }
+    }
+#pragma unroll
+    for (int kk = 0; kk < 2; ++kk)
+#pragma unroll
+      for (int i = 0; i < TI; ++i) {
         if constexpr (PASSES == 3) {
-          const bf8v al = *reinterpret_cast<const bf8v *>(st + A_BYTES + o);
 #pragma unroll
           for (int j = 0; j < TJ; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bl[j], ah, acc[i][j], 0, 0, 0);
+            acc[i][j] =
+                __builtin_amdgcn_mfma_f32_32x32x16_bf16(bl[kk][j], ah[kk][i], acc[i][j], 0, 0, 0);
 #pragma unroll
           for (int j = 0; j < TJ; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bh[j], al, acc[i][j], 0, 0, 0);
+            acc[i][j] =
+                __builtin_amdgcn_mfma_f32_32x32x16_bf16(bh[kk][j], al[kk][i], acc[i][j], 0, 0, 0);
         }
 #pragma unroll
         for (int j = 0; j < TJ; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bh[j], ah, acc[i][j], 0, 0, 0);
+          acc[i][j] =
+              __builtin_amdgcn_mfma_f32_32x32x16_bf16(bh[kk][j], ah[kk][i], acc[i][j], 0, 0, 0);
       }
+    // schedule: the first step's reads, then the second step's reads
+    // interleaved one to one with the first step's MFMAs, then the rest
+    // (SCHED = 1; SCHED = 0 leaves the order to the compiler)
+    if constexpr (SCHED == 1) {
+      constexpr int R = (TI + TJ) * (PASSES == 3 ? 2 : 1), Q = TI * TJ * PASSES;
+      __builtin_amdgcn_sched_group_barrier(0x100, R, 0);
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+      }
+      __builtin_amdgcn_sched_group_barrier(0x008, 2 * Q - R, 0);
     }
   };
 
@@ -474,12 +510,7 @@ __global__ __launch_bounds__(64 * WGM * WGN, 1) void gemm_b3d_kernel(B3Args p) {
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
           const int n = n0 + wn + 32 * j + 8 * (r >> 2) + 4 * lh + (r & 3);
-          if (n < g.N) {
-            const int64_t o = om + b3_off_n(g, n);
-            double v = g.alpha * double(acc[i][j][r]);
-            if (g.beta != 0.0) v += g.beta * C[o];
-            C[o] = v;
-          }
+          if (n < g.N) b3_store(g, C, om + b3_off_n(g, n), acc[i][j][r]);
         }
     }
   } else {
@@ -782,15 +813,16 @@ static void launch_b3p(const B3Args &p, dim3 grid, hipStream_t s) {
                      p);
 }
 
-template <int BM, int BN, int WGM, int WGN, int PASSES, int STAGES, int SWZ>
+template <int BM, int BN, int WGM, int WGN, int PASSES, int STAGES, int SWZ, int SCHED = 0>
 static void launch_b3d(const B3Args &p, dim3 grid, hipStream_t s) {
   constexpr int lds = b3d_lds_bytes<BM, BN, PASSES, STAGES>();
   static_assert(lds <= 160 * 1024, "LDS of one CU");
   static const hipError_t attr = hipFuncSetAttribute(
-      reinterpret_cast<const void *>(&gemm_b3d_kernel<BM, BN, WGM, WGN, PASSES, STAGES, SWZ>),
+      reinterpret_cast<const void *>(
+          &gemm_b3d_kernel<BM, BN, WGM, WGN, PASSES, STAGES, SWZ, SCHED>),
       hipFuncAttributeMaxDynamicSharedMemorySize, lds);
   SISI_HIP(attr);
-  hipLaunchKernelGGL((gemm_b3d_kernel<BM, BN, WGM, WGN, PASSES, STAGES, SWZ>), grid,
+  hipLaunchKernelGGL((gemm_b3d_kernel<BM, BN, WGM, WGN, PASSES, STAGES, SWZ, SCHED>), grid,
                      dim3(64 * WGM * WGN), lds, s, p);
 }
 
@@ -814,6 +846,7 @@ static const B3Variant kB3Variants[] = {
     {256, 256, launch_b3d<256, 256, 2, 4, 3, 2, 1>, "dma256x256_w8_s2_x1"},
     {256, 128, launch_b3d<256, 128, 2, 2, 3, 3, 2>, "dma256x128_w4_s3_x2"},
     {256, 256, launch_b3d<256, 256, 2, 2, 3, 2, 2>, "dma256x256_w4_s2_x2"},
+    {256, 256, launch_b3d<256, 256, 2, 4, 3, 2, 2, 1>, "dma256x256_w8_s2_x2_il"},
 };
 static const B3Variant kB1Variants[] = {
     {128, 128, launch_b3p<128, 128, 64, 64, 1, 1, 1>, "lds128x128"},
@@ -822,6 +855,7 @@ static const B3Variant kB1Variants[] = {
     {256, 256, launch_b3d<256, 256, 2, 4, 1, 4, 1>, "dma256x256_w8_s4_x1"},
     {256, 128, launch_b3d<256, 128, 2, 2, 1, 4, 2>, "dma256x128_w4_s4_x2"},
     {256, 256, launch_b3d<256, 256, 2, 2, 1, 4, 2>, "dma256x256_w4_s4_x2"},
+    {256, 256, launch_b3d<256, 256, 2, 4, 1, 4, 2, 1>, "dma256x256_w8_s4_x2_il"},
 };
 constexpr int kNumB3Variants = int(sizeof(kB3Variants) / sizeof(kB3Variants[0]));
 constexpr int kNumB1Variants = int(sizeof(kB1Variants) / sizeof(kB1Variants[0]));
@@ -867,7 +901,7 @@ static void launch_variant(Engine &e, GemmArgs g, const B3Variant &v, int xcd, c
   g.split_out = !(g.dm0 >= g.M && g.dn0 >= g.N);
   int ksplit = 1;
   const int64_t fill = v.bm >= 256 ? 256 : 512;  // workgroups that fill the chip
-  if (Kp > 4 * 32 && tiles < fill) {
+  if (Kp > 4 * 32 && tiles < fill && !g.out_planes) {
     const int64_t want = (fill + tiles - 1) / tiles;
     const int64_t smax = std::max<int64_t>(1, std::min<int64_t>(Kp / 256, 64));
     ksplit = round_filling_split(tiles, std::min(want, smax), smax, e.cus);
@@ -908,6 +942,7 @@ static TuneChoice tune_b3(Engine &e, const GemmArgs &g0, int passes, const Plane
   g.dm1 = g.dn1 = 1;
   g.alpha = 1.0;
   g.beta = 0.0;
+  g.out_planes = 0;
   const bool stats = e.timers->enabled;
   e.timers->enabled = false;
   hipEvent_t t0, t1;
@@ -950,7 +985,9 @@ static void run_b3(Engine &e, GemmArgs g, int passes, const B3FrozenA *fa = null
   const int Kp = ((g.K + 31) / 32) * 32;
   const int Mp = ((g.M + kB3RowPad - 1) / kB3RowPad) * kB3RowPad;
   const int Np = ((g.N + kB3RowPad - 1) / kB3RowPad) * kB3RowPad;
-  if (passes == 1 && fa && (b3_use_library() || b1_use_library())) {
+  SISI_REQUIRE(!g.out_planes || (g.beta == 0.0 && g.batch == 1), SISI_ERR_INVALID,
+               "gemm_b3: plane output needs beta 0 and batch 1");
+  if (passes == 1 && fa && (b3_use_library() || b1_use_library()) && !g.out_planes) {
     // frozen hi plane: rows of Kp bf16, the first M rows used
     g.split_out = !(g.dm0 >= g.M && g.dn0 >= g.N);
     Planes la, lb;
@@ -960,7 +997,7 @@ static void run_b3(Engine &e, GemmArgs g, int passes, const B3FrozenA *fa = null
     const Planes &B = split_operand(e, g.B + 0, g.sbn, g.sbk, g.N, g.K, g.N, Kp, lb, 0);
     if (b3_library(e, g, la, B, Kp, 1)) return;
   }
-  if ((b3_use_library() || (passes == 1 && b1_use_library())) && !fa) {
+  if ((b3_use_library() || (passes == 1 && b1_use_library())) && !fa && !g.out_planes) {
     g.split_out = !(g.dm0 >= g.M && g.dn0 >= g.N);
     Planes la, lb;
     const int lay_a = passes == 3 ? 1 : 0, lay_b = passes == 3 ? 2 : 0;

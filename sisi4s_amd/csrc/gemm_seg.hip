@@ -14,6 +14,7 @@
 #include "engine.hpp"
 #include "gemm.hpp"
 #include "gemm_device.hpp"
+#include "tune.hpp"
 
 namespace sisi {
 
@@ -27,7 +28,12 @@ __device__ __forceinline__ int seg_k(const SegGemmArgs &a, int s) {
   return s == 0 ? a.seg[0].K : s == 1 ? a.seg[1].K : s == 2 ? a.seg[2].K : a.seg[3].K;
 }
 
-template <int PREC>
+// TBM x TBN block tile of WGM x WGN waves (each (TBM/WGM) x (TBN/WGN), a
+// multiple of 16 in both directions: 128 x 128 as 2 x 2 waves of 64 x 64,
+// and 80 x 128 / 128 x 80 as 1 x 4 / 4 x 1 waves of 80 x 32 / 32 x 80 for
+// an output dimension of a multiple of 80 that 128-wide tiles would leave
+// ragged, such as the 400 virtuals of the (T) blocks).
+template <int PREC, int TBM, int TBN, int WGM, int WGN>
 __global__ __launch_bounds__(256, 2) void gemm_seg_k4_kernel(SegGemmArgs a) {
   using LT = typename std::conditional<PREC == SISI_FP64, double, float>::type;
   using AT = typename std::conditional<PREC == SISI_FP64, d4, f4>::type;
@@ -35,8 +41,12 @@ __global__ __launch_bounds__(256, 2) void gemm_seg_k4_kernel(SegGemmArgs a) {
   constexpr int PAD = (PREC == SISI_FP64) ? 2 : 4;
   constexpr int LSTR = (PREC == SISI_FP64) ? 1 : 4;
   constexpr int RSTR = (PREC == SISI_FP64) ? 4 : 1;
-  __shared__ LT As[BK][BM + PAD];
-  __shared__ LT Bs[BK][BN + PAD];
+  constexpr int WM = TBM / WGM, WN = TBN / WGN, TI = WM / 16, TJ = WN / 16;
+  constexpr int NA = TBM * BK / 256, NB = TBN * BK / 256;  // loads per thread
+  static_assert(WGM * WGN == 4 && TI * 16 == WM && TJ * 16 == WN, "four waves, 16-multiples");
+  static_assert(NA * 256 == TBM * BK && NB * 256 == TBN * BK, "whole loads per thread");
+  __shared__ LT As[BK][TBM + PAD];
+  __shared__ LT Bs[BK][TBN + PAD];
   const GemmArgs &g = a.g;
 
   const int tid = threadIdx.x;
@@ -44,14 +54,16 @@ __global__ __launch_bounds__(256, 2) void gemm_seg_k4_kernel(SegGemmArgs a) {
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: scalar branches
   int tm, tn;
   tile_coords(g, blockIdx.x, tm, tn);
-  const int m0 = tm * BM, n0 = tn * BN;
+  const int m0 = tm * TBM, n0 = tn * TBN;
 
-  double ra[8], rb[8];
+  double ra[NA], rb[NB];
   int lak = 0, lbnf = 0;
   // Out-of-range elements load from a clamped (in-range) address, so the
   // loads need no branch; at the LDS store they are multiplied by zero (the
   // segment's scale otherwise), one chunk later, so the loads stay in flight
-  // across the MFMAs of the previous chunk.
+  // across the MFMAs of the previous chunk.  Element q of a thread is
+  // e = tid + 256 q of the tile, walked along the operand's unit-stride
+  // index (m or n fastest: e % TB, e / TB; k fastest: e / 16, e % 16).
   const int mlast = g.M - 1, nlast = g.N - 1;
   int lk0 = 0, lK = 0;
   double lsc = 1.0;
@@ -64,82 +76,44 @@ __global__ __launch_bounds__(256, 2) void gemm_seg_k4_kernel(SegGemmArgs a) {
     lk0 = k0;
     lK = sg.K;
     lsc = sg.scale;
-    if (!sg.ak) {  // A m-fastest: 32 threads x 4 consecutive m, k rows t/32 and +8
-      const int mm = (tid & 31) * 4, kk = tid >> 5;
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int k = min(k0 + kk + 8 * h, klast);
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-          ra[h * 4 + j] = sg.A[int64_t(min(m0 + mm + j, mlast)) * sg.sam + int64_t(k) * sg.sak];
-      }
-    } else {  // A k-fastest: 16 threads along k, m = t/16 + 16 j
-      const int kk = tid & 15, mm = tid >> 4;
-      const int k = min(k0 + kk, klast);
-#pragma unroll
-      for (int j = 0; j < 8; ++j)
-        ra[j] = sg.A[int64_t(min(m0 + mm + 16 * j, mlast)) * sg.sam + int64_t(k) * sg.sak];
+    for (int q = 0; q < NA; ++q) {
+      const int e = tid + 256 * q;
+      const int mm = sg.ak ? e / BK : e % TBM, kk = sg.ak ? e % BK : e / TBM;
+      ra[q] = sg.A[int64_t(min(m0 + mm, mlast)) * sg.sam + int64_t(min(k0 + kk, klast)) * sg.sak];
     }
-    if (sg.bnf) {  // B n-fastest
-      const int nn = (tid & 31) * 4, kk = tid >> 5;
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int k = min(k0 + kk + 8 * h, klast);
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-          rb[h * 4 + j] = sg.B[int64_t(k) * sg.sbk + int64_t(min(n0 + nn + j, nlast)) * sg.sbn];
-      }
-    } else {  // B k-fastest
-      const int kk = tid & 15, nn = tid >> 4;
-      const int k = min(k0 + kk, klast);
-#pragma unroll
-      for (int j = 0; j < 8; ++j)
-        rb[j] = sg.B[int64_t(k) * sg.sbk + int64_t(min(n0 + nn + 16 * j, nlast)) * sg.sbn];
+    for (int q = 0; q < NB; ++q) {
+      const int e = tid + 256 * q;
+      const int nn = sg.bnf ? e % TBN : e / BK, kk = sg.bnf ? e / TBN : e % BK;
+      rb[q] = sg.B[int64_t(min(k0 + kk, klast)) * sg.sbk + int64_t(min(n0 + nn, nlast)) * sg.sbn];
     }
   };
 
   auto store = [&]() {
-    auto za = [&](bool ok) { return ok ? lsc : 0.0; };
-    auto zb = [](bool ok) { return ok ? 1.0 : 0.0; };
-    if (!lak) {
-      const int mm = (tid & 31) * 4, kk = tid >> 5;
 #pragma unroll
-      for (int h = 0; h < 2; ++h)
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-          As[kk + 8 * h][mm + j] =
-              LT(ra[h * 4 + j] * za(m0 + mm + j <= mlast && lk0 + kk + 8 * h < lK));
-    } else {
-      const int kk = tid & 15, mm = tid >> 4;
-#pragma unroll
-      for (int j = 0; j < 8; ++j)
-        As[kk][mm + 16 * j] = LT(ra[j] * za(m0 + mm + 16 * j <= mlast && lk0 + kk < lK));
+    for (int q = 0; q < NA; ++q) {
+      const int e = tid + 256 * q;
+      const int mm = lak ? e / BK : e % TBM, kk = lak ? e % BK : e / TBM;
+      As[kk][mm] = LT(ra[q] * ((m0 + mm <= mlast && lk0 + kk < lK) ? lsc : 0.0));
     }
-    if (lbnf) {
-      const int nn = (tid & 31) * 4, kk = tid >> 5;
 #pragma unroll
-      for (int h = 0; h < 2; ++h)
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-          Bs[kk + 8 * h][nn + j] =
-              LT(rb[h * 4 + j] * zb(n0 + nn + j <= nlast && lk0 + kk + 8 * h < lK));
-    } else {
-      const int kk = tid & 15, nn = tid >> 4;
-#pragma unroll
-      for (int j = 0; j < 8; ++j)
-        Bs[kk][nn + 16 * j] = LT(rb[j] * zb(n0 + nn + 16 * j <= nlast && lk0 + kk < lK));
+    for (int q = 0; q < NB; ++q) {
+      const int e = tid + 256 * q;
+      const int nn = lbnf ? e % TBN : e / BK, kk = lbnf ? e / TBN : e % BK;
+      Bs[kk][nn] = LT(rb[q] * ((n0 + nn <= nlast && lk0 + kk < lK) ? 1.0 : 0.0));
     }
   };
 
-  AT acc[4][4];
+  AT acc[TI][TJ];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < TI; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = AT{0, 0, 0, 0};
+    for (int j = 0; j < TJ; ++j) acc[i][j] = AT{0, 0, 0, 0};
 
-  const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+  const int wm = (wave % WGM) * WM, wn = (wave / WGM) * WN;
   const int l16 = lane & 15, lk = lane >> 4;
-  // a wave whose 64x64 sub-tile lies wholly outside the output skips its MFMAs
+  // a wave whose sub-tile lies wholly outside the output skips its MFMAs
   const bool active = (m0 + wm < g.M) && (n0 + wn < g.N);
 
   int s = 0, k0 = 0;
@@ -159,15 +133,15 @@ __global__ __launch_bounds__(256, 2) void gemm_seg_k4_kernel(SegGemmArgs a) {
 #pragma unroll
       for (int kq = 0; kq < BK / 4; ++kq) {
         const int k = kq * 4 + lk;
-        LT av[4], bv[4];
+        LT av[TI], bv[TJ];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) av[i] = As[k][wm + i * 16 + l16];
+        for (int i = 0; i < TI; ++i) av[i] = As[k][wm + i * 16 + l16];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) bv[j] = Bs[k][wn + j * 16 + l16];
+        for (int j = 0; j < TJ; ++j) bv[j] = Bs[k][wn + j * 16 + l16];
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+        for (int i = 0; i < TI; ++i)
 #pragma unroll
-          for (int j = 0; j < 4; ++j) {
+          for (int j = 0; j < TJ; ++j) {
             if constexpr (PREC == SISI_FP64)
               acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j], av[i], acc[i][j], 0, 0, 0);
             else
@@ -182,12 +156,12 @@ __global__ __launch_bounds__(256, 2) void gemm_seg_k4_kernel(SegGemmArgs a) {
   }
 
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
+  for (int i = 0; i < TI; ++i) {
     const int m = m0 + wm + i * 16 + l16;
     if (m >= g.M) continue;
     const int64_t om = out_off_m(g, m);
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < TJ; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int n = n0 + wn + j * 16 + LSTR * lk + RSTR * r;
@@ -392,6 +366,108 @@ __global__ __launch_bounds__(256, 2) void gemm_seg_b3_kernel(SegGemmArgs a) {
 
 }  // namespace
 
+// Tile variants of the FP64 / FP32 segmented kernel (ids of the tuning
+// cache's family 64 / 32): 128 x 128, 80 x 128, 128 x 80.
+struct SegVariant {
+  int bm, bn;
+  void (*launch)(const SegGemmArgs &, dim3, hipStream_t);
+  const char *name;
+};
+
+template <int PREC, int TBM, int TBN, int WGM, int WGN>
+static void launch_seg(const SegGemmArgs &a, dim3 grid, hipStream_t s) {
+  hipLaunchKernelGGL((gemm_seg_k4_kernel<PREC, TBM, TBN, WGM, WGN>), grid, dim3(256), 0, s, a);
+}
+
+static const SegVariant kSeg64[] = {
+    {128, 128, launch_seg<SISI_FP64, 128, 128, 2, 2>, "seg128x128"},
+    {80, 128, launch_seg<SISI_FP64, 80, 128, 1, 4>, "seg80x128"},
+    {128, 80, launch_seg<SISI_FP64, 128, 80, 4, 1>, "seg128x80"},
+};
+static const SegVariant kSeg32[] = {
+    {128, 128, launch_seg<SISI_FP32, 128, 128, 2, 2>, "seg128x128"},
+    {80, 128, launch_seg<SISI_FP32, 80, 128, 1, 4>, "seg80x128"},
+    {128, 80, launch_seg<SISI_FP32, 128, 80, 4, 1>, "seg128x80"},
+};
+constexpr int kNumSeg = 3;
+
+// Size rule when no tuning record exists: an 80-wide tile along a dimension
+// that is a multiple of 80 but not of 128, when that leaves the other at 128.
+static int default_seg_variant(const GemmArgs &g) {
+  const bool m80 = g.M % 80 == 0 && g.M % 128 != 0, n80 = g.N % 80 == 0 && g.N % 128 != 0;
+  if (m80 && !n80) return 1;
+  if (n80 && !m80) return 2;
+  return 0;
+}
+
+static void launch_seg_variant(Engine &eng, SegGemmArgs a, int precision, const SegVariant &v,
+                               int xcd, double flops) {
+  GemmArgs &g = a.g;
+  g.tiles_m = (g.M + v.bm - 1) / v.bm;
+  g.tiles_n = (g.N + v.bn - 1) / v.bn;
+  g.xcd_map = xcd;
+  const int64_t tiles = int64_t(g.tiles_m) * g.tiles_n;
+  SISI_REQUIRE(tiles < (int64_t(1) << 31), SISI_ERR_INVALID, "gemm_segments: too many tiles");
+  const Kind kind = precision == SISI_FP64 ? Kind::Gemm64 : Kind::Gemm32;
+  Timed t(*eng.timers, kind, flops);
+  v.launch(a, dim3(unsigned(tiles), 1u, 1u), eng.stream);
+  SISI_HIP(hipGetLastError());
+}
+
+// Time every tile variant (both tile walks) of one segmented product class
+// into a scratch output; the fastest is recorded for the class.
+static TuneChoice tune_seg(Engine &eng, const SegGemmArgs &a0, int precision, const TuneKey &key,
+                           double flops) {
+  const SegVariant *table = precision == SISI_FP64 ? kSeg64 : kSeg32;
+  SegGemmArgs a = a0;
+  GemmArgs &g = a.g;
+  DBuf scratch(&eng.pool, size_t(g.M) * g.N);
+  g.C = scratch.get();
+  g.cm0 = 1;
+  g.cn0 = g.M;
+  g.cm1 = g.cm2 = g.cn1 = g.cn2 = 0;
+  g.dm0 = g.M;
+  g.dn0 = g.N;
+  g.dm1 = g.dn1 = 1;
+  g.split_out = 0;
+  g.alpha = 1.0;
+  g.beta = 0.0;
+  const bool stats = eng.timers->enabled;
+  eng.timers->enabled = false;
+  hipEvent_t t0, t1;
+  SISI_HIP(hipEventCreate(&t0));
+  SISI_HIP(hipEventCreate(&t1));
+  TuneChoice best;
+  double best_us = 1e300;
+  for (int v = 0; v < kNumSeg; ++v)
+    for (int xcd = 1; xcd >= 0; --xcd) {
+      launch_seg_variant(eng, a, precision, table[v], xcd, flops);
+      SISI_HIP(hipEventRecord(t0, eng.stream));
+      for (int r = 0; r < 3; ++r) launch_seg_variant(eng, a, precision, table[v], xcd, flops);
+      SISI_HIP(hipEventRecord(t1, eng.stream));
+      SISI_HIP(hipEventSynchronize(t1));
+      float ms = 0.f;
+      SISI_HIP(hipEventElapsedTime(&ms, t0, t1));
+      const double us = 1e3 * double(ms) / 3;
+      if (us < best_us) {
+        best_us = us;
+        best.variant = v;
+        best.xcd = xcd;
+        best.us = us;
+      }
+    }
+  SISI_HIP(hipEventDestroy(t0));
+  SISI_HIP(hipEventDestroy(t1));
+  eng.timers->enabled = stats;
+  tune_record(key, best, table[best.variant].name);
+  return best;
+}
+
+static bool capturing(hipStream_t s) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  return hipStreamIsCapturing(s, &st) == hipSuccess && st != hipStreamCaptureStatusNone;
+}
+
 void gemm_segments(Engine &eng, int precision, SegGemmArgs a) {
   GemmArgs &g = a.g;
   if (g.M <= 0 || g.N <= 0) return;
@@ -400,6 +476,7 @@ void gemm_segments(Engine &eng, int precision, SegGemmArgs a) {
   SISI_REQUIRE(precision == SISI_FP64 || precision == SISI_FP32 || precision == SISI_BF16X3,
                SISI_ERR_INVALID, "gemm_segments: unknown precision");
   double flops = 0.0;
+  int64_t ktot = 0;
   for (int s = 0; s < a.nseg; ++s) {
     GemmSeg &sg = a.seg[s];
     SISI_REQUIRE(sg.K > 0 && sg.A && sg.B, SISI_ERR_INVALID,
@@ -408,32 +485,37 @@ void gemm_segments(Engine &eng, int precision, SegGemmArgs a) {
     sg.ak = (sg.sak == 1 && sg.sam != 1) ? 1 : 0;
     sg.bnf = (sg.sbn == 1 && sg.sbk != 1) ? 1 : 0;
     flops += 2.0 * double(g.M) * double(g.N) * double(sg.K);
+    ktot += sg.K;
   }
   g.batch = 1;
   g.ksplit = 1;
   g.kchunk = 0;
-  g.tiles_m = (g.M + BM - 1) / BM;
-  g.tiles_n = (g.N + BN - 1) / BN;
-  const int64_t tiles = int64_t(g.tiles_m) * g.tiles_n;
-  SISI_REQUIRE(tiles < (int64_t(1) << 31), SISI_ERR_INVALID, "gemm_segments: too many tiles");
   g.split_out = !(g.dm0 >= g.M && g.dn0 >= g.N);
   {
-    static const int env_map = getenv("SISI_GEMM_XCD") ? atoi(getenv("SISI_GEMM_XCD")) : 1;
     static const int env_group = getenv("SISI_GEMM_GROUP") ? atoi(getenv("SISI_GEMM_GROUP")) : 8;
-    g.xcd_map = env_map;
     g.group = env_group > 0 ? env_group : 8;
   }
-  const Kind kind = precision == SISI_FP64 ? Kind::Gemm64
-                    : precision == SISI_FP32 ? Kind::Gemm32 : Kind::GemmB3;
-  Timed t(*eng.timers, kind, flops);
-  const dim3 grid{unsigned(tiles), 1u, 1u};
-  if (precision == SISI_FP64)
-    hipLaunchKernelGGL(gemm_seg_k4_kernel<SISI_FP64>, grid, dim3(256), 0, eng.stream, a);
-  else if (precision == SISI_FP32)
-    hipLaunchKernelGGL(gemm_seg_k4_kernel<SISI_FP32>, grid, dim3(256), 0, eng.stream, a);
-  else
-    hipLaunchKernelGGL(gemm_seg_b3_kernel, grid, dim3(256), 0, eng.stream, a);
-  SISI_HIP(hipGetLastError());
+  if (precision == SISI_BF16X3) {
+    g.tiles_m = (g.M + BM - 1) / BM;
+    g.tiles_n = (g.N + BN - 1) / BN;
+    g.xcd_map = 1;
+    const int64_t tiles = int64_t(g.tiles_m) * g.tiles_n;
+    SISI_REQUIRE(tiles < (int64_t(1) << 31), SISI_ERR_INVALID, "gemm_segments: too many tiles");
+    Timed t(*eng.timers, Kind::GemmB3, flops);
+    hipLaunchKernelGGL(gemm_seg_b3_kernel, dim3(unsigned(tiles), 1u, 1u), dim3(256), 0,
+                       eng.stream, a);
+    SISI_HIP(hipGetLastError());
+    return;
+  }
+  const SegVariant *table = precision == SISI_FP64 ? kSeg64 : kSeg32;
+  const TuneKey key = tune_key(precision == SISI_FP64 ? TuneFamily::Seg64 : TuneFamily::Seg32,
+                               g.M, g.N, ktot);
+  TuneChoice c = tune_lookup(key);
+  if (c.variant < 0 && tune_enabled() && !capturing(eng.stream))
+    c = tune_seg(eng, a, precision, key, flops);
+  int vid = c.variant >= 0 && c.variant < kNumSeg ? c.variant : default_seg_variant(g);
+  const int xcd = c.variant >= 0 ? c.xcd : 1;
+  launch_seg_variant(eng, a, precision, table[vid], xcd, flops);
 }
 
 }  // namespace sisi

@@ -2,28 +2,18 @@
 // access (unit-stride index across consecutive threads on both the read and
 // the write side, using an LDS tile when the two unit strides differ).
 #include "tensor_ops.hpp"
+#include "tape.hpp"
 
 #include <algorithm>
 
 namespace sisi {
 
-struct CopyArgs {
-  double *dst;
-  const double *src;
-  int nd;
-  int64_t len[kMaxOrder];
-  int64_t ds[kMaxOrder];
-  int64_t ss[kMaxOrder];
-  int64_t total;
-  double alpha, beta;
-  // tiled path: tile dims (0 = dst unit-stride dim, q = src unit-stride dim)
-  int q;
-  int64_t rest;  // product of the other dims
-};
 
-__global__ void copy_generic_kernel(CopyArgs a) {
-  for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < a.total;
-       e += int64_t(gridDim.x) * blockDim.x) {
+// Kernel bodies take their block index and block count explicitly, so the
+// grouped launches of the batched solver run them on virtual blocks.
+__device__ __forceinline__ void copy_generic_body(const CopyArgs &a, unsigned bx, unsigned gx) {
+  for (int64_t e = int64_t(bx) * blockDim.x + threadIdx.x; e < a.total;
+       e += int64_t(gx) * blockDim.x) {
     int64_t r = e, od = 0, os = 0;
 #pragma unroll
     for (int k = 0; k < kMaxOrder; ++k) {
@@ -40,14 +30,17 @@ __global__ void copy_generic_kernel(CopyArgs a) {
   }
 }
 
+__global__ void copy_generic_kernel(CopyArgs a) { copy_generic_body(a, blockIdx.x, gridDim.x); }
+
 // 32 x 32 tile over (dim 0, dim q): read along q (src unit stride), write
 // along 0 (dst unit stride).
-__global__ void copy_tiled_kernel(CopyArgs a) {
+__device__ __forceinline__ void copy_tiled_body(const CopyArgs &a, unsigned bx, unsigned by,
+                                                unsigned gy) {
   __shared__ double tile[32][33];
   const int64_t n0 = a.len[0], nq = a.len[a.q];
   const int64_t t0 = (n0 + 31) / 32;
-  const int64_t tx = blockIdx.x % t0, tq = blockIdx.x / t0;
-  for (int64_t rr = blockIdx.y; rr < a.rest; rr += gridDim.y) {
+  const int64_t tx = bx % t0, tq = bx / t0;
+  for (int64_t rr = by; rr < a.rest; rr += gy) {
     // decode the other dims
     int64_t r = rr, od = 0, os = 0;
     for (int k = 1; k < a.nd; ++k) {
@@ -78,6 +71,8 @@ __global__ void copy_tiled_kernel(CopyArgs a) {
     __syncthreads();
   }
 }
+
+__global__ void copy_tiled_kernel(CopyArgs a) { copy_tiled_body(a, blockIdx.x, blockIdx.y, gridDim.y); }
 
 void copy(Engine &e, const View &dst, const View &src, double alpha, double beta) {
   SISI_REQUIRE(dst.order == src.order, SISI_ERR_INVALID, "copy: order mismatch");
@@ -130,23 +125,64 @@ void copy(Engine &e, const View &dst, const View &src, double alpha, double beta
     a.rest = a.total / (a.len[0] * a.len[q]);
     const int64_t tiles = ((a.len[0] + 31) / 32) * ((a.len[q] + 31) / 32);
     SISI_REQUIRE(tiles < (int64_t(1) << 31), SISI_ERR_INVALID, "copy: too many tiles");
-    dim3 grid(unsigned(tiles), unsigned(std::min<int64_t>(a.rest, 65535)));
+    dim3 grid(unsigned(tiles), unsigned(std::min<int64_t>(a.rest, e.tape ? 64 : 65535)));
+    a.gx = grid.x;
+    if (e.tape) {
+      TapeOp op;
+      op.kind = OpKind::CopyTiled;
+      op.gx = grid.x;
+      op.gy = grid.y;
+      op.c = a;
+      e.tape->ops.push_back(op);
+      return;
+    }
     hipLaunchKernelGGL(copy_tiled_kernel, grid, dim3(32, 8), 0, e.stream, a);
   } else {
-    const int blocks = int(std::min<int64_t>((a.total + 255) / 256, 1 << 16));
+    const int blocks =
+        int(std::min<int64_t>((a.total + 255) / 256, e.tape ? 256 : int64_t(1) << 16));
+    if (e.tape) {
+      TapeOp op;
+      op.kind = OpKind::CopyGeneric;
+      op.gx = unsigned(blocks);
+      op.c = a;
+      e.tape->ops.push_back(op);
+      return;
+    }
     hipLaunchKernelGGL(copy_generic_kernel, dim3(blocks), dim3(256), 0, e.stream, a);
   }
   SISI_HIP(hipGetLastError());
 }
 
+__device__ __forceinline__ void fill_body(const ElemArgs &a, unsigned bx, unsigned gx) {
+  for (int64_t i = int64_t(bx) * blockDim.x + threadIdx.x; i < a.n; i += int64_t(gx) * blockDim.x)
+    a.y[i] = a.a;
+}
+
 __global__ void fill_kernel(double *p, int64_t n, double v) {
-  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-       i += int64_t(gridDim.x) * blockDim.x)
-    p[i] = v;
+  ElemArgs a;
+  a.y = p;
+  a.n = n;
+  a.a = v;
+  fill_body(a, blockIdx.x, gridDim.x);
+}
+
+// Records an element-wise launch on the engine's tape; false without a tape.
+static bool tape_elem(Engine &e, OpKind kind, const ElemArgs &a, int64_t work) {
+  if (!e.tape) return false;
+  TapeOp op;
+  op.kind = kind;
+  op.gx = unsigned(std::max<int64_t>(1, std::min<int64_t>((work + 255) / 256, 256)));
+  op.e = a;
+  e.tape->ops.push_back(op);
+  return true;
 }
 
 void zero(Engine &e, double *p, int64_t n) {
   if (n <= 0) return;
+  ElemArgs ta;
+  ta.y = p;
+  ta.n = n;
+  if (tape_elem(e, OpKind::Fill, ta, n)) return;
   Timed t(*e.timers, Kind::Bandwidth, 8.0 * double(n));
   const int blocks = int(std::min<int64_t>((n + 255) / 256, 1 << 16));
   hipLaunchKernelGGL(fill_kernel, dim3(blocks), dim3(256), 0, e.stream, p, n, 0.0);
@@ -209,28 +245,50 @@ void scale_pairs(Engine &e, double *C, int64_t Rx, int64_t Ry, int64_t nij, cons
   SISI_HIP(hipGetLastError());
 }
 
-__global__ void axpby_kernel(int64_t n, double a, const double *x, double b, double *y) {
-  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-       i += int64_t(gridDim.x) * blockDim.x) {
-    double v = a * x[i];
-    if (b != 0.0) v += b * y[i];
-    y[i] = v;
+__device__ __forceinline__ void axpby_body(const ElemArgs &p, unsigned bx, unsigned gx) {
+  for (int64_t i = int64_t(bx) * blockDim.x + threadIdx.x; i < p.n;
+       i += int64_t(gx) * blockDim.x) {
+    double v = p.a * p.x0[i];
+    if (p.b != 0.0) v += p.b * p.y[i];
+    p.y[i] = v;
   }
+}
+
+__global__ void axpby_kernel(int64_t n, double a, const double *x, double b, double *y) {
+  ElemArgs p;
+  p.n = n;
+  p.a = a;
+  p.x0 = x;
+  p.b = b;
+  p.y = y;
+  axpby_body(p, blockIdx.x, gridDim.x);
 }
 
 void axpby(Engine &e, int64_t n, double a, const double *x, double b, double *y) {
   if (n <= 0) return;
+  {
+    ElemArgs ta;
+    ta.n = n;
+    ta.a = a;
+    ta.x0 = x;
+    ta.b = b;
+    ta.y = y;
+    if (tape_elem(e, OpKind::Axpby, ta, n)) return;
+  }
   Timed t(*e.timers, Kind::Bandwidth, (b != 0.0 ? 24.0 : 16.0) * double(n));
   const int blocks = int(std::min<int64_t>((n + 255) / 256, 1 << 16));
   hipLaunchKernelGGL(axpby_kernel, dim3(blocks), dim3(256), 0, e.stream, n, a, x, b, y);
   SISI_HIP(hipGetLastError());
 }
 
-__global__ void add_t1t1_kernel(double *X, const double *T2, const double *T1,
-                                const double *T1b, double f, int Nv, int No) {
+__device__ __forceinline__ void add_t1t1_body(const ElemArgs &p, unsigned bx, unsigned gx) {
+  double *X = p.y;
+  const double *T2 = p.x0, *T1 = p.x1, *T1b = p.x2;
+  const double f = p.a;
+  const int Nv = p.i0, No = p.i1;
   const int64_t n = int64_t(Nv) * Nv * No * No;
-  for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < n;
-       e += int64_t(gridDim.x) * blockDim.x) {
+  for (int64_t e = int64_t(bx) * blockDim.x + threadIdx.x; e < n;
+       e += int64_t(gx) * blockDim.x) {
     int64_t r = e;
     const int a = int(r % Nv);
     r /= Nv;
@@ -242,9 +300,33 @@ __global__ void add_t1t1_kernel(double *X, const double *T2, const double *T1,
   }
 }
 
+__global__ void add_t1t1_kernel(double *X, const double *T2, const double *T1,
+                                const double *T1b, double f, int Nv, int No) {
+  ElemArgs p;
+  p.y = X;
+  p.x0 = T2;
+  p.x1 = T1;
+  p.x2 = T1b;
+  p.a = f;
+  p.i0 = Nv;
+  p.i1 = No;
+  add_t1t1_body(p, blockIdx.x, gridDim.x);
+}
+
 void add_t1t1(Engine &e, double *X, const double *T2, const double *T1, double f, int Nv,
               int No, const double *T1b) {
   const int64_t n = int64_t(Nv) * Nv * No * No;
+  {
+    ElemArgs ta;
+    ta.y = X;
+    ta.x0 = T2;
+    ta.x1 = T1;
+    ta.x2 = T1b ? T1b : T1;
+    ta.a = f;
+    ta.i0 = Nv;
+    ta.i1 = No;
+    if (tape_elem(e, OpKind::AddT1T1, ta, n)) return;
+  }
   Timed t(*e.timers, Kind::Bandwidth, 16.0 * double(n));
   const int blocks = int(std::min<int64_t>((n + 255) / 256, 1 << 16));
   hipLaunchKernelGGL(add_t1t1_kernel, dim3(blocks), dim3(256), 0, e.stream, X, T2, T1,
@@ -269,21 +351,39 @@ __device__ __forceinline__ double block_sum(double v) {
   return r;
 }
 
-__global__ void dot_partial_kernel(int64_t n, const double *x, const double *y, double *part) {
+__device__ __forceinline__ void dot_partial_body(const ElemArgs &p, unsigned bx, unsigned gx) {
   double s = 0.0;
-  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-       i += int64_t(gridDim.x) * blockDim.x)
-    s += x[i] * y[i];
+  for (int64_t i = int64_t(bx) * blockDim.x + threadIdx.x; i < p.n;
+       i += int64_t(gx) * blockDim.x)
+    s += p.x0[i] * p.x1[i];
   s = block_sum(s);
-  if (threadIdx.x == 0) part[blockIdx.x] = s;
+  if (threadIdx.x == 0) p.y[bx] = s;
+}
+
+__global__ void dot_partial_kernel(int64_t n, const double *x, const double *y, double *part) {
+  ElemArgs p;
+  p.n = n;
+  p.x0 = x;
+  p.x1 = y;
+  p.y = part;
+  dot_partial_body(p, blockIdx.x, gridDim.x);
+}
+
+// y[c] = sum_b x0[b * i1 + c] over b < i0, for column c = bx (< i2 blocks)
+__device__ __forceinline__ void sum_partials_body(const ElemArgs &p, unsigned bx) {
+  double s = 0.0;
+  for (int b = threadIdx.x; b < p.i0; b += blockDim.x) s += p.x0[int64_t(b) * p.i1 + bx];
+  s = block_sum(s);
+  if (threadIdx.x == 0) p.y[bx] = s;
 }
 
 __global__ void sum_partials_kernel(const double *part, int n, int stride, double *out) {
-  // out[c] = sum_b part[b*stride + c] for c = blockIdx.x
-  double s = 0.0;
-  for (int b = threadIdx.x; b < n; b += blockDim.x) s += part[int64_t(b) * stride + blockIdx.x];
-  s = block_sum(s);
-  if (threadIdx.x == 0) out[blockIdx.x] = s;
+  ElemArgs p;
+  p.x0 = part;
+  p.i0 = n;
+  p.i1 = stride;
+  p.y = out;
+  sum_partials_body(p, blockIdx.x);
 }
 
 double dot(Engine &e, int64_t n, const double *x, const double *y) {
@@ -304,11 +404,15 @@ double dot(Engine &e, int64_t n, const double *x, const double *y) {
   return r;
 }
 
-__global__ void estimate_kernel(int order, int Nv, int No, double *R, const double *T,
-                                double *dT, const double *eps_i, const double *eps_a,
-                                double shift, int64_t n, int64_t nelem) {
-  for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < n;
-       e += int64_t(gridDim.x) * blockDim.x) {
+__device__ __forceinline__ void estimate_body(const ElemArgs &p, unsigned bx, unsigned gx) {
+  const int order = p.i0, Nv = p.i1, No = p.i2;
+  double *R = p.y;
+  const double *T = p.x0, *eps_i = p.x2, *eps_a = p.x3;
+  double *dT = const_cast<double *>(p.x1);
+  const double shift = p.a;
+  const int64_t n = p.n, nelem = p.nelem;
+  for (int64_t e = int64_t(bx) * blockDim.x + threadIdx.x; e < n;
+       e += int64_t(gx) * blockDim.x) {
     double D;
     const int64_t x = e % nelem;  // element within its (real or imaginary) plane
     if (order == 2) {
@@ -332,10 +436,43 @@ __global__ void estimate_kernel(int order, int Nv, int No, double *R, const doub
   }
 }
 
+__global__ void estimate_kernel(int order, int Nv, int No, double *R, const double *T,
+                                double *dT, const double *eps_i, const double *eps_a,
+                                double shift, int64_t n, int64_t nelem) {
+  ElemArgs p;
+  p.i0 = order;
+  p.i1 = Nv;
+  p.i2 = No;
+  p.y = R;
+  p.x0 = T;
+  p.x1 = dT;
+  p.x2 = eps_i;
+  p.x3 = eps_a;
+  p.a = shift;
+  p.n = n;
+  p.nelem = nelem;
+  estimate_body(p, blockIdx.x, gridDim.x);
+}
+
 void estimate_amplitudes(Engine &e, int order, double *R, const double *T, double *dT,
                          double shift) {
   const int64_t n = order == 2 ? int64_t(e.n1()) : int64_t(e.n2());
   const int64_t nelem = order == 2 ? int64_t(e.n1e()) : int64_t(e.n2e());
+  {
+    ElemArgs ta;
+    ta.i0 = order;
+    ta.i1 = e.Nv;
+    ta.i2 = e.No;
+    ta.y = R;
+    ta.x0 = T;
+    ta.x1 = dT;
+    ta.x2 = e.eps_i.get();
+    ta.x3 = e.eps_a.get();
+    ta.a = shift;
+    ta.n = n;
+    ta.nelem = nelem;
+    if (tape_elem(e, OpKind::Estimate, ta, n)) return;
+  }
   Timed t(*e.timers, Kind::Bandwidth, 32.0 * double(n));
   const int blocks = int(std::min<int64_t>((n + 255) / 256, 1 << 16));
   hipLaunchKernelGGL(estimate_kernel, dim3(blocks), dim3(256), 0, e.stream, order, e.Nv, e.No,
@@ -343,12 +480,15 @@ void estimate_amplitudes(Engine &e, int order, double *R, const double *T, doubl
   SISI_HIP(hipGetLastError());
 }
 
-__global__ void energy_partial_kernel(int Nv, int No, const double *T1, const double *T2,
-                                      const double *V, const double *Vx, double *part) {
+__device__ __forceinline__ void energy_partial_body(const ElemArgs &p, unsigned bx,
+                                                    unsigned gx) {
+  const int Nv = p.i0, No = p.i1;
+  const double *T1 = p.x0, *T2 = p.x1, *V = p.x2, *Vx = p.x3;
+  double *part = p.y;
   const int64_t n = int64_t(Nv) * Nv * No * No;
   double sd = 0.0, sx = 0.0;
-  for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < n;
-       e += int64_t(gridDim.x) * blockDim.x) {
+  for (int64_t e = int64_t(bx) * blockDim.x + threadIdx.x; e < n;
+       e += int64_t(gx) * blockDim.x) {
     int64_t r = e;
     const int a = int(r % Nv);
     r /= Nv;
@@ -363,9 +503,22 @@ __global__ void energy_partial_kernel(int Nv, int No, const double *T1, const do
   sd = block_sum(sd);
   sx = block_sum(sx);
   if (threadIdx.x == 0) {
-    part[2 * blockIdx.x] = sd;
-    part[2 * blockIdx.x + 1] = sx;
+    part[2 * bx] = sd;
+    part[2 * bx + 1] = sx;
   }
+}
+
+__global__ void energy_partial_kernel(int Nv, int No, const double *T1, const double *T2,
+                                      const double *V, const double *Vx, double *part) {
+  ElemArgs p;
+  p.i0 = Nv;
+  p.i1 = No;
+  p.x0 = T1;
+  p.x1 = T2;
+  p.x2 = V;
+  p.x3 = Vx;
+  p.y = part;
+  energy_partial_body(p, blockIdx.x, gridDim.x);
 }
 
 void energy_terms(Engine &e, const double *T1, const double *T2, double &direct,
@@ -385,6 +538,130 @@ void energy_terms(Engine &e, const double *T1, const double *T2, double &direct,
   SISI_HIP(hipStreamSynchronize(e.stream));
   direct = 2.0 * r[0];
   exchange = -1.0 * r[1];
+}
+
+// Taped deterministic reductions of the batched solver: the sums land in
+// device memory (out) and are read back once for all systems.
+void dot_into(Engine &e, int64_t n, const double *x, const double *y, double *part,
+              double *out) {
+  SISI_REQUIRE(e.tape, SISI_ERR_STATE, "dot_into: taped engines only");
+  ElemArgs a;
+  a.n = n;
+  a.x0 = x;
+  a.x1 = y;
+  a.y = part;
+  TapeOp op;
+  op.kind = OpKind::DotPartial;
+  op.gx = kTapeRedBlocks;
+  op.e = a;
+  e.tape->ops.push_back(op);
+  ElemArgs b;
+  b.x0 = part;
+  b.i0 = kTapeRedBlocks;
+  b.i1 = 1;
+  b.i2 = 1;
+  b.y = out;
+  op.kind = OpKind::SumPartials;
+  op.gx = 1;
+  op.e = b;
+  e.tape->ops.push_back(op);
+}
+
+void energy_into(Engine &e, const double *T1, const double *T2, double *part, double *out) {
+  SISI_REQUIRE(e.tape, SISI_ERR_STATE, "energy_into: taped engines only");
+  ElemArgs a;
+  a.i0 = e.Nv;
+  a.i1 = e.No;
+  a.x0 = T1;
+  a.x1 = T2;
+  a.x2 = e.Vabij.get();
+  a.x3 = e.Vbaij.get();
+  a.y = part;
+  TapeOp op;
+  op.kind = OpKind::EnergyPartial;
+  op.gx = kTapeRedBlocks;
+  op.e = a;
+  e.tape->ops.push_back(op);
+  ElemArgs b;
+  b.x0 = part;
+  b.i0 = kTapeRedBlocks;
+  b.i1 = 2;
+  b.i2 = 2;
+  b.y = out;
+  op.kind = OpKind::SumPartials;
+  op.gx = 2;
+  op.e = b;
+  e.tape->ops.push_back(op);
+}
+
+// Record p of a grouped launch: the largest p with start[p] <= block.
+__device__ __forceinline__ int group_record(const unsigned *start, int n, unsigned block) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= block) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ void copy_generic_grouped(const CopyArgs *const *args, const unsigned *start, int n) {
+  const int p = group_record(start, n, blockIdx.x);
+  const CopyArgs a = *args[p];
+  copy_generic_body(a, blockIdx.x - start[p], start[p + 1] - start[p]);
+}
+
+__global__ void copy_tiled_grouped(const CopyArgs *const *args, const unsigned *start, int n) {
+  const int p = group_record(start, n, blockIdx.x);
+  const CopyArgs a = *args[p];
+  const unsigned local = blockIdx.x - start[p], gx = a.gx;
+  const unsigned gy = (start[p + 1] - start[p]) / gx;
+  copy_tiled_body(a, local % gx, local / gx, gy);
+}
+
+template <int KIND>
+__global__ __launch_bounds__(256) void elem_grouped(const ElemArgs *const *args,
+                                                    const unsigned *start, int n) {
+  const int p = group_record(start, n, blockIdx.x);
+  const unsigned bx = blockIdx.x - start[p], gx = start[p + 1] - start[p];
+  const ElemArgs a = *args[p];
+  if constexpr (KIND == int(OpKind::Fill)) fill_body(a, bx, gx);
+  else if constexpr (KIND == int(OpKind::Axpby)) axpby_body(a, bx, gx);
+  else if constexpr (KIND == int(OpKind::AddT1T1)) add_t1t1_body(a, bx, gx);
+  else if constexpr (KIND == int(OpKind::Estimate)) estimate_body(a, bx, gx);
+  else if constexpr (KIND == int(OpKind::DotPartial)) dot_partial_body(a, bx, gx);
+  else if constexpr (KIND == int(OpKind::SumPartials)) sum_partials_body(a, bx);
+  else if constexpr (KIND == int(OpKind::EnergyPartial)) energy_partial_body(a, bx, gx);
+}
+
+void launch_copy_grouped(OpKind kind, const CopyArgs *const *args, const unsigned *start, int n,
+                         unsigned total, hipStream_t s) {
+  if (kind == OpKind::CopyTiled)
+    hipLaunchKernelGGL(copy_tiled_grouped, dim3(total), dim3(32, 8), 0, s, args, start, n);
+  else
+    hipLaunchKernelGGL(copy_generic_grouped, dim3(total), dim3(256), 0, s, args, start, n);
+  SISI_HIP(hipGetLastError());
+}
+
+void launch_elem_grouped(OpKind kind, const ElemArgs *const *args, const unsigned *start, int n,
+                         unsigned total, hipStream_t s) {
+  switch (kind) {
+#define SISI_ELEM_CASE(K)                                                                      \
+  case OpKind::K:                                                                            \
+    hipLaunchKernelGGL(elem_grouped<int(OpKind::K)>, dim3(total), dim3(256), 0, s, args, start, \
+                       n);                                                                   \
+    break;
+    SISI_ELEM_CASE(Fill)
+    SISI_ELEM_CASE(Axpby)
+    SISI_ELEM_CASE(AddT1T1)
+    SISI_ELEM_CASE(Estimate)
+    SISI_ELEM_CASE(DotPartial)
+    SISI_ELEM_CASE(SumPartials)
+    SISI_ELEM_CASE(EnergyPartial)
+#undef SISI_ELEM_CASE
+    default: throw Error(SISI_ERR_INVALID, "grouped launch: not an element-wise kind");
+  }
+  SISI_HIP(hipGetLastError());
 }
 
 }  // namespace sisi

@@ -43,6 +43,14 @@ double dot(Engine &e, int64_t n, const double *x, const double *y);
 void estimate_amplitudes(Engine &e, int order, double *R, const double *T,
                          double *dT, double shift);
 
+// Taped reductions (engines with a tape, batched solver): the deterministic
+// sum lands in device memory at out (part: kTapeRedBlocks doubles of
+// partials; 2 kTapeRedBlocks and two outputs, direct and exchange halves of
+// energy_terms without the factors 2 and -1, for energy_into).
+void dot_into(Engine &e, int64_t n, const double *x, const double *y, double *part,
+              double *out);
+void energy_into(Engine &e, const double *T1, const double *T2, double *part, double *out);
+
 // Energy terms of getEnergy (cxx:131-189): direct = 2 sum tau V,
 // exchange = -sum tau Vbaij, tau = T2 + T1 T1.
 void energy_terms(Engine &e, const double *T1, const double *T2,

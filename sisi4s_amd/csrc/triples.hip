@@ -73,7 +73,6 @@ struct EnergyArgs {
 };
 
 // sigma as reads (x_s0, x_s1, x_s2): identity, three transpositions, two 3-cycles
-__constant__ int kSig[6][3] = {{0, 1, 2}, {1, 0, 2}, {2, 1, 0}, {0, 2, 1}, {1, 2, 0}, {2, 0, 1}};
 constexpr int hSig[6][3] = {{0, 1, 2}, {1, 0, 2}, {2, 1, 0}, {0, 2, 1}, {1, 2, 0}, {2, 0, 1}};
 constexpr double hSf[6] = {8.0, -4.0, -4.0, -4.0, 2.0, 2.0};
 
@@ -101,66 +100,137 @@ __device__ __forceinline__ double block_sum_t(double v) {
   return r;
 }
 
-__global__ __launch_bounds__(256) void triples_energy_kernel(EnergyArgs g) {
-  __shared__ double tile[6][512];
-  __shared__ int comp[6][6];
-  const int code = g.orbits[blockIdx.x];
-  const int X[3] = {code & 1023, (code >> 10) & 1023, code >> 20};
-  const int Nv = g.Nv;
-  const int64_t s1 = Nv, s2 = int64_t(Nv) * Nv;
-  if (threadIdx.x < 36) {
-    const int q = threadIdx.x / 6, t = threadIdx.x % 6;
-    int r = 0;
-    for (; r < 6; ++r)
-      if (kSig[r][0] == kSig[q][kSig[t][0]] && kSig[r][1] == kSig[q][kSig[t][1]] &&
-          kSig[r][2] == kSig[q][kSig[t][2]])
-        break;
-    comp[q][t] = r;
+// LDS layout of one 8x8x8 cube: (l0, l1, l2) at l0 + 9 l1 + 81 l2.  The odd
+// strides spread the permuted reads (an index of the wave's lanes moved to
+// the stride-9 or stride-81 position) over the banks.
+constexpr int kCubeS1 = 9, kCubeS2 = 81, kCubePad = 648;
+
+__device__ __forceinline__ int cube_off(int p0, int p1, int p2) {
+  return p0 + kCubeS1 * p1 + kCubeS2 * p2;
+}
+
+// composition sig[q] o sig[t] as an index of hSig (folded at compile time
+// for unrolled q, t)
+__host__ __device__ constexpr int comp_idx(int q, int t) {
+  int r = 0;
+  for (; r < 6; ++r) {
+    bool ok = true;
+    for (int k = 0; k < 3; ++k) ok = ok && (hSig[r][k] == hSig[q][hSig[t][k]]);
+    if (ok) break;
   }
-  // tile r: the cube X.sigma_r, local (l0, l1, l2) at l0 + 8 l1 + 64 l2
+  return r;
+}
+
+__device__ __forceinline__ int sel3(int a, int b, int c, int i) {
+  return i == 0 ? a : (i == 1 ? b : c);
+}
+
+// One workgroup per orbit of cubes.  Phase 1 issues every global load of
+// the orbit at once: the (up to) six permuted W cubes, and for each
+// distinct cube Y = X.sigma_q and each singles term t the factors
+// st[q][t][l] = T1[8 Y_pi0 + l, i_pi0] / 2 and sv[q][t][8 l1 + l2] =
+// V[8 Y_pi1 + l1, 8 Y_pi2 + l2, i_pi1, i_pi2] (S(i.pi)[x.pi] is their
+// product), and the particle energies of the orbit's three cube ranges.
+// After one barrier, phase 2 evaluates every element of every distinct
+// cube from LDS alone.  TWO: a second singles product (complex amplitudes).
+template <bool TWO>
+__global__ __launch_bounds__(256) void triples_energy_kernel(EnergyArgs g) {
+  __shared__ double tile[6][kCubePad];
+  __shared__ double sv[6][6][64], st[6][6][8];
+  __shared__ double sv2[TWO ? 6 : 1][6][64], st2[TWO ? 6 : 1][6][8];
+  __shared__ double ea[3][8];
+  const int code = g.orbits[blockIdx.x];
+  const int X0 = code & 1023, X1 = (code >> 10) & 1023, X2 = code >> 20;
+  const int Nv = g.Nv, tid = threadIdx.x, nterms = g.nterms;
+  const int64_t s1 = Nv, s2 = int64_t(Nv) * Nv;
+  // distinct cubes of the orbit: q is distinct when no earlier p gives the
+  // same cube (uniform)
+  bool distinct[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    const int y0 = sel3(X0, X1, X2, hSig[q][0]), y1 = sel3(X0, X1, X2, hSig[q][1]),
+              y2 = sel3(X0, X1, X2, hSig[q][2]);
+    bool dup = false;
+#pragma unroll
+    for (int p = 0; p < q; ++p)
+      dup = dup || (sel3(X0, X1, X2, hSig[p][0]) == y0 && sel3(X0, X1, X2, hSig[p][1]) == y1 &&
+                    sel3(X0, X1, X2, hSig[p][2]) == y2);
+    distinct[q] = !dup;
+  }
+  // phase 1: tile r is the cube X.sigma_r (rows of 8 doubles, 8 lanes a row);
+  // phase 2 reads every tile of the composition table, so a repeated cube is
+  // staged under each of its indices (its second read hits L2)
 #pragma unroll
   for (int r = 0; r < 6; ++r) {
-    const int b0 = X[kSig[r][0]] * 8, b1 = X[kSig[r][1]] * 8, b2 = X[kSig[r][2]] * 8;
-    for (int e = threadIdx.x; e < 512; e += 256) {
-      const int u0 = b0 + (e & 7), u1 = b1 + ((e >> 3) & 7), u2 = b2 + (e >> 6);
-      tile[r][e] = (u0 < Nv && u1 < Nv && u2 < Nv) ? g.W[u0 + u1 * s1 + u2 * s2] : 0.0;
+    const int b0 = sel3(X0, X1, X2, hSig[r][0]) * 8, b1 = sel3(X0, X1, X2, hSig[r][1]) * 8,
+              b2 = sel3(X0, X1, X2, hSig[r][2]) * 8;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int e = tid + 256 * h;
+      const int l0 = e & 7, l1 = (e >> 3) & 7, l2 = e >> 6;
+      const int u0 = b0 + l0, u1 = b1 + l1, u2 = b2 + l2;
+      tile[r][cube_off(l0, l1, l2)] =
+          (u0 < Nv && u1 < Nv && u2 < Nv) ? g.W[u0 + u1 * s1 + u2 * s2] : 0.0;
     }
   }
-  __syncthreads();
-  double acc = 0.0;
+#pragma unroll
   for (int q = 0; q < 6; ++q) {
-    // the cube Y = X.sigma_q, once per distinct cube of the orbit
-    const int Y[3] = {X[kSig[q][0]], X[kSig[q][1]], X[kSig[q][2]]};
-    bool dup = false;
-    for (int p = 0; p < q; ++p)
-      dup = dup || (X[kSig[p][0]] == Y[0] && X[kSig[p][1]] == Y[1] && X[kSig[p][2]] == Y[2]);
-    if (dup) continue;
-    for (int e = threadIdx.x; e < 512; e += 256) {
-      const int l[3] = {e & 7, (e >> 3) & 7, e >> 6};
-      const int x[3] = {Y[0] * 8 + l[0], Y[1] * 8 + l[1], Y[2] * 8 + l[2]};
-      if (x[0] >= Nv || x[1] >= Nv || x[2] >= Nv) continue;
+    if (!distinct[q]) continue;
+    const int Y[3] = {sel3(X0, X1, X2, hSig[q][0]), sel3(X0, X1, X2, hSig[q][1]),
+                      sel3(X0, X1, X2, hSig[q][2])};
+    for (int w = tid; w < nterms * 64; w += 256) {
+      const int t = w >> 6, l1 = (w >> 3) & 7, l2 = w & 7;
+      const PermTerm &p = g.terms[t];
+      const int u1 = sel3(Y[0], Y[1], Y[2], p.pi[1]) * 8 + l1;
+      const int u2 = sel3(Y[0], Y[1], Y[2], p.pi[2]) * 8 + l2;
+      const bool in = u1 < Nv && u2 < Nv;
+      const int64_t vv = u1 + s1 * u2 + s2 * (p.idx[1] + int64_t(g.No) * p.idx[2]);
+      sv[q][t][8 * l1 + l2] = in ? g.V[vv] : 0.0;
+      if (TWO) sv2[q][t][8 * l1 + l2] = in ? g.Vb[vv] : 0.0;
+      if (l1 == 0) {
+        const int u0 = sel3(Y[0], Y[1], Y[2], p.pi[0]) * 8 + l2;
+        const int64_t t1 = u0 + int64_t(Nv) * p.idx[0];
+        st[q][t][l2] = u0 < Nv ? 0.5 * g.T1[t1] : 0.0;
+        if (TWO) st2[q][t][l2] = u0 < Nv ? 0.5 * g.sb * g.T1b[t1] : 0.0;
+      }
+    }
+  }
+  if (tid < 24) {
+    const int k = tid >> 3, l = tid & 7, u = sel3(X0, X1, X2, k) * 8 + l;
+    ea[k][l] = u < Nv ? g.eps_a[u] : 0.0;
+  }
+  __syncthreads();
+  // phase 2
+  double acc = 0.0;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    if (!distinct[q]) continue;
+    const int Y0 = sel3(X0, X1, X2, hSig[q][0]), Y1 = sel3(X0, X1, X2, hSig[q][1]),
+              Y2 = sel3(X0, X1, X2, hSig[q][2]);
+    for (int e = tid; e < 512; e += 256) {
+      const int l0 = e & 7, l1 = (e >> 3) & 7, l2 = e >> 6;
+      if (Y0 * 8 + l0 >= Nv || Y1 * 8 + l1 >= Nv || Y2 * 8 + l2 >= Nv) continue;
       double f = 0.0;
 #pragma unroll
       for (int t = 0; t < 6; ++t)
-        f += g.sfp[t] *
-             tile[comp[q][t]][l[kSig[t][0]] + 8 * l[kSig[t][1]] + 64 * l[kSig[t][2]]];
-      const double D = g.eps_ijk - g.eps_a[x[0]] - g.eps_a[x[1]] - g.eps_a[x[2]];
-      f /= D;
-      double z = tile[q][e];
-      for (int t = 0; t < g.nterms; ++t) {
+        f += g.sfp[t] * tile[comp_idx(q, t)][cube_off(sel3(l0, l1, l2, hSig[t][0]),
+                                                      sel3(l0, l1, l2, hSig[t][1]),
+                                                      sel3(l0, l1, l2, hSig[t][2]))];
+      // axis k of Y is axis sigma_q(k) of X
+      const double D = g.eps_ijk - ea[hSig[q][0]][l0] - ea[hSig[q][1]][l1] - ea[hSig[q][2]][l2];
+      double z = tile[q][cube_off(l0, l1, l2)];
+      for (int t = 0; t < nterms; ++t) {
         const PermTerm &p = g.terms[t];
-        // S(i.pi)[u] = 0.5 T1[u0, i_pi0] V[u1, u2, i_pi1, i_pi2] read at u_k = x_pi(k)
-        const int u0 = x[p.pi[0]], u1 = x[p.pi[1]], u2 = x[p.pi[2]];
-        const int64_t t1 = u0 + int64_t(Nv) * p.idx[0];
-        const int64_t vv = u1 + s1 * u2 + s2 * (p.idx[1] + int64_t(g.No) * p.idx[2]);
-        z += 0.5 * g.T1[t1] * g.V[vv];
-        if (g.T1b) z += 0.5 * g.sb * g.T1b[t1] * g.Vb[vv];
+        const int a = sel3(l0, l1, l2, p.pi[0]);
+        const int b = 8 * sel3(l0, l1, l2, p.pi[1]) + sel3(l0, l1, l2, p.pi[2]);
+        z += st[q][t][a] * sv[q][t][b];
+        if (TWO) z += st2[q][t][a] * sv2[q][t][b];
       }
-      acc += f * z;
+      acc += f * z / D;
     }
   }
   acc = block_sum_t(acc);
-  if (threadIdx.x == 0) g.part[blockIdx.x] = acc;
+  if (tid == 0) g.part[blockIdx.x] = acc;
 }
 
 __global__ void sum_into_kernel(const double *part, int n, double *out) {
@@ -462,7 +532,7 @@ double ztriples_energy(Engine &e, const double *T1, const double *T2, int prec,
       g.No = int(o);
       block_energy_args(g, ib, perms, tup, distinct, eps_i);
       Timed tm(*e.timers, Kind::Bandwidth, 8.0 * double(v3));
-      hipLaunchKernelGGL(triples_energy_kernel, dim3(norbits), dim3(256), 0, e.stream, g);
+      hipLaunchKernelGGL(triples_energy_kernel<true>, dim3(norbits), dim3(256), 0, e.stream, g);
       hipLaunchKernelGGL(sum_into_kernel, dim3(1), dim3(256), 0, e.stream, part.get(), norbits,
                          Eb.get() + 2 * bi + plane);
       SISI_HIP(hipGetLastError());
@@ -557,7 +627,7 @@ double triples_energy(Engine &e, const double *T1, const double *T2, int prec,
     block_energy_args(g, ib, perms, tup, distinct, eps_i);
     {
       Timed tm(*e.timers, Kind::Bandwidth, 8.0 * double(v * v * v));
-      hipLaunchKernelGGL(triples_energy_kernel, dim3(norbits), dim3(256), 0, e.stream, g);
+      hipLaunchKernelGGL(triples_energy_kernel<false>, dim3(norbits), dim3(256), 0, e.stream, g);
       hipLaunchKernelGGL(sum_into_kernel, dim3(1), dim3(256), 0, e.stream, part.get(), norbits,
                          Eb.get() + bi);
       SISI_HIP(hipGetLastError());

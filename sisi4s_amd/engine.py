@@ -61,6 +61,14 @@ class CcsdReport(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class LockstepStats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_double), ("launches", ctypes.c_double),
+                ("records", ctypes.c_double), ("plans", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class Stats(ctypes.Structure):
     _fields_ = [("gemm_seconds", ctypes.c_double * 3), ("gemm_flops", ctypes.c_double * 3),
                 ("gemm_launches", ctypes.c_int64 * 3), ("other_seconds", ctypes.c_double),
@@ -129,6 +137,11 @@ SIGNATURES = [
      [ctypes.c_int, ctypes.POINTER(Dims), ctypes.POINTER(_c_double_p), ctypes.POINTER(_c_double_p),
       ctypes.POINTER(_c_double_p), ctypes.POINTER(CcsdOpts), ctypes.c_int, _c_double_p,
       ctypes.POINTER(CcsdReport)]),
+    ("sisi_engine_set_ladder_planes", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("sisi_ccsd_solve_lockstep", ctypes.c_int,
+     [ctypes.c_int, ctypes.POINTER(Dims), ctypes.POINTER(_c_double_p), ctypes.POINTER(_c_double_p),
+      ctypes.POINTER(_c_double_p), ctypes.POINTER(CcsdOpts), _c_double_p,
+      ctypes.POINTER(CcsdReport), ctypes.POINTER(LockstepStats)]),
     ("sisi_engine_enable_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("sisi_engine_get_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Stats)]),
     ("sisi_engine_reset_stats", ctypes.c_int, [ctypes.c_void_p]),
@@ -308,6 +321,41 @@ def gemm(A, B, C=None, alpha=1.0, beta=0.0, transA=False, transB=False, precisio
                                  B.ctypes.data_as(_c_double_p), max(B.shape[0], 1), beta,
                                  Cb.ctypes.data_as(_c_double_p), max(M, 1)))
     return Cb
+
+
+def _batch_inputs(systems):
+    n = len(systems)
+    dims = (Dims * max(n, 1))()
+    bufs = []
+    gp = (_c_double_p * max(n, 1))()
+    ip = (_c_double_p * max(n, 1))()
+    ap = (_c_double_p * max(n, 1))()
+    for s, (g, ei, ea) in enumerate(systems):
+        No, Nv = len(ei), len(ea)
+        NG = int(np.shape(g)[0])
+        dims[s] = Dims(No, Nv, NG)
+        b = (_sized(g, NG * (No + Nv) ** 2, "system %d gamma[NG,Np,Np]" % s), _buf(ei), _buf(ea))
+        bufs.append(b)
+        gp[s], ip[s], ap[s] = _ptr(b[0]), _ptr(b[1]), _ptr(b[2])
+    return dims, bufs, gp, ip, ap
+
+
+def ccsd_solve_lockstep(systems, opts=None, strict=True):
+    """The batch of ccsd_solve_batched iterated in lockstep: every launch of
+    an iteration is one grouped launch over all systems.  Returns (energies,
+    reports, stats) with stats the launch counters of the run."""
+    n = len(systems)
+    dims, bufs, gp, ip, ap = _batch_inputs(systems)
+    o = opts if opts is not None else default_opts()
+    energies = np.zeros(max(n, 1))
+    reps = (CcsdReport * max(n, 1))()
+    st = LockstepStats()
+    rc = load().sisi_ccsd_solve_lockstep(n, dims, gp, ip, ap, ctypes.byref(o), _ptr(energies),
+                                         reps, ctypes.byref(st))
+    del bufs
+    if strict:
+        _check(rc)
+    return energies[:n].copy(), [reps[s].as_dict() for s in range(n)], st.as_dict()
 
 
 def ccsd_solve_batched(systems, opts=None, workers=8, strict=True):
@@ -501,6 +549,11 @@ class Engine:
         _check(self._lib.sisi_eom_ea_sigma(self._h, _ptr(t1), _ptr(t2), _ptr(x1), _ptr(x2),
                                            _ptr(b1), _ptr(b2)))
         return _unbuf(b1, (v,)), _unbuf(b2, (o, v, v))
+
+    def set_ladder_planes(self, on=True):
+        """Reduced-precision ladder with plane output (default on): the V~
+        slice-pair product writes the bf16 planes the X contraction reads."""
+        _check(self._lib.sisi_engine_set_ladder_planes(self._h, int(bool(on))))
 
     def set_energy_variant(self, unrestricted=False, antisymmetrize=False):
         """`unrestricted` (spins = 1) and `antisymmetrize` energies of getEnergy

@@ -597,3 +597,35 @@ def test_energy_variants_match_oracle(unrestricted, anti, fock):
     finally:
         e.set_energy_variant(False, False)
         e.set_hp_fock(None)
+
+
+@pytest.mark.parametrize("prec", ["bf16x3", "bf16"])
+def test_ladder_plane_output_matches_stored_slice_pairs(prec):
+    """The reduced-precision ladder with plane output (the V~ product writes
+    the bf16 planes the X contraction reads) against the same residual with
+    the FP64 slice pair stored and split: the same products at the same
+    precision, so equal up to summation order (1e-6 of the residual scale;
+    a layout error would be O(1)), and both within the precision's bound of
+    the FP64 residual.  No=16, Nv=96, NG=384, slices of 48 (every pair large
+    enough for the plane path)."""
+    from sisi4s_amd import engine
+    No, Nv, NG = 16, 96, 384
+    e, g, ei, ea = make_engine(20261203, No, Nv, NG)
+    T1, T2 = random_amplitudes(Nv, No, seed=4, scale=0.02)
+    announce("ladder plane output %s No=16 Nv=96 NG=384" % prec)
+    R1f, R2f = e.residual(1, T1, T2, slice_size=48)
+    e.set_precision(prec)
+    e.set_ladder_planes(True)
+    R1p, R2p = e.residual(1, T1, T2, slice_size=48)
+    e.set_ladder_planes(False)
+    R1s, R2s = e.residual(1, T1, T2, slice_size=48)
+    e.set_ladder_planes(True)
+    e.set_precision("fp64")
+    scale = float(np.abs(R2f).max())
+    d = float(np.abs(R2p - R2s).max())
+    print("%s plane vs stored ladder: max |dR2| = %.3e of %.3e (bitwise %s)"
+          % (prec, d, scale, bool(np.array_equal(R2p, R2s))), flush=True)
+    assert d <= 1e-6 * scale
+    assert float(np.abs(R1p - R1s).max()) <= 1e-6 * float(np.abs(R1f).max())
+    tol = 2e-5 if prec == "bf16x3" else 2e-2
+    assert float(np.abs(R2p - R2f).max()) <= tol * scale
