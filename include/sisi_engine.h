@@ -381,6 +381,10 @@ typedef struct {
   double records;    /* per-system launch records those launches covered   */
   double plans;      /* residual schedules built (precision switches and
                         systems leaving the batch rebuild it)             */
+  double setup_seconds;    /* engines, integrals, recorded residuals (wall) */
+  double residual_seconds; /* device time of the grouped residuals          */
+  double update_seconds;   /* device time of the grouped updates            */
+  double total_seconds;    /* the whole call (wall)                          */
 } sisi_lockstep_stats;
 
 /* The batch of sisi_ccsd_solve_batched iterated in lockstep instead of on

@@ -23,7 +23,10 @@
 // mixed run (bulk and coarse stages) run their products on FP32 MFMA, the
 // remaining iterations in FP64.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include "ccsd.hpp"
@@ -37,13 +40,13 @@ namespace {
 
 // Byte size of the argument record of a tape op.
 size_t record_bytes(OpKind k) {
-  if (k == OpKind::Gemm) return sizeof(GemmArgs);
+  if (k == OpKind::Gemm || k == OpKind::GemmReduce) return sizeof(GemmArgs);
   if (k == OpKind::CopyTiled || k == OpKind::CopyGeneric) return sizeof(CopyArgs);
   return sizeof(ElemArgs);
 }
 
 const void *record_of(const TapeOp &op) {
-  if (op.kind == OpKind::Gemm) return &op.g;
+  if (op.kind == OpKind::Gemm || op.kind == OpKind::GemmReduce) return &op.g;
   if (op.kind == OpKind::CopyTiled || op.kind == OpKind::CopyGeneric) return &op.c;
   return &op.e;
 }
@@ -137,7 +140,8 @@ void GroupedPlan::build(Pool &pool, const std::vector<const Tape *> &tapes, hipS
     g.n = int(mg.second.size());
     g.ptrs = align_up(host.size(), 16);
     g.start = g.ptrs + sizeof(void *) * size_t(g.n);
-    host.resize(g.start + sizeof(unsigned) * size_t(g.n + 1));
+    g.gxs = g.start + sizeof(unsigned) * size_t(g.n + 1);
+    host.resize(g.gxs + (g.kind == OpKind::GemmReduce ? sizeof(unsigned) * size_t(g.n) : 0));
     groups_.push_back(g);
   }
   records_ = 0;
@@ -157,10 +161,29 @@ void GroupedPlan::build(Pool &pool, const std::vector<const Tape *> &tapes, hipS
       std::memcpy(host.data() + g.ptrs + sizeof(void *) * size_t(p), &rec, sizeof(rec));
       const unsigned st = unsigned(total);
       std::memcpy(host.data() + g.start + sizeof(unsigned) * size_t(p), &st, sizeof(st));
+      if (g.kind == OpKind::GemmReduce)
+        std::memcpy(host.data() + g.gxs + sizeof(unsigned) * size_t(p), &op.gx, sizeof(op.gx));
       total += uint64_t(op.blocks());
     }
     SISI_REQUIRE(total > 0 && total < (uint64_t(1) << 31), SISI_ERR_INVALID,
                  "grouped launch: block count out of range");
+    if (profile_shapes) {
+      const TapeOp *big = nullptr;
+      for (const Member &m : merged[gi].second) {
+        const TapeOp &op = tapes[size_t(m.tape)]->ops[m.op];
+        if (!big || op.blocks() > big->blocks()) big = &op;
+      }
+      char buf[160];
+      if (g.kind == OpKind::Gemm || g.kind == OpKind::GemmReduce)
+        snprintf(buf, sizeof(buf), "%s/%d M=%d N=%d K=%d b=%d ks=%d", g.kind == OpKind::Gemm ? "gemm" : "reduce",
+                 g.sub, big->g.M, big->g.N, big->g.K, big->g.batch, big->g.ksplit);
+      else if (g.kind == OpKind::CopyTiled || g.kind == OpKind::CopyGeneric)
+        snprintf(buf, sizeof(buf), "copy/%d n=%lld nd=%d", int(g.kind), (long long)big->c.total,
+                 big->c.nd);
+      else
+        snprintf(buf, sizeof(buf), "elem/%d n=%lld", int(g.kind), (long long)big->e.n);
+      g.label = buf;
+    }
     const unsigned st = unsigned(total);
     std::memcpy(host.data() + g.start + sizeof(unsigned) * size_t(g.n), &st, sizeof(st));
     g.total = st;
@@ -171,13 +194,17 @@ void GroupedPlan::build(Pool &pool, const std::vector<const Tape *> &tapes, hipS
 }
 
 void GroupedPlan::run(hipStream_t s, bool graph) {
-  auto launch_all = [&]() {
+  auto launch_one = [&](const Group &g) {
     const unsigned char *base = reinterpret_cast<const unsigned char *>(dev_.get());
-    for (const Group &g : groups_) {
+    {
       const unsigned *start = reinterpret_cast<const unsigned *>(base + g.start);
       if (g.kind == OpKind::Gemm)
         launch_gemm_grouped(g.sub, reinterpret_cast<const GemmArgs *const *>(base + g.ptrs), start,
                             g.n, g.total, s);
+      else if (g.kind == OpKind::GemmReduce)
+        launch_gemm_reduce_grouped(reinterpret_cast<const GemmArgs *const *>(base + g.ptrs),
+                                   start, reinterpret_cast<const unsigned *>(base + g.gxs), g.n,
+                                   g.total, s);
       else if (g.kind == OpKind::CopyTiled || g.kind == OpKind::CopyGeneric)
         launch_copy_grouped(g.kind, reinterpret_cast<const CopyArgs *const *>(base + g.ptrs),
                             start, g.n, g.total, s);
@@ -186,6 +213,39 @@ void GroupedPlan::run(hipStream_t s, bool graph) {
                             start, g.n, g.total, s);
     }
   };
+  auto launch_all = [&]() {
+    for (const Group &g : groups_) launch_one(g);
+  };
+  if (profile) {
+    std::vector<hipEvent_t> ev(2 * groups_.size());
+    for (auto &x : ev) SISI_HIP(hipEventCreate(&x));
+    for (size_t i = 0; i < groups_.size(); ++i) {
+      SISI_HIP(hipEventRecord(ev[2 * i], s));
+      launch_one(groups_[i]);
+      SISI_HIP(hipEventRecord(ev[2 * i + 1], s));
+    }
+    SISI_HIP(hipStreamSynchronize(s));
+    profile->resize(256);
+    for (size_t i = 0; i < groups_.size(); ++i) {
+      float ms = 0.f;
+      SISI_HIP(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
+      ProfileEntry &p = (*profile)[size_t(int(groups_[i].kind) * 16 + groups_[i].sub)];
+      p.ms += double(ms);
+      p.launches += 1.0;
+      p.records += double(groups_[i].n);
+      p.blocks += double(groups_[i].total);
+      if (profile_shapes) {
+        ProfileEntry &q = (*profile_shapes)[groups_[i].label];
+        q.ms += double(ms);
+        q.launches += 1.0;
+        q.records += double(groups_[i].n);
+        q.blocks += double(groups_[i].total);
+      }
+    }
+    for (auto &x : ev) (void)hipEventDestroy(x);
+    ++runs_;
+    return;
+  }
   if (exec_) {
     SISI_HIP(hipGraphLaunch(exec_, s));
   } else if (graph && runs_ >= 2) {
@@ -281,16 +341,34 @@ void ccsd_solve_lockstep(int nsys, const sisi_dims *dims, const double *const *g
   SISI_REQUIRE(is_precision(o.precision), SISI_ERR_INVALID, "unknown precision");
   SISI_REQUIRE(o.coarse_threshold <= 0.0 || is_precision(o.coarse_precision), SISI_ERR_INVALID,
                "coarse_precision must be a precision");
+  using clock = std::chrono::steady_clock;
+  const auto t_start = clock::now();
+  auto since = [](clock::time_point t0) {
+    return std::chrono::duration<double>(clock::now() - t0).count();
+  };
   Pool pool;
   hipStream_t stream = nullptr;
   SISI_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  // device time of the grouped residual / update launches (events read after
+  // the read-back that ends every update)
+  hipEvent_t ev[4] = {};
   struct StreamGuard {
     hipStream_t s;
+    hipEvent_t *ev;
     ~StreamGuard() {
       (void)hipStreamSynchronize(s);
+      for (int k = 0; k < 4; ++k)
+        if (ev[k]) (void)hipEventDestroy(ev[k]);
       (void)hipStreamDestroy(s);
     }
-  } guard{stream};
+  } guard{stream, ev};
+  for (int k = 0; k < 4; ++k) SISI_HIP(hipEventCreate(&ev[k]));
+  double res_ms = 0.0, upd_ms = 0.0;
+  auto elapsed = [&](int a, int b) {
+    float ms = 0.f;
+    SISI_HIP(hipEventElapsedTime(&ms, ev[a], ev[b]));
+    return double(ms);
+  };
   std::vector<System> sys(static_cast<size_t>(nsys));
   const int N = o.mixer == SISI_MIXER_DIIS ? o.max_residua : 0;
   const bool bulk_run = o.precision != SISI_FP64;
@@ -327,6 +405,7 @@ void ccsd_solve_lockstep(int nsys, const sisi_dims *dims, const double *const *g
     if (bulk_run) record_residual(s, 1, o.slice_size, 1, s.res[1], &s.held[1], pool, stream);
     record_residual(s, 0, o.slice_size, 1, s.res[0], &s.held[0], pool, stream);
   }
+  if (stats) stats->setup_seconds = since(t_start);
   // reduction results of all systems in one array (one read-back)
   DBuf outs(&pool, size_t(std::max(nsys, 1)) * kOutSlots);
   std::vector<double> hout(size_t(std::max(nsys, 1)) * kOutSlots);
@@ -349,6 +428,15 @@ void ccsd_solve_lockstep(int nsys, const sisi_dims *dims, const double *const *g
   // quarter of the systems it runs have stopped (stopped systems keep
   // running in the schedule until then; their results are not read)
   GroupedPlan res_plan, step_plan;
+  // diagnostics: SISI_LOCKSTEP_PROFILE=1 times every grouped launch of the
+  // residual (eagerly, no graph) and prints the time per kernel kind to stderr
+  std::vector<GroupedPlan::ProfileEntry> prof;
+  const char *penv = getenv("SISI_LOCKSTEP_PROFILE");
+  std::map<std::string, GroupedPlan::ProfileEntry> prof_shapes;
+  if (penv && atoi(penv) > 0) {
+    res_plan.profile = &prof;
+    res_plan.profile_shapes = &prof_shapes;
+  }
   std::vector<std::pair<int, int>> res_members;  // (system, precision index)
   const double shift = o.level_shift;
   for (int iter = 0;; ++iter) {
@@ -397,13 +485,16 @@ void ccsd_solve_lockstep(int nsys, const sisi_dims *dims, const double *const *g
         res_plan.build(pool, tp, stream);
         if (stats) stats->plans += 1.0;
       }
+      SISI_HIP(hipEventRecord(ev[0], stream));
       res_plan.run(stream, true);
+      SISI_HIP(hipEventRecord(ev[1], stream));
       if (stats) {
         stats->launches += double(res_plan.launches());
         stats->records += double(res_plan.records());
       }
     }
     // update, part 1: estimate and the mixer's overlaps
+    SISI_HIP(hipEventRecord(ev[2], stream));
     std::vector<const Tape *> tp;
     for (int k : act) {
       System &s = sys[size_t(k)];
@@ -452,7 +543,10 @@ void ccsd_solve_lockstep(int nsys, const sisi_dims *dims, const double *const *g
       tp.push_back(&s.step);
     }
     run_tapes(tp, step_plan);
+    SISI_HIP(hipEventRecord(ev[3], stream));
     read_outs();
+    if (iter > 0) res_ms += elapsed(0, 1);
+    upd_ms += elapsed(2, 3);
     // update, part 2 (DIIS): combination, norms and energy
     if (N > 0) {
       tp.clear();
@@ -501,8 +595,11 @@ void ccsd_solve_lockstep(int nsys, const sisi_dims *dims, const double *const *g
         e.tape = nullptr;
         tp.push_back(&s.step);
       }
+      SISI_HIP(hipEventRecord(ev[2], stream));
       run_tapes(tp, step_plan);
+      SISI_HIP(hipEventRecord(ev[3], stream));
       read_outs();
+      upd_ms += elapsed(2, 3);
     }
     // energies, convergence and the precision schedule per system
     for (int k : act) {
@@ -551,6 +648,10 @@ void ccsd_solve_lockstep(int nsys, const sisi_dims *dims, const double *const *g
     if (stats) stats->iterations += 1.0;
   }
   SISI_HIP(hipStreamSynchronize(stream));
+  if (stats) {
+    stats->residual_seconds = 1e-3 * res_ms;
+    stats->update_seconds = 1e-3 * upd_ms;
+  }
   for (int k = 0; k < nsys; ++k) {
     System &s = sys[size_t(k)];
     energies[k] = s.status == SISI_OK ? s.energy : std::nan("");
@@ -576,6 +677,26 @@ void ccsd_solve_lockstep(int nsys, const sisi_dims *dims, const double *const *g
       s.e->pool.release(s.held[p]);
       s.held[p].clear();
     }
+  if (stats) stats->total_seconds = since(t_start);
+  if (!prof.empty()) {
+    static const char *kinds[] = {"gemm", "copy_tiled", "copy_generic", "fill", "axpby",
+                                  "add_t1t1", "estimate", "dot_partial", "sum_partials",
+                                  "energy_partial", "gemm_reduce"};
+    fprintf(stderr, "[lockstep profile] kind/variant: ms, launches, records, blocks\n");
+    for (size_t k = 0; k < prof.size(); ++k)
+      if (prof[k].launches > 0)
+        fprintf(stderr, "[lockstep profile] %s/%d: %.1f ms, %.0f, %.0f, %.0f\n",
+                k / 16 < 11 ? kinds[k / 16] : "?", int(k % 16), prof[k].ms, prof[k].launches,
+                prof[k].records, prof[k].blocks);
+    std::vector<std::pair<double, std::string>> top;
+    for (auto &kv : prof_shapes) top.push_back({kv.second.ms, kv.first});
+    std::sort(top.rbegin(), top.rend());
+    for (size_t i = 0; i < top.size() && i < 40; ++i) {
+      const auto &q = prof_shapes[top[i].second];
+      fprintf(stderr, "[lockstep shape] %8.1f ms %7.0f launches %8.0f records %10.0f blocks  %s\n",
+              q.ms, q.launches, q.records, q.blocks, top[i].second.c_str());
+    }
+  }
   for (int k = 0; k < nsys; ++k)
     if (sys[size_t(k)].status != SISI_OK)
       throw Error(sys[size_t(k)].status,

@@ -340,6 +340,8 @@ void ccsd_residual(Engine &e, int it, bool initial_doubles, int slice_size, cons
           g2.N = int(o * o);
           g2.K = int(v * v);
           g2.C = Rxy.get();
+          g2.dm0 = g2.M;
+          g2.dn0 = g2.N;
           g2.cm0 = 1;
           g2.cn0 = Nx * Ny;
           gemm_b3_frozen_a(e, g2, fa, pl == SISI_BF16 ? 1 : 3);

@@ -228,7 +228,7 @@ __global__ __launch_bounds__(256, 2) void gemm_k4_kernel(GemmArgs g) {
 }
 
 // Grouped launch of n products (batched solver): record p owns the blocks
-// [start[p], start[p + 1]), tiles_m * tiles_n per batch entry, no split-K.
+// [start[p], start[p + 1]): tiles_m * tiles_n per k-split and batch entry.
 template <int PREC, bool AK, bool BNF>
 __global__ __launch_bounds__(256, 2) void gemm_k4_grouped(const GemmArgs *const *args,
                                                           const unsigned *__restrict__ start,
@@ -242,7 +242,8 @@ __global__ __launch_bounds__(256, 2) void gemm_k4_grouped(const GemmArgs *const 
   const GemmArgs g = *args[lo];
   const int tiles = g.tiles_m * g.tiles_n;
   const int local = int(blockIdx.x - start[lo]);
-  gemm_k4_body<PREC, AK, BNF>(g, local % tiles, 0, local / tiles);
+  const int r = local / tiles;
+  gemm_k4_body<PREC, AK, BNF>(g, local % tiles, r % g.ksplit, r / g.ksplit);
 }
 
 // ---------------------------------------------------------------------------
@@ -451,14 +452,17 @@ __global__ __launch_bounds__(256, 2) void gemm_bf16x3_kernel(GemmArgs g) {
 // block takes 64 consecutive outputs; its four waves sum the partials
 // s = q, q + 4, ... (q = wave) and the four sums are added in a fixed order,
 // so the result does not depend on timing.
-__global__ __launch_bounds__(256) void gemm_reduce_kernel(GemmArgs g) {
+// Split-K reduction of batch entry bz, workgroup bx of gx: 64 outputs per
+// step, the four waves summing every fourth partial, the four sums added in
+// a fixed order (deterministic).
+__device__ __forceinline__ void gemm_reduce_body(const GemmArgs &g, unsigned bx, unsigned gx,
+                                                 int bz) {
   const int64_t MN = int64_t(g.M) * g.N;
-  const int bz = blockIdx.y;
   double *C = g.C + int64_t(bz) * g.bsc;
   const double *W = g.work + int64_t(bz) * g.ksplit * MN;
   __shared__ double part[4][64];
   const int t = threadIdx.x & 63, q = threadIdx.x >> 6;
-  for (int64_t e0 = int64_t(blockIdx.x) * 64; e0 < MN; e0 += int64_t(gridDim.x) * 64) {
+  for (int64_t e0 = int64_t(bx) * 64; e0 < MN; e0 += int64_t(gx) * 64) {
     const int64_t e = e0 + t;
     double s = 0.0;
     if (e < MN) {
@@ -478,6 +482,27 @@ __global__ __launch_bounds__(256) void gemm_reduce_kernel(GemmArgs g) {
     }
     __syncthreads();
   }
+}
+
+__global__ __launch_bounds__(256) void gemm_reduce_kernel(GemmArgs g) {
+  gemm_reduce_body(g, blockIdx.x, gridDim.x, blockIdx.y);
+}
+
+// Grouped split-K reductions: record p owns gx * batch blocks (gx = the
+// record's eager grid x).
+__global__ __launch_bounds__(256) void gemm_reduce_grouped(const GemmArgs *const *args,
+                                                           const unsigned *__restrict__ start,
+                                                           const unsigned *__restrict__ gxs,
+                                                           int n) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= blockIdx.x) lo = mid;
+    else hi = mid - 1;
+  }
+  const GemmArgs g = *args[lo];
+  const unsigned local = blockIdx.x - start[lo], gx = gxs[lo];
+  gemm_reduce_body(g, local % gx, gx, int(local / gx));
 }
 
 // ---------------------------------------------------------------------------
@@ -624,23 +649,54 @@ void launch_gemm_grouped(int sub, const GemmArgs *const *args, const unsigned *s
   SISI_HIP(hipGetLastError());
 }
 
-// A product on a taped engine: recorded for the grouped launch, without
-// split-K (the systems of the group fill the device together) and with the
-// reduced precisions on FP32 MFMA.
+void launch_gemm_reduce_grouped(const GemmArgs *const *args, const unsigned *start,
+                                const unsigned *gxs, int n, unsigned total, hipStream_t s) {
+  hipLaunchKernelGGL(gemm_reduce_grouped, dim3(total), dim3(256), 0, s, args, start, gxs, n);
+  SISI_HIP(hipGetLastError());
+}
+
+// A product on a taped engine, recorded for the grouped launch, with the
+// reduced precisions on FP32 MFMA.  The systems of a grouped launch fill the
+// device together, so only a product with few output tiles is split along
+// K: into parts of about 32 k-chunks (at most 1024 parts, workspace of at
+// most 2^24 doubles), whose partial sums a recorded reduction adds in a
+// fixed order.  A product with a small output and a long K (Kki of the
+// residual: 16 x 16 outputs over K = Nv^2 No) would otherwise run its whole
+// K in one workgroup.
 static void tape_gemm(Engine &eng, int precision, GemmArgs g) {
   if (precision != SISI_FP64) precision = SISI_FP32;
   bool AKf, BNFf;
   plan_k4(g, precision, eng.cus, AKf, BNFf);
-  set_ksplit(g, 1, 16);
-  SISI_REQUIRE(int64_t(g.tiles_m) * g.tiles_n * g.batch < (int64_t(1) << 31), SISI_ERR_INVALID,
-               "gemm: too many tiles");
+  const int64_t tb = int64_t(g.tiles_m) * g.tiles_n * g.batch;
+  const int64_t MNb = int64_t(g.M) * g.N * g.batch;
+  int64_t ks = 1;
+  if (tb < 256 && g.K > 2 * 512) {
+    ks = std::min<int64_t>((g.K + 511) / 512, std::max<int64_t>(1, 1024 / tb));
+    ks = std::min<int64_t>(ks, std::max<int64_t>(1, (int64_t(1) << 24) / MNb));
+  }
+  set_ksplit(g, int(ks), 16);
+  SISI_REQUIRE(tb * g.ksplit < (int64_t(1) << 31), SISI_ERR_INVALID, "gemm: too many tiles");
+  DBuf work;
+  if (g.ksplit > 1) {
+    work = DBuf(&eng.pool, size_t(g.ksplit) * size_t(MNb));
+    g.work = work.get();
+  }
   TapeOp op;
   op.kind = OpKind::Gemm;
   op.sub = tape_gemm_sub(precision, AKf, BNFf);
   op.gx = unsigned(g.tiles_m * g.tiles_n);
+  op.gy = unsigned(g.ksplit);
   op.gz = unsigned(g.batch);
   op.g = g;
   eng.tape->ops.push_back(op);
+  if (g.ksplit > 1) {
+    TapeOp red;
+    red.kind = OpKind::GemmReduce;
+    red.gx = unsigned(std::min<int64_t>((int64_t(g.M) * g.N + 63) / 64, 64));
+    red.gz = unsigned(g.batch);
+    red.g = g;
+    eng.tape->ops.push_back(red);
+  }
 }
 
 void gemm(Engine &eng, int precision, GemmArgs g, bool exact) {

@@ -12,6 +12,8 @@
 #pragma once
 
 #include <cstdint>
+#include <map>
+#include <string>
 #include <vector>
 
 #include "common.hpp"
@@ -46,7 +48,7 @@ struct ElemArgs {
 };
 
 enum class OpKind : int {
-  Gemm = 0,      // GemmArgs, sub = precision * 4 + AK * 2 + BNF
+  Gemm = 0,      // GemmArgs, sub = precision * 4 + AK * 2 + BNF; grid (tiles, ksplit, batch)
   CopyTiled,     // CopyArgs, block (32, 8)
   CopyGeneric,   // CopyArgs
   Fill,          // y[0..n) = a
@@ -58,6 +60,8 @@ enum class OpKind : int {
   SumPartials,   // y[c] = sum_b x0[b * i1 + c], c < i2, b < i0
   EnergyPartial, // y[2 block + {0,1}] = direct / exchange partials: T1 = x0, T2 = x1,
                  // Vabij = x2, Vbaij = x3, i0 = Nv, i1 = No
+  GemmReduce,    // GemmArgs: split-K reduction of the preceding Gemm record, gx per
+                 // batch entry
 };
 
 struct TapeOp {
@@ -96,6 +100,9 @@ constexpr int kTapeRedBlocks = 64;
 // block in the launch (start[n] = total blocks).
 void launch_gemm_grouped(int sub, const GemmArgs *const *args, const unsigned *start, int n,
                          unsigned total, hipStream_t s);
+// gxs[p]: the record's workgroups per batch entry
+void launch_gemm_reduce_grouped(const GemmArgs *const *args, const unsigned *start,
+                                const unsigned *gxs, int n, unsigned total, hipStream_t s);
 void launch_copy_grouped(OpKind kind, const CopyArgs *const *args, const unsigned *start, int n,
                          unsigned total, hipStream_t s);
 void launch_elem_grouped(OpKind kind, const ElemArgs *const *args, const unsigned *start, int n,
@@ -118,14 +125,25 @@ class GroupedPlan {
   void reset();
   size_t launches() const { return groups_.size(); }
   size_t records() const { return records_; }
+  // Diagnostics: with a profile attached, run() launches eagerly with an
+  // event pair around each grouped launch and adds the device time of each
+  // (kind, variant) to profile[key] (key = kind * 16 + variant; milliseconds,
+  // launches, records, blocks).
+  struct ProfileEntry {
+    double ms = 0.0, launches = 0.0, records = 0.0, blocks = 0.0;
+  };
+  std::vector<ProfileEntry> *profile = nullptr;
+  // and per label (kind, variant and the shape of the group's largest record)
+  std::map<std::string, ProfileEntry> *profile_shapes = nullptr;
 
  private:
   struct Group {
     OpKind kind;
     int sub;
-    size_t ptrs, start;  // byte offsets in dev_
+    size_t ptrs, start, gxs;  // byte offsets in dev_ (gxs: GemmReduce only)
     int n;
     unsigned total;
+    std::string label;  // diagnostics: shape of the largest record
   };
   std::vector<Group> groups_;
   DBuf dev_;

@@ -63,7 +63,9 @@ class CcsdReport(ctypes.Structure):
 
 class LockstepStats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_double), ("launches", ctypes.c_double),
-                ("records", ctypes.c_double), ("plans", ctypes.c_double)]
+                ("records", ctypes.c_double), ("plans", ctypes.c_double),
+                ("setup_seconds", ctypes.c_double), ("residual_seconds", ctypes.c_double),
+                ("update_seconds", ctypes.c_double), ("total_seconds", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
