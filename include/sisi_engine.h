@@ -456,6 +456,22 @@ const char *sisi_gemm_variant_name(int passes, int variant);
 int sisi_tune_enable(int on);
 int sisi_tune_dump(char *buf, int len, int *needed);
 
+/* C = alpha*op(A)*op(B) + beta*C (column-major, FP64 or FP32 MFMA) on
+ * device arrays the caller owns, on the caller's HIP stream (NULL: the null
+ * stream), with no allocation of any kind: the split-K partial sums live in
+ * a 32 MiB device array inside the library (one per GPU, held until the
+ * process ends) whose address is looked up on the first call on each
+ * device.  ksplit = 0 picks the split sisi_gemm_host would (fewer partial
+ * sums when they would not fit); ksplit > 0 asks for that split and fails
+ * with SISI_ERR_INVALID when ksplit*M*N doubles exceed the array.  Calls
+ * that overlap on one GPU share the array and race.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, allocation-free harnesses). */
+int sisi_gemm_device_noalloc(int precision, int transA, int transB, int M, int N, int K,
+                             double alpha, const double *A, int lda, const double *B,
+                             int ldb, double beta, double *C, int ldc, int ksplit,
+                             void *stream);
+
 /* Timing of one GEMM shape on device-resident operands (A m x k and B k x n,
  * both k-contiguous, C column-major): average microseconds per call of the
  * GEMM kernels (HIP events around each launch) and of the whole call

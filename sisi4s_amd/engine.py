@@ -165,6 +165,10 @@ SIGNATURES = [
     ("sisi_gemm_variant_name", ctypes.c_char_p, [ctypes.c_int, ctypes.c_int]),
     ("sisi_tune_enable", ctypes.c_int, [ctypes.c_int]),
     ("sisi_tune_dump", ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, _c_int_p]),
+    ("sisi_gemm_device_noalloc", ctypes.c_int,
+     [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+      ctypes.c_double, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+      ctypes.c_double, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
     ("sisi_gemm_bench", ctypes.c_int,
      [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_double_p,
       _c_double_p]),
