@@ -545,7 +545,13 @@ void ccsd_solve_lockstep(int nsys, const sisi_dims *dims, const double *const *g
     run_tapes(tp, step_plan);
     SISI_HIP(hipEventRecord(ev[3], stream));
     read_outs();
-    if (iter > 0) res_ms += elapsed(0, 1);
+    if (iter > 0) {
+      const double ms = elapsed(0, 1);
+      res_ms += ms;
+      if (res_plan.profile && (iter % 10 == 0 || act.size() < 4))
+        fprintf(stderr, "[lockstep iteration] %d: %zu systems, %zu launches, residual %.2f ms\n",
+                iter, act.size(), res_plan.launches(), ms);
+    }
     upd_ms += elapsed(2, 3);
     // update, part 2 (DIIS): combination, norms and energy
     if (N > 0) {

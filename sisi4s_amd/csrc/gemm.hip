@@ -657,12 +657,12 @@ void launch_gemm_reduce_grouped(const GemmArgs *const *args, const unsigned *sta
 
 // A product on a taped engine, recorded for the grouped launch, with the
 // reduced precisions on FP32 MFMA.  The systems of a grouped launch fill the
-// device together, so only a product with few output tiles is split along
-// K: into parts of about 32 k-chunks (at most 1024 parts, workspace of at
-// most 2^24 doubles), whose partial sums a recorded reduction adds in a
-// fixed order.  A product with a small output and a long K (Kki of the
-// residual: 16 x 16 outputs over K = Nv^2 No) would otherwise run its whole
-// K in one workgroup.
+// device together, so only a product with very few output tiles and a long
+// K is split along K: into parts of about 64 k-chunks (at most 512 / tiles
+// parts, workspace of at most 2^24 doubles), whose partial sums a recorded
+// reduction adds in a fixed order.  A product with a small output and a
+// long K (Kki of the residual: 16 x 16 outputs over K = Nv^2 No) would
+// otherwise run its whole K in one workgroup.
 static void tape_gemm(Engine &eng, int precision, GemmArgs g) {
   if (precision != SISI_FP64) precision = SISI_FP32;
   bool AKf, BNFf;
@@ -670,8 +670,8 @@ static void tape_gemm(Engine &eng, int precision, GemmArgs g) {
   const int64_t tb = int64_t(g.tiles_m) * g.tiles_n * g.batch;
   const int64_t MNb = int64_t(g.M) * g.N * g.batch;
   int64_t ks = 1;
-  if (tb < 256 && g.K > 2 * 512) {
-    ks = std::min<int64_t>((g.K + 511) / 512, std::max<int64_t>(1, 1024 / tb));
+  if (tb < 32 && g.K >= 2048) {
+    ks = std::min<int64_t>((g.K + 1023) / 1024, std::max<int64_t>(1, 512 / tb));
     ks = std::min<int64_t>(ks, std::max<int64_t>(1, (int64_t(1) << 24) / MNb));
   }
   set_ksplit(g, int(ks), 16);
@@ -692,7 +692,7 @@ static void tape_gemm(Engine &eng, int precision, GemmArgs g) {
   if (g.ksplit > 1) {
     TapeOp red;
     red.kind = OpKind::GemmReduce;
-    red.gx = unsigned(std::min<int64_t>((int64_t(g.M) * g.N + 63) / 64, 64));
+    red.gx = unsigned(std::min<int64_t>((int64_t(g.M) * g.N + 63) / 64, 1024));
     red.gz = unsigned(g.batch);
     red.g = g;
     eng.tape->ops.push_back(red);

@@ -45,6 +45,12 @@ struct GemmArgs {
   // operand split of the FP64 result would give
   int out_planes = 0;
   int64_t plane_lo = 0;
+  // out_cube = 1: each output index splits into two coordinates (x0 = m %
+  // dm0, x1 = m / dm0, likewise for n), and a coordinate x contributes
+  // (x & 7) * lo + (x >> 3) * hi: lo/hi = cm0/cm1 for x0 and cm2/cm3 for x1
+  // (cn* for n).  This writes tensors stored as contiguous 8 x 8 x 8 cubes.
+  int out_cube = 0;
+  int64_t cm3 = 0, cn3 = 0;
 };
 
 // One product of a segmented GEMM: A(m,k) = A[m*sam + k*sak],

@@ -14,6 +14,7 @@
 #include "blaslt.hpp"
 #include "engine.hpp"
 #include "gemm.hpp"
+#include "gemm_device.hpp"
 #include "tune.hpp"
 
 namespace sisi {
@@ -120,11 +121,13 @@ struct B3Args {
 };
 
 __device__ __forceinline__ int64_t b3_off_m(const GemmArgs &g, int m) {
+  if (g.out_cube) return cube_coord(m % g.dm0, g.cm0, g.cm1) + cube_coord(m / g.dm0, g.cm2, g.cm3);
   if (!g.split_out) return int64_t(m) * g.cm0;
   int m0 = m % g.dm0, t = m / g.dm0, m1 = t % g.dm1, m2 = t / g.dm1;
   return int64_t(m0) * g.cm0 + int64_t(m1) * g.cm1 + int64_t(m2) * g.cm2;
 }
 __device__ __forceinline__ int64_t b3_off_n(const GemmArgs &g, int n) {
+  if (g.out_cube) return cube_coord(n % g.dn0, g.cn0, g.cn1) + cube_coord(n / g.dn0, g.cn2, g.cn3);
   if (!g.split_out) return int64_t(n) * g.cn0;
   int n0 = n % g.dn0, t = n / g.dn0, n1 = t % g.dn1, n2 = t / g.dn1;
   return int64_t(n0) * g.cn0 + int64_t(n1) * g.cn1 + int64_t(n2) * g.cn2;
@@ -443,6 +446,31 @@ __global__ __launch_bounds__(64 * WGM * WGN, 1) void gemm_b3d_kernel(B3Args p) {
         if constexpr (PASSES == 3) al[kk][i] = *reinterpret_cast<const bf8v *>(st + A_BYTES + o);
       }
     }
+    if constexpr (SCHED == 2 && PASSES == 3) {
+      // pass-outer order: TI * TJ independent accumulations between two
+      // MFMAs into the same accumulator
+#pragma unroll
+      for (int kk = 0; kk < 2; ++kk) {
+#pragma unroll
+        for (int i = 0; i < TI; ++i)
+#pragma unroll
+          for (int j = 0; j < TJ; ++j)
+            acc[i][j] =
+                __builtin_amdgcn_mfma_f32_32x32x16_bf16(bl[kk][j], ah[kk][i], acc[i][j], 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < TI; ++i)
+#pragma unroll
+          for (int j = 0; j < TJ; ++j)
+            acc[i][j] =
+                __builtin_amdgcn_mfma_f32_32x32x16_bf16(bh[kk][j], al[kk][i], acc[i][j], 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < TI; ++i)
+#pragma unroll
+          for (int j = 0; j < TJ; ++j)
+            acc[i][j] =
+                __builtin_amdgcn_mfma_f32_32x32x16_bf16(bh[kk][j], ah[kk][i], acc[i][j], 0, 0, 0);
+      }
+    } else {
 #pragma unroll
     for (int kk = 0; kk < 2; ++kk)
 #pragma unroll
@@ -462,6 +490,7 @@ __global__ __launch_bounds__(64 * WGM * WGN, 1) void gemm_b3d_kernel(B3Args p) {
           acc[i][j] =
               __builtin_amdgcn_mfma_f32_32x32x16_bf16(bh[kk][j], ah[kk][i], acc[i][j], 0, 0, 0);
       }
+    }
     // schedule: the first step's reads, then the second step's reads
     // interleaved one to one with the first step's MFMAs, then the rest
     // (SCHED = 1; SCHED = 0 leaves the order to the compiler)
@@ -847,6 +876,7 @@ static const B3Variant kB3Variants[] = {
     {256, 128, launch_b3d<256, 128, 2, 2, 3, 3, 2>, "dma256x128_w4_s3_x2"},
     {256, 256, launch_b3d<256, 256, 2, 2, 3, 2, 2>, "dma256x256_w4_s2_x2"},
     {256, 256, launch_b3d<256, 256, 2, 4, 3, 2, 2, 1>, "dma256x256_w8_s2_x2_il"},
+    {256, 256, launch_b3d<256, 256, 2, 4, 3, 2, 2, 2>, "dma256x256_w8_s2_x2_po"},
 };
 static const B3Variant kB1Variants[] = {
     {128, 128, launch_b3p<128, 128, 64, 64, 1, 1, 1>, "lds128x128"},

@@ -13,7 +13,13 @@ typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
 constexpr int BM = 128, BN = 128;
 
+// Cube-blocked coordinate: (x & 7) * lo + (x >> 3) * hi.
+__device__ __forceinline__ int64_t cube_coord(int x, int64_t lo, int64_t hi) {
+  return int64_t(x & 7) * lo + int64_t(x >> 3) * hi;
+}
+
 __device__ __forceinline__ int64_t out_off_m(const GemmArgs &g, int m) {
+  if (g.out_cube) return cube_coord(m % g.dm0, g.cm0, g.cm1) + cube_coord(m / g.dm0, g.cm2, g.cm3);
   if (!g.split_out) return int64_t(m) * g.cm0;
   int m0 = m % g.dm0;
   int t = m / g.dm0;
@@ -23,6 +29,7 @@ __device__ __forceinline__ int64_t out_off_m(const GemmArgs &g, int m) {
 }
 
 __device__ __forceinline__ int64_t out_off_n(const GemmArgs &g, int n) {
+  if (g.out_cube) return cube_coord(n % g.dn0, g.cn0, g.cn1) + cube_coord(n / g.dn0, g.cn2, g.cn3);
   if (!g.split_out) return int64_t(n) * g.cn0;
   int n0 = n % g.dn0;
   int t = n / g.dn0;

@@ -21,10 +21,12 @@
 // positions, so sf'(sigma) = sf(sigma) + sf(sigma.tau) = +4 for sigma in
 // {id, tau}, -2 otherwise; when all three are equal sf' = 0 and the block
 // contributes nothing (as in the reference, whose i=j=k terms cancel).
-// The energy kernel walks orbits of 8x8x8 cubes under index permutations:
-// the (up to) six permuted cubes of an orbit are staged in LDS once and every
-// cube of the orbit is evaluated from them, so W is read once; the sum is
-// reduced in FP64.
+// W is stored cube-blocked (each 8x8x8 cube of indices contiguous, written so
+// by the GEMM epilogues).  The energy kernel walks orbits of cubes under
+// index permutations: the (up to) six permuted cubes of an orbit are staged
+// in LDS once and every cube of the orbit is evaluated from them, so W is
+// read once, in whole 4 KB cubes; the kernel is persistent and fetches the
+// next orbit while it evaluates the current one; the sum is reduced in FP64.
 #include "triples.hpp"
 
 #include "contract.hpp"
@@ -68,9 +70,14 @@ struct EnergyArgs {
   double *part;
   double eps_ijk;
   double sfp[6];
-  int Nv, No, nterms;
-  PermTerm terms[6];
+  int Nv, No, norbits;
+  int tmask;          // bit p: permutation p (kPerm3[p]) is a distinct singles term
+  PermTerm terms[6];  // by permutation index p (pi = kPerm3[p], idx = i.pi)
 };
+
+// Permutation<3>(p) for p = 0..5 (math/Permutation.hpp:52-62), the order of
+// permutation3(); the energy kernel unrolls over it.
+constexpr int kPerm3[6][3] = {{0, 1, 2}, {1, 0, 2}, {1, 2, 0}, {0, 2, 1}, {2, 0, 1}, {2, 1, 0}};
 
 // sigma as reads (x_s0, x_s1, x_s2): identity, three transpositions, two 3-cycles
 constexpr int hSig[6][3] = {{0, 1, 2}, {1, 0, 2}, {2, 1, 0}, {0, 2, 1}, {1, 2, 0}, {2, 0, 1}};
@@ -100,13 +107,13 @@ __device__ __forceinline__ double block_sum_t(double v) {
   return r;
 }
 
-// LDS layout of one 8x8x8 cube: (l0, l1, l2) at l0 + 9 l1 + 81 l2.  The odd
-// strides spread the permuted reads (an index of the wave's lanes moved to
-// the stride-9 or stride-81 position) over the banks.
-constexpr int kCubeS1 = 9, kCubeS2 = 81, kCubePad = 648;
-
-__device__ __forceinline__ int cube_off(int p0, int p1, int p2) {
-  return p0 + kCubeS1 * p1 + kCubeS2 * p2;
+// LDS layout of one 8x8x8 cube: (a, b, c) at 64 c + 8 b + (a ^ ((b + c) & 7)).
+// A wave holds one l2 plane of 8 x 8 elements; reading the cube permuted puts
+// the lane-varying indices at strides 1, 8 or 64, and the XOR spreads the
+// strided ones over the banks (the pair (2k, 2k+1) stays a pair, so the two
+// doubles a lane loads land in one aligned 16-byte slot).
+__device__ __forceinline__ int sw_off(int a, int b, int c) {
+  return 64 * c + 8 * b + (a ^ ((b + c) & 7));
 }
 
 // composition sig[q] o sig[t] as an index of hSig (folded at compile time
@@ -125,109 +132,164 @@ __device__ __forceinline__ int sel3(int a, int b, int c, int i) {
   return i == 0 ? a : (i == 1 ? b : c);
 }
 
-// One workgroup per orbit of cubes.  Phase 1 issues every global load of
-// the orbit at once: the (up to) six permuted W cubes, and for each
-// distinct cube Y = X.sigma_q and each singles term t the factors
-// st[q][t][l] = T1[8 Y_pi0 + l, i_pi0] / 2 and sv[q][t][8 l1 + l2] =
-// V[8 Y_pi1 + l1, 8 Y_pi2 + l2, i_pi1, i_pi2] (S(i.pi)[x.pi] is their
-// product), and the particle energies of the orbit's three cube ranges.
-// After one barrier, phase 2 evaluates every element of every distinct
-// cube from LDS alone.  TWO: a second singles product (complex amplitudes).
+// Persistent kernel over the orbits of 8x8x8 cubes under index permutations
+// (workgroup b takes orbits b, b + grid, ...).  W is cube-blocked, so each of
+// the six permuted cubes X.sigma_r of an orbit is one contiguous 4 KB read,
+// 16 bytes per lane.  The loads of the next orbit (the cubes, and for each
+// cube Y = X.sigma_q and singles term t the factors st[q][t][l] =
+// T1[8 Y_pi0 + l, i_pi0] / 2 and sv[q][t][8 l1 + l2] = V[8 Y_pi1 + l1,
+// 8 Y_pi2 + l2, i_pi1, i_pi2], whose product is S(i.pi)[x.pi], and the
+// particle energies of the orbit's cube ranges) are issued into registers
+// before the current orbit is evaluated from LDS, and written to LDS after
+// it.  Every element of every distinct cube: F = (sum_sigma sf'(sigma)
+// W[x.sigma]) / D times Z = W + sum_pi S_pi.  TWO: a second singles product
+// (complex amplitudes).  Partial sums per workgroup in a fixed order.
 template <bool TWO>
 __global__ __launch_bounds__(256) void triples_energy_kernel(EnergyArgs g) {
-  __shared__ double tile[6][kCubePad];
+  __shared__ double tile[6][512];
   __shared__ double sv[6][6][64], st[6][6][8];
   __shared__ double sv2[TWO ? 6 : 1][6][64], st2[TWO ? 6 : 1][6][8];
   __shared__ double ea[3][8];
-  const int code = g.orbits[blockIdx.x];
-  const int X0 = code & 1023, X1 = (code >> 10) & 1023, X2 = code >> 20;
-  const int Nv = g.Nv, tid = threadIdx.x, nterms = g.nterms;
+  const int Nv = g.Nv, tid = threadIdx.x, norbits = g.norbits, tmask = g.tmask;
+  const int C = (Nv + 7) >> 3;
   const int64_t s1 = Nv, s2 = int64_t(Nv) * Nv;
-  // distinct cubes of the orbit: q is distinct when no earlier p gives the
-  // same cube (uniform)
-  bool distinct[6];
+  // registers of the orbit being fetched; the singles factors are fetched per
+  // (cube q, permutation t) pair, pair q * 6 + t = wave + 4 k for k < 9, so
+  // that the pair's indices are uniform across the wave (scalar operands)
+  double rt[6][2];
+  double rsv[9], rst[9], rsv2[TWO ? 9 : 1], rst2[TWO ? 9 : 1], rea = 0.0;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+
+  auto decode = [](int code, int &X0, int &X1, int &X2) {
+    X0 = code & 1023;
+    X1 = (code >> 10) & 1023;
+    X2 = code >> 20;
+  };
+  auto fetch = [&](int code) {
+    int X0, X1, X2;
+    decode(code, X0, X1, X2);
 #pragma unroll
-  for (int q = 0; q < 6; ++q) {
-    const int y0 = sel3(X0, X1, X2, hSig[q][0]), y1 = sel3(X0, X1, X2, hSig[q][1]),
-              y2 = sel3(X0, X1, X2, hSig[q][2]);
-    bool dup = false;
-#pragma unroll
-    for (int p = 0; p < q; ++p)
-      dup = dup || (sel3(X0, X1, X2, hSig[p][0]) == y0 && sel3(X0, X1, X2, hSig[p][1]) == y1 &&
-                    sel3(X0, X1, X2, hSig[p][2]) == y2);
-    distinct[q] = !dup;
-  }
-  // phase 1: tile r is the cube X.sigma_r (rows of 8 doubles, 8 lanes a row);
-  // phase 2 reads every tile of the composition table, so a repeated cube is
-  // staged under each of its indices (its second read hits L2)
-#pragma unroll
-  for (int r = 0; r < 6; ++r) {
-    const int b0 = sel3(X0, X1, X2, hSig[r][0]) * 8, b1 = sel3(X0, X1, X2, hSig[r][1]) * 8,
-              b2 = sel3(X0, X1, X2, hSig[r][2]) * 8;
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int e = tid + 256 * h;
-      const int l0 = e & 7, l1 = (e >> 3) & 7, l2 = e >> 6;
-      const int u0 = b0 + l0, u1 = b1 + l1, u2 = b2 + l2;
-      tile[r][cube_off(l0, l1, l2)] =
-          (u0 < Nv && u1 < Nv && u2 < Nv) ? g.W[u0 + u1 * s1 + u2 * s2] : 0.0;
+    for (int r = 0; r < 6; ++r) {
+      const int64_t base = 512 * (int64_t(sel3(X0, X1, X2, hSig[r][0])) +
+                                  int64_t(C) * (sel3(X0, X1, X2, hSig[r][1]) +
+                                                int64_t(C) * sel3(X0, X1, X2, hSig[r][2])));
+      const double2 w2 = *reinterpret_cast<const double2 *>(g.W + base + 2 * tid);
+      rt[r][0] = w2.x;
+      rt[r][1] = w2.y;
     }
-  }
 #pragma unroll
-  for (int q = 0; q < 6; ++q) {
-    if (!distinct[q]) continue;
-    const int Y[3] = {sel3(X0, X1, X2, hSig[q][0]), sel3(X0, X1, X2, hSig[q][1]),
-                      sel3(X0, X1, X2, hSig[q][2])};
-    for (int w = tid; w < nterms * 64; w += 256) {
-      const int t = w >> 6, l1 = (w >> 3) & 7, l2 = w & 7;
-      const PermTerm &p = g.terms[t];
-      const int u1 = sel3(Y[0], Y[1], Y[2], p.pi[1]) * 8 + l1;
-      const int u2 = sel3(Y[0], Y[1], Y[2], p.pi[2]) * 8 + l2;
-      const bool in = u1 < Nv && u2 < Nv;
-      const int64_t vv = u1 + s1 * u2 + s2 * (p.idx[1] + int64_t(g.No) * p.idx[2]);
-      sv[q][t][8 * l1 + l2] = in ? g.V[vv] : 0.0;
-      if (TWO) sv2[q][t][8 * l1 + l2] = in ? g.Vb[vv] : 0.0;
-      if (l1 == 0) {
-        const int u0 = sel3(Y[0], Y[1], Y[2], p.pi[0]) * 8 + l2;
-        const int64_t t1 = u0 + int64_t(Nv) * p.idx[0];
-        st[q][t][l2] = u0 < Nv ? 0.5 * g.T1[t1] : 0.0;
-        if (TWO) st2[q][t][l2] = u0 < Nv ? 0.5 * g.sb * g.T1b[t1] : 0.0;
-      }
-    }
-  }
-  if (tid < 24) {
-    const int k = tid >> 3, l = tid & 7, u = sel3(X0, X1, X2, k) * 8 + l;
-    ea[k][l] = u < Nv ? g.eps_a[u] : 0.0;
-  }
-  __syncthreads();
-  // phase 2
-  double acc = 0.0;
-#pragma unroll
-  for (int q = 0; q < 6; ++q) {
-    if (!distinct[q]) continue;
-    const int Y0 = sel3(X0, X1, X2, hSig[q][0]), Y1 = sel3(X0, X1, X2, hSig[q][1]),
-              Y2 = sel3(X0, X1, X2, hSig[q][2]);
-    for (int e = tid; e < 512; e += 256) {
-      const int l0 = e & 7, l1 = (e >> 3) & 7, l2 = e >> 6;
-      if (Y0 * 8 + l0 >= Nv || Y1 * 8 + l1 >= Nv || Y2 * 8 + l2 >= Nv) continue;
-      double f = 0.0;
-#pragma unroll
-      for (int t = 0; t < 6; ++t)
-        f += g.sfp[t] * tile[comp_idx(q, t)][cube_off(sel3(l0, l1, l2, hSig[t][0]),
-                                                      sel3(l0, l1, l2, hSig[t][1]),
-                                                      sel3(l0, l1, l2, hSig[t][2]))];
-      // axis k of Y is axis sigma_q(k) of X
-      const double D = g.eps_ijk - ea[hSig[q][0]][l0] - ea[hSig[q][1]][l1] - ea[hSig[q][2]][l2];
-      double z = tile[q][cube_off(l0, l1, l2)];
-      for (int t = 0; t < nterms; ++t) {
+    for (int k = 0; k < 9; ++k) {
+      const int pair = wave + 4 * k, q = pair / 6, t = pair - 6 * q;
+      double x = 0.0, x2 = 0.0, y = 0.0, y2 = 0.0;
+      if ((tmask >> t) & 1) {
         const PermTerm &p = g.terms[t];
-        const int a = sel3(l0, l1, l2, p.pi[0]);
-        const int b = 8 * sel3(l0, l1, l2, p.pi[1]) + sel3(l0, l1, l2, p.pi[2]);
-        z += st[q][t][a] * sv[q][t][b];
-        if (TWO) z += st2[q][t][a] * sv2[q][t][b];
+        const int Y0 = sel3(X0, X1, X2, hSig[q][0]), Y1 = sel3(X0, X1, X2, hSig[q][1]),
+                  Y2 = sel3(X0, X1, X2, hSig[q][2]);
+        const int l1 = lane >> 3, l2 = lane & 7;
+        const int u1 = sel3(Y0, Y1, Y2, p.pi[1]) * 8 + l1, u2 = sel3(Y0, Y1, Y2, p.pi[2]) * 8 + l2;
+        if (u1 < Nv && u2 < Nv) {
+          const int64_t vv = u1 + s1 * u2 + s2 * (p.idx[1] + int64_t(g.No) * p.idx[2]);
+          x = g.V[vv];
+          if (TWO) x2 = g.Vb[vv];
+        }
+        const int u0 = sel3(Y0, Y1, Y2, p.pi[0]) * 8 + lane;
+        if (lane < 8 && u0 < Nv) {
+          const int64_t t1 = u0 + int64_t(Nv) * p.idx[0];
+          y = 0.5 * g.T1[t1];
+          if (TWO) y2 = 0.5 * g.sb * g.T1b[t1];
+        }
       }
-      acc += f * z / D;
+      rsv[k] = x;
+      rst[k] = y;
+      if (TWO) {
+        rsv2[k] = x2;
+        rst2[k] = y2;
+      }
     }
+    if (tid < 24) {
+      const int k = tid >> 3, l = tid & 7, u = sel3(X0, X1, X2, k) * 8 + l;
+      rea = u < Nv ? g.eps_a[u] : 0.0;
+    }
+  };
+  auto stash = [&]() {
+    const int e0 = 2 * tid, a = e0 & 7, b = (e0 >> 3) & 7, c = e0 >> 6;
+#pragma unroll
+    for (int r = 0; r < 6; ++r) {
+      tile[r][sw_off(a, b, c)] = rt[r][0];
+      tile[r][sw_off(a + 1, b, c)] = rt[r][1];
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      const int pair = wave + 4 * k, q = pair / 6, t = pair - 6 * q;
+      sv[q][t][lane] = rsv[k];
+      if (TWO) sv2[q][t][lane] = rsv2[k];
+      if (lane < 8) {
+        st[q][t][lane] = rst[k];
+        if (TWO) st2[q][t][lane] = rst2[k];
+      }
+    }
+    if (tid < 24) ea[tid >> 3][tid & 7] = rea;
+  };
+
+  // this thread's elements e = tid + 256 h of a cube and, per sigma_t, the
+  // LDS offset of the element x.sigma_t (the same for every orbit)
+  int lx[2][3], off[2][6];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int e = tid + 256 * h;
+    lx[h][0] = e & 7;
+    lx[h][1] = (e >> 3) & 7;
+    lx[h][2] = e >> 6;
+#pragma unroll
+    for (int t = 0; t < 6; ++t)
+      off[h][t] = sw_off(lx[h][hSig[t][0]], lx[h][hSig[t][1]], lx[h][hSig[t][2]]);
+  }
+  double acc = 0.0;
+  int o = blockIdx.x;
+  if (o < norbits) fetch(g.orbits[o]);
+  while (o < norbits) {
+    const int code = g.orbits[o];
+    __syncthreads();  // the previous orbit's reads of LDS are done
+    stash();
+    __syncthreads();
+    const int nxt = o + int(gridDim.x);
+    if (nxt < norbits) fetch(g.orbits[nxt]);  // in flight while this orbit is evaluated
+    int X0, X1, X2;
+    decode(code, X0, X1, X2);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      // the cube Y = X.sigma_q, once per distinct cube of the orbit (uniform)
+      const int Y0 = sel3(X0, X1, X2, hSig[q][0]), Y1 = sel3(X0, X1, X2, hSig[q][1]),
+                Y2 = sel3(X0, X1, X2, hSig[q][2]);
+      bool dup = false;
+#pragma unroll
+      for (int p = 0; p < q; ++p)
+        dup = dup || (sel3(X0, X1, X2, hSig[p][0]) == Y0 && sel3(X0, X1, X2, hSig[p][1]) == Y1 &&
+                      sel3(X0, X1, X2, hSig[p][2]) == Y2);
+      if (dup) continue;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int l0 = lx[h][0], l1 = lx[h][1], l2 = lx[h][2];
+        if (Y0 * 8 + l0 >= Nv || Y1 * 8 + l1 >= Nv || Y2 * 8 + l2 >= Nv) continue;
+        double f = 0.0;
+#pragma unroll
+        for (int t = 0; t < 6; ++t) f += g.sfp[t] * tile[comp_idx(q, t)][off[h][t]];
+        // axis k of Y is axis sigma_q(k) of X
+        const double D =
+            g.eps_ijk - ea[hSig[q][0]][l0] - ea[hSig[q][1]][l1] - ea[hSig[q][2]][l2];
+        double z = tile[q][off[h][0]];
+#pragma unroll
+        for (int t = 0; t < 6; ++t) {
+          if (!((tmask >> t) & 1)) continue;
+          const int a = lx[h][kPerm3[t][0]];
+          const int b = 8 * lx[h][kPerm3[t][1]] + lx[h][kPerm3[t][2]];
+          z += st[q][t][a] * sv[q][t][b];
+          if (TWO) z += st2[q][t][a] * sv2[q][t][b];
+        }
+        acc += f * z / D;
+      }
+    }
+    o = nxt;
   }
   acc = block_sum_t(acc);
   if (tid == 0) g.part[blockIdx.x] = acc;
@@ -247,6 +309,40 @@ struct TriplesOps {
   const double *VP;            // VP[l,c,j,k] = V_jklc
 };
 
+// Output addressing of the W products for output index j as the GEMM row
+// (j = 0: rows x0, columns x1 + v x2; else rows x_p + v x_q, columns x_j):
+// W is stored as contiguous 8 x 8 x 8 cubes (w_cube_size), element (x0, x1,
+// x2) at 512 (X0 + C X1 + C^2 X2) + l0 + 8 l1 + 64 l2 with X = x >> 3,
+// l = x & 7, C = ceil(v / 8) cubes per index.
+void w_cube_addressing(GemmArgs &g, int v, int j, int p, int q) {
+  const int64_t C = (v + 7) / 8;
+  const int64_t lo[3] = {1, 8, 64}, hi[3] = {512, 512 * C, 512 * C * C};
+  g.out_cube = 1;
+  g.dm0 = v;
+  g.dn0 = v;
+  if (j == 0) {
+    g.cm0 = lo[0];
+    g.cm1 = hi[0];
+    g.cn0 = lo[1];
+    g.cn1 = hi[1];
+    g.cn2 = lo[2];
+    g.cn3 = hi[2];
+  } else {
+    g.cm0 = lo[p];
+    g.cm1 = hi[p];
+    g.cm2 = lo[q];
+    g.cm3 = hi[q];
+    g.cn0 = lo[j];
+    g.cn1 = hi[j];
+  }
+}
+
+// Doubles of a cube-blocked W of v^3 elements.
+int64_t w_cube_size(int64_t v) {
+  const int64_t C = (v + 7) / 8;
+  return 512 * C * C * C;
+}
+
 // W[x] = sum over the given permutations of W(i.pi)[x.pi], three launches.
 // The first factor of every product (T2 of the particle terms, VP of the hole
 // terms) comes from opA, the second from opB, the products are scaled by
@@ -256,7 +352,6 @@ void build_w(Engine &e, int prec, const TriplesOps &opA, const TriplesOps &opB,
              const std::array<int, 3> &ib, const std::array<std::array<int, 3>, 6> &perms,
              const bool *use, double *W, double sign = 1.0, bool accumulate = false) {
   const int64_t o = e.No, v = e.Nv;
-  const int64_t st[3] = {1, v, v * v};
   bool first = !accumulate;
   for (int j = 0; j < 3; ++j) {
     const int p = j == 0 ? 1 : 0, q = j == 2 ? 1 : 2;  // the other two positions, p < q
@@ -297,13 +392,9 @@ void build_w(Engine &e, int prec, const TriplesOps &opA, const TriplesOps &opB,
     g.beta = first ? 0.0 : 1.0;
     first = false;
     if (j == 0) {
-      // rows x0 (unit stride), columns (x1, x2) at x1*v + x2*v^2 = n*v
+      // rows x0, columns (x1, x2) as n = x1 + v x2
       g.M = int(v);
       g.N = int(v * v);
-      g.cm0 = 1;
-      g.dm0 = int(v);
-      g.cn0 = v;
-      g.dn0 = int(v * v);
     } else {
       // rows and columns exchanged so that lanes run along x0: C^T = B^T A^T
       for (int s = 0; s < a.nseg; ++s) {
@@ -317,13 +408,8 @@ void build_w(Engine &e, int prec, const TriplesOps &opA, const TriplesOps &opB,
       }
       g.M = int(v * v);
       g.N = int(v);
-      g.cm0 = st[p];
-      g.dm0 = int(v);
-      g.cm1 = st[q];
-      g.dm1 = int(v);
-      g.cn0 = st[j];
-      g.dn0 = int(v);
     }
+    w_cube_addressing(g, int(v), j, p, q);
     gemm_segments(e, prec, a);
   }
 }
@@ -340,7 +426,6 @@ struct TriplesPlanes {
 void build_w_b3(Engine &e, const TriplesPlanes &pl, const std::array<int, 3> &ib,
                 const std::array<std::array<int, 3>, 6> &perms, const bool *use, double *W) {
   const int64_t o = e.No, v = e.Nv;
-  const int64_t st[3] = {1, v, v * v};
   bool first = true;
   for (int j = 0; j < 3; ++j) {
     const int p = j == 0 ? 1 : 0, q = j == 2 ? 1 : 2;
@@ -377,10 +462,6 @@ void build_w_b3(Engine &e, const TriplesPlanes &pl, const std::array<int, 3> &ib
     if (j == 0) {
       g.M = int(v);
       g.N = int(v * v);
-      g.cm0 = 1;
-      g.dm0 = int(v);
-      g.cn0 = v;
-      g.dn0 = int(v * v);
     } else {  // C^T = B^T A^T so lanes run along x0
       for (int s = 0; s < a.nseg; ++s) {
         B3Seg &sg = a.seg[s];
@@ -389,13 +470,8 @@ void build_w_b3(Engine &e, const TriplesPlanes &pl, const std::array<int, 3> &ib
       }
       g.M = int(v * v);
       g.N = int(v);
-      g.cm0 = st[p];
-      g.dm0 = int(v);
-      g.cm1 = st[q];
-      g.dm1 = int(v);
-      g.cn0 = st[j];
-      g.dn0 = int(v);
     }
+    w_cube_addressing(g, int(v), j, p, q);
     gemm_b3_segments(e, a);
   }
 }
@@ -423,14 +499,15 @@ void block_energy_args(EnergyArgs &g, const std::array<int, 3> &ib,
   // tau: the swap of the equal positions (sig index 1 swaps 0,1; 3 swaps 1,2)
   const int tau = ib[0] == ib[1] ? 1 : ib[1] == ib[2] ? 3 : -1;
   for (int s = 0; s < 6; ++s) g.sfp[s] = hSf[s] + (tau >= 0 ? hSf[compose_idx(s, tau)] : 0.0);
-  g.nterms = 0;
+  g.tmask = 0;
   for (int p = 0; p < 6; ++p) {
-    if (!distinct[p]) continue;
-    PermTerm &t = g.terms[g.nterms++];
+    PermTerm &t = g.terms[p];
     for (int k = 0; k < 3; ++k) {
+      SISI_REQUIRE(perms[p][k] == kPerm3[p][k], SISI_ERR_STATE, "triples: permutation table");
       t.pi[k] = perms[p][k];
       t.idx[k] = tup[p][k];
     }
+    if (distinct[p]) g.tmask |= 1 << p;
   }
 }
 
@@ -446,6 +523,13 @@ std::vector<std::array<int, 3>> checked_blocks(const Engine &e,
     SISI_REQUIRE(b[0] >= 0 && b[0] <= b[1] && b[1] <= b[2] && b[2] < o, SISI_ERR_INVALID,
                  "triples: blocks must be sorted i0 <= i1 <= i2 < No");
   return blocks;
+}
+
+// Workgroups of the persistent energy kernel: as many as are resident at
+// once (three per CU, two with the second singles product's LDS), at most
+// one per orbit.
+int energy_grid(const Engine &e, int norbits, bool two) {
+  return std::max(1, std::min(norbits, (two ? 2 : 3) * e.cus));
 }
 
 // Cube orbits X0 <= X1 <= X2 of the energy kernel, uploaded.
@@ -497,7 +581,7 @@ double ztriples_energy(Engine &e, const double *T1, const double *T2, int prec,
                       e.VP.get() + nvp};
   int norbits = 0;
   DBuf orbits = orbit_codes(e, norbits);
-  const int64_t v3 = v * v * v;
+  const int64_t v3 = w_cube_size(v);
   DBuf W(&e.pool, 2 * v3);
   DBuf part(&e.pool, norbits), Eb(&e.pool, 2 * blocks.size());
   zero(e, Eb.get(), int64_t(2 * blocks.size()));
@@ -531,9 +615,11 @@ double ztriples_energy(Engine &e, const double *T1, const double *T2, int prec,
       g.Nv = int(v);
       g.No = int(o);
       block_energy_args(g, ib, perms, tup, distinct, eps_i);
-      Timed tm(*e.timers, Kind::Bandwidth, 8.0 * double(v3));
-      hipLaunchKernelGGL(triples_energy_kernel<true>, dim3(norbits), dim3(256), 0, e.stream, g);
-      hipLaunchKernelGGL(sum_into_kernel, dim3(1), dim3(256), 0, e.stream, part.get(), norbits,
+      g.norbits = norbits;
+      const int grid = energy_grid(e, norbits, true);
+      Timed tm(*e.timers, Kind::Bandwidth, 8.0 * double(v * v * v));
+      hipLaunchKernelGGL(triples_energy_kernel<true>, dim3(grid), dim3(256), 0, e.stream, g);
+      hipLaunchKernelGGL(sum_into_kernel, dim3(1), dim3(256), 0, e.stream, part.get(), grid,
                          Eb.get() + 2 * bi + plane);
       SISI_HIP(hipGetLastError());
     }
@@ -600,7 +686,7 @@ double triples_energy(Engine &e, const double *T1, const double *T2, int prec,
 
   int norbits = 0;
   DBuf orbits_buf = orbit_codes(e, norbits);
-  DBuf W(&e.pool, v * v * v);
+  DBuf W(&e.pool, w_cube_size(v));
   DBuf part(&e.pool, norbits), Eb(&e.pool, blocks.size());
   zero(e, Eb.get(), blocks.size());
   std::array<std::array<int, 3>, 6> perms;
@@ -625,10 +711,12 @@ double triples_energy(Engine &e, const double *T1, const double *T2, int prec,
     g.Nv = int(v);
     g.No = int(o);
     block_energy_args(g, ib, perms, tup, distinct, eps_i);
+    g.norbits = norbits;
     {
+      const int grid = energy_grid(e, norbits, false);
       Timed tm(*e.timers, Kind::Bandwidth, 8.0 * double(v * v * v));
-      hipLaunchKernelGGL(triples_energy_kernel<false>, dim3(norbits), dim3(256), 0, e.stream, g);
-      hipLaunchKernelGGL(sum_into_kernel, dim3(1), dim3(256), 0, e.stream, part.get(), norbits,
+      hipLaunchKernelGGL(triples_energy_kernel<false>, dim3(grid), dim3(256), 0, e.stream, g);
+      hipLaunchKernelGGL(sum_into_kernel, dim3(1), dim3(256), 0, e.stream, part.get(), grid,
                          Eb.get() + bi);
       SISI_HIP(hipGetLastError());
     }
