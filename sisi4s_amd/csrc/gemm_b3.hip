@@ -973,6 +973,8 @@ static TuneChoice tune_b3(Engine &e, const GemmArgs &g0, int passes, const Plane
   g.alpha = 1.0;
   g.beta = 0.0;
   g.out_planes = 0;
+  g.out_cube = 0;
+  g.cm3 = g.cn3 = 0;
   const bool stats = e.timers->enabled;
   e.timers->enabled = false;
   hipEvent_t t0, t1;

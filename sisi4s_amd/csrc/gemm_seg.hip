@@ -383,13 +383,17 @@ static const SegVariant kSeg64[] = {
     {128, 128, launch_seg<SISI_FP64, 128, 128, 2, 2>, "seg128x128"},
     {80, 128, launch_seg<SISI_FP64, 80, 128, 1, 4>, "seg80x128"},
     {128, 80, launch_seg<SISI_FP64, 128, 80, 4, 1>, "seg128x80"},
+    {80, 192, launch_seg<SISI_FP64, 80, 192, 1, 4>, "seg80x192"},
+    {256, 80, launch_seg<SISI_FP64, 256, 80, 4, 1>, "seg256x80"},
 };
 static const SegVariant kSeg32[] = {
     {128, 128, launch_seg<SISI_FP32, 128, 128, 2, 2>, "seg128x128"},
     {80, 128, launch_seg<SISI_FP32, 80, 128, 1, 4>, "seg80x128"},
     {128, 80, launch_seg<SISI_FP32, 128, 80, 4, 1>, "seg128x80"},
+    {80, 192, launch_seg<SISI_FP32, 80, 192, 1, 4>, "seg80x192"},
+    {256, 80, launch_seg<SISI_FP32, 256, 80, 4, 1>, "seg256x80"},
 };
-constexpr int kNumSeg = 3;
+constexpr int kNumSeg = 5;
 
 // Size rule when no tuning record exists: an 80-wide tile along a dimension
 // that is a multiple of 80 but not of 128, when that leaves the other at 128.
@@ -425,11 +429,13 @@ static TuneChoice tune_seg(Engine &eng, const SegGemmArgs &a0, int precision, co
   g.C = scratch.get();
   g.cm0 = 1;
   g.cn0 = g.M;
-  g.cm1 = g.cm2 = g.cn1 = g.cn2 = 0;
+  g.cm1 = g.cm2 = g.cm3 = g.cn1 = g.cn2 = g.cn3 = 0;
   g.dm0 = g.M;
   g.dn0 = g.N;
   g.dm1 = g.dn1 = 1;
   g.split_out = 0;
+  g.out_cube = 0;  // dense M x N scratch, whatever the product writes
+  g.out_planes = 0;
   g.alpha = 1.0;
   g.beta = 0.0;
   const bool stats = eng.timers->enabled;

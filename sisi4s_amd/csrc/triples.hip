@@ -147,8 +147,10 @@ __device__ __forceinline__ int sel3(int a, int b, int c, int i) {
 template <bool TWO>
 __global__ __launch_bounds__(256) void triples_energy_kernel(EnergyArgs g) {
   __shared__ double tile[6][512];
-  __shared__ double sv[6][6][64], st[6][6][8];
-  __shared__ double sv2[TWO ? 6 : 1][6][64], st2[TWO ? 6 : 1][6][8];
+  // V blocks as 8 rows of 9 (one pad double per row: reads along the rows'
+  // first index hit distinct banks)
+  __shared__ double sv[6][6][72], st[6][6][8];
+  __shared__ double sv2[TWO ? 6 : 1][6][72], st2[TWO ? 6 : 1][6][8];
   __shared__ double ea[3][8];
   const int Nv = g.Nv, tid = threadIdx.x, norbits = g.norbits, tmask = g.tmask;
   const int C = (Nv + 7) >> 3;
@@ -159,6 +161,21 @@ __global__ __launch_bounds__(256) void triples_energy_kernel(EnergyArgs g) {
   double rt[6][2];
   double rsv[9], rst[9], rsv2[TWO ? 9 : 1], rst2[TWO ? 9 : 1], rea = 0.0;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+  // the orbit-independent part of pair k's singles loads: which orbit index
+  // X_c gives each of the three cube coordinates (c = sigma_q(pi_k)), the
+  // occupied-index offsets into V and T1, and whether the term is used
+  // (packed: scode = c0 | c1 << 2 | c2 << 4 | used << 6)
+  int scode[9], stoff[9];
+  int64_t svoff[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    const int pair = wave + 4 * k, q = pair / 6, t = pair - 6 * q;
+    const PermTerm &p = g.terms[t];
+    scode[k] = hSig[q][p.pi[0]] | (hSig[q][p.pi[1]] << 2) | (hSig[q][p.pi[2]] << 4) |
+               (((tmask >> t) & 1) << 6);
+    svoff[k] = s2 * (p.idx[1] + int64_t(g.No) * p.idx[2]);
+    stoff[k] = Nv * p.idx[0];
+  }
 
   auto decode = [](int code, int &X0, int &X1, int &X2) {
     X0 = code & 1023;
@@ -179,22 +196,19 @@ __global__ __launch_bounds__(256) void triples_energy_kernel(EnergyArgs g) {
     }
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
-      const int pair = wave + 4 * k, q = pair / 6, t = pair - 6 * q;
       double x = 0.0, x2 = 0.0, y = 0.0, y2 = 0.0;
-      if ((tmask >> t) & 1) {
-        const PermTerm &p = g.terms[t];
-        const int Y0 = sel3(X0, X1, X2, hSig[q][0]), Y1 = sel3(X0, X1, X2, hSig[q][1]),
-                  Y2 = sel3(X0, X1, X2, hSig[q][2]);
+      if ((scode[k] >> 6) & 1) {
         const int l1 = lane >> 3, l2 = lane & 7;
-        const int u1 = sel3(Y0, Y1, Y2, p.pi[1]) * 8 + l1, u2 = sel3(Y0, Y1, Y2, p.pi[2]) * 8 + l2;
+        const int u1 = sel3(X0, X1, X2, (scode[k] >> 2) & 3) * 8 + l1;
+        const int u2 = sel3(X0, X1, X2, (scode[k] >> 4) & 3) * 8 + l2;
         if (u1 < Nv && u2 < Nv) {
-          const int64_t vv = u1 + s1 * u2 + s2 * (p.idx[1] + int64_t(g.No) * p.idx[2]);
+          const int64_t vv = u1 + s1 * u2 + svoff[k];
           x = g.V[vv];
           if (TWO) x2 = g.Vb[vv];
         }
-        const int u0 = sel3(Y0, Y1, Y2, p.pi[0]) * 8 + lane;
+        const int u0 = sel3(X0, X1, X2, scode[k] & 3) * 8 + lane;
         if (lane < 8 && u0 < Nv) {
-          const int64_t t1 = u0 + int64_t(Nv) * p.idx[0];
+          const int64_t t1 = u0 + int64_t(stoff[k]);
           y = 0.5 * g.T1[t1];
           if (TWO) y2 = 0.5 * g.sb * g.T1b[t1];
         }
@@ -221,8 +235,9 @@ __global__ __launch_bounds__(256) void triples_energy_kernel(EnergyArgs g) {
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
       const int pair = wave + 4 * k, q = pair / 6, t = pair - 6 * q;
-      sv[q][t][lane] = rsv[k];
-      if (TWO) sv2[q][t][lane] = rsv2[k];
+      const int pos = 9 * (lane >> 3) + (lane & 7);
+      sv[q][t][pos] = rsv[k];
+      if (TWO) sv2[q][t][pos] = rsv2[k];
       if (lane < 8) {
         st[q][t][lane] = rst[k];
         if (TWO) st2[q][t][lane] = rst2[k];
@@ -282,7 +297,7 @@ __global__ __launch_bounds__(256) void triples_energy_kernel(EnergyArgs g) {
         for (int t = 0; t < 6; ++t) {
           if (!((tmask >> t) & 1)) continue;
           const int a = lx[h][kPerm3[t][0]];
-          const int b = 8 * lx[h][kPerm3[t][1]] + lx[h][kPerm3[t][2]];
+          const int b = 9 * lx[h][kPerm3[t][1]] + lx[h][kPerm3[t][2]];
           z += st[q][t][a] * sv[q][t][b];
           if (TWO) z += st2[q][t][a] * sv2[q][t][b];
         }
