@@ -324,7 +324,7 @@ void ccsd_residual(Engine &e, int it, bool initial_doubles, int slice_size, cons
           g1.cn0 = v;
           g1.cn1 = Nx * Kp;
           g1.C = fa.store.get();
-          g1.out_planes = 1;
+          g1.out_planes = pl == SISI_BF16 ? 2 : 1;
           g1.plane_lo = Mp * Kp;
           gemm_b3(e, g1, pl == SISI_BF16 ? 1 : 3);
           fa.M = int(Nx * Ny);
@@ -591,11 +591,119 @@ void ccsd_residual_delta(Engine &e, int slice_size, const double *T1c, const dou
     axpby(e, G * v * v, 1.0, ddGb.get(), 1.0, Gnb.get());
     View GA = V3(A("dGabP"), G, v, v), Gn = V3(Gnb.get(), G, v, v);
     const int nslices = int((v + slice_size - 1) / slice_size);
-    std::vector<DBuf> sA(nslices), sN(nslices), sD(nslices);
     std::vector<int64_t> lo(nslices), hi(nslices);
     for (int s = 0; s < nslices; ++s) {
       lo[s] = int64_t(s) * slice_size;
       hi[s] = std::min<int64_t>(lo[s] + slice_size, v);
+    }
+    const int64_t smax = std::min<int64_t>(slice_size, v);
+    const int pl = e.prec_ladder;
+    const int npairs = nslices * (nslices + 1) / 2;
+    // V~(T_A) of each pair from the anchor's bf16 planes when the ladder runs
+    // in bf16 or split-BF16x3 (no rebuild, no split), else rebuilt here
+    const bool planes = (pl == SISI_BF16 || pl == SISI_BF16X3) && an.ladder_slice == slice_size &&
+                        an.ladder.size() == size_t(npairs);
+    // fused pairs: dV~ as one product over 2 NG, [dG_n; G_A,n]^T [G_m(T); dG_m],
+    // written straight into bf16 planes (no FP64 dV~, no split), then
+    // R~ = dV~ X(T) + V~(T_A) dX from frozen planes; taken when every pair's
+    // anchor planes exist and every product is large enough for the bf16
+    // kernels (the precision of each product is then the one below)
+    const int64_t smin = hi[nslices - 1] - lo[nslices - 1];
+    bool fused = planes && e.ladder_planes && !e.tape && get_b3_backend() == 0 &&
+                 2.0 * double(smin * smin) * double(v * v) * double(std::min(G, o * o)) >=
+                     kB3MinFlops;
+    for (int p = 0; fused && p < npairs; ++p) fused = an.ladder[size_t(p)].store.get() != nullptr;
+    if (fused) {
+      const int passes = pl == SISI_BF16 ? 1 : 3;
+      const int64_t Kp = (v * v + 31) / 32 * 32;
+      const int64_t Mmax = (smax * smax + 255) / 256 * 256;
+      std::vector<DBuf> LA(nslices), LB(nslices);
+      for (int s = 0; s < nslices; ++s) {
+        const int64_t Ns = hi[s] - lo[s];
+        LA[s] = DBuf(&e.pool, 2 * G * Ns * v);
+        LB[s] = DBuf(&e.pool, 2 * G * Ns * v);
+        // element (g, x, c) at g + 2NG c + 2NG v x, g < NG first factor, g >= NG second
+        auto lay = [&](double *dst, View src) {
+          copy(e, dense_strided(dst, {G, Ns, v}, {1, 2 * G * v, 2 * G}),
+               subview(src, {0, lo[s], 0}, {G, hi[s], v}));
+        };
+        lay(LA[s].get(), ddG);
+        lay(LA[s].get() + G, GA);
+        lay(LB[s].get(), Gn);
+        lay(LB[s].get() + G, ddG);
+        e.frozen.push_back(LA[s].get());  // split into planes once, used by every pair
+        e.frozen.push_back(LB[s].get());
+      }
+      Gnb.reset();
+      DBuf Rxy(&e.pool, smax * smax * o * o);
+      B3FrozenA fd;
+      fd.store = DBuf(&e.pool, size_t((2 * Mmax * Kp * 2 + 7) / 8));
+      if (Kp != v * v) zero(e, fd.store.get(), int64_t(fd.store.size()));
+      int pair = 0;
+      for (int m = 0; m < nslices; ++m)
+        for (int n = m; n < nslices; ++n, ++pair) {
+          const int64_t Nx = hi[n] - lo[n], Ny = hi[m] - lo[m];
+          const int64_t Mp = (Nx * Ny + 255) / 256 * 256;
+          View Rxyij = V4(Rxy.get(), Nx, Ny, o, o);
+          // dV~[x,y,c,d] as plane row x + Nx y, column c + v d
+          GemmArgs g1;
+          g1.A = LA[n].get();
+          g1.sam = 2 * G;
+          g1.sak = 1;
+          g1.B = LB[m].get();
+          g1.sbk = 1;
+          g1.sbn = 2 * G;
+          g1.M = int(v * Nx);
+          g1.N = int(v * Ny);
+          g1.K = int(2 * G);
+          g1.dm0 = int(v);
+          g1.dm1 = int(Nx);
+          g1.cm0 = 1;
+          g1.cm1 = Kp;
+          g1.dn0 = int(v);
+          g1.dn1 = int(Ny);
+          g1.cn0 = v;
+          g1.cn1 = Nx * Kp;
+          g1.C = fd.store.get();
+          g1.out_planes = passes == 1 ? 2 : 1;
+          g1.plane_lo = Mp * Kp;
+          gemm_b3(e, g1, passes);
+          fd.M = int(Nx * Ny);
+          fd.K = int(v * v);
+          fd.Mp = int(Mp);
+          fd.Kp = int(Kp);
+          // Rxyij(m = x + Nx y, n = i + o j) = dV~ X(T) + V~_A dX
+          GemmArgs g2;
+          g2.sbk = 1;
+          g2.sbn = v * v;
+          g2.M = int(Nx * Ny);
+          g2.N = int(o * o);
+          g2.K = int(v * v);
+          g2.C = Rxy.get();
+          g2.dm0 = g2.M;
+          g2.dn0 = g2.N;
+          g2.cm0 = 1;
+          g2.cn0 = Nx * Ny;
+          g2.alpha = 1.0;
+          g2.B = Xnb.get();
+          g2.beta = 0.0;
+          gemm_b3_frozen_a(e, g2, fd, passes);
+          g2.B = dXb.get();
+          g2.beta = 1.0;
+          gemm_b3_frozen_a(e, g2, an.ladder[size_t(pair)], passes);
+          const int64_t a = lo[n], b = lo[m];
+          copy(e, subview(dR2, {a, b, 0, 0}, {a + Nx, b + Ny, o, o}), Rxyij, 1.0, 1.0);
+          if (a > b)
+            sum(e, 1.0, Rxyij, "xyij", 1.0, subview(dR2, {b, a, 0, 0}, {b + Ny, a + Nx, o, o}),
+                "yxji");
+        }
+      for (int s2 = 0; s2 < nslices; ++s2) {  // slices return to the pool at block end
+        e.unfreeze(LA[s2].get());
+        e.unfreeze(LB[s2].get());
+      }
+    }
+    std::vector<DBuf> sA(fused ? 0 : nslices), sN(fused ? 0 : nslices), sD(fused ? 0 : nslices);
+    for (int s = 0; s < nslices && !fused; ++s) {
       const int64_t n = G * (hi[s] - lo[s]) * v;
       sA[s] = DBuf(&e.pool, n);
       sN[s] = DBuf(&e.pool, n);
@@ -607,16 +715,14 @@ void ccsd_residual_delta(Engine &e, int slice_size, const double *T1c, const dou
       e.frozen.push_back(sN[s].get());
       e.frozen.push_back(sD[s].get());
     }
-    const int64_t smax = std::min<int64_t>(slice_size, v);
-    DBuf VAb(&e.pool, smax * smax * v * v), dVb(&e.pool, smax * smax * v * v),
-        Rxy(&e.pool, smax * smax * o * o);
-    const int pl = e.prec_ladder;
-    // V~(T_A) of each pair from the anchor's bf16 planes when the ladder runs
-    // in bf16 or split-BF16x3 (no rebuild, no split), else rebuilt here
-    const bool planes = (pl == SISI_BF16 || pl == SISI_BF16X3) && an.ladder_slice == slice_size &&
-                        an.ladder.size() == size_t(nslices * (nslices + 1) / 2);
+    DBuf VAb, dVb, Rxy;
+    if (!fused) {
+      VAb = DBuf(&e.pool, smax * smax * v * v);
+      dVb = DBuf(&e.pool, smax * smax * v * v);
+      Rxy = DBuf(&e.pool, smax * smax * o * o);
+    }
     int pair = 0;
-    for (int m = 0; m < nslices; ++m)
+    for (int m = 0; m < nslices && !fused; ++m)
       for (int n = m; n < nslices; ++n, ++pair) {
         const int64_t Nx = hi[n] - lo[n], Ny = hi[m] - lo[m];
         View VA = V4(VAb.get(), Nx, Ny, v, v), dV = V4(dVb.get(), Nx, Ny, v, v);
@@ -657,7 +763,7 @@ void ccsd_residual_delta(Engine &e, int slice_size, const double *T1c, const dou
           sum(e, 1.0, Rxyij, "xyij", 1.0, subview(dR2, {b, a, 0, 0}, {b + Ny, a + Nx, o, o}),
               "yxji");
       }
-    for (int s2 = 0; s2 < nslices; ++s2) {  // slices return to the pool at block end
+    for (int s2 = 0; s2 < nslices && !fused; ++s2) {  // slices return to the pool at block end
       e.unfreeze(sA[s2].get());
       e.unfreeze(sN[s2].get());
       e.unfreeze(sD[s2].get());

@@ -42,7 +42,8 @@ struct GemmArgs {
   // bf16 hi/lo planes instead of FP64 (C is then the hi plane as bf16, the
   // output offsets count bf16 elements, lo = hi + plane_lo; beta must be 0):
   // hi = bf16(x), lo = bf16(x - hi) of x = float(alpha * acc), the split an
-  // operand split of the FP64 result would give
+  // operand split of the FP64 result would give; out_planes = 2 writes the
+  // hi plane only (a result read by plain-bf16 products)
   int out_planes = 0;
   int64_t plane_lo = 0;
   // out_cube = 1: each output index splits into two coordinates (x0 = m %

@@ -53,7 +53,7 @@ __device__ __forceinline__ void b3_store(const GemmArgs &g, double *C, int64_t o
     const __bf16 hb = *reinterpret_cast<const __bf16 *>(&h);
     u16 *P = reinterpret_cast<u16 *>(C);
     P[o] = h;
-    P[o + g.plane_lo] = bf16_bits(x - float(hb));
+    if (g.out_planes == 1) P[o + g.plane_lo] = bf16_bits(x - float(hb));
   } else {
     double v = g.alpha * double(acc);
     if (g.beta != 0.0) v += g.beta * C[o];

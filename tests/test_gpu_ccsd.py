@@ -356,7 +356,9 @@ def test_refinement_residual_reduced_precision_error_scales_with_change():
 @pytest.mark.parametrize("prec", ["bf16", "bf16x3"])
 def test_refinement_residual_with_anchor_ladder_planes(prec):
     """Ladder slice pairs large enough for the bf16 kernels take V~(T_A) from
-    the bf16 planes the FP64 anchor kept (No=16, Nv=160, slices of 40): the
+    the bf16 planes the FP64 anchor kept (No=16, Nv=160, slices of 40), with
+    dV~ written as bf16 planes by one product over 2 NG (the default) or
+    formed in FP64 and split (ladder planes off): the
     refinement residual matches the FP64 residual to 1e-9 of its max norm in
     split-BF16x3 for a change of 2e-6 (a wrong plane layout would leave an
     error of the size of the change times |V~|), and in plain bf16 to 1e-6
@@ -369,16 +371,22 @@ def test_refinement_residual_with_anchor_ladder_planes(prec):
     P1, P2 = e.residual(1, TA1 + D1, TA2 + D2, slice_size=40)
     e.residual_anchor(TA1, TA2, slice_size=40)
     e.set_precision(prec)
-    R1, R2 = e.residual_refine(TA1 + D1, TA2 + D2, slice_size=40)
+    R1, R2 = e.residual_refine(TA1 + D1, TA2 + D2, slice_size=40)   # fused dV~ planes
+    e.set_ladder_planes(False)
+    S1, S2 = e.residual_refine(TA1 + D1, TA2 + D2, slice_size=40)   # FP64 dV~, split
+    e.set_ladder_planes(True)
     Q1, Q2 = e.residual(1, TA1 + D1, TA2 + D2, slice_size=40)  # plain reduced precision
     e.set_precision("fp64")
     scale = float(np.abs(P2).max())
     err, err_plain = float(np.abs(R2 - P2).max()), float(np.abs(Q2 - P2).max())
-    print("%s refinement with planes: err %.2e, plain %.2e of |R| %.2e"
-          % (prec, err / scale, err_plain / scale, scale), flush=True)
+    err_s = float(np.abs(S2 - P2).max())
+    print("%s refinement with planes: err %.2e (stored dV~ %.2e), plain %.2e of |R| %.2e"
+          % (prec, err / scale, err_s / scale, err_plain / scale, scale), flush=True)
     tol = 1e-9 if prec == "bf16x3" else 1e-6
-    assert err <= tol * scale and err <= 1e-3 * err_plain
-    assert float(np.abs(R1 - P1).max()) <= tol * scale
+    for E1, E2 in ((R1, R2), (S1, S2)):
+        assert float(np.abs(E2 - P2).max()) <= tol * scale
+        assert float(np.abs(E2 - P2).max()) <= 1e-3 * err_plain
+        assert float(np.abs(E1 - P1).max()) <= tol * scale
 
 
 def test_refined_mixed_solve_matches_fp64():
