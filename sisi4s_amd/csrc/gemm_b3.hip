@@ -262,6 +262,15 @@ __global__ __launch_bounds__(256, MINB) void gemm_b3p_kernel(B3Args p) {
 
   if (g.ksplit == 1) {
     double *C = g.C + int64_t(bz) * g.bsc;
+    // runs of four consecutive columns per lane: offsets hoisted as in the
+    // 256-row kernel's epilogue
+    const bool run4 = (!g.split_out && !g.out_cube) || (g.dn0 & 3) == 0;
+    int64_t offn[TJ];
+#pragma unroll
+    for (int j = 0; j < TJ; ++j) {
+      const int n = n0 + wn + j * 16 + 4 * lk;
+      offn[j] = run4 && n < g.N ? b3_off_n(g, n) : 0;
+    }
 #pragma unroll
     for (int i = 0; i < TI; ++i) {
       const int m = m0 + wm + i * 16 + l16;
@@ -272,7 +281,9 @@ __global__ __launch_bounds__(256, MINB) void gemm_b3p_kernel(B3Args p) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int n = n0 + wn + j * 16 + 4 * lk + r;
-          if (n < g.N) b3_store(g, C, om + b3_off_n(g, n), acc[i][j][r]);
+          if (n < g.N)
+            b3_store(g, C, om + (run4 ? offn[j] + int64_t(r) * g.cn0 : b3_off_n(g, n)),
+                     acc[i][j][r]);
         }
     }
   } else {
@@ -529,6 +540,19 @@ __global__ __launch_bounds__(64 * WGM * WGN, 1) void gemm_b3d_kernel(B3Args p) {
 
   if (g.ksplit == 1) {
     double *C = g.C + int64_t(bz) * g.bsc;
+    // the lane's columns come in aligned runs of four; when the output split
+    // of n keeps such a run together (dn0 a multiple of 4, or no split) the
+    // run's offsets are the first one plus multiples of cn0, computed once
+    // here instead of per row and element
+    const bool run4 = (!g.split_out && !g.out_cube) || (g.dn0 & 3) == 0;
+    int64_t offn[TJ][4];
+#pragma unroll
+    for (int j = 0; j < TJ; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int n = n0 + wn + 32 * j + 8 * q + 4 * lh;
+        offn[j][q] = run4 && n < g.N ? b3_off_n(g, n) : 0;
+      }
 #pragma unroll
     for (int i = 0; i < TI; ++i) {
       const int m = m0 + wm + 32 * i + l32;
@@ -539,7 +563,10 @@ __global__ __launch_bounds__(64 * WGM * WGN, 1) void gemm_b3d_kernel(B3Args p) {
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
           const int n = n0 + wn + 32 * j + 8 * (r >> 2) + 4 * lh + (r & 3);
-          if (n < g.N) b3_store(g, C, om + b3_off_n(g, n), acc[i][j][r]);
+          if (n < g.N)
+            b3_store(g, C,
+                     om + (run4 ? offn[j][r >> 2] + int64_t(r & 3) * g.cn0 : b3_off_n(g, n)),
+                     acc[i][j][r]);
         }
     }
   } else {
@@ -956,13 +983,16 @@ static void launch_variant(Engine &e, GemmArgs g, const B3Variant &v, int xcd, c
 }
 
 // Time every variant (both tile walks) of one product class on its real
-// planes into a scratch output; the fastest is recorded for the class.
+// planes into a scratch output (FP64, or dense bf16 planes for a
+// plane-output product); the fastest is recorded for the class.
 static TuneChoice tune_b3(Engine &e, const GemmArgs &g0, int passes, const Planes &A,
                           const Planes &B, int Kp, const TuneKey &key) {
   const B3Variant *table = passes == 1 ? kB1Variants : kB3Variants;
   const int count = b3_variant_count(passes);
   GemmArgs g = g0;
-  DBuf scratch(&e.pool, size_t(g.M) * g.N);
+  const int64_t mn = int64_t(g.M) * g.N;
+  const int nplanes = g.out_planes == 1 ? 2 : 1;
+  DBuf scratch(&e.pool, g.out_planes ? size_t((nplanes * mn * 2 + 7) / 8) : size_t(mn));
   g.C = scratch.get();
   g.cm0 = 1;
   g.cn0 = g.M;
@@ -972,7 +1002,7 @@ static TuneChoice tune_b3(Engine &e, const GemmArgs &g0, int passes, const Plane
   g.dm1 = g.dn1 = 1;
   g.alpha = 1.0;
   g.beta = 0.0;
-  g.out_planes = 0;
+  g.plane_lo = g.out_planes == 1 ? mn : 0;
   g.out_cube = 0;
   g.cm3 = g.cn3 = 0;
   const bool stats = e.timers->enabled;
@@ -1051,7 +1081,9 @@ static void run_b3(Engine &e, GemmArgs g, int passes, const B3FrozenA *fa = null
   const int count = b3_variant_count(passes);
   int vid = forced_variant(passes).load(), xcd = 1;
   if (vid < 0) {
-    const TuneKey key = tune_key(passes == 1 ? TuneFamily::B1 : TuneFamily::B3, g.M, g.N, g.K);
+    const TuneFamily fam = g.out_planes ? (passes == 1 ? TuneFamily::B1P : TuneFamily::B3P)
+                                        : (passes == 1 ? TuneFamily::B1 : TuneFamily::B3);
+    const TuneKey key = tune_key(fam, g.M, g.N, g.K);
     TuneChoice c = tune_lookup(key);
     if (c.variant < 0 && tune_enabled() && !stream_capturing(e.stream))
       c = tune_b3(e, g, passes, A, B, Kp, key);

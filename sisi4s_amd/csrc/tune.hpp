@@ -21,9 +21,9 @@
 
 namespace sisi {
 
-// family ids: 3 / 1 split-BF16x3 / bf16 products; 64 / 32 FP64 / FP32
-// segmented products
-enum class TuneFamily { B3 = 3, B1 = 1, Seg64 = 64, Seg32 = 32 };
+// family ids: 3 / 1 split-BF16x3 / bf16 products; 103 / 101 the same with
+// bf16 plane output; 64 / 32 FP64 / FP32 segmented products
+enum class TuneFamily { B3 = 3, B1 = 1, B3P = 103, B1P = 101, Seg64 = 64, Seg32 = 32 };
 
 struct TuneChoice {
   int variant = -1;
