@@ -257,13 +257,19 @@ def main():
         log("mixed bulk iteration %.3fs" % mixed["seconds_per_iteration"])
     refine = {}
     if args.refine_steps > 0:
-        # switch at once (threshold 1): iteration 1 bulk, then an FP64 anchor,
-        # then plain-bf16 refinement iterations until the next anchor
+        # switch early (threshold 1): bulk iterations, an FP64 anchor, then
+        # plain-bf16 refinement iterations until the next anchor
         orf = eng.default_opts(precision="bf16x3", switch_threshold=1.0,
                                refine_interval=args.refine_interval,
                                refine_precision=args.refine_precision, **kw)
         e.ccsd_begin(orf)
         rep_r = e.ccsd_iterate(2)
+        # the switch waits for |dT|/|T| <= 1: bulk iterations (untimed) until
+        # the first FP64 anchor
+        for k in range(30):
+            if rep_r["fp64_iterations"] > 0:
+                break
+            rep_r = e.ccsd_iterate(1)
         anchor_t, ref_t = [], []
         for k in range(args.refine_steps + 1):
             n64 = rep_r["fp64_iterations"]

@@ -267,8 +267,56 @@ void ccsd_residual(Engine &e, int it, bool initial_doubles, int slice_size, cons
       for (int m = 0; m < nslices; ++m)
         for (int n = m; n < nslices; ++n)
           bytes += 4.0 * double(((hi[n] - lo[n]) * (hi[m] - lo[m]) + 127) / 128 * 128) * double(Kp);
+      const int64_t Mps = (o * v + 255) / 256 * 256;
+      const double s_bytes = 4.0 * double(Mps) * double(Kp);
+      // device memory free or cached in the pool's free lists (a run's
+      // earlier phases leave buffers there; an allocation that does not fit
+      // gives them back first)
       size_t free_b = 0, total_b = 0;
-      keep_planes = hipMemGetInfo(&free_b, &total_b) == hipSuccess && bytes <= 0.5 * double(free_b);
+      const bool info = hipMemGetInfo(&free_b, &total_b) == hipSuccess;
+      const size_t cached = e.pool.cached_bytes();
+      if (getenv("SISI_VERBOSE"))
+        fprintf(stderr, "[sisi] anchor ladder planes: %.1f GB (+ %.1f GB S) against half of "
+                "%.1f GB free + %.1f GB cached\n", bytes / 1e9, s_bytes / 1e9,
+                double(free_b) / 1e9, double(cached) / 1e9);
+      free_b += cached;
+      keep_planes = info && bytes <= 0.5 * double(free_b);
+      e.anchor.ladder_s = B3FrozenA();
+      if (keep_planes && info && bytes + s_bytes <= 0.5 * double(free_b) &&
+          get_b3_backend() == 0 && 2.0 * double(o * v) * double(v * v) * double(G) >= kB3MinFlops) {
+        // S[c,k,b,d] = sum_G Gai[G,c,k] dGab[G,b,d]: product rows m = c + v k,
+        // columns n = b + v d, written as split-BF16x3 planes at row k + o b,
+        // column c + v d
+        B3FrozenA &S = e.anchor.ladder_s;
+        S.store = DBuf(&e.pool, size_t((2 * Mps * Kp * 2 + 7) / 8));
+        if (Kp != v * v) zero(e, S.store.get(), int64_t(S.store.size()));
+        GemmArgs gs;
+        gs.A = e.Gai.get();
+        gs.sam = G;
+        gs.sak = 1;
+        gs.B = dGabb.get();
+        gs.sbk = 1;
+        gs.sbn = G;
+        gs.M = int(v * o);
+        gs.N = int(v * v);
+        gs.K = int(G);
+        gs.dm0 = int(v);
+        gs.dm1 = int(o);
+        gs.cm0 = 1;
+        gs.cm1 = Kp;
+        gs.dn0 = int(v);
+        gs.dn1 = int(v);
+        gs.cn0 = o * Kp;
+        gs.cn1 = v;
+        gs.C = S.store.get();
+        gs.out_planes = 1;
+        gs.plane_lo = Mps * Kp;
+        gemm_b3(e, gs, 3);
+        S.M = int(o * v);
+        S.K = int(v * v);
+        S.Mp = int(Mps);
+        S.Kp = int(Kp);
+      }
     }
     // reduced-precision ladder with plane output: the slices are re-laid as
     // [G, c, x] so that the V~ product's rows m = c + v x run along c, the
@@ -584,12 +632,6 @@ void ccsd_residual_delta(Engine &e, int slice_size, const double *T1c, const dou
     // ladder: V~(T) X(T) - V~(T_A) X(T_A) = dV~ X(T) + V~(T_A) dX per slice pair,
     // dV~ = dG[n]^T G(T)[m] + G_A[n]^T dG[m] for the dressed Gamma_ab
     if (slice_size <= 0) slice_size = int(o);
-    DBuf ddGb(&e.pool, G * v * v), Gnb(&e.pool, G * v * v);
-    View ddG = V3(ddGb.get(), G, v, v);
-    ctr(e, -1.0, "Gbk,ak->Gab", Gai, d1, 0.0, ddG, prec);
-    axpby(e, G * v * v, 1.0, A("dGabP"), 0.0, Gnb.get());
-    axpby(e, G * v * v, 1.0, ddGb.get(), 1.0, Gnb.get());
-    View GA = V3(A("dGabP"), G, v, v), Gn = V3(Gnb.get(), G, v, v);
     const int nslices = int((v + slice_size - 1) / slice_size);
     std::vector<int64_t> lo(nslices), hi(nslices);
     for (int s = 0; s < nslices; ++s) {
@@ -603,107 +645,91 @@ void ccsd_residual_delta(Engine &e, int slice_size, const double *T1c, const dou
     // in bf16 or split-BF16x3 (no rebuild, no split), else rebuilt here
     const bool planes = (pl == SISI_BF16 || pl == SISI_BF16X3) && an.ladder_slice == slice_size &&
                         an.ladder.size() == size_t(npairs);
-    // fused pairs: dV~ as one product over 2 NG, [dG_n; G_A,n]^T [G_m(T); dG_m],
-    // written straight into bf16 planes (no FP64 dV~, no split), then
-    // R~ = dV~ X(T) + V~(T_A) dX from frozen planes; taken when every pair's
-    // anchor planes exist and every product is large enough for the bf16
-    // kernels (the precision of each product is then the one below)
+    // low-rank form, taken when the anchor kept S and every pair's planes and
+    // every product is large enough for the bf16 kernels (the precision of
+    // each product is then the one of the slice-pair form): with
+    // D = dGab(T) - dGab(T_A) = -sum_k d1[a,k] Gai[G,b,k],
+    //   dV~ = D^T G_A + G_A^T D + D^T D, and contracted with X(T):
+    //   (D^T G_A) X -> -sum_k d1[a,k] QA[k,b,i,j], QA = S X (one product),
+    //   (G_A^T D) X -> the same at (b,a,j,i) (X[c,d,i,j] = X[d,c,j,i]),
+    //   (D^T D) X   -> sum_kl d1[a,k] d1[b,l] QW[k,l,i,j], QW = V X (V = Vabij);
+    // every term carries a d1 factor, as the per-pair products do
     const int64_t smin = hi[nslices - 1] - lo[nslices - 1];
-    bool fused = planes && e.ladder_planes && !e.tape && get_b3_backend() == 0 &&
-                 2.0 * double(smin * smin) * double(v * v) * double(std::min(G, o * o)) >=
-                     kB3MinFlops;
-    for (int p = 0; fused && p < npairs; ++p) fused = an.ladder[size_t(p)].store.get() != nullptr;
-    if (fused) {
+    const B3FrozenA &Sf = an.ladder_s;
+    bool lowrank = planes && e.ladder_planes && !e.tape && get_b3_backend() == 0 &&
+                   2.0 * double(smin * smin) * double(v * v) * double(std::min(G, o * o)) >=
+                       kB3MinFlops &&
+                   Sf.store.get() && Sf.M == int(o * v) && Sf.K == int(v * v);
+    for (int p = 0; lowrank && p < npairs; ++p) lowrank = an.ladder[size_t(p)].store.get() != nullptr;
+    // dressed Gamma_ab change and value at T (the slice-pair forms)
+    DBuf ddGb, Gnb;
+    if (!lowrank) {
+      ddGb = DBuf(&e.pool, G * v * v);
+      Gnb = DBuf(&e.pool, G * v * v);
+      ctr(e, -1.0, "Gbk,ak->Gab", Gai, d1, 0.0, V3(ddGb.get(), G, v, v), prec);
+      axpby(e, G * v * v, 1.0, A("dGabP"), 0.0, Gnb.get());
+      axpby(e, G * v * v, 1.0, ddGb.get(), 1.0, Gnb.get());
+    }
+    View ddG = V3(ddGb.get(), G, v, v);
+    View GA = V3(A("dGabP"), G, v, v), Gn = V3(Gnb.get(), G, v, v);
+    if (lowrank) {
       const int passes = pl == SISI_BF16 ? 1 : 3;
-      const int64_t Kp = (v * v + 31) / 32 * 32;
-      const int64_t Mmax = (smax * smax + 255) / 256 * 256;
-      std::vector<DBuf> LA(nslices), LB(nslices);
-      for (int s = 0; s < nslices; ++s) {
-        const int64_t Ns = hi[s] - lo[s];
-        LA[s] = DBuf(&e.pool, 2 * G * Ns * v);
-        LB[s] = DBuf(&e.pool, 2 * G * Ns * v);
-        // element (g, x, c) at g + 2NG c + 2NG v x, g < NG first factor, g >= NG second
-        auto lay = [&](double *dst, View src) {
-          copy(e, dense_strided(dst, {G, Ns, v}, {1, 2 * G * v, 2 * G}),
-               subview(src, {0, lo[s], 0}, {G, hi[s], v}));
-        };
-        lay(LA[s].get(), ddG);
-        lay(LA[s].get() + G, GA);
-        lay(LB[s].get(), Gn);
-        lay(LB[s].get() + G, ddG);
-        e.frozen.push_back(LA[s].get());  // split into planes once, used by every pair
-        e.frozen.push_back(LB[s].get());
-      }
-      Gnb.reset();
-      DBuf Rxy(&e.pool, smax * smax * o * o);
-      B3FrozenA fd;
-      fd.store = DBuf(&e.pool, size_t((2 * Mmax * Kp * 2 + 7) / 8));
-      if (Kp != v * v) zero(e, fd.store.get(), int64_t(fd.store.size()));
+      DBuf QAb(&e.pool, o * v * o * o), QWb(&e.pool, o * o * o * o), Ycb(&e.pool, v * o * o * o),
+          tAb(&e.pool, n2);
+      GemmArgs gq;
+      gq.B = Xnb.get();
+      gq.sbk = 1;
+      gq.sbn = v * v;
+      gq.M = int(o * v);
+      gq.N = int(o * o);
+      gq.K = int(v * v);
+      gq.C = QAb.get();
+      gq.cm0 = 1;
+      gq.cn0 = o * v;
+      gq.dm0 = gq.M;
+      gq.dn0 = gq.N;
+      gq.alpha = 1.0;
+      gq.beta = 0.0;
+      gemm_b3_frozen_a(e, gq, Sf, passes);  // QA[k,b,i,j], rows k + o b
+      // L collects the half of the ladder change that dR2 receives twice,
+      // L + L^(baji): the first-order term, half the D^T D term, and per
+      // slice pair V~(T_A) dX (halved on diagonal pairs, whose block holds
+      // both orders of a and b), written by the products in place
+      View QA = V4(QAb.get(), o, v, o, o), QW = V4(QWb.get(), o, o, o, o),
+           Yc = V4(Ycb.get(), v, o, o, o), L = V4(tAb.get(), v, v, o, o);
+      ctr(e, -1.0, "ak,kbij->abij", d1, QA, 0.0, L, prec);
+      ctr(e, 0.5, "cdkl,cdij->klij", V, Xn, 0.0, QW, prec);
+      ctr(e, 1.0, "ak,klij->alij", d1, QW, 0.0, Yc, prec);
+      ctr(e, 1.0, "bl,alij->abij", d1, Yc, 1.0, L, prec);
       int pair = 0;
       for (int m = 0; m < nslices; ++m)
         for (int n = m; n < nslices; ++n, ++pair) {
+          // L[a0 + x, b0 + y, i, j] += V~_A(x + Nx y, c + v d) dX(c + v d, i + o j)
           const int64_t Nx = hi[n] - lo[n], Ny = hi[m] - lo[m];
-          const int64_t Mp = (Nx * Ny + 255) / 256 * 256;
-          View Rxyij = V4(Rxy.get(), Nx, Ny, o, o);
-          // dV~[x,y,c,d] as plane row x + Nx y, column c + v d
-          GemmArgs g1;
-          g1.A = LA[n].get();
-          g1.sam = 2 * G;
-          g1.sak = 1;
-          g1.B = LB[m].get();
-          g1.sbk = 1;
-          g1.sbn = 2 * G;
-          g1.M = int(v * Nx);
-          g1.N = int(v * Ny);
-          g1.K = int(2 * G);
-          g1.dm0 = int(v);
-          g1.dm1 = int(Nx);
-          g1.cm0 = 1;
-          g1.cm1 = Kp;
-          g1.dn0 = int(v);
-          g1.dn1 = int(Ny);
-          g1.cn0 = v;
-          g1.cn1 = Nx * Kp;
-          g1.C = fd.store.get();
-          g1.out_planes = passes == 1 ? 2 : 1;
-          g1.plane_lo = Mp * Kp;
-          gemm_b3(e, g1, passes);
-          fd.M = int(Nx * Ny);
-          fd.K = int(v * v);
-          fd.Mp = int(Mp);
-          fd.Kp = int(Kp);
-          // Rxyij(m = x + Nx y, n = i + o j) = dV~ X(T) + V~_A dX
+          const int64_t a0 = lo[n], b0 = lo[m];
           GemmArgs g2;
+          g2.B = dXb.get();
           g2.sbk = 1;
           g2.sbn = v * v;
           g2.M = int(Nx * Ny);
           g2.N = int(o * o);
           g2.K = int(v * v);
-          g2.C = Rxy.get();
-          g2.dm0 = g2.M;
-          g2.dn0 = g2.N;
+          g2.C = tAb.get() + a0 + v * b0;
+          g2.dm0 = int(Nx);
+          g2.dm1 = int(Ny);
           g2.cm0 = 1;
-          g2.cn0 = Nx * Ny;
-          g2.alpha = 1.0;
-          g2.B = Xnb.get();
-          g2.beta = 0.0;
-          gemm_b3_frozen_a(e, g2, fd, passes);
-          g2.B = dXb.get();
+          g2.cm1 = v;
+          g2.dn0 = g2.N;
+          g2.cn0 = v * v;
+          g2.alpha = n == m ? 0.5 : 1.0;
           g2.beta = 1.0;
           gemm_b3_frozen_a(e, g2, an.ladder[size_t(pair)], passes);
-          const int64_t a = lo[n], b = lo[m];
-          copy(e, subview(dR2, {a, b, 0, 0}, {a + Nx, b + Ny, o, o}), Rxyij, 1.0, 1.0);
-          if (a > b)
-            sum(e, 1.0, Rxyij, "xyij", 1.0, subview(dR2, {b, a, 0, 0}, {b + Ny, a + Nx, o, o}),
-                "yxji");
         }
-      for (int s2 = 0; s2 < nslices; ++s2) {  // slices return to the pool at block end
-        e.unfreeze(LA[s2].get());
-        e.unfreeze(LB[s2].get());
-      }
+      sum(e, 1.0, L, "abij", 1.0, dR2, "abij");
+      sum(e, 1.0, L, "baji", 1.0, dR2, "abij");
     }
-    std::vector<DBuf> sA(fused ? 0 : nslices), sN(fused ? 0 : nslices), sD(fused ? 0 : nslices);
-    for (int s = 0; s < nslices && !fused; ++s) {
+    std::vector<DBuf> sA(lowrank ? 0 : nslices), sN(lowrank ? 0 : nslices), sD(lowrank ? 0 : nslices);
+    for (int s = 0; s < nslices && !lowrank; ++s) {
       const int64_t n = G * (hi[s] - lo[s]) * v;
       sA[s] = DBuf(&e.pool, n);
       sN[s] = DBuf(&e.pool, n);
@@ -716,13 +742,13 @@ void ccsd_residual_delta(Engine &e, int slice_size, const double *T1c, const dou
       e.frozen.push_back(sD[s].get());
     }
     DBuf VAb, dVb, Rxy;
-    if (!fused) {
+    if (!lowrank) {
       VAb = DBuf(&e.pool, smax * smax * v * v);
       dVb = DBuf(&e.pool, smax * smax * v * v);
       Rxy = DBuf(&e.pool, smax * smax * o * o);
     }
     int pair = 0;
-    for (int m = 0; m < nslices && !fused; ++m)
+    for (int m = 0; m < nslices && !lowrank; ++m)
       for (int n = m; n < nslices; ++n, ++pair) {
         const int64_t Nx = hi[n] - lo[n], Ny = hi[m] - lo[m];
         View VA = V4(VAb.get(), Nx, Ny, v, v), dV = V4(dVb.get(), Nx, Ny, v, v);
@@ -763,7 +789,7 @@ void ccsd_residual_delta(Engine &e, int slice_size, const double *T1c, const dou
           sum(e, 1.0, Rxyij, "xyij", 1.0, subview(dR2, {b, a, 0, 0}, {b + Ny, a + Nx, o, o}),
               "yxji");
       }
-    for (int s2 = 0; s2 < nslices && !fused; ++s2) {  // slices return to the pool at block end
+    for (int s2 = 0; s2 < nslices && !lowrank; ++s2) {  // slices return to the pool at block end
       e.unfreeze(sA[s2].get());
       e.unfreeze(sN[s2].get());
       e.unfreeze(sD[s2].get());
@@ -821,6 +847,7 @@ void ccsd_residual_anchor(Engine &e, int slice_size, const double *T1, const dou
   e.prec_ladder = SISI_FP64;
   e.anchor.valid = false;
   e.anchor.ladder.clear();
+  e.anchor.ladder_s = B3FrozenA();
   e.recording = true;
   try {
     ccsd_residual(e, 1, true, slice_size, T1, T2, R1, R2);

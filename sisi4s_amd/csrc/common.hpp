@@ -125,6 +125,13 @@ class Pool {
       if (it != size_.end() && !live_.count(p)) free_[it->second].push_back(p);
     }
   }
+  // Bytes held in the free lists (device memory the pool can hand out
+  // again, or give back with trim()).
+  size_t cached_bytes() const {
+    size_t b = 0;
+    for (const auto &kv : free_) b += kv.first * kv.second.size();
+    return b;
+  }
   void trim() {
     std::lock_guard<std::recursive_mutex> lock(device_wide_mutex());
     (void)hipDeviceSynchronize();

@@ -68,6 +68,11 @@ struct Anchor {
   // pair too small for the bf16 kernels
   std::vector<B3FrozenA> ladder;
   int ladder_slice = 0;
+  // bf16 hi/lo planes of S[c,k,b,d] = sum_G Gamma_ck^G dressed Gamma_bd^G(T_A)
+  // as rows k + No b, columns c + Nv d: the change of the ladder vertex is
+  // linear in the T1 change through Gamma_ck, so with S the refinement forms
+  // dV~ X(T) from one product instead of one per slice pair
+  B3FrozenA ladder_s;
 };
 
 struct Tape;

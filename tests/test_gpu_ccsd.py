@@ -357,8 +357,8 @@ def test_refinement_residual_reduced_precision_error_scales_with_change():
 def test_refinement_residual_with_anchor_ladder_planes(prec):
     """Ladder slice pairs large enough for the bf16 kernels take V~(T_A) from
     the bf16 planes the FP64 anchor kept (No=16, Nv=160, slices of 40), with
-    dV~ written as bf16 planes by one product over 2 NG (the default) or
-    formed in FP64 and split (ladder planes off): the
+    dV~ X(T) in the low-rank form from the anchor's S planes (the default) or
+    from dV~ formed per slice pair in FP64 and split (ladder planes off): the
     refinement residual matches the FP64 residual to 1e-9 of its max norm in
     split-BF16x3 for a change of 2e-6 (a wrong plane layout would leave an
     error of the size of the change times |V~|), and in plain bf16 to 1e-6
@@ -371,7 +371,7 @@ def test_refinement_residual_with_anchor_ladder_planes(prec):
     P1, P2 = e.residual(1, TA1 + D1, TA2 + D2, slice_size=40)
     e.residual_anchor(TA1, TA2, slice_size=40)
     e.set_precision(prec)
-    R1, R2 = e.residual_refine(TA1 + D1, TA2 + D2, slice_size=40)   # fused dV~ planes
+    R1, R2 = e.residual_refine(TA1 + D1, TA2 + D2, slice_size=40)   # low-rank form
     e.set_ladder_planes(False)
     S1, S2 = e.residual_refine(TA1 + D1, TA2 + D2, slice_size=40)   # FP64 dV~, split
     e.set_ladder_planes(True)
