@@ -1221,6 +1221,12 @@ void ccsd_iterate(Engine &e, int n) {
       const double tt = fock_dot(e, e.T1.get(), e.T2.get(), e.T1.get(), e.T2.get());
       e.last_delta = std::sqrt(std::fabs(dd / tt));
     }
+    // mixed runs keep T2 exactly symmetric (T_abij = T_baji): the reduced-
+    // precision and refinement residuals treat an antisymmetric part of T2
+    // differently from the FP64 one, and DIIS lets that part grow when it is
+    // not projected out
+    if (o.precision != SISI_FP64 && !e.cplx && e.formulation == SISI_FORMULATION_VERTEX)
+      symmetrize_doubles(e, e.T2.get(), e.Nv, e.No);
     double E, dir, exc, Ei;
     ccsd_energy(e, e.T1.get(), e.T2.get(), E, dir, exc, &Ei);
     SISI_REQUIRE(std::isfinite(E) && std::isfinite(e.last_delta), SISI_ERR_NUMERIC,

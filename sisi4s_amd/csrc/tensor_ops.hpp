@@ -26,6 +26,10 @@ void axpby(Engine &e, int64_t n, double a, const double *x, double b, double *y)
 void add_t1t1(Engine &e, double *X, const double *T2, const double *T1,
               double f, int Nv, int No, const double *T1b = nullptr);
 
+// T2[a,b,i,j] = T2[b,a,j,i] = (T2[a,b,i,j] + T2[b,a,j,i]) / 2 in place (real
+// doubles, exactly symmetric afterwards).
+void symmetrize_doubles(Engine &e, double *T2, int Nv, int No);
+
 // C[R,S,ij] *= V[R0 + R, S0 + S]: C planar complex (Rx x Ry x nij per plane),
 // V planar complex with leading dimension ldV and plane size nV; real: only
 // the real planes (both operands real).

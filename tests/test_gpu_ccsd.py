@@ -389,6 +389,56 @@ def test_refinement_residual_with_anchor_ladder_planes(prec):
         assert float(np.abs(E1 - P1).max()) <= tol * scale
 
 
+@pytest.mark.parametrize("scale", [2e-6, 1e-3])
+def test_refinement_low_rank_with_ragged_slices(scale):
+    """The low-rank refinement ladder (anchor S planes) with a ragged last
+    slice (No=20, Nv=224, NG=900, slices of 64: 64+64+64+32) against the
+    slice-pair form and the FP64 residual, for a small change and for a
+    change large enough that the second-order D^T D term matters: both
+    forms within the same bf16 bound of the FP64 residual."""
+    No, Nv, NG = 20, 224, 900
+    e, g, ei, ea = make_engine(31, No, Nv, NG)
+    TA1, TA2 = random_amplitudes(Nv, No, seed=6, scale=0.02)
+    D1, D2 = random_amplitudes(Nv, No, seed=7, scale=scale)
+    announce("low-rank refinement, ragged slices, change %g" % scale)
+    P1, P2 = e.residual(1, TA1 + D1, TA2 + D2, slice_size=64)
+    e.residual_anchor(TA1, TA2, slice_size=64)
+    e.set_precision("bf16")
+    R1, R2 = e.residual_refine(TA1 + D1, TA2 + D2, slice_size=64)
+    e.set_ladder_planes(False)
+    S1, S2 = e.residual_refine(TA1 + D1, TA2 + D2, slice_size=64)
+    e.set_ladder_planes(True)
+    e.set_precision("fp64")
+    A1, A2 = e.residual(1, TA1, TA2, slice_size=64)
+    sc = float(np.abs(P2).max())
+    dr = float(np.abs(P2 - A2).max())   # size of the residual change
+    err, err_s = float(np.abs(R2 - P2).max()), float(np.abs(S2 - P2).max())
+    print("change %g: |dR| %.2e, low-rank err %.2e, slice-pair err %.2e (of |R| %.2e)"
+          % (scale, dr, err, err_s, sc), flush=True)
+    assert err <= 1e-2 * dr and err <= 2.0 * err_s + 1e-12 * sc
+    assert float(np.abs(R1 - P1).max()) <= 1e-2 * dr
+
+
+def test_refined_mixed_solve_low_rank_matches_fp64():
+    """A mixed run whose refinement residuals take the low-rank ladder (No=16,
+    Nv=160, NG=640, slices of 40, anchors every 8) converges to the FP64
+    energy within 1 uEh."""
+    from sisi4s_amd import engine
+    No, Nv, NG = 16, 160, 640
+    e, g, ei, ea = make_engine(11, No, Nv, NG)
+    kw = dict(mixer="DiisMixer", max_residua=6, max_iterations=200, slice_size=40,
+              level_shift=2.0, amplitudes_convergence=1e-8, energy_convergence=1e-10)
+    announce("low-rank refined mixed solve No=16 Nv=160 NG=640")
+    r64 = e.ccsd_solve(engine.default_opts(**kw))
+    rm = e.ccsd_solve(engine.default_opts(precision="bf16x3", switch_threshold=1e-5,
+                                          refine_interval=8, refine_precision="bf16", **kw))
+    print("low-rank mixed: fp64 %d it, mixed %d it (%d fp64), |dE| %.2e"
+          % (r64["iterations"], rm["iterations"], rm["fp64_iterations"],
+             abs(rm["energy"] - r64["energy"])), flush=True)
+    assert r64["converged"] and rm["converged"]
+    assert abs(rm["energy"] - r64["energy"]) <= 1e-6
+
+
 def test_refined_mixed_solve_matches_fp64():
     from sisi4s_amd import engine
     No, Nv, NG = 5, 20, 80
