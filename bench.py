@@ -131,6 +131,9 @@ def main():
     ap.add_argument("--refine-interval", type=int, default=16,
                     help="mixed runs: FP64 anchor every k iterations after the switch (0: plain "
                          "FP64 iterations)")
+    ap.add_argument("--bulk-refine-interval", type=int, default=0,
+                    help="mixed runs: split-BF16x3 anchor every k bulk iterations and "
+                         "refinement residuals between (0: plain bulk iterations)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -655,6 +658,8 @@ def converge(args):
                              level_shift=args.level_shift,
                              refine_interval=args.refine_interval if prec != "fp64" else 0,
                              refine_precision=args.refine_precision,
+                             bulk_refine_interval=(args.bulk_refine_interval
+                                                   if prec != "fp64" else 0),
                              coarse_threshold=args.coarse_threshold if prec != "fp64" else 0.0)
         t0 = time.perf_counter()
         e.ccsd_begin(o)
@@ -698,6 +703,12 @@ def converge(args):
                           triples_seconds=t_setup + (t_t if blocks is None
                                                      else t_t * nblk / len(blocks)),
                           triples_sampled=blocks is not None, triples_energy_sample=et)
+        if prec != "fp64":
+            runs[name]["schedule"] = dict(switch_threshold=o.switch_threshold,
+                                          refine_interval=o.refine_interval,
+                                          refine_precision=args.refine_precision,
+                                          bulk_refine_interval=o.bulk_refine_interval,
+                                          coarse_threshold=o.coarse_threshold)
         log("%s done: %s" % (name, runs[name]))
     if len(runs) < 2:
         print(json.dumps({"mode": "converge", "level_shift": args.level_shift, "runs": runs}))

@@ -119,6 +119,12 @@ typedef struct {
                                     once per precision in a hipGraph and
                                     replayed (small, launch-bound systems);
                                     default 0                             */
+  int bulk_refine_interval;      /* before the switch: 0 = plain bulk-
+                                    precision iterations (default); k > 0 =
+                                    an anchor residual in the bulk precision
+                                    every k iterations and refinement
+                                    residuals in refine_precision between
+                                    (vertex formulation, real amplitudes) */
 } sisi_ccsd_opts;
 
 typedef struct {

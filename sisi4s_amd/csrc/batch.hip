@@ -338,6 +338,8 @@ void ccsd_solve_lockstep(int nsys, const sisi_dims *dims, const double *const *g
   SISI_REQUIRE(o.refine_interval == 0, SISI_ERR_INVALID,
                "the lockstep solver runs plain FP64 iterations after the switch "
                "(refine_interval 0)");
+  SISI_REQUIRE(o.bulk_refine_interval == 0, SISI_ERR_INVALID,
+               "the lockstep solver runs plain bulk iterations (bulk_refine_interval 0)");
   SISI_REQUIRE(is_precision(o.precision), SISI_ERR_INVALID, "unknown precision");
   SISI_REQUIRE(o.coarse_threshold <= 0.0 || is_precision(o.coarse_precision), SISI_ERR_INVALID,
                "coarse_precision must be a precision");

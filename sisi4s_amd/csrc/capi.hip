@@ -87,6 +87,7 @@ void sisi_ccsd_opts_default(sisi_ccsd_opts *o) {
   o->coarse_threshold = 0.0;
   o->coarse_precision = SISI_BF16;
   o->graphs = 0;
+  o->bulk_refine_interval = 0;
 }
 
 int sisi_engine_create(const sisi_dims *dims, sisi_engine **out) {

@@ -838,13 +838,14 @@ void ccsd_residual_delta(Engine &e, int slice_size, const double *T1c, const dou
 
 // FP64 residual at T that also becomes the new refinement anchor.
 void ccsd_residual_anchor(Engine &e, int slice_size, const double *T1, const double *T2,
-                          double *R1, double *R2) {
+                          double *R1, double *R2, int precision) {
   SISI_REQUIRE(!e.cplx, SISI_ERR_INVALID, "refinement residuals are not built for complex amplitudes");
   SISI_REQUIRE(e.NR == 0, SISI_ERR_INVALID, "refinement residuals are not built with CoulombFactors");
+  SISI_REQUIRE(is_precision(precision), SISI_ERR_INVALID, "unknown anchor precision");
   const int64_t n1 = int64_t(e.n1()), n2 = int64_t(e.n2());
   const int p = e.precision, pl = e.prec_ladder;
-  e.precision = SISI_FP64;
-  e.prec_ladder = SISI_FP64;
+  e.precision = precision;
+  e.prec_ladder = precision;
   e.anchor.valid = false;
   e.anchor.ladder.clear();
   e.anchor.ladder_s = B3FrozenA();
@@ -871,6 +872,7 @@ void ccsd_residual_anchor(Engine &e, int slice_size, const double *T1, const dou
   axpby(e, n2, 1.0, R2, 0.0, an.R2.get());
   an.valid = true;
   an.age = 0;
+  an.precision = precision;
   an.ppl = e.ppl;
   an.distinguishable = e.distinguishable;
 }
@@ -1077,6 +1079,12 @@ void ccsd_begin(Engine &e, const sisi_ccsd_opts &opts, const double *T1h, const 
   SISI_REQUIRE(opts.mixer != SISI_MIXER_DIIS || opts.max_residua >= 1, SISI_ERR_INVALID,
                "maxResidua must be >= 1");
   SISI_REQUIRE(opts.refine_interval >= 0, SISI_ERR_INVALID, "refine_interval must be >= 0");
+  SISI_REQUIRE(opts.bulk_refine_interval >= 0, SISI_ERR_INVALID,
+               "bulk_refine_interval must be >= 0");
+  SISI_REQUIRE(opts.bulk_refine_interval == 0 || opts.precision == SISI_FP64 ||
+                   (!e.cplx && e.NR == 0 && e.formulation == SISI_FORMULATION_VERTEX),
+               SISI_ERR_INVALID,
+               "bulk refinement residuals are built for the real vertex formulation only");
   SISI_REQUIRE(!e.cplx || opts.precision == SISI_FP64 || opts.refine_interval == 0,
                SISI_ERR_INVALID, "refinement residuals are not built for complex amplitudes");
   SISI_REQUIRE(e.NR == 0 || opts.precision == SISI_FP64 || opts.refine_interval == 0,
@@ -1134,12 +1142,29 @@ void ccsd_iterate(Engine &e, int n) {
     // afterwards (and FP64 throughout for the plain path)
     const bool bulk = o.precision != SISI_FP64 && e.cur_precision != SISI_FP64;
     const bool refine = !bulk && o.precision != SISI_FP64 && o.refine_interval > 0;
+    // bulk refinement: anchors in the bulk precision (not in the coarse
+    // stage) every bulk_refine_interval iterations, refinement residuals
+    // between; the anchor error (the bulk precision's) is what the bulk
+    // phase tolerates anyway, the refinement error scales with the change
+    const bool bulk_refine = bulk && !e.coarse_active && o.bulk_refine_interval > 0;
     int prec = bulk ? (e.coarse_active ? o.coarse_precision : o.precision) : SISI_FP64;
     DBuf R1(&e.pool, e.n1()), R2(&e.pool, e.n2());
-    if (refine) {
+    if (bulk_refine) {
+      if (!e.anchor.valid || e.anchor.precision != prec ||
+          e.anchor.age >= o.bulk_refine_interval) {
+        set_precision(e, prec);
+        ccsd_residual_anchor(e, o.slice_size, e.T1.get(), e.T2.get(), R1.get(), R2.get(), prec);
+      } else {
+        set_precision(e, o.refine_precision);
+        ccsd_residual_delta(e, o.slice_size, e.T1.get(), e.T2.get(), R1.get(), R2.get());
+        e.anchor.age++;
+        prec = o.refine_precision;
+      }
+    } else if (refine) {
       // refinement phase: FP64 anchor every refine_interval iterations,
       // reduced-precision refinement residuals in between
-      if (!e.anchor.valid || e.anchor.age >= o.refine_interval) {
+      if (!e.anchor.valid || e.anchor.precision != SISI_FP64 ||
+          e.anchor.age >= o.refine_interval) {
         set_precision(e, SISI_FP64);
         ccsd_residual_anchor(e, o.slice_size, e.T1.get(), e.T2.get(), R1.get(), R2.get());
         prec = SISI_FP64;

@@ -14,10 +14,11 @@ void integrals_from_vertex(Engine &e);
 void ccsd_residual(Engine &e, int iteration, bool initial_doubles, int slice_size,
                    const double *T1, const double *T2, double *R1, double *R2);
 
-// FP64 residual that records itself (and its intermediates) as the
-// refinement anchor; and the residual at T by refinement around it.
+// Residual (FP64, or a bulk precision for the bulk phase's anchors) that
+// records itself (and its intermediates) as the refinement anchor; and the
+// residual at T by refinement around it.
 void ccsd_residual_anchor(Engine &e, int slice_size, const double *T1, const double *T2,
-                          double *R1, double *R2);
+                          double *R1, double *R2, int precision = SISI_FP64);
 void ccsd_residual_delta(Engine &e, int slice_size, const double *T1, const double *T2, double *R1,
                          double *R2);
 

@@ -60,6 +60,7 @@ class Timed {
 struct Anchor {
   bool valid = false;
   int age = 0;  // refinement iterations since the anchor was set
+  int precision = SISI_FP64;  // of the anchor residual (FP64, or a bulk precision)
   bool ppl = true, distinguishable = false;  // residual variant it was recorded with
   DBuf T1, T2, R1, R2;
   std::map<std::string, DBuf> v;

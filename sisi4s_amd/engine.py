@@ -47,7 +47,8 @@ class CcsdOpts(ctypes.Structure):
                 ("switch_threshold", ctypes.c_double), ("refine_interval", ctypes.c_int),
                 ("refine_precision", ctypes.c_int), ("ppl", ctypes.c_int),
                 ("distinguishable", ctypes.c_int), ("coarse_threshold", ctypes.c_double),
-                ("coarse_precision", ctypes.c_int), ("graphs", ctypes.c_int)]
+                ("coarse_precision", ctypes.c_int), ("graphs", ctypes.c_int),
+                ("bulk_refine_interval", ctypes.c_int)]
 
 
 class CcsdReport(ctypes.Structure):

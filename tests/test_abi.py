@@ -83,3 +83,4 @@ def test_opts_defaults_follow_reference(lib):
     assert o.max_iterations == 16 and o.amplitudes_convergence == 1e-5
     assert o.energy_convergence == 1e-6 and o.level_shift == 0.0
     assert o.mixer == engine.MIXER_LINEAR and o.max_residua == 4 and o.mixing_ratio == 1.0
+    assert o.refine_interval == 0 and o.bulk_refine_interval == 0

@@ -439,6 +439,28 @@ def test_refined_mixed_solve_low_rank_matches_fp64():
     assert abs(rm["energy"] - r64["energy"]) <= 1e-6
 
 
+def test_bulk_refined_mixed_solve_matches_fp64():
+    """Bulk phase on split-BF16x3 anchors every 6 iterations with plain-bf16
+    refinement residuals between, then FP64 anchors every 8 (No=16, Nv=160,
+    NG=640, slices of 40: the low-rank ladder in both phases): the run
+    converges to the FP64 energy within 1 uEh."""
+    from sisi4s_amd import engine
+    No, Nv, NG = 16, 160, 640
+    e, g, ei, ea = make_engine(11, No, Nv, NG)
+    kw = dict(mixer="DiisMixer", max_residua=6, max_iterations=200, slice_size=40,
+              level_shift=2.0, amplitudes_convergence=1e-8, energy_convergence=1e-10)
+    announce("bulk-refined mixed solve No=16 Nv=160 NG=640")
+    r64 = e.ccsd_solve(engine.default_opts(**kw))
+    rb = e.ccsd_solve(engine.default_opts(precision="bf16x3", switch_threshold=1e-5,
+                                          refine_interval=8, refine_precision="bf16",
+                                          bulk_refine_interval=6, **kw))
+    print("bulk-refined mixed: fp64 %d it, mixed %d it (%d fp64), |dE| %.2e"
+          % (r64["iterations"], rb["iterations"], rb["fp64_iterations"],
+             abs(rb["energy"] - r64["energy"])), flush=True)
+    assert r64["converged"] and rb["converged"]
+    assert abs(rb["energy"] - r64["energy"]) <= 1e-6
+
+
 def test_refined_mixed_solve_matches_fp64():
     from sisi4s_amd import engine
     No, Nv, NG = 5, 20, 80

@@ -20,6 +20,7 @@ def test_lockstep_rejects_non_finite_input_and_names_the_system():
 
 @pytest.mark.parametrize("kw,msg", [
     (dict(precision="bf16x3", refine_interval=4), "refine_interval"),
+    (dict(precision="bf16x3", bulk_refine_interval=4), "bulk_refine_interval"),
     (dict(mixer="DiisMixer", max_residua=31), "maxResidua"),
     (dict(mixer="DiisMixer", max_residua=0), "maxResidua"),
 ])
