@@ -128,10 +128,10 @@ def main():
                     help="(T) precision of the mixed run")
     ap.add_argument("--coarse-threshold", type=float, default=0.0,
                     help="mixed runs: plain-bf16 coarse stage until |dT|/|T| reaches this value")
-    ap.add_argument("--refine-interval", type=int, default=16,
+    ap.add_argument("--refine-interval", type=int, default=32,
                     help="mixed runs: FP64 anchor every k iterations after the switch (0: plain "
                          "FP64 iterations)")
-    ap.add_argument("--bulk-refine-interval", type=int, default=0,
+    ap.add_argument("--bulk-refine-interval", type=int, default=16,
                     help="mixed runs: split-BF16x3 anchor every k bulk iterations and "
                          "refinement residuals between (0: plain bulk iterations)")
     args = ap.parse_args()
@@ -264,15 +264,36 @@ def main():
         # plain-bf16 refinement iterations until the next anchor
         orf = eng.default_opts(precision="bf16x3", switch_threshold=1.0,
                                refine_interval=args.refine_interval,
-                               refine_precision=args.refine_precision, **kw)
+                               refine_precision=args.refine_precision,
+                               bulk_refine_interval=args.bulk_refine_interval, **kw)
         e.ccsd_begin(orf)
         rep_r = e.ccsd_iterate(2)
-        # the switch waits for |dT|/|T| <= 1: bulk iterations (untimed) until
-        # the first FP64 anchor
+        # the switch waits for |dT|/|T| <= 1: bulk iterations until the first
+        # FP64 anchor, timed as bulk-refinement iterations (split-BF16x3
+        # anchors every bulk_refine_interval, plain-bf16 refinements between;
+        # the anchors are the slow ones)
+        bulk_t = []
         for k in range(30):
             if rep_r["fp64_iterations"] > 0:
                 break
+            t0 = time.perf_counter()
             rep_r = e.ccsd_iterate(1)
+            bulk_t.append(time.perf_counter() - t0)
+        if bulk_t and rep_r["fp64_iterations"] > 0:
+            bulk_t = bulk_t[:-1]   # the last one was the FP64 anchor
+        if args.bulk_refine_interval > 0 and len(bulk_t) >= 3:
+            med = float(np.median(bulk_t))
+            slow = [t for t in bulk_t if t > 1.5 * med]
+            fast = [t for t in bulk_t if t <= 1.5 * med]
+            refine_bulk = dict(bulk_refinement_seconds_per_iteration=float(np.mean(fast)),
+                               bulk_anchor_seconds=float(np.mean(slow)) if slow else None,
+                               iterations_timed=len(bulk_t),
+                               bulk_refine_interval=args.bulk_refine_interval)
+            log("bulk refinement iteration %.3f s, bulk anchor %s s"
+                % (refine_bulk["bulk_refinement_seconds_per_iteration"],
+                   refine_bulk["bulk_anchor_seconds"]))
+        else:
+            refine_bulk = None
         anchor_t, ref_t = [], []
         for k in range(args.refine_steps + 1):
             n64 = rep_r["fp64_iterations"]
@@ -284,7 +305,7 @@ def main():
                       anchor_seconds=float(np.mean(anchor_t)) if anchor_t else None,
                       refinement_iterations_timed=len(ref_t), anchors_timed=len(anchor_t),
                       refine_precision=args.refine_precision,
-                      refine_interval=args.refine_interval)
+                      refine_interval=args.refine_interval, bulk=refine_bulk)
         log("refinement iteration %s s, anchor %s s" % (refine["refinement_seconds_per_iteration"],
                                                         refine["anchor_seconds"]))
 

@@ -261,6 +261,9 @@ void ccsd_residual(Engine &e, int it, bool initial_doubles, int slice_size, cons
     bool keep_planes = false;
     if (e.recording) {
       e.anchor.ladder.clear();
+      e.anchor.ladder_stack = DBuf();
+      e.anchor.ladder_rows = 0;
+      e.anchor.ladder_off.clear();
       e.anchor.ladder_slice = slice_size;
       const int64_t Kp = ((v * v + 31) / 32) * 32;
       double bytes = 0.0;
@@ -399,8 +402,26 @@ void ccsd_residual(Engine &e, int it, bool initial_doubles, int slice_size, cons
         }
       for (auto &b : slt) e.unfreeze(b.get());
     }
-    for (int m = 0; m < nslices && !planes; ++m)
-      for (int n = m; n < nslices; ++n) {  // cxx:418-451
+    // kept planes: stacked in one store when every pair is large enough for
+    // the bf16 kernels (rows of pair p from ladder_off[p], each pair padded
+    // to 256 rows), else one store per pair
+    const int64_t smin_k = hi[nslices - 1] - lo[nslices - 1];
+    const int64_t Kp_k = (v * v + 31) / 32 * 32;
+    uint16_t *stack_hi = nullptr;
+    if (keep_planes && 2.0 * double(smin_k * smin_k) * double(o * o) * double(v * v) >= kB3MinFlops) {
+      Anchor &an = e.anchor;
+      int64_t rows = 0;
+      for (int m = 0; m < nslices; ++m)
+        for (int n = m; n < nslices; ++n) {
+          an.ladder_off.push_back(rows);
+          rows += ((hi[n] - lo[n]) * (hi[m] - lo[m]) + 255) / 256 * 256;
+        }
+      an.ladder_rows = rows;
+      an.ladder_stack = DBuf(&e.pool, size_t((2 * rows * Kp_k * 2 + 7) / 8));
+      stack_hi = reinterpret_cast<uint16_t *>(an.ladder_stack.get());
+    }
+    for (int m = 0, pair = 0; m < nslices && !planes; ++m)
+      for (int n = m; n < nslices; ++n, ++pair) {  // cxx:418-451
         const int64_t Nx = hi[n] - lo[n], Ny = hi[m] - lo[m];
         View Vxycd = V4(Vxy.get(), Nx, Ny, v, v);
         View Rxyij = V4(Rxy.get(), Nx, Ny, o, o);
@@ -408,8 +429,21 @@ void ccsd_residual(Engine &e, int it, bool initial_doubles, int slice_size, cons
             Vxycd, e.prec_ladder);  // cxx:431-434
         if (keep_planes) {
           e.anchor.ladder.emplace_back();
-          if (2.0 * double(Nx * Ny) * double(o * o) * double(v * v) >= kB3MinFlops)
-            b3_freeze_a(e, Vxy.get(), 1, Nx * Ny, int(Nx * Ny), int(v * v), e.anchor.ladder.back());
+          B3FrozenA &fz = e.anchor.ladder.back();
+          if (stack_hi) {
+            const int64_t off = e.anchor.ladder_off[size_t(pair)] * Kp_k;
+            const int Mp = int((Nx * Ny + 255) / 256 * 256);
+            uint16_t *h = stack_hi + off, *l = stack_hi + e.anchor.ladder_rows * Kp_k + off;
+            b3_split_into(e, Vxy.get(), 1, Nx * Ny, int(Nx * Ny), int(v * v), Mp, int(Kp_k), h, l);
+            fz.M = int(Nx * Ny);
+            fz.K = int(v * v);
+            fz.Mp = Mp;
+            fz.Kp = int(Kp_k);
+            fz.ext = h;
+            fz.lo_off = e.anchor.ladder_rows * Kp_k;
+          } else if (2.0 * double(Nx * Ny) * double(o * o) * double(v * v) >= kB3MinFlops) {
+            b3_freeze_a(e, Vxy.get(), 1, Nx * Ny, int(Nx * Ny), int(v * v), fz);
+          }
         }
         ctr(e, 1.0, "xycd,cdij->xyij", Vxycd, X, 0.0, Rxyij, e.prec_ladder);  // cxx:443
         // sliceIntoResiduum (ClusterSinglesDoublesAlgorithm.cxx:887-915)
@@ -660,7 +694,7 @@ void ccsd_residual_delta(Engine &e, int slice_size, const double *T1c, const dou
                    2.0 * double(smin * smin) * double(v * v) * double(std::min(G, o * o)) >=
                        kB3MinFlops &&
                    Sf.store.get() && Sf.M == int(o * v) && Sf.K == int(v * v);
-    for (int p = 0; lowrank && p < npairs; ++p) lowrank = an.ladder[size_t(p)].store.get() != nullptr;
+    for (int p = 0; lowrank && p < npairs; ++p) lowrank = an.ladder[size_t(p)].valid();
     // dressed Gamma_ab change and value at T (the slice-pair forms)
     DBuf ddGb, Gnb;
     if (!lowrank) {
@@ -701,8 +735,46 @@ void ccsd_residual_delta(Engine &e, int slice_size, const double *T1c, const dou
       ctr(e, 0.5, "cdkl,cdij->klij", V, Xn, 0.0, QW, prec);
       ctr(e, 1.0, "ak,klij->alij", d1, QW, 0.0, Yc, prec);
       ctr(e, 1.0, "bl,alij->abij", d1, Yc, 1.0, L, prec);
+      if (an.ladder_rows > 0) {
+        // every pair in one product over the stacked planes, then each pair's
+        // rows (x + Nx y from its offset) added into its block of L
+        const int64_t rows = an.ladder_rows;
+        DBuf Cst(&e.pool, rows * o * o);
+        B3FrozenA st;
+        st.M = int(rows);
+        st.K = int(v * v);
+        st.Mp = int(rows);
+        st.Kp = an.ladder[0].Kp;
+        st.ext = an.ladder[0].ext;
+        st.lo_off = an.ladder[0].lo_off;
+        GemmArgs gs;
+        gs.B = dXb.get();
+        gs.sbk = 1;
+        gs.sbn = v * v;
+        gs.M = int(rows);
+        gs.N = int(o * o);
+        gs.K = int(v * v);
+        gs.C = Cst.get();
+        gs.cm0 = 1;
+        gs.cn0 = rows;
+        gs.dm0 = gs.M;
+        gs.dn0 = gs.N;
+        gs.alpha = 1.0;
+        gs.beta = 0.0;
+        gemm_b3_frozen_a(e, gs, st, passes);
+        int pair = 0;
+        for (int m = 0; m < nslices; ++m)
+          for (int n = m; n < nslices; ++n, ++pair) {
+            const int64_t Nx = hi[n] - lo[n], Ny = hi[m] - lo[m];
+            const int64_t a0 = lo[n], b0 = lo[m];
+            View Cp = dense_strided(Cst.get() + an.ladder_off[size_t(pair)], {Nx, Ny, o, o},
+                                    {1, Nx, rows, rows * o});
+            copy(e, subview(L, {a0, b0, 0, 0}, {a0 + Nx, b0 + Ny, o, o}), Cp,
+                 n == m ? 0.5 : 1.0, 1.0);
+          }
+      }
       int pair = 0;
-      for (int m = 0; m < nslices; ++m)
+      for (int m = 0; m < nslices && an.ladder_rows == 0; ++m)
         for (int n = m; n < nslices; ++n, ++pair) {
           // L[a0 + x, b0 + y, i, j] += V~_A(x + Nx y, c + v d) dX(c + v d, i + o j)
           const int64_t Nx = hi[n] - lo[n], Ny = hi[m] - lo[m];
@@ -754,7 +826,7 @@ void ccsd_residual_delta(Engine &e, int slice_size, const double *T1c, const dou
         View VA = V4(VAb.get(), Nx, Ny, v, v), dV = V4(dVb.get(), Nx, Ny, v, v);
         View Rxyij = V4(Rxy.get(), Nx, Ny, o, o);
         const B3FrozenA *fa =
-            planes && an.ladder[size_t(pair)].store.get() ? &an.ladder[size_t(pair)] : nullptr;
+            planes && an.ladder[size_t(pair)].valid() ? &an.ladder[size_t(pair)] : nullptr;
         if (!fa)
           ctr(e, 1.0, "Gxc,Gyd->xycd", V3(sA[n].get(), G, Nx, v), V3(sA[m].get(), G, Ny, v), 0.0,
               VA, pl);

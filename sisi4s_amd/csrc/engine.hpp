@@ -69,6 +69,12 @@ struct Anchor {
   // pair too small for the bf16 kernels
   std::vector<B3FrozenA> ladder;
   int ladder_slice = 0;
+  // when every pair is large enough, their planes are stacked in one store
+  // (pair p at rows ladder_off[p], ladder_rows rows in all, lo plane after
+  // the hi plane): the refinement contracts all pairs with dX in one product
+  DBuf ladder_stack;
+  int64_t ladder_rows = 0;
+  std::vector<int64_t> ladder_off;
   // bf16 hi/lo planes of S[c,k,b,d] = sum_G Gamma_ck^G dressed Gamma_bd^G(T_A)
   // as rows k + No b, columns c + Nv d: the change of the ladder vertex is
   // linear in the T1 change through Gamma_ck, so with S the refinement forms

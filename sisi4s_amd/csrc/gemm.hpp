@@ -148,12 +148,25 @@ int round_filling_split(int64_t groups, int64_t s, int64_t smax, int cus);
 // 256, so every kernel variant reads them) and kept
 // across residual evaluations (the refinement anchor's ladder slices);
 // `store` owns both planes.
+// The planes may instead live in memory owned elsewhere (ext: the hi plane,
+// lo_off: bf16 elements from hi to lo; 0 means Mp * Kp), e.g. one pair's
+// rows inside a stack of several operands' planes.
 struct B3FrozenA {
   DBuf store;
   int M = 0, K = 0, Mp = 0, Kp = 0;
+  const void *ext = nullptr;
+  int64_t lo_off = 0;
+  const void *hi() const { return ext ? ext : static_cast<const void *>(store.get()); }
+  bool valid() const { return hi() != nullptr; }
+  int64_t lo_offset() const { return lo_off ? lo_off : int64_t(Mp) * Kp; }
 };
 void b3_freeze_a(Engine &e, const double *A, int64_t sam, int64_t sak, int M, int K,
                  B3FrozenA &out);
+// Split A (M x K, element (m,k) at A[m*sam + k*sak]) into caller-owned planes:
+// rows [0, Mp) of hi and lo with row stride Kp (rows past M and columns past K
+// are written as zeros).
+void b3_split_into(Engine &e, const double *A, int64_t sam, int64_t sak, int M, int K, int Mp,
+                   int Kp, void *hi, void *lo);
 // C = alpha * A B + beta * C with A taken from frozen planes (g.A unused; g.M
 // and g.K must match them); passes = 1 (bf16) or 3 (split-BF16x3).
 void gemm_b3_frozen_a(Engine &e, GemmArgs g, const B3FrozenA &A, int passes);

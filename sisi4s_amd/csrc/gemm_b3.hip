@@ -1053,7 +1053,7 @@ static void run_b3(Engine &e, GemmArgs g, int passes, const B3FrozenA *fa = null
     // frozen hi plane: rows of Kp bf16, the first M rows used
     g.split_out = !(g.dm0 >= g.M && g.dn0 >= g.N);
     Planes la, lb;
-    la.hi = reinterpret_cast<const u16 *>(fa->store.get());
+    la.hi = reinterpret_cast<const u16 *>(fa->hi());
     la.ld = Kp;
     SISI_REQUIRE(fa->M == g.M && fa->Kp == Kp, SISI_ERR_INVALID, "frozen A planes do not match");
     const Planes &B = split_operand(e, g.B + 0, g.sbn, g.sbk, g.N, g.K, g.N, Kp, lb, 0);
@@ -1069,10 +1069,10 @@ static void run_b3(Engine &e, GemmArgs g, int passes, const B3FrozenA *fa = null
   }
   Planes pa, pb;
   if (fa) {  // planes split earlier in the layout split_operand gives (layout 0)
-    SISI_REQUIRE(fa->store.get() && fa->M == g.M && fa->K == g.K && fa->Mp == Mp && fa->Kp == Kp,
+    SISI_REQUIRE(fa->valid() && fa->M == g.M && fa->K == g.K && fa->Mp == Mp && fa->Kp == Kp,
                  SISI_ERR_INVALID, "frozen A planes do not match the product");
-    pa.hi = reinterpret_cast<const u16 *>(fa->store.get());
-    pa.lo = pa.hi + int64_t(Mp) * Kp;
+    pa.hi = reinterpret_cast<const u16 *>(fa->hi());
+    pa.lo = pa.hi + fa->lo_offset();
     pa.base = pa.hi;
     pa.ld = Kp;
   }
@@ -1161,6 +1161,17 @@ void b3_freeze_a(Engine &e, const double *A, int64_t sam, int64_t sak, int M, in
   out.K = K;
   out.Mp = Mp;
   out.Kp = Kp;
+}
+
+void b3_split_into(Engine &e, const double *A, int64_t sam, int64_t sak, int M, int K, int Mp,
+                   int Kp, void *hi, void *lo) {
+  SISI_REQUIRE(A && hi && lo && M > 0 && K > 0 && Mp >= M && Kp >= K && Kp % 32 == 0,
+               SISI_ERR_INVALID, "b3_split_into: bad shape");
+  SplitArgs a{A, sam, sak, M, K, Mp, Kp, reinterpret_cast<u16 *>(hi), reinterpret_cast<u16 *>(lo)};
+  Timed t(*e.timers, Kind::Bandwidth, 8.0 * double(M) * K + 4.0 * double(Mp) * Kp);
+  dim3 grid(unsigned((Kp + 63) / 64), unsigned((Mp + 63) / 64));
+  hipLaunchKernelGGL(split_bf16_kernel, grid, dim3(256), 0, e.stream, a);
+  SISI_HIP(hipGetLastError());
 }
 
 void gemm_b3_frozen_a(Engine &e, GemmArgs g, const B3FrozenA &A, int passes) {
