@@ -234,6 +234,7 @@ class CcsdEnergyFromCoulombIntegrals : public Algorithm {
     opts.precision = precision_argument(*this);
     opts.switch_threshold = getRealArgument("switchThreshold", 1e-5);
     opts.refine_interval = int(getIntegerArgument("refineInterval", 0));
+    opts.bulk_refine_interval = int(getIntegerArgument("bulkRefineInterval", 0));
     if (vertex_arguments) {
       opts.ppl = getIntegerArgument("ppl", 1) != 0;                          // cxx:324
       opts.distinguishable = getIntegerArgument("distinguishable", 0) != 0;  // cxx:123-124
