@@ -702,24 +702,30 @@ __global__ __launch_bounds__(256) void gemm_b3p_seg_kernel(B3SegArgs p) {
     have = more;
   }
 
+  // row offsets once per row, each column offset once per column (the
+  // cube-blocked output splits cost divisions)
+  int64_t om[TI];
 #pragma unroll
   for (int i = 0; i < TI; ++i) {
     const int m = m0 + wm + i * 16 + l16;
-    if (m >= g.M) continue;
-    const int64_t om = b3_off_m(g, m);
-#pragma unroll
-    for (int j = 0; j < TJ; ++j)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int n = n0 + wn + j * 16 + 4 * lk + r;
-        if (n < g.N) {
-          const int64_t o = om + b3_off_n(g, n);
-          double v = g.alpha * double(acc[i][j][r]);
-          if (g.beta != 0.0) v += g.beta * g.C[o];
-          g.C[o] = v;
-        }
-      }
+    om[i] = m < g.M ? b3_off_m(g, m) : -1;
   }
+#pragma unroll
+  for (int j = 0; j < TJ; ++j)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int n = n0 + wn + j * 16 + 4 * lk + r;
+      if (n >= g.N) continue;
+      const int64_t on = b3_off_n(g, n);
+#pragma unroll
+      for (int i = 0; i < TI; ++i) {
+        if (om[i] < 0) continue;
+        const int64_t o = om[i] + on;
+        double v = g.alpha * double(acc[i][j][r]);
+        if (g.beta != 0.0) v += g.beta * g.C[o];
+        g.C[o] = v;
+      }
+    }
 }
 
 namespace {
