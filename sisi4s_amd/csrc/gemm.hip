@@ -187,24 +187,30 @@ __device__ __forceinline__ void gemm_k4_body(const GemmArgs &g, int pid, int ks,
   // epilogue
   if (g.ksplit == 1) {
     double *C = g.C + int64_t(bz) * g.bsc;
+    // row offsets once per row, each column offset once per column (the
+    // output splits cost divisions)
+    int64_t om[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int m = m0 + wm + i * 16 + l16;
-      if (m >= g.M) continue;
-      const int64_t om = out_off_m(g, m);
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int n = n0 + wn + j * 16 + LSTR * lk + RSTR * r;
-          if (n < g.N) {
-            const int64_t o = om + out_off_n(g, n);
-            double v = g.alpha * double(acc[i][j][r]);
-            if (g.beta != 0.0) v += g.beta * C[o];
-            C[o] = v;
-          }
-        }
+      om[i] = m < g.M ? out_off_m(g, m) : -1;
     }
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = n0 + wn + j * 16 + LSTR * lk + RSTR * r;
+        if (n >= g.N) continue;
+        const int64_t on = out_off_n(g, n);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          if (om[i] < 0) continue;
+          const int64_t o = om[i] + on;
+          double v = g.alpha * double(acc[i][j][r]);
+          if (g.beta != 0.0) v += g.beta * C[o];
+          C[o] = v;
+        }
+      }
   } else {
     double *W = g.work + (int64_t(bz) * g.ksplit + ks) * int64_t(g.M) * g.N;
 #pragma unroll
@@ -413,24 +419,30 @@ __global__ __launch_bounds__(256, 2) void gemm_bf16x3_kernel(GemmArgs g) {
 
   if (g.ksplit == 1) {
     double *C = g.C + int64_t(bz) * g.bsc;
+    // row offsets once per row, each column offset once per column (the
+    // output splits cost divisions)
+    int64_t om[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int m = m0 + wm + i * 16 + l16;
-      if (m >= g.M) continue;
-      const int64_t om = out_off_m(g, m);
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int n = n0 + wn + j * 16 + 4 * lk + r;
-          if (n < g.N) {
-            const int64_t o = om + out_off_n(g, n);
-            double v = g.alpha * double(acc[i][j][r]);
-            if (g.beta != 0.0) v += g.beta * C[o];
-            C[o] = v;
-          }
-        }
+      om[i] = m < g.M ? out_off_m(g, m) : -1;
     }
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = n0 + wn + j * 16 + 4 * lk + r;
+        if (n >= g.N) continue;
+        const int64_t on = out_off_n(g, n);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          if (om[i] < 0) continue;
+          const int64_t o = om[i] + on;
+          double v = g.alpha * double(acc[i][j][r]);
+          if (g.beta != 0.0) v += g.beta * C[o];
+          C[o] = v;
+        }
+      }
   } else {
     double *W = g.work + (int64_t(bz) * g.ksplit + ks) * int64_t(g.M) * g.N;
 #pragma unroll
