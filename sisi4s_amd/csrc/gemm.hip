@@ -245,7 +245,9 @@ __global__ __launch_bounds__(256, 2) void gemm_k4_grouped(const GemmArgs *const 
     if (start[mid] <= blockIdx.x) lo = mid;
     else hi = mid - 1;
   }
-  const GemmArgs g = *args[lo];
+  // the record is read in place (a copy kept the whole struct live in
+  // registers and made the kernel spill)
+  const GemmArgs &g = *args[lo];
   const int tiles = g.tiles_m * g.tiles_n;
   const int local = int(blockIdx.x - start[lo]);
   const int r = local / tiles;

@@ -654,7 +654,8 @@ __global__ void copy_generic_grouped(const CopyArgs *const *args, const unsigned
 
 __global__ void copy_tiled_grouped(const CopyArgs *const *args, const unsigned *start, int n) {
   const int p = group_record(start, n, blockIdx.x);
-  const CopyArgs a = *args[p];
+  // the record is read in place: a copy indexed by dimension went to scratch
+  const CopyArgs &a = *args[p];
   const unsigned local = blockIdx.x - start[p], gx = a.gx;
   const unsigned gy = (start[p + 1] - start[p]) / gx;
   copy_tiled_body(a, local % gx, local / gx, gy);
