@@ -1184,6 +1184,8 @@ void ccsd_begin(Engine &e, const sisi_ccsd_opts &opts, const double *T1h, const 
   e.last_energy = e.last_direct = e.last_exchange = e.last_energy_imag = 0.0;
   e.last_delta = 1.0;
   e.converged = false;
+  e.confirm_pending = false;
+  e.confirm_hold = 0;
   e.energies.clear();
   e.diis.clear();
   e.B.clear();
@@ -1351,8 +1353,21 @@ void ccsd_iterate(Engine &e, int n) {
     const bool met = de < std::fabs(o.energy_convergence) &&
                      e.last_delta * e.last_delta <
                          std::fabs(o.amplitudes_convergence * o.amplitudes_convergence);
+    // A confirmation that fails (DIIS keeps the energy oscillating near the
+    // threshold) holds further requests for a few refinement iterations, so
+    // that the run does not spend an FP64 anchor on every other iteration.
+    constexpr int kConfirmHold = 4;
     if (met && prec == SISI_FP64) e.converged = true;
-    else if (met && refine) e.anchor.age = o.refine_interval;
+    else if (refine && prec == SISI_FP64 && e.confirm_pending) e.confirm_hold = kConfirmHold;
+    if (prec == SISI_FP64) e.confirm_pending = false;
+    if (refine && prec != SISI_FP64) {
+      if (met && e.confirm_hold == 0) {
+        e.anchor.age = o.refine_interval;
+        e.confirm_pending = true;
+      } else if (e.confirm_hold > 0) {
+        --e.confirm_hold;
+      }
+    }
     e.prev_energy = E;
     e.prev_energy_imag = Ei;
   }

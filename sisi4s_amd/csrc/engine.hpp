@@ -241,6 +241,11 @@ class Engine {
          last_exchange = 0.0, last_delta = 0.0;
   double prev_energy_imag = 0.0, last_energy_imag = 0.0;  // complex mode
   bool converged = false;
+  // refinement phase: an anchor asked for by a refinement iteration that met
+  // the convergence test is pending; after one that did not confirm it, the
+  // next confirm_hold refinement iterations do not ask again
+  bool confirm_pending = false;
+  int confirm_hold = 0;
   DBuf T1, T2;
   std::vector<double> energies;
   // DIIS ring (DiisMixer.cxx:48-180)
