@@ -128,7 +128,7 @@ def main():
                     help="(T) precision of the mixed run")
     ap.add_argument("--coarse-threshold", type=float, default=0.0,
                     help="mixed runs: plain-bf16 coarse stage until |dT|/|T| reaches this value")
-    ap.add_argument("--refine-interval", type=int, default=32,
+    ap.add_argument("--refine-interval", type=int, default=128,
                     help="mixed runs: FP64 anchor every k iterations after the switch (0: plain "
                          "FP64 iterations)")
     ap.add_argument("--bulk-refine-interval", type=int, default=16,
