@@ -176,6 +176,28 @@ void contract(Engine &e, double alpha, const View &Av, const std::string &ia, co
   }
   std::string Mord = in_order(A, gM), Nord = in_order(B, gN), Bord = in_order(C, gB);
   std::string KA = in_order(A, gK), KB = in_order(B, gK), Kord;
+  // The epilogue's lanes run along the first M (or, swapped, N) label.  When
+  // C's unit-stride label is an M or N label that does not lead its group,
+  // the group follows C's strides instead, provided the operand that then
+  // needs a permuted copy is small against C (e.g. "jika,bk->abij": the
+  // output is Nv/No times larger than V_ijka, and led by j its writes would
+  // be Nv^2 No elements apart).
+  {
+    auto lead_unit = [&](const std::string &grp) {
+      return !grp.empty() && C.stride.at(grp[0]) == 1;
+    };
+    auto by_c = [&](std::string grp) {
+      std::stable_sort(grp.begin(), grp.end(),
+                       [&](char x, char y) { return C.stride.at(x) < C.stride.at(y); });
+      return grp;
+    };
+    const int64_t nA = prod(A.lab, len), nB = prod(B.lab, len), nC = prod(C.lab, len);
+    if (!lead_unit(Mord) && !lead_unit(Nord)) {
+      const std::string cm = by_c(Mord), cn = by_c(Nord);
+      if (lead_unit(cm) && 4 * nA <= nC) Mord = cm;
+      else if (lead_unit(cn) && 4 * nB <= nC) Nord = cn;
+    }
+  }
 
   // operands: choose the k order that needs the fewest copies
   DBuf storeA, storeB, storeC;
