@@ -459,39 +459,44 @@ __global__ void estimate_kernel(int order, int Nv, int No, double *R, const doub
 // both tiles are staged in LDS, then each element and its partner get the
 // same average (an addition of the same two values in either order).
 __global__ __launch_bounds__(256) void symmetrize_doubles_kernel(double *T, int Nv, int No) {
-  const int ta = int(blockIdx.x), tb = int(blockIdx.y), ij = int(blockIdx.z);
-  const int i = ij % No, j = ij / No;
-  if (i > j || (i == j && ta > tb)) return;
-  __shared__ double s0[32][33], s1[32][33];
-  const int64_t nv2 = int64_t(Nv) * Nv;
-  double *P = T + nv2 * (i + int64_t(No) * j);  // [a, b] of (i, j) at a + Nv b
-  double *Q = T + nv2 * (j + int64_t(No) * i);  // [a, b] of (j, i)
+  const int ta = int(blockIdx.x), tb = int(blockIdx.y);
   const int x = int(threadIdx.x) & 31, y0 = int(threadIdx.x) >> 5;
+  const int64_t nv2 = int64_t(Nv) * Nv;
+  __shared__ double s0[32][33], s1[32][33];
+  // (i, j) pairs strided by the grid's z extent (block-uniform skips)
+  for (int ij = int(blockIdx.z); ij < No * No; ij += int(gridDim.z)) {
+    const int i = ij % No, j = ij / No;
+    if (i > j || (i == j && ta > tb)) continue;
+    double *P = T + nv2 * (i + int64_t(No) * j);  // [a, b] of (i, j) at a + Nv b
+    double *Q = T + nv2 * (j + int64_t(No) * i);  // [a, b] of (j, i)
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int y = y0 + 8 * r;
-    const int pa = 32 * ta + x, pb = 32 * tb + y, qa = 32 * tb + x, qb = 32 * ta + y;
-    s0[y][x] = (pa < Nv && pb < Nv) ? P[pa + int64_t(Nv) * pb] : 0.0;
-    s1[y][x] = (qa < Nv && qb < Nv) ? Q[qa + int64_t(Nv) * qb] : 0.0;
-  }
-  __syncthreads();
+    for (int r = 0; r < 4; ++r) {
+      const int y = y0 + 8 * r;
+      const int pa = 32 * ta + x, pb = 32 * tb + y, qa = 32 * tb + x, qb = 32 * ta + y;
+      s0[y][x] = (pa < Nv && pb < Nv) ? P[pa + int64_t(Nv) * pb] : 0.0;
+      s1[y][x] = (qa < Nv && qb < Nv) ? Q[qa + int64_t(Nv) * qb] : 0.0;
+    }
+    __syncthreads();
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int y = y0 + 8 * r;
-    const int pa = 32 * ta + x, pb = 32 * tb + y, qa = 32 * tb + x, qb = 32 * ta + y;
-    // partners: P[pa, pb] <-> Q[pb, pa] = s1[x][y], Q[qa, qb] <-> P[qb, qa] = s0[x][y]
-    if (pa < Nv && pb < Nv) P[pa + int64_t(Nv) * pb] = 0.5 * (s0[y][x] + s1[x][y]);
-    if (qa < Nv && qb < Nv) Q[qa + int64_t(Nv) * qb] = 0.5 * (s1[y][x] + s0[x][y]);
+    for (int r = 0; r < 4; ++r) {
+      const int y = y0 + 8 * r;
+      const int pa = 32 * ta + x, pb = 32 * tb + y, qa = 32 * tb + x, qb = 32 * ta + y;
+      // partners: P[pa, pb] <-> Q[pb, pa] = s1[x][y], Q[qa, qb] <-> P[qb, qa] = s0[x][y]
+      if (pa < Nv && pb < Nv) P[pa + int64_t(Nv) * pb] = 0.5 * (s0[y][x] + s1[x][y]);
+      if (qa < Nv && qb < Nv) Q[qa + int64_t(Nv) * qb] = 0.5 * (s1[y][x] + s0[x][y]);
+    }
+    __syncthreads();  // the tiles are refilled for the next (i, j)
   }
 }
 
 void symmetrize_doubles(Engine &e, double *T2, int Nv, int No) {
-  SISI_REQUIRE(Nv > 0 && No > 0 && int64_t(No) * No <= 65535, SISI_ERR_INVALID,
-               "symmetrize_doubles: No * No must fit a grid dimension");
+  SISI_REQUIRE(Nv > 0 && No > 0 && int64_t(No) * No < (int64_t(1) << 31), SISI_ERR_INVALID,
+               "symmetrize_doubles: bad shape");
   const unsigned tiles = unsigned((Nv + 31) / 32);
+  const unsigned gz = unsigned(std::min<int64_t>(int64_t(No) * No, 65535));
   Timed t(*e.timers, Kind::Bandwidth, 16.0 * double(Nv) * Nv * No * No);
-  hipLaunchKernelGGL(symmetrize_doubles_kernel, dim3(tiles, tiles, unsigned(No * No)), dim3(256),
-                     0, e.stream, T2, Nv, No);
+  hipLaunchKernelGGL(symmetrize_doubles_kernel, dim3(tiles, tiles, gz), dim3(256), 0, e.stream,
+                     T2, Nv, No);
   SISI_HIP(hipGetLastError());
 }
 
