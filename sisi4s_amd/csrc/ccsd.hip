@@ -269,7 +269,7 @@ void ccsd_residual(Engine &e, int it, bool initial_doubles, int slice_size, cons
       double bytes = 0.0;
       for (int m = 0; m < nslices; ++m)
         for (int n = m; n < nslices; ++n)
-          bytes += 4.0 * double(((hi[n] - lo[n]) * (hi[m] - lo[m]) + 127) / 128 * 128) * double(Kp);
+          bytes += 4.0 * double(((hi[n] - lo[n]) * (hi[m] - lo[m]) + 255) / 256 * 256) * double(Kp);
       const int64_t Mps = (o * v + 255) / 256 * 256;
       const double s_bytes = 4.0 * double(Mps) * double(Kp);
       // device memory free or cached in the pool's free lists (a run's
