@@ -98,11 +98,13 @@ __global__ __launch_bounds__(256) void split_bf16_kernel(SplitArgs a) {
   }
   const int64_t row = int64_t(r) * (a.ld ? a.ld : a.Kp) + k;
   U128 *ph = reinterpret_cast<U128 *>(a.hi + row);
-  U128 *pl = reinterpret_cast<U128 *>(a.lo + row);
   ph[0] = *reinterpret_cast<U128 *>(&h[0]);
   ph[1] = *reinterpret_cast<U128 *>(&h[8]);
-  pl[0] = *reinterpret_cast<U128 *>(&l[0]);
-  pl[1] = *reinterpret_cast<U128 *>(&l[8]);
+  if (a.lo) {  // hi-only planes (plain-bf16 consumers) have no lo plane
+    U128 *pl = reinterpret_cast<U128 *>(a.lo + row);
+    pl[0] = *reinterpret_cast<U128 *>(&l[0]);
+    pl[1] = *reinterpret_cast<U128 *>(&l[8]);
+  }
   if (a.hi2) {
     U128 *p2 = reinterpret_cast<U128 *>(a.hi2 + row);
     p2[0] = *reinterpret_cast<U128 *>(&h[0]);
@@ -741,12 +743,19 @@ struct Planes {
 // kernels); 1 = rows [hi | hi | lo] and 2 = rows [hi | lo | hi] of 3 Kp bf16,
 // so that sum_k' A'[m,k'] B'[n,k'] over k' < 3 Kp is hi*hi + hi*lo + lo*hi
 // with A in layout 1 and B in layout 2: one library GEMM with K tripled.
+// Plane layouts: 0 hi plane then lo plane; 1 / 2 rows of (hi, hi, lo) /
+// (hi, lo, hi) for the library path; kHiOnly the hi plane alone.
+constexpr int kHiOnly = 3;
+
 void plane_pointers(u16 *base, int64_t Rp, int64_t Kp, int layout, u16 *&hi, u16 *&lo,
                     u16 *&hi2, int64_t &ld) {
   hi = base;
   hi2 = nullptr;
   if (layout == 0) {
     lo = base + Rp * Kp;
+    ld = Kp;
+  } else if (layout == kHiOnly) {  // plain-bf16 consumers read the hi plane alone
+    lo = nullptr;
     ld = Kp;
   } else {
     ld = 3 * Kp;
@@ -763,7 +772,7 @@ const Planes &split_operand(Engine &e, const double *src, int64_t sr, int64_t sk
   std::string key;
   const bool cached = allow_cache && e.is_frozen(src);
   const int64_t elems = int64_t(Rp) * Kp;  // bf16 per plane
-  const int nplanes = layout == 0 ? 2 : 3;
+  const int nplanes = layout == 0 ? 2 : layout == kHiOnly ? 1 : 3;
   auto fill = [&](u16 *base) {
     u16 *hi, *lo, *hi2;
     int64_t ld;
@@ -1082,8 +1091,9 @@ static void run_b3(Engine &e, GemmArgs g, int passes, const B3FrozenA *fa = null
     pa.base = pa.hi;
     pa.ld = Kp;
   }
-  const Planes &A = fa ? pa : split_operand(e, g.A + 0, g.sam, g.sak, g.M, g.K, Mp, Kp, pa);
-  const Planes &B = split_operand(e, g.B + 0, g.sbn, g.sbk, g.N, g.K, Np, Kp, pb);
+  const int lay = passes == 1 ? kHiOnly : 0;  // plain bf16 reads no lo plane
+  const Planes &A = fa ? pa : split_operand(e, g.A + 0, g.sam, g.sak, g.M, g.K, Mp, Kp, pa, lay);
+  const Planes &B = split_operand(e, g.B + 0, g.sbn, g.sbk, g.N, g.K, Np, Kp, pb, lay);
   const int count = b3_variant_count(passes);
   int vid = forced_variant(passes).load(), xcd = 1;
   if (vid < 0) {
